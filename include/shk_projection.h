@@ -1,0 +1,194 @@
+/*
+ * shk_projection.h -- C-ABI of libshk_projection.so, the MI355X (gfx950)
+ * MAC liquid pressure projection.
+ *
+ * The library replaces two Shiokaze plugins behind their own contracts:
+ *   - macproject3_interface (include/shiokaze/projection/macproject3_interface.h:39-116),
+ *     implemented by macpressuresolver3 (src/projection/macpressuresolver3.cpp:40-372);
+ *   - RCMatrix_solver_interface<size_t,double>::solve
+ *     (include/shiokaze/linsolver/RCMatrix_solver.h:77), implemented by the "pcg"
+ *     plugin (src/linsolver/pcg.cpp:32-82).
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Field pointers passed to shk_project /
+ *     shk_solve_csr / shk_divergence are DEVICE pointers (hipMalloc'd or any
+ *     allocation visible to the context's device) unless the context was given
+ *     "HostPointers=1", in which case they are host pointers and the library
+ *     stages them (the PCIe copies are then part of the call).
+ *   - Dense x-fastest layouts, the same as the reference's array3 / macarray3
+ *     shapes (include/shiokaze/math/shape.h:700-716):
+ *       cell  nx*ny*nz            index i + nx*(j + ny*k)
+ *       u     (nx+1)*ny*nz        index i + (nx+1)*(j + ny*k)
+ *       v     nx*(ny+1)*nz        index i + nx*(j + (ny+1)*k)
+ *       w     nx*ny*(nz+1)        index i + nx*(j + ny*k)
+ *       solid (nx+1)*(ny+1)*(nz+1) nodal level set
+ *     Velocity, level sets and pressure are float (the reference's Real,
+ *     include/shiokaze/core/config.h:34); the linear system is double.
+ *   - Every entry point returns an shk_status; nothing calls exit() or throws.
+ *     shk_last_error() returns the message of the most recent failure.
+ *   - One context owns one HIP stream and its device workspace.  A context is
+ *     not thread-safe; separate contexts may be used from separate threads.
+ *     Calls are synchronous on return.
+ */
+#ifndef SHK_PROJECTION_H
+#define SHK_PROJECTION_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHK_ABI_VERSION 1
+
+typedef struct shk_ctx shk_ctx;
+
+typedef enum shk_status {
+    SHK_OK = 0,
+    /* iteration cap reached; the reference returns silently with the residual
+       (local/include/pcgsolver/pcg_solver.h:292-294) */
+    SHK_NOT_CONVERGED = 1,
+    /* NaN in the Krylov scalars (the reference prints "NaN Error !!" and
+       returns 0 iterations, pcg_solver.h:267-271) */
+    SHK_NAN = 2,
+    SHK_BAD_ARG = 3,
+    SHK_HIP_ERROR = 4,
+    SHK_NO_DEVICE = 5,
+    /* rho == 0 or p.Ap <= 0: the system is singular or not SPD on the unknowns
+       (pcg_solver.h:263-266 returns false with 0 iterations for rho == 0) */
+    SHK_BREAKDOWN = 6,
+    SHK_OUT_OF_MEMORY = 7
+} shk_status;
+
+/* Result of a solve.  count / reresid are RCMatrix_solver_interface::Result's
+ * fields (include/shiokaze/linsolver/RCMatrix_solver.h:44-57); the rest are the
+ * figures this library adds. */
+typedef struct shk_result {
+    int64_t count;          /* Krylov iterations (all outer steps summed) */
+    double reresid;         /* max|r|/max|b| of the residual the stop test used */
+    double true_relres_inf; /* max|b - A x| / max|b|, FP64 */
+    double true_relres_l2;  /* ||b - A x||_2 / ||b||_2, FP64 */
+    double b_inf;           /* max|b| */
+    int64_t unknowns;       /* N, the number of rows */
+    int64_t active_bricks;  /* 8x8x8 bricks that hold at least one unknown */
+    int32_t outer;          /* refinement steps (1 for the FP64 path) */
+    int32_t status;         /* shk_status of the solve */
+    int32_t fallback;       /* 1 if the learned flexible CG handed over to plain PCG */
+    int32_t precision;      /* 0 = fp64, 1 = mixed */
+    double ms_setup;        /* fractions, labels, bricks, coefficients, rhs */
+    double ms_solve;        /* Krylov loop incl. refinement */
+    double ms_update;       /* pressure scatter + velocity update */
+} shk_result;
+
+/* ------------------------------------------------------------------ context */
+
+/* macproject3_interface::initialize(shape3, dx) (macproject3_interface.h:104-112).
+ * device < 0 selects the current HIP device. */
+int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out);
+void shk_destroy(shk_ctx *ctx);
+
+/* configure() keys (same names and defaults as the reference):
+ *   Residual (1e-4)          pcg.cpp:40, 76       relative max-norm stop
+ *   MaxIterations (30000)    pcg.cpp:43, 77
+ *   ModifiedIC, MinDiagRatio pcg.cpp:41-42        accepted; the GPU path has no MIC(0)
+ *   SecondOrderAccurateFluid (1), SecondOrderAccurateSolid (1)
+ *                            macpressuresolver3.cpp:302-303
+ *   EpsFluid (1e-2), EpsSolid (1e-2)   macutility3.cpp:502-503
+ *   Gain, WarmStart          macpressuresolver3.cpp:304-305  accepted; only the
+ *                            defaults (volume correction and warm start off) are built
+ * Keys of this library:
+ *   Precision      fp64 | mixed            (fp64)
+ *   Preconditioner none | jacobi | unet    (jacobi)
+ *   OuterMax       refinement steps for mixed (3)
+ *   InnerResidual  relative stop of each mixed inner solve (1e-5, floor)
+ *   CheckInterval  iterations launched between host convergence polls (32)
+ *   HostPointers   0 | 1                   (0)
+ *   UnetWeights    path of a trained preconditioner blob (for Preconditioner=unet)
+ *   ProfileKernels 0 | 1: time every Krylov kernel launch with HIP events on the
+ *                  context's stream (read with shk_kernel_stats; resets the counters)
+ * Unknown keys and unparsable values return SHK_BAD_ARG. */
+int shk_configure(shk_ctx *ctx, const char *key, const char *value);
+
+/* Most recent error message of this context ("" if none).  ctx may be NULL for
+ * errors of shk_create. */
+const char *shk_last_error(const shk_ctx *ctx);
+
+/* ------------------------------------------------------------------ projection */
+
+/* macproject3_interface::project (macproject3_interface.h:79-87, implemented at
+ * macpressuresolver3.cpp:54-299) with surface_tension = 0 and no rigid bodies.
+ * u, v, w are updated in place.  solid is the nodal solid level set, fluid the
+ * cell-centred liquid level set, us/vs/ws the solid face velocities.  When the
+ * scene has no unknowns the velocity is left untouched (macpressuresolver3.cpp:170). */
+int shk_project(shk_ctx *ctx, double dt, float *u, float *v, float *w,
+                const float *solid, const float *us, const float *vs, const float *ws,
+                const float *fluid, shk_result *result);
+
+/* get_pressure() (macproject3_interface.h:93): nx*ny*nz float device array of the
+ * last projection (0 on non-unknowns).  Owned by the context. */
+const float *shk_pressure(const shk_ctx *ctx);
+
+/* Copies get_pressure() into dst (a device buffer of nx*ny*nz floats, or a host
+ * buffer when the context has HostPointers=1). */
+int shk_copy_pressure(shk_ctx *ctx, float *dst);
+
+/* Copies the last projection's face fractions ("areas" from
+ * macutility3::compute_area_fraction, "rhos" from compute_fluid_fraction) into
+ * caller buffers of the face sizes above.  Any pointer may be NULL. */
+int shk_export_fractions(shk_ctx *ctx, float *area_u, float *area_v, float *area_w,
+                         float *rho_u, float *rho_v, float *rho_w);
+
+/* Copies the last projection's linear system in dense cell layout: label[c] = 1
+ * for unknowns else 0; diag, the three low-face couplings (A[c, c-e_d] = -c_d[c]
+ * when both cells are unknowns), rhs and the FP64 solution x.  Any pointer may be NULL. */
+int shk_export_system(shk_ctx *ctx, uint8_t *label, double *diag, double *cx, double *cy,
+                      double *cz, double *rhs, double *x);
+
+/* FP64 weighted divergence max over the last projection's unknowns of a face
+ * field: the negative of the rhs macpressuresolver3.cpp:209-213 would assemble
+ * from (u, v, w).  With use_fp64_update = 1 the faces are first updated in FP64
+ * with the FP64 solution (u - dt*(p_i - p_{i-1})/(rho*dx)), which is the
+ * post-projection divergence without float storage rounding. */
+int shk_divergence(shk_ctx *ctx, double dt, const float *u, const float *v, const float *w,
+                   const float *us, const float *vs, const float *ws, int use_fp64_update,
+                   double *max_abs_div);
+
+/* ------------------------------------------------------------------ linear solver */
+
+/* RCMatrix_solver_interface<size_t,double>::solve (RCMatrix_solver.h:77) as the
+ * pcg plugin does it (pcg.cpp:45-73): x0 = 0, stop when max|r| <= Residual*max|b|.
+ * CSR with sorted int64 columns, rows = number of rows; device pointers. */
+int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64_t *cols,
+                  const double *vals, const double *b, double *x, shk_result *result);
+
+/* Accumulated launch count and total milliseconds of one Krylov kernel
+ * ("spmv_dot", "axpy_norm" or "update_dir") since ProfileKernels was last set.
+ * Only launches that did work are counted. */
+int shk_kernel_stats(shk_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+
+/* ------------------------------------------------------------------ scenes */
+
+/* Synthetic scenes for tests and benchmarks, generated on the host into HOST
+ * buffers of the layouts above (needs no GPU).
+ *   kind 1: pool y < 0.3 plus a sphere r = 0.15 at (0.5, 0.7, 0.5), no solids
+ *           (walls at the domain boundary only).
+ *   kind 3: 20 seeded solid boxes/spheres (radii 0.03-0.12), pool y < 0.4 plus
+ *           8 seeded liquid spheres, liquid clipped by the solid as
+ *           macutility3.cpp:352-361 clips it; liquid components with no free
+ *           surface are turned to air.
+ * Face velocities are uniform in [-1, 1) from a counter-based hash of
+ * (seed, component, face index); solid velocities are 0.  dx is the cell size.
+ * Returns SHK_OK; *dropped (nullable) receives the number of cells turned to air. */
+int shk_scene_generate(int kind, int nx, int ny, int nz, double dx, uint64_t seed,
+                       float *fluid, float *solid, float *u, float *v, float *w,
+                       float *us, float *vs, float *ws, int64_t *dropped);
+
+/* ABI version (SHK_ABI_VERSION) and a build string. */
+int shk_abi_version(void);
+const char *shk_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHK_PROJECTION_H */

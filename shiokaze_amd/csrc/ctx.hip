@@ -1,0 +1,706 @@
+// ctx.hip -- the C-ABI of libshk_projection.so (include/shk_projection.h).
+//
+// shk_project follows macpressuresolver3::project (src/projection/macpressuresolver3.cpp:54-299):
+//   fractions -> unknown labels -> stencil coefficients + rhs -> solve -> pressure
+//   scatter -> velocity update,
+// with the matrix kept matrix-free on 8x8x8 bricks that hold at least one unknown,
+// so the Krylov loop streams only the liquid part of the grid.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "krylov.h"
+#include "setup.h"
+#include "shk_common.h"
+#include "shk_fractions.h"
+#include "shk_projection.h"
+#include "ctx_internal.h"
+
+using namespace shk;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+namespace {
+
+bool parse_double(const char *s, double &out) {
+    if (!s || !*s) return false;
+    char *end = nullptr;
+    errno = 0;
+    const double v = strtod(s, &end);
+    if (errno || !end || *end) return false;
+    out = v;
+    return true;
+}
+bool parse_int(const char *s, int64_t &out) {
+    double v;
+    if (!parse_double(s, v) || v != std::floor(v) || std::fabs(v) > 9e15) return false;
+    out = (int64_t)v;
+    return true;
+}
+bool parse_bool(const char *s, bool &out) {
+    if (!s) return false;
+    std::string v(s);
+    for (auto &ch : v) ch = (char)tolower(ch);
+    if (v == "1" || v == "true" || v == "yes" || v == "on") return out = true, true;
+    if (v == "0" || v == "false" || v == "no" || v == "off") return out = false, true;
+    return false;
+}
+
+int sync_and_time(shk_ctx *c, int a, int b, double &ms) {
+    float t = 0.0f;
+    HIPC(c, hipEventSynchronize(c->ev[b]));
+    HIPC(c, hipEventElapsedTime(&t, c->ev[a], c->ev[b]));
+    ms = t;
+    return SHK_OK;
+}
+
+int ensure_partials(shk_ctx *c, int blocks) {
+    HIPC(c, c->pa.ensure((size_t)blocks + 64));
+    HIPC(c, c->pb.ensure(3 * (size_t)blocks + 64));
+    return SHK_OK;
+}
+
+// Builds the active brick list, slot map and neighbour table from the brick flags.
+int build_bricks(shk_ctx *c) {
+    const int64_t nb = (int64_t)c->nbx * c->nby * c->nbz;
+    c->h_flag.resize(nb);
+    HIPC(c, hipMemcpyAsync(c->h_flag.data(), c->brick_flag.p, nb, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    c->h_slot.assign(nb, -1);
+    c->h_coord.clear();
+    for (int bz = 0; bz < c->nbz; ++bz)
+        for (int by = 0; by < c->nby; ++by)
+            for (int bx = 0; bx < c->nbx; ++bx) {
+                const int64_t b = bx + (int64_t)c->nbx * (by + (int64_t)c->nby * bz);
+                if (c->h_flag[b]) {
+                    c->h_slot[b] = (int)c->h_coord.size();
+                    c->h_coord.push_back(make_int4(bx, by, bz, 0));
+                }
+            }
+    c->n_active = (int)c->h_coord.size();
+    c->h_nbr.assign(6 * (size_t)std::max(c->n_active, 1), -1);
+    for (int s = 0; s < c->n_active; ++s) {
+        const int4 q = c->h_coord[s];
+        const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+        for (int f = 0; f < 6; ++f) {
+            const int x = q.x + off[f][0], y = q.y + off[f][1], z = q.z + off[f][2];
+            if (x < 0 || y < 0 || z < 0 || x >= c->nbx || y >= c->nby || z >= c->nbz) continue;
+            c->h_nbr[6 * s + f] = c->h_slot[x + (int64_t)c->nbx * (y + (int64_t)c->nby * z)];
+        }
+    }
+    HIPC(c, c->brick_slot.ensure(nb));
+    HIPC(c, hipMemcpyAsync(c->brick_slot.p, c->h_slot.data(), sizeof(int) * nb, hipMemcpyHostToDevice, c->stream));
+    if (c->n_active == 0) return SHK_OK;
+    const size_t n = (size_t)c->n_active * kBrickCells;
+    HIPC(c, c->coord.ensure(c->n_active));
+    HIPC(c, c->nbr.ensure(6 * (size_t)c->n_active));
+    HIPC(c, hipMemcpyAsync(c->coord.p, c->h_coord.data(), sizeof(int4) * c->n_active, hipMemcpyHostToDevice,
+                           c->stream));
+    HIPC(c, hipMemcpyAsync(c->nbr.p, c->h_nbr.data(), sizeof(int) * 6 * c->n_active, hipMemcpyHostToDevice,
+                           c->stream));
+    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->x64, &c->r64, &c->s64, &c->q64})
+        HIPC(c, b->ensure(n));
+    if (c->prm.precision == 1 || c->prm.precond == 2) {
+        for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32, &c->s32, &c->q32})
+            HIPC(c, b->ensure(n));
+    }
+    // host-side buffers must outlive the async copies above
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return ensure_partials(c, c->n_active);
+}
+
+}  // namespace
+
+namespace shk {
+
+Reduction reduction(shk_ctx *c) { return Reduction{c->pa.p, c->pb.p, c->state.p}; }
+
+int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
+    HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    const int interval = std::max(1, c->prm.check_interval);
+    if (c->profile && c->pev.size() < (size_t)(3 * interval + 1)) {
+        const size_t old = c->pev.size();
+        c->pev.resize(3 * interval + 1);
+        for (size_t i = old; i < c->pev.size(); ++i) HIPC(c, hipEventCreate(&c->pev[i]));
+    }
+    while (!c->h_state->done) {
+        const int64_t it0 = c->h_state->iter;
+        if (c->profile) {
+            c->pcur = 0;
+            c->mark();
+        }
+        for (int t = 0; t < interval; ++t) iterate(arg);
+        HIPC(c, hipGetLastError());
+        HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        if (c->profile && c->pcur == (size_t)(3 * interval + 1)) {
+            // only launches that did work count: the first `delta` iterations; the
+            // direction update of the final iteration returns at once
+            const int64_t delta = c->h_state->iter - it0;
+            for (int64_t t = 0; t < delta && t < interval; ++t) {
+                for (int kk = 0; kk < 3; ++kk) {
+                    if (kk == 2 && c->h_state->done && t == delta - 1) continue;
+                    float ms = 0.0f;
+                    HIPC(c, hipEventElapsedTime(&ms, c->pev[3 * t + kk], c->pev[3 * t + kk + 1]));
+                    c->pcount[kk] += 1;
+                    c->pms[kk] += ms;
+                }
+            }
+        }
+    }
+    return SHK_OK;
+}
+
+BrickOp<double> op64(shk_ctx *c) {
+    return BrickOp<double>{c->nbr.p, c->diag64.p, c->cx64.p, c->cy64.p, c->cz64.p, c->n_active};
+}
+BrickOp<float> op32(shk_ctx *c) {
+    return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active};
+}
+
+int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4]) {
+    launch_residual(c->stream, op64(c), c->b64.p, x, r_out, reduction(c), c->misc.p);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(c->h_misc, c->misc.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 4; ++i) out[i] = c->h_misc[i];
+    return SHK_OK;
+}
+
+void fill_true_residual(shk_result *res, const double rr[4]) {
+    res->b_inf = rr[3];
+    res->true_relres_inf = rr[3] > 0 ? rr[0] / rr[3] : 0.0;
+    res->true_relres_l2 = rr[2] > 0 ? std::sqrt(rr[1] / rr[2]) : 0.0;
+}
+
+// FP64 Krylov on the brick stencil: identity or Jacobi preconditioner.
+int solve_fp64(shk_ctx *c, shk_result *res) {
+    const int64_t n = (int64_t)c->n_active * kBrickCells;
+    const bool jac = c->prm.precond != 0;
+    launch_cg_init<double, double, double>(c->stream, n, c->b64.p, c->diag64.p, c->x64.p, c->r64.p, c->s64.p,
+                                           reduction(c), c->prm.residual, c->prm.max_iterations, jac);
+    const BrickOp<double> op = op64(c);
+    const Reduction red = reduction(c);
+    int rc = drive(c, [&]() {
+        launch_spmv_dot<double, double>(c->stream, op, c->s64.p, c->q64.p, red);
+        c->mark();
+        launch_axpy_norm<double, double>(c->stream, n, op.diag, c->x64.p, c->r64.p, c->s64.p, c->q64.p, red, jac);
+        c->mark();
+        launch_update_dir<double, double>(c->stream, n, op.diag, c->r64.p, c->s64.p, red, jac);
+        c->mark();
+    });
+    if (rc) return rc;
+    res->count = c->h_state->iter;
+    res->status = c->h_state->status;
+    res->reresid = c->h_state->rnorm0 > 0 ? c->h_state->rnorm / c->h_state->rnorm0 : c->h_state->rnorm;
+    res->outer = 1;
+    double rr[4];
+    if ((rc = true_residual(c, c->x64.p, nullptr, rr))) return rc;
+    fill_true_residual(res, rr);
+    return SHK_OK;
+}
+
+}  // namespace shk
+
+namespace {
+
+// Mixed precision: FP32 stencil/AXPY/Jacobi inner CG with FP64 dots, FP64 true
+// residual and correction between inner solves (iterative refinement).
+int solve_mixed(shk_ctx *c, shk_result *res) {
+    const int64_t n = (int64_t)c->n_active * kBrickCells;
+    const bool jac = c->prm.precond != 0;
+    HIPC(c, hipMemsetAsync(c->x64.p, 0, sizeof(double) * n, c->stream));
+    double rr[4];
+    int rc;
+    // r64 = b - A*0 = b, and the norms of b
+    if ((rc = true_residual(c, c->x64.p, c->r64.p, rr))) return rc;
+    const double b_inf = rr[3];
+    const double tol_abs = std::max(c->prm.residual, 1e-30) * b_inf;
+    double r_inf = rr[0];
+    int64_t total = 0;
+    int outer = 0;
+    int status = SHK_OK;
+    const BrickOp<float> op = op32(c);
+    while (r_inf > tol_abs && outer < std::max(1, c->prm.outer_max)) {
+        double inner_rel = std::max(c->prm.inner_residual, 0.5 * tol_abs / r_inf);
+        if (inner_rel > 0.5) inner_rel = 0.5;
+        launch_cg_init<float, float, double>(c->stream, n, c->r64.p, c->diag32.p, c->x32.p, c->r32.p, c->s32.p,
+                                             reduction(c), inner_rel, c->prm.max_iterations - total, jac);
+        const Reduction red = reduction(c);
+        rc = drive(c, [&]() {
+            launch_spmv_dot<float, float>(c->stream, op, c->s32.p, c->q32.p, red);
+            c->mark();
+            launch_axpy_norm<float, float>(c->stream, n, op.diag, c->x32.p, c->r32.p, c->s32.p, c->q32.p, red, jac);
+            c->mark();
+            launch_update_dir<float, float>(c->stream, n, op.diag, c->r32.p, c->s32.p, red, jac);
+            c->mark();
+        });
+        if (rc) return rc;
+        total += c->h_state->iter;
+        ++outer;
+        const int st = c->h_state->status;
+        launch_add_correction(c->stream, n, c->x64.p, c->x32.p);
+        if ((rc = true_residual(c, c->x64.p, c->r64.p, rr))) return rc;
+        r_inf = rr[0];
+        if (st == SHK_NAN || st == SHK_BREAKDOWN) {
+            status = st;
+            break;
+        }
+        if (total >= c->prm.max_iterations) break;
+    }
+    if (status == SHK_OK && r_inf > tol_abs) status = SHK_NOT_CONVERGED;
+    if (!(r_inf == r_inf)) status = SHK_NAN;
+    res->count = total;
+    res->outer = outer;
+    res->status = status;
+    res->reresid = b_inf > 0 ? r_inf / b_inf : 0.0;
+    fill_true_residual(res, rr);
+    return SHK_OK;
+}
+
+int validate_fields(shk_ctx *c, const void *const *ptrs, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!ptrs[i]) return c->fail(SHK_BAD_ARG, "field pointer %d is NULL", i);
+    return SHK_OK;
+}
+
+}  // namespace
+
+// ===================================================================== C-ABI
+
+extern "C" {
+
+int shk_abi_version(void) { return SHK_ABI_VERSION; }
+
+const char *shk_build_info(void) {
+    return "libshk_projection (gfx950; brick 8^3 matrix-free MAC pressure projection; FP64 + mixed CG; "
+           "learned flexible-CG preconditioner)";
+}
+
+const char *shk_last_error(const shk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out) {
+    if (!out) {
+        g_create_error = "out is NULL";
+        return SHK_BAD_ARG;
+    }
+    *out = nullptr;
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(dx > 0.0) || (int64_t)nx * ny * nz > (int64_t)1 << 31) {
+        g_create_error = "bad shape or dx";
+        return SHK_BAD_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_error = "no HIP device";
+        return SHK_NO_DEVICE;
+    }
+    if (device < 0) {
+        if (hipGetDevice(&device) != hipSuccess) device = 0;
+    }
+    if (device >= ndev) {
+        g_create_error = "device index out of range";
+        return SHK_BAD_ARG;
+    }
+    shk_ctx *c = new shk_ctx();
+    c->device = device;
+    c->dims = Dims{nx, ny, nz};
+    c->dx = dx;
+    c->nbx = (nx + kBrick - 1) / kBrick;
+    c->nby = (ny + kBrick - 1) / kBrick;
+    c->nbz = (nz + kBrick - 1) / kBrick;
+    auto bail = [&](hipError_t e, const char *what) {
+        g_create_error = std::string(what) + ": " + hipGetErrorString(e);
+        shk_destroy(c);
+        return e == hipErrorOutOfMemory ? SHK_OUT_OF_MEMORY : SHK_HIP_ERROR;
+    };
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    for (auto &ev : c->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return bail(e, "hipEventCreate");
+    if ((e = hipHostMalloc((void **)&c->h_state, sizeof(SolveState), hipHostMallocDefault)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void **)&c->h_misc, 16 * sizeof(double), hipHostMallocDefault)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
+    if ((e = c->state.ensure(1)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemset(c->state.p, 0, sizeof(SolveState))) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = c->misc.ensure(16)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = c->count.ensure(1)) != hipSuccess) return bail(e, "hipMalloc");
+    const int64_t G = (int64_t)nx * ny * nz;
+    for (int a = 0; a < 3; ++a) {
+        const int64_t nf = face_count(c->dims, a);
+        if ((e = c->area[a].ensure(nf)) != hipSuccess) return bail(e, "hipMalloc");
+        if ((e = c->rho[a].ensure(nf)) != hipSuccess) return bail(e, "hipMalloc");
+    }
+    if ((e = c->label.ensure(G)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = c->pressure.ensure(G)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMemset(c->pressure.p, 0, sizeof(float) * G)) != hipSuccess) return bail(e, "hipMemset");
+    if ((e = c->brick_flag.ensure((size_t)c->nbx * c->nby * c->nbz)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = c->brick_slot.ensure((size_t)c->nbx * c->nby * c->nbz)) != hipSuccess) return bail(e, "hipMalloc");
+    *out = c;
+    return SHK_OK;
+}
+
+void shk_destroy(shk_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->unet) unet_destroy(c->unet);
+    for (auto &b : c->area) b.release();
+    for (auto &b : c->rho) b.release();
+    for (auto &b : c->stage) b.release();
+    c->label.release();
+    c->pressure.release();
+    c->brick_flag.release();
+    c->brick_slot.release();
+    c->count.release();
+    c->coord.release();
+    c->nbr.release();
+    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->x64, &c->r64, &c->s64, &c->q64,
+                              &c->pa, &c->pb, &c->misc})
+        b->release();
+    for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32, &c->s32, &c->q32})
+        b->release();
+    c->state.release();
+    if (c->h_state) (void)hipHostFree(c->h_state);
+    if (c->h_misc) (void)hipHostFree(c->h_misc);
+    for (auto &ev : c->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto &ev : c->pev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int shk_configure(shk_ctx *c, const char *key, const char *value) {
+    if (!c) return SHK_BAD_ARG;
+    if (!key || !value) return c->fail(SHK_BAD_ARG, "key/value is NULL");
+    std::string k(key);
+    // accept the reference's scoped names ("PCG.Residual", "MacUtility.EpsFluid")
+    const size_t dot = k.rfind('.');
+    if (dot != std::string::npos) k = k.substr(dot + 1);
+    Params &p = c->prm;
+    double d;
+    int64_t i;
+    bool b;
+    if (k == "Residual") {
+        if (!parse_double(value, d) || !(d >= 0.0)) return c->fail(SHK_BAD_ARG, "bad Residual '%s'", value);
+        p.residual = d;
+    } else if (k == "MaxIterations") {
+        if (!parse_int(value, i) || i < 0) return c->fail(SHK_BAD_ARG, "bad MaxIterations '%s'", value);
+        p.max_iterations = i;
+    } else if (k == "ModifiedIC") {
+        if (!parse_double(value, d)) return c->fail(SHK_BAD_ARG, "bad ModifiedIC '%s'", value);
+        p.modified_ic = d;
+    } else if (k == "MinDiagRatio") {
+        if (!parse_double(value, d)) return c->fail(SHK_BAD_ARG, "bad MinDiagRatio '%s'", value);
+        p.min_diag_ratio = d;
+    } else if (k == "SecondOrderAccurateFluid") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad SecondOrderAccurateFluid '%s'", value);
+        p.so_fluid = b;
+    } else if (k == "SecondOrderAccurateSolid") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad SecondOrderAccurateSolid '%s'", value);
+        p.so_solid = b;
+    } else if (k == "Gain") {
+        if (!parse_double(value, d)) return c->fail(SHK_BAD_ARG, "bad Gain '%s'", value);
+        p.gain = d;
+    } else if (k == "WarmStart") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WarmStart '%s'", value);
+        if (b) return c->fail(SHK_BAD_ARG, "WarmStart=1 is not supported (only the default cold start)");
+    } else if (k == "EpsFluid") {
+        if (!parse_double(value, d) || d < 0) return c->fail(SHK_BAD_ARG, "bad EpsFluid '%s'", value);
+        p.eps_fluid = d;
+    } else if (k == "EpsSolid") {
+        if (!parse_double(value, d) || d < 0) return c->fail(SHK_BAD_ARG, "bad EpsSolid '%s'", value);
+        p.eps_solid = d;
+    } else if (k == "Precision") {
+        std::string v(value);
+        if (v == "fp64")
+            p.precision = 0;
+        else if (v == "mixed")
+            p.precision = 1;
+        else
+            return c->fail(SHK_BAD_ARG, "Precision must be fp64 or mixed, got '%s'", value);
+    } else if (k == "Preconditioner") {
+        std::string v(value);
+        if (v == "none")
+            p.precond = 0;
+        else if (v == "jacobi")
+            p.precond = 1;
+        else if (v == "unet")
+            p.precond = 2;
+        else
+            return c->fail(SHK_BAD_ARG, "Preconditioner must be none, jacobi or unet, got '%s'", value);
+    } else if (k == "OuterMax") {
+        if (!parse_int(value, i) || i < 1 || i > 50) return c->fail(SHK_BAD_ARG, "bad OuterMax '%s'", value);
+        p.outer_max = (int)i;
+    } else if (k == "InnerResidual") {
+        if (!parse_double(value, d) || !(d > 0.0) || d >= 1.0)
+            return c->fail(SHK_BAD_ARG, "bad InnerResidual '%s'", value);
+        p.inner_residual = d;
+    } else if (k == "CheckInterval") {
+        if (!parse_int(value, i) || i < 1 || i > 4096) return c->fail(SHK_BAD_ARG, "bad CheckInterval '%s'", value);
+        p.check_interval = (int)i;
+    } else if (k == "HostPointers") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad HostPointers '%s'", value);
+        p.host_pointers = b;
+    } else if (k == "ProfileKernels") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad ProfileKernels '%s'", value);
+        c->profile = b;
+        for (int q = 0; q < 3; ++q) {
+            c->pcount[q] = 0;
+            c->pms[q] = 0.0;
+        }
+    } else if (k == "UnetWeights") {
+        p.unet_weights = value;
+        if (c->unet) {
+            unet_destroy(c->unet);
+            c->unet = nullptr;
+        }
+    } else {
+        return c->fail(SHK_BAD_ARG, "unknown key '%s'", key);
+    }
+    return SHK_OK;
+}
+
+int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float *solid, const float *us,
+                const float *vs, const float *ws, const float *fluid, shk_result *res) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    const void *ptrs[8] = {u, v, w, solid, us, vs, ws, fluid};
+    int rc = validate_fields(c, ptrs, 8);
+    if (rc) return rc;
+    if (!(dt == dt) || std::isinf(dt)) return c->fail(SHK_BAD_ARG, "dt is not finite");
+    HIPC(c, hipSetDevice(c->device));
+    shk_result local{};
+    if (!res) res = &local;
+    *res = shk_result{};
+    res->precision = c->prm.precision;
+    const Dims d = c->dims;
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    const int64_t nf[3] = {face_count(d, 0), face_count(d, 1), face_count(d, 2)};
+    float *vel[3] = {u, v, w};
+    const float *svel[3] = {us, vs, ws};
+    const float *dsolid = solid, *dfluid = fluid;
+    if (c->prm.host_pointers) {
+        // stage host fields: 0..2 u v w, 3..5 us vs ws, 6 solid, 7 fluid
+        const float *src[8] = {u, v, w, us, vs, ws, solid, fluid};
+        const int64_t cnt[8] = {nf[0], nf[1], nf[2], nf[0], nf[1], nf[2], nn, G};
+        for (int i = 0; i < 8; ++i) {
+            HIPC(c, c->stage[i].ensure(cnt[i]));
+            HIPC(c, hipMemcpyAsync(c->stage[i].p, src[i], sizeof(float) * cnt[i], hipMemcpyHostToDevice, c->stream));
+        }
+        for (int a = 0; a < 3; ++a) {
+            vel[a] = c->stage[a].p;
+            svel[a] = c->stage[3 + a].p;
+        }
+        dsolid = c->stage[6].p;
+        dfluid = c->stage[7].p;
+    }
+    float *area[3] = {c->area[0].p, c->area[1].p, c->area[2].p};
+    float *rho[3] = {c->rho[0].p, c->rho[1].p, c->rho[2].p};
+
+    HIPC(c, hipEventRecord(c->ev[0], c->stream));
+    launch_fractions(c->stream, d, dsolid, dfluid, c->prm.eps_solid, c->prm.eps_fluid, c->prm.so_solid,
+                     c->prm.so_fluid, area, rho);
+    HIPC(c, hipMemsetAsync(c->brick_flag.p, 0, (size_t)c->nbx * c->nby * c->nbz, c->stream));
+    HIPC(c, hipMemsetAsync(c->count.p, 0, sizeof(unsigned long long), c->stream));
+    launch_label(c->stream, d, dfluid, area, rho, c->label.p, c->brick_flag.p, c->nbx, c->nby, c->count.p);
+    HIPC(c, hipGetLastError());
+    unsigned long long nunk = 0;
+    HIPC(c, hipMemcpyAsync(&nunk, c->count.p, sizeof nunk, hipMemcpyDeviceToHost, c->stream));
+    if ((rc = build_bricks(c))) return rc;
+    c->unknowns = (int64_t)nunk;
+    res->unknowns = c->unknowns;
+    res->active_bricks = c->n_active;
+    if (nunk == 0) {
+        // macpressuresolver3.cpp:170: nothing is solved and the velocity is untouched
+        c->have_system = false;
+        HIPC(c, hipStreamSynchronize(c->stream));
+        return SHK_OK;
+    }
+    const bool need32 = c->prm.precision == 1 || c->prm.precond == 2;
+    launch_coeff(c->stream, d, dt, c->dx, c->n_active, c->coord.p, area, rho, c->label.p, vel, svel, c->diag64.p,
+                 c->cx64.p, c->cy64.p, c->cz64.p, c->b64.p, need32 ? c->diag32.p : nullptr,
+                 need32 ? c->cx32.p : nullptr, need32 ? c->cy32.p : nullptr, need32 ? c->cz32.p : nullptr);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipEventRecord(c->ev[1], c->stream));
+
+    if (c->prm.precond == 2) {
+        rc = unet_solve(c, res);
+    } else if (c->prm.precision == 1) {
+        rc = solve_mixed(c, res);
+    } else {
+        rc = solve_fp64(c, res);
+    }
+    if (rc) return rc;
+    c->have_system = true;
+    HIPC(c, hipEventRecord(c->ev[2], c->stream));
+    launch_scatter(c->stream, d, c->brick_slot.p, c->nbx, c->nby, c->label.p, c->x64.p, c->pressure.p);
+    launch_update_velocity(c->stream, d, dt, c->dx, c->pressure.p, area, rho, svel, vel);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipEventRecord(c->ev[3], c->stream));
+    if (c->prm.host_pointers) {
+        for (int a = 0; a < 3; ++a)
+            HIPC(c, hipMemcpyAsync((a == 0 ? u : (a == 1 ? v : w)), vel[a], sizeof(float) * nf[a],
+                                   hipMemcpyDeviceToHost, c->stream));
+        c->h_pressure.resize(G);
+        HIPC(c, hipMemcpyAsync(c->h_pressure.data(), c->pressure.p, sizeof(float) * G, hipMemcpyDeviceToHost,
+                               c->stream));
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if ((rc = sync_and_time(c, 0, 1, res->ms_setup))) return rc;
+    if ((rc = sync_and_time(c, 1, 2, res->ms_solve))) return rc;
+    if ((rc = sync_and_time(c, 2, 3, res->ms_update))) return rc;
+    return res->status;
+}
+
+const float *shk_pressure(const shk_ctx *c) {
+    if (!c) return nullptr;
+    if (c->prm.host_pointers) return c->h_pressure.empty() ? nullptr : c->h_pressure.data();
+    return c->pressure.p;
+}
+
+int shk_copy_pressure(shk_ctx *c, float *dst) {
+    if (!c || !dst) return SHK_BAD_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    const int64_t G = (int64_t)c->dims.nx * c->dims.ny * c->dims.nz;
+    const hipMemcpyKind kind = c->prm.host_pointers ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIPC(c, hipMemcpyAsync(dst, c->pressure.p, sizeof(float) * G, kind, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_kernel_stats(shk_ctx *c, const char *kernel, int64_t *launches, double *total_ms) {
+    if (!c || !kernel || !launches || !total_ms) return SHK_BAD_ARG;
+    const std::string k(kernel);
+    const int idx = k == "spmv_dot" ? 0 : (k == "axpy_norm" ? 1 : (k == "update_dir" ? 2 : -1));
+    if (idx < 0) return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (spmv_dot, axpy_norm, update_dir)", kernel);
+    *launches = c->pcount[idx];
+    *total_ms = c->pms[idx];
+    return SHK_OK;
+}
+
+int shk_export_fractions(shk_ctx *c, float *au, float *av, float *aw, float *ru, float *rv, float *rw) {
+    if (!c) return SHK_BAD_ARG;
+    HIPC(c, hipSetDevice(c->device));
+    float *dst[6] = {au, av, aw, ru, rv, rw};
+    const hipMemcpyKind kind = c->prm.host_pointers ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    for (int i = 0; i < 6; ++i) {
+        if (!dst[i]) continue;
+        const int a = i % 3;
+        const float *src = i < 3 ? c->area[a].p : c->rho[a].p;
+        HIPC(c, hipMemcpyAsync(dst[i], src, sizeof(float) * face_count(c->dims, a), kind, c->stream));
+    }
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_export_system(shk_ctx *c, uint8_t *label, double *diag, double *cx, double *cy, double *cz, double *rhs,
+                      double *x) {
+    if (!c) return SHK_BAD_ARG;
+    if (!c->have_system) return c->fail(SHK_BAD_ARG, "no linear system (no projection with unknowns yet)");
+    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_export_system takes device pointers");
+    HIPC(c, hipSetDevice(c->device));
+    launch_export(c->stream, c->dims, c->brick_slot.p, c->nbx, c->nby, c->label.p, c->diag64.p, c->cx64.p, c->cy64.p,
+                  c->cz64.p, c->b64.p, c->x64.p, label, diag, cx, cy, cz, rhs, x);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_divergence(shk_ctx *c, double dt, const float *u, const float *v, const float *w, const float *us,
+                   const float *vs, const float *ws, int use_fp64_update, double *max_abs_div) {
+    if (!c || !max_abs_div) return SHK_BAD_ARG;
+    const void *ptrs[6] = {u, v, w, us, vs, ws};
+    int rc = validate_fields(c, ptrs, 6);
+    if (rc) return rc;
+    if (!c->have_system) return c->fail(SHK_BAD_ARG, "no linear system (no projection with unknowns yet)");
+    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_divergence takes device pointers");
+    HIPC(c, hipSetDevice(c->device));
+    const int maxb = 4096;
+    HIPC(c, c->pa.ensure(maxb));
+    float *area[3] = {c->area[0].p, c->area[1].p, c->area[2].p};
+    float *rho[3] = {c->rho[0].p, c->rho[1].p, c->rho[2].p};
+    const float *vel[3] = {u, v, w};
+    const float *svel[3] = {us, vs, ws};
+    const int g = launch_divergence(c->stream, c->dims, dt, c->dx, area, rho, c->label.p, vel, svel, use_fp64_update,
+                                    c->brick_slot.p, c->nbx, c->nby, c->x64.p, c->pa.p, maxb);
+    HIPC(c, hipGetLastError());
+    std::vector<double> part(g);
+    HIPC(c, hipMemcpyAsync(part.data(), c->pa.p, sizeof(double) * g, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double m = 0.0;
+    for (double p : part) m = std::max(m, p);
+    *max_abs_div = m;
+    return SHK_OK;
+}
+
+int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                  const double *b, double *x, shk_result *res) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    if (rows < 0 || rows > ((int64_t)1 << 40)) return c->fail(SHK_BAD_ARG, "bad row count");
+    if (!rowptr || !cols || !vals || !b || !x) return c->fail(SHK_BAD_ARG, "NULL CSR/vector pointer");
+    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_solve_csr takes device pointers");
+    HIPC(c, hipSetDevice(c->device));
+    shk_result local{};
+    if (!res) res = &local;
+    *res = shk_result{};
+    res->unknowns = rows;
+    res->outer = 1;
+    if (rows == 0) return SHK_OK;
+    const int blocks = blocks_for(rows);
+    int rc = ensure_partials(c, blocks);
+    if (rc) return rc;
+    HIPC(c, c->diag64.ensure(rows));
+    HIPC(c, c->r64.ensure(rows));
+    HIPC(c, c->s64.ensure(rows));
+    HIPC(c, c->q64.ensure(rows));
+    CsrOp op{rowptr, cols, vals, c->diag64.p, rows};
+    const bool jac = c->prm.precond != 0;
+    HIPC(c, hipEventRecord(c->ev[0], c->stream));
+    launch_csr_diag(c->stream, op, c->diag64.p);
+    launch_cg_init<double, double, double>(c->stream, rows, b, c->diag64.p, x, c->r64.p, c->s64.p, reduction(c),
+                                           c->prm.residual, c->prm.max_iterations, jac);
+    const Reduction red = reduction(c);
+    rc = drive(c, [&]() {
+        launch_spmv_dot_csr(c->stream, op, c->s64.p, c->q64.p, red);
+        c->mark();
+        launch_axpy_norm<double, double>(c->stream, rows, c->diag64.p, x, c->r64.p, c->s64.p, c->q64.p, red, jac);
+        c->mark();
+        launch_update_dir<double, double>(c->stream, rows, c->diag64.p, c->r64.p, c->s64.p, red, jac);
+        c->mark();
+    });
+    if (rc) return rc;
+    HIPC(c, hipEventRecord(c->ev[1], c->stream));
+    res->count = c->h_state->iter;
+    res->status = c->h_state->status;
+    res->reresid = c->h_state->rnorm0 > 0 ? c->h_state->rnorm / c->h_state->rnorm0 : c->h_state->rnorm;
+    launch_residual_csr(c->stream, op, b, x, nullptr, reduction(c), c->misc.p);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(c->h_misc, c->misc.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double rr[4] = {c->h_misc[0], c->h_misc[1], c->h_misc[2], c->h_misc[3]};
+    fill_true_residual(res, rr);
+    if ((rc = sync_and_time(c, 0, 1, res->ms_solve))) return rc;
+    c->have_system = false;
+    return res->status;
+}
+
+}  // extern "C"

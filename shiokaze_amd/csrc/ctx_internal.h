@@ -1,0 +1,161 @@
+// ctx_internal.h -- the context behind the C-ABI handle, shared by ctx.hip and
+// the learned preconditioner (unet.hip).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "krylov.h"
+#include "shk_common.h"
+#include "shk_fractions.h"
+#include "shk_projection.h"
+
+namespace shk {
+struct UnetPrecond;
+}
+
+namespace shk {
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            n = 0;
+            if (e != hipSuccess) return e;
+        }
+        if (want == 0) want = 1;
+        hipError_t e = hipMalloc((void **)&p, sizeof(T) * want);
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct Params {
+    double residual = 1e-4;           // pcg.cpp:76
+    int64_t max_iterations = 30000;   // pcg.cpp:77
+    double modified_ic = 0.97;        // pcg.cpp:78
+    double min_diag_ratio = 0.25;     // pcg.cpp:79
+    bool so_fluid = true;             // macpressuresolver3.cpp:352
+    bool so_solid = true;             // macpressuresolver3.cpp:353
+    double gain = 1.0;                // macpressuresolver3.cpp:350
+    double eps_fluid = 1e-2;          // macutility3.cpp:522
+    double eps_solid = 1e-2;          // macutility3.cpp:523
+    int precision = 0;                // 0 fp64, 1 mixed
+    int precond = 1;                  // 0 none, 1 jacobi, 2 unet
+    int outer_max = 3;
+    double inner_residual = 1e-5;
+    int check_interval = 32;
+    bool host_pointers = false;
+    std::string unet_weights;
+};
+
+}  // namespace shk
+
+struct shk_ctx {
+    using Dims = shk::Dims;
+    using Params = shk::Params;
+    template <class T>
+    using DevBuf = shk::DevBuf<T>;
+    using SolveState = shk::SolveState;
+
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Dims dims{0, 0, 0};
+    double dx = 0.0;
+    int nbx = 0, nby = 0, nbz = 0;
+    Params prm;
+    std::string err;
+
+    // dense workspace
+    DevBuf<float> area[3], rho[3];
+    DevBuf<uint8_t> label;
+    DevBuf<float> pressure;
+    DevBuf<uint8_t> brick_flag;
+    DevBuf<int> brick_slot;
+    DevBuf<unsigned long long> count;
+    // host-pointer staging
+    DevBuf<float> stage[10];
+    std::vector<float> h_pressure;
+    // bricks
+    int n_active = 0;
+    int64_t unknowns = 0;
+    bool have_system = false;
+    std::vector<int4> h_coord;
+    std::vector<int> h_nbr, h_slot;
+    std::vector<uint8_t> h_flag;
+    DevBuf<int4> coord;
+    DevBuf<int> nbr;
+    DevBuf<double> diag64, cx64, cy64, cz64, b64, x64, r64, s64, q64;
+    DevBuf<float> diag32, cx32, cy32, cz32, x32, r32, s32, q32;
+    DevBuf<double> pa, pb, misc;
+    DevBuf<SolveState> state;
+    SolveState *h_state = nullptr;
+    double *h_misc = nullptr;
+    hipEvent_t ev[6] = {};
+    // learned preconditioner
+    shk::UnetPrecond *unet = nullptr;
+    // per-kernel timing of the Krylov loop (ProfileKernels=1)
+    bool profile = false;
+    std::vector<hipEvent_t> pev;
+    size_t pcur = 0;
+    int64_t pcount[3] = {0, 0, 0};
+    double pms[3] = {0.0, 0.0, 0.0};
+    // records the next profiling event after a Krylov kernel launch
+    void mark() {
+        if (profile && pcur < pev.size()) (void)hipEventRecord(pev[pcur++], stream);
+    }
+
+    int fail(int code, const char *fmt, ...) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPC(ctx, expr)                                                                                     \
+    do {                                                                                                    \
+        hipError_t _e = (expr);                                                                             \
+        if (_e != hipSuccess)                                                                               \
+            return (ctx)->fail(_e == hipErrorOutOfMemory ? SHK_OUT_OF_MEMORY : SHK_HIP_ERROR, "%s: %s (%s:%d)", \
+                               #expr, hipGetErrorString(_e), __FILE__, __LINE__);                           \
+    } while (0)
+
+namespace shk {
+
+// Iterates `iterate` CheckInterval times between polls of the device state.
+int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg);
+template <class F>
+int drive(shk_ctx *c, F &&f) {
+    return drive_impl(c, [](void *p) { (*static_cast<F *>(p))(); }, &f);
+}
+Reduction reduction(shk_ctx *c);
+BrickOp<double> op64(shk_ctx *c);
+BrickOp<float> op32(shk_ctx *c);
+// r = b - A x (FP64); out = {max|r|, sum r^2, sum b^2, max|b|}
+int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4]);
+void fill_true_residual(shk_result *res, const double rr[4]);
+int solve_fp64(shk_ctx *c, shk_result *res);
+
+// learned preconditioner (unet.hip)
+void unet_destroy(UnetPrecond *u);
+int unet_solve(shk_ctx *c, shk_result *res);
+
+}  // namespace shk

@@ -1,0 +1,39 @@
+// setup.h -- host launchers of the per-projection kernels (setup.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "shk_fractions.h"
+
+namespace shk {
+
+void launch_fractions(hipStream_t st, Dims d, const float *solid, const float *fluid, double eps_solid,
+                      double eps_fluid, int so_solid, int so_fluid, float *const area[3], float *const rho[3]);
+
+void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[3], float *const rho[3], uint8_t *label,
+                  uint8_t *brick_flag, int nbx, int nby, unsigned long long *count);
+
+void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
+                  float *const area[3], float *const rho[3], const uint8_t *label, const float *const vel[3],
+                  const float *const svel[3], double *diag, double *cx, double *cy, double *cz, double *rhs,
+                  float *diag32, float *cx32, float *cy32, float *cz32);
+
+void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
+                    const double *x, float *pressure);
+
+void launch_update_velocity(hipStream_t st, Dims d, double dt, double dx, const float *pressure, float *const area[3],
+                            float *const rho[3], const float *const svel[3], float *const vel[3]);
+
+// Writes one partial max per block into `partial`; returns the block count.
+int launch_divergence(hipStream_t st, Dims d, double dt, double dx, float *const area[3], float *const rho[3],
+                      const uint8_t *label, const float *const vel[3], const float *const svel[3], int use_fp64_update,
+                      const int *brick_slot, int nbx, int nby, const double *x, double *partial, int max_blocks);
+
+void launch_export(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
+                   const double *diag, const double *cx, const double *cy, const double *cz, const double *rhs,
+                   const double *x, uint8_t *o_label, double *o_diag, double *o_cx, double *o_cy, double *o_cz,
+                   double *o_rhs, double *o_x);
+
+}  // namespace shk

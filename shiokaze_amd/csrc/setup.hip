@@ -1,0 +1,366 @@
+// setup.hip -- per-projection kernels around the Krylov loop:
+//   face fractions, unknown labels, brick activity, stencil coefficients + rhs,
+//   pressure scatter, velocity update (gradient subtract), diagnostics.
+//
+// All floating point here rounds exactly as the reference's CPU code
+// (no contraction, IEEE division), so areas, rhos, couplings, rhs, pressure and
+// the updated velocity are bit-identical to a faithful CPU evaluation.
+#include "setup.h"
+
+#include "shk_common.h"
+#include "shk_fractions.h"
+
+namespace shk {
+
+namespace {
+
+constexpr int kNT = 256;
+
+inline unsigned grid_for(int64_t n, int nt = kNT) {
+    int64_t g = (n + nt - 1) / nt;
+    if (g > (1 << 20)) g = 1 << 20;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
+// ---------------------------------------------------------------- fractions (a3, a4, a5)
+
+template <int DIM>
+__global__ void k_fractions(Dims d, const float *__restrict__ solid, const float *__restrict__ fluid,
+                            double eps_solid, double eps_fluid, int second_order_solid, int second_order_fluid,
+                            float *__restrict__ area, float *__restrict__ rho) {
+#pragma clang fp contract(off)
+    const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
+    const int64_t n = face_count(d, DIM);
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < n; f += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(f % fx);
+        const int j = (int)((f / fx) % fy);
+        const int k = (int)(f / ((int64_t)fx * fy));
+        double a = face_area(d, solid, DIM, i, j, k, eps_solid);
+        double r = face_rho(d, fluid, DIM, i, j, k, eps_fluid);
+        // first-order switches (macpressuresolver3.cpp:83-93): nonzero -> 1
+        float af = (float)a, rf = (float)r;
+        if (!second_order_solid && af != 0.0f) af = 1.0f;
+        if (!second_order_fluid && rf != 0.0f) rf = 1.0f;
+        area[f] = af;
+        rho[f] = rf;
+    }
+}
+
+// ---------------------------------------------------------------- labels (a6)
+
+struct Faces {
+    const float *a[3];
+    const float *r[3];
+};
+
+__device__ inline bool is_unknown(const Dims &d, const float *fluid, const Faces &fc, int i, int j, int k) {
+    if (!(fluid[cell_index(d, i, j, k)] < 0.0f)) return false;
+    for (int nq = 0; nq < 6; ++nq) {
+        int qi, qj, qk, fi, fj, fk, dim, sgn;
+        neighbour(nq, qi, qj, qk, fi, fj, fk, dim, sgn);
+        qi += i;
+        qj += j;
+        qk += k;
+        if (qi < 0 || qi >= d.nx || qj < 0 || qj >= d.ny || qk < 0 || qk >= d.nz) continue;
+        if (!(fluid[cell_index(d, qi, qj, qk)] < 0.0f)) continue;
+        const int64_t f = face_index(d, dim, i + fi, j + fj, k + fk);
+        if (fc.a[dim][f] != 0.0f && fc.r[dim][f] != 0.0f) return true;
+    }
+    return false;
+}
+
+__global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8_t *__restrict__ label,
+                        uint8_t *__restrict__ brick_flag, int nbx, int nby, unsigned long long *__restrict__ count) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    unsigned long long local = 0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        const bool u = is_unknown(d, fluid, fc, i, j, k);
+        label[c] = u ? 1 : 0;
+        if (u) {
+            brick_flag[(i / kBrick) + (int64_t)nbx * ((j / kBrick) + (int64_t)nby * (k / kBrick))] = 1;
+            ++local;
+        }
+    }
+    // integer count: order independent
+    for (int o = kWave / 2; o > 0; o >>= 1) local += __shfl_xor(local, o, kWave);
+    if ((threadIdx.x % kWave) == 0 && local) atomicAdd(count, local);
+}
+
+// ---------------------------------------------------------------- coefficients + rhs (a7)
+
+__global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ brick_coord, Faces fc,
+                        const uint8_t *__restrict__ label, const float *__restrict__ u, const float *__restrict__ v,
+                        const float *__restrict__ w, const float *__restrict__ us, const float *__restrict__ vs,
+                        const float *__restrict__ ws, double *__restrict__ diag, double *__restrict__ cx,
+                        double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ rhs,
+                        float *__restrict__ diag32, float *__restrict__ cx32, float *__restrict__ cy32,
+                        float *__restrict__ cz32) {
+#pragma clang fp contract(off)
+    const int slot = blockIdx.x;
+    const int l = threadIdx.x;
+    const int4 bc = brick_coord[slot];
+    const int i = bc.x * kBrick + (l & 7), j = bc.y * kBrick + ((l >> 3) & 7), k = bc.z * kBrick + (l >> 6);
+    const int64_t e = (int64_t)slot * kBrickCells + l;
+    double cd = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, b = 0.0;
+    if (i < d.nx && j < d.ny && k < d.nz) {
+        const double dx2 = dx * dx;
+        // low-face couplings dt*area/(dx^2*rho) (macpressuresolver3.cpp:199), stored
+        // for every cell so a neighbour's high face can be read from it.
+        const int coord[3] = {i, j, k};
+        double low[3];
+        for (int a = 0; a < 3; ++a) {
+            const int64_t f = face_index(d, a, i, j, k);
+            const double area = fc.a[a][f], rho = fc.r[a][f];
+            low[a] = (coord[a] > 0 && area != 0.0 && rho != 0.0) ? dt * area / (dx2 * rho) : 0.0;
+        }
+        c0 = low[0];
+        c1 = low[1];
+        c2 = low[2];
+        if (label[cell_index(d, i, j, k)]) {
+            const float *vel[3] = {u, v, w};
+            const float *svel[3] = {us, vs, ws};
+            double diagonal = 0.0;
+            for (int nq = 0; nq < 6; ++nq) {
+                int qi, qj, qk, fi, fj, fk, dim, sgn;
+                neighbour(nq, qi, qj, qk, fi, fj, fk, dim, sgn);
+                qi += i;
+                qj += j;
+                qk += k;
+                const int64_t f = face_index(d, dim, i + fi, j + fj, k + fk);
+                const double area = fc.a[dim][f];
+                if (!(qi < 0 || qi >= d.nx || qj < 0 || qj >= d.ny || qk < 0 || qk >= d.nz)) {
+                    if (area != 0.0) {
+                        const double rho = fc.r[dim][f];
+                        if (rho != 0.0) diagonal += dt * area / (dx2 * rho);
+                    }
+                }
+                b += -sgn * area * (double)vel[dim][f] / dx;
+                if (area < 1.0) {
+                    const double usf = svel[dim][f];
+                    b += -sgn * (1.0 - area) * usf / dx;
+                }
+            }
+            cd = diagonal;
+        } else {
+            b = 0.0;
+        }
+    }
+    diag[e] = cd;
+    cx[e] = c0;
+    cy[e] = c1;
+    cz[e] = c2;
+    rhs[e] = b;
+    if (diag32) {
+        diag32[e] = (float)cd;
+        cx32[e] = (float)c0;
+        cy32[e] = (float)c1;
+        cz32[e] = (float)c2;
+    }
+}
+
+// ---------------------------------------------------------------- scatter + update (a14, a15)
+
+__device__ inline int64_t brick_elem(const Dims &d, const int *brick_slot, int nbx, int nby, int i, int j, int k) {
+    const int s = brick_slot[(i / kBrick) + (int64_t)nbx * ((j / kBrick) + (int64_t)nby * (k / kBrick))];
+    if (s < 0) return -1;
+    return (int64_t)s * kBrickCells + (i & 7) + 8 * (j & 7) + 64 * (k & 7);
+}
+
+__global__ void k_scatter(Dims d, const int *__restrict__ brick_slot, int nbx, int nby, const uint8_t *__restrict__ label,
+                          const double *__restrict__ x, float *__restrict__ pressure) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        float p = 0.0f;
+        if (label[c]) {
+            const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+            if (e >= 0) p = (float)x[e];
+        }
+        pressure[c] = p;
+    }
+}
+
+template <int DIM>
+__global__ void k_update_velocity(Dims d, double dt, double dx, const float *__restrict__ pressure,
+                                  const float *__restrict__ area, const float *__restrict__ rho,
+                                  const float *__restrict__ svel, float *__restrict__ vel) {
+#pragma clang fp contract(off)
+    const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
+    const int n = DIM == 0 ? d.nx : (DIM == 1 ? d.ny : d.nz);
+    const int64_t nf = face_count(d, DIM);
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(f % fx);
+        const int j = (int)((f / fx) % fy);
+        const int k = (int)(f / ((int64_t)fx * fy));
+        const int pi = DIM == 0 ? i : (DIM == 1 ? j : k);
+        const double r = rho[f];
+        if (area[f] != 0.0f && r != 0.0) {
+            if (pi == 0 || pi == n) {
+                vel[f] = svel[f];
+            } else {
+                const float p1 = pressure[cell_index(d, i, j, k)];
+                const float p0 = pressure[cell_index(d, i - (DIM == 0), j - (DIM == 1), k - (DIM == 2))];
+                const float dp = p1 - p0;
+                const float inc = (float)(dt * (double)dp / (r * dx));
+                vel[f] = vel[f] - inc;
+            }
+        } else {
+            vel[f] = 0.0f;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- diagnostics
+
+__device__ inline double xval(const Dims &d, const int *brick_slot, int nbx, int nby, const double *x, int i, int j,
+                              int k) {
+    if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.0;
+    const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+    return e >= 0 ? x[e] : 0.0;
+}
+
+__global__ void k_divergence(Dims d, double dt, double dx, Faces fc, const uint8_t *__restrict__ label,
+                             const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
+                             const float *__restrict__ us, const float *__restrict__ vs, const float *__restrict__ ws,
+                             int use_fp64_update, const int *__restrict__ brick_slot, int nbx, int nby,
+                             const double *__restrict__ x, double *__restrict__ partial) {
+#pragma clang fp contract(off)
+    __shared__ double lds[kNT / kWave];
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    const float *vel[3] = {u, v, w};
+    const float *svel[3] = {us, vs, ws};
+    double best = 0.0;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        if (!label[c]) continue;
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        double acc = 0.0;
+        for (int nq = 0; nq < 6; ++nq) {
+            int qi, qj, qk, fi, fj, fk, dim, sgn;
+            neighbour(nq, qi, qj, qk, fi, fj, fk, dim, sgn);
+            const int ai = i + fi, aj = j + fj, ak = k + fk;  // face (dim, ai, aj, ak)
+            const int64_t f = face_index(d, dim, ai, aj, ak);
+            const double area = fc.a[dim][f];
+            double uf = vel[dim][f];
+            if (use_fp64_update) {
+                const double r = fc.r[dim][f];
+                const int pi = dim == 0 ? ai : (dim == 1 ? aj : ak);
+                const int n = dim == 0 ? d.nx : (dim == 1 ? d.ny : d.nz);
+                if (area != 0.0 && r != 0.0) {
+                    if (pi == 0 || pi == n) {
+                        uf = svel[dim][f];
+                    } else {
+                        const double p1 = xval(d, brick_slot, nbx, nby, x, ai, aj, ak);
+                        const double p0 = xval(d, brick_slot, nbx, nby, x, ai - (dim == 0), aj - (dim == 1),
+                                               ak - (dim == 2));
+                        uf = uf - dt * (p1 - p0) / (r * dx);
+                    }
+                } else {
+                    uf = 0.0;
+                }
+            }
+            acc += -sgn * area * uf / dx;
+            if (area < 1.0) acc += -sgn * (1.0 - area) * (double)svel[dim][f] / dx;
+        }
+        best = fmax(best, fabs(acc));
+    }
+    const double m = block_max<kNT>(best, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+__global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, int nby, const uint8_t *__restrict__ label,
+                         const double *__restrict__ diag, const double *__restrict__ cx, const double *__restrict__ cy,
+                         const double *__restrict__ cz, const double *__restrict__ rhs, const double *__restrict__ x,
+                         uint8_t *o_label, double *o_diag, double *o_cx, double *o_cy, double *o_cz, double *o_rhs,
+                         double *o_x) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+        if (o_label) o_label[c] = label[c];
+        if (o_diag) o_diag[c] = e >= 0 ? diag[e] : 0.0;
+        if (o_cx) o_cx[c] = e >= 0 ? cx[e] : 0.0;
+        if (o_cy) o_cy[c] = e >= 0 ? cy[e] : 0.0;
+        if (o_cz) o_cz[c] = e >= 0 ? cz[e] : 0.0;
+        if (o_rhs) o_rhs[c] = e >= 0 ? rhs[e] : 0.0;
+        if (o_x) o_x[c] = e >= 0 ? x[e] : 0.0;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+void launch_fractions(hipStream_t st, Dims d, const float *solid, const float *fluid, double eps_solid,
+                      double eps_fluid, int so_solid, int so_fluid, float *const area[3], float *const rho[3]) {
+    hipLaunchKernelGGL(k_fractions<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
+                       eps_fluid, so_solid, so_fluid, area[0], rho[0]);
+    hipLaunchKernelGGL(k_fractions<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
+                       eps_fluid, so_solid, so_fluid, area[1], rho[1]);
+    hipLaunchKernelGGL(k_fractions<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
+                       eps_fluid, so_solid, so_fluid, area[2], rho[2]);
+}
+
+void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[3], float *const rho[3], uint8_t *label,
+                  uint8_t *brick_flag, int nbx, int nby, unsigned long long *count) {
+    Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_label, dim3(grid_for(G)), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
+}
+
+void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
+                  float *const area[3], float *const rho[3], const uint8_t *label, const float *const vel[3],
+                  const float *const svel[3], double *diag, double *cx, double *cy, double *cz, double *rhs,
+                  float *diag32, float *cx32, float *cy32, float *cz32) {
+    Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
+    if (n_active <= 0) return;
+    hipLaunchKernelGGL(k_coeff, dim3(n_active), dim3(kBrickCells), 0, st, d, dt, dx, brick_coord, fc, label, vel[0],
+                       vel[1], vel[2], svel[0], svel[1], svel[2], diag, cx, cy, cz, rhs, diag32, cx32, cy32, cz32);
+}
+
+void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
+                    const double *x, float *pressure) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, x, pressure);
+}
+
+void launch_update_velocity(hipStream_t st, Dims d, double dt, double dx, const float *pressure, float *const area[3],
+                            float *const rho[3], const float *const svel[3], float *const vel[3]) {
+    hipLaunchKernelGGL(k_update_velocity<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, dt, dx, pressure,
+                       area[0], rho[0], svel[0], vel[0]);
+    hipLaunchKernelGGL(k_update_velocity<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, pressure,
+                       area[1], rho[1], svel[1], vel[1]);
+    hipLaunchKernelGGL(k_update_velocity<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, dt, dx, pressure,
+                       area[2], rho[2], svel[2], vel[2]);
+}
+
+int launch_divergence(hipStream_t st, Dims d, double dt, double dx, float *const area[3], float *const rho[3],
+                      const uint8_t *label, const float *const vel[3], const float *const svel[3], int use_fp64_update,
+                      const int *brick_slot, int nbx, int nby, const double *x, double *partial, int max_blocks) {
+    Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    unsigned g = grid_for(G);
+    if ((int)g > max_blocks) g = (unsigned)max_blocks;
+    hipLaunchKernelGGL(k_divergence, dim3(g), dim3(kNT), 0, st, d, dt, dx, fc, label, vel[0], vel[1], vel[2], svel[0],
+                       svel[1], svel[2], use_fp64_update, brick_slot, nbx, nby, x, partial);
+    return (int)g;
+}
+
+void launch_export(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
+                   const double *diag, const double *cx, const double *cy, const double *cz, const double *rhs,
+                   const double *x, uint8_t *o_label, double *o_diag, double *o_cx, double *o_cy, double *o_cz,
+                   double *o_rhs, double *o_x) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_export, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, diag, cx, cy, cz,
+                       rhs, x, o_label, o_diag, o_cx, o_cy, o_cz, o_rhs, o_x);
+}
+
+}  // namespace shk

@@ -1,0 +1,125 @@
+// shk_common.h -- device helpers shared by the gfx950 kernels: 64-lane wave
+// reductions, block reductions, the deterministic last-block grid reduction and
+// the Krylov solve state that lives in device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace shk {
+
+constexpr int kBrick = 8;                 // brick edge (cells)
+constexpr int kBrickCells = 512;          // 8^3 cells, one workgroup per brick
+constexpr int kWave = 64;                 // CDNA wavefront width
+constexpr int kXcds = 8;                  // MI355X accelerator complex dies
+
+// Krylov state kept on the device so that iterations are launched back to back
+// without a host round trip; the host polls `done` every few iterations.
+struct SolveState {
+    double rho;      // r.z of the current residual
+    double alpha;    // step of the current iteration
+    double beta;     // direction update of the current iteration
+    double rnorm;    // max |r| after the last update
+    double rnorm0;   // max |rhs| at the start
+    double tol_abs;  // stop when rnorm <= tol_abs
+    double sigma;    // s.As of the current iteration (diagnostic)
+    double pad0;
+    int64_t iter;    // completed iterations
+    int64_t max_iter;
+    int32_t done;    // 1 once converged, capped or broken down
+    int32_t status;  // shk_status
+    uint32_t ticket[6];
+};
+
+enum Ticket { kTicketInit = 0, kTicketAlpha = 1, kTicketAxpy = 2, kTicketMisc = 3 };
+
+// Workgroup -> slot mapping that hands each XCD one contiguous range of slots.
+// Workgroups are dispatched to the 8 XCDs round-robin by id, and each XCD has its
+// own L2, so contiguous slots (spatial neighbours share halo cells) should land
+// on the same XCD.  Bijective for any n.
+__device__ inline int xcd_slot(int b, int n) {
+    const int per = n / kXcds, rem = n % kXcds;
+    const int xcd = b % kXcds, idx = b / kXcds;
+    return xcd < rem ? xcd * (per + 1) + idx : rem * (per + 1) + (xcd - rem) * per + idx;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+    return v;
+}
+
+// Sum / max over a workgroup of NT threads; the result is valid in thread 0.
+// Fixed butterfly + fixed wave order: deterministic.
+template <int NT>
+__device__ inline double block_sum(double v, double *lds) {
+    v = wave_sum(v);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (wave == 0) {
+        r = lane < NT / kWave ? lds[lane] : 0.0;
+        r = wave_sum(r);
+    }
+    __syncthreads();
+    return r;
+}
+template <int NT>
+__device__ inline double block_max(double v, double *lds) {
+    v = wave_max(v);
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (wave == 0) {
+        r = lane < NT / kWave ? lds[lane] : 0.0;
+        r = wave_max(r);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Publishes this block's partial(s) (already stored by thread 0) and returns true
+// in every thread of the block that arrives last.  The release fence before the
+// ticket and the acquire fence after it make every block's partial visible to the
+// last block across XCD L2s.  The last block resets the ticket.
+__device__ inline bool last_block_arrive(uint32_t *ticket) {
+    __shared__ int s_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = atomicAdd(ticket, 1u);
+        s_last = (t == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    const bool last = s_last != 0;
+    if (last) {
+        __threadfence();
+        if (threadIdx.x == 0) atomicExch(ticket, 0u);
+    }
+    return last;
+}
+
+// Fixed-order reduction of n partials by one workgroup of NT threads (thread t
+// folds t, t+NT, ... in order, then the block tree).  Result valid in thread 0.
+template <int NT>
+__device__ inline double reduce_partials_sum(const double *p, int n, double *lds) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) acc += p[i];
+    return block_sum<NT>(acc, lds);
+}
+template <int NT>
+__device__ inline double reduce_partials_max(const double *p, int n, double *lds) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) acc = fmax(acc, p[i]);
+    return block_max<NT>(acc, lds);
+}
+
+}  // namespace shk

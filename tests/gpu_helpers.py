@@ -1,0 +1,79 @@
+"""Helpers for the GPU parity tests: move scenes to the device and run the
+projection through the C-ABI (shiokaze_amd.MacPressureSolver3)."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+
+FIELDS = ("fluid", "solid", "u", "v", "w", "us", "vs", "ws")
+
+
+def log(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def to_device(scene: dict):
+    import torch
+    return {k: torch.from_numpy(np.ascontiguousarray(scene[k])).cuda() for k in FIELDS}
+
+
+def run_projection(scene: dict, config: dict, dt: float = 1.0, want_system: bool = True):
+    """Runs shk_project on a copy of `scene`; returns a dict of host arrays."""
+    import torch
+
+    from shiokaze_amd import MacPressureSolver3
+
+    nx, ny, nz = scene["shape"]
+    d = to_device(scene)
+    vel0 = [d[k].clone() for k in ("u", "v", "w")]
+    out = {}
+    with MacPressureSolver3((nx, ny, nz), scene["dx"], config=config) as ps:
+        res = ps.project(dt, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+        out["result"] = res
+        out["pressure"] = ps.get_pressure().cpu().numpy()
+        out["velocity"] = [d[k].cpu().numpy() for k in ("u", "v", "w")]
+        fr = ps.export_fractions()
+        out["areas"] = [fr[f"area{a}"].cpu().numpy() for a in range(3)]
+        out["rhos"] = [fr[f"rho{a}"].cpu().numpy() for a in range(3)]
+        if want_system and res["unknowns"]:
+            sysd = ps.export_system()
+            out["system"] = {k: v.cpu().numpy() for k, v in sysd.items()}
+            svel = (d["us"], d["vs"], d["ws"])
+            out["div_before"] = ps.divergence(dt, vel0, svel, fp64_update=False)
+            out["div_after_fp64"] = ps.divergence(dt, vel0, svel, fp64_update=True)
+            out["div_after_f32"] = ps.divergence(dt, (d["u"], d["v"], d["w"]), svel, fp64_update=False)
+        torch.cuda.synchronize()
+    return out
+
+
+def oracle_dense_system(O, ora: dict):
+    """Oracle CSR -> dense (k, j, i) diag / low-face couplings / rhs / x."""
+    im = ora["index_map"]
+    shape = im.shape
+    nz, ny, nx = shape
+    rowptr, cols, vals = ora["rowptr"], ora["cols"], ora["vals"]
+    n = ora["n"]
+    cell_of_row = np.empty(n, dtype=np.int64)
+    flat = im.ravel()
+    rows = np.nonzero(flat >= 0)[0]
+    cell_of_row[flat[rows]] = rows
+    diag = np.zeros(nx * ny * nz)
+    cx = np.zeros(nx * ny * nz)
+    cy = np.zeros(nx * ny * nz)
+    cz = np.zeros(nx * ny * nz)
+    rhs = np.zeros(nx * ny * nz)
+    x = np.zeros(nx * ny * nz)
+    rr = np.repeat(np.arange(n), np.diff(rowptr))
+    cell_r = cell_of_row[rr]
+    cell_c = cell_of_row[cols]
+    delta = cell_c - cell_r
+    diag[cell_r[delta == 0]] = vals[delta == 0]
+    cx[cell_r[delta == -1]] = -vals[delta == -1]
+    cy[cell_r[delta == -nx]] = -vals[delta == -nx]
+    cz[cell_r[delta == -nx * ny]] = -vals[delta == -nx * ny]
+    rhs[cell_of_row] = ora["rhs"]
+    if "x" in ora:
+        x[cell_of_row] = ora["x"]
+    return {k: v.reshape(shape) for k, v in dict(diag=diag, cx=cx, cy=cy, cz=cz, rhs=rhs, x=x).items()}

@@ -1,0 +1,128 @@
+"""GPU parity of the projection (shk_project through the C-ABI) against the CPU oracle.
+
+Bit-exact: face fractions, unknown labels, stencil coefficients, rhs, and the
+velocity update given the same pressure.  Within tolerance: the Krylov solution
+(iteration counts within 2 of the oracle's CG, solution relative L2 and the FP64
+post-projection divergence)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gpu_helpers import log, oracle_dense_system, run_projection
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(1, 32, 0), (3, 32, 7), (3, (24, 16, 40), 3), (1, (13, 9, 21), 0)]
+
+
+def _ids(c):
+    return f"kind{c[0]}-n{c[1]}-seed{c[2]}".replace(" ", "")
+
+
+@pytest.fixture(scope="module")
+def projected(scenes, oracle):
+    cache = {}
+
+    def get(kind, n, seed, precond="none", precision="fp64", tol=1e-6):
+        key = (kind, str(n), seed, precond, precision, tol)
+        if key not in cache:
+            sc = scenes(kind, n, seed)
+            log(f"[gpu] project kind={kind} n={n} seed={seed} precond={precond} precision={precision} tol={tol}")
+            g = run_projection(sc, {"Residual": tol, "Preconditioner": precond, "Precision": precision})
+            mode = {"none": oracle.SHK_OR_CG, "jacobi": oracle.SHK_OR_JACOBI}[precond]
+            o = oracle.project(sc, tol=tol, mode=mode)
+            cache[key] = (sc, g, o)
+        return cache[key]
+
+    return get
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_fractions_bitexact(projected, case):
+    sc, g, o = projected(*case)
+    for a in range(3):
+        np.testing.assert_array_equal(g["areas"][a].astype(np.float64), o["areas"][a])
+        np.testing.assert_array_equal(g["rhos"][a].astype(np.float64), o["rhos"][a])
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_system_bitexact(projected, oracle, case):
+    sc, g, o = projected(*case)
+    assert g["result"]["unknowns"] == o["n"]
+    s = g["system"]
+    np.testing.assert_array_equal(s["label"].astype(bool), o["index_map"] >= 0)
+    ref = oracle_dense_system(oracle, o)
+    lab = o["index_map"] >= 0
+    np.testing.assert_array_equal(s["diag"][lab], ref["diag"][lab])
+    np.testing.assert_array_equal(s["rhs"][lab], ref["rhs"][lab])
+    # off-diagonals exist where both cells are unknowns
+    nz, ny, nx = lab.shape
+    both_x = lab[:, :, 1:] & lab[:, :, :-1]
+    both_y = lab[:, 1:, :] & lab[:, :-1, :]
+    both_z = lab[1:, :, :] & lab[:-1, :, :]
+    np.testing.assert_array_equal(s["cx"][:, :, 1:][both_x], ref["cx"][:, :, 1:][both_x])
+    np.testing.assert_array_equal(s["cy"][:, 1:, :][both_y], ref["cy"][:, 1:, :][both_y])
+    np.testing.assert_array_equal(s["cz"][1:, :, :][both_z], ref["cz"][1:, :, :][both_z])
+
+
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+def test_solve_parity(projected, case, precond):
+    sc, g, o = projected(*case, precond=precond)
+    r = g["result"]
+    log(f"[gpu] iters gpu={r['count']} oracle={o['iters']} relres={r['reresid']:.3e} true={r['true_relres_inf']:.3e}")
+    assert r["status"] == 0
+    assert abs(r["count"] - o["iters"]) <= 2
+    assert r["true_relres_inf"] <= 1e-6 * 1.0001
+    x = g["system"]["x"][o["index_map"] >= 0]
+    xo = np.zeros_like(x)
+    xo[:] = o["x"][o["index_map"][o["index_map"] >= 0]]
+    rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    log(f"[gpu] x relL2 vs oracle (same tolerance) = {rel:.3e}")
+    assert rel < 1e-4
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_velocity_update_bitexact(projected, oracle, case):
+    """The update is checked bit for bit by feeding the GPU's pressure to the oracle."""
+    sc, g, o = projected(*case)
+    p = g["pressure"].astype(np.float64)
+    vel = oracle.update_velocity(1.0, sc["dx"], p, o["areas"], o["rhos"], [sc["us"], sc["vs"], sc["ws"]],
+                                 [sc["u"], sc["v"], sc["w"]], real_is_double=False)
+    for a in range(3):
+        np.testing.assert_array_equal(g["velocity"][a].astype(np.float64), vel[a])
+    # and the pressure is the float rounding of the FP64 solution on unknowns, 0 elsewhere
+    lab = o["index_map"] >= 0
+    np.testing.assert_array_equal(g["pressure"][lab], g["system"]["x"][lab].astype(np.float32))
+    assert np.all(g["pressure"][~lab] == 0)
+
+
+@pytest.mark.parametrize("case", CASES, ids=_ids)
+def test_post_divergence(projected, case):
+    """FP64 post-projection divergence relative to the initial one equals the true
+    relative residual (D(u_new) = -(b - A p)) and meets the 1e-6 bar."""
+    sc, g, o = projected(*case)
+    r = g["result"]
+    rel = g["div_after_fp64"] / g["div_before"]
+    log(f"[gpu] div before={g['div_before']:.3e} after(fp64)={g['div_after_fp64']:.3e} rel={rel:.3e} "
+        f"after(f32 storage)={g['div_after_f32']:.3e}")
+    assert g["div_before"] == pytest.approx(r["b_inf"], rel=1e-12)
+    assert rel <= 1e-6 * 1.0001
+    assert rel == pytest.approx(r["true_relres_inf"], rel=1e-6, abs=1e-12)
+
+
+@pytest.mark.parametrize("case", CASES[:2], ids=_ids)
+def test_mixed_precision(projected, oracle, scenes, case):
+    sc, g, o = projected(*case, precond="jacobi", precision="mixed")
+    r = g["result"]
+    truth = oracle.project(sc, tol=1e-12, mode=oracle.SHK_OR_JACOBI)
+    lab = truth["index_map"] >= 0
+    x = g["system"]["x"][lab]
+    xt = truth["x"][truth["index_map"][lab]]
+    rel = np.linalg.norm(x - xt) / np.linalg.norm(xt)
+    log(f"[gpu] mixed: iters={r['count']} outer={r['outer']} true relres={r['true_relres_inf']:.3e} "
+        f"p relL2 vs 1e-12 oracle={rel:.3e}")
+    assert r["status"] == 0
+    assert r["true_relres_inf"] <= 1e-6
+    assert rel < 1e-4
