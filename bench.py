@@ -1,0 +1,247 @@
+"""Benchmark: pressure solves/sec of the MAC liquid projection on one MI355X.
+
+A step is one whole projection (shk_project: fractions, labels, coefficients,
+rhs, Krylov solve to the relative max-norm residual, pressure scatter, velocity
+update) of the same synthetic scene, with the input velocity restored first; all
+fields stay resident in HBM.  Default workload: the pool+sphere scene at 256^3,
+FP32 stencil/AXPY/Jacobi inner CG with FP64 dots and FP64 iterative refinement,
+to 1e-6.
+
+Prints one JSON line (rank 0).  `roofline` is for the dominant Krylov kernel:
+algorithmic bytes per launch / its average launch duration, timed with HIP events
+on the library's stream during the timed steps (ProfileKernels=1).
+`cpu_baseline` times the oracle (the FP64 CPU restatement of the reference's
+CG, with its discarded MIC(0) work) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "pressure solves/sec at 256^3 to 1e-6 rel. residual; PCG iters; % HBM peak"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
+    """Algorithmic HBM bytes per brick cell of one launch of each Krylov kernel:
+    every array element the kernel needs is read or written once."""
+    t = 8 if precision == "fp64" else 4      # vector element
+    c = 8 if precision == "fp64" else 4      # coefficient element
+    jac = precond == "jacobi"
+    if kernel == "spmv_dot":                  # s, diag, cx, cy, cz in; q out
+        return 2 * t + 4 * c
+    if kernel == "axpy_norm":                 # x, r, s, q, diag in; x, r out
+        return 6 * t + c
+    if kernel == "update_dir":                # r, s, (diag) in; s out
+        return 3 * t + (c if jac else 0)
+    raise ValueError(kernel)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--scene", type=int, default=1, help="1 pool+sphere, 3 irregular")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--preconditioner", default="jacobi", choices=["none", "jacobi", "unet"])
+    ap.add_argument("--residual", type=float, default=1e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--extra", action="store_true", help="also time FP64 Jacobi and plain CG solves")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the projection runs on a single MI355X (no multi-GPU path is built); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import shiokaze_amd
+    from shiokaze_amd import scene as S
+
+    n = args.n
+    t0 = time.time()
+    sc = S.generate(args.scene, n, args.seed)
+    log(f"scene {args.scene} n={n} generated in {time.time() - t0:.1f}s (dropped {sc['dropped']})")
+    dev = {k: torch.from_numpy(np.ascontiguousarray(sc[k])).cuda()
+           for k in ("fluid", "solid", "u", "v", "w", "us", "vs", "ws")}
+    vel0 = [dev[k].clone() for k in ("u", "v", "w")]
+    vel = (dev["u"], dev["v"], dev["w"])
+    svel = (dev["us"], dev["vs"], dev["ws"])
+
+    def make(config):
+        return shiokaze_amd.MacPressureSolver3((n, n, n), sc["dx"], config=config)
+
+    def step(ps):
+        for a in range(3):
+            vel[a].copy_(vel0[a])
+        return ps.project(1.0, vel, dev["solid"], svel, dev["fluid"])
+
+    cfg = {"Residual": args.residual, "Precision": args.precision, "Preconditioner": args.preconditioner}
+    ps = make(cfg)
+    for _ in range(args.warmup):
+        r = step(ps)
+        log(f"warmup: iters={r['count']} outer={r['outer']} status={r['status_name']} "
+            f"setup={r['ms_setup']:.2f}ms solve={r['ms_solve']:.2f}ms update={r['ms_update']:.2f}ms")
+    if not args.no_profile:
+        ps.configure("ProfileKernels", 1)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    results = []
+    for _ in range(args.steps):
+        results.append(step(ps))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1e3 * elapsed / args.steps
+    last = results[-1]
+    value = args.steps / elapsed
+    log(f"timed: {args.steps} steps in {elapsed:.3f}s -> {value:.3f} solves/s, iters={last['count']} "
+        f"outer={last['outer']} true relres={last['true_relres_inf']:.3e}")
+
+    n_cells = last["active_bricks"] * 512
+    kernels = {}
+    roofline = None
+    if not args.no_profile:
+        for k in ("spmv_dot", "axpy_norm", "update_dir"):
+            cnt, ms = ps.kernel_stats(k)
+            if cnt == 0:
+                continue
+            avg_us = 1e3 * ms / cnt
+            if k.startswith("finish"):
+                kernels[k] = {"launches": cnt, "avg_us": round(avg_us, 3), "total_ms": round(ms, 3)}
+                continue
+            bpc = kernel_bytes_per_cell(k, args.precision, args.preconditioner)
+            traffic = bpc * n_cells
+            gbs = traffic / (avg_us * 1e-6) / 1e9
+            kernels[k] = {"launches": cnt, "avg_us": round(avg_us, 3), "bytes_per_cell": bpc,
+                          "bytes_per_launch": traffic, "GBs": round(gbs, 1),
+                          "frac": round(gbs / HBM_PEAK_GBS, 4), "total_ms": round(ms, 3)}
+        if kernels:
+            dom = max((k for k in kernels if "GBs" in kernels[k]), key=lambda k: kernels[k]["total_ms"])
+            kd = kernels[dom]
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": kd["frac"], "traffic": None,
+                        "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
+            solve_bytes = sum(kernels[k].get("bytes_per_launch", 0) * kernels[k]["launches"] for k in kernels)
+            solve_ms = sum(kernels[k]["total_ms"] for k in kernels)
+            roofline["krylov_loop_GBs"] = round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1)
+
+    # ---- accuracy against an FP64 truth solve (Jacobi PCG to 1e-12 on the GPU; the
+    # FP64 GPU path is checked against the CPU oracle by the parity tests)
+    accuracy = {}
+    if not args.no_accuracy:
+        sysb = ps.export_system()
+        lab = sysb["label"].bool()
+        xb = sysb["x"][lab]
+        div_before = ps.divergence(1.0, vel0, svel, fp64_update=False)
+        div_after = ps.divergence(1.0, vel0, svel, fp64_update=True)
+        truth = make({"Residual": 1e-12, "Precision": "fp64", "Preconditioner": "jacobi", "CheckInterval": 128})
+        rt = step(truth)
+        xt = truth.export_system()["x"][lab]
+        accuracy = {
+            "pressure_rel_l2_vs_fp64_1e-12": float(torch.linalg.norm(xb - xt) / torch.linalg.norm(xt)),
+            "post_div_rel_fp64": div_after / div_before,
+            "true_relres_inf": last["true_relres_inf"],
+            "true_relres_l2": last["true_relres_l2"],
+            "truth_iters": rt["count"],
+        }
+        truth.close()
+        log(f"accuracy: {accuracy}")
+
+    extra = {}
+    plain_cg_iters = None
+    if args.extra or not args.no_cpu_baseline:
+        # plain FP64 CG (the reference's effective iteration) for the CPU extrapolation
+        pcg = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "none", "CheckInterval": 64})
+        rp = step(pcg)
+        plain_cg_iters = rp["count"]
+        if args.extra:
+            t1 = time.perf_counter()
+            rp = step(pcg)
+            extra["fp64_plain_cg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rp["count"]}
+        pcg.close()
+        if args.extra:
+            pj = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "jacobi"})
+            step(pj)
+            t1 = time.perf_counter()
+            rj = step(pj)
+            extra["fp64_jacobi_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rj["count"]}
+            pj.close()
+        log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.set_threads(1)
+        t_runs = {}
+        for k in (10, 30):
+            rr = O.project_timed(sc, tol=args.residual, max_iterations=k, mode=O.SHK_OR_REFERENCE)
+            t_runs[k] = rr["t_ms"]
+            log(f"cpu oracle {k} its: setup={rr['t_ms'][0]:.0f}ms solve={rr['t_ms'][1]:.0f}ms "
+                f"update={rr['t_ms'][2]:.0f}ms")
+        per_it = (t_runs[30][1] - t_runs[10][1]) / 20.0
+        factor = max(t_runs[10][1] - 10 * per_it, 0.0)
+        setup = min(t_runs[10][0], t_runs[30][0])
+        update = min(t_runs[10][2], t_runs[30][2])
+        total_ms = setup + factor + plain_cg_iters * per_it + update
+        cpu = {"value": 1e3 / total_ms, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": (f"{n}^3 scene {args.scene}: oracle reference-faithful CG (MIC(0) factor and two discarded "
+                          f"triangular solves per iteration) timed for 10 and 30 iterations, extrapolated to "
+                          f"{plain_cg_iters} iterations (the plain CG count on this scene); "
+                          f"{per_it:.1f} ms/iteration, setup {setup:.0f} ms, single thread"),
+               "ms_per_solve": round(total_ms, 1)}
+        log(f"cpu baseline: {cpu}")
+
+    prec_name = {"mixed": "fp32 stencil/axpy + fp64 dots/refinement", "fp64": "fp64"}[args.precision]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if args.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{'pool+sphere' if args.scene == 1 else 'irregular'} scene {n}^3, {prec_name}, "
+                         f"{args.preconditioner} preconditioner, to {args.residual:g} relative max-norm residual"),
+            "n": n, "scene": args.scene, "seed": args.seed, "unknowns": last["unknowns"],
+            "active_bricks": last["active_bricks"], "pcg_iters": last["count"], "outer": last["outer"],
+            "precision": args.precision, "preconditioner": args.preconditioner,
+            "ms_setup": round(last["ms_setup"], 3), "ms_solve": round(last["ms_solve"], 3),
+            "ms_update": round(last["ms_update"], 3),
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+        "accuracy": accuracy,
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    ps.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -109,9 +109,14 @@ def load() -> ctypes.CDLL:
     return lib
 
 
-def check(code: int, ctx=None) -> int:
-    """Raises ShkError for a failing status (NOT_CONVERGED is returned, not raised)."""
-    if code in (SHK_OK, SHK_NOT_CONVERGED):
+SOLVER_OUTCOMES = (SHK_OK, SHK_NOT_CONVERGED, SHK_NAN, SHK_BREAKDOWN)
+
+
+def check(code: int, ctx=None, outcomes=(SHK_OK,)) -> int:
+    """Raises ShkError unless `code` is one of `outcomes`.  Solver calls pass
+    SOLVER_OUTCOMES: like the reference (pcg_solver.h:263-294) a capped, NaN or
+    broken-down solve returns its status instead of failing."""
+    if code in outcomes:
         return code
     lib = load()
     msg = lib.shk_last_error(ctx).decode() if lib.shk_last_error(ctx) else ""

@@ -66,9 +66,10 @@ int sync_and_time(shk_ctx *c, int a, int b, double &ms) {
     return SHK_OK;
 }
 
-int ensure_partials(shk_ctx *c, int blocks) {
-    HIPC(c, c->pa.ensure((size_t)blocks + 64));
-    HIPC(c, c->pb.ensure(3 * (size_t)blocks + 64));
+int ensure_partials(shk_ctx *c) {
+    HIPC(c, c->pa.ensure((size_t)kMaxBlocks + 64));
+    HIPC(c, c->pb.ensure(2 * (size_t)kMaxBlocks + 64));
+    HIPC(c, c->pc.ensure(4 * (size_t)kMaxBlocks + 64));
     return SHK_OK;
 }
 
@@ -110,15 +111,22 @@ int build_bricks(shk_ctx *c) {
                            c->stream));
     HIPC(c, hipMemcpyAsync(c->nbr.p, c->h_nbr.data(), sizeof(int) * 6 * c->n_active, hipMemcpyHostToDevice,
                            c->stream));
-    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->x64, &c->r64, &c->s64, &c->q64})
+    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->x64, &c->r64,
+                              &c->s64, &c->q64})
         HIPC(c, b->ensure(n));
+    if (c->prm.precond != 0 && c->prm.precision == 0)
+        for (DevBuf<double> *b : {&c->hx64, &c->hy64, &c->hz64}) HIPC(c, b->ensure(n));
     if (c->prm.precision == 1 || c->prm.precond == 2) {
-        for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32, &c->s32, &c->q32})
-            HIPC(c, b->ensure(n));
+        for (DevBuf<float> *b : {&c->x32, &c->r32, &c->s32, &c->q32, &c->rs32}) HIPC(c, b->ensure(n));
+        if (c->prm.precond == 0)
+            for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32}) HIPC(c, b->ensure(n));
+        else
+            for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32}) HIPC(c, b->ensure(n));
     }
+    c->grid = make_grid((int64_t)c->n_active * kBrickCells);
     // host-side buffers must outlive the async copies above
     HIPC(c, hipStreamSynchronize(c->stream));
-    return ensure_partials(c, c->n_active);
+    return ensure_partials(c);
 }
 
 }  // namespace
@@ -131,9 +139,10 @@ int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
     HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     const int interval = std::max(1, c->prm.check_interval);
-    if (c->profile && c->pev.size() < (size_t)(3 * interval + 1)) {
+    constexpr int K = shk_ctx::kKernels;
+    if (c->profile && c->pev.size() < (size_t)(K * interval + 1)) {
         const size_t old = c->pev.size();
-        c->pev.resize(3 * interval + 1);
+        c->pev.resize(K * interval + 1);
         for (size_t i = old; i < c->pev.size(); ++i) HIPC(c, hipEventCreate(&c->pev[i]));
     }
     while (!c->h_state->done) {
@@ -146,15 +155,15 @@ int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
         HIPC(c, hipGetLastError());
         HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
-        if (c->profile && c->pcur == (size_t)(3 * interval + 1)) {
+        if (c->profile && c->pcur == (size_t)(K * interval + 1)) {
             // only launches that did work count: the first `delta` iterations; the
             // direction update of the final iteration returns at once
             const int64_t delta = c->h_state->iter - it0;
             for (int64_t t = 0; t < delta && t < interval; ++t) {
-                for (int kk = 0; kk < 3; ++kk) {
-                    if (kk == 2 && c->h_state->done && t == delta - 1) continue;
+                for (int kk = 0; kk < K; ++kk) {
+                    if (kk == K - 1 && c->h_state->done && t == delta - 1) continue;
                     float ms = 0.0f;
-                    HIPC(c, hipEventElapsedTime(&ms, c->pev[3 * t + kk], c->pev[3 * t + kk + 1]));
+                    HIPC(c, hipEventElapsedTime(&ms, c->pev[K * t + kk], c->pev[K * t + kk + 1]));
                     c->pcount[kk] += 1;
                     c->pms[kk] += ms;
                 }
@@ -168,11 +177,12 @@ BrickOp<double> op64(shk_ctx *c) {
     return BrickOp<double>{c->nbr.p, c->diag64.p, c->cx64.p, c->cy64.p, c->cz64.p, c->n_active};
 }
 BrickOp<float> op32(shk_ctx *c) {
-    return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active};
+    if (c->prm.precond == 0) return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active};
+    return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active};
 }
 
-int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4]) {
-    launch_residual(c->stream, op64(c), c->b64.p, x, r_out, reduction(c), c->misc.p);
+int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4], const double *sig, float *rs) {
+    launch_residual(c->stream, c->grid, op64(c), c->b64.p, x, r_out, c->pc.p, c->misc.p, sig, rs);
     HIPC(c, hipGetLastError());
     HIPC(c, hipMemcpyAsync(c->h_misc, c->misc.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -186,23 +196,27 @@ void fill_true_residual(shk_result *res, const double rr[4]) {
     res->true_relres_l2 = rr[2] > 0 ? std::sqrt(rr[1] / rr[2]) : 0.0;
 }
 
-// FP64 Krylov on the brick stencil: identity or Jacobi preconditioner.
+// FP64 Krylov on the brick stencil: plain CG, or Jacobi PCG as CG on the
+// symmetrically scaled system (x = sig * x_scaled).
 int solve_fp64(shk_ctx *c, shk_result *res) {
-    const int64_t n = (int64_t)c->n_active * kBrickCells;
     const bool jac = c->prm.precond != 0;
-    launch_cg_init<double, double, double>(c->stream, n, c->b64.p, c->diag64.p, c->x64.p, c->r64.p, c->s64.p,
-                                           reduction(c), c->prm.residual, c->prm.max_iterations, jac);
-    const BrickOp<double> op = op64(c);
+    const Grid &g = c->grid;
+    const double *sig = jac ? c->sig64.p : nullptr;
+    const BrickOp<double> op =
+        jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active} : op64(c);
+    launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, reduction(c),
+                                   c->prm.residual, c->prm.max_iterations);
     const Reduction red = reduction(c);
     int rc = drive(c, [&]() {
-        launch_spmv_dot<double, double>(c->stream, op, c->s64.p, c->q64.p, red);
+        launch_spmv_dot<double, double>(c->stream, g, op, c->s64.p, c->q64.p, red);
         c->mark();
-        launch_axpy_norm<double, double>(c->stream, n, op.diag, c->x64.p, c->r64.p, c->s64.p, c->q64.p, red, jac);
+        launch_axpy_norm<double>(c->stream, g, sig, c->x64.p, c->r64.p, c->s64.p, c->q64.p, red);
         c->mark();
-        launch_update_dir<double, double>(c->stream, n, op.diag, c->r64.p, c->s64.p, red, jac);
+        launch_update_dir<double>(c->stream, g, c->r64.p, c->s64.p, red);
         c->mark();
     });
     if (rc) return rc;
+    if (jac) launch_unscale(c->stream, g.n, c->x64.p, c->sig64.p);
     res->count = c->h_state->iter;
     res->status = c->h_state->status;
     res->reresid = c->h_state->rnorm0 > 0 ? c->h_state->rnorm / c->h_state->rnorm0 : c->h_state->rnorm;
@@ -220,13 +234,14 @@ namespace {
 // Mixed precision: FP32 stencil/AXPY/Jacobi inner CG with FP64 dots, FP64 true
 // residual and correction between inner solves (iterative refinement).
 int solve_mixed(shk_ctx *c, shk_result *res) {
-    const int64_t n = (int64_t)c->n_active * kBrickCells;
+    const Grid &g = c->grid;
     const bool jac = c->prm.precond != 0;
-    HIPC(c, hipMemsetAsync(c->x64.p, 0, sizeof(double) * n, c->stream));
+    const double *sig = jac ? c->sig64.p : nullptr;
+    HIPC(c, hipMemsetAsync(c->x64.p, 0, sizeof(double) * g.n, c->stream));
     double rr[4];
     int rc;
-    // r64 = b - A*0 = b, and the norms of b
-    if ((rc = true_residual(c, c->x64.p, c->r64.p, rr))) return rc;
+    // r64 = b - A*0 = b and its scaled FP32 copy, the first inner right-hand side
+    if ((rc = true_residual(c, c->x64.p, c->r64.p, rr, sig, c->rs32.p))) return rc;
     const double b_inf = rr[3];
     const double tol_abs = std::max(c->prm.residual, 1e-30) * b_inf;
     double r_inf = rr[0];
@@ -234,26 +249,26 @@ int solve_mixed(shk_ctx *c, shk_result *res) {
     int outer = 0;
     int status = SHK_OK;
     const BrickOp<float> op = op32(c);
+    const Reduction red = reduction(c);
     while (r_inf > tol_abs && outer < std::max(1, c->prm.outer_max)) {
         double inner_rel = std::max(c->prm.inner_residual, 0.5 * tol_abs / r_inf);
         if (inner_rel > 0.5) inner_rel = 0.5;
-        launch_cg_init<float, float, double>(c->stream, n, c->r64.p, c->diag32.p, c->x32.p, c->r32.p, c->s32.p,
-                                             reduction(c), inner_rel, c->prm.max_iterations - total, jac);
-        const Reduction red = reduction(c);
+        launch_cg_init<float, float>(c->stream, g, c->rs32.p, nullptr, c->x32.p, c->r32.p, c->s32.p, red, inner_rel,
+                                     c->prm.max_iterations - total);
         rc = drive(c, [&]() {
-            launch_spmv_dot<float, float>(c->stream, op, c->s32.p, c->q32.p, red);
+            launch_spmv_dot<float, float>(c->stream, g, op, c->s32.p, c->q32.p, red);
             c->mark();
-            launch_axpy_norm<float, float>(c->stream, n, op.diag, c->x32.p, c->r32.p, c->s32.p, c->q32.p, red, jac);
+            launch_axpy_norm<float>(c->stream, g, nullptr, c->x32.p, c->r32.p, c->s32.p, c->q32.p, red);
             c->mark();
-            launch_update_dir<float, float>(c->stream, n, op.diag, c->r32.p, c->s32.p, red, jac);
+            launch_update_dir<float>(c->stream, g, c->r32.p, c->s32.p, red);
             c->mark();
         });
         if (rc) return rc;
         total += c->h_state->iter;
         ++outer;
         const int st = c->h_state->status;
-        launch_add_correction(c->stream, n, c->x64.p, c->x32.p);
-        if ((rc = true_residual(c, c->x64.p, c->r64.p, rr))) return rc;
+        launch_add_correction(c->stream, g.n, c->x64.p, c->x32.p, sig);
+        if ((rc = true_residual(c, c->x64.p, c->r64.p, rr, sig, c->rs32.p))) return rc;
         r_inf = rr[0];
         if (st == SHK_NAN || st == SHK_BREAKDOWN) {
             status = st;
@@ -370,10 +385,11 @@ void shk_destroy(shk_ctx *c) {
     c->count.release();
     c->coord.release();
     c->nbr.release();
-    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->x64, &c->r64, &c->s64, &c->q64,
-                              &c->pa, &c->pb, &c->misc})
+    for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->hx64, &c->hy64,
+                              &c->hz64, &c->x64, &c->r64, &c->s64, &c->q64, &c->pa, &c->pb, &c->pc, &c->misc})
         b->release();
-    for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32, &c->s32, &c->q32})
+    for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
+                             &c->s32, &c->q32, &c->rs32})
         b->release();
     c->state.release();
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -461,7 +477,7 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
     } else if (k == "ProfileKernels") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad ProfileKernels '%s'", value);
         c->profile = b;
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < shk_ctx::kKernels; ++q) {
             c->pcount[q] = 0;
             c->pms[q] = 0.0;
         }
@@ -535,9 +551,17 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         return SHK_OK;
     }
     const bool need32 = c->prm.precision == 1 || c->prm.precond == 2;
+    const bool plain32 = need32 && c->prm.precond == 0;
     launch_coeff(c->stream, d, dt, c->dx, c->n_active, c->coord.p, area, rho, c->label.p, vel, svel, c->diag64.p,
-                 c->cx64.p, c->cy64.p, c->cz64.p, c->b64.p, need32 ? c->diag32.p : nullptr,
-                 need32 ? c->cx32.p : nullptr, need32 ? c->cy32.p : nullptr, need32 ? c->cz32.p : nullptr);
+                 c->cx64.p, c->cy64.p, c->cz64.p, c->b64.p, c->sig64.p, plain32 ? c->diag32.p : nullptr,
+                 plain32 ? c->cx32.p : nullptr, plain32 ? c->cy32.p : nullptr, plain32 ? c->cz32.p : nullptr);
+    if (c->prm.precond != 0) {
+        const bool s64 = c->prm.precision == 0 && c->prm.precond == 1;
+        const bool s32 = need32;
+        launch_scale_coeff(c->stream, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p,
+                           s64 ? c->hx64.p : nullptr, s64 ? c->hy64.p : nullptr, s64 ? c->hz64.p : nullptr,
+                           s32 ? c->hx32.p : nullptr, s32 ? c->hy32.p : nullptr, s32 ? c->hz32.p : nullptr);
+    }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ev[1], c->stream));
 
@@ -589,8 +613,12 @@ int shk_copy_pressure(shk_ctx *c, float *dst) {
 int shk_kernel_stats(shk_ctx *c, const char *kernel, int64_t *launches, double *total_ms) {
     if (!c || !kernel || !launches || !total_ms) return SHK_BAD_ARG;
     const std::string k(kernel);
-    const int idx = k == "spmv_dot" ? 0 : (k == "axpy_norm" ? 1 : (k == "update_dir" ? 2 : -1));
-    if (idx < 0) return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (spmv_dot, axpy_norm, update_dir)", kernel);
+    static const char *names[shk_ctx::kKernels] = {"spmv_dot", "axpy_norm", "update_dir"};
+    int idx = -1;
+    for (int q = 0; q < shk_ctx::kKernels; ++q)
+        if (k == names[q]) idx = q;
+    if (idx < 0)
+        return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (spmv_dot, axpy_norm, update_dir)", kernel);
     *launches = c->pcount[idx];
     *total_ms = c->pms[idx];
     return SHK_OK;
@@ -665,34 +693,49 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     res->unknowns = rows;
     res->outer = 1;
     if (rows == 0) return SHK_OK;
-    const int blocks = blocks_for(rows);
-    int rc = ensure_partials(c, blocks);
+    int rc = ensure_partials(c);
     if (rc) return rc;
     HIPC(c, c->diag64.ensure(rows));
+    HIPC(c, c->sig64.ensure(rows));
     HIPC(c, c->r64.ensure(rows));
     HIPC(c, c->s64.ensure(rows));
     HIPC(c, c->q64.ensure(rows));
-    CsrOp op{rowptr, cols, vals, c->diag64.p, rows};
+    const CsrOp op{rowptr, cols, vals, rows};
     const bool jac = c->prm.precond != 0;
+    const Grid g = make_grid(rows);
     HIPC(c, hipEventRecord(c->ev[0], c->stream));
-    launch_csr_diag(c->stream, op, c->diag64.p);
-    launch_cg_init<double, double, double>(c->stream, rows, b, c->diag64.p, x, c->r64.p, c->s64.p, reduction(c),
-                                           c->prm.residual, c->prm.max_iterations, jac);
+    launch_csr_diag(c->stream, op, c->diag64.p, c->sig64.p);
+    CsrOp sop = op;
+    if (jac) {
+        HIPC(c, c->cx64.ensure((size_t)0));
+        // scaled copy of the values: the Jacobi-PCG iterates as CG on D^-1/2 A D^-1/2
+        int64_t nnz = 0;
+        HIPC(c, hipMemcpy(&nnz, rowptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (nnz < 0) return c->fail(SHK_BAD_ARG, "rowptr[rows] is negative");
+        HIPC(c, c->q64.ensure(std::max<int64_t>(rows, 1)));
+        HIPC(c, c->hx64.ensure(std::max<int64_t>(nnz, 1)));
+        launch_csr_scale(c->stream, op, c->sig64.p, c->hx64.p);
+        sop.vals = c->hx64.p;
+    }
+    const double *sig = jac ? c->sig64.p : nullptr;
     const Reduction red = reduction(c);
+    launch_cg_init<double, double>(c->stream, g, b, sig, x, c->r64.p, c->s64.p, red, c->prm.residual,
+                                   c->prm.max_iterations);
     rc = drive(c, [&]() {
-        launch_spmv_dot_csr(c->stream, op, c->s64.p, c->q64.p, red);
+        launch_spmv_dot_csr(c->stream, g, sop, c->s64.p, c->q64.p, red);
         c->mark();
-        launch_axpy_norm<double, double>(c->stream, rows, c->diag64.p, x, c->r64.p, c->s64.p, c->q64.p, red, jac);
+        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, c->s64.p, c->q64.p, red);
         c->mark();
-        launch_update_dir<double, double>(c->stream, rows, c->diag64.p, c->r64.p, c->s64.p, red, jac);
+        launch_update_dir<double>(c->stream, g, c->r64.p, c->s64.p, red);
         c->mark();
     });
     if (rc) return rc;
+    if (jac) launch_unscale(c->stream, rows, x, c->sig64.p);
     HIPC(c, hipEventRecord(c->ev[1], c->stream));
     res->count = c->h_state->iter;
     res->status = c->h_state->status;
     res->reresid = c->h_state->rnorm0 > 0 ? c->h_state->rnorm / c->h_state->rnorm0 : c->h_state->rnorm;
-    launch_residual_csr(c->stream, op, b, x, nullptr, reduction(c), c->misc.p);
+    launch_residual_csr(c->stream, g, op, b, x, nullptr, c->pc.p, c->misc.p);
     HIPC(c, hipGetLastError());
     HIPC(c, hipMemcpyAsync(c->h_misc, c->misc.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));

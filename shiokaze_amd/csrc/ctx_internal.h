@@ -99,9 +99,18 @@ struct shk_ctx {
     std::vector<uint8_t> h_flag;
     DevBuf<int4> coord;
     DevBuf<int> nbr;
-    DevBuf<double> diag64, cx64, cy64, cz64, b64, x64, r64, s64, q64;
-    DevBuf<float> diag32, cx32, cy32, cz32, x32, r32, s32, q32;
-    DevBuf<double> pa, pb, misc;
+    // unscaled FP64 operator (diag, couplings zero off unknown pairs), rhs, sig = 1/sqrt(diag)
+    DevBuf<double> diag64, cx64, cy64, cz64, b64, sig64;
+    // couplings of the Jacobi-scaled operator, FP64 and FP32
+    DevBuf<double> hx64, hy64, hz64;
+    DevBuf<float> hx32, hy32, hz32;
+    // unscaled FP32 operator (mixed precision without preconditioner)
+    DevBuf<float> diag32, cx32, cy32, cz32;
+    // Krylov vectors
+    DevBuf<double> x64, r64, s64, q64;
+    DevBuf<float> x32, r32, s32, q32, rs32;
+    DevBuf<double> pa, pb, pc, misc;
+    shk::Grid grid;
     DevBuf<SolveState> state;
     SolveState *h_state = nullptr;
     double *h_misc = nullptr;
@@ -112,8 +121,9 @@ struct shk_ctx {
     bool profile = false;
     std::vector<hipEvent_t> pev;
     size_t pcur = 0;
-    int64_t pcount[3] = {0, 0, 0};
-    double pms[3] = {0.0, 0.0, 0.0};
+    static constexpr int kKernels = 3;  // launches per Krylov iteration
+    int64_t pcount[kKernels] = {0, 0, 0};
+    double pms[kKernels] = {0.0, 0.0, 0.0};
     // records the next profiling event after a Krylov kernel launch
     void mark() {
         if (profile && pcur < pev.size()) (void)hipEventRecord(pev[pcur++], stream);
@@ -149,8 +159,10 @@ int drive(shk_ctx *c, F &&f) {
 Reduction reduction(shk_ctx *c);
 BrickOp<double> op64(shk_ctx *c);
 BrickOp<float> op32(shk_ctx *c);
-// r = b - A x (FP64); out = {max|r|, sum r^2, sum b^2, max|b|}
-int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4]);
+// r = b - A x (FP64); out = {max|r|, sum r^2, sum b^2, max|b|}; optionally the
+// scaled FP32 copy rs = (float)(sig*r) for the next inner solve
+int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4], const double *sig = nullptr,
+                  float *rs = nullptr);
 void fill_true_residual(shk_result *res, const double rr[4]);
 int solve_fp64(shk_ctx *c, shk_result *res);
 

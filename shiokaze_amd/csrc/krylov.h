@@ -20,15 +20,17 @@
 namespace shk {
 
 struct Reduction {
-    double *pa;       // per-workgroup partials (>= blocks)
-    double *pb;
+    double *pa;       // s.q partials (>= blocks)
+    double *pb;       // r.r and max|r| partials (>= 2 * blocks)
     SolveState *st;   // device state
 };
 
 // Operator views.  Brick: vectors hold n_active bricks of 8x8x8 cells, x-fastest
 // inside a brick; nbr[6*slot + {-x,+x,-y,+y,-z,+z}] is the neighbour brick's slot
-// or -1.  The matrix is diag - sum_faces c_face * neighbour, with c_face the
-// coupling of the face stored at its high cell (cx/cy/cz = low face of a cell).
+// or -1.  The matrix is diag - sum_faces c_face * neighbour with c_face stored at
+// the face's high cell (cx/cy/cz = low face of a cell) and zero on every face
+// that touches a non-unknown; diag == nullptr means a unit diagonal (the
+// symmetrically Jacobi-scaled system).
 template <class C>
 struct BrickOp {
     const int *nbr;
@@ -38,38 +40,54 @@ struct BrickOp {
 struct CsrOp {
     const int64_t *rowptr, *cols;
     const double *vals;
-    const double *diag;  // extracted diagonal (Jacobi, masking)
     int64_t rows;
 };
 
-inline int blocks_for(int64_t n) { return (int)((n + kBrickCells - 1) / kBrickCells); }
+constexpr int kMaxBlocks = 1024;  // workgroups of a Krylov kernel: 4 per CU
 
-// x = 0, r = rhs, s = M^-1 r; rho = r.s, rnorm0 = max|r|, tol_abs = tol_rel*rnorm0.
-template <class T, class C, class TB>
-void launch_cg_init(hipStream_t st, int64_t n, const TB *rhs, const C *diag, T *x, T *r, T *s, Reduction red,
-                    double tol_rel, int64_t max_iter, bool jacobi);
+__host__ __device__ inline int blocks_for(int64_t n) { return (int)((n + kBrickCells - 1) / kBrickCells); }
 
-// One CG iteration is three kernels: q = A s with s.q (spmv_dot), the x/r update
-// with r.z and max|r| (axpy_norm) and s = z + beta s (update_dir).
+// Launch shape of the Krylov kernels over n elements (tiles of 512).
+struct Grid {
+    int64_t n = 0;
+    int tiles = 0;   // ceil(n / 512)
+    int per = 1;     // tiles per workgroup
+    int blocks = 1;  // workgroups (<= max_blocks)
+};
+Grid make_grid(int64_t n, int max_blocks = kMaxBlocks);
+
+// x = 0, r = s = sig*rhs (or rhs); rho = r.r; rnorm0 = max|rhs| over unknowns;
+// tol_abs = tol_rel * rnorm0.  sig != nullptr selects the scaled system.
+template <class T, class TB>
+void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *sig, T *x, T *r, T *s, Reduction red,
+                    double tol_rel, int64_t max_iter);
+
+// One CG iteration: spmv_dot, axpy_norm, update_dir (see krylov.hip).  With
+// sig != nullptr the stop test uses the unscaled residual max|r/sig|.
 template <class T, class C>
-void launch_spmv_dot(hipStream_t st, const BrickOp<C> &op, const T *s, T *q, Reduction red);
-void launch_spmv_dot_csr(hipStream_t st, const CsrOp &op, const double *s, double *q, Reduction red);
-template <class T, class C>
-void launch_axpy_norm(hipStream_t st, int64_t n, const C *diag, T *x, T *r, const T *s, const T *q, Reduction red,
-                      bool jacobi);
-template <class T, class C>
-void launch_update_dir(hipStream_t st, int64_t n, const C *diag, const T *r, T *s, Reduction red, bool jacobi);
+void launch_spmv_dot(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *s, T *q, Reduction red);
+void launch_spmv_dot_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *s, double *q, Reduction red);
+template <class T>
+void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
+                      Reduction red);
+template <class T>
+void launch_update_dir(hipStream_t st, const Grid &g, const T *r, T *s, Reduction red);
 
-// r = b - A x in FP64; out[0] = max|r|, out[1] = sum r^2, out[2] = sum b^2, out[3] = max|b|.
-void launch_residual(hipStream_t st, const BrickOp<double> &op, const double *b, const double *x, double *r,
-                     Reduction red, double *out);
-void launch_residual_csr(hipStream_t st, const CsrOp &op, const double *b, const double *x, double *r, Reduction red,
-                         double *out);
+// r = b - A x in FP64 (unscaled operator with its diagonal); part holds 4*blocks
+// partials; out = {max|r|, sum r^2, sum b^2, max|b|}.  Optionally rs = (float)
+// (sig * r) (or (float) r without sig): the next inner solve's right-hand side.
+void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,
+                     double *r, double *part, double *out, const double *sig = nullptr, float *rs = nullptr);
+void launch_residual_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *b, const double *x, double *r,
+                         double *part, double *out);
 
-// x64 += (double) d32 (refinement correction); optionally r32 = (float) r64.
-void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d);
-
-// diag[row] = A[row,row] of a CSR matrix.
-void launch_csr_diag(hipStream_t st, const CsrOp &op, double *diag);
+// x64 += sig * (double) d32 (sig may be nullptr): the refinement correction.
+void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d, const double *sig);
+// x *= sig: back from the scaled unknowns.
+void launch_unscale(hipStream_t st, int64_t n, double *x, const double *sig);
+// diag[row] = A[row,row], sig[row] = 1/sqrt(diag) (0 if not positive).
+void launch_csr_diag(hipStream_t st, const CsrOp &op, double *diag, double *sig);
+// out = values of D^-1/2 A D^-1/2 on A's pattern.
+void launch_csr_scale(hipStream_t st, const CsrOp &op, const double *sig, double *out);
 
 }  // namespace shk

@@ -1,12 +1,23 @@
-// krylov.hip -- fused conjugate gradient kernels for gfx950.
+// krylov.hip -- conjugate gradient kernels for gfx950.
 //
-// One workgroup of 512 threads (8 waves of 64) per 8x8x8 brick / per 512 rows.
-// Per iteration:
-//   k_spmv_dot    q = A s, partial s.q          -> last block: alpha
-//   k_axpy_norm   x += alpha s, r -= alpha q, z = M^-1 r, partials r.z, max|r|
-//                                               -> last block: stop test, beta
-//   k_update_dir  s = z + beta s
-// Dot products are accumulated in FP64 whatever the vector type.
+// The iteration is PCGSolver::solve (local/include/pcgsolver/pcg_solver.h:246-295).
+// Jacobi preconditioning is applied as a symmetric scaling of the system
+// (D^-1/2 A D^-1/2 has a unit diagonal), which gives the Jacobi-PCG iterates
+// while the kernels never read a diagonal; off-diagonal couplings are zero on
+// every face that touches a non-unknown, so no masks are needed either.  In both
+// the plain and the scaled form z = r.
+//
+// Three kernels per iteration, no host round trip and no grid-wide fences:
+//   k_spmv_dot    q = A s over the block's chunk of tiles, partial s.q
+//   k_axpy_norm   every block folds the <= kMaxBlocks s.q partials in the same
+//                 fixed order (bit-identical alpha everywhere), x += alpha s,
+//                 r -= alpha q, partials r.r and max|r|
+//   k_update_dir  every block folds those partials: stop test, beta,
+//                 s = r + beta s
+// A tile is 512 elements: one 8x8x8 brick, or 512 CSR rows.  Block g of a grid of
+// G processes tiles [g*per, (g+1)*per) after the XCD remap, so each XCD streams
+// one contiguous range.  State written by block 0 of a kernel is only read by
+// later kernels (or double-buffered through *_prev fields).
 #include "krylov.h"
 
 #include "shk_projection.h"
@@ -17,81 +28,106 @@ namespace {
 
 constexpr int NT = kBrickCells;
 
-__device__ inline int64_t flat_elem(int64_t n) {
-    return (int64_t)xcd_slot((int)blockIdx.x, (int)gridDim.x) * NT + threadIdx.x;
+__device__ inline void chunk_range(int tiles, int per, int &t0, int &t1) {
+    const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    t0 = g * per;
+    t1 = t0 + per < tiles ? t0 + per : tiles;
 }
 
-template <class T>
-__device__ inline T absv(T v) {
-    return v < T(0) ? -v : v;
+// Deterministic fold of n partials, result broadcast to every thread.
+__device__ inline double fold_sum(const double *p, int n, double *lds, double *bcast) {
+    const double v = reduce_partials_sum<NT>(p, n, lds);
+    if (threadIdx.x == 0) *bcast = v;
+    __syncthreads();
+    const double r = *bcast;
+    __syncthreads();
+    return r;
+}
+__device__ inline double fold_max(const double *p, int n, double *lds, double *bcast) {
+    const double v = reduce_partials_max<NT>(p, n, lds);
+    if (threadIdx.x == 0) *bcast = v;
+    __syncthreads();
+    const double r = *bcast;
+    __syncthreads();
+    return r;
 }
 
 // ---------------------------------------------------------------- init
 
-template <class T, class C, class TB, bool JAC>
-__global__ __launch_bounds__(NT) void k_cg_init(int64_t n, const TB *__restrict__ rhs, const C *__restrict__ diag,
-                                                T *__restrict__ x, T *__restrict__ r, T *__restrict__ s, Reduction red,
-                                                double tol_rel, int64_t max_iter) {
+template <class T, class TB, bool SCALED>
+__global__ __launch_bounds__(NT) void k_cg_init(int64_t n, int tiles, int per, const TB *__restrict__ rhs,
+                                                const double *__restrict__ sig, T *__restrict__ x, T *__restrict__ r,
+                                                T *__restrict__ s, Reduction red) {
     __shared__ double lds[NT / kWave];
-    const int64_t e = flat_elem(n);
-    double rz = 0.0, rm = 0.0;
-    if (e < n) {
-        const C d = diag[e];
-        const T rv = d != C(0) ? (T)rhs[e] : T(0);
-        const T z = (JAC && d != C(0)) ? rv / (T)d : rv;
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    double rr = 0.0, rm = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int64_t e = (int64_t)t * NT + threadIdx.x;
+        if (e >= n) continue;
+        const double b = (double)rhs[e];
+        const T rv = SCALED ? (T)(sig[e] * b) : (T)b;
         x[e] = T(0);
         r[e] = rv;
-        s[e] = z;
-        rz = (double)rv * (double)z;
-        rm = fabs((double)rv);
+        s[e] = rv;
+        rr += (double)rv * (double)rv;
+        // with a scaled system the stop test is on the unscaled residual: max|b|
+        rm = fmax(rm, SCALED ? (sig[e] > 0.0 ? fabs(b) : 0.0) : fabs((double)rv));
     }
-    const double bs = block_sum<NT>(rz, lds);
+    const double bs = block_sum<NT>(rr, lds);
     const double bm = block_max<NT>(rm, lds);
     if (threadIdx.x == 0) {
         red.pa[blockIdx.x] = bs;
         red.pb[blockIdx.x] = bm;
     }
-    if (last_block_arrive(&red.st->ticket[kTicketInit])) {
-        const double rho = reduce_partials_sum<NT>(red.pa, gridDim.x, lds);
-        const double bmax = reduce_partials_max<NT>(red.pb, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            SolveState *st = red.st;
-            double tol = tol_rel < 1e-30 ? 1e-30 : tol_rel;  // pcg_solver.h:235
-            st->rho = rho;
-            st->alpha = 0.0;
-            st->beta = 0.0;
-            st->sigma = 0.0;
-            st->rnorm = bmax;
-            st->rnorm0 = bmax;
-            st->tol_abs = tol * bmax;
-            st->iter = 0;
-            st->max_iter = max_iter;
-            st->done = 0;
-            st->status = SHK_OK;
-            if (bmax == 0.0) {
-                st->done = 1;
-            } else if (rho == 0.0) {
-                st->done = 1;
-                st->status = SHK_BREAKDOWN;
-            } else if (rho != rho) {
-                st->done = 1;
-                st->status = SHK_NAN;
-            } else if (max_iter <= 0) {
-                st->done = 1;
-                st->status = SHK_NOT_CONVERGED;
-            }
+}
+
+constexpr int NF = 1024;
+
+__global__ __launch_bounds__(NF) void k_finish_init(int nblocks, Reduction red, double tol_rel, int64_t max_iter) {
+    __shared__ double lds[NF / kWave];
+    const double rho = reduce_partials_sum<NF>(red.pa, nblocks, lds);
+    const double bmax = reduce_partials_max<NF>(red.pb, nblocks, lds);
+    if (threadIdx.x == 0) {
+        SolveState *st = red.st;
+        const double tol = tol_rel < 1e-30 ? 1e-30 : tol_rel;  // pcg_solver.h:235
+        st->rho = rho;
+        st->rho_prev = rho;
+        st->alpha = 0.0;
+        st->beta = 0.0;
+        st->sigma = 0.0;
+        st->rnorm = bmax;
+        st->rnorm0 = bmax;
+        st->tol_abs = tol * bmax;
+        st->iter = 0;
+        st->iter_prev = 0;
+        st->max_iter = max_iter;
+        st->done = 0;
+        st->status = SHK_OK;
+        // pcg_solver.h:254-271: zero rhs, rho == 0, NaN
+        if (bmax == 0.0) {
+            st->done = 1;
+        } else if (rho == 0.0) {
+            st->done = 1;
+            st->status = SHK_BREAKDOWN;
+        } else if (rho != rho) {
+            st->done = 1;
+            st->status = SHK_NAN;
+        } else if (max_iter <= 0) {
+            st->done = 1;
+            st->status = SHK_NOT_CONVERGED;
         }
     }
 }
 
 // ---------------------------------------------------------------- stencil
 
-template <class T, class C>
-__device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, C dval, const C *__restrict__ cx,
-                                const C *__restrict__ cy, const C *__restrict__ cz, const T *__restrict__ v) {
+template <class T, class C, bool UNIT>
+__device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, const C *__restrict__ diag,
+                                const C *__restrict__ cx, const C *__restrict__ cy, const C *__restrict__ cz,
+                                const T *__restrict__ v) {
     const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
-    T acc = (T)dval * v[e];
-    // -x / +x
+    T acc = UNIT ? v[e] : (T)diag[e] * v[e];
     if (i > 0) {
         acc -= (T)cx[e] * v[e - 1];
     } else {
@@ -107,7 +143,6 @@ __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, C
             acc -= (T)cx[m] * v[m];
         }
     }
-    // -y / +y
     if (j > 0) {
         acc -= (T)cy[e] * v[e - 8];
     } else {
@@ -123,7 +158,6 @@ __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, C
             acc -= (T)cy[m] * v[m];
         }
     }
-    // -z / +z
     if (k > 0) {
         acc -= (T)cz[e] * v[e - 64];
     } else {
@@ -142,221 +176,255 @@ __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, C
     return acc;
 }
 
-__device__ inline void finish_alpha(const Reduction &red, double *lds) {
-    const double sigma = reduce_partials_sum<NT>(red.pa, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-        SolveState *st = red.st;
-        st->sigma = sigma;
-        st->alpha = st->rho / sigma;
-        if (!(sigma > 0.0)) {
-            st->done = 1;
-            st->status = (sigma != sigma) ? SHK_NAN : SHK_BREAKDOWN;
-        }
-    }
-}
-
-template <class T, class C>
-__global__ __launch_bounds__(NT) void k_spmv_dot(BrickOp<C> op, const T *__restrict__ s, T *__restrict__ q,
+template <class T, class C, bool UNIT>
+__global__ __launch_bounds__(NT) void k_spmv_dot(BrickOp<C> op, int per, const T *__restrict__ s, T *__restrict__ q,
                                                  Reduction red) {
     __shared__ double lds[NT / kWave];
     if (red.st->done) return;
-    const int slot = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    int t0, t1;
+    chunk_range(op.n_active, per, t0, t1);
     const int l = threadIdx.x;
-    const int64_t e = (int64_t)slot * NT + l;
-    const C d = op.diag[e];
-    T qv = T(0);
-    double prod = 0.0;
-    if (d != C(0)) {
-        qv = brick_apply<T, C>(op.nbr + 6 * slot, e, l, d, op.cx, op.cy, op.cz, s);
-        prod = (double)s[e] * (double)qv;
+    double acc = 0.0;
+    for (int slot = t0; slot < t1; ++slot) {
+        const int64_t e = (int64_t)slot * NT + l;
+        const T qv = brick_apply<T, C, UNIT>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, s);
+        q[e] = qv;
+        acc += (double)s[e] * (double)qv;
     }
-    q[e] = qv;
-    const double bs = block_sum<NT>(prod, lds);
+    const double bs = block_sum<NT>(acc, lds);
     if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
-    if (last_block_arrive(&red.st->ticket[kTicketAlpha])) finish_alpha(red, lds);
 }
 
-__global__ __launch_bounds__(NT) void k_spmv_dot_csr(CsrOp op, const double *__restrict__ s, double *__restrict__ q,
-                                                     Reduction red) {
+__global__ __launch_bounds__(NT) void k_spmv_dot_csr(CsrOp op, int per, const double *__restrict__ s,
+                                                     double *__restrict__ q, Reduction red) {
     __shared__ double lds[NT / kWave];
     if (red.st->done) return;
-    const int64_t e = flat_elem(op.rows);
-    double prod = 0.0;
-    if (e < op.rows) {
-        double acc = 0.0;
-        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) acc += op.vals[p] * s[op.cols[p]];
-        q[e] = acc;
-        prod = s[e] * acc;
+    int t0, t1;
+    chunk_range(blocks_for(op.rows), per, t0, t1);
+    double acc = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int64_t e = (int64_t)t * NT + threadIdx.x;
+        if (e >= op.rows) continue;
+        double a = 0.0;
+        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) a += op.vals[p] * s[op.cols[p]];
+        q[e] = a;
+        acc += s[e] * a;
     }
-    const double bs = block_sum<NT>(prod, lds);
+    const double bs = block_sum<NT>(acc, lds);
     if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
-    if (last_block_arrive(&red.st->ticket[kTicketAlpha])) finish_alpha(red, lds);
 }
 
 // ---------------------------------------------------------------- axpy + norm
 
-template <class T, class C, bool JAC>
-__global__ __launch_bounds__(NT) void k_axpy_norm(int64_t n, const C *__restrict__ diag, T *__restrict__ x,
-                                                  T *__restrict__ r, const T *__restrict__ s, const T *__restrict__ q,
-                                                  Reduction red) {
+template <class T, bool SCALED>
+__global__ __launch_bounds__(NT) void k_axpy_norm(int64_t n, int tiles, int per, const double *__restrict__ sig,
+                                                  T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
+                                                  const T *__restrict__ q, Reduction red) {
     __shared__ double lds[NT / kWave];
-    if (red.st->done) return;
-    const T a = (T)red.st->alpha;
-    const int64_t e = flat_elem(n);
-    double rz = 0.0, rm = 0.0;
-    if (e < n) {
-        const C d = diag[e];
-        if (d != C(0)) {
-            x[e] += a * s[e];
-            const T rv = r[e] - a * q[e];
-            r[e] = rv;
-            const T z = JAC ? rv / (T)d : rv;
-            rz = (double)rv * (double)z;
-            rm = fabs((double)rv);
+    __shared__ double bc;
+    SolveState *st = red.st;
+    if (st->done) return;
+    const double sigma = fold_sum(red.pa, gridDim.x, lds, &bc);
+    const double rho = st->rho;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (lead) {
+        st->sigma = sigma;
+        st->rho_prev = rho;
+        st->iter_prev = st->iter;
+    }
+    if (!(sigma > 0.0)) {
+        if (lead) {
+            st->done = 1;
+            st->status = (sigma != sigma) ? SHK_NAN : SHK_BREAKDOWN;
+        }
+        return;
+    }
+    const double alpha = rho / sigma;
+    if (lead) st->alpha = alpha;
+    const T a = (T)alpha;
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    double rr = 0.0, rm = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int64_t e = (int64_t)t * NT + threadIdx.x;
+        if (e >= n) continue;
+        x[e] += a * s[e];
+        const T rv = r[e] - a * q[e];
+        r[e] = rv;
+        rr += (double)rv * (double)rv;
+        if (SCALED) {
+            const double sg = sig[e];
+            if (sg > 0.0) rm = fmax(rm, fabs((double)rv) / sg);
+        } else {
+            rm = fmax(rm, fabs((double)rv));
         }
     }
-    const double bs = block_sum<NT>(rz, lds);
+    const double bs = block_sum<NT>(rr, lds);
     const double bm = block_max<NT>(rm, lds);
     if (threadIdx.x == 0) {
-        red.pa[blockIdx.x] = bs;
-        red.pb[blockIdx.x] = bm;
-    }
-    if (last_block_arrive(&red.st->ticket[kTicketAxpy])) {
-        const double rho_new = reduce_partials_sum<NT>(red.pa, gridDim.x, lds);
-        const double rn = reduce_partials_max<NT>(red.pb, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            SolveState *st = red.st;
-            const int64_t it = st->iter + 1;
-            st->iter = it;
-            st->rnorm = rn;
-            if (rn <= st->tol_abs) {
-                st->done = 1;
-                st->status = SHK_OK;
-            } else if (rho_new != rho_new) {
-                st->done = 1;
-                st->status = SHK_NAN;
-            } else if (it >= st->max_iter) {
-                st->done = 1;
-                st->status = SHK_NOT_CONVERGED;
-            } else {
-                st->beta = rho_new / st->rho;
-                st->rho = rho_new;
-            }
-        }
+        red.pb[blockIdx.x] = bs;
+        red.pb[gridDim.x + blockIdx.x] = bm;
     }
 }
 
-template <class T, class C, bool JAC>
-__global__ __launch_bounds__(NT) void k_update_dir(int64_t n, const C *__restrict__ diag, const T *__restrict__ r,
-                                                   T *__restrict__ s, const SolveState *__restrict__ st) {
+template <class T>
+__global__ __launch_bounds__(NT) void k_update_dir(int64_t n, int tiles, int per, const T *__restrict__ r,
+                                                   T *__restrict__ s, Reduction red) {
+    __shared__ double lds[NT / kWave];
+    __shared__ double bc;
+    SolveState *st = red.st;
     if (st->done) return;
-    const T b = (T)st->beta;
-    const int64_t e = flat_elem(n);
-    if (e < n) {
-        const C d = diag[e];
-        if (d != C(0)) {
-            const T rv = r[e];
-            const T z = JAC ? rv / (T)d : rv;
-            s[e] = z + b * s[e];
+    const double rho_new = fold_sum(red.pb, gridDim.x, lds, &bc);
+    const double rn = fold_max(red.pb + gridDim.x, gridDim.x, lds, &bc);
+    const double rho_prev = st->rho_prev;
+    const int64_t it = st->iter_prev + 1;
+    // pcg_solver.h:281-290: the max-norm stop test comes first, then the direction
+    const bool stop = rn <= st->tol_abs;
+    const bool nan = rho_new != rho_new;
+    const bool cap = it >= st->max_iter;
+    const double beta = rho_new / rho_prev;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->iter = it;
+        st->rnorm = rn;
+        if (stop) {
+            st->done = 1;
+            st->status = SHK_OK;
+        } else if (nan) {
+            st->done = 1;
+            st->status = SHK_NAN;
+        } else if (cap) {
+            st->done = 1;
+            st->status = SHK_NOT_CONVERGED;
+        } else {
+            st->beta = beta;
+            st->rho = rho_new;
         }
+    }
+    if (stop || nan || cap) return;
+    const T b = (T)beta;
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    for (int t = t0; t < t1; ++t) {
+        const int64_t e = (int64_t)t * NT + threadIdx.x;
+        if (e >= n) continue;
+        s[e] = r[e] + b * s[e];
     }
 }
 
 // ---------------------------------------------------------------- true residual
 
-__device__ inline void finish_residual(const Reduction &red, double *lds, double *out) {
-    const double rmax = reduce_partials_max<NT>(red.pa, gridDim.x, lds);
-    const double r2 = reduce_partials_sum<NT>(red.pb, gridDim.x, lds);
-    if (threadIdx.x == 0) {
-        out[0] = rmax;
-        out[1] = r2;
-    }
-}
-
-__global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, const double *__restrict__ b,
-                                                 const double *__restrict__ x, double *__restrict__ r, Reduction red,
-                                                 double *__restrict__ b_partial, double *__restrict__ out) {
+__global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, int per, const double *__restrict__ b,
+                                                 const double *__restrict__ x, double *__restrict__ r,
+                                                 double *__restrict__ part, const double *__restrict__ sig,
+                                                 float *__restrict__ rs) {
     __shared__ double lds[NT / kWave];
-    const int slot = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    int t0, t1;
+    chunk_range(op.n_active, per, t0, t1);
     const int l = threadIdx.x;
-    const int64_t e = (int64_t)slot * NT + l;
-    const double d = op.diag[e];
-    double rv = 0.0, bv = 0.0;
-    if (d != 0.0) {
-        bv = b[e];
-        rv = bv - brick_apply<double, double>(op.nbr + 6 * slot, e, l, d, op.cx, op.cy, op.cz, x);
-    }
-    if (r) r[e] = rv;
-    const double m = block_max<NT>(fabs(rv), lds);
-    const double s2 = block_sum<NT>(rv * rv, lds);
-    const double b2 = block_sum<NT>(bv * bv, lds);
-    const double bm = block_max<NT>(fabs(bv), lds);
-    if (threadIdx.x == 0) {
-        red.pa[blockIdx.x] = m;
-        red.pb[blockIdx.x] = s2;
-        b_partial[blockIdx.x] = b2;
-        b_partial[gridDim.x + blockIdx.x] = bm;
-    }
-    if (last_block_arrive(&red.st->ticket[kTicketMisc])) {
-        finish_residual(red, lds, out);
-        const double bb = reduce_partials_sum<NT>(b_partial, gridDim.x, lds);
-        const double bx = reduce_partials_max<NT>(b_partial + gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            out[2] = bb;
-            out[3] = bx;
+    double m = 0.0, s2 = 0.0, b2 = 0.0, bm = 0.0;
+    for (int slot = t0; slot < t1; ++slot) {
+        const int64_t e = (int64_t)slot * NT + l;
+        double rv = 0.0, bv = 0.0;
+        if (op.diag[e] != 0.0) {
+            bv = b[e];
+            rv = bv - brick_apply<double, double, false>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, x);
         }
+        if (r) r[e] = rv;
+        if (rs) rs[e] = (float)(sig ? sig[e] * rv : rv);
+        m = fmax(m, fabs(rv));
+        s2 += rv * rv;
+        b2 += bv * bv;
+        bm = fmax(bm, fabs(bv));
+    }
+    const double v0 = block_max<NT>(m, lds);
+    const double v1 = block_sum<NT>(s2, lds);
+    const double v2 = block_sum<NT>(b2, lds);
+    const double v3 = block_max<NT>(bm, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = v0;
+        part[gridDim.x + blockIdx.x] = v1;
+        part[2 * gridDim.x + blockIdx.x] = v2;
+        part[3 * gridDim.x + blockIdx.x] = v3;
     }
 }
 
-__global__ __launch_bounds__(NT) void k_residual_csr(CsrOp op, const double *__restrict__ b,
+__global__ __launch_bounds__(NT) void k_residual_csr(CsrOp op, int per, const double *__restrict__ b,
                                                      const double *__restrict__ x, double *__restrict__ r,
-                                                     Reduction red, double *__restrict__ b_partial,
-                                                     double *__restrict__ out) {
+                                                     double *__restrict__ part) {
     __shared__ double lds[NT / kWave];
-    const int64_t e = flat_elem(op.rows);
-    double rv = 0.0, bv = 0.0;
-    if (e < op.rows) {
-        double acc = 0.0;
-        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) acc += op.vals[p] * x[op.cols[p]];
-        bv = b[e];
-        rv = bv - acc;
+    int t0, t1;
+    chunk_range(blocks_for(op.rows), per, t0, t1);
+    double m = 0.0, s2 = 0.0, b2 = 0.0, bm = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int64_t e = (int64_t)t * NT + threadIdx.x;
+        if (e >= op.rows) continue;
+        double a = 0.0;
+        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) a += op.vals[p] * x[op.cols[p]];
+        const double bv = b[e];
+        const double rv = bv - a;
         if (r) r[e] = rv;
+        m = fmax(m, fabs(rv));
+        s2 += rv * rv;
+        b2 += bv * bv;
+        bm = fmax(bm, fabs(bv));
     }
-    const double m = block_max<NT>(fabs(rv), lds);
-    const double s2 = block_sum<NT>(rv * rv, lds);
-    const double b2 = block_sum<NT>(bv * bv, lds);
-    const double bm = block_max<NT>(fabs(bv), lds);
+    const double v0 = block_max<NT>(m, lds);
+    const double v1 = block_sum<NT>(s2, lds);
+    const double v2 = block_sum<NT>(b2, lds);
+    const double v3 = block_max<NT>(bm, lds);
     if (threadIdx.x == 0) {
-        red.pa[blockIdx.x] = m;
-        red.pb[blockIdx.x] = s2;
-        b_partial[blockIdx.x] = b2;
-        b_partial[gridDim.x + blockIdx.x] = bm;
+        part[blockIdx.x] = v0;
+        part[gridDim.x + blockIdx.x] = v1;
+        part[2 * gridDim.x + blockIdx.x] = v2;
+        part[3 * gridDim.x + blockIdx.x] = v3;
     }
-    if (last_block_arrive(&red.st->ticket[kTicketMisc])) {
-        finish_residual(red, lds, out);
-        const double bb = reduce_partials_sum<NT>(b_partial, gridDim.x, lds);
-        const double bx = reduce_partials_max<NT>(b_partial + gridDim.x, gridDim.x, lds);
-        if (threadIdx.x == 0) {
-            out[2] = bb;
-            out[3] = bx;
-        }
+}
+
+__global__ __launch_bounds__(NF) void k_finish_residual(int nblocks, const double *__restrict__ part,
+                                                       double *__restrict__ out) {
+    __shared__ double lds[NF / kWave];
+    const double v0 = reduce_partials_max<NF>(part, nblocks, lds);
+    const double v1 = reduce_partials_sum<NF>(part + nblocks, nblocks, lds);
+    const double v2 = reduce_partials_sum<NF>(part + 2 * nblocks, nblocks, lds);
+    const double v3 = reduce_partials_max<NF>(part + 3 * nblocks, nblocks, lds);
+    if (threadIdx.x == 0) {
+        out[0] = v0;
+        out[1] = v1;
+        out[2] = v2;
+        out[3] = v3;
     }
 }
 
 // ---------------------------------------------------------------- misc
 
-__global__ __launch_bounds__(NT) void k_add_correction(int64_t n, double *__restrict__ x, const float *__restrict__ d) {
+__global__ __launch_bounds__(NT) void k_add_correction(int64_t n, double *__restrict__ x, const float *__restrict__ d,
+                                                       const double *__restrict__ sig) {
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e < n) x[e] += (double)d[e];
+    if (e < n) x[e] += sig ? sig[e] * (double)d[e] : (double)d[e];
 }
 
-__global__ __launch_bounds__(NT) void k_csr_diag(CsrOp op, double *__restrict__ diag) {
+__global__ __launch_bounds__(NT) void k_unscale(int64_t n, double *__restrict__ x, const double *__restrict__ sig) {
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e < n) x[e] *= sig[e];
+}
+
+__global__ __launch_bounds__(NT) void k_csr_diag(CsrOp op, double *__restrict__ diag, double *__restrict__ sig) {
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e < op.rows) {
         double dv = 0.0;
         for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p)
             if (op.cols[p] == e) dv = op.vals[p];
         diag[e] = dv;
+        if (sig) sig[e] = dv > 0.0 ? 1.0 / sqrt(dv) : 0.0;
+    }
+}
+
+// Scaled CSR copy: out[p] = sig[row] * vals[p] * sig[col].
+__global__ __launch_bounds__(NT) void k_csr_scale(CsrOp op, const double *__restrict__ sig, double *__restrict__ out) {
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e < op.rows) {
+        const double se = sig[e];
+        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) out[p] = se * op.vals[p] * sig[op.cols[p]];
     }
 }
 
@@ -364,81 +432,101 @@ __global__ __launch_bounds__(NT) void k_csr_diag(CsrOp op, double *__restrict__ 
 
 // ---------------------------------------------------------------- launchers
 
-template <class T, class C, class TB>
-void launch_cg_init(hipStream_t st, int64_t n, const TB *rhs, const C *diag, T *x, T *r, T *s, Reduction red,
-                    double tol_rel, int64_t max_iter, bool jacobi) {
-    const int g = blocks_for(n);
-    if (jacobi)
-        hipLaunchKernelGGL((k_cg_init<T, C, TB, true>), dim3(g), dim3(NT), 0, st, n, rhs, diag, x, r, s, red, tol_rel,
-                           max_iter);
+Grid make_grid(int64_t n, int max_blocks) {
+    Grid g;
+    g.n = n;
+    g.tiles = blocks_for(n);
+    if (max_blocks < 1) max_blocks = kMaxBlocks;
+    g.per = (g.tiles + max_blocks - 1) / max_blocks;
+    if (g.per < 1) g.per = 1;
+    g.blocks = (g.tiles + g.per - 1) / g.per;
+    if (g.blocks < 1) g.blocks = 1;
+    return g;
+}
+
+template <class T, class TB>
+void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *sig, T *x, T *r, T *s, Reduction red,
+                    double tol_rel, int64_t max_iter) {
+    if (sig)
+        hipLaunchKernelGGL((k_cg_init<T, TB, true>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, rhs, sig, x,
+                           r, s, red);
     else
-        hipLaunchKernelGGL((k_cg_init<T, C, TB, false>), dim3(g), dim3(NT), 0, st, n, rhs, diag, x, r, s, red, tol_rel,
-                           max_iter);
+        hipLaunchKernelGGL((k_cg_init<T, TB, false>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, rhs, sig,
+                           x, r, s, red);
+    hipLaunchKernelGGL(k_finish_init, dim3(1), dim3(NF), 0, st, g.blocks, red, tol_rel, max_iter);
 }
 
 template <class T, class C>
-void launch_spmv_dot(hipStream_t st, const BrickOp<C> &op, const T *s, T *q, Reduction red) {
-    hipLaunchKernelGGL((k_spmv_dot<T, C>), dim3(op.n_active), dim3(NT), 0, st, op, s, q, red);
-}
-
-void launch_spmv_dot_csr(hipStream_t st, const CsrOp &op, const double *s, double *q, Reduction red) {
-    hipLaunchKernelGGL(k_spmv_dot_csr, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, s, q, red);
-}
-
-template <class T, class C>
-void launch_axpy_norm(hipStream_t st, int64_t n, const C *diag, T *x, T *r, const T *s, const T *q, Reduction red,
-                      bool jacobi) {
-    const int g = blocks_for(n);
-    if (jacobi)
-        hipLaunchKernelGGL((k_axpy_norm<T, C, true>), dim3(g), dim3(NT), 0, st, n, diag, x, r, s, q, red);
+void launch_spmv_dot(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *s, T *q, Reduction red) {
+    if (op.diag)
+        hipLaunchKernelGGL((k_spmv_dot<T, C, false>), dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
     else
-        hipLaunchKernelGGL((k_axpy_norm<T, C, false>), dim3(g), dim3(NT), 0, st, n, diag, x, r, s, q, red);
+        hipLaunchKernelGGL((k_spmv_dot<T, C, true>), dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
 }
 
-template <class T, class C>
-void launch_update_dir(hipStream_t st, int64_t n, const C *diag, const T *r, T *s, Reduction red, bool jacobi) {
-    const int g = blocks_for(n);
-    if (jacobi)
-        hipLaunchKernelGGL((k_update_dir<T, C, true>), dim3(g), dim3(NT), 0, st, n, diag, r, s, red.st);
+void launch_spmv_dot_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *s, double *q, Reduction red) {
+    hipLaunchKernelGGL(k_spmv_dot_csr, dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
+}
+
+template <class T>
+void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
+                      Reduction red) {
+    if (sig)
+        hipLaunchKernelGGL((k_axpy_norm<T, true>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, sig, x, r, s,
+                           q, red);
     else
-        hipLaunchKernelGGL((k_update_dir<T, C, false>), dim3(g), dim3(NT), 0, st, n, diag, r, s, red.st);
+        hipLaunchKernelGGL((k_axpy_norm<T, false>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, sig, x, r,
+                           s, q, red);
 }
 
-void launch_residual(hipStream_t st, const BrickOp<double> &op, const double *b, const double *x, double *r,
-                     Reduction red, double *out) {
-    const int g = op.n_active;
-    // pb doubles as the r^2 partials; the b partials follow the a/b partial arrays
-    hipLaunchKernelGGL(k_residual, dim3(g), dim3(NT), 0, st, op, b, x, r, red, red.pb + g, out);
+template <class T>
+void launch_update_dir(hipStream_t st, const Grid &g, const T *r, T *s, Reduction red) {
+    hipLaunchKernelGGL((k_update_dir<T>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, r, s, red);
 }
 
-void launch_residual_csr(hipStream_t st, const CsrOp &op, const double *b, const double *x, double *r, Reduction red,
-                         double *out) {
-    const int g = blocks_for(op.rows);
-    hipLaunchKernelGGL(k_residual_csr, dim3(g), dim3(NT), 0, st, op, b, x, r, red, red.pb + g, out);
+void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,
+                     double *r, double *part, double *out, const double *sig, float *rs) {
+    hipLaunchKernelGGL(k_residual, dim3(g.blocks), dim3(NT), 0, st, op, g.per, b, x, r, part, sig, rs);
+    hipLaunchKernelGGL(k_finish_residual, dim3(1), dim3(NF), 0, st, g.blocks, part, out);
 }
 
-void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d) {
-    hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(n)), dim3(NT), 0, st, n, x, d);
+void launch_residual_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *b, const double *x, double *r,
+                         double *part, double *out) {
+    hipLaunchKernelGGL(k_residual_csr, dim3(g.blocks), dim3(NT), 0, st, op, g.per, b, x, r, part);
+    hipLaunchKernelGGL(k_finish_residual, dim3(1), dim3(NF), 0, st, g.blocks, part, out);
 }
 
-void launch_csr_diag(hipStream_t st, const CsrOp &op, double *diag) {
-    hipLaunchKernelGGL(k_csr_diag, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, diag);
+void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d, const double *sig) {
+    hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(n)), dim3(NT), 0, st, n, x, d, sig);
 }
 
-template void launch_cg_init<double, double, double>(hipStream_t, int64_t, const double *, const double *, double *,
-                                                     double *, double *, Reduction, double, int64_t, bool);
-template void launch_cg_init<float, float, double>(hipStream_t, int64_t, const double *, const float *, float *,
-                                                   float *, float *, Reduction, double, int64_t, bool);
-template void launch_spmv_dot<double, double>(hipStream_t, const BrickOp<double> &, const double *, double *,
-                                              Reduction);
-template void launch_spmv_dot<float, float>(hipStream_t, const BrickOp<float> &, const float *, float *, Reduction);
-template void launch_axpy_norm<double, double>(hipStream_t, int64_t, const double *, double *, double *,
-                                               const double *, const double *, Reduction, bool);
-template void launch_axpy_norm<float, float>(hipStream_t, int64_t, const float *, float *, float *, const float *,
-                                             const float *, Reduction, bool);
-template void launch_update_dir<double, double>(hipStream_t, int64_t, const double *, const double *, double *,
-                                                Reduction, bool);
-template void launch_update_dir<float, float>(hipStream_t, int64_t, const float *, const float *, float *, Reduction,
-                                              bool);
+void launch_unscale(hipStream_t st, int64_t n, double *x, const double *sig) {
+    hipLaunchKernelGGL(k_unscale, dim3(blocks_for(n)), dim3(NT), 0, st, n, x, sig);
+}
+
+void launch_csr_diag(hipStream_t st, const CsrOp &op, double *diag, double *sig) {
+    hipLaunchKernelGGL(k_csr_diag, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, diag, sig);
+}
+
+void launch_csr_scale(hipStream_t st, const CsrOp &op, const double *sig, double *out) {
+    hipLaunchKernelGGL(k_csr_scale, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, sig, out);
+}
+
+template void launch_cg_init<double, double>(hipStream_t, const Grid &, const double *, const double *, double *,
+                                             double *, double *, Reduction, double, int64_t);
+template void launch_cg_init<float, double>(hipStream_t, const Grid &, const double *, const double *, float *,
+                                            float *, float *, Reduction, double, int64_t);
+template void launch_cg_init<float, float>(hipStream_t, const Grid &, const float *, const double *, float *, float *,
+                                           float *, Reduction, double, int64_t);
+template void launch_spmv_dot<double, double>(hipStream_t, const Grid &, const BrickOp<double> &, const double *,
+                                              double *, Reduction);
+template void launch_spmv_dot<float, float>(hipStream_t, const Grid &, const BrickOp<float> &, const float *, float *,
+                                            Reduction);
+template void launch_axpy_norm<double>(hipStream_t, const Grid &, const double *, double *, double *, const double *,
+                                       const double *, Reduction);
+template void launch_axpy_norm<float>(hipStream_t, const Grid &, const double *, float *, float *, const float *,
+                                      const float *, Reduction);
+template void launch_update_dir<double>(hipStream_t, const Grid &, const double *, double *, Reduction);
+template void launch_update_dir<float>(hipStream_t, const Grid &, const float *, float *, Reduction);
 
 }  // namespace shk

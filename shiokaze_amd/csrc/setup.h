@@ -15,10 +15,18 @@ void launch_fractions(hipStream_t st, Dims d, const float *solid, const float *f
 void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[3], float *const rho[3], uint8_t *label,
                   uint8_t *brick_flag, int nbx, int nby, unsigned long long *count);
 
+// diag (0 on non-unknowns), low-face off-diagonal couplings (0 unless both cells
+// are unknowns), rhs, sig = 1/sqrt(diag) (0 on non-unknowns) and optional FP32
+// copies of diag and couplings, all in brick layout.
 void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
                   float *const area[3], float *const rho[3], const uint8_t *label, const float *const vel[3],
                   const float *const svel[3], double *diag, double *cx, double *cy, double *cz, double *rhs,
-                  float *diag32, float *cx32, float *cy32, float *cz32);
+                  double *sig, float *diag32, float *cx32, float *cy32, float *cz32);
+
+// Couplings of D^-1/2 A D^-1/2 (FP64 and/or FP32 outputs; any may be nullptr).
+void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const double *sig, const double *cx,
+                        const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
+                        float *hy32, float *hz32);
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
                     const double *x, float *pressure);

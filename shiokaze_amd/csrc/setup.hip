@@ -96,8 +96,8 @@ __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ b
                         const float *__restrict__ w, const float *__restrict__ us, const float *__restrict__ vs,
                         const float *__restrict__ ws, double *__restrict__ diag, double *__restrict__ cx,
                         double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ rhs,
-                        float *__restrict__ diag32, float *__restrict__ cx32, float *__restrict__ cy32,
-                        float *__restrict__ cz32) {
+                        double *__restrict__ sig, float *__restrict__ diag32, float *__restrict__ cx32,
+                        float *__restrict__ cy32, float *__restrict__ cz32) {
 #pragma clang fp contract(off)
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
@@ -105,59 +105,93 @@ __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ b
     const int i = bc.x * kBrick + (l & 7), j = bc.y * kBrick + ((l >> 3) & 7), k = bc.z * kBrick + (l >> 6);
     const int64_t e = (int64_t)slot * kBrickCells + l;
     double cd = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, b = 0.0;
-    if (i < d.nx && j < d.ny && k < d.nz) {
+    if (i < d.nx && j < d.ny && k < d.nz && label[cell_index(d, i, j, k)]) {
         const double dx2 = dx * dx;
-        // low-face couplings dt*area/(dx^2*rho) (macpressuresolver3.cpp:199), stored
-        // for every cell so a neighbour's high face can be read from it.
+        // off-diagonal -dt*area/(dx^2*rho) (macpressuresolver3.cpp:199-203) of the
+        // cell's low faces, kept only where both cells are unknowns; a neighbour
+        // reads its high faces from here
         const int coord[3] = {i, j, k};
         double low[3];
         for (int a = 0; a < 3; ++a) {
+            low[a] = 0.0;
+            if (coord[a] == 0) continue;
+            if (!label[cell_index(d, i - (a == 0), j - (a == 1), k - (a == 2))]) continue;
             const int64_t f = face_index(d, a, i, j, k);
             const double area = fc.a[a][f], rho = fc.r[a][f];
-            low[a] = (coord[a] > 0 && area != 0.0 && rho != 0.0) ? dt * area / (dx2 * rho) : 0.0;
+            if (area != 0.0 && rho != 0.0) low[a] = dt * area / (dx2 * rho);
         }
         c0 = low[0];
         c1 = low[1];
         c2 = low[2];
-        if (label[cell_index(d, i, j, k)]) {
-            const float *vel[3] = {u, v, w};
-            const float *svel[3] = {us, vs, ws};
-            double diagonal = 0.0;
-            for (int nq = 0; nq < 6; ++nq) {
-                int qi, qj, qk, fi, fj, fk, dim, sgn;
-                neighbour(nq, qi, qj, qk, fi, fj, fk, dim, sgn);
-                qi += i;
-                qj += j;
-                qk += k;
-                const int64_t f = face_index(d, dim, i + fi, j + fj, k + fk);
-                const double area = fc.a[dim][f];
-                if (!(qi < 0 || qi >= d.nx || qj < 0 || qj >= d.ny || qk < 0 || qk >= d.nz)) {
-                    if (area != 0.0) {
-                        const double rho = fc.r[dim][f];
-                        if (rho != 0.0) diagonal += dt * area / (dx2 * rho);
-                    }
-                }
-                b += -sgn * area * (double)vel[dim][f] / dx;
-                if (area < 1.0) {
-                    const double usf = svel[dim][f];
-                    b += -sgn * (1.0 - area) * usf / dx;
+        const float *vel[3] = {u, v, w};
+        const float *svel[3] = {us, vs, ws};
+        double diagonal = 0.0;
+        for (int nq = 0; nq < 6; ++nq) {
+            int qi, qj, qk, fi, fj, fk, dim, sgn;
+            neighbour(nq, qi, qj, qk, fi, fj, fk, dim, sgn);
+            qi += i;
+            qj += j;
+            qk += k;
+            const int64_t f = face_index(d, dim, i + fi, j + fj, k + fk);
+            const double area = fc.a[dim][f];
+            if (!(qi < 0 || qi >= d.nx || qj < 0 || qj >= d.ny || qk < 0 || qk >= d.nz)) {
+                if (area != 0.0) {
+                    const double rho = fc.r[dim][f];
+                    if (rho != 0.0) diagonal += dt * area / (dx2 * rho);
                 }
             }
-            cd = diagonal;
-        } else {
-            b = 0.0;
+            b += -sgn * area * (double)vel[dim][f] / dx;
+            if (area < 1.0) {
+                const double usf = svel[dim][f];
+                b += -sgn * (1.0 - area) * usf / dx;
+            }
         }
+        cd = diagonal;
     }
     diag[e] = cd;
     cx[e] = c0;
     cy[e] = c1;
     cz[e] = c2;
     rhs[e] = b;
+    sig[e] = cd > 0.0 ? 1.0 / sqrt(cd) : 0.0;
     if (diag32) {
         diag32[e] = (float)cd;
         cx32[e] = (float)c0;
         cy32[e] = (float)c1;
         cz32[e] = (float)c2;
+    }
+}
+
+// Couplings of the Jacobi-scaled system D^-1/2 A D^-1/2: c * sig_cell * sig_low.
+__global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restrict__ sig,
+                              const double *__restrict__ cx, const double *__restrict__ cy,
+                              const double *__restrict__ cz, double *__restrict__ hx, double *__restrict__ hy,
+                              double *__restrict__ hz, float *__restrict__ hx32, float *__restrict__ hy32,
+                              float *__restrict__ hz32) {
+    const int slot = blockIdx.x;
+    const int l = threadIdx.x;
+    const int loc[3] = {l & 7, (l >> 3) & 7, l >> 6};
+    const int stride[3] = {1, 8, 64};
+    const int64_t e = (int64_t)slot * kBrickCells + l;
+    const double *c[3] = {cx, cy, cz};
+    double *h[3] = {hx, hy, hz};
+    float *h32[3] = {hx32, hy32, hz32};
+    const double se = sig[e];
+    for (int a = 0; a < 3; ++a) {
+        const double ca = c[a][e];
+        double v = 0.0;
+        if (ca != 0.0) {
+            int64_t m;
+            if (loc[a] > 0) {
+                m = e - stride[a];
+            } else {
+                const int nb = nbr[6 * slot + 2 * a];
+                m = nb >= 0 ? (int64_t)nb * kBrickCells + l + 7 * stride[a] : -1;
+            }
+            if (m >= 0) v = ca * se * sig[m];
+        }
+        if (h[a]) h[a][e] = v;
+        if (h32[a]) h32[a][e] = (float)v;
     }
 }
 
@@ -319,11 +353,20 @@ void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[
 void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
                   float *const area[3], float *const rho[3], const uint8_t *label, const float *const vel[3],
                   const float *const svel[3], double *diag, double *cx, double *cy, double *cz, double *rhs,
-                  float *diag32, float *cx32, float *cy32, float *cz32) {
+                  double *sig, float *diag32, float *cx32, float *cy32, float *cz32) {
     Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
     if (n_active <= 0) return;
     hipLaunchKernelGGL(k_coeff, dim3(n_active), dim3(kBrickCells), 0, st, d, dt, dx, brick_coord, fc, label, vel[0],
-                       vel[1], vel[2], svel[0], svel[1], svel[2], diag, cx, cy, cz, rhs, diag32, cx32, cy32, cz32);
+                       vel[1], vel[2], svel[0], svel[1], svel[2], diag, cx, cy, cz, rhs, sig, diag32, cx32, cy32,
+                       cz32);
+}
+
+void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const double *sig, const double *cx,
+                        const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
+                        float *hy32, float *hz32) {
+    if (n_active <= 0) return;
+    hipLaunchKernelGGL(k_scale_coeff, dim3(n_active), dim3(kBrickCells), 0, st, nbr, sig, cx, cy, cz, hx, hy, hz, hx32,
+                       hy32, hz32);
 }
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,

@@ -16,23 +16,23 @@ constexpr int kXcds = 8;                  // MI355X accelerator complex dies
 
 // Krylov state kept on the device so that iterations are launched back to back
 // without a host round trip; the host polls `done` every few iterations.
+// Fields written by block 0 of one kernel are read only by later kernels; the
+// *_prev copies carry a value across the kernel that overwrites it.
 struct SolveState {
-    double rho;      // r.z of the current residual
-    double alpha;    // step of the current iteration
-    double beta;     // direction update of the current iteration
-    double rnorm;    // max |r| after the last update
-    double rnorm0;   // max |rhs| at the start
-    double tol_abs;  // stop when rnorm <= tol_abs
-    double sigma;    // s.As of the current iteration (diagnostic)
-    double pad0;
-    int64_t iter;    // completed iterations
+    double rho;       // r.z of the current residual
+    double rho_prev;  // rho as the current iteration's axpy_norm read it
+    double alpha;     // step of the current iteration
+    double beta;      // direction update of the current iteration
+    double rnorm;     // max |r| after the last update
+    double rnorm0;    // max |rhs| at the start
+    double tol_abs;   // stop when rnorm <= tol_abs
+    double sigma;     // s.As of the current iteration (diagnostic)
+    int64_t iter;     // completed iterations
+    int64_t iter_prev;
     int64_t max_iter;
-    int32_t done;    // 1 once converged, capped or broken down
-    int32_t status;  // shk_status
-    uint32_t ticket[6];
+    int32_t done;     // 1 once converged, capped or broken down
+    int32_t status;   // shk_status
 };
-
-enum Ticket { kTicketInit = 0, kTicketAlpha = 1, kTicketAxpy = 2, kTicketMisc = 3 };
 
 // Workgroup -> slot mapping that hands each XCD one contiguous range of slots.
 // Workgroups are dispatched to the 8 XCDs round-robin by id, and each XCD has its
@@ -84,27 +84,6 @@ __device__ inline double block_max(double v, double *lds) {
     }
     __syncthreads();
     return r;
-}
-
-// Publishes this block's partial(s) (already stored by thread 0) and returns true
-// in every thread of the block that arrives last.  The release fence before the
-// ticket and the acquire fence after it make every block's partial visible to the
-// last block across XCD L2s.  The last block resets the ticket.
-__device__ inline bool last_block_arrive(uint32_t *ticket) {
-    __shared__ int s_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t t = atomicAdd(ticket, 1u);
-        s_last = (t == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    const bool last = s_last != 0;
-    if (last) {
-        __threadfence();
-        if (threadIdx.x == 0) atomicExch(ticket, 0u);
-    }
-    return last;
 }
 
 // Fixed-order reduction of n partials by one workgroup of NT threads (thread t

@@ -51,7 +51,7 @@ class PCGSolver:
         torch.cuda.synchronize(self.device)
         rc = self.lib.shk_solve_csr(self.ctx, rows, *[ctypes.c_void_p(t.data_ptr()) for t in (rowptr, cols, vals, b, x)],
                                     ctypes.byref(res))
-        _lib.check(rc, self.ctx)
+        _lib.check(rc, self.ctx, _lib.SOLVER_OUTCOMES)
         out = res.as_dict()
         out["status_name"] = _lib.STATUS_NAMES.get(res.status, str(res.status))
         return x, out

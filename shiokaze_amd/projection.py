@@ -99,7 +99,7 @@ class MacPressureSolver3:
         torch.cuda.synchronize(self.device)
         res = _lib.ShkResult()
         rc = self.lib.shk_project(self.ctx, float(dt), *ptrs, ctypes.byref(res))
-        _lib.check(rc, self.ctx)
+        _lib.check(rc, self.ctx, _lib.SOLVER_OUTCOMES)
         out = res.as_dict()
         out["status_name"] = _lib.STATUS_NAMES.get(res.status, str(res.status))
         return out

@@ -14,6 +14,42 @@ def log(msg: str) -> None:
     sys.stdout.flush()
 
 
+def iteration_band(oracle_iters: int) -> int:
+    """Allowed |GPU - oracle| CG iteration difference.
+
+    The max-norm stop point of CG moves with the rounding of its dot products:
+    on the CPU, the oracle's sequential dot stops the 24^3 pool+sphere scene at
+    111 iterations while the same CG with pairwise sums stops at 108 for every
+    random renumbering of the unknowns, and at 64^3 the renumberings give 414-433
+    against the oracle's 432 (test_oracle_solver.test_iteration_spread_under_renumbering
+    reproduces this).  The GPU's tree reductions are one more summation order."""
+    return max(3, int(0.05 * oracle_iters + 0.5))
+
+
+def iteration_spread(O, rowptr, cols, vals, b, tol, mode, n_perm: int = 6, seed: int = 0):
+    """(min, max) CG iteration count of the oracle on the system and on `n_perm`
+    random symmetric renumberings of it.  In exact arithmetic every renumbering
+    takes the same iterations; the spread is what rounding order alone does."""
+    import scipy.sparse as sp
+    n = len(b)
+    A = sp.csr_matrix((vals, cols, rowptr), shape=(n, n))
+    counts = [O.pcg(rowptr, cols, vals, b, tol, 100000, mode)[1]]
+    rng = np.random.default_rng(seed)
+    for _ in range(n_perm):
+        perm = rng.permutation(n)
+        Ap = A[perm][:, perm].tocsr()
+        Ap.sort_indices()
+        counts.append(O.pcg(Ap.indptr.astype(np.int64), Ap.indices.astype(np.int64), Ap.data, b[perm], tol, 100000,
+                            mode)[1])
+    return min(counts), max(counts)
+
+
+def within_spread(count: int, spread) -> bool:
+    lo, hi = spread
+    m = max(3, int(0.02 * hi + 0.5))
+    return lo - m <= count <= hi + m
+
+
 def to_device(scene: dict):
     import torch
     return {k: torch.from_numpy(np.ascontiguousarray(scene[k])).cuda() for k in FIELDS}
