@@ -9,7 +9,9 @@ to 1e-6.
 
 Prints one JSON line (rank 0).  `roofline` is for the dominant Krylov kernel:
 algorithmic bytes per launch / its average launch duration, timed with HIP events
-on the library's stream during the timed steps (ProfileKernels=1).
+on the library's stream (ProfileKernels=1) over --profile-steps further steps of
+the same context right after the timed ones (the events add a little time per
+launch, so the timed steps themselves run without them).
 `cpu_baseline` times the oracle (the FP64 CPU restatement of the reference's
 CG, with its discarded MIC(0) work) on a bounded sample of the same workload.
 """
@@ -37,16 +39,15 @@ def log(msg):
 
 def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
     """Algorithmic HBM bytes per brick cell of one launch of each Krylov kernel:
-    every array element the kernel needs is read or written once."""
+    every array element the kernel needs is read or written once.  With Jacobi the
+    system is the symmetrically scaled one (unit diagonal, no diagonal array)."""
     t = 8 if precision == "fp64" else 4      # vector element
     c = 8 if precision == "fp64" else 4      # coefficient element
     jac = precond == "jacobi"
-    if kernel == "spmv_dot":                  # s, diag, cx, cy, cz in; q out
-        return 2 * t + 4 * c
-    if kernel == "axpy_norm":                 # x, r, s, q, diag in; x, r out
-        return 6 * t + c
-    if kernel == "update_dir":                # r, s, (diag) in; s out
-        return 3 * t + (c if jac else 0)
+    if kernel == "spmv_dot":                  # r, s in; cx, cy, cz (+ diag) in; s', q out
+        return 4 * t + 3 * c + (0 if jac else c)
+    if kernel == "axpy_norm":                 # x, r, s, q in; x, r out (+ 1/sqrt(diag) for the FP64 stop test)
+        return 6 * t + (8 if (jac and precision == "fp64") else 0)
     raise ValueError(kernel)
 
 
@@ -65,6 +66,12 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--extra", action="store_true", help="also time FP64 Jacobi and plain CG solves")
+    ap.add_argument("--profile-steps", type=int, default=2,
+                    help="steps after the timed ones with per-kernel HIP event timing")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB first (a known-size kernel for HBM counter scaling)")
+    ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
+                    help="extra shk_configure keys for the timed context")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -76,6 +83,14 @@ def main():
 
     import shiokaze_amd
     from shiokaze_amd import scene as S
+
+    if args.calibrate_copy > 0:
+        src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        del src, dst
 
     n = args.n
     t0 = time.time()
@@ -96,13 +111,14 @@ def main():
         return ps.project(1.0, vel, dev["solid"], svel, dev["fluid"])
 
     cfg = {"Residual": args.residual, "Precision": args.precision, "Preconditioner": args.preconditioner}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        cfg[k] = v
     ps = make(cfg)
     for _ in range(args.warmup):
         r = step(ps)
         log(f"warmup: iters={r['count']} outer={r['outer']} status={r['status_name']} "
             f"setup={r['ms_setup']:.2f}ms solve={r['ms_solve']:.2f}ms update={r['ms_update']:.2f}ms")
-    if not args.no_profile:
-        ps.configure("ProfileKernels", 1)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     results = []
@@ -110,6 +126,13 @@ def main():
         results.append(step(ps))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
+    if not args.no_profile and args.profile_steps > 0:
+        # per-kernel durations: HIP events recorded on the library's stream around
+        # every Krylov launch of these extra steps (same context, same scene)
+        ps.configure("ProfileKernels", 1)
+        for _ in range(args.profile_steps):
+            step(ps)
+        torch.cuda.synchronize()
     ms_per_step = 1e3 * elapsed / args.steps
     last = results[-1]
     value = args.steps / elapsed
@@ -120,7 +143,7 @@ def main():
     kernels = {}
     roofline = None
     if not args.no_profile:
-        for k in ("spmv_dot", "axpy_norm", "update_dir"):
+        for k in ("spmv_dot", "axpy_norm"):
             cnt, ms = ps.kernel_stats(k)
             if cnt == 0:
                 continue

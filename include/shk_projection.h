@@ -101,7 +101,7 @@ void shk_destroy(shk_ctx *ctx);
  *   Precision      fp64 | mixed            (fp64)
  *   Preconditioner none | jacobi | unet    (jacobi)
  *   OuterMax       refinement steps for mixed (3)
- *   InnerResidual  relative stop of each mixed inner solve (1e-5, floor)
+ *   InnerResidual  relative stop of each mixed inner solve (1e-6, floor)
  *   CheckInterval  iterations launched between host convergence polls (32)
  *   HostPointers   0 | 1                   (0)
  *   UnetWeights    path of a trained preconditioner blob (for Preconditioner=unet)
@@ -163,7 +163,8 @@ int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64
                   const double *vals, const double *b, double *x, shk_result *result);
 
 /* Accumulated launch count and total milliseconds of one Krylov kernel
- * ("spmv_dot", "axpy_norm" or "update_dir") since ProfileKernels was last set.
+ * ("spmv_dot": direction update fused with the stencil and s.As, or "axpy_norm")
+ * since ProfileKernels was last set.
  * Only launches that did work are counted. */
 int shk_kernel_stats(shk_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 

@@ -112,16 +112,20 @@ int build_bricks(shk_ctx *c) {
     HIPC(c, hipMemcpyAsync(c->nbr.p, c->h_nbr.data(), sizeof(int) * 6 * c->n_active, hipMemcpyHostToDevice,
                            c->stream));
     for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->x64, &c->r64,
-                              &c->s64, &c->q64})
+                              &c->s64, &c->t64, &c->q64})
         HIPC(c, b->ensure(n));
-    if (c->prm.precond != 0 && c->prm.precision == 0)
+    if (c->prm.precond != 0 && c->prm.precision == 0) {
         for (DevBuf<double> *b : {&c->hx64, &c->hy64, &c->hz64}) HIPC(c, b->ensure(n));
+        HIPC(c, c->kappa64.ensure(c->n_active));
+    }
     if (c->prm.precision == 1 || c->prm.precond == 2) {
-        for (DevBuf<float> *b : {&c->x32, &c->r32, &c->s32, &c->q32, &c->rs32}) HIPC(c, b->ensure(n));
+        for (DevBuf<float> *b : {&c->x32, &c->r32, &c->s32, &c->t32, &c->q32, &c->rs32}) HIPC(c, b->ensure(n));
         if (c->prm.precond == 0)
             for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32}) HIPC(c, b->ensure(n));
-        else
+        else {
             for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32}) HIPC(c, b->ensure(n));
+            HIPC(c, c->kappa32.ensure(c->n_active));
+        }
     }
     c->grid = make_grid((int64_t)c->n_active * kBrickCells);
     // host-side buffers must outlive the async copies above
@@ -178,7 +182,7 @@ BrickOp<double> op64(shk_ctx *c) {
 }
 BrickOp<float> op32(shk_ctx *c) {
     if (c->prm.precond == 0) return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active};
-    return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active};
+    return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
 }
 
 int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4], const double *sig, float *rs) {
@@ -203,17 +207,20 @@ int solve_fp64(shk_ctx *c, shk_result *res) {
     const Grid &g = c->grid;
     const double *sig = jac ? c->sig64.p : nullptr;
     const BrickOp<double> op =
-        jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active} : op64(c);
-    launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, reduction(c),
-                                   c->prm.residual, c->prm.max_iterations);
+        jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active, c->kappa64.p} : op64(c);
     const Reduction red = reduction(c);
+    launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, red, c->prm.residual,
+                                   c->prm.max_iterations);
+    double *S[2] = {c->s64.p, c->t64.p};
+    int m = 0;
+    launch_dir_spmv<double, double>(c->stream, g, op, c->r64.p, S[0], S[1], c->q64.p, red);
+    ++m;
     int rc = drive(c, [&]() {
-        launch_spmv_dot<double, double>(c->stream, g, op, c->s64.p, c->q64.p, red);
+        launch_axpy_norm<double>(c->stream, g, sig, c->x64.p, c->r64.p, S[m & 1], c->q64.p, red);
         c->mark();
-        launch_axpy_norm<double>(c->stream, g, sig, c->x64.p, c->r64.p, c->s64.p, c->q64.p, red);
+        launch_dir_spmv<double, double>(c->stream, g, op, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red);
         c->mark();
-        launch_update_dir<double>(c->stream, g, c->r64.p, c->s64.p, red);
-        c->mark();
+        ++m;
     });
     if (rc) return rc;
     if (jac) launch_unscale(c->stream, g.n, c->x64.p, c->sig64.p);
@@ -255,13 +262,16 @@ int solve_mixed(shk_ctx *c, shk_result *res) {
         if (inner_rel > 0.5) inner_rel = 0.5;
         launch_cg_init<float, float>(c->stream, g, c->rs32.p, nullptr, c->x32.p, c->r32.p, c->s32.p, red, inner_rel,
                                      c->prm.max_iterations - total);
+        float *S[2] = {c->s32.p, c->t32.p};
+        int m = 0;
+        launch_dir_spmv<float, float>(c->stream, g, op, c->r32.p, S[0], S[1], c->q32.p, red);
+        ++m;
         rc = drive(c, [&]() {
-            launch_spmv_dot<float, float>(c->stream, g, op, c->s32.p, c->q32.p, red);
+            launch_axpy_norm<float>(c->stream, g, nullptr, c->x32.p, c->r32.p, S[m & 1], c->q32.p, red);
             c->mark();
-            launch_axpy_norm<float>(c->stream, g, nullptr, c->x32.p, c->r32.p, c->s32.p, c->q32.p, red);
+            launch_dir_spmv<float, float>(c->stream, g, op, c->r32.p, S[m & 1], S[(m + 1) & 1], c->q32.p, red);
             c->mark();
-            launch_update_dir<float>(c->stream, g, c->r32.p, c->s32.p, red);
-            c->mark();
+            ++m;
         });
         if (rc) return rc;
         total += c->h_state->iter;
@@ -386,10 +396,11 @@ void shk_destroy(shk_ctx *c) {
     c->coord.release();
     c->nbr.release();
     for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->hx64, &c->hy64,
-                              &c->hz64, &c->x64, &c->r64, &c->s64, &c->q64, &c->pa, &c->pb, &c->pc, &c->misc})
+                              &c->hz64, &c->kappa64, &c->x64, &c->r64, &c->s64, &c->t64, &c->q64, &c->pa, &c->pb,
+                              &c->pc, &c->misc})
         b->release();
-    for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
-                             &c->s32, &c->q32, &c->rs32})
+    for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->kappa32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
+                             &c->s32, &c->t32, &c->q32, &c->rs32})
         b->release();
     c->state.release();
     if (c->h_state) (void)hipHostFree(c->h_state);
@@ -561,6 +572,10 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         launch_scale_coeff(c->stream, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p,
                            s64 ? c->hx64.p : nullptr, s64 ? c->hy64.p : nullptr, s64 ? c->hz64.p : nullptr,
                            s32 ? c->hx32.p : nullptr, s32 ? c->hy32.p : nullptr, s32 ? c->hz32.p : nullptr);
+        if (s64) launch_brick_uniform<double>(c->stream, c->n_active, c->nbr.p, c->hx64.p, c->hy64.p, c->hz64.p,
+                                              c->kappa64.p);
+        if (s32) launch_brick_uniform<float>(c->stream, c->n_active, c->nbr.p, c->hx32.p, c->hy32.p, c->hz32.p,
+                                             c->kappa32.p);
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ev[1], c->stream));
@@ -613,12 +628,12 @@ int shk_copy_pressure(shk_ctx *c, float *dst) {
 int shk_kernel_stats(shk_ctx *c, const char *kernel, int64_t *launches, double *total_ms) {
     if (!c || !kernel || !launches || !total_ms) return SHK_BAD_ARG;
     const std::string k(kernel);
-    static const char *names[shk_ctx::kKernels] = {"spmv_dot", "axpy_norm", "update_dir"};
+    static const char *names[shk_ctx::kKernels] = {"axpy_norm", "spmv_dot"};
     int idx = -1;
     for (int q = 0; q < shk_ctx::kKernels; ++q)
         if (k == names[q]) idx = q;
     if (idx < 0)
-        return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (spmv_dot, axpy_norm, update_dir)", kernel);
+        return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (axpy_norm, spmv_dot)", kernel);
     *launches = c->pcount[idx];
     *total_ms = c->pms[idx];
     return SHK_OK;
@@ -699,6 +714,7 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     HIPC(c, c->sig64.ensure(rows));
     HIPC(c, c->r64.ensure(rows));
     HIPC(c, c->s64.ensure(rows));
+    HIPC(c, c->t64.ensure(rows));
     HIPC(c, c->q64.ensure(rows));
     const CsrOp op{rowptr, cols, vals, rows};
     const bool jac = c->prm.precond != 0;
@@ -721,13 +737,16 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     const Reduction red = reduction(c);
     launch_cg_init<double, double>(c->stream, g, b, sig, x, c->r64.p, c->s64.p, red, c->prm.residual,
                                    c->prm.max_iterations);
+    double *S[2] = {c->s64.p, c->t64.p};
+    int m = 0;
+    launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[0], S[1], c->q64.p, red);
+    ++m;
     rc = drive(c, [&]() {
-        launch_spmv_dot_csr(c->stream, g, sop, c->s64.p, c->q64.p, red);
+        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, S[m & 1], c->q64.p, red);
         c->mark();
-        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, c->s64.p, c->q64.p, red);
+        launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red);
         c->mark();
-        launch_update_dir<double>(c->stream, g, c->r64.p, c->s64.p, red);
-        c->mark();
+        ++m;
     });
     if (rc) return rc;
     if (jac) launch_unscale(c->stream, rows, x, c->sig64.p);

@@ -57,7 +57,7 @@ struct Params {
     int precision = 0;                // 0 fp64, 1 mixed
     int precond = 1;                  // 0 none, 1 jacobi, 2 unet
     int outer_max = 3;
-    double inner_residual = 1e-5;
+    double inner_residual = 1e-6;
     int check_interval = 32;
     bool host_pointers = false;
     std::string unet_weights;
@@ -102,13 +102,13 @@ struct shk_ctx {
     // unscaled FP64 operator (diag, couplings zero off unknown pairs), rhs, sig = 1/sqrt(diag)
     DevBuf<double> diag64, cx64, cy64, cz64, b64, sig64;
     // couplings of the Jacobi-scaled operator, FP64 and FP32
-    DevBuf<double> hx64, hy64, hz64;
-    DevBuf<float> hx32, hy32, hz32;
+    DevBuf<double> hx64, hy64, hz64, kappa64;
+    DevBuf<float> hx32, hy32, hz32, kappa32;
     // unscaled FP32 operator (mixed precision without preconditioner)
     DevBuf<float> diag32, cx32, cy32, cz32;
     // Krylov vectors
-    DevBuf<double> x64, r64, s64, q64;
-    DevBuf<float> x32, r32, s32, q32, rs32;
+    DevBuf<double> x64, r64, s64, t64, q64;
+    DevBuf<float> x32, r32, s32, t32, q32, rs32;
     DevBuf<double> pa, pb, pc, misc;
     shk::Grid grid;
     DevBuf<SolveState> state;
@@ -121,9 +121,9 @@ struct shk_ctx {
     bool profile = false;
     std::vector<hipEvent_t> pev;
     size_t pcur = 0;
-    static constexpr int kKernels = 3;  // launches per Krylov iteration
-    int64_t pcount[kKernels] = {0, 0, 0};
-    double pms[kKernels] = {0.0, 0.0, 0.0};
+    static constexpr int kKernels = 2;  // launches per Krylov iteration: axpy_norm, dir_spmv
+    int64_t pcount[kKernels] = {0, 0};
+    double pms[kKernels] = {0.0, 0.0};
     // records the next profiling event after a Krylov kernel launch
     void mark() {
         if (profile && pcur < pev.size()) (void)hipEventRecord(pev[pcur++], stream);

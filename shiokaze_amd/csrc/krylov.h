@@ -36,6 +36,8 @@ struct BrickOp {
     const int *nbr;
     const C *diag, *cx, *cy, *cz;
     int n_active;
+    // optional, unit-diagonal operators only: per-brick constant coupling (0 = not uniform)
+    const C *kappa = nullptr;
 };
 struct CsrOp {
     const int64_t *rowptr, *cols;
@@ -43,7 +45,7 @@ struct CsrOp {
     int64_t rows;
 };
 
-constexpr int kMaxBlocks = 1024;  // workgroups of a Krylov kernel: 4 per CU
+constexpr int kMaxBlocks = 1024;  // workgroups of a Krylov kernel: 4 of 4 waves per CU
 
 __host__ __device__ inline int blocks_for(int64_t n) { return (int)((n + kBrickCells - 1) / kBrickCells); }
 
@@ -62,16 +64,19 @@ template <class T, class TB>
 void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *sig, T *x, T *r, T *s, Reduction red,
                     double tol_rel, int64_t max_iter);
 
-// One CG iteration: spmv_dot, axpy_norm, update_dir (see krylov.hip).  With
-// sig != nullptr the stop test uses the unscaled residual max|r/sig|.
+// One CG iteration is two kernels (see krylov.hip): dir_spmv (stop test, beta,
+// s_out = r + beta s_in, q = A s_out, partial s_out.q) and axpy_norm (alpha,
+// x += alpha s, r -= alpha q, partials r.r and max|r|).  The first dir_spmv after
+// init takes beta = 0.  s alternates between two buffers.  With sig != nullptr the
+// stop test uses the unscaled residual max|r/sig|.
 template <class T, class C>
-void launch_spmv_dot(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *s, T *q, Reduction red);
-void launch_spmv_dot_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *s, double *q, Reduction red);
+void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
+                     Reduction red);
+void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
+                         double *s_out, double *q, Reduction red);
 template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
                       Reduction red);
-template <class T>
-void launch_update_dir(hipStream_t st, const Grid &g, const T *r, T *s, Reduction red);
 
 // r = b - A x in FP64 (unscaled operator with its diagonal); part holds 4*blocks
 // partials; out = {max|r|, sum r^2, sum b^2, max|b|}.  Optionally rs = (float)

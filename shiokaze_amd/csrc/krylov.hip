@@ -7,17 +7,21 @@
 // every face that touches a non-unknown, so no masks are needed either.  In both
 // the plain and the scaled form z = r.
 //
-// Three kernels per iteration, no host round trip and no grid-wide fences:
-//   k_spmv_dot    q = A s over the block's chunk of tiles, partial s.q
-//   k_axpy_norm   every block folds the <= kMaxBlocks s.q partials in the same
-//                 fixed order (bit-identical alpha everywhere), x += alpha s,
-//                 r -= alpha q, partials r.r and max|r|
-//   k_update_dir  every block folds those partials: stop test, beta,
-//                 s = r + beta s
+// Two kernels per iteration, no host round trip and no grid-wide fences:
+//   k_dir_spmv   every block folds the previous axpy's partials (r.r, max|r|) in
+//                one fixed order: stop test and beta, bit-identical in every
+//                block; then s' = r + beta s and q = A s' in one pass, partial s'.q
+//   k_axpy_norm  every block folds the s'.q partials: alpha; x += alpha s',
+//                r -= alpha q, partials r.r and max|r|
+// s is double-buffered (the stencil reads neighbours' old s while s' is written).
 // A tile is 512 elements: one 8x8x8 brick, or 512 CSR rows.  Block g of a grid of
-// G processes tiles [g*per, (g+1)*per) after the XCD remap, so each XCD streams
-// one contiguous range.  State written by block 0 of a kernel is only read by
-// later kernels (or double-buffered through *_prev fields).
+// G <= kMaxBlocks processes tiles [g*per, (g+1)*per) after the XCD remap, so each
+// XCD streams one contiguous range.
+//
+// The brick stencil runs one wavefront per brick with one z-column of 8 cells per
+// lane (lane = i + 8 j): x and y neighbours come from lane shuffles, z neighbours
+// from the lane's own registers, and only the brick's edge lanes read the six
+// neighbour bricks.
 #include "krylov.h"
 
 #include "shk_projection.h"
@@ -26,7 +30,9 @@ namespace shk {
 
 namespace {
 
-constexpr int NT = kBrickCells;
+constexpr int NB = 256;               // threads of the Krylov kernels (4 waves)
+constexpr int NT = kBrickCells;       // threads of the residual kernel (one per brick cell)
+constexpr int NF = 1024;              // threads of the one-workgroup finishing kernels
 
 __device__ inline void chunk_range(int tiles, int per, int &t0, int &t1) {
     const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
@@ -34,17 +40,19 @@ __device__ inline void chunk_range(int tiles, int per, int &t0, int &t1) {
     t1 = t0 + per < tiles ? t0 + per : tiles;
 }
 
-// Deterministic fold of n partials, result broadcast to every thread.
+// Deterministic fold of n partials, the result broadcast to every thread.
+template <int NTH>
 __device__ inline double fold_sum(const double *p, int n, double *lds, double *bcast) {
-    const double v = reduce_partials_sum<NT>(p, n, lds);
+    const double v = reduce_partials_sum<NTH>(p, n, lds);
     if (threadIdx.x == 0) *bcast = v;
     __syncthreads();
     const double r = *bcast;
     __syncthreads();
     return r;
 }
+template <int NTH>
 __device__ inline double fold_max(const double *p, int n, double *lds, double *bcast) {
-    const double v = reduce_partials_max<NT>(p, n, lds);
+    const double v = reduce_partials_max<NTH>(p, n, lds);
     if (threadIdx.x == 0) *bcast = v;
     __syncthreads();
     const double r = *bcast;
@@ -55,16 +63,15 @@ __device__ inline double fold_max(const double *p, int n, double *lds, double *b
 // ---------------------------------------------------------------- init
 
 template <class T, class TB, bool SCALED>
-__global__ __launch_bounds__(NT) void k_cg_init(int64_t n, int tiles, int per, const TB *__restrict__ rhs,
+__global__ __launch_bounds__(NB) void k_cg_init(int64_t n, int tiles, int per, const TB *__restrict__ rhs,
                                                 const double *__restrict__ sig, T *__restrict__ x, T *__restrict__ r,
                                                 T *__restrict__ s, Reduction red) {
-    __shared__ double lds[NT / kWave];
+    __shared__ double lds[NB / kWave];
     int t0, t1;
     chunk_range(tiles, per, t0, t1);
     double rr = 0.0, rm = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const int64_t e = (int64_t)t * NT + threadIdx.x;
-        if (e >= n) continue;
+    const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
+    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
         const double b = (double)rhs[e];
         const T rv = SCALED ? (T)(sig[e] * b) : (T)b;
         x[e] = T(0);
@@ -74,15 +81,13 @@ __global__ __launch_bounds__(NT) void k_cg_init(int64_t n, int tiles, int per, c
         // with a scaled system the stop test is on the unscaled residual: max|b|
         rm = fmax(rm, SCALED ? (sig[e] > 0.0 ? fabs(b) : 0.0) : fabs((double)rv));
     }
-    const double bs = block_sum<NT>(rr, lds);
-    const double bm = block_max<NT>(rm, lds);
+    const double bs = block_sum<NB>(rr, lds);
+    const double bm = block_max<NB>(rm, lds);
     if (threadIdx.x == 0) {
         red.pa[blockIdx.x] = bs;
         red.pb[blockIdx.x] = bm;
     }
 }
-
-constexpr int NF = 1024;
 
 __global__ __launch_bounds__(NF) void k_finish_init(int nblocks, Reduction red, double tol_rel, int64_t max_iter) {
     __shared__ double lds[NF / kWave];
@@ -91,8 +96,8 @@ __global__ __launch_bounds__(NF) void k_finish_init(int nblocks, Reduction red, 
     if (threadIdx.x == 0) {
         SolveState *st = red.st;
         const double tol = tol_rel < 1e-30 ? 1e-30 : tol_rel;  // pcg_solver.h:235
-        st->rho = rho;
-        st->rho_prev = rho;
+        st->rho_k1 = rho;
+        st->rho_k2 = rho;
         st->alpha = 0.0;
         st->beta = 0.0;
         st->sigma = 0.0;
@@ -100,7 +105,6 @@ __global__ __launch_bounds__(NF) void k_finish_init(int nblocks, Reduction red, 
         st->rnorm0 = bmax;
         st->tol_abs = tol * bmax;
         st->iter = 0;
-        st->iter_prev = 0;
         st->max_iter = max_iter;
         st->done = 0;
         st->status = SHK_OK;
@@ -120,14 +124,317 @@ __global__ __launch_bounds__(NF) void k_finish_init(int nblocks, Reduction red, 
     }
 }
 
-// ---------------------------------------------------------------- stencil
+// ---------------------------------------------------------------- direction + stencil
+
+// Folds the previous axpy's partials: the stop test of pcg_solver.h:281-284 and
+// beta of :287-289.  Returns false when the solve has stopped.  With no
+// iteration done yet beta = 0 (s = r, pcg_solver.h:272).
+__device__ inline bool direction_step(const Reduction &red, double *lds, double *bc, double &beta) {
+    SolveState *st = red.st;
+    const int64_t it = st->iter;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (it == 0) {
+        beta = 0.0;
+        if (lead) st->rho_k1 = st->rho_k2;
+        return true;
+    }
+    const double rho_new = fold_sum<NB>(red.pb, gridDim.x, lds, bc);
+    const double rn = fold_max<NB>(red.pb + gridDim.x, gridDim.x, lds, bc);
+    const bool stop = rn <= st->tol_abs;
+    const bool nan = rho_new != rho_new;
+    const bool cap = it >= st->max_iter;
+    beta = rho_new / st->rho_k2;
+    if (lead) {
+        st->rnorm = rn;
+        if (stop) {
+            st->done = 1;
+            st->status = SHK_OK;
+        } else if (nan) {
+            st->done = 1;
+            st->status = SHK_NAN;
+        } else if (cap) {
+            st->done = 1;
+            st->status = SHK_NOT_CONVERGED;
+        } else {
+            st->beta = beta;
+            st->rho_k1 = rho_new;
+        }
+    }
+    return !(stop || nan || cap);
+}
+
+template <class T>
+__device__ inline T lane_shfl(T v, int src) {
+    return __shfl(v, src & (kWave - 1), kWave);
+}
 
 template <class T, class C, bool UNIT>
+__global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T *__restrict__ r,
+                                                 const T *__restrict__ sin, T *__restrict__ sout,
+                                                 T *__restrict__ q, Reduction red) {
+    __shared__ double lds[NB / kWave];
+    __shared__ double bc;
+    if (red.st->done) return;
+    double beta;
+    if (!direction_step(red, lds, &bc, beta)) return;
+    const T b = (T)beta;
+    int t0, t1;
+    chunk_range(op.n_active, per, t0, t1);
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    const int i = lane & 7, j = lane >> 3;
+    double acc = 0.0;
+    for (int slot = t0 + wave; slot < t1; slot += NB / kWave) {
+        const int *nb6 = op.nbr + 6 * slot;
+        const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
+        const int64_t base = (int64_t)slot * NT + lane;
+        const C kap = (UNIT && op.kappa) ? op.kappa[slot] : C(0);
+        if (UNIT && kap != C(0)) {
+            // uniform brick: one coupling value, no coefficient traffic
+            const T kv = (T)kap;
+            T sv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t e = base + 64 * k;
+                sv[k] = r[e] + b * sin[e];
+            }
+            T szlo = T(0), szhi = T(0);
+            if (nzm >= 0) {
+                const int64_t e = (int64_t)nzm * NT + lane + 448;
+                szlo = r[e] + b * sin[e];
+            }
+            if (nzp >= 0) {
+                const int64_t e = (int64_t)nzp * NT + lane;
+                szhi = r[e] + b * sin[e];
+            }
+            T hxlo[8], hxhi[8], hylo[8], hyhi[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) hxlo[k] = hxhi[k] = hylo[k] = hyhi[k] = T(0);
+            if (i == 0 && nxm >= 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t e = (int64_t)nxm * NT + lane + 7 + 64 * k;
+                    hxlo[k] = r[e] + b * sin[e];
+                }
+            }
+            if (i == 7 && nxp >= 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t e = (int64_t)nxp * NT + lane - 7 + 64 * k;
+                    hxhi[k] = r[e] + b * sin[e];
+                }
+            }
+            if (j == 0 && nym >= 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t e = (int64_t)nym * NT + lane + 56 + 64 * k;
+                    hylo[k] = r[e] + b * sin[e];
+                }
+            }
+            if (j == 7 && nyp >= 0) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int64_t e = (int64_t)nyp * NT + lane - 56 + 64 * k;
+                    hyhi[k] = r[e] + b * sin[e];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const T sk = sv[k];
+                T xm = lane_shfl(sk, lane - 1);
+                T xp = lane_shfl(sk, lane + 1);
+                T ym = lane_shfl(sk, lane - 8);
+                T yp = lane_shfl(sk, lane + 8);
+                if (i == 0) xm = hxlo[k];
+                if (i == 7) xp = hxhi[k];
+                if (j == 0) ym = hylo[k];
+                if (j == 7) yp = hyhi[k];
+                const T zm = k > 0 ? sv[k - 1] : szlo;
+                const T zp = k < 7 ? sv[k + 1] : szhi;
+                // same association as the general path: c*a + c*b + ... summed left to right
+                const T qv = sk - (kv * xm + kv * xp + kv * ym + kv * yp + kv * zm + kv * zp);
+                const int64_t e = base + 64 * k;
+                sout[e] = sk;
+                q[e] = qv;
+                acc += (double)sk * (double)qv;
+            }
+            continue;
+        }
+        T sv[8], cxv[8], cyv[8], czv[8], dv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t e = base + 64 * k;
+            sv[k] = r[e] + b * sin[e];
+            cxv[k] = (T)op.cx[e];
+            cyv[k] = (T)op.cy[e];
+            czv[k] = (T)op.cz[e];
+            if (!UNIT) dv[k] = (T)op.diag[e];
+        }
+        // halos: s' of the neighbour bricks' facing planes, and their couplings
+        T szlo = T(0), szhi = T(0), czhi = T(0);
+        if (nzm >= 0) {
+            const int64_t e = (int64_t)nzm * NT + lane + 448;
+            szlo = r[e] + b * sin[e];
+        }
+        if (nzp >= 0) {
+            const int64_t e = (int64_t)nzp * NT + lane;
+            szhi = r[e] + b * sin[e];
+            czhi = (T)op.cz[e];
+        }
+        T hxlo[8], hxhi[8], cxhi[8], hylo[8], hyhi[8], cyhi[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) hxlo[k] = hxhi[k] = cxhi[k] = hylo[k] = hyhi[k] = cyhi[k] = T(0);
+        if (i == 0 && nxm >= 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t e = (int64_t)nxm * NT + lane + 7 + 64 * k;
+                hxlo[k] = r[e] + b * sin[e];
+            }
+        }
+        if (i == 7 && nxp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t e = (int64_t)nxp * NT + lane - 7 + 64 * k;
+                hxhi[k] = r[e] + b * sin[e];
+                cxhi[k] = (T)op.cx[e];
+            }
+        }
+        if (j == 0 && nym >= 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t e = (int64_t)nym * NT + lane + 56 + 64 * k;
+                hylo[k] = r[e] + b * sin[e];
+            }
+        }
+        if (j == 7 && nyp >= 0) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t e = (int64_t)nyp * NT + lane - 56 + 64 * k;
+                hyhi[k] = r[e] + b * sin[e];
+                cyhi[k] = (T)op.cy[e];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const T sk = sv[k];
+            T xm = lane_shfl(sk, lane - 1);
+            T xp = lane_shfl(sk, lane + 1);
+            T ym = lane_shfl(sk, lane - 8);
+            T yp = lane_shfl(sk, lane + 8);
+            T cxp = lane_shfl(cxv[k], lane + 1);
+            T cyp = lane_shfl(cyv[k], lane + 8);
+            if (i == 0) xm = hxlo[k];
+            if (i == 7) {
+                xp = hxhi[k];
+                cxp = cxhi[k];
+            }
+            if (j == 0) ym = hylo[k];
+            if (j == 7) {
+                yp = hyhi[k];
+                cyp = cyhi[k];
+            }
+            const T zm = k > 0 ? sv[k - 1] : szlo;
+            const T zp = k < 7 ? sv[k + 1] : szhi;
+            const T czp = k < 7 ? czv[k + 1] : czhi;
+            T qv = UNIT ? sk : dv[k] * sk;
+            qv -= cxv[k] * xm + cxp * xp + cyv[k] * ym + cyp * yp + czv[k] * zm + czp * zp;
+            const int64_t e = base + 64 * k;
+            sout[e] = sk;
+            q[e] = qv;
+            acc += (double)sk * (double)qv;
+        }
+    }
+    const double bs = block_sum<NB>(acc, lds);
+    if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
+}
+
+__global__ __launch_bounds__(NB) void k_dir_spmv_csr(CsrOp op, int per, const double *__restrict__ r,
+                                                     const double *__restrict__ sin, double *__restrict__ sout,
+                                                     double *__restrict__ q, Reduction red) {
+    __shared__ double lds[NB / kWave];
+    __shared__ double bc;
+    if (red.st->done) return;
+    double beta;
+    if (!direction_step(red, lds, &bc, beta)) return;
+    int t0, t1;
+    chunk_range(blocks_for(op.rows), per, t0, t1);
+    const int64_t end = (int64_t)t1 * NT < op.rows ? (int64_t)t1 * NT : op.rows;
+    double acc = 0.0;
+    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
+        double a = 0.0;
+        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) {
+            const int64_t c = op.cols[p];
+            a += op.vals[p] * (r[c] + beta * sin[c]);
+        }
+        const double se = r[e] + beta * sin[e];
+        sout[e] = se;
+        q[e] = a;
+        acc += se * a;
+    }
+    const double bs = block_sum<NB>(acc, lds);
+    if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
+}
+
+// ---------------------------------------------------------------- axpy + norm
+
+template <class T, bool SCALED>
+__global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per, const double *__restrict__ sig,
+                                                  T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
+                                                  const T *__restrict__ q, Reduction red) {
+    __shared__ double lds[NB / kWave];
+    __shared__ double bc;
+    SolveState *st = red.st;
+    if (st->done) return;
+    const double sigma = fold_sum<NB>(red.pa, gridDim.x, lds, &bc);
+    const double rho = st->rho_k1;
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (!(sigma > 0.0)) {
+        if (lead) {
+            st->sigma = sigma;
+            st->done = 1;
+            st->status = (sigma != sigma) ? SHK_NAN : SHK_BREAKDOWN;
+        }
+        return;
+    }
+    const double alpha = rho / sigma;
+    if (lead) {
+        st->sigma = sigma;
+        st->alpha = alpha;
+        st->rho_k2 = rho;
+        st->iter = st->iter + 1;
+    }
+    const T a = (T)alpha;
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
+    double rr = 0.0, rm = 0.0;
+    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
+        x[e] += a * s[e];
+        const T rv = r[e] - a * q[e];
+        r[e] = rv;
+        rr += (double)rv * (double)rv;
+        if (SCALED) {
+            const double sg = sig[e];
+            if (sg > 0.0) rm = fmax(rm, fabs((double)rv) / sg);
+        } else {
+            rm = fmax(rm, fabs((double)rv));
+        }
+    }
+    const double bs = block_sum<NB>(rr, lds);
+    const double bm = block_max<NB>(rm, lds);
+    if (threadIdx.x == 0) {
+        red.pb[blockIdx.x] = bs;
+        red.pb[gridDim.x + blockIdx.x] = bm;
+    }
+}
+
+// ---------------------------------------------------------------- true residual
+
+template <class T, class C>
 __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, const C *__restrict__ diag,
                                 const C *__restrict__ cx, const C *__restrict__ cy, const C *__restrict__ cz,
                                 const T *__restrict__ v) {
     const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
-    T acc = UNIT ? v[e] : (T)diag[e] * v[e];
+    T acc = (T)diag[e] * v[e];
     if (i > 0) {
         acc -= (T)cx[e] * v[e - 1];
     } else {
@@ -176,143 +483,6 @@ __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, c
     return acc;
 }
 
-template <class T, class C, bool UNIT>
-__global__ __launch_bounds__(NT) void k_spmv_dot(BrickOp<C> op, int per, const T *__restrict__ s, T *__restrict__ q,
-                                                 Reduction red) {
-    __shared__ double lds[NT / kWave];
-    if (red.st->done) return;
-    int t0, t1;
-    chunk_range(op.n_active, per, t0, t1);
-    const int l = threadIdx.x;
-    double acc = 0.0;
-    for (int slot = t0; slot < t1; ++slot) {
-        const int64_t e = (int64_t)slot * NT + l;
-        const T qv = brick_apply<T, C, UNIT>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, s);
-        q[e] = qv;
-        acc += (double)s[e] * (double)qv;
-    }
-    const double bs = block_sum<NT>(acc, lds);
-    if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
-}
-
-__global__ __launch_bounds__(NT) void k_spmv_dot_csr(CsrOp op, int per, const double *__restrict__ s,
-                                                     double *__restrict__ q, Reduction red) {
-    __shared__ double lds[NT / kWave];
-    if (red.st->done) return;
-    int t0, t1;
-    chunk_range(blocks_for(op.rows), per, t0, t1);
-    double acc = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const int64_t e = (int64_t)t * NT + threadIdx.x;
-        if (e >= op.rows) continue;
-        double a = 0.0;
-        for (int64_t p = op.rowptr[e]; p < op.rowptr[e + 1]; ++p) a += op.vals[p] * s[op.cols[p]];
-        q[e] = a;
-        acc += s[e] * a;
-    }
-    const double bs = block_sum<NT>(acc, lds);
-    if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
-}
-
-// ---------------------------------------------------------------- axpy + norm
-
-template <class T, bool SCALED>
-__global__ __launch_bounds__(NT) void k_axpy_norm(int64_t n, int tiles, int per, const double *__restrict__ sig,
-                                                  T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
-                                                  const T *__restrict__ q, Reduction red) {
-    __shared__ double lds[NT / kWave];
-    __shared__ double bc;
-    SolveState *st = red.st;
-    if (st->done) return;
-    const double sigma = fold_sum(red.pa, gridDim.x, lds, &bc);
-    const double rho = st->rho;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-    if (lead) {
-        st->sigma = sigma;
-        st->rho_prev = rho;
-        st->iter_prev = st->iter;
-    }
-    if (!(sigma > 0.0)) {
-        if (lead) {
-            st->done = 1;
-            st->status = (sigma != sigma) ? SHK_NAN : SHK_BREAKDOWN;
-        }
-        return;
-    }
-    const double alpha = rho / sigma;
-    if (lead) st->alpha = alpha;
-    const T a = (T)alpha;
-    int t0, t1;
-    chunk_range(tiles, per, t0, t1);
-    double rr = 0.0, rm = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const int64_t e = (int64_t)t * NT + threadIdx.x;
-        if (e >= n) continue;
-        x[e] += a * s[e];
-        const T rv = r[e] - a * q[e];
-        r[e] = rv;
-        rr += (double)rv * (double)rv;
-        if (SCALED) {
-            const double sg = sig[e];
-            if (sg > 0.0) rm = fmax(rm, fabs((double)rv) / sg);
-        } else {
-            rm = fmax(rm, fabs((double)rv));
-        }
-    }
-    const double bs = block_sum<NT>(rr, lds);
-    const double bm = block_max<NT>(rm, lds);
-    if (threadIdx.x == 0) {
-        red.pb[blockIdx.x] = bs;
-        red.pb[gridDim.x + blockIdx.x] = bm;
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(NT) void k_update_dir(int64_t n, int tiles, int per, const T *__restrict__ r,
-                                                   T *__restrict__ s, Reduction red) {
-    __shared__ double lds[NT / kWave];
-    __shared__ double bc;
-    SolveState *st = red.st;
-    if (st->done) return;
-    const double rho_new = fold_sum(red.pb, gridDim.x, lds, &bc);
-    const double rn = fold_max(red.pb + gridDim.x, gridDim.x, lds, &bc);
-    const double rho_prev = st->rho_prev;
-    const int64_t it = st->iter_prev + 1;
-    // pcg_solver.h:281-290: the max-norm stop test comes first, then the direction
-    const bool stop = rn <= st->tol_abs;
-    const bool nan = rho_new != rho_new;
-    const bool cap = it >= st->max_iter;
-    const double beta = rho_new / rho_prev;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->iter = it;
-        st->rnorm = rn;
-        if (stop) {
-            st->done = 1;
-            st->status = SHK_OK;
-        } else if (nan) {
-            st->done = 1;
-            st->status = SHK_NAN;
-        } else if (cap) {
-            st->done = 1;
-            st->status = SHK_NOT_CONVERGED;
-        } else {
-            st->beta = beta;
-            st->rho = rho_new;
-        }
-    }
-    if (stop || nan || cap) return;
-    const T b = (T)beta;
-    int t0, t1;
-    chunk_range(tiles, per, t0, t1);
-    for (int t = t0; t < t1; ++t) {
-        const int64_t e = (int64_t)t * NT + threadIdx.x;
-        if (e >= n) continue;
-        s[e] = r[e] + b * s[e];
-    }
-}
-
-// ---------------------------------------------------------------- true residual
-
 __global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, int per, const double *__restrict__ b,
                                                  const double *__restrict__ x, double *__restrict__ r,
                                                  double *__restrict__ part, const double *__restrict__ sig,
@@ -327,7 +497,7 @@ __global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, int per, co
         double rv = 0.0, bv = 0.0;
         if (op.diag[e] != 0.0) {
             bv = b[e];
-            rv = bv - brick_apply<double, double, false>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, x);
+            rv = bv - brick_apply<double, double>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, x);
         }
         if (r) r[e] = rv;
         if (rs) rs[e] = (float)(sig ? sig[e] * rv : rv);
@@ -448,40 +618,39 @@ template <class T, class TB>
 void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *sig, T *x, T *r, T *s, Reduction red,
                     double tol_rel, int64_t max_iter) {
     if (sig)
-        hipLaunchKernelGGL((k_cg_init<T, TB, true>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, rhs, sig, x,
+        hipLaunchKernelGGL((k_cg_init<T, TB, true>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, rhs, sig, x,
                            r, s, red);
     else
-        hipLaunchKernelGGL((k_cg_init<T, TB, false>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, rhs, sig,
+        hipLaunchKernelGGL((k_cg_init<T, TB, false>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, rhs, sig,
                            x, r, s, red);
     hipLaunchKernelGGL(k_finish_init, dim3(1), dim3(NF), 0, st, g.blocks, red, tol_rel, max_iter);
 }
 
 template <class T, class C>
-void launch_spmv_dot(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *s, T *q, Reduction red) {
+void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
+                     Reduction red) {
     if (op.diag)
-        hipLaunchKernelGGL((k_spmv_dot<T, C, false>), dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
+        hipLaunchKernelGGL((k_dir_spmv<T, C, false>), dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q,
+                           red);
     else
-        hipLaunchKernelGGL((k_spmv_dot<T, C, true>), dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
+        hipLaunchKernelGGL((k_dir_spmv<T, C, true>), dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q,
+                           red);
 }
 
-void launch_spmv_dot_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *s, double *q, Reduction red) {
-    hipLaunchKernelGGL(k_spmv_dot_csr, dim3(g.blocks), dim3(NT), 0, st, op, g.per, s, q, red);
+void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
+                         double *s_out, double *q, Reduction red) {
+    hipLaunchKernelGGL(k_dir_spmv_csr, dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q, red);
 }
 
 template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
                       Reduction red) {
     if (sig)
-        hipLaunchKernelGGL((k_axpy_norm<T, true>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, sig, x, r, s,
+        hipLaunchKernelGGL((k_axpy_norm<T, true>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, sig, x, r, s,
                            q, red);
     else
-        hipLaunchKernelGGL((k_axpy_norm<T, false>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, sig, x, r,
+        hipLaunchKernelGGL((k_axpy_norm<T, false>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, sig, x, r,
                            s, q, red);
-}
-
-template <class T>
-void launch_update_dir(hipStream_t st, const Grid &g, const T *r, T *s, Reduction red) {
-    hipLaunchKernelGGL((k_update_dir<T>), dim3(g.blocks), dim3(NT), 0, st, g.n, g.tiles, g.per, r, s, red);
 }
 
 void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,
@@ -518,15 +687,13 @@ template void launch_cg_init<float, double>(hipStream_t, const Grid &, const dou
                                             float *, float *, Reduction, double, int64_t);
 template void launch_cg_init<float, float>(hipStream_t, const Grid &, const float *, const double *, float *, float *,
                                            float *, Reduction, double, int64_t);
-template void launch_spmv_dot<double, double>(hipStream_t, const Grid &, const BrickOp<double> &, const double *,
-                                              double *, Reduction);
-template void launch_spmv_dot<float, float>(hipStream_t, const Grid &, const BrickOp<float> &, const float *, float *,
-                                            Reduction);
+template void launch_dir_spmv<double, double>(hipStream_t, const Grid &, const BrickOp<double> &, const double *,
+                                              const double *, double *, double *, Reduction);
+template void launch_dir_spmv<float, float>(hipStream_t, const Grid &, const BrickOp<float> &, const float *,
+                                            const float *, float *, float *, Reduction);
 template void launch_axpy_norm<double>(hipStream_t, const Grid &, const double *, double *, double *, const double *,
                                        const double *, Reduction);
 template void launch_axpy_norm<float>(hipStream_t, const Grid &, const double *, float *, float *, const float *,
                                       const float *, Reduction);
-template void launch_update_dir<double>(hipStream_t, const Grid &, const double *, double *, Reduction);
-template void launch_update_dir<float>(hipStream_t, const Grid &, const float *, float *, Reduction);
 
 }  // namespace shk

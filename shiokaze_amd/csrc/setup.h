@@ -28,6 +28,11 @@ void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const doub
                         const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
                         float *hy32, float *hz32);
 
+// kappa[slot] = the one coupling value of a brick whose stencil is uniform, else 0.
+template <class C>
+void launch_brick_uniform(hipStream_t st, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
+                          C *kappa);
+
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
                     const double *x, float *pressure);
 

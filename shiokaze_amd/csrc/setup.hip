@@ -71,6 +71,7 @@ __device__ inline bool is_unknown(const Dims &d, const float *fluid, const Faces
 
 __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8_t *__restrict__ label,
                         uint8_t *__restrict__ brick_flag, int nbx, int nby, unsigned long long *__restrict__ count) {
+    __shared__ unsigned long long wsum[kNT / kWave];
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     unsigned long long local = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
@@ -84,9 +85,15 @@ __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8
             ++local;
         }
     }
-    // integer count: order independent
+    // integer count (order independent): one atomic per workgroup
     for (int o = kWave / 2; o > 0; o >>= 1) local += __shfl_xor(local, o, kWave);
-    if ((threadIdx.x % kWave) == 0 && local) atomicAdd(count, local);
+    if ((threadIdx.x % kWave) == 0) wsum[threadIdx.x / kWave] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kNT / kWave; ++w) t += wsum[w];
+        if (t) atomicAdd(count, t);
+    }
 }
 
 // ---------------------------------------------------------------- coefficients + rhs (a7)
@@ -193,6 +200,36 @@ __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restr
         if (h[a]) h[a][e] = v;
         if (h32[a]) h32[a][e] = (float)v;
     }
+}
+
+// Per-brick constant coupling: kappa[slot] = c when every coupling touching the
+// brick's cells (its 3 low faces per cell and the far faces stored in the +x/+y/+z
+// neighbour bricks) equals one nonzero value c bit for bit, else 0.  Such a brick
+// has only unknowns and the stencil needs no coefficient loads there.
+template <class C>
+__global__ void k_brick_uniform(const int *__restrict__ nbr, const C *__restrict__ cx, const C *__restrict__ cy,
+                                const C *__restrict__ cz, C *__restrict__ kappa) {
+    const int slot = blockIdx.x;
+    const int l = threadIdx.x;
+    const int loc[3] = {l & 7, (l >> 3) & 7, l >> 6};
+    const int stride[3] = {1, 8, 64};
+    const C *c[3] = {cx, cy, cz};
+    const int64_t e = (int64_t)slot * kBrickCells + l;
+    const C k0 = cx[(int64_t)slot * kBrickCells];
+    bool ok = k0 != C(0);
+    for (int a = 0; a < 3; ++a) {
+        ok = ok && c[a][e] == k0;
+        int64_t m;
+        if (loc[a] < 7) {
+            m = e + stride[a];
+        } else {
+            const int nb = nbr[6 * slot + 2 * a + 1];
+            m = nb >= 0 ? (int64_t)nb * kBrickCells + l - 7 * stride[a] : -1;
+        }
+        ok = ok && m >= 0 && c[a][m] == k0;
+    }
+    const int all = __syncthreads_and(ok ? 1 : 0);
+    if (l == 0) kappa[slot] = all ? k0 : C(0);
 }
 
 // ---------------------------------------------------------------- scatter + update (a14, a15)
@@ -347,7 +384,9 @@ void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[
                   uint8_t *brick_flag, int nbx, int nby, unsigned long long *count) {
     Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_label, dim3(grid_for(G)), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
+    unsigned g = grid_for(G);
+    if (g > 4096) g = 4096;  // grid-stride: bounded number of count atomics
+    hipLaunchKernelGGL(k_label, dim3(g), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
 }
 
 void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
@@ -368,6 +407,17 @@ void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const doub
     hipLaunchKernelGGL(k_scale_coeff, dim3(n_active), dim3(kBrickCells), 0, st, nbr, sig, cx, cy, cz, hx, hy, hz, hx32,
                        hy32, hz32);
 }
+
+template <class C>
+void launch_brick_uniform(hipStream_t st, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
+                          C *kappa) {
+    if (n_active <= 0) return;
+    hipLaunchKernelGGL(k_brick_uniform<C>, dim3(n_active), dim3(kBrickCells), 0, st, nbr, cx, cy, cz, kappa);
+}
+template void launch_brick_uniform<double>(hipStream_t, int, const int *, const double *, const double *,
+                                           const double *, double *);
+template void launch_brick_uniform<float>(hipStream_t, int, const int *, const float *, const float *, const float *,
+                                          float *);
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
                     const double *x, float *pressure) {

@@ -16,19 +16,18 @@ constexpr int kXcds = 8;                  // MI355X accelerator complex dies
 
 // Krylov state kept on the device so that iterations are launched back to back
 // without a host round trip; the host polls `done` every few iterations.
-// Fields written by block 0 of one kernel are read only by later kernels; the
-// *_prev copies carry a value across the kernel that overwrites it.
+// Each field has one writer kernel (its block 0) and is read only by the other
+// kernel of the iteration, so no field is read and written in the same launch.
 struct SolveState {
-    double rho;       // r.z of the current residual
-    double rho_prev;  // rho as the current iteration's axpy_norm read it
-    double alpha;     // step of the current iteration
-    double beta;      // direction update of the current iteration
-    double rnorm;     // max |r| after the last update
+    double rho_k1;    // r.r folded by the direction kernel (read by axpy for alpha)
+    double rho_k2;    // the rho the last axpy used (read by the direction kernel for beta)
+    double alpha;     // step of the last iteration (axpy)
+    double beta;      // direction update of the current iteration (direction kernel)
+    double rnorm;     // max |r| after the last update (direction kernel)
     double rnorm0;    // max |rhs| at the start
     double tol_abs;   // stop when rnorm <= tol_abs
-    double sigma;     // s.As of the current iteration (diagnostic)
-    int64_t iter;     // completed iterations
-    int64_t iter_prev;
+    double sigma;     // s.As of the last iteration (axpy)
+    int64_t iter;     // completed iterations (axpy)
     int64_t max_iter;
     int32_t done;     // 1 once converged, capped or broken down
     int32_t status;   // shk_status
