@@ -168,6 +168,63 @@ int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64
  * Only launches that did work are counted. */
 int shk_kernel_stats(shk_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
 
+/* ------------------------------------------------------------------ learned preconditioner */
+
+/* A 4-level 3D U-Net (16/32/64/128 channels, BF16 MFMA convolutions, FP32
+ * accumulation) used as the preconditioner of flexible CG when a context has
+ * Preconditioner=unet and UnetWeights=<file written by shk_unet_save>.  It has no
+ * counterpart in the reference, whose pcg plugin applies no preconditioner
+ * (pcg_solver.h:379-384); its right-hand sides and labels are the reference
+ * system of the projection above.
+ *
+ * All grids below are dense, x-fastest, FP32 device arrays of the Jacobi-scaled
+ * system A^ = D^-1/2 A D^-1/2 (unit diagonal) with fine dims that are multiples
+ * of 16; the network works on the half-resolution grid. */
+typedef struct shk_unet shk_unet;
+
+int shk_unet_create(uint64_t seed, int device, shk_unet **out);
+void shk_unet_destroy(shk_unet *net);
+const char *shk_unet_last_error(const shk_unet *net);
+int shk_unet_num_params(const shk_unet *net, int64_t *count);
+/* Copies the FP32 parameters (grads = 0) or their accumulated gradients (grads = 1)
+ * into a host array of shk_unet_num_params floats. */
+int shk_unet_get_params(shk_unet *net, float *host, int grads);
+/* Sets the parameters from a host array and clears the gradients. */
+int shk_unet_set_params(shk_unet *net, const float *host);
+int shk_unet_save(shk_unet *net, const char *path);
+int shk_unet_load(shk_unet *net, const char *path);
+
+/* One training sample: the A^-norm error loss of M(r) against xstar = A^-1 r,
+ * L = (M(r) - xstar)' A^ (M(r) - xstar) / (xstar' r), with
+ * M(r) = mask * (omega r + s P U(R r / s, ...)).  Gradients accumulate; with
+ * lr > 0 an Adam step is taken and the gradients are cleared. */
+int shk_unet_train_step(shk_unet *net, int nx, int ny, int nz, const float *r, const float *mask, const float *cx,
+                        const float *cy, const float *cz, const float *xstar, double omega, float lr,
+                        double *loss);
+/* z = M(r) (the preconditioner application). */
+int shk_unet_apply(shk_unet *net, int nx, int ny, int nz, const float *r, const float *mask, const float *cx,
+                   const float *cy, const float *cz, double omega, float *z);
+
+/* r = b - A^ x on a dense grid with the couplings above (device pointers; r aliases no
+ * input).  Builds training residuals. */
+int shk_unet_residual(int nx, int ny, int nz, const float *cx, const float *cy, const float *cz, const float *x,
+                      const float *b, float *r);
+
+/* Dense FP32 copies (nx*ny*nz, device) of the last projection's scaled system:
+ * b^ = D^-1/2 b, mask (1 on unknowns), the low-face couplings of A^ and
+ * x^ = D^1/2 x (the scaled solution).  Any pointer may be NULL. */
+int shk_export_scaled(shk_ctx *ctx, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat);
+
+/* Layer-level access to the network's convolution kernels (device pointers):
+ * x and y are channels-last BF16 grids, w is W[tap][ci][co] FP32 with
+ * tap = (dz+1)*9 + (dy+1)*3 + (dx+1); cin a multiple of 8, cout of 16.
+ * y = act(conv3x3x3(x, w) + bias) with zero padding (act = ReLU when relu). */
+int shk_nn_conv3d(int nx, int ny, int nz, const void *x, int cin, const float *w, const float *bias, int cout,
+                  int relu, void *y);
+/* dx = input gradient (BF16, NULL to skip), dw / db += weight / bias gradients (FP32). */
+int shk_nn_conv3d_backward(int nx, int ny, int nz, const void *x, int cin, const float *w, int cout, const void *dz,
+                           void *dx, float *dw, float *db);
+
 /* ------------------------------------------------------------------ scenes */
 
 /* Synthetic scenes for tests and benchmarks, generated on the host into HOST

@@ -79,6 +79,23 @@ _SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_double)]),
     "shk_scene_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_uint64] + [_vp] * 8 + [ctypes.POINTER(ctypes.c_int64)]),
+    "shk_unet_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "shk_unet_destroy": (None, [_vp]),
+    "shk_unet_last_error": (ctypes.c_char_p, [_vp]),
+    "shk_unet_num_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
+    "shk_unet_get_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
+    "shk_unet_set_params": (ctypes.c_int, [_vp, _vp]),
+    "shk_unet_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "shk_unet_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "shk_unet_train_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 6
+                            + [ctypes.c_double, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]),
+    "shk_unet_apply": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 5
+                       + [ctypes.c_double, _vp]),
+    "shk_export_scaled": (ctypes.c_int, [_vp] * 7),
+    "shk_unet_residual": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp] * 6),
+    "shk_nn_conv3d": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                                          _vp]),
+    "shk_nn_conv3d_backward": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, ctypes.c_int] + [_vp] * 4),
 }
 
 _lib = None

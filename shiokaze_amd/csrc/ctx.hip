@@ -114,7 +114,7 @@ int build_bricks(shk_ctx *c) {
     for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->x64, &c->r64,
                               &c->s64, &c->t64, &c->q64})
         HIPC(c, b->ensure(n));
-    if (c->prm.precond != 0 && c->prm.precision == 0) {
+    if (c->prm.precond == 1 && c->prm.precision == 0) {
         for (DevBuf<double> *b : {&c->hx64, &c->hy64, &c->hz64}) HIPC(c, b->ensure(n));
         HIPC(c, c->kappa64.ensure(c->n_active));
     }

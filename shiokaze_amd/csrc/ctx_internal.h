@@ -169,5 +169,6 @@ int solve_fp64(shk_ctx *c, shk_result *res);
 // learned preconditioner (unet.hip)
 void unet_destroy(UnetPrecond *u);
 int unet_solve(shk_ctx *c, shk_result *res);
+int export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat);
 
 }  // namespace shk

@@ -1,0 +1,85 @@
+// unet_kernels.h -- kernels of the learned preconditioner's 3D U-Net (gfx950).
+//
+// Activations are dense channels-last BF16 grids: element (voxel v, channel c) at
+// v * C + c, voxel v = x + nx*(y + ny*z).  Convolutions are 3x3x3, stride 1, zero
+// padding 1, computed as implicit GEMMs on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation.  The reduction index k = tap * Cin + ci runs over the 27 taps
+// (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) and input channels; Cin is a multiple of 8.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+
+namespace shk {
+
+using bf16 = __hip_bfloat16;
+
+struct Grid3 {
+    int nx, ny, nz;
+    __host__ __device__ int64_t vox() const { return (int64_t)nx * ny * nz; }
+};
+
+// K chunks of 32 of a 3x3x3 convolution with cin input channels.
+inline int conv_kchunks(int cin) { return (27 * cin + 31) / 32; }
+// Packed weight elements (bf16) for cout outputs: cout/16 M-tiles x kchunks x 64 lanes x 8.
+inline int64_t conv_packed_elems(int cin, int cout) { return (int64_t)(cout / 16) * conv_kchunks(cin) * 64 * 8; }
+
+// Packs FP32 master weights W[tap][ci][co] into the MFMA A-fragment order:
+// transposed = false: forward (A[m = co][k = tap*cin + ci]);
+// transposed = true : input gradient (A[m = ci][k = tap'*cout + co] = W[26 - tap'][ci][co]).
+void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool transposed, bf16 *packed);
+
+// y[v][co] = act(bias[co] + sum_k w * x); act = ReLU when relu.  out_f32 (nullable)
+// receives channel 0 in FP32 instead of bf16 output (the network's final layer).
+void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
+                     bool relu, bf16 *y, float *out_f32);
+
+// dw[tap][ci][co] += sum_v x[v + tap][ci] * dz[v][co]; db[co] += sum_v dz[v][co]
+// (FP32 atomics, split over voxel chunks).
+void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,
+                       float *db);
+
+// dz = dy * (y > 0) in place on dy (ReLU backward), n elements.
+void launch_relu_bwd(hipStream_t st, int64_t n, const bf16 *y, bf16 *dy);
+// coarse[V][c] = mean of the 8 fine children (fine grid is 2x in each axis).
+void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out);
+// fine_grad[v][c] = coarse_grad[parent(v)][c] / 8 (pool backward).
+void launch_pool_bwd(hipStream_t st, Grid3 coarse, int c, const bf16 *coarse_grad, bf16 *fine_grad);
+// out[v][c_off + c] = coarse[parent(v)][c] for c < c_coarse (nearest upsampling into
+// a concat buffer of cstride channels).
+void launch_upsample(hipStream_t st, Grid3 cgrid, int c_coarse, const bf16 *coarse, int cstride, bf16 *out);
+// coarse_grad[V][c] = sum over the 8 children of grad[child][c_off + c].
+void launch_upsample_bwd(hipStream_t st, Grid3 coarse, int c_coarse, const bf16 *grad, int cstride, bf16 *coarse_grad);
+// dst[v][d_off + c] = src[v][c], c < cs (copy a channel block between buffers);
+// with add = true, dst += src.
+void launch_copy_channels(hipStream_t st, int64_t vox, int cs, const bf16 *src, int sstride, int soff, bf16 *dst,
+                          int dstride, int doff, bool add);
+// out = a + b elementwise (bf16, n elements).
+void launch_add_bf16(hipStream_t st, int64_t n, const bf16 *a, const bf16 *b, bf16 *out);
+
+// Network input on the coarse grid (fine grid = 2x): 8 channels
+//   [R(r)/scale, R(mask), R(cx), R(cy), R(cz), 0, 0, 0] with R = 2x2x2 average.
+void launch_features(hipStream_t st, Grid3 coarse, const float *r, double inv_scale, const float *mask,
+                     const float *cx, const float *cy, const float *cz, bf16 *feat);
+// z[v] = mask[v] * (omega * r[v] + scale * u[parent(v)]) on the fine grid (P = nearest).
+void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, double omega, const float *r,
+                    const float *mask, float *z);
+// grad_u[V] = sum over the 8 children of mask * g (P^T), times scale.
+void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale, float *grad_u);
+
+// Dense 7-point unit-diagonal stencil: y = x - sum c_face * x_nbr (couplings at
+// the high cell of each face, zero on faces touching non-unknowns).
+void launch_dense_stencil(hipStream_t st, Grid3 g, const float *cx, const float *cy, const float *cz, const float *x,
+                          float *y);
+
+// Adam step on n FP32 parameters with gradient g (zeroed afterwards).
+void launch_adam(hipStream_t st, int64_t n, float *p, float *g, float *m, float *v, float lr, float b1, float b2,
+                 float eps, float bc1, float bc2);
+
+// float <-> bf16 conversion of a 1-channel grid into an 8-channel slot is handled by
+// launch_features; this converts an FP32 array to bf16.
+void launch_f32_to_bf16(hipStream_t st, int64_t n, const float *src, bf16 *dst);
+
+}  // namespace shk

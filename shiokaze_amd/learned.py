@@ -1,0 +1,148 @@
+"""Host-side handle of the learned preconditioner (shk_unet_* in libshk_projection).
+
+The reference's pcg plugin applies no preconditioner (pcg_solver.h:379-384); this
+network is an addition of this library, used by MacPressureSolver3 when it is
+configured with ``Preconditioner=unet`` and ``UnetWeights=<file>``.  Grids are
+dense FP32 torch tensors on the GPU of shape (nz, ny, nx) with dims that are
+multiples of 16, holding the Jacobi-scaled system that
+:meth:`MacPressureSolver3.export_scaled` writes.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .projection import _ptr
+
+
+class UNet:
+    """A 4-level 3D U-Net (BF16 MFMA convolutions) and its Adam trainer on one GPU."""
+
+    def __init__(self, seed: int = 0, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.shk_unet_create(int(seed), int(device), ctypes.byref(h)))
+        self.h = h
+        self.device = int(device)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.shk_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != _lib.SHK_OK:
+            msg = self.lib.shk_unet_last_error(self.h)
+            raise _lib.ShkError(rc, msg.decode() if msg else "")
+
+    @property
+    def num_params(self) -> int:
+        n = ctypes.c_int64(0)
+        self._check(self.lib.shk_unet_num_params(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def get_params(self, grads: bool = False) -> np.ndarray:
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.lib.shk_unet_get_params(self.h, out.ctypes.data_as(ctypes.c_void_p), int(bool(grads))))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        self._check(self.lib.shk_unet_set_params(self.h, p.ctypes.data_as(ctypes.c_void_p)))
+
+    def save(self, path) -> None:
+        self._check(self.lib.shk_unet_save(self.h, str(path).encode()))
+
+    def load(self, path) -> None:
+        self._check(self.lib.shk_unet_load(self.h, str(path).encode()))
+
+    @staticmethod
+    def _grid(t: torch.Tensor):
+        if t.dim() != 3:
+            raise ValueError("grids are (nz, ny, nx) tensors")
+        nz, ny, nx = t.shape
+        return int(nx), int(ny), int(nz), int(t.numel())
+
+    def train_step(self, r, mask, cx, cy, cz, xstar, omega: float = 1.0, lr: float = 0.0) -> float:
+        """A-norm loss of M(r) against xstar; gradients accumulate, lr > 0 takes an Adam step."""
+        nx, ny, nz, n = self._grid(r)
+        ptrs = [_ptr(t, n, nm) for t, nm in ((r, "r"), (mask, "mask"), (cx, "cx"), (cy, "cy"), (cz, "cz"),
+                                            (xstar, "xstar"))]
+        loss = ctypes.c_double(0.0)
+        torch.cuda.synchronize(self.device)
+        self._check(self.lib.shk_unet_train_step(self.h, nx, ny, nz, *ptrs, float(omega), float(lr),
+                                                 ctypes.byref(loss)))
+        return float(loss.value)
+
+    def apply(self, r, mask, cx, cy, cz, omega: float = 1.0) -> torch.Tensor:
+        """z = M(r)."""
+        nx, ny, nz, n = self._grid(r)
+        z = torch.empty_like(r)
+        ptrs = [_ptr(t, n, nm) for t, nm in ((r, "r"), (mask, "mask"), (cx, "cx"), (cy, "cy"), (cz, "cz"))]
+        torch.cuda.synchronize(self.device)
+        self._check(self.lib.shk_unet_apply(self.h, nx, ny, nz, *ptrs, float(omega), _ptr(z, n, "z")))
+        return z
+
+
+def residual(cx, cy, cz, x, b) -> torch.Tensor:
+    """r = b - A^ x for the unit-diagonal operator with low-face couplings cx, cy, cz."""
+    lib = _lib.load()
+    nz, ny, nx = b.shape
+    n = b.numel()
+    r = torch.empty_like(b)
+    torch.cuda.synchronize(b.device)
+    _lib.check(lib.shk_unet_residual(nx, ny, nz, *[_ptr(t, n, nm) for t, nm in
+                                                   ((cx, "cx"), (cy, "cy"), (cz, "cz"), (x, "x"), (b, "b"))],
+                                     _ptr(r, n, "r")))
+    return r
+
+
+def conv3d(x: torch.Tensor, w: torch.Tensor, bias: torch.Tensor | None, relu: bool) -> torch.Tensor:
+    """One 3x3x3 convolution layer of the network's kernels: x (nz, ny, nx, cin) bf16,
+    w (27, cin, cout) fp32 with tap = (dz+1)*9 + (dy+1)*3 + (dx+1); returns bf16."""
+    lib = _lib.load()
+    nz, ny, nx, cin = x.shape
+    cout = w.shape[2]
+    y = torch.empty((nz, ny, nx, cout), dtype=torch.bfloat16, device=x.device)
+    nv = nx * ny * nz
+    bp = _ptr(bias, cout, "bias") if bias is not None else None
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.shk_nn_conv3d(nx, ny, nz, _ptr(x, nv * cin, "x", torch.bfloat16), cin,
+                                 _ptr(w, 27 * cin * cout, "w"), bp, cout, int(bool(relu)),
+                                 _ptr(y, nv * cout, "y", torch.bfloat16)))
+    return y
+
+
+def conv3d_backward(x: torch.Tensor, w: torch.Tensor, dz: torch.Tensor, need_dx: bool = True):
+    """(dx bf16 or None, dw fp32, db fp32) of conv3d for the output gradient dz."""
+    lib = _lib.load()
+    nz, ny, nx, cin = x.shape
+    cout = w.shape[2]
+    nv = nx * ny * nz
+    dx = torch.empty_like(x) if need_dx else None
+    dw = torch.zeros((27, cin, cout), dtype=torch.float32, device=x.device)
+    db = torch.zeros(cout, dtype=torch.float32, device=x.device)
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.shk_nn_conv3d_backward(nx, ny, nz, _ptr(x, nv * cin, "x", torch.bfloat16), cin,
+                                          _ptr(w, 27 * cin * cout, "w"), cout,
+                                          _ptr(dz, nv * cout, "dz", torch.bfloat16),
+                                          _ptr(dx, nv * cin, "dx", torch.bfloat16) if need_dx else None,
+                                          _ptr(dw, dw.numel(), "dw"), _ptr(db, cout, "db")))
+    return dx, dw, db

@@ -137,6 +137,19 @@ class MacPressureSolver3:
             self.ctx)
         return out
 
+    def export_scaled(self) -> dict:
+        """Dense FP32 (nz, ny, nx) grids of the last system scaled by D^-1/2:
+        rhat = D^-1/2 b, mask (1 on unknowns), low-face couplings hx/hy/hz of the
+        unit-diagonal operator and xhat = D^1/2 x (the last solution)."""
+        nx, ny, nz = self.shape
+        out = {k: torch.empty((nz, ny, nx), dtype=torch.float32, device=f"cuda:{self.device}")
+               for k in ("rhat", "mask", "hx", "hy", "hz", "xhat")}
+        torch.cuda.synchronize(self.device)
+        _lib.check(self.lib.shk_export_scaled(
+            self.ctx, *[ctypes.c_void_p(out[k].data_ptr()) for k in ("rhat", "mask", "hx", "hy", "hz", "xhat")]),
+            self.ctx)
+        return out
+
     def divergence(self, dt: float, velocity, solid_velocity, fp64_update: bool) -> float:
         u, v, w = velocity
         us, vs, ws = solid_velocity

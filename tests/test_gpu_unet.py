@@ -1,0 +1,247 @@
+"""The learned preconditioner's HIP kernels against the FP64 oracle (oracle/unet_oracle.py):
+BF16 MFMA convolution forward / input gradient / weight gradient, the whole
+network's loss and parameter gradients, Adam, weights files, and flexible CG
+inside shk_project (converging to the same residual as the plain solver)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gpu_helpers import log, run_projection
+from oracle import unet_oracle as U
+from unet_cases import init_params, training_sample
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=None):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _bf16_tensor(a):
+    import torch
+    return _dev(np.asarray(a, dtype=np.float32), torch.bfloat16)
+
+
+def _host(t):
+    import torch
+    return t.to(torch.float64).cpu().numpy()
+
+
+CONV_CASES = [  # (nz, ny, nx), cin, cout, relu
+    ((5, 6, 7), 8, 16, True),
+    ((8, 8, 8), 16, 32, True),
+    ((4, 9, 6), 48, 16, False),
+    ((6, 4, 8), 64, 128, True),
+    ((4, 4, 4), 192, 64, True),
+    ((3, 5, 4), 32, 48, False),
+]
+
+
+@pytest.mark.parametrize("shape,cin,cout,relu", CONV_CASES)
+def test_conv3d_forward(shape, cin, cout, relu):
+    from shiokaze_amd import learned
+    log(f"[gpu] conv3d forward {shape} {cin}->{cout} relu={relu}")
+    rng = np.random.default_rng(cin * 1000 + cout)
+    x = U.bf16_round(rng.standard_normal(shape + (cin,)))
+    w = rng.standard_normal((27, cin, cout)) * np.sqrt(2.0 / (27 * cin))
+    b = rng.standard_normal(cout) * 0.1
+    y = _host(learned.conv3d(_bf16_tensor(x), _dev(w.astype(np.float32)), _dev(b.astype(np.float32)), relu))
+    wq = U.bf16_round(w)
+    ref = U.conv3d(x, wq, b.astype(np.float32).astype(np.float64))
+    if relu:
+        ref = np.maximum(ref, 0.0)
+    scale = U.conv3d(np.abs(x), np.abs(wq))
+    # BF16 output rounding (2^-9 relative) plus FP32 accumulation over 27*cin terms
+    tol = 2.0 ** -8 * np.abs(ref) + 1e-5 * scale + 1e-30
+    err = np.abs(y - ref)
+    log(f"[gpu]   max err/tol = {np.max(err / tol):.3f}")
+    assert np.all(err <= tol)
+
+
+@pytest.mark.parametrize("shape,cin,cout", [((5, 6, 7), 16, 16), ((4, 5, 6), 32, 64), ((4, 4, 4), 192, 64),
+                                            ((3, 4, 5), 8, 16), ((4, 4, 3), 96, 32)])
+def test_conv3d_backward(shape, cin, cout):
+    from shiokaze_amd import learned
+    log(f"[gpu] conv3d backward {shape} {cin}->{cout}")
+    rng = np.random.default_rng(cin * 7 + cout)
+    x = U.bf16_round(rng.standard_normal(shape + (cin,)))
+    w = rng.standard_normal((27, cin, cout)) * np.sqrt(2.0 / (27 * cin))
+    dz = U.bf16_round(rng.standard_normal(shape + (cout,)))
+    need_dx = cin % 16 == 0
+    dx, dw, db = learned.conv3d_backward(_bf16_tensor(x), _dev(w.astype(np.float32)), _bf16_tensor(dz), need_dx)
+    wq = U.bf16_round(w)
+    if need_dx:
+        ref = U.conv3d_dgrad(dz, wq)
+        scale = U.conv3d_dgrad(np.abs(dz), np.abs(wq))
+        err = np.abs(_host(dx) - ref)
+        tol = 2.0 ** -8 * np.abs(ref) + 1e-5 * scale + 1e-30
+        log(f"[gpu]   dx max err/tol = {np.max(err / tol):.3f}")
+        assert np.all(err <= tol)
+    rdw, rdb = U.conv3d_wgrad(x, dz)
+    sdw, sdb = U.conv3d_wgrad(np.abs(x), np.abs(dz))
+    edw = np.abs(dw.cpu().numpy().astype(np.float64) - rdw)
+    edb = np.abs(db.cpu().numpy().astype(np.float64) - rdb)
+    log(f"[gpu]   dw max err/scale = {np.max(edw / sdw):.2e}, db {np.max(edb / sdb):.2e}")
+    assert np.all(edw <= 1e-5 * sdw + 1e-30)
+    assert np.all(edb <= 1e-5 * sdb + 1e-30)
+
+
+def _sample_dev(shape, seed):
+    r, mask, cx, cy, cz, xstar = training_sample(shape, seed)
+    f32 = [a.astype(np.float32) for a in (r, mask, cx, cy, cz, xstar)]
+    return f32, [_dev(a) for a in f32]
+
+
+def test_network_loss_and_gradients():
+    from shiokaze_amd import UNet
+    log("[gpu] network loss/gradients 32^3")
+    (r, mask, cx, cy, cz, xs), dev = _sample_dev((32, 32, 32), 5)
+    p = init_params(6).astype(np.float32)
+    with UNet(seed=1) as net:
+        assert net.num_params == U.num_params()
+        net.set_params(p)
+        np.testing.assert_array_equal(net.get_params(), p)
+        loss = net.train_step(*dev, omega=0.8, lr=0.0)
+        g = net.get_params(grads=True).astype(np.float64)
+    rl, rg, _ = U.loss_and_grad(p.astype(np.float64), *[a.astype(np.float64) for a in (r, mask, cx, cy, cz, xs)],
+                                0.8, bf16=True)
+    log(f"[gpu]   loss gpu {loss:.6f} oracle {rl:.6f}")
+    assert loss == pytest.approx(rl, rel=1e-2)
+    cos = float(g @ rg / (np.linalg.norm(g) * np.linalg.norm(rg)))
+    log(f"[gpu]   gradient cosine {cos:.6f}")
+    for l, (wo, bo) in enumerate(U.layer_offsets()[0]):
+        cout = U.LAYERS[l][1]
+        seg = slice(wo, bo + cout)
+        rel = np.linalg.norm(g[seg] - rg[seg]) / np.linalg.norm(rg[seg])
+        log(f"[gpu]   layer {l}: rel L2 grad error {rel:.3e}")
+        assert rel < 5e-2
+    assert cos > 0.998
+
+
+def test_apply_matches_oracle_and_is_homogeneous():
+    from shiokaze_amd import UNet
+    log("[gpu] apply 32x48x16")
+    (r, mask, cx, cy, cz, xs), dev = _sample_dev((16, 48, 32), 8)
+    p = init_params(9).astype(np.float32)
+    with UNet() as net:
+        net.set_params(p)
+        z = net.apply(*dev[:5], omega=0.5).cpu().numpy().astype(np.float64)
+        z2 = net.apply(dev[0] * 4.0, *dev[1:5], omega=0.5).cpu().numpy().astype(np.float64)
+    rz, _, _ = U.precondition(p.astype(np.float64), *[a.astype(np.float64) for a in (r, mask, cx, cy, cz)], 0.5,
+                              bf16=True)
+    rel = np.linalg.norm(z - rz) / np.linalg.norm(rz)
+    log(f"[gpu]   rel L2 error {rel:.3e}")
+    assert rel < 1e-2
+    assert np.all(z[mask == 0] == 0)
+    # the RMS normalisation makes M positively homogeneous (exactly, up to FP32 scaling)
+    np.testing.assert_allclose(z2, 4.0 * z, rtol=1e-5, atol=1e-6 * np.abs(z).max())
+
+
+def test_residual_kernel():
+    from shiokaze_amd import learned
+    (r, mask, cx, cy, cz, xs), dev = _sample_dev((16, 24, 32), 12)
+    x = (np.random.default_rng(1).standard_normal(r.shape) * mask).astype(np.float32)
+    res = learned.residual(dev[2], dev[3], dev[4], _dev(x), dev[0]).cpu().numpy().astype(np.float64)
+    ref = r.astype(np.float64) - U.stencil(cx.astype(np.float64), cy.astype(np.float64), cz.astype(np.float64),
+                                           x.astype(np.float64))
+    np.testing.assert_allclose(res, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+
+
+def test_adam_step_and_weights_file(tmp_path):
+    from shiokaze_amd import ShkError, UNet
+    log("[gpu] adam + save/load")
+    _, dev = _sample_dev((16, 16, 16), 3)
+    p0 = init_params(4).astype(np.float32)
+    lr = 1e-3
+    with UNet() as net:
+        net.set_params(p0)
+        net.train_step(*dev, omega=1.0, lr=0.0)
+        g = net.get_params(grads=True)
+        net.train_step(*dev, omega=1.0, lr=lr)   # accumulates g again, then Adam step 1
+        p1 = net.get_params()
+        assert np.all(net.get_params(grads=True) == 0)
+        path = tmp_path / "w.bin"
+        net.save(path)
+        with UNet(seed=99) as other:
+            assert not np.array_equal(other.get_params(), p1)
+            other.load(path)
+            np.testing.assert_array_equal(other.get_params(), p1)
+        bad = tmp_path / "bad.bin"
+        bad.write_bytes(b"not a weights file")
+        with pytest.raises(ShkError):
+            net.load(bad)
+    # first Adam step from zero moments: p1 = p0 - lr * G / (|G| + eps), G = accumulated gradient ~ 2g
+    big = np.abs(g) > 1e-3 * np.abs(g).max()
+    step = (p0 - p1)[big]
+    np.testing.assert_allclose(step, lr * np.sign(g[big]), rtol=1e-3)
+
+
+def _scene_config(weights):
+    return {"Residual": 1e-6, "Preconditioner": "unet", "UnetWeights": str(weights)}
+
+
+@pytest.mark.parametrize("kind,n,seed", [(1, 32, 0), (3, (24, 40, 16), 3), (3, 48, 5)])
+def test_projection_with_learned_preconditioner(tmp_path, scenes, kind, n, seed):
+    """Flexible CG with the (untrained) network converges to the residual asked for,
+    falling back to Jacobi-PCG if it stalls; the pressure matches the plain solver's."""
+    from shiokaze_amd import UNet
+    log(f"[gpu] unet projection kind={kind} n={n} seed={seed}")
+    path = tmp_path / "net.bin"
+    with UNet(seed=seed) as net:
+        net.save(path)
+    sc = scenes(kind, n, seed)
+    g = run_projection(sc, _scene_config(path), want_system=False)
+    ref = run_projection(sc, {"Residual": 1e-10, "Preconditioner": "jacobi"}, want_system=False)
+    r = g["result"]
+    log(f"[gpu]   status={r['status_name']} count={r['count']} outer={r['outer']} fallback={r['fallback']} "
+        f"relres={r['true_relres_inf']:.2e}")
+    assert r["status"] == 0
+    assert r["true_relres_inf"] <= 1e-6
+    pr = ref["pressure"]
+    scale = np.abs(pr).max()
+    assert np.abs(g["pressure"] - pr).max() <= 1e-2 * scale
+
+
+def test_unet_config_errors(scenes, tmp_path):
+    import shiokaze_amd
+    from gpu_helpers import to_device
+    sc = scenes(1, 16, 0)
+    d = to_device(sc)
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config={"Preconditioner": "unet"}) as ps:
+        with pytest.raises(shiokaze_amd.ShkError):
+            ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+    missing = tmp_path / "missing.bin"
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=_scene_config(missing)) as ps:
+        with pytest.raises(shiokaze_amd.ShkError):
+            ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+
+
+def test_export_scaled_matches_system(scenes):
+    """b^ = D^-1/2 b, couplings c / sqrt(d_i d_j), x^ = D^1/2 x from the exported system."""
+    import shiokaze_amd
+    from gpu_helpers import to_device
+    sc = scenes(3, 32, 2)
+    d = to_device(sc)
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config={"Residual": 1e-8}) as ps:
+        ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+        s = {k: v.cpu().numpy().astype(np.float64) for k, v in ps.export_scaled().items()}
+        sy = {k: v.cpu().numpy() for k, v in ps.export_system().items()}
+    lab = sy["label"] != 0
+    np.testing.assert_array_equal(s["mask"], lab.astype(np.float64))
+    dg = np.where(lab, sy["diag"], 1.0)
+    sig = np.where(lab, 1.0 / np.sqrt(dg), 0.0)
+    np.testing.assert_allclose(s["rhat"], (sig * sy["rhs"]).astype(np.float32), rtol=1e-6, atol=0)
+    np.testing.assert_allclose(s["xhat"], np.where(lab, sy["x"] / np.where(lab, sig, 1.0), 0.0).astype(np.float32),
+                               rtol=1e-6, atol=0)
+    for key, ax in (("hx", 2), ("hy", 1), ("hz", 0)):
+        lo = np.roll(sig, 1, axis=ax)
+        sl = [slice(None)] * 3
+        sl[ax] = 0
+        lo[tuple(sl)] = 0.0
+        c = sy["c" + key[1]]
+        np.testing.assert_allclose(s[key], (c * sig * lo).astype(np.float32), rtol=1e-6, atol=1e-30)
