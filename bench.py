@@ -70,6 +70,8 @@ def main():
                     help="steps after the timed ones with per-kernel HIP event timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a known-size kernel for HBM counter scaling)")
+    ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128.bin"),
+                    help="trained network for the learned-preconditioner line of --extra")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")
     args = ap.parse_args()
@@ -111,6 +113,8 @@ def main():
         return ps.project(1.0, vel, dev["solid"], svel, dev["fluid"])
 
     cfg = {"Residual": args.residual, "Precision": args.precision, "Preconditioner": args.preconditioner}
+    if args.preconditioner == "unet":
+        cfg["UnetWeights"] = args.unet_weights
     for kv in args.set:
         k, v = kv.split("=", 1)
         cfg[k] = v
@@ -207,7 +211,32 @@ def main():
             t1 = time.perf_counter()
             rj = step(pj)
             extra["fp64_jacobi_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rj["count"]}
+            extra["mixed_jacobi_vs_fp64_plain_cg"] = round(value / extra["fp64_plain_cg"]["solves_per_s"], 2)
             pj.close()
+        if args.extra and Path(args.unet_weights).exists():
+            # learned preconditioner: flexible CG with the trained network (plain PCG fallback)
+            from shiokaze_amd.learned import flops_per_apply
+            pu = make({"Residual": args.residual, "Preconditioner": "unet", "UnetWeights": args.unet_weights})
+            step(pu)
+            t1 = time.perf_counter()
+            ru = step(pu)
+            t_solve = time.perf_counter() - t1
+            pu.configure("ProfileKernels", 1)
+            step(pu)
+            cnt, ms = pu.kernel_stats("unet_apply")
+            entry = {"solves_per_s": 1.0 / t_solve, "iters": ru["count"], "outer": ru["outer"],
+                     "fallback": ru["fallback"], "true_relres_inf": ru["true_relres_inf"],
+                     "weights": Path(args.unet_weights).name}
+            if cnt:
+                fl = flops_per_apply((n, n, n))
+                tf = fl / (ms / cnt * 1e-3) / 1e12
+                entry["net_apply_ms"] = round(ms / cnt, 4)
+                entry["mfma"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": round(tf / 2500.0, 4), "flops_per_apply": fl}
+            if "fp64_plain_cg" in extra:
+                entry["vs_fp64_plain_cg"] = round(entry["solves_per_s"] / extra["fp64_plain_cg"]["solves_per_s"], 2)
+            extra["learned_fcg"] = entry
+            pu.close()
         log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")
 
     cpu = None

@@ -195,9 +195,12 @@ int shk_unet_save(shk_unet *net, const char *path);
 int shk_unet_load(shk_unet *net, const char *path);
 
 /* One training sample: the A^-norm error loss of M(r) against xstar = A^-1 r,
- * L = (M(r) - xstar)' A^ (M(r) - xstar) / (xstar' r), with
- * M(r) = mask * (omega r + s P U(R r / s, ...)).  Gradients accumulate; with
- * lr > 0 an Adam step is taken and the gradients are cleared. */
+ * L = (M(r) - xstar)' A^ (M(r) - xstar) / (xstar' r), where
+ *   z0 = mask * s * P U(R r / s, R mask, R cx, R cy, R cz)   (R: 2x2x2 average,
+ *        P: cell-centred trilinear prolongation, s: RMS of r over the mask)
+ *   z_{i+1} = z_i + omega * mask * (r - A^ z_i), two steps;  M(r) = z_2.
+ * Gradients accumulate; with lr > 0 an Adam step is taken and the gradients are
+ * cleared.  omega is kept with the network and saved with its weights. */
 int shk_unet_train_step(shk_unet *net, int nx, int ny, int nz, const float *r, const float *mask, const float *cx,
                         const float *cy, const float *cz, const float *xstar, double omega, float lr,
                         double *loss);

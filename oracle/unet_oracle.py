@@ -215,13 +215,54 @@ def backward(c, du, bf16=False):
     return np.concatenate([np.concatenate([dw.ravel(), db]) for dw, db in grads])
 
 
+SMOOTH_STEPS = 2  # weighted-Jacobi steps after the coarse correction
+
+
+def _taps_1d(n_fine):
+    """Per fine index x: the two coarse taps (x >> 1 with weight 3/4, and its clamped
+    neighbour on x's side with weight 1/4) of cell-centred linear interpolation."""
+    x = np.arange(n_fine)
+    c = n_fine // 2
+    t0 = x >> 1
+    t1 = np.clip(t0 + np.where(x & 1, 1, -1), 0, c - 1)
+    return t0, t1
+
+
+def prolong(u):
+    """Cell-centred trilinear prolongation of a coarse grid (nz, ny, nx) to 2x, with
+    clamped indexing at the box edges (a constant stays constant)."""
+    nz, ny, nx = u.shape
+    out = np.zeros((2 * nz, 2 * ny, 2 * nx))
+    tz, ty, tx = _taps_1d(2 * nz), _taps_1d(2 * ny), _taps_1d(2 * nx)
+    for a, wa in ((0, 0.75), (1, 0.25)):
+        for b, wb in ((0, 0.75), (1, 0.25)):
+            for c, wc in ((0, 0.75), (1, 0.25)):
+                out += wa * wb * wc * u[np.ix_(tz[a], ty[b], tx[c])]
+    return out
+
+
+def prolong_T(g):
+    """Adjoint of prolong (the fine-to-coarse transpose)."""
+    nz, ny, nx = (s // 2 for s in g.shape)
+    out = np.zeros((nz, ny, nx))
+    tz, ty, tx = _taps_1d(2 * nz), _taps_1d(2 * ny), _taps_1d(2 * nx)
+    for a, wa in ((0, 0.75), (1, 0.25)):
+        for b, wb in ((0, 0.75), (1, 0.25)):
+            for c, wc in ((0, 0.75), (1, 0.25)):
+                np.add.at(out, np.ix_(tz[a], ty[b], tx[c]), wa * wb * wc * g)
+    return out
+
+
 def precondition(p, r, mask, cx, cy, cz, omega, bf16=False, rms=None):
-    """z = mask * (omega r + s P U(R r / s, R mask, R cx, R cy, R cz)), s = RMS of r over
-    the mask (or the given rms).  Returns (z, cache, s)."""
+    """z = M(r): z0 = mask * s * P U(R r / s, R mask, R cx, R cy, R cz), then SMOOTH_STEPS
+    steps z += omega * mask * (r - A z); s = RMS of r over the mask (or the given rms).
+    Returns (z, cache, s)."""
     s = rms_over_mask(r, mask) if rms is None else rms
     s = s if s > 0 else 1.0
     u, cache = forward(p, features(r, mask, cx, cy, cz, 1.0 / s), bf16)
-    z = mask * (omega * r + s * upsample(u))
+    z = mask * (s * prolong(u))
+    for _ in range(SMOOTH_STEPS):
+        z = z + omega * mask * (r - stencil(cx, cy, cz, z))
     return z, cache, s
 
 
@@ -235,5 +276,7 @@ def loss_and_grad(p, r, mask, cx, cy, cz, xstar, omega, bf16=False, need_grad=Tr
     if not need_grad:
         return loss, None, z
     gz = 2.0 * ae / den
-    du = s * upsample_bwd(mask * gz)
+    for _ in range(SMOOTH_STEPS):  # z_{i+1} = z_i + w M (r - A z_i):  g_i = g_{i+1} - w A (M g_{i+1})
+        gz = gz - omega * stencil(cx, cy, cz, mask * gz)
+    du = s * prolong_T(mask * gz)
     return loss, backward(cache, du, bf16), z

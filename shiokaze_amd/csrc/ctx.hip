@@ -407,6 +407,8 @@ void shk_destroy(shk_ctx *c) {
     if (c->h_misc) (void)hipHostFree(c->h_misc);
     for (auto &ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto &ev : c->uev)
+        if (ev) (void)hipEventDestroy(ev);
     for (auto &ev : c->pev)
         if (ev) (void)hipEventDestroy(ev);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -492,6 +494,8 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
             c->pcount[q] = 0;
             c->pms[q] = 0.0;
         }
+        c->ucount = 0;
+        c->ums = 0.0;
     } else if (k == "UnetWeights") {
         p.unet_weights = value;
         if (c->unet) {
@@ -629,11 +633,16 @@ int shk_kernel_stats(shk_ctx *c, const char *kernel, int64_t *launches, double *
     if (!c || !kernel || !launches || !total_ms) return SHK_BAD_ARG;
     const std::string k(kernel);
     static const char *names[shk_ctx::kKernels] = {"axpy_norm", "spmv_dot"};
+    if (k == "unet_apply") {
+        *launches = c->ucount;
+        *total_ms = c->ums;
+        return SHK_OK;
+    }
     int idx = -1;
     for (int q = 0; q < shk_ctx::kKernels; ++q)
         if (k == names[q]) idx = q;
     if (idx < 0)
-        return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (axpy_norm, spmv_dot)", kernel);
+        return c->fail(SHK_BAD_ARG, "unknown kernel '%s' (axpy_norm, spmv_dot, unet_apply)", kernel);
     *launches = c->pcount[idx];
     *total_ms = c->pms[idx];
     return SHK_OK;

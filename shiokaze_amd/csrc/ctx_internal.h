@@ -124,6 +124,10 @@ struct shk_ctx {
     static constexpr int kKernels = 2;  // launches per Krylov iteration: axpy_norm, dir_spmv
     int64_t pcount[kKernels] = {0, 0};
     double pms[kKernels] = {0.0, 0.0};
+    // network applications inside flexible CG (ProfileKernels=1)
+    hipEvent_t uev[2] = {};
+    int64_t ucount = 0;
+    double ums = 0.0;
     // records the next profiling event after a Krylov kernel launch
     void mark() {
         if (profile && pcur < pev.size()) (void)hipEventRecord(pev[pcur++], stream);

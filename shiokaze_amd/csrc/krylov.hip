@@ -175,19 +175,34 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
     if (red.st->done) return;
-    double beta;
-    if (!direction_step(red, lds, &bc, beta)) return;
-    const T b = (T)beta;
     int t0, t1;
     chunk_range(op.n_active, per, t0, t1);
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int i = lane & 7, j = lane >> 3;
+    // brick metadata (neighbour slots, uniform coupling) does not depend on beta: the
+    // first brick's is loaded before the fold, each next one while the current brick runs
+    int slot = t0 + wave;
+    int mnb[6] = {-1, -1, -1, -1, -1, -1};
+    C mkap = C(0);
+    if (slot < t1) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) mnb[f] = op.nbr[6 * slot + f];
+        if (UNIT && op.kappa) mkap = op.kappa[slot];
+    }
+    double beta;
+    if (!direction_step(red, lds, &bc, beta)) return;
+    const T b = (T)beta;
     double acc = 0.0;
-    for (int slot = t0 + wave; slot < t1; slot += NB / kWave) {
-        const int *nb6 = op.nbr + 6 * slot;
-        const int nxm = nb6[0], nxp = nb6[1], nym = nb6[2], nyp = nb6[3], nzm = nb6[4], nzp = nb6[5];
+    for (; slot < t1; slot += NB / kWave) {
+        const int nxm = mnb[0], nxp = mnb[1], nym = mnb[2], nyp = mnb[3], nzm = mnb[4], nzp = mnb[5];
+        const C kap = mkap;
+        const int nslot = slot + NB / kWave;
+        if (nslot < t1) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) mnb[f] = op.nbr[6 * nslot + f];
+            if (UNIT && op.kappa) mkap = op.kappa[nslot];
+        }
         const int64_t base = (int64_t)slot * NT + lane;
-        const C kap = (UNIT && op.kappa) ? op.kappa[slot] : C(0);
         if (UNIT && kap != C(0)) {
             // uniform brick: one coupling value, no coefficient traffic
             const T kv = (T)kap;

@@ -3,9 +3,12 @@
 // and backward kernels, applied inside flexible CG with plain PCG as fallback.
 //
 // Preconditioner, on the Jacobi-scaled system A^ = D^-1/2 A D^-1/2 (unit diagonal):
-//   M(r) = mask * (omega * r + s * P U(R(r) / s, R(mask), R(c^x), R(c^y), R(c^z)))
-// R: 2x2x2 average to the half-resolution grid, P: nearest prolongation back,
-// s: the RMS of r over the unknowns (M is positively homogeneous), U: the U-Net.
+//   z0 = mask * s * P U(R(r) / s, R(mask), R(c^x), R(c^y), R(c^z))
+//   z_{i+1} = z_i + omega * mask * (r - A^ z_i),  i < kSmooth;   M(r) = z_kSmooth
+// R: 2x2x2 average to the half-resolution grid, P: cell-centred trilinear
+// prolongation back, s: the RMS of r over the unknowns (M is positively
+// homogeneous), U: the U-Net; the weighted-Jacobi steps remove the high-frequency
+// error that a half-resolution correction cannot represent.
 // Training minimises the A^-norm error of M(b^) against the label x^* = A^-1 b^
 // computed by the plain solver: L = (M(b^) - x^*)' A^ (M(b^) - x^*) / (x^*' b^).
 //
@@ -43,7 +46,8 @@ constexpr int kLayers = 8;
 const LayerSpec kSpec[kLayers] = {{8, 16, true, false},  {16, 32, true, true},  {32, 64, true, true},
                                   {64, 128, true, true}, {192, 64, true, true}, {96, 32, true, true},
                                   {48, 16, true, true},  {16, 16, false, true}};
-constexpr uint64_t kMagic = 0x314E45554B485353ull;  // "SSHKUEN1"
+constexpr uint64_t kMagic = 0x324E45554B485353ull;  // "SSHKUEN2"
+constexpr int kSmooth = 2;                          // weighted-Jacobi steps after the coarse correction
 
 template <class T>
 hipError_t dalloc(T *&p, size_t n) {
@@ -174,6 +178,7 @@ struct UnetNet {
     int64_t woff[kLayers], boff[kLayers];
     float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     int64_t step = 0;
+    double omega = 0.8;  // smoothing weight the network was trained with (saved with it)
     bf16 *pk[kLayers] = {}, *pkt[kLayers] = {};
     bool packed_dirty = true;
     // activations for the current half-resolution grid
@@ -187,7 +192,7 @@ struct UnetNet {
     float *u = nullptr, *du = nullptr;
     // fine-level training work arrays
     int64_t fine_cap = 0;
-    float *fz = nullptr, *fe = nullptr, *fae = nullptr;
+    float *fz = nullptr, *fe = nullptr, *fae = nullptr, *ftmp = nullptr;
     double *part = nullptr, *h_part = nullptr;
     std::string err;
 };
@@ -246,6 +251,7 @@ int ensure_fine(UnetNet *n, int64_t nf) {
     UHIP(n, dalloc(n->fz, (size_t)nf));
     UHIP(n, dalloc(n->fe, (size_t)nf));
     UHIP(n, dalloc(n->fae, (size_t)nf));
+    UHIP(n, dalloc(n->ftmp, (size_t)nf));
     n->fine_cap = nf;
     return SHK_OK;
 }
@@ -350,11 +356,16 @@ int precondition(UnetNet *n, Grid3 fine, const float *r, const float *mask, cons
     int rc;
     const Grid3 m{fine.nx / 2, fine.ny / 2, fine.nz / 2};
     if ((rc = ensure_grid(n, m))) return rc;
+    if ((rc = ensure_fine(n, fine.vox()))) return rc;
     if ((rc = repack(n))) return rc;
     const double s = rms > 0 ? rms : 1.0;
     launch_features(n->stream, m, r, 1.0 / s, mask, cx, cy, cz, n->feat);
     forward(n);
-    launch_prolong(n->stream, fine, n->u, s, omega, r, mask, z);
+    launch_prolong(n->stream, fine, n->u, s, mask, z);
+    for (int i = 0; i < kSmooth; ++i) {
+        launch_dense_stencil(n->stream, fine, cx, cy, cz, z, n->ftmp);
+        launch_smooth(n->stream, fine.vox(), omega, r, mask, n->ftmp, z);
+    }
     UHIP(n, hipGetLastError());
     return SHK_OK;
 }
@@ -382,6 +393,7 @@ void unet_net_destroy(UnetNet *n) {
     dfree(n->fz);
     dfree(n->fe);
     dfree(n->fae);
+    dfree(n->ftmp);
     dfree(n->part);
     if (n->h_part) (void)hipHostFree(n->h_part);
     delete n;
@@ -437,6 +449,7 @@ int unet_net_train(UnetNet *n, Grid3 fine, const float *r, const float *mask, co
     int rc;
     const int64_t nf = fine.vox();
     if ((rc = ensure_fine(n, nf))) return rc;
+    n->omega = omega;
     double d3[3];
     // s = RMS of r over unknowns; den = x*' b
     if ((rc = reduce3(n, nf, r, r, xstar, r, mask, d3))) return rc;
@@ -451,8 +464,14 @@ int unet_net_train(UnetNet *n, Grid3 fine, const float *r, const float *mask, co
     launch_dense_stencil(n->stream, fine, cx, cy, cz, n->fe, n->fae);
     if ((rc = reduce3(n, nf, n->fe, n->fae, nullptr, nullptr, nullptr, d3))) return rc;
     *loss = d3[0] / den;
-    // dL/dz = 2 A e / den, then through the prolongation (masked, times s)
+    // dL/dz = 2 A e / den; through the smoothing steps g <- g - omega A (mask g);
+    // then through the prolongation (masked, times s)
     hipLaunchKernelGGL(k_scale_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, (float)(2.0 / den), n->fae);
+    for (int i = 0; i < kSmooth; ++i) {
+        launch_mask_mul(n->stream, nf, mask, n->fae, n->fe);
+        launch_dense_stencil(n->stream, fine, cx, cy, cz, n->fe, n->ftmp);
+        hipLaunchKernelGGL(k_axpy_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, (float)-omega, n->ftmp, n->fae);
+    }
     launch_prolong_bwd(n->stream, n->L[0], n->fae, mask, rms, n->du);
     backward(n);
     if (lr > 0.0f) {
@@ -490,7 +509,8 @@ int unet_net_save(UnetNet *n, const char *path) {
         const int32_t io[2] = {kSpec[l].cin, kSpec[l].cout};
         ok = fwrite(io, 4, 2, f) == 2;
     }
-    ok = ok && fwrite(&n->nparam, 8, 1, f) == 1 && fwrite(h.data(), 4, n->nparam, f) == (size_t)n->nparam;
+    ok = ok && fwrite(&n->nparam, 8, 1, f) == 1 && fwrite(&n->omega, 8, 1, f) == 1 &&
+         fwrite(h.data(), 4, n->nparam, f) == (size_t)n->nparam;
     fclose(f);
     return ok ? SHK_OK : fail(n, SHK_BAD_ARG, "short write of weights file");
 }
@@ -506,12 +526,15 @@ int unet_net_load(UnetNet *n, const char *path) {
         ok = fread(io, 4, 2, f) == 2 && io[0] == kSpec[l].cin && io[1] == kSpec[l].cout;
     }
     int64_t np = 0;
-    ok = ok && fread(&np, 8, 1, f) == 1 && np == n->nparam;
+    double omega = 0.0;
+    ok = ok && fread(&np, 8, 1, f) == 1 && np == n->nparam && fread(&omega, 8, 1, f) == 1 && omega >= 0.0 &&
+         omega <= 2.0;
     std::vector<float> h(ok ? n->nparam : 0);
     ok = ok && fread(h.data(), 4, n->nparam, f) == (size_t)n->nparam;
     fclose(f);
     if (!ok) return fail(n, SHK_BAD_ARG, "weights file does not match this network");
     UHIP(n, hipMemcpy(n->param, h.data(), sizeof(float) * n->nparam, hipMemcpyHostToDevice));
+    n->omega = omega;
     n->packed_dirty = true;
     return SHK_OK;
 }
@@ -566,6 +589,27 @@ int ensure_precond(shk_ctx *c) {
 
 }  // namespace
 
+// z = M(r) inside flexible CG; with ProfileKernels=1 the application is timed with
+// events on the context's stream (read back at the next stream synchronisation).
+int timed_precondition(shk_ctx *c, UnetNet *net, Grid3 P, UnetPrecond *u, double omega, double rms) {
+    if (c->profile && !c->uev[0]) {
+        HIPC(c, hipEventCreate(&c->uev[0]));
+        HIPC(c, hipEventCreate(&c->uev[1]));
+    }
+    if (c->profile) HIPC(c, hipEventRecord(c->uev[0], c->stream));
+    const int rc = precondition(net, P, u->r, u->mask, u->hx, u->hy, u->hz, omega, rms, u->z);
+    if (rc) return rc;
+    if (c->profile) {
+        HIPC(c, hipEventRecord(c->uev[1], c->stream));
+        HIPC(c, hipEventSynchronize(c->uev[1]));
+        float ms = 0.0f;
+        HIPC(c, hipEventElapsedTime(&ms, c->uev[0], c->uev[1]));
+        c->ucount += 1;
+        c->ums += ms;
+    }
+    return SHK_OK;
+}
+
 // Learned flexible CG (FCG(1), Notay 2000) on the dense padded scaled system in
 // FP32 with FP64 dots, inside the same FP64 refinement loop as the mixed solver.
 // Falls back to plain Jacobi-PCG inner solves when an inner FCG stalls or hits
@@ -579,7 +623,7 @@ int unet_solve(shk_ctx *c, shk_result *res) {
     const int64_t np = P.vox();
     const Dims d = c->dims;
     hipStream_t st = c->stream;
-    const double omega = 1.0;
+    const double omega = net->omega;
     // dense scaled operator and mask
     auto to_dense = [&](const float *src, float *dst) {
         hipLaunchKernelGGL((k_brick_to_dense<float, float>), dim3(g1(np)), dim3(256), 0, st, d.nx, d.ny, d.nz, P.nx,
@@ -617,7 +661,7 @@ int unet_solve(shk_ctx *c, shk_result *res) {
             if ((rc = reduce3(net, np, u->r, u->r, nullptr, nullptr, u->r, v3))) return c->fail(rc, "%s", net->err.c_str());
             const double r0 = v3[2];
             double rn = r0;
-            if ((rc = precondition(net, P, u->r, u->mask, u->hx, u->hy, u->hz, omega, std::sqrt(v3[0] / nunk), u->z)))
+            if ((rc = timed_precondition(c, net, P, u, omega, std::sqrt(v3[0] / nunk))))
                 return c->fail(rc, "%s", net->err.c_str());
             HIPC(c, hipMemcpyAsync(u->p, u->z, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
             int64_t it = 0;
@@ -650,8 +694,7 @@ int unet_solve(shk_ctx *c, shk_result *res) {
                     break;
                 }
                 if (rn <= inner_rel * r0) break;
-                if ((rc = precondition(net, P, u->r, u->mask, u->hx, u->hy, u->hz, omega, std::sqrt(v3[0] / nunk),
-                                       u->z)))
+                if ((rc = timed_precondition(c, net, P, u, omega, std::sqrt(v3[0] / nunk))))
                     return c->fail(rc, "%s", net->err.c_str());
                 if ((rc = reduce3(net, np, u->z, u->q, nullptr, nullptr, nullptr, v3))) return c->fail(rc, "%s", net->err.c_str());
                 const double beta = -v3[0] / delta;

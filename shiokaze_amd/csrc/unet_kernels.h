@@ -3,8 +3,8 @@
 // Activations are dense channels-last BF16 grids: element (voxel v, channel c) at
 // v * C + c, voxel v = x + nx*(y + ny*z).  Convolutions are 3x3x3, stride 1, zero
 // padding 1, computed as implicit GEMMs on v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation.  The reduction index k = tap * Cin + ci runs over the 27 taps
-// (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) and input channels; Cin is a multiple of 8.
+// accumulation over the 27 taps (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)) and the input
+// channels; Cin is a multiple of 8.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,14 +21,24 @@ struct Grid3 {
     __host__ __device__ int64_t vox() const { return (int64_t)nx * ny * nz; }
 };
 
-// K chunks of 32 of a 3x3x3 convolution with cin input channels.
-inline int conv_kchunks(int cin) { return (27 * cin + 31) / 32; }
+// The convolution stages its input through LDS CC channels at a time: CC is the
+// largest divisor of cin that is a multiple of 8 and at most 32.  The reduction
+// index runs stage by stage, k = (stage, tap, c), padded to whole chunks of 32.
+inline int conv_cc(int cin) {
+    for (int c = 32; c >= 8; c -= 8)
+        if (cin % c == 0) return c;
+    return 8;
+}
+// K chunks of 32 per stage, and in total
+inline int conv_kpc(int cin) { return (27 * conv_cc(cin) + 31) / 32; }
+inline int conv_kchunks(int cin) { return (cin / conv_cc(cin)) * conv_kpc(cin); }
 // Packed weight elements (bf16) for cout outputs: cout/16 M-tiles x kchunks x 64 lanes x 8.
 inline int64_t conv_packed_elems(int cin, int cout) { return (int64_t)(cout / 16) * conv_kchunks(cin) * 64 * 8; }
 
 // Packs FP32 master weights W[tap][ci][co] into the MFMA A-fragment order:
-// transposed = false: forward (A[m = co][k = tap*cin + ci]);
-// transposed = true : input gradient (A[m = ci][k = tap'*cout + co] = W[26 - tap'][ci][co]).
+// transposed = false: forward (A[m = co][k = (stage, tap, c)], ci = stage*CC + c);
+// transposed = true : input gradient (A[m = ci][k = (stage, tap', c)] = W[26 - tap'][ci][co],
+//                     co = stage*CC + c, CC = conv_cc(cout)).
 void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool transposed, bf16 *packed);
 
 // y[v][co] = act(bias[co] + sum_k w * x); act = ReLU when relu.  out_f32 (nullable)
@@ -63,11 +73,16 @@ void launch_add_bf16(hipStream_t st, int64_t n, const bf16 *a, const bf16 *b, bf
 //   [R(r)/scale, R(mask), R(cx), R(cy), R(cz), 0, 0, 0] with R = 2x2x2 average.
 void launch_features(hipStream_t st, Grid3 coarse, const float *r, double inv_scale, const float *mask,
                      const float *cx, const float *cy, const float *cz, bf16 *feat);
-// z[v] = mask[v] * (omega * r[v] + scale * u[parent(v)]) on the fine grid (P = nearest).
-void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, double omega, const float *r,
-                    const float *mask, float *z);
-// grad_u[V] = sum over the 8 children of mask * g (P^T), times scale.
+// z = mask * scale * P u on the fine grid, P = cell-centred trilinear interpolation
+// (weights 3/4, 1/4 per axis, clamped at the box edges).
+void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, const float *mask, float *z);
+// grad_u = scale * P^T (mask * g).
 void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale, float *grad_u);
+// z += omega * mask * (r - t): one weighted-Jacobi step given t = A z.
+void launch_smooth(hipStream_t st, int64_t n, double omega, const float *r, const float *mask, const float *t,
+                   float *z);
+// out = mask * g.
+void launch_mask_mul(hipStream_t st, int64_t n, const float *mask, const float *g, float *out);
 
 // Dense 7-point unit-diagonal stencil: y = x - sum c_face * x_nbr (couplings at
 // the high cell of each face, zero on faces touching non-unknowns).

@@ -47,92 +47,126 @@ inline unsigned grid1(int64_t n, int nt) {
 
 // ---------------------------------------------------------------- weight packing
 
-__global__ void k_conv_pack(const float *__restrict__ w, int cin, int cout, int transposed, int kchunks,
-                            uint16_t *__restrict__ packed, int64_t total) {
+__global__ void k_conv_pack(const float *__restrict__ w, int cin, int cout, int transposed, int cc, int kpc,
+                            int kchunks, uint16_t *__restrict__ packed, int64_t total) {
     const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (idx >= total) return;
     const int j = (int)(idx & 7);
     const int lane = (int)((idx >> 3) & 63);
     const int64_t rest = idx >> 9;
-    const int kc = (int)(rest % kchunks);
+    const int kcg = (int)(rest % kchunks);
     const int mt = (int)(rest / kchunks);
+    const int stage = kcg / kpc, kc = kcg - stage * kpc;
     const int m = mt * 16 + (lane & 15);
-    const int k = kc * 32 + 8 * (lane >> 4) + j;
+    const int k = kc * 32 + 8 * (lane >> 4) + j;  // within the stage
     float val = 0.0f;
-    if (!transposed) {
-        // A[m = co][k = tap * cin + ci]
-        if (k < 27 * cin) {
-            const int tap = k / cin, ci = k - tap * cin;
-            val = w[((int64_t)tap * cin + ci) * cout + m];
-        }
-    } else {
-        // A[m = ci][k = tap' * cout + co] = W[26 - tap'][ci][co]
-        if (k < 27 * cout) {
-            const int tp = k / cout, co = k - tp * cout;
-            val = w[((int64_t)(26 - tp) * cin + m) * cout + co];
-        }
+    if (k < 27 * cc) {
+        const int tap = k / cc, c = stage * cc + (k - tap * cc);
+        if (!transposed)
+            val = w[((int64_t)tap * cin + c) * cout + m];             // A[co = m][ci = c]
+        else
+            val = w[((int64_t)(26 - tap) * cin + m) * cout + c];      // A[ci = m][co = c], flipped tap
     }
     packed[idx] = bf16_bits(val);
 }
 
 // ---------------------------------------------------------------- forward
 
-template <int MT>
-__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ x, int cin,
+// Block: 256 threads (4 waves) computing a 16 x 4 x 4 voxel tile (x, y, z) for MTB
+// of the MT = cout/16 output tiles (blockIdx.z = z-tile * MT/MTB + output group; small
+// grids split the outputs over blocks to fill the chip).  The 18 x 6 x 6 input halo
+// of the tile is staged in LDS CC channels at a time (one 16-byte row per voxel and 8
+// channels, padded by 16 bytes per voxel).  Wave w owns the tile's z-plane w: 4 rows
+// of 16 voxels = 4 MFMA N-tiles, times MTB M-tiles of 16 outputs, so each A fragment
+// (weights, from L1/L2) feeds 4 MFMAs and each B fragment (inputs, from LDS) MTB.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;
+
+template <int MTB, int CC>
+__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total,
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
                                                   int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32) {
-    const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int64_t nvox = g.vox();
-    const int64_t v0 = tile * 16;
-    if (v0 >= nvox) return;
+    constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
+    constexpr int VS = G8 + 1;        // voxel stride in uint4 (one group of padding)
+    constexpr int KPC = (27 * CC + 31) / 32;
+    __shared__ uint4 lds[kHalo * VS];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
-    const int64_t vb = v0 + n;
-    const bool valid = vb < nvox;
-    const int vx = (int)(vb % g.nx), vy = (int)((vb / g.nx) % g.ny), vz = (int)(vb / ((int64_t)g.nx * g.ny));
-    const int kchunks = (27 * cin + 31) / 32;
-    f32x4 acc[MT];
+    const int mgroups = mt_total / MTB;
+    const int ztile = blockIdx.z / mgroups, mg = blockIdx.z - ztile * mgroups;
+    const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = ztile * kTZ;
+    const int stages = cin / CC;
+    const int kchunks = stages * KPC;
+    f32x4 acc[MTB][4];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) acc[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kc = 0; kc < kchunks; ++kc) {
-        const int k0 = kc * 32 + kq * 8;
-        Frag b;
-        b.u = make_uint4(0u, 0u, 0u, 0u);
-        if (valid && k0 < 27 * cin) {
-            const int tap = k0 / cin, ci0 = k0 - tap * cin;
-            int dx, dy, dz;
-            tap_offset(tap, dx, dy, dz);
-            const int xx = vx + dx, yy = vy + dy, zz = vz + dz;
-            if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz) {
-                const int64_t src = ((int64_t)zz * g.ny + yy) * g.nx + xx;
-                b.u = *reinterpret_cast<const uint4 *>(x + src * cin + ci0);
+    for (int mt = 0; mt < MTB; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int stage = 0; stage < stages; ++stage) {
+        __syncthreads();  // the previous stage's reads are done
+        for (int idx = threadIdx.x; idx < kHalo * G8; idx += 256) {
+            const int vox = idx / G8, part = idx - vox * G8;
+            const int hx = vox % kHX, hy = (vox / kHX) % kHY, hz = vox / (kHX * kHY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
+                v = *reinterpret_cast<const uint4 *>(x + (((int64_t)gz * g.ny + gy) * g.nx + gx) * cin + stage * CC +
+                                                     part * 8);
+            lds[vox * VS + part] = v;
+        }
+        __syncthreads();
+        for (int kc = 0; kc < KPC; ++kc) {
+            const int k0 = kc * 32 + kq * 8;
+            Frag b[4];
+            if (k0 < 27 * CC) {
+                const int tap = k0 / CC, grp = (k0 - tap * CC) / 8;
+                int dx, dy, dz;
+                tap_offset(tap, dx, dy, dz);
+                const int base = ((w + dz + 1) * kHY + (dy + 1)) * kHX + (n + dx + 1);
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) b[nt].u = lds[(base + nt * kHX) * VS + grp];
+            } else {
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt) b[nt].u = make_uint4(0u, 0u, 0u, 0u);
+            }
+            const int kcg = stage * KPC + kc;
+#pragma unroll
+            for (int mt = 0; mt < MTB; ++mt) {
+                Frag a;
+                a.u = packed[((int64_t)(mg * MTB + mt) * kchunks + kcg) * 64 + lane];
+#pragma unroll
+                for (int nt = 0; nt < 4; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b[nt].v, acc[mt][nt], 0, 0, 0);
             }
         }
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            Frag a;
-            a.u = packed[((int64_t)mt * kchunks + kc) * 64 + lane];
-            acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[mt], 0, 0, 0);
-        }
     }
-    if (!valid) return;
-    const int cout = MT * 16;
+    // lane holds D[co = (mg*MTB + mt)*16 + 4*kq + j][voxel n of row nt]
+    const int cout = mt_total * 16;
+    const int gx = x0 + n, gz = z0 + w;
+    if (gx >= g.nx || gz >= g.nz) return;
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int co = mt * 16 + 4 * kq;
-        float o[4];
+    for (int nt = 0; nt < 4; ++nt) {
+        const int gy = y0 + nt;
+        if (gy >= g.ny) continue;
+        const int64_t vb = ((int64_t)gz * g.ny + gy) * g.nx + gx;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            o[j] = acc[mt][j] + (bias ? bias[co + j] : 0.0f);
-            if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
-        }
-        if (out_f32) {
-            if (co == 0) out_f32[vb] = o[0];
-        } else {
-            uint2 packed_out;
-            packed_out.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
-            packed_out.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
-            *reinterpret_cast<uint2 *>(y + vb * cout + co) = packed_out;
+        for (int mt = 0; mt < MTB; ++mt) {
+            const int co = (mg * MTB + mt) * 16 + 4 * kq;
+            float o[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                o[j] = acc[mt][nt][j] + (bias ? bias[co + j] : 0.0f);
+                if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
+            }
+            if (out_f32) {
+                if (co == 0) out_f32[vb] = o[0];
+            } else {
+                uint2 po;
+                po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+                po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+                *reinterpret_cast<uint2 *>(y + vb * cout + co) = po;
+            }
         }
     }
 }
@@ -320,29 +354,84 @@ __global__ void k_features(Grid3 c, const float *__restrict__ r, double inv_scal
     }
 }
 
-__global__ void k_prolong(Grid3 f, const float *__restrict__ u, double scale, double omega,
-                          const float *__restrict__ r, const float *__restrict__ mask, float *__restrict__ z) {
+// Cell-centred linear interpolation taps along one axis: fine index x has coarse taps
+// x >> 1 (weight 3/4) and its neighbour on x's side (weight 1/4), clamped to [0, c).
+__device__ inline void taps1(int x, int c, int &t0, int &t1) {
+    t0 = x >> 1;
+    t1 = t0 + ((x & 1) ? 1 : -1);
+    t1 = t1 < 0 ? 0 : (t1 >= c ? c - 1 : t1);
+}
+// weight of coarse index X in the interpolation of fine index x
+__device__ inline double tapw(int x, int X, int c) {
+    int t0, t1;
+    taps1(x, c, t0, t1);
+    return (t0 == X ? 0.75 : 0.0) + (t1 == X ? 0.25 : 0.0);
+}
+
+__global__ void k_prolong(Grid3 f, const float *__restrict__ u, double scale, const float *__restrict__ mask,
+                          float *__restrict__ z) {
     const int64_t nv = f.vox();
-    const int cnx = f.nx / 2, cny = f.ny / 2;
+    const int cnx = f.nx / 2, cny = f.ny / 2, cnz = f.nz / 2;
     for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
         const int x = (int)(v % f.nx), y = (int)((v / f.nx) % f.ny), zc = (int)(v / ((int64_t)f.nx * f.ny));
-        const int64_t V = ((int64_t)(zc >> 1) * cny + (y >> 1)) * cnx + (x >> 1);
-        z[v] = mask[v] * (float)(omega * (double)r[v] + scale * (double)u[V]);
+        int X[2], Y[2], Z[2];
+        taps1(x, cnx, X[0], X[1]);
+        taps1(y, cny, Y[0], Y[1]);
+        taps1(zc, cnz, Z[0], Z[1]);
+        const double w[2] = {0.75, 0.25};
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+                    acc += w[c] * w[b] * w[a] * (double)u[((int64_t)Z[c] * cny + Y[b]) * cnx + X[a]];
+        z[v] = (float)((double)mask[v] * scale * acc);
     }
 }
 
+// grad_u[V] = scale * sum over the fine cells whose taps include V of weight * mask * g
 __global__ void k_prolong_bwd(Grid3 c, const float *__restrict__ g, const float *__restrict__ mask, double scale,
                               float *__restrict__ gu) {
     const int64_t nv = c.vox();
+    const int fx = 2 * c.nx, fy = 2 * c.ny, fz = 2 * c.nz;
     for (int64_t V = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; V < nv; V += (int64_t)gridDim.x * blockDim.x) {
         const int X = (int)(V % c.nx), Y = (int)((V / c.nx) % c.ny), Z = (int)(V / ((int64_t)c.nx * c.ny));
         double s = 0.0;
-        for (int a = 0; a < 8; ++a) {
-            const int64_t f = child(c, X, Y, Z, a);
-            s += (double)mask[f] * (double)g[f];
+        for (int zz = 2 * Z - 1; zz <= 2 * Z + 2; ++zz) {
+            if (zz < 0 || zz >= fz) continue;
+            const double wz = tapw(zz, Z, c.nz);
+            if (wz == 0.0) continue;
+            for (int yy = 2 * Y - 1; yy <= 2 * Y + 2; ++yy) {
+                if (yy < 0 || yy >= fy) continue;
+                const double wy = tapw(yy, Y, c.ny);
+                if (wy == 0.0) continue;
+                for (int xx = 2 * X - 1; xx <= 2 * X + 2; ++xx) {
+                    if (xx < 0 || xx >= fx) continue;
+                    const double wx = tapw(xx, X, c.nx);
+                    if (wx == 0.0) continue;
+                    const int64_t f = ((int64_t)zz * fy + yy) * fx + xx;
+                    s += wz * wy * wx * (double)mask[f] * (double)g[f];
+                }
+            }
         }
         gu[V] = (float)(scale * s);
     }
+}
+
+// z += omega * mask * (r - t)   (one weighted-Jacobi step with t = A z)
+__global__ void k_smooth(int64_t n, double omega, const float *__restrict__ r, const float *__restrict__ mask,
+                         const float *__restrict__ t, float *__restrict__ z) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        z[i] = (float)((double)z[i] + omega * (double)mask[i] * ((double)r[i] - (double)t[i]));
+}
+
+// out = mask * g
+__global__ void k_mask_mul(int64_t n, const float *__restrict__ mask, const float *__restrict__ g,
+                           float *__restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = mask[i] * g[i];
 }
 
 __global__ void k_dense_stencil(Grid3 g, const float *__restrict__ cx, const float *__restrict__ cy,
@@ -388,43 +477,64 @@ inline uint16_t *U(bf16 *p) { return reinterpret_cast<uint16_t *>(p); }
 // ---------------------------------------------------------------- launchers
 
 void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool transposed, bf16 *packed) {
-    const int kchunks = transposed ? conv_kchunks(cout) : conv_kchunks(cin);
+    const int gin = transposed ? cout : cin;  // GEMM reduction channels
+    const int kchunks = conv_kchunks(gin);
     const int mtiles = transposed ? cin / 16 : cout / 16;
     const int64_t total = (int64_t)mtiles * kchunks * 64 * 8;
     hipLaunchKernelGGL(k_conv_pack, dim3(grid1(total, 256)), dim3(256), 0, st, w, cin, cout, transposed ? 1 : 0,
-                       kchunks, U(packed), total);
+                       conv_cc(gin), conv_kpc(gin), kchunks, U(packed), total);
 }
+
+namespace {
+template <int MTB>
+void conv_fwd_mt(hipStream_t st, dim3 grid, Grid3 g, const bf16 *x, int cin, int mt, const uint4 *pk,
+                 const float *bias, bool relu, uint16_t *yy, float *out_f32) {
+    switch (conv_cc(cin)) {
+        case 8:
+            hipLaunchKernelGGL((k_conv_fwd<MTB, 8>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
+                               out_f32);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_conv_fwd<MTB, 16>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
+                               out_f32);
+            break;
+        case 24:
+            hipLaunchKernelGGL((k_conv_fwd<MTB, 24>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
+                               out_f32);
+            break;
+        default:
+            hipLaunchKernelGGL((k_conv_fwd<MTB, 32>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
+                               out_f32);
+            break;
+    }
+}
+}  // namespace
 
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
                      bool relu, bf16 *y, float *out_f32) {
-    const int64_t tiles = (g.vox() + 15) / 16;
-    const dim3 grid((unsigned)((tiles + 3) / 4));
+    const int mt = cout / 16;
+    const int bx = (g.nx + kTX - 1) / kTX, by = (g.ny + kTY - 1) / kTY, bz = (g.nz + kTZ - 1) / kTZ;
+    // output tiles per block: all of them, unless the grid is too small to fill the
+    // chip, then the largest divisor of mt that gives >= 1024 blocks (or 1)
+    const int64_t spatial = (int64_t)bx * by * bz;
+    int mtb = mt;
+    for (int d = mt; d >= 1; --d) {
+        if (mt % d) continue;
+        mtb = d;
+        if (spatial * (mt / d) >= 1024) break;
+    }
+    const dim3 grid((unsigned)bx, (unsigned)by, (unsigned)(bz * (mt / mtb)));
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     uint16_t *yy = y ? U(y) : nullptr;
-    switch (cout / 16) {
-        case 1:
-            hipLaunchKernelGGL(k_conv_fwd<1>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 2:
-            hipLaunchKernelGGL(k_conv_fwd<2>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 3:
-            hipLaunchKernelGGL(k_conv_fwd<3>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 4:
-            hipLaunchKernelGGL(k_conv_fwd<4>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 6:
-            hipLaunchKernelGGL(k_conv_fwd<6>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 8:
-            hipLaunchKernelGGL(k_conv_fwd<8>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        case 12:
-            hipLaunchKernelGGL(k_conv_fwd<12>, grid, dim3(256), 0, st, g, U(x), cin, pk, bias, relu, yy, out_f32);
-            break;
-        default:
-            break;  // unsupported widths are rejected by the caller
+    switch (mtb) {
+        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        default: break;  // unsupported widths are rejected by the caller
     }
 }
 
@@ -479,15 +589,23 @@ void launch_features(hipStream_t st, Grid3 coarse, const float *r, double inv_sc
                        cy, cz, U(feat));
 }
 
-void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, double omega, const float *r,
-                    const float *mask, float *z) {
-    hipLaunchKernelGGL(k_prolong, dim3(grid1(fine.vox(), 256)), dim3(256), 0, st, fine, u, scale, omega, r, mask, z);
+void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, const float *mask, float *z) {
+    hipLaunchKernelGGL(k_prolong, dim3(grid1(fine.vox(), 256)), dim3(256), 0, st, fine, u, scale, mask, z);
 }
 
 void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale,
                         float *grad_u) {
     hipLaunchKernelGGL(k_prolong_bwd, dim3(grid1(coarse.vox(), 256)), dim3(256), 0, st, coarse, g, mask, scale,
                        grad_u);
+}
+
+void launch_smooth(hipStream_t st, int64_t n, double omega, const float *r, const float *mask, const float *t,
+                   float *z) {
+    hipLaunchKernelGGL(k_smooth, dim3(grid1(n, 256)), dim3(256), 0, st, n, omega, r, mask, t, z);
+}
+
+void launch_mask_mul(hipStream_t st, int64_t n, const float *mask, const float *g, float *out) {
+    hipLaunchKernelGGL(k_mask_mul, dim3(grid1(n, 256)), dim3(256), 0, st, n, mask, g, out);
 }
 
 void launch_dense_stencil(hipStream_t st, Grid3 g, const float *cx, const float *cy, const float *cz, const float *x,

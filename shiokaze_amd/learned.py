@@ -101,6 +101,23 @@ class UNet:
         return z
 
 
+# (cin, cout, level) of the network's 3x3x3 convolutions; level l runs on the fine grid / 2^(l+1)
+LAYERS = [(8, 16, 0), (16, 32, 1), (32, 64, 2), (64, 128, 3), (192, 64, 2), (96, 32, 1), (48, 16, 0), (16, 16, 0)]
+
+
+def flops_per_apply(shape) -> float:
+    """Dense MFMA FLOPs of one network forward for a fine grid (nx, ny, nz), padded to
+    multiples of 16 as the solver pads it: 2 * 27 * cin * cout per output voxel."""
+    pad = [(int(d) + 15) // 16 * 16 for d in shape]
+    total = 0.0
+    for cin, cout, lvl in LAYERS:
+        v = 1
+        for d in pad:
+            v *= d >> (lvl + 1)
+        total += 2.0 * 27 * cin * cout * v
+    return total
+
+
 def residual(cx, cy, cz, x, b) -> torch.Tensor:
     """r = b - A^ x for the unit-diagonal operator with low-face couplings cx, cy, cz."""
     lib = _lib.load()

@@ -144,7 +144,7 @@ def main(argv=None):
     ap.add_argument("--steps-per-scene", type=int, default=4)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-4)
-    ap.add_argument("--omega", type=float, default=1.0)
+    ap.add_argument("--omega", type=float, default=0.8, help="weighted-Jacobi smoothing weight")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--label-residual", type=float, default=1e-10)
     ap.add_argument("--eval-residual", type=float, default=1e-6)

@@ -119,9 +119,9 @@ def test_resampling_adjoints():
     np.testing.assert_allclose(U.pool(U.upsample(c)), c, rtol=1e-14)
 
 
-def test_stencil_is_the_assembled_matrix():
-    shape = (3, 4, 5)
-    mask, (cx, cy, cz) = _random_system(shape, 7)
+def _dense_A(cx, cy, cz):
+    """The operator as an explicit matrix, assembled face by face."""
+    shape = cx.shape
     n = int(np.prod(shape))
     A = np.eye(n)
     idx = np.arange(n).reshape(shape)
@@ -134,9 +134,40 @@ def test_stencil_is_the_assembled_matrix():
             a, b = idx[v], idx[tuple(u)]
             A[a, b] -= c[v]
             A[b, a] -= c[v]
+    return A
+
+
+def test_stencil_is_the_assembled_matrix():
+    shape = (3, 4, 5)
+    mask, (cx, cy, cz) = _random_system(shape, 7)
+    A = _dense_A(cx, cy, cz)
     x = _rng(8).standard_normal(shape)
     np.testing.assert_allclose(U.stencil(cx, cy, cz, x).ravel(), A @ x.ravel(), rtol=1e-13, atol=1e-13)
     assert np.allclose(A, A.T)
+
+
+def test_prolongation():
+    rng = _rng(9)
+    u = rng.standard_normal((3, 4, 5))
+    g = rng.standard_normal((6, 8, 10))
+    # adjoint pair
+    assert np.isclose((U.prolong(u) * g).sum(), (u * U.prolong_T(g)).sum(), rtol=1e-12)
+    # constants stay constant (clamped edges), and the weights sum to one
+    np.testing.assert_allclose(U.prolong(np.full((3, 4, 5), 1.5)), 1.5, rtol=1e-15)
+    # a linear function of the cell centres is reproduced exactly away from the edges
+    Z, Y, X = np.meshgrid(np.arange(4), np.arange(4), np.arange(4), indexing="ij")
+    lin = 0.3 * X - 1.1 * Y + 0.7 * Z + 2.0
+    fz, fy, fx = np.meshgrid(np.arange(8), np.arange(8), np.arange(8), indexing="ij")
+    fine = 0.3 * ((fx + 0.5) / 2 - 0.5) - 1.1 * ((fy + 0.5) / 2 - 0.5) + 0.7 * ((fz + 0.5) / 2 - 0.5) + 2.0
+    np.testing.assert_allclose(U.prolong(lin)[1:-1, 1:-1, 1:-1], fine[1:-1, 1:-1, 1:-1], rtol=1e-13, atol=1e-13)
+    # brute force at one interior fine cell: weights (3/4, 1/4) per axis toward the nearer neighbour
+    f = (3, 4, 5)  # z odd -> neighbour +1; y even -> -1; x odd -> +1
+    ref = 0.0
+    for dz, wz in ((0, .75), (1, .25)):
+        for dy, wy in ((0, .75), (-1, .25)):
+            for dx, wx in ((0, .75), (1, .25)):
+                ref += wz * wy * wx * u[1 + dz, 2 + dy, 2 + dx]
+    assert np.isclose(U.prolong(u)[f], ref, rtol=1e-14)
 
 
 def test_loss_closed_forms():
@@ -147,12 +178,18 @@ def test_loss_closed_forms():
     # omega = 0: z = 0, L = x*'A x* / x*'r = 1 because r = A x*
     loss, _, z = U.loss_and_grad(p, r, mask, cx, cy, cz, xstar, 0.0, need_grad=False)
     assert np.all(z == 0) and abs(loss - 1.0) < 1e-12
-    # omega = 1: z = r, L = (r - x*)'A(r - x*) / x*'r, computed here with the explicit A
-    loss1, _, z1 = U.loss_and_grad(p, r, mask, cx, cy, cz, xstar, 1.0, need_grad=False)
-    np.testing.assert_array_equal(z1, r * mask)
-    e = (r - xstar).ravel()
-    Ae = U.stencil(cx, cy, cz, (r - xstar)).ravel()
-    assert abs(loss1 - e @ Ae / (xstar.ravel() @ r.ravel())) < 1e-12
+    # U == 0, omega = w: two weighted-Jacobi steps from zero, z = w M r + w M (r - A w M r),
+    # evaluated with the explicit matrix
+    w = 0.6
+    loss1, _, z1 = U.loss_and_grad(p, r, mask, cx, cy, cz, xstar, w, need_grad=False)
+    A = _dense_A(cx, cy, cz)
+    M = mask.ravel()
+    rv = r.ravel()
+    za = w * M * rv
+    zb = za + w * M * (rv - A @ za)
+    np.testing.assert_allclose(z1.ravel(), zb, rtol=1e-12, atol=1e-14)
+    e = zb - xstar.ravel()
+    assert abs(loss1 - e @ A @ e / (xstar.ravel() @ rv)) < 1e-12
     # positive homogeneity: M(2r) = 2 M(r)
     p2 = _init_params(13)
     za, _, _ = U.precondition(p2, r, mask, cx, cy, cz, 0.7)
