@@ -37,6 +37,13 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# HBM bytes per brick cell of one launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, factors from a 256 MiB copy in the same run) on the pool+sphere scene at
+# 256^3, mixed precision, Jacobi: profiles/hbm_traffic_pool_sphere_256_mixed_jacobi.md.
+# Reported as roofline.traffic only for that workload.
+MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 24.19, "axpy_norm": 24.03}
+
+
 def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
     """Algorithmic HBM bytes per brick cell of one launch of each Krylov kernel:
     every array element the kernel needs is read or written once.  With Jacobi the
@@ -164,8 +171,11 @@ def main():
         if kernels:
             dom = max((k for k in kernels if "GBs" in kernels[k]), key=lambda k: kernels[k]["total_ms"])
             kd = kernels[dom]
+            measured = (args.scene == 1 and n == 256 and args.precision == "mixed"
+                        and args.preconditioner == "jacobi" and not args.set)
+            traffic = round(MEASURED_TRAFFIC_PER_CELL[dom] * n_cells) if measured else None
             roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": kd["frac"], "traffic": None,
+                        "frac": kd["frac"], "traffic": traffic,
                         "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}
             solve_bytes = sum(kernels[k].get("bytes_per_launch", 0) * kernels[k]["launches"] for k in kernels)
             solve_ms = sum(kernels[k]["total_ms"] for k in kernels)

@@ -144,22 +144,19 @@ int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
     HIPC(c, hipStreamSynchronize(c->stream));
     const int interval = std::max(1, c->prm.check_interval);
     constexpr int K = shk_ctx::kKernels;
-    if (c->profile && c->pev.size() < (size_t)(K * interval + 1)) {
+    if (c->profile && c->pev.size() < (size_t)(2 * K * interval)) {
         const size_t old = c->pev.size();
-        c->pev.resize(K * interval + 1);
+        c->pev.resize(2 * K * interval);
         for (size_t i = old; i < c->pev.size(); ++i) HIPC(c, hipEventCreate(&c->pev[i]));
     }
     while (!c->h_state->done) {
         const int64_t it0 = c->h_state->iter;
-        if (c->profile) {
-            c->pcur = 0;
-            c->mark();
-        }
+        c->pcur = 0;
         for (int t = 0; t < interval; ++t) iterate(arg);
         HIPC(c, hipGetLastError());
         HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
-        if (c->profile && c->pcur == (size_t)(K * interval + 1)) {
+        if (c->profile && c->pcur == (size_t)(2 * K * interval)) {
             // only launches that did work count: the first `delta` iterations; the
             // direction update of the final iteration returns at once
             const int64_t delta = c->h_state->iter - it0;
@@ -167,7 +164,8 @@ int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
                 for (int kk = 0; kk < K; ++kk) {
                     if (kk == K - 1 && c->h_state->done && t == delta - 1) continue;
                     float ms = 0.0f;
-                    HIPC(c, hipEventElapsedTime(&ms, c->pev[K * t + kk], c->pev[K * t + kk + 1]));
+                    const size_t l = (size_t)(K * t + kk);
+                    HIPC(c, hipEventElapsedTime(&ms, c->pev[2 * l], c->pev[2 * l + 1]));
                     c->pcount[kk] += 1;
                     c->pms[kk] += ms;
                 }
@@ -216,10 +214,8 @@ int solve_fp64(shk_ctx *c, shk_result *res) {
     launch_dir_spmv<double, double>(c->stream, g, op, c->r64.p, S[0], S[1], c->q64.p, red);
     ++m;
     int rc = drive(c, [&]() {
-        launch_axpy_norm<double>(c->stream, g, sig, c->x64.p, c->r64.p, S[m & 1], c->q64.p, red);
-        c->mark();
-        launch_dir_spmv<double, double>(c->stream, g, op, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red);
-        c->mark();
+        launch_axpy_norm<double>(c->stream, g, sig, c->x64.p, c->r64.p, S[m & 1], c->q64.p, red, c->next_events());
+        launch_dir_spmv<double, double>(c->stream, g, op, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red, c->next_events());
         ++m;
     });
     if (rc) return rc;
@@ -267,10 +263,8 @@ int solve_mixed(shk_ctx *c, shk_result *res) {
         launch_dir_spmv<float, float>(c->stream, g, op, c->r32.p, S[0], S[1], c->q32.p, red);
         ++m;
         rc = drive(c, [&]() {
-            launch_axpy_norm<float>(c->stream, g, nullptr, c->x32.p, c->r32.p, S[m & 1], c->q32.p, red);
-            c->mark();
-            launch_dir_spmv<float, float>(c->stream, g, op, c->r32.p, S[m & 1], S[(m + 1) & 1], c->q32.p, red);
-            c->mark();
+            launch_axpy_norm<float>(c->stream, g, nullptr, c->x32.p, c->r32.p, S[m & 1], c->q32.p, red, c->next_events());
+            launch_dir_spmv<float, float>(c->stream, g, op, c->r32.p, S[m & 1], S[(m + 1) & 1], c->q32.p, red, c->next_events());
             ++m;
         });
         if (rc) return rc;
@@ -751,10 +745,8 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[0], S[1], c->q64.p, red);
     ++m;
     rc = drive(c, [&]() {
-        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, S[m & 1], c->q64.p, red);
-        c->mark();
-        launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red);
-        c->mark();
+        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, S[m & 1], c->q64.p, red, c->next_events());
+        launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red, c->next_events());
         ++m;
     });
     if (rc) return rc;

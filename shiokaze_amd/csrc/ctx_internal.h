@@ -128,9 +128,12 @@ struct shk_ctx {
     hipEvent_t uev[2] = {};
     int64_t ucount = 0;
     double ums = 0.0;
-    // records the next profiling event after a Krylov kernel launch
-    void mark() {
-        if (profile && pcur < pev.size()) (void)hipEventRecord(pev[pcur++], stream);
+    // the start/stop events of the next timed Krylov launch (none when not profiling)
+    shk::KEvents next_events() {
+        if (!profile || pcur + 2 > pev.size()) return shk::KEvents{};
+        const shk::KEvents e{pev[pcur], pev[pcur + 1]};
+        pcur += 2;
+        return e;
     }
 
     int fail(int code, const char *fmt, ...) {

@@ -64,6 +64,12 @@ template <class T, class TB>
 void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *sig, T *x, T *r, T *s, Reduction red,
                     double tol_rel, int64_t max_iter);
 
+// Optional timing of one launch: the start/stop events are recorded by the dispatch
+// itself (hipExtLaunchKernelGGL), so their interval is the kernel's own duration.
+struct KEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // One CG iteration is two kernels (see krylov.hip): dir_spmv (stop test, beta,
 // s_out = r + beta s_in, q = A s_out, partial s_out.q) and axpy_norm (alpha,
 // x += alpha s, r -= alpha q, partials r.r and max|r|).  The first dir_spmv after
@@ -71,12 +77,12 @@ void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *
 // stop test uses the unscaled residual max|r/sig|.
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
-                     Reduction red);
+                     Reduction red, KEvents ev = {});
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
-                         double *s_out, double *q, Reduction red);
+                         double *s_out, double *q, Reduction red, KEvents ev = {});
 template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
-                      Reduction red);
+                      Reduction red, KEvents ev = {});
 
 // r = b - A x in FP64 (unscaled operator with its diagonal); part holds 4*blocks
 // partials; out = {max|r|, sum r^2, sum b^2, max|b|}.  Optionally rs = (float)

@@ -24,6 +24,8 @@
 // neighbour bricks.
 #include "krylov.h"
 
+#include <hip/hip_ext.h>
+
 #include "shk_projection.h"
 
 namespace shk {
@@ -641,31 +643,38 @@ void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *
     hipLaunchKernelGGL(k_finish_init, dim3(1), dim3(NF), 0, st, g.blocks, red, tol_rel, max_iter);
 }
 
+// plain launch, or one whose dispatch records ev.start / ev.stop
+#define SHK_LAUNCH(ev, kernel, grid, block, st, ...)                                                 \
+    do {                                                                                            \
+        if ((ev).start)                                                                             \
+            hipExtLaunchKernelGGL(kernel, grid, block, 0, st, (ev).start, (ev).stop, 0, __VA_ARGS__); \
+        else                                                                                        \
+            hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__);                            \
+    } while (0)
+
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
-                     Reduction red) {
+                     Reduction red, KEvents ev) {
     if (op.diag)
-        hipLaunchKernelGGL((k_dir_spmv<T, C, false>), dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q,
-                           red);
+        SHK_LAUNCH(ev, (k_dir_spmv<T, C, false>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q, red);
     else
-        hipLaunchKernelGGL((k_dir_spmv<T, C, true>), dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q,
-                           red);
+        SHK_LAUNCH(ev, (k_dir_spmv<T, C, true>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q, red);
 }
 
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
-                         double *s_out, double *q, Reduction red) {
-    hipLaunchKernelGGL(k_dir_spmv_csr, dim3(g.blocks), dim3(NB), 0, st, op, g.per, r, s_in, s_out, q, red);
+                         double *s_out, double *q, Reduction red, KEvents ev) {
+    SHK_LAUNCH(ev, k_dir_spmv_csr, dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q, red);
 }
 
 template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
-                      Reduction red) {
+                      Reduction red, KEvents ev) {
     if (sig)
-        hipLaunchKernelGGL((k_axpy_norm<T, true>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, sig, x, r, s,
-                           q, red);
+        SHK_LAUNCH(ev, (k_axpy_norm<T, true>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s, q,
+                   red);
     else
-        hipLaunchKernelGGL((k_axpy_norm<T, false>), dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, sig, x, r,
-                           s, q, red);
+        SHK_LAUNCH(ev, (k_axpy_norm<T, false>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s, q,
+                   red);
 }
 
 void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,
@@ -703,12 +712,12 @@ template void launch_cg_init<float, double>(hipStream_t, const Grid &, const dou
 template void launch_cg_init<float, float>(hipStream_t, const Grid &, const float *, const double *, float *, float *,
                                            float *, Reduction, double, int64_t);
 template void launch_dir_spmv<double, double>(hipStream_t, const Grid &, const BrickOp<double> &, const double *,
-                                              const double *, double *, double *, Reduction);
+                                              const double *, double *, double *, Reduction, KEvents);
 template void launch_dir_spmv<float, float>(hipStream_t, const Grid &, const BrickOp<float> &, const float *,
-                                            const float *, float *, float *, Reduction);
+                                            const float *, float *, float *, Reduction, KEvents);
 template void launch_axpy_norm<double>(hipStream_t, const Grid &, const double *, double *, double *, const double *,
-                                       const double *, Reduction);
+                                       const double *, Reduction, KEvents);
 template void launch_axpy_norm<float>(hipStream_t, const Grid &, const double *, float *, float *, const float *,
-                                      const float *, Reduction);
+                                      const float *, Reduction, KEvents);
 
 }  // namespace shk

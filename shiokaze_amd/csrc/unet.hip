@@ -717,10 +717,8 @@ int unet_solve(shk_ctx *c, shk_result *res) {
             launch_dir_spmv<float, float>(st, g, op, c->r32.p, S[0], S[1], c->q32.p, red);
             ++m;
             rc = drive(c, [&]() {
-                launch_axpy_norm<float>(st, g, nullptr, c->x32.p, c->r32.p, S[m & 1], c->q32.p, red);
-                c->mark();
-                launch_dir_spmv<float, float>(st, g, op, c->r32.p, S[m & 1], S[(m + 1) & 1], c->q32.p, red);
-                c->mark();
+                launch_axpy_norm<float>(st, g, nullptr, c->x32.p, c->r32.p, S[m & 1], c->q32.p, red, c->next_events());
+                launch_dir_spmv<float, float>(st, g, op, c->r32.p, S[m & 1], S[(m + 1) & 1], c->q32.p, red, c->next_events());
                 ++m;
             });
             if (rc) return rc;
