@@ -50,7 +50,7 @@ def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
     system is the symmetrically scaled one (unit diagonal, no diagonal array)."""
     t = 8 if precision == "fp64" else 4      # vector element
     c = 8 if precision == "fp64" else 4      # coefficient element
-    jac = precond == "jacobi"
+    jac = precond in ("jacobi", "mg", "unet")  # all three run CG on the scaled unit-diagonal system
     if kernel == "spmv_dot":                  # r, s in; cx, cy, cz (+ diag) in; s', q out
         return 4 * t + 3 * c + (0 if jac else c)
     if kernel == "axpy_norm":                 # x, r, s, q in; x, r out (+ 1/sqrt(diag) for the FP64 stop test)
@@ -67,7 +67,7 @@ def main():
     ap.add_argument("--scene", type=int, default=1, help="1 pool+sphere, 3 irregular")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
-    ap.add_argument("--preconditioner", default="jacobi", choices=["none", "jacobi", "unet"])
+    ap.add_argument("--preconditioner", default="jacobi", choices=["none", "jacobi", "unet", "mg"])
     ap.add_argument("--residual", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
@@ -223,6 +223,18 @@ def main():
             extra["fp64_jacobi_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rj["count"]}
             extra["mixed_jacobi_vs_fp64_plain_cg"] = round(value / extra["fp64_plain_cg"]["solves_per_s"], 2)
             pj.close()
+        if args.extra:
+            # aggregation-multigrid PCG (mixed precision)
+            pm = make({"Residual": args.residual, "Preconditioner": "mg"})
+            step(pm)
+            t1 = time.perf_counter()
+            rm = step(pm)
+            extra["mg_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rm["count"],
+                               "outer": rm["outer"], "true_relres_inf": rm["true_relres_inf"]}
+            if "fp64_plain_cg" in extra:
+                extra["mg_pcg"]["vs_fp64_plain_cg"] = round(extra["mg_pcg"]["solves_per_s"]
+                                                            / extra["fp64_plain_cg"]["solves_per_s"], 2)
+            pm.close()
         if args.extra and Path(args.unet_weights).exists():
             # learned preconditioner: flexible CG with the trained network (plain PCG fallback)
             from shiokaze_amd.learned import flops_per_apply

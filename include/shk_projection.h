@@ -99,7 +99,13 @@ void shk_destroy(shk_ctx *ctx);
  *                            defaults (volume correction and warm start off) are built
  * Keys of this library:
  *   Precision      fp64 | mixed            (fp64)
- *   Preconditioner none | jacobi | unet    (jacobi)
+ *   Preconditioner none | jacobi | unet | mg   (jacobi)
+ *                  none: the reference's effective iteration (plain CG, z = r);
+ *                  jacobi: diag(A) as symmetric scaling; unet: flexible CG with the
+ *                  learned network below; mg: CG with an aggregation-multigrid V-cycle
+ *                  (mixed precision, the counterpart of the reference's "amg" LinSolver,
+ *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr applies jacobi for
+ *                  any of jacobi, unet and mg.
  *   OuterMax       refinement steps for mixed (3)
  *   InnerResidual  relative stop of each mixed inner solve (1e-6, floor)
  *   CheckInterval  iterations launched between host convergence polls (32)
@@ -167,6 +173,16 @@ int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64
  * since ProfileKernels was last set.
  * Only launches that did work are counted. */
 int shk_kernel_stats(shk_ctx *ctx, const char *kernel, int64_t *launches, double *total_ms);
+
+/* ------------------------------------------------------------------ multigrid preconditioner */
+
+/* z = M r with the multigrid V-cycle of the last projection (Preconditioner=mg): dense
+ * FP32 device grids (nx*ny*nz) in the Jacobi-scaled unknowns (r^ = D^-1/2 r,
+ * z^ = D^1/2 z).  The hierarchy: 2x2x2 aggregates with Galerkin coarse operators,
+ * one damped-Jacobi pre- and post-smoothing step (w = 0.8), coarse-correction factor
+ * 1.5, dense solve on the coarsest level (<= 4 cells per axis).  A test and
+ * inspection entry point; shk_project applies the same V-cycle inside CG. */
+int shk_mg_apply(shk_ctx *ctx, const float *r, float *z);
 
 /* ------------------------------------------------------------------ learned preconditioner */
 

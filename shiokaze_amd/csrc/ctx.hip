@@ -118,7 +118,7 @@ int build_bricks(shk_ctx *c) {
         for (DevBuf<double> *b : {&c->hx64, &c->hy64, &c->hz64}) HIPC(c, b->ensure(n));
         HIPC(c, c->kappa64.ensure(c->n_active));
     }
-    if (c->prm.precision == 1 || c->prm.precond == 2) {
+    if (c->prm.precision == 1 || c->prm.precond >= 2) {
         for (DevBuf<float> *b : {&c->x32, &c->r32, &c->s32, &c->t32, &c->q32, &c->rs32}) HIPC(c, b->ensure(n));
         if (c->prm.precond == 0)
             for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32}) HIPC(c, b->ensure(n));
@@ -139,10 +139,10 @@ namespace shk {
 
 Reduction reduction(shk_ctx *c) { return Reduction{c->pa.p, c->pb.p, c->state.p}; }
 
-int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg) {
+int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg, int interval_override) {
     HIPC(c, hipMemcpyAsync(c->h_state, c->state.p, sizeof(SolveState), hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
-    const int interval = std::max(1, c->prm.check_interval);
+    const int interval = std::max(1, interval_override > 0 ? interval_override : c->prm.check_interval);
     constexpr int K = shk_ctx::kKernels;
     if (c->profile && c->pev.size() < (size_t)(2 * K * interval)) {
         const size_t old = c->pev.size();
@@ -306,7 +306,7 @@ int shk_abi_version(void) { return SHK_ABI_VERSION; }
 
 const char *shk_build_info(void) {
     return "libshk_projection (gfx950; brick 8^3 matrix-free MAC pressure projection; FP64 + mixed CG; "
-           "learned flexible-CG preconditioner)";
+           "aggregation-multigrid PCG; learned flexible-CG preconditioner)";
 }
 
 const char *shk_last_error(const shk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -379,6 +379,7 @@ void shk_destroy(shk_ctx *c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->unet) unet_destroy(c->unet);
+    if (c->mg) mg_destroy(c->mg);
     for (auto &b : c->area) b.release();
     for (auto &b : c->rho) b.release();
     for (auto &b : c->stage) b.release();
@@ -466,8 +467,10 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
             p.precond = 1;
         else if (v == "unet")
             p.precond = 2;
+        else if (v == "mg")
+            p.precond = 3;
         else
-            return c->fail(SHK_BAD_ARG, "Preconditioner must be none, jacobi or unet, got '%s'", value);
+            return c->fail(SHK_BAD_ARG, "Preconditioner must be none, jacobi, unet or mg, got '%s'", value);
     } else if (k == "OuterMax") {
         if (!parse_int(value, i) || i < 1 || i > 50) return c->fail(SHK_BAD_ARG, "bad OuterMax '%s'", value);
         p.outer_max = (int)i;
@@ -559,7 +562,7 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         HIPC(c, hipStreamSynchronize(c->stream));
         return SHK_OK;
     }
-    const bool need32 = c->prm.precision == 1 || c->prm.precond == 2;
+    const bool need32 = c->prm.precision == 1 || c->prm.precond >= 2;
     const bool plain32 = need32 && c->prm.precond == 0;
     launch_coeff(c->stream, d, dt, c->dx, c->n_active, c->coord.p, area, rho, c->label.p, vel, svel, c->diag64.p,
                  c->cx64.p, c->cy64.p, c->cz64.p, c->b64.p, c->sig64.p, plain32 ? c->diag32.p : nullptr,
@@ -578,7 +581,9 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ev[1], c->stream));
 
-    if (c->prm.precond == 2) {
+    if (c->prm.precond == 3) {
+        rc = solve_mg(c, res);
+    } else if (c->prm.precond == 2) {
         rc = unet_solve(c, res);
     } else if (c->prm.precision == 1) {
         rc = solve_mixed(c, res);

@@ -16,6 +16,7 @@
 
 namespace shk {
 struct UnetPrecond;
+struct MgHierarchy;
 }
 
 namespace shk {
@@ -55,7 +56,7 @@ struct Params {
     double eps_fluid = 1e-2;          // macutility3.cpp:522
     double eps_solid = 1e-2;          // macutility3.cpp:523
     int precision = 0;                // 0 fp64, 1 mixed
-    int precond = 1;                  // 0 none, 1 jacobi, 2 unet
+    int precond = 1;                  // 0 none, 1 jacobi, 2 unet, 3 mg
     int outer_max = 3;
     double inner_residual = 1e-6;
     int check_interval = 32;
@@ -117,6 +118,8 @@ struct shk_ctx {
     hipEvent_t ev[6] = {};
     // learned preconditioner
     shk::UnetPrecond *unet = nullptr;
+    // multigrid preconditioner hierarchy
+    shk::MgHierarchy *mg = nullptr;
     // per-kernel timing of the Krylov loop (ProfileKernels=1)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -158,10 +161,12 @@ struct shk_ctx {
 namespace shk {
 
 // Iterates `iterate` CheckInterval times between polls of the device state.
-int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg);
+// Runs iterate() in batches of `interval` launches-worth of iterations (default:
+// CheckInterval) until the device state says done.
+int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg, int interval);
 template <class F>
-int drive(shk_ctx *c, F &&f) {
-    return drive_impl(c, [](void *p) { (*static_cast<F *>(p))(); }, &f);
+int drive(shk_ctx *c, F &&f, int interval = 0) {
+    return drive_impl(c, [](void *p) { (*static_cast<F *>(p))(); }, &f, interval);
 }
 Reduction reduction(shk_ctx *c);
 BrickOp<double> op64(shk_ctx *c);
@@ -175,6 +180,8 @@ int solve_fp64(shk_ctx *c, shk_result *res);
 
 // learned preconditioner (unet.hip)
 void unet_destroy(UnetPrecond *u);
+void mg_destroy(MgHierarchy *h);
+int solve_mg(shk_ctx *c, shk_result *res);
 int unet_solve(shk_ctx *c, shk_result *res);
 int export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat);
 

@@ -23,6 +23,9 @@ struct Reduction {
     double *pa;       // s.q partials (>= blocks)
     double *pb;       // r.r and max|r| partials (>= 2 * blocks)
     SolveState *st;   // device state
+    // with a preconditioner z = M r: the r.z partials (>= blocks) written by the last
+    // kernel of M; dir_spmv is then given z in place of r and takes rho = r.z
+    double *pz = nullptr;
 };
 
 // Operator views.  Brick: vectors hold n_active bricks of 8x8x8 cells, x-fastest

@@ -137,10 +137,15 @@ __device__ inline bool direction_step(const Reduction &red, double *lds, double 
     const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
     if (it == 0) {
         beta = 0.0;
-        if (lead) st->rho_k1 = st->rho_k2;
+        if (red.pz) {
+            const double rz = fold_sum<NB>(red.pz, gridDim.x, lds, bc);
+            if (lead) st->rho_k1 = rz;
+        } else if (lead) {
+            st->rho_k1 = st->rho_k2;
+        }
         return true;
     }
-    const double rho_new = fold_sum<NB>(red.pb, gridDim.x, lds, bc);
+    const double rho_new = fold_sum<NB>(red.pz ? red.pz : red.pb, gridDim.x, lds, bc);
     const double rn = fold_max<NB>(red.pb + gridDim.x, gridDim.x, lds, bc);
     const bool stop = rn <= st->tol_abs;
     const bool nan = rho_new != rho_new;

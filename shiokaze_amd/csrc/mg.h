@@ -1,0 +1,73 @@
+// mg.h -- aggregation multigrid V-cycle on the brick layout (mg.hip), the
+// preconditioner of Preconditioner=mg.
+//
+// Level 0 is the Jacobi-scaled system A^ = D^-1/2 A D^-1/2 (unit diagonal on the
+// unknowns) in the solver's 8^3 bricks.  Level l+1 aggregates the cells of level l
+// 2x2x2 (cell (X,Y,Z) owns the cells (2X+a, 2Y+b, 2Z+c)) with the Galerkin operator
+// P0' A_l P0 of piecewise-constant prolongation P0: again a 7-point operator,
+//   couplings  c_C = sum of the 4 fine couplings across the aggregates' shared face,
+//   diagonal   d_C = sum of the 8 fine diagonals - 2 * (the 12 couplings inside C).
+// Coarsening stops when every axis has at most 4 cells; that level (<= 64 cells, one
+// brick) is solved exactly with a dense inverse computed on the host.
+//
+// V-cycle (symmetric, so M is a fixed SPD operator and plain PCG applies):
+//   x_l = w D_l^-1 b_l;  t_l = b_l - A_l x_l;  b_{l+1} = P0' t_l;  (recurse)
+//   x_l += alpha P0 x_{l+1};  x_l += w D_l^-1 (b_l - A_l x_l)
+// with w = 0.8 and alpha = 1.5 (the over-correction that unsmoothed aggregation needs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "krylov.h"
+
+namespace shk {
+
+struct MgOp {
+    const int *nbr;
+    const float *d, *cx, *cy, *cz;
+    int n_active;
+};
+
+struct MgLevelDev {
+    int nx = 0, ny = 0, nz = 0;     // cells
+    int nbx = 0, nby = 0, nbz = 0;  // bricks
+    int n_active = 0;
+    int *slot = nullptr;    // dense brick table -> slot or -1
+    int4 *coord = nullptr;  // brick coordinates per slot
+    int *nbr = nullptr;     // 6 neighbour slots per slot
+    int *parent = nullptr;  // slot of the parent brick on the next level (levels < coarsest)
+    float *d = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
+    float *b = nullptr, *x = nullptr, *t = nullptr;
+    MgOp op() const { return MgOp{nbr, d, cx, cy, cz, n_active}; }
+};
+
+constexpr float kMgOmega = 0.8f;
+constexpr float kMgAlpha = 1.5f;
+constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)
+
+// level-0 diagonal of the scaled system: 1 on unknowns (sig > 0), else 0
+void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0);
+// coarse operator of level `c` from level `f`
+void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c);
+// x = w b / d (level 0 pre-smoothing from zero); st may be nullptr
+void launch_mg_jacobi0(hipStream_t st, int64_t n, const float *b, const float *d, float *x, const SolveState *state);
+// t = b - A x
+void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
+                        const SolveState *state);
+// b_c = P0' t_f, x_c = w b_c / d_c (or 0 with w = 0)
+void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
+                        const SolveState *state);
+// x_f += alpha P0 x_c on the fine unknowns
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state);
+// y = x + w (b - A x) / d; with `g` and `part` (level 0): launched on the Krylov grid
+// and part[block] = sum of b*y over the block (the r.z partials of PCG)
+void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
+                      const SolveState *state, const Grid *g = nullptr, double *part = nullptr);
+// coarsest level: x[idx[i]] = sum_j inv[i][j] b[idx[j]] (n <= kMgCoarseMax), other cells 0
+void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x,
+                      int cells, const SolveState *state);
+
+}  // namespace shk

@@ -1,0 +1,245 @@
+// mg.hip -- aggregation multigrid V-cycle kernels on the brick layout (see mg.h).
+//
+// Every kernel returns at once when the Krylov state it is given has stopped, so
+// the V-cycles queued behind a converged iteration cost only their launches.
+#include "mg.h"
+
+namespace shk {
+
+namespace {
+
+constexpr int NT = 256;
+
+inline unsigned blocks(int64_t n) { return (unsigned)((n + NT - 1) / NT > 0 ? (n + NT - 1) / NT : 1); }
+
+__device__ inline bool stopped(const SolveState *s) { return s && s->done; }
+
+// (A x)_e for the 7-point brick operator with an explicit diagonal (FP32).
+__device__ inline float apply(const MgOp &op, int slot, int l, const float *__restrict__ x) {
+    const int64_t e = (int64_t)slot * kBrickCells + l;
+    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    const int *nb = op.nbr + 6 * slot;
+    float acc = op.d[e] * x[e];
+    if (i > 0) {
+        acc -= op.cx[e] * x[e - 1];
+    } else if (nb[0] >= 0) {
+        acc -= op.cx[e] * x[(int64_t)nb[0] * kBrickCells + l + 7];
+    }
+    if (i < 7) {
+        acc -= op.cx[e + 1] * x[e + 1];
+    } else if (nb[1] >= 0) {
+        const int64_t m = (int64_t)nb[1] * kBrickCells + l - 7;
+        acc -= op.cx[m] * x[m];
+    }
+    if (j > 0) {
+        acc -= op.cy[e] * x[e - 8];
+    } else if (nb[2] >= 0) {
+        acc -= op.cy[e] * x[(int64_t)nb[2] * kBrickCells + l + 56];
+    }
+    if (j < 7) {
+        acc -= op.cy[e + 8] * x[e + 8];
+    } else if (nb[3] >= 0) {
+        const int64_t m = (int64_t)nb[3] * kBrickCells + l - 56;
+        acc -= op.cy[m] * x[m];
+    }
+    if (k > 0) {
+        acc -= op.cz[e] * x[e - 64];
+    } else if (nb[4] >= 0) {
+        acc -= op.cz[e] * x[(int64_t)nb[4] * kBrickCells + l + 448];
+    }
+    if (k < 7) {
+        acc -= op.cz[e + 64] * x[e + 64];
+    } else if (nb[5] >= 0) {
+        const int64_t m = (int64_t)nb[5] * kBrickCells + l - 448;
+        acc -= op.cz[m] * x[m];
+    }
+    return acc;
+}
+
+// fine slot holding the 8 children of coarse cell (slot_c, l) and the children's base
+// local offset; -1 when the fine brick is outside the grid or inactive
+__device__ inline int children(const MgLevelDev &f, const MgLevelDev &c, int slot_c, int l, int &base) {
+    const int ci = l & 7, cj = (l >> 3) & 7, ck = l >> 6;
+    const int4 B = c.coord[slot_c];
+    if (B.x * 8 + ci >= c.nx || B.y * 8 + cj >= c.ny || B.z * 8 + ck >= c.nz) return -1;
+    const int fbx = 2 * B.x + (ci >> 2), fby = 2 * B.y + (cj >> 2), fbz = 2 * B.z + (ck >> 2);
+    if (fbx >= f.nbx || fby >= f.nby || fbz >= f.nbz) return -1;
+    base = 2 * (ci & 3) + 8 * (2 * (cj & 3)) + 64 * (2 * (ck & 3));
+    return f.slot[fbx + f.nbx * (fby + f.nby * fbz)];
+}
+
+__global__ __launch_bounds__(NT) void k_mg_mask(int64_t n, const double *__restrict__ sig, float *__restrict__ d0) {
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e < n) d0[e] = sig[e] > 0.0 ? 1.0f : 0.0f;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
+    const int64_t E = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (E >= (int64_t)c.n_active * kBrickCells) return;
+    const int slot_c = (int)(E >> 9), l = (int)(E & 511);
+    int base = 0;
+    const int fs = children(f, c, slot_c, l, base);
+    double d = 0.0, inner = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    if (fs >= 0) {
+        const int64_t fb = (int64_t)fs * kBrickCells + base;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
+            const int64_t e = fb + a + 8 * b + 64 * cc;
+            d += (double)f.d[e];
+            if (a) inner += (double)f.cx[e]; else cx += (double)f.cx[e];
+            if (b) inner += (double)f.cy[e]; else cy += (double)f.cy[e];
+            if (cc) inner += (double)f.cz[e]; else cz += (double)f.cz[e];
+        }
+    }
+    double dc = d - 2.0 * inner;
+    if (!(dc > 1e-6 * d)) dc = 0.0;  // no unknown child, or an aggregate decoupled from every boundary
+    c.d[E] = (float)dc;
+    c.cx[E] = (float)cx;
+    c.cy[E] = (float)cy;
+    c.cz[E] = (float)cz;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_jacobi0(int64_t n, const float *__restrict__ b, const float *__restrict__ d,
+                                                   float *__restrict__ x, const SolveState *state) {
+    if (stopped(state)) return;
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e < n) x[e] = d[e] > 0.0f ? kMgOmega * b[e] / d[e] : 0.0f;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
+                                                    float *__restrict__ t, const SolveState *state) {
+    if (stopped(state)) return;
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e >= (int64_t)op.n_active * kBrickCells) return;
+    const int slot = (int)(e >> 9), l = (int)(e & 511);
+    t[e] = op.d[e] > 0.0f ? b[e] - apply(op, slot, l, x) : 0.0f;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, const float *__restrict__ tf,
+                                                    float omega, const SolveState *state) {
+    if (stopped(state)) return;
+    const int64_t E = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (E >= (int64_t)c.n_active * kBrickCells) return;
+    const int slot_c = (int)(E >> 9), l = (int)(E & 511);
+    int base = 0;
+    const int fs = children(f, c, slot_c, l, base);
+    float s = 0.0f;
+    if (fs >= 0) {
+        const int64_t fb = (int64_t)fs * kBrickCells + base;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += tf[fb + (q & 1) + 8 * ((q >> 1) & 1) + 64 * (q >> 2)];
+    }
+    const float dc = c.d[E];
+    c.b[E] = dc > 0.0f ? s : 0.0f;
+    c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
+}
+
+__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state) {
+    if (stopped(state)) return;
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e >= (int64_t)f.n_active * kBrickCells) return;
+    if (!(f.d[e] > 0.0f)) return;
+    const int slot = (int)(e >> 9), l = (int)(e & 511);
+    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    const int4 B = f.coord[slot];
+    const int ps = f.parent[slot];
+    if (ps < 0) return;  // cannot happen: the parent of an active brick is active
+    const int cl = ((B.x & 1) * 4 + (i >> 1)) + 8 * ((B.y & 1) * 4 + (j >> 1)) + 64 * ((B.z & 1) * 4 + (k >> 1));
+    f.x[e] += kMgAlpha * c.x[(int64_t)ps * kBrickCells + cl];
+}
+
+// one thread per cell (no partials) or, on the Krylov grid, one block per slot range
+template <bool PART>
+__global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
+                                                  float *__restrict__ y, const SolveState *state, int per,
+                                                  double *__restrict__ part) {
+    if (stopped(state)) return;
+    if (!PART) {
+        const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+        if (e >= (int64_t)op.n_active * kBrickCells) return;
+        const int slot = (int)(e >> 9), l = (int)(e & 511);
+        const float dv = op.d[e];
+        y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x)) / dv : 0.0f;
+        return;
+    }
+    __shared__ double lds[NT / kWave];
+    const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    const int t0 = g * per, t1 = t0 + per < op.n_active ? t0 + per : op.n_active;
+    double acc = 0.0;
+    for (int slot = t0; slot < t1; ++slot) {
+        for (int l = threadIdx.x; l < kBrickCells; l += NT) {
+            const int64_t e = (int64_t)slot * kBrickCells + l;
+            const float dv = op.d[e];
+            const float bv = b[e];
+            const float yv = dv > 0.0f ? x[e] + kMgOmega * (bv - apply(op, slot, l, x)) / dv : 0.0f;
+            y[e] = yv;
+            acc += (double)bv * (double)yv;
+        }
+    }
+    const double s = block_sum<NT>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kMgCoarseMax) void k_mg_coarse(int n, const int *__restrict__ idx,
+                                                            const float *__restrict__ inv, const float *__restrict__ b,
+                                                            float *__restrict__ x, int cells,
+                                                            const SolveState *state) {
+    if (stopped(state)) return;
+    __shared__ float bs[kMgCoarseMax];
+    const int t = threadIdx.x;
+    if (t < n) bs[t] = b[idx[t]];
+    __syncthreads();
+    float acc = 0.0f;
+    if (t < n)
+        for (int j = 0; j < n; ++j) acc += inv[t * n + j] * bs[j];
+    for (int e = t; e < cells; e += kMgCoarseMax) x[e] = 0.0f;
+    __syncthreads();
+    if (t < n) x[idx[t]] = acc;
+}
+
+}  // namespace
+
+void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0) {
+    hipLaunchKernelGGL(k_mg_mask, dim3(blocks(n)), dim3(NT), 0, st, n, sig, d0);
+}
+
+void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c) {
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c);
+}
+
+void launch_mg_jacobi0(hipStream_t st, int64_t n, const float *b, const float *d, float *x, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_jacobi0, dim3(blocks(n)), dim3(NT), 0, st, n, b, d, x, state);
+}
+
+void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
+                        const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_residual, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x, t,
+                       state);
+}
+
+void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
+                        const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c, tf,
+                       omega, state);
+}
+
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state);
+}
+
+void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
+                      const SolveState *state, const Grid *g, double *part) {
+    if (g && part)
+        hipLaunchKernelGGL(k_mg_smooth<true>, dim3(g->blocks), dim3(NT), 0, st, op, b, x, y, state, g->per, part);
+    else
+        hipLaunchKernelGGL(k_mg_smooth<false>, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op,
+                           b, x, y, state, 0, nullptr);
+}
+
+void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x, int cells,
+                      const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(kMgCoarseMax), 0, st, n, idx, inv, b, x, cells, state);
+}
+
+}  // namespace shk
