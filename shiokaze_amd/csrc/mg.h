@@ -29,6 +29,9 @@ struct MgOp {
     const int *nbr;
     const float *d, *cx, *cy, *cz;
     int n_active;
+    // level 0 only: per-brick uniform coupling (unit diagonal, one coupling value; 0 =
+    // not uniform), which lets those bricks skip the diagonal and coupling reads
+    const float *kappa = nullptr;
 };
 
 struct MgLevelDev {
@@ -53,15 +56,16 @@ void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0);
 // coarse operator of level `c` from level `f`
 void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c);
 // x = w b / d (level 0 pre-smoothing from zero); st may be nullptr
-void launch_mg_jacobi0(hipStream_t st, int64_t n, const float *b, const float *d, float *x, const SolveState *state);
+void launch_mg_jacobi0(hipStream_t st, const MgOp &op, const float *b, float *x, const SolveState *state);
 // t = b - A x
 void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
                         const SolveState *state);
 // b_c = P0' t_f, x_c = w b_c / d_c (or 0 with w = 0)
 void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
                         const SolveState *state);
-// x_f += alpha P0 x_c on the fine unknowns
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state);
+// x_f += alpha P0 x_c on the fine unknowns (kappa: the fine level's uniform bricks, or nullptr)
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state,
+                       const float *kappa = nullptr);
 // y = x + w (b - A x) / d; with `g` and `part` (level 0): launched on the Krylov grid
 // and part[block] = sum of b*y over the block (the r.z partials of PCG)
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,

@@ -14,11 +14,28 @@ inline unsigned blocks(int64_t n) { return (unsigned)((n + NT - 1) / NT > 0 ? (n
 
 __device__ inline bool stopped(const SolveState *s) { return s && s->done; }
 
-// (A x)_e for the 7-point brick operator with an explicit diagonal (FP32).
-__device__ inline float apply(const MgOp &op, int slot, int l, const float *__restrict__ x) {
+// uniform coupling of the brick (level 0), 0 when none
+__device__ inline float brick_kappa(const MgOp &op, int slot) { return op.kappa ? op.kappa[slot] : 0.0f; }
+
+// diagonal of cell e: 1 in a uniform brick
+__device__ inline float diag_of(const MgOp &op, int64_t e, float kap) { return kap != 0.0f ? 1.0f : op.d[e]; }
+
+// (A x)_e for the 7-point brick operator with an explicit diagonal (FP32); in a
+// uniform brick (kap != 0: unit diagonal, every face coupled with kap, all six
+// neighbours exist) A x = x - kap * (sum of the neighbours).
+__device__ inline float apply(const MgOp &op, int slot, int l, const float *__restrict__ x, float kap = 0.0f) {
     const int64_t e = (int64_t)slot * kBrickCells + l;
     const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
     const int *nb = op.nbr + 6 * slot;
+    if (kap != 0.0f) {
+        const float xm = i > 0 ? x[e - 1] : x[(int64_t)nb[0] * kBrickCells + l + 7];
+        const float xp = i < 7 ? x[e + 1] : x[(int64_t)nb[1] * kBrickCells + l - 7];
+        const float ym = j > 0 ? x[e - 8] : x[(int64_t)nb[2] * kBrickCells + l + 56];
+        const float yp = j < 7 ? x[e + 8] : x[(int64_t)nb[3] * kBrickCells + l - 56];
+        const float zm = k > 0 ? x[e - 64] : x[(int64_t)nb[4] * kBrickCells + l + 448];
+        const float zp = k < 7 ? x[e + 64] : x[(int64_t)nb[5] * kBrickCells + l - 448];
+        return x[e] - kap * (xm + xp + ym + yp + zm + zp);
+    }
     float acc = op.d[e] * x[e];
     if (i > 0) {
         acc -= op.cx[e] * x[e - 1];
@@ -100,11 +117,13 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
     c.cz[E] = (float)cz;
 }
 
-__global__ __launch_bounds__(NT) void k_mg_jacobi0(int64_t n, const float *__restrict__ b, const float *__restrict__ d,
-                                                   float *__restrict__ x, const SolveState *state) {
+__global__ __launch_bounds__(NT) void k_mg_jacobi0(MgOp op, const float *__restrict__ b, float *__restrict__ x,
+                                                   const SolveState *state) {
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e < n) x[e] = d[e] > 0.0f ? kMgOmega * b[e] / d[e] : 0.0f;
+    if (e >= (int64_t)op.n_active * kBrickCells) return;
+    const float dv = diag_of(op, e, brick_kappa(op, (int)(e >> 9)));
+    x[e] = dv > 0.0f ? kMgOmega * b[e] / dv : 0.0f;
 }
 
 __global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
@@ -113,7 +132,8 @@ __global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__rest
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)op.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
-    t[e] = op.d[e] > 0.0f ? b[e] - apply(op, slot, l, x) : 0.0f;
+    const float kap = brick_kappa(op, slot);
+    t[e] = diag_of(op, e, kap) > 0.0f ? b[e] - apply(op, slot, l, x, kap) : 0.0f;
 }
 
 __global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, const float *__restrict__ tf,
@@ -135,12 +155,13 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, 
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
 
-__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state) {
+__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state,
+                                                   const float *__restrict__ kappa) {
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)f.n_active * kBrickCells) return;
-    if (!(f.d[e] > 0.0f)) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
+    if (!(kappa && kappa[slot] != 0.0f) && !(f.d[e] > 0.0f)) return;
     const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
     const int4 B = f.coord[slot];
     const int ps = f.parent[slot];
@@ -149,35 +170,40 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, c
     f.x[e] += kMgAlpha * c.x[(int64_t)ps * kBrickCells + cl];
 }
 
-// one thread per cell (no partials) or, on the Krylov grid, one block per slot range
-template <bool PART>
+// one thread per cell (no partials)
 __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
-                                                  float *__restrict__ y, const SolveState *state, int per,
-                                                  double *__restrict__ part) {
+                                                  float *__restrict__ y, const SolveState *state) {
     if (stopped(state)) return;
-    if (!PART) {
-        const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-        if (e >= (int64_t)op.n_active * kBrickCells) return;
-        const int slot = (int)(e >> 9), l = (int)(e & 511);
-        const float dv = op.d[e];
-        y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x)) / dv : 0.0f;
-        return;
-    }
-    __shared__ double lds[NT / kWave];
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e >= (int64_t)op.n_active * kBrickCells) return;
+    const int slot = (int)(e >> 9), l = (int)(e & 511);
+    const float kap = brick_kappa(op, slot);
+    const float dv = diag_of(op, e, kap);
+    y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
+}
+
+// on the Krylov grid (block g owns a slot range, as the CG kernels do) with
+// part[block] = sum of b*y: 1024 threads keep two bricks in flight per block
+constexpr int NP = 1024;
+__global__ __launch_bounds__(NP) void k_mg_smooth_part(MgOp op, const float *__restrict__ b,
+                                                       const float *__restrict__ x, float *__restrict__ y,
+                                                       const SolveState *state, int per, double *__restrict__ part) {
+    if (stopped(state)) return;
+    __shared__ double lds[NP / kWave];
     const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
     const int t0 = g * per, t1 = t0 + per < op.n_active ? t0 + per : op.n_active;
+    const int l = threadIdx.x & (kBrickCells - 1);
     double acc = 0.0;
-    for (int slot = t0; slot < t1; ++slot) {
-        for (int l = threadIdx.x; l < kBrickCells; l += NT) {
-            const int64_t e = (int64_t)slot * kBrickCells + l;
-            const float dv = op.d[e];
-            const float bv = b[e];
-            const float yv = dv > 0.0f ? x[e] + kMgOmega * (bv - apply(op, slot, l, x)) / dv : 0.0f;
-            y[e] = yv;
-            acc += (double)bv * (double)yv;
-        }
+    for (int slot = t0 + (threadIdx.x >> 9); slot < t1; slot += NP / kBrickCells) {
+        const int64_t e = (int64_t)slot * kBrickCells + l;
+        const float kap = brick_kappa(op, slot);
+        const float dv = diag_of(op, e, kap);
+        const float bv = b[e];
+        const float yv = dv > 0.0f ? x[e] + kMgOmega * (bv - apply(op, slot, l, x, kap)) / dv : 0.0f;
+        y[e] = yv;
+        acc += (double)bv * (double)yv;
     }
-    const double s = block_sum<NT>(acc, lds);
+    const double s = block_sum<NP>(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
@@ -208,8 +234,9 @@ void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c)
     hipLaunchKernelGGL(k_mg_coarsen, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c);
 }
 
-void launch_mg_jacobi0(hipStream_t st, int64_t n, const float *b, const float *d, float *x, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_jacobi0, dim3(blocks(n)), dim3(NT), 0, st, n, b, d, x, state);
+void launch_mg_jacobi0(hipStream_t st, const MgOp &op, const float *b, float *x, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_jacobi0, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x,
+                       state);
 }
 
 void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
@@ -224,17 +251,19 @@ void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c
                        omega, state);
 }
 
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state);
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state,
+                       const float *kappa) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state,
+                       kappa);
 }
 
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
                       const SolveState *state, const Grid *g, double *part) {
     if (g && part)
-        hipLaunchKernelGGL(k_mg_smooth<true>, dim3(g->blocks), dim3(NT), 0, st, op, b, x, y, state, g->per, part);
+        hipLaunchKernelGGL(k_mg_smooth_part, dim3(g->blocks), dim3(NP), 0, st, op, b, x, y, state, g->per, part);
     else
-        hipLaunchKernelGGL(k_mg_smooth<false>, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op,
-                           b, x, y, state, 0, nullptr);
+        hipLaunchKernelGGL(k_mg_smooth, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x,
+                           y, state);
 }
 
 void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x, int cells,

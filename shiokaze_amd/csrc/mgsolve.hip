@@ -290,9 +290,8 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
     hipStream_t st = c->stream;
     const int L = (int)h->lv.size() - 1;
     MgLevelHost &L0 = h->lv[0];
-    const int64_t n0 = (int64_t)c->n_active * kBrickCells;
-    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active};
-    launch_mg_jacobi0(st, n0, r, L0.d.p, L0.v.x, state);
+    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
+    launch_mg_jacobi0(st, op0, r, L0.v.x, state);
     launch_mg_residual(st, op0, r, L0.v.x, L0.v.t, state);
     for (int l = 1; l < L; ++l) {
         MgLevelHost &F = h->lv[l - 1], &C = h->lv[l];
@@ -313,7 +312,7 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
         std::swap(F.v.x, F.v.t);
         std::swap(F.x.p, F.t.p);
     }
-    launch_mg_prolong(st, L0.v, h->lv[1].v, state);
+    launch_mg_prolong(st, L0.v, h->lv[1].v, state, c->kappa32.p);
     launch_mg_smooth(st, op0, r, L0.v.x, z, state, g, part);
 }
 

@@ -76,7 +76,7 @@ __global__ void k_conv_pack(const float *__restrict__ w, int cin, int cout, int 
 // of the MT = cout/16 output tiles (blockIdx.z = z-tile * MT/MTB + output group; small
 // grids split the outputs over blocks to fill the chip).  The 18 x 6 x 6 input halo
 // of the tile is staged in LDS CC channels at a time (one 16-byte row per voxel and 8
-// channels, padded by 16 bytes per voxel).  Wave w owns the tile's z-plane w: 4 rows
+// channels, the voxel stride padded to an odd number of 16-byte rows).  Wave w owns the tile's z-plane w: 4 rows
 // of 16 voxels = 4 MFMA N-tiles, times MTB M-tiles of 16 outputs, so each A fragment
 // (weights, from L1/L2) feeds 4 MFMAs and each B fragment (inputs, from LDS) MTB.
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
@@ -88,7 +88,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
                                                   int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32) {
     constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
-    constexpr int VS = G8 + 1;        // voxel stride in uint4 (one group of padding)
+    // voxel stride in uint4, odd so that 16 lanes reading 16 consecutive voxels hit 16
+    // disjoint 4-bank groups of the 64-bank LDS
+    constexpr int VS = (G8 & 1) ? G8 : G8 + 1;
     constexpr int KPC = (27 * CC + 31) / 32;
     __shared__ uint4 lds[kHalo * VS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
