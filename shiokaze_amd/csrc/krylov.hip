@@ -20,8 +20,9 @@
 //
 // The brick stencil runs one wavefront per brick with one z-column of 8 cells per
 // lane (lane = i + 8 j): x and y neighbours come from lane shuffles, z neighbours
-// from the lane's own registers, and only the brick's edge lanes read the six
-// neighbour bricks.
+// from the lane's own registers.  Each lateral neighbour's facing plane (8 x 8
+// cells) is read with one coalesced load per array across the wave and handed to
+// the edge lanes by shuffles; the z planes map onto the lanes directly.
 #include "krylov.h"
 
 #include <hip/hip_ext.h>
@@ -210,55 +211,45 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             if (UNIT && op.kappa) mkap = op.kappa[nslot];
         }
         const int64_t base = (int64_t)slot * NT + lane;
+        // halo planes of the four lateral neighbours, one coalesced load per plane: lane
+        // L = a + 8c holds the plane's cell (a, c) (a along the plane's in-brick axis,
+        // c = z); the edge lanes fetch them below with a shuffle from lane j + 8k / i + 8k
+        const int pa = lane & 7, pc = lane >> 3;
+        T hxl = T(0), hxh = T(0), hyl = T(0), hyh = T(0);
+        if (nxm >= 0) {
+            const int64_t e = (int64_t)nxm * NT + 7 + 8 * pa + 64 * pc;
+            hxl = r[e] + b * sin[e];
+        }
+        if (nxp >= 0) {
+            const int64_t e = (int64_t)nxp * NT + 8 * pa + 64 * pc;
+            hxh = r[e] + b * sin[e];
+        }
+        if (nym >= 0) {
+            const int64_t e = (int64_t)nym * NT + pa + 56 + 64 * pc;
+            hyl = r[e] + b * sin[e];
+        }
+        if (nyp >= 0) {
+            const int64_t e = (int64_t)nyp * NT + pa + 64 * pc;
+            hyh = r[e] + b * sin[e];
+        }
+        T szlo = T(0), szhi = T(0);
+        if (nzm >= 0) {
+            const int64_t e = (int64_t)nzm * NT + lane + 448;
+            szlo = r[e] + b * sin[e];
+        }
+        if (nzp >= 0) {
+            const int64_t e = (int64_t)nzp * NT + lane;
+            szhi = r[e] + b * sin[e];
+        }
+        T sv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t e = base + 64 * k;
+            sv[k] = r[e] + b * sin[e];
+        }
         if (UNIT && kap != C(0)) {
             // uniform brick: one coupling value, no coefficient traffic
             const T kv = (T)kap;
-            T sv[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t e = base + 64 * k;
-                sv[k] = r[e] + b * sin[e];
-            }
-            T szlo = T(0), szhi = T(0);
-            if (nzm >= 0) {
-                const int64_t e = (int64_t)nzm * NT + lane + 448;
-                szlo = r[e] + b * sin[e];
-            }
-            if (nzp >= 0) {
-                const int64_t e = (int64_t)nzp * NT + lane;
-                szhi = r[e] + b * sin[e];
-            }
-            T hxlo[8], hxhi[8], hylo[8], hyhi[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k) hxlo[k] = hxhi[k] = hylo[k] = hyhi[k] = T(0);
-            if (i == 0 && nxm >= 0) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int64_t e = (int64_t)nxm * NT + lane + 7 + 64 * k;
-                    hxlo[k] = r[e] + b * sin[e];
-                }
-            }
-            if (i == 7 && nxp >= 0) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int64_t e = (int64_t)nxp * NT + lane - 7 + 64 * k;
-                    hxhi[k] = r[e] + b * sin[e];
-                }
-            }
-            if (j == 0 && nym >= 0) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int64_t e = (int64_t)nym * NT + lane + 56 + 64 * k;
-                    hylo[k] = r[e] + b * sin[e];
-                }
-            }
-            if (j == 7 && nyp >= 0) {
-#pragma unroll
-                for (int k = 0; k < 8; ++k) {
-                    const int64_t e = (int64_t)nyp * NT + lane - 56 + 64 * k;
-                    hyhi[k] = r[e] + b * sin[e];
-                }
-            }
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const T sk = sv[k];
@@ -266,10 +257,14 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
                 T xp = lane_shfl(sk, lane + 1);
                 T ym = lane_shfl(sk, lane - 8);
                 T yp = lane_shfl(sk, lane + 8);
-                if (i == 0) xm = hxlo[k];
-                if (i == 7) xp = hxhi[k];
-                if (j == 0) ym = hylo[k];
-                if (j == 7) yp = hyhi[k];
+                const T hxm = lane_shfl(hxl, j + 8 * k);
+                const T hxp = lane_shfl(hxh, j + 8 * k);
+                const T hym = lane_shfl(hyl, i + 8 * k);
+                const T hyp = lane_shfl(hyh, i + 8 * k);
+                if (i == 0) xm = hxm;
+                if (i == 7) xp = hxp;
+                if (j == 0) ym = hym;
+                if (j == 7) yp = hyp;
                 const T zm = k > 0 ? sv[k - 1] : szlo;
                 const T zp = k < 7 ? sv[k + 1] : szhi;
                 // same association as the general path: c*a + c*b + ... summed left to right
@@ -281,60 +276,20 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             }
             continue;
         }
-        T sv[8], cxv[8], cyv[8], czv[8], dv[8];
+        T cxv[8], cyv[8], czv[8], dv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int64_t e = base + 64 * k;
-            sv[k] = r[e] + b * sin[e];
             cxv[k] = (T)op.cx[e];
             cyv[k] = (T)op.cy[e];
             czv[k] = (T)op.cz[e];
             if (!UNIT) dv[k] = (T)op.diag[e];
         }
-        // halos: s' of the neighbour bricks' facing planes, and their couplings
-        T szlo = T(0), szhi = T(0), czhi = T(0);
-        if (nzm >= 0) {
-            const int64_t e = (int64_t)nzm * NT + lane + 448;
-            szlo = r[e] + b * sin[e];
-        }
-        if (nzp >= 0) {
-            const int64_t e = (int64_t)nzp * NT + lane;
-            szhi = r[e] + b * sin[e];
-            czhi = (T)op.cz[e];
-        }
-        T hxlo[8], hxhi[8], cxhi[8], hylo[8], hyhi[8], cyhi[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) hxlo[k] = hxhi[k] = cxhi[k] = hylo[k] = hyhi[k] = cyhi[k] = T(0);
-        if (i == 0 && nxm >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t e = (int64_t)nxm * NT + lane + 7 + 64 * k;
-                hxlo[k] = r[e] + b * sin[e];
-            }
-        }
-        if (i == 7 && nxp >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t e = (int64_t)nxp * NT + lane - 7 + 64 * k;
-                hxhi[k] = r[e] + b * sin[e];
-                cxhi[k] = (T)op.cx[e];
-            }
-        }
-        if (j == 0 && nym >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t e = (int64_t)nym * NT + lane + 56 + 64 * k;
-                hylo[k] = r[e] + b * sin[e];
-            }
-        }
-        if (j == 7 && nyp >= 0) {
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const int64_t e = (int64_t)nyp * NT + lane - 56 + 64 * k;
-                hyhi[k] = r[e] + b * sin[e];
-                cyhi[k] = (T)op.cy[e];
-            }
-        }
+        // couplings across the +x / +y / +z faces live in the neighbour bricks
+        T cxh = T(0), cyh = T(0), czhi = T(0);
+        if (nxp >= 0) cxh = (T)op.cx[(int64_t)nxp * NT + 8 * pa + 64 * pc];
+        if (nyp >= 0) cyh = (T)op.cy[(int64_t)nyp * NT + pa + 64 * pc];
+        if (nzp >= 0) czhi = (T)op.cz[(int64_t)nzp * NT + lane];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const T sk = sv[k];
@@ -344,15 +299,21 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             T yp = lane_shfl(sk, lane + 8);
             T cxp = lane_shfl(cxv[k], lane + 1);
             T cyp = lane_shfl(cyv[k], lane + 8);
-            if (i == 0) xm = hxlo[k];
+            const T hxm = lane_shfl(hxl, j + 8 * k);
+            const T hxp = lane_shfl(hxh, j + 8 * k);
+            const T hym = lane_shfl(hyl, i + 8 * k);
+            const T hyp = lane_shfl(hyh, i + 8 * k);
+            const T hcx = lane_shfl(cxh, j + 8 * k);
+            const T hcy = lane_shfl(cyh, i + 8 * k);
+            if (i == 0) xm = hxm;
             if (i == 7) {
-                xp = hxhi[k];
-                cxp = cxhi[k];
+                xp = hxp;
+                cxp = hcx;
             }
-            if (j == 0) ym = hylo[k];
+            if (j == 0) ym = hym;
             if (j == 7) {
-                yp = hyhi[k];
-                cyp = cyhi[k];
+                yp = hyp;
+                cyp = hcy;
             }
             const T zm = k > 0 ? sv[k - 1] : szlo;
             const T zp = k < 7 ? sv[k + 1] : szhi;

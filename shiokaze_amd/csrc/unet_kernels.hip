@@ -76,9 +76,10 @@ __global__ void k_conv_pack(const float *__restrict__ w, int cin, int cout, int 
 // of the MT = cout/16 output tiles (blockIdx.z = z-tile * MT/MTB + output group; small
 // grids split the outputs over blocks to fill the chip).  The 18 x 6 x 6 input halo
 // of the tile is staged in LDS CC channels at a time (one 16-byte row per voxel and 8
-// channels, the voxel stride padded to an odd number of 16-byte rows).  Wave w owns the tile's z-plane w: 4 rows
-// of 16 voxels = 4 MFMA N-tiles, times MTB M-tiles of 16 outputs, so each A fragment
-// (weights, from L1/L2) feeds 4 MFMAs and each B fragment (inputs, from LDS) MTB.
+// channels, the voxel stride padded to an odd number of 16-byte rows).  Wave w owns
+// the tile's z-plane w: 4 rows of 16 voxels = 4 MFMA N-tiles, times MTB M-tiles of 16
+// outputs, so each A fragment (weights, from L1/L2) feeds 4 MFMAs and each B fragment
+// (inputs, from LDS) MTB.
 constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
