@@ -55,21 +55,24 @@ constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)
 void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0);
 // coarse operator of level `c` from level `f`
 void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c);
-// x = w b / d (level 0 pre-smoothing from zero); st may be nullptr
-void launch_mg_jacobi0(hipStream_t st, const MgOp &op, const float *b, float *x, const SolveState *state);
 // t = b - A x
 void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
                         const SolveState *state);
 // b_c = P0' t_f, x_c = w b_c / d_c (or 0 with w = 0)
 void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
                         const SolveState *state);
-// x_f += alpha P0 x_c on the fine unknowns (kappa: the fine level's uniform bricks, or nullptr)
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state,
-                       const float *kappa = nullptr);
-// y = x + w (b - A x) / d; with `g` and `part` (level 0): launched on the Krylov grid
-// and part[block] = sum of b*y over the block (the r.z partials of PCG)
+// x_f += alpha P0 x_c on the fine unknowns (levels >= 1)
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state);
+// y = x + w (b - A x) / d (levels >= 1)
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
-                      const SolveState *state, const Grid *g = nullptr, double *part = nullptr);
+                      const SolveState *state);
+// level 0 (scaled system, unit diagonal on the unknowns, vectors zero elsewhere):
+// t = r - A (w r), the residual after the pre-smoothing step x0 = w r
+void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state);
+// y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 (prolongation fused into the
+// post-smoothing), on the Krylov grid g with part[block] = sum of r*y
+void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
+                     const float *xc, float *y, const SolveState *state, const Grid &g, double *part);
 // coarsest level: x[idx[i]] = sum_j inv[i][j] b[idx[j]] (n <= kMgCoarseMax), other cells 0
 void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x,
                       int cells, const SolveState *state);

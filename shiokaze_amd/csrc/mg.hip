@@ -117,15 +117,6 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
     c.cz[E] = (float)cz;
 }
 
-__global__ __launch_bounds__(NT) void k_mg_jacobi0(MgOp op, const float *__restrict__ b, float *__restrict__ x,
-                                                   const SolveState *state) {
-    if (stopped(state)) return;
-    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e >= (int64_t)op.n_active * kBrickCells) return;
-    const float dv = diag_of(op, e, brick_kappa(op, (int)(e >> 9)));
-    x[e] = dv > 0.0f ? kMgOmega * b[e] / dv : 0.0f;
-}
-
 __global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
                                                     float *__restrict__ t, const SolveState *state) {
     if (stopped(state)) return;
@@ -155,13 +146,12 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, 
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
 
-__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state,
-                                                   const float *__restrict__ kappa) {
+__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state) {
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)f.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
-    if (!(kappa && kappa[slot] != 0.0f) && !(f.d[e] > 0.0f)) return;
+    if (!(f.d[e] > 0.0f)) return;
     const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
     const int4 B = f.coord[slot];
     const int ps = f.parent[slot];
@@ -170,7 +160,7 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, c
     f.x[e] += kMgAlpha * c.x[(int64_t)ps * kBrickCells + cl];
 }
 
-// one thread per cell (no partials)
+// y = x + w (b - A x) / d, one thread per cell
 __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
                                                   float *__restrict__ y, const SolveState *state) {
     if (stopped(state)) return;
@@ -182,26 +172,93 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restri
     y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
 }
 
-// on the Krylov grid (block g owns a slot range, as the CG kernels do) with
-// part[block] = sum of b*y: 1024 threads keep two bricks in flight per block
-constexpr int NP = 1024;
-__global__ __launch_bounds__(NP) void k_mg_smooth_part(MgOp op, const float *__restrict__ b,
-                                                       const float *__restrict__ x, float *__restrict__ y,
-                                                       const SolveState *state, int per, double *__restrict__ part) {
+constexpr int NP = 1024;  // threads of the level-0 post-smoothing (two bricks in flight per block)
+
+// ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
+// unknowns and the diagonal is 1 on them, so x0 = w r needs no division and is never
+// stored: the pre-smoothing residual and the post-smoothing recompute it from r.
+
+// cell index on level 1 of the level-0 cell (brick coord B, local l)
+__device__ inline int coarse_local(int4 B, int l) {
+    return ((B.x & 1) * 4 + ((l & 7) >> 1)) + 8 * ((B.y & 1) * 4 + (((l >> 3) & 7) >> 1)) +
+           64 * ((B.z & 1) * 4 + ((l >> 6) >> 1));
+}
+
+// t0 = r - A (w r) on the unknowns (0 elsewhere)
+__global__ __launch_bounds__(NT) void k_mg_pre0(MgOp op, const float *__restrict__ r, float *__restrict__ t,
+                                                const SolveState *state) {
+    if (stopped(state)) return;
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e >= (int64_t)op.n_active * kBrickCells) return;
+    const int slot = (int)(e >> 9), l = (int)(e & 511);
+    const float kap = brick_kappa(op, slot);
+    // A (w r) = w A r by linearity
+    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - kMgOmega * apply(op, slot, l, r, kap) : 0.0f;
+}
+
+// value of x0' = w r + alpha P0 x1 at level-0 cell (slot, l) of brick B with parent ps
+__device__ inline float xprime(const float *__restrict__ r, const float *__restrict__ xc, int slot, int l, int4 B,
+                               int ps) {
+    return kMgOmega * r[(int64_t)slot * kBrickCells + l] + kMgAlpha * xc[(int64_t)ps * kBrickCells + coarse_local(B, l)];
+}
+
+// y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
+// prolongation fused into the post-smoothing), on the Krylov grid with r.y partials
+__global__ __launch_bounds__(NP) void k_mg_post0(MgOp op, const int4 *__restrict__ coord,
+                                                 const int *__restrict__ parent, const float *__restrict__ r,
+                                                 const float *__restrict__ xc, float *__restrict__ y,
+                                                 const SolveState *state, int per, double *__restrict__ part) {
     if (stopped(state)) return;
     __shared__ double lds[NP / kWave];
     const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
     const int t0 = g * per, t1 = t0 + per < op.n_active ? t0 + per : op.n_active;
     const int l = threadIdx.x & (kBrickCells - 1);
+    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
     double acc = 0.0;
     for (int slot = t0 + (threadIdx.x >> 9); slot < t1; slot += NP / kBrickCells) {
         const int64_t e = (int64_t)slot * kBrickCells + l;
         const float kap = brick_kappa(op, slot);
         const float dv = diag_of(op, e, kap);
-        const float bv = b[e];
-        const float yv = dv > 0.0f ? x[e] + kMgOmega * (bv - apply(op, slot, l, x, kap)) / dv : 0.0f;
+        float yv = 0.0f;
+        const float rv = r[e];
+        if (dv > 0.0f) {
+            const int4 B = coord[slot];
+            const int ps = parent[slot];
+            const int *nb = op.nbr + 6 * slot;
+            const float xs = xprime(r, xc, slot, l, B, ps);
+            // neighbour (slot', l') of each face, the face coupling, and x0' there
+            float sum = 0.0f;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                const int axis = f >> 1, hi = f & 1;
+                const int pos = axis == 0 ? i : (axis == 1 ? j : k);
+                const int stride = axis == 0 ? 1 : (axis == 1 ? 8 : 64);
+                int ns = slot, nl;
+                if (hi ? pos < 7 : pos > 0) {
+                    nl = hi ? l + stride : l - stride;
+                } else {
+                    ns = nb[f];
+                    nl = hi ? l - 7 * stride : l + 7 * stride;
+                }
+                if (ns < 0) continue;
+                // coupling: low face at this cell, high face at the neighbour
+                float c;
+                if (kap != 0.0f) {
+                    c = kap;
+                } else {
+                    const float *cc = axis == 0 ? op.cx : (axis == 1 ? op.cy : op.cz);
+                    c = hi ? cc[(int64_t)ns * kBrickCells + nl] : cc[e];
+                }
+                if (c == 0.0f) continue;
+                const int4 NB = ns == slot ? B : coord[ns];
+                const int nps = ns == slot ? ps : parent[ns];
+                sum += c * xprime(r, xc, ns, nl, NB, nps);
+            }
+            const float ax = xs - sum;
+            yv = xs + kMgOmega * (rv - ax);
+        }
         y[e] = yv;
-        acc += (double)bv * (double)yv;
+        acc += (double)rv * (double)yv;
     }
     const double s = block_sum<NP>(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
@@ -234,11 +291,6 @@ void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c)
     hipLaunchKernelGGL(k_mg_coarsen, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c);
 }
 
-void launch_mg_jacobi0(hipStream_t st, const MgOp &op, const float *b, float *x, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_jacobi0, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x,
-                       state);
-}
-
 void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
                         const SolveState *state) {
     hipLaunchKernelGGL(k_mg_residual, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x, t,
@@ -251,19 +303,23 @@ void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c
                        omega, state);
 }
 
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state,
-                       const float *kappa) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state,
-                       kappa);
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state);
 }
 
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
-                      const SolveState *state, const Grid *g, double *part) {
-    if (g && part)
-        hipLaunchKernelGGL(k_mg_smooth_part, dim3(g->blocks), dim3(NP), 0, st, op, b, x, y, state, g->per, part);
-    else
-        hipLaunchKernelGGL(k_mg_smooth, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x,
-                           y, state);
+                      const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_smooth, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x, y,
+                       state);
+}
+
+void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_pre0, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, r, t, state);
+}
+
+void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
+                     const float *xc, float *y, const SolveState *state, const Grid &g, double *part) {
+    hipLaunchKernelGGL(k_mg_post0, dim3(g.blocks), dim3(NP), 0, st, op, coord, parent, r, xc, y, state, g.per, part);
 }
 
 void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x, int cells,

@@ -140,7 +140,9 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
         } else {
             HIPC(c, L.d.ensure(n));
         }
-        for (DevBuf<float> *b : {&L.b, &L.x, &L.t}) HIPC(c, b->ensure(n));
+        if (l > 0)
+            for (DevBuf<float> *b : {&L.b, &L.x}) HIPC(c, b->ensure(n));
+        HIPC(c, L.t.ensure(n));  // level 0 keeps only the pre-smoothing residual
         if (!L.h_parent.empty()) {
             HIPC(c, L.parent.ensure(L.h_parent.size()));
             HIPC(c, hipMemcpyAsync(L.parent.p, L.h_parent.data(), sizeof(int) * L.h_parent.size(),
@@ -283,16 +285,15 @@ int mg_setup(shk_ctx *c) {
     return SHK_OK;
 }
 
-// z = M r on level 0 (brick layout).  With g and part, the last smoothing runs on the
-// Krylov grid and writes the r.z partials.
+// z = M r on level 0 (brick layout).  The last kernel runs on the Krylov grid g and
+// writes the r.z partials into part.
 void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, const Grid *g, double *part) {
     MgHierarchy *h = c->mg;
     hipStream_t st = c->stream;
     const int L = (int)h->lv.size() - 1;
     MgLevelHost &L0 = h->lv[0];
     const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
-    launch_mg_jacobi0(st, op0, r, L0.v.x, state);
-    launch_mg_residual(st, op0, r, L0.v.x, L0.v.t, state);
+    launch_mg_pre0(st, op0, r, L0.v.t, state);
     for (int l = 1; l < L; ++l) {
         MgLevelHost &F = h->lv[l - 1], &C = h->lv[l];
         if (!C.v.n_active) continue;
@@ -312,8 +313,7 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
         std::swap(F.v.x, F.v.t);
         std::swap(F.x.p, F.t.p);
     }
-    launch_mg_prolong(st, L0.v, h->lv[1].v, state, c->kappa32.p);
-    launch_mg_smooth(st, op0, r, L0.v.x, z, state, g, part);
+    launch_mg_post0(st, op0, c->coord.p, L0.parent.p, r, h->lv[1].v.x, z, state, *g, part);
 }
 
 // Mixed-precision refinement loop with MG-preconditioned inner CG.
@@ -419,7 +419,7 @@ extern "C" int shk_mg_apply(shk_ctx *c, const float *r, float *z) {
     const unsigned gb = (unsigned)((nb + 255) / 256), gd = (unsigned)((nd + 255) / 256);
     hipLaunchKernelGGL(k_dense_to_bricks, dim3(gb), dim3(256), 0, st, d.nx, d.ny, d.nz, c->coord.p, c->n_active, r,
                        c->r32.p);
-    mg_vcycle(c, c->r32.p, c->mg->z.p, nullptr, nullptr, nullptr);
+    mg_vcycle(c, c->r32.p, c->mg->z.p, nullptr, &c->grid, c->mg->pz.p);
     hipLaunchKernelGGL(k_bricks_to_dense, dim3(gd), dim3(256), 0, st, d.nx, d.ny, d.nz, c->brick_slot.p, c->nbx,
                        c->nby, c->mg->z.p, z);
     HIPC(c, hipGetLastError());

@@ -612,8 +612,9 @@ int timed_precondition(shk_ctx *c, UnetNet *net, Grid3 P, UnetPrecond *u, double
 
 // Learned flexible CG (FCG(1), Notay 2000) on the dense padded scaled system in
 // FP32 with FP64 dots, inside the same FP64 refinement loop as the mixed solver.
-// Falls back to plain Jacobi-PCG inner solves when an inner FCG stalls or hits
-// its cap; the result records the fallback.
+// Falls back to plain Jacobi-PCG inner solves when an inner FCG stalls (less than
+// 10% residual reduction in 20 iterations) or hits its cap; the result records the
+// fallback.
 int unet_solve(shk_ctx *c, shk_result *res) {
     int rc;
     if ((rc = ensure_precond(c))) return rc;
@@ -686,10 +687,10 @@ int unet_solve(shk_ctx *c, shk_result *res) {
                     stalled = true;
                     break;
                 }
-                if (rn < 0.7 * best) {
+                if (rn < 0.9 * best) {
                     best = rn;
                     best_it = it;
-                } else if (it - best_it >= 10) {  // no 30% progress in 10 iterations
+                } else if (it - best_it >= 20) {  // less than 10% progress in 20 iterations
                     stalled = true;
                     break;
                 }

@@ -19,10 +19,9 @@ relative residual, and reports iterations, times and fallbacks.
 from __future__ import annotations
 
 import argparse
+import collections
 import json
-import queue
 import sys
-import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -81,21 +80,24 @@ class SampleMaker:
 
 
 def _scene_stream(kind, n, seeds, workers):
-    """Host scene generation on a thread pool (the generator releases the GIL)."""
-    q: queue.Queue = queue.Queue(maxsize=2 * workers)
-
-    def run():
-        with ThreadPoolExecutor(workers) as ex:
-            for sc in ex.map(lambda s: scn.generate(kind, n, s), seeds):
-                q.put(sc)
-        q.put(None)
-
-    threading.Thread(target=run, daemon=True).start()
-    while True:
-        sc = q.get()
-        if sc is None:
-            return
-        yield sc
+    """Host scene generation on a thread pool (the generator releases the GIL), at most
+    2 * workers scenes ahead of the consumer; with a list of sizes, seed s is generated
+    at n[s % len(n)]."""
+    sizes = n if isinstance(n, (list, tuple)) else [n]
+    seeds = list(seeds)
+    with ThreadPoolExecutor(workers) as ex:
+        pending = collections.deque()
+        it = iter(seeds)
+        for s in it:
+            pending.append(ex.submit(scn.generate, kind, sizes[s % len(sizes)], s))
+            if len(pending) >= 2 * workers:
+                break
+        while pending:
+            sc = pending.popleft().result()
+            nxt = next(it, None)
+            if nxt is not None:
+                pending.append(ex.submit(scn.generate, kind, sizes[nxt % len(sizes)], nxt))
+            yield sc
 
 
 def evaluate(weights: str, n: int, kind: int, seeds, residual_tol: float, workers: int = 8) -> dict:
@@ -136,6 +138,9 @@ def evaluate(weights: str, n: int, kind: int, seeds, residual_tol: float, worker
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--sizes", default=None,
+                    help="comma-separated grid sizes to train on in turn (default: --n); seed s uses sizes[s % k]")
+    ap.add_argument("--eval-sizes", default=None, help="comma-separated grid sizes to evaluate on (default: --n)")
     ap.add_argument("--kind", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=2000, help="training scenes (seeds 0..scenes-1)")
     ap.add_argument("--eval-seed0", type=int, default=2000)
@@ -159,19 +164,20 @@ def main(argv=None):
     net = UNet(seed=args.seed)
     if args.init:
         net.load(args.init)
-    maker = SampleMaker(args.n, args.kind, args.label_residual)
+    sizes = [int(v) for v in args.sizes.split(",")] if args.sizes else [args.n]
+    makers = {n: SampleMaker(n, args.kind, args.label_residual) for n in sizes}
     ring: list = []
     curve = []
     t0 = time.perf_counter()
     step = 0
     total_steps = args.scenes * args.steps_per_scene
-    log(f"n={args.n} scenes={args.scenes} params={net.num_params} steps<={total_steps} budget={args.time_budget}s")
+    log(f"sizes={sizes} scenes={args.scenes} params={net.num_params} steps<={total_steps} budget={args.time_budget}s")
     t_gen = t_train = 0.0
     seen = 0
     ema = None
-    for sc in _scene_stream(args.kind, args.n, range(args.scenes), args.workers):
+    for sc in _scene_stream(args.kind, sizes, range(args.scenes), args.workers):
         ta = time.perf_counter()
-        samples = maker.samples(sc)
+        samples = makers[sc["shape"][0]].samples(sc)
         torch.cuda.synchronize()
         t_gen += time.perf_counter() - ta
         if not samples:
@@ -200,16 +206,20 @@ def main(argv=None):
             log(f"time budget reached after {seen} scenes")
             break
     net.save(args.out)
-    maker.close()
+    for m in makers.values():
+        m.close()
     net.close()
     log(f"saved {args.out} after {step} steps on {seen} scenes")
-    report = {"n": args.n, "scenes_used": seen, "steps": step, "final_loss_ema": ema, "train_s": t_train,
+    report = {"sizes": sizes, "scenes_used": seen, "steps": step, "final_loss_ema": ema, "train_s": t_train,
               "gen_s": t_gen, "curve": curve}
     if args.eval > 0:
-        ev = evaluate(args.out, args.n, args.kind, range(args.eval_seed0, args.eval_seed0 + args.eval),
-                      args.eval_residual, workers=args.workers)
-        log("eval " + json.dumps(ev))
-        report["eval"] = ev
+        report["eval"] = {}
+        for n in ([int(v) for v in args.eval_sizes.split(",")] if args.eval_sizes else [args.n]):
+            count = args.eval if n <= 128 else max(1, args.eval // 4)
+            ev = evaluate(args.out, n, args.kind, range(args.eval_seed0, args.eval_seed0 + count), args.eval_residual,
+                          workers=args.workers)
+            log(f"eval {n}: " + json.dumps(ev))
+            report["eval"][str(n)] = ev
     if args.report:
         with open(args.report, "w") as f:
             json.dump(report, f, indent=1)
