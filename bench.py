@@ -77,7 +77,7 @@ def main():
                     help="steps after the timed ones with per-kernel HIP event timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a known-size kernel for HBM counter scaling)")
-    ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128.bin"),
+    ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128_256.bin"),
                     help="trained network for the learned-preconditioner line of --extra")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")

@@ -4,10 +4,11 @@
 // x0 = 0, r = b, z = M^-1 r, s = z, then per iteration
 //   q = A s; alpha = rho / s.q; x += alpha s; r -= alpha q;
 //   stop if max|r| <= tol * max|b|; z = M^-1 r; rho' = z.r; s = z + (rho'/rho) s.
-// M is the identity (the reference's effective preconditioner, pcg_solver.h:379-384)
-// or diag(A) (Jacobi).  Each of the three fused kernels publishes one partial per
-// workgroup; the last workgroup to finish folds the partials in a fixed order and
-// updates the SolveState, so results are deterministic and no host round trip is
+// M is the identity (the reference's effective preconditioner, pcg_solver.h:379-384),
+// diag(A) (Jacobi, as a symmetric scaling of the system) or a multigrid V-cycle
+// (mg.h).  Each kernel publishes one partial per workgroup; every workgroup of the
+// next kernel folds them in one fixed order, so results are deterministic, every
+// workgroup sees the same scalars, and no host round trip or grid-wide fence is
 // needed between iterations.
 #pragma once
 
