@@ -41,7 +41,7 @@ def log(msg):
 # WRITE_SIZE x 1.0, factors from a 256 MiB copy in the same run) on the pool+sphere scene at
 # 256^3, mixed precision, Jacobi: profiles/hbm_traffic_pool_sphere_256_mixed_jacobi.md.
 # Reported as roofline.traffic only for that workload.
-MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 24.19, "axpy_norm": 24.03}
+MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 26.08, "axpy_norm": 24.03}
 
 
 def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
@@ -205,7 +205,7 @@ def main():
 
     extra = {}
     plain_cg_iters = None
-    if args.extra or not args.no_cpu_baseline:
+    if args.extra or (not args.no_cpu_baseline and n > 96):
         # plain FP64 CG (the reference's effective iteration) for the CPU extrapolation
         pcg = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "none", "CheckInterval": 64})
         rp = step(pcg)
@@ -262,7 +262,24 @@ def main():
         log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and n <= 96:
+        # small grids (config 1): the whole reference-faithful solve on one host core,
+        # and the GPU pressure compared with the CPU one of the same run
+        from oracle import oracle as O
+        O.set_threads(1)
+        rr = O.project_timed(sc, tol=args.residual, mode=O.SHK_OR_REFERENCE)
+        total_ms = float(sum(rr["t_ms"][:3]))
+        p_gpu = ps.get_pressure().cpu().numpy().astype(np.float64)
+        p_cpu = np.asarray(rr["pressure"], dtype=np.float64)
+        p_rel = float(np.linalg.norm(p_gpu - p_cpu) / max(np.linalg.norm(p_cpu), 1e-300))
+        cpu = {"value": 1e3 / total_ms, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": (f"{n}^3 scene {args.scene}: the whole oracle projection, reference-faithful CG (MIC(0) "
+                          f"factor and two discarded triangular solves per iteration) to {args.residual:g}, "
+                          f"{rr['iters']} iterations, single thread"),
+               "ms_per_solve": round(total_ms, 1), "iters": rr["iters"],
+               "pressure_rel_l2_gpu_vs_cpu": p_rel}
+        log(f"cpu baseline: {cpu}")
+    elif not args.no_cpu_baseline:
         from oracle import oracle as O
         O.set_threads(1)
         t_runs = {}

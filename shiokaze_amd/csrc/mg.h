@@ -70,9 +70,11 @@ void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const floa
 // t = r - A (w r), the residual after the pre-smoothing step x0 = w r
 void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state);
 // y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 (prolongation fused into the
-// post-smoothing), on the Krylov grid g with part[block] = sum of r*y
+// post-smoothing), one thread per cell; the r.y sums of its blocks (cell_part, one per
+// 256 cells) are folded in order into g.blocks partials `part` for the Krylov kernels
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
-                     const float *xc, float *y, const SolveState *state, const Grid &g, double *part);
+                     const float *xc, float *y, const SolveState *state, const Grid &g, double *cell_part,
+                     double *part);
 // coarsest level: x[idx[i]] = sum_j inv[i][j] b[idx[j]] (n <= kMgCoarseMax), other cells 0
 void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x,
                       int cells, const SolveState *state);

@@ -172,8 +172,6 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restri
     y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
 }
 
-constexpr int NP = 1024;  // threads of the level-0 post-smoothing (two bricks in flight per block)
-
 // ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
 // unknowns and the diagonal is 1 on them, so x0 = w r needs no division and is never
 // stored: the pre-smoothing residual and the post-smoothing recompute it from r.
@@ -203,20 +201,19 @@ __device__ inline float xprime(const float *__restrict__ r, const float *__restr
 }
 
 // y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
-// prolongation fused into the post-smoothing), on the Krylov grid with r.y partials
-__global__ __launch_bounds__(NP) void k_mg_post0(MgOp op, const int4 *__restrict__ coord,
+// prolongation fused into the post-smoothing), one thread per cell; part[block] =
+// sum of r*y over the block's 256 cells
+__global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict__ coord,
                                                  const int *__restrict__ parent, const float *__restrict__ r,
                                                  const float *__restrict__ xc, float *__restrict__ y,
-                                                 const SolveState *state, int per, double *__restrict__ part) {
+                                                 const SolveState *state, double *__restrict__ part) {
     if (stopped(state)) return;
-    __shared__ double lds[NP / kWave];
-    const int g = xcd_slot((int)blockIdx.x, (int)gridDim.x);
-    const int t0 = g * per, t1 = t0 + per < op.n_active ? t0 + per : op.n_active;
-    const int l = threadIdx.x & (kBrickCells - 1);
-    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    __shared__ double lds[NT / kWave];
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     double acc = 0.0;
-    for (int slot = t0 + (threadIdx.x >> 9); slot < t1; slot += NP / kBrickCells) {
-        const int64_t e = (int64_t)slot * kBrickCells + l;
+    if (e < (int64_t)op.n_active * kBrickCells) {
+        const int slot = (int)(e >> 9), l = (int)(e & 511);
+        const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
         const float kap = brick_kappa(op, slot);
         const float dv = diag_of(op, e, kap);
         float yv = 0.0f;
@@ -226,7 +223,7 @@ __global__ __launch_bounds__(NP) void k_mg_post0(MgOp op, const int4 *__restrict
             const int ps = parent[slot];
             const int *nb = op.nbr + 6 * slot;
             const float xs = xprime(r, xc, slot, l, B, ps);
-            // neighbour (slot', l') of each face, the face coupling, and x0' there
+            // neighbour (slot', l') across each face, the face coupling, and x0' there
             float sum = 0.0f;
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
@@ -254,14 +251,26 @@ __global__ __launch_bounds__(NP) void k_mg_post0(MgOp op, const int4 *__restrict
                 const int nps = ns == slot ? ps : parent[ns];
                 sum += c * xprime(r, xc, ns, nl, NB, nps);
             }
-            const float ax = xs - sum;
-            yv = xs + kMgOmega * (rv - ax);
+            yv = xs + kMgOmega * (rv - (xs - sum));
         }
         y[e] = yv;
-        acc += (double)rv * (double)yv;
+        acc = (double)rv * (double)yv;
     }
-    const double s = block_sum<NP>(acc, lds);
+    const double s = block_sum<NT>(acc, lds);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// out[t] = sum of in[t*per .. (t+1)*per) for t < n_out, in a fixed order (the
+// per-block partials of k_mg_post0 folded to the Krylov grid's partial count)
+__global__ __launch_bounds__(NT) void k_mg_fold(int n_in, const double *__restrict__ in, int n_out,
+                                                double *__restrict__ out, const SolveState *state) {
+    if (stopped(state)) return;
+    const int t = blockIdx.x * NT + threadIdx.x;
+    if (t >= n_out) return;
+    const int per = (n_in + n_out - 1) / n_out;
+    double s = 0.0;
+    for (int q = t * per; q < (t + 1) * per && q < n_in; ++q) s += in[q];
+    out[t] = s;
 }
 
 __global__ __launch_bounds__(kMgCoarseMax) void k_mg_coarse(int n, const int *__restrict__ idx,
@@ -318,8 +327,12 @@ void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, co
 }
 
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
-                     const float *xc, float *y, const SolveState *state, const Grid &g, double *part) {
-    hipLaunchKernelGGL(k_mg_post0, dim3(g.blocks), dim3(NP), 0, st, op, coord, parent, r, xc, y, state, g.per, part);
+                     const float *xc, float *y, const SolveState *state, const Grid &g, double *cell_part,
+                     double *part) {
+    const unsigned nb = blocks((int64_t)op.n_active * kBrickCells);
+    hipLaunchKernelGGL(k_mg_post0, dim3(nb), dim3(NT), 0, st, op, coord, parent, r, xc, y, state, cell_part);
+    hipLaunchKernelGGL(k_mg_fold, dim3((g.blocks + NT - 1) / NT), dim3(NT), 0, st, (int)nb, cell_part, g.blocks, part,
+                       state);
 }
 
 void launch_mg_coarse(hipStream_t st, int n, const int *idx, const float *inv, const float *b, float *x, int cells,

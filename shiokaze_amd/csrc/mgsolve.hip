@@ -27,7 +27,8 @@ struct MgLevelHost {
 struct MgHierarchy {
     std::vector<MgLevelHost> lv;
     DevBuf<float> z;          // the preconditioned residual z = M r (level 0 bricks)
-    DevBuf<double> pz;        // r.z partials
+    DevBuf<double> pz;        // r.z partials (one per Krylov workgroup)
+    DevBuf<double> pcell;     // r.z sums of the post-smoothing blocks (one per 256 cells)
     DevBuf<int> cidx;         // coarsest: active cell indices
     DevBuf<float> cinv;       // coarsest: dense inverse on them
     int ncoarse = 0;
@@ -225,6 +226,7 @@ void mg_destroy(MgHierarchy *h) {
     for (MgLevelHost &L : h->lv) release(L);
     h->z.release();
     h->pz.release();
+    h->pcell.release();
     h->cidx.release();
     h->cinv.release();
     delete h;
@@ -242,6 +244,7 @@ int mg_setup(shk_ctx *c) {
     const int64_t n0 = (int64_t)c->n_active * kBrickCells;
     HIPC(c, h->z.ensure((size_t)std::max<int64_t>(n0, 1)));
     HIPC(c, h->pz.ensure((size_t)kMaxBlocks + 64));
+    HIPC(c, h->pcell.ensure((size_t)std::max<int64_t>((n0 + 255) / 256, 1)));
     launch_mg_mask(st, n0, c->sig64.p, L0.d.p);
     for (size_t l = 0; l < h->lv.size(); ++l) {
         MgLevelHost &L = h->lv[l];
@@ -313,7 +316,7 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
         std::swap(F.v.x, F.v.t);
         std::swap(F.x.p, F.t.p);
     }
-    launch_mg_post0(st, op0, c->coord.p, L0.parent.p, r, h->lv[1].v.x, z, state, *g, part);
+    launch_mg_post0(st, op0, c->coord.p, L0.parent.p, r, h->lv[1].v.x, z, state, *g, h->pcell.p, part);
 }
 
 // Mixed-precision refinement loop with MG-preconditioned inner CG.

@@ -90,3 +90,36 @@ def test_multigrid_reports_nan(scenes):
     ps, res = _solver_after_projection(sc, {"Residual": 1e-6, "Preconditioner": "mg"})
     ps.close()
     assert res["status"] == shiokaze_amd._lib.SHK_NAN
+
+
+def test_multigrid_iteration_cap(scenes):
+    import shiokaze_amd
+    sc = scenes(3, 32, 2)
+    ps, res = _solver_after_projection(sc, {"Residual": 1e-8, "Preconditioner": "mg", "MaxIterations": 3})
+    ps.close()
+    assert res["status"] == shiokaze_amd._lib.SHK_NOT_CONVERGED
+    assert res["count"] == 3
+
+
+def test_multigrid_is_deterministic_and_host_pointers_agree(scenes):
+    import ctypes
+
+    import shiokaze_amd
+    sc = scenes(3, (24, 40, 16), 3)
+    cfg = {"Residual": 1e-6, "Preconditioner": "mg"}
+    a = run_projection(sc, cfg, want_system=False)
+    b = run_projection(sc, cfg, want_system=False)
+    np.testing.assert_array_equal(a["pressure"], b["pressure"])
+    assert a["result"]["count"] == b["result"]["count"]
+    nx, ny, nz = sc["shape"]
+    host = {k: np.ascontiguousarray(sc[k]).copy() for k in ("fluid", "solid", "u", "v", "w", "us", "vs", "ws")}
+    ps = shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=dict(cfg, HostPointers=1))
+    res = shiokaze_amd.ShkResult()
+    p = lambda arr: arr.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    assert ps.lib.shk_project(ps.ctx, 1.0, p(host["u"]), p(host["v"]), p(host["w"]), p(host["solid"]),
+                              p(host["us"]), p(host["vs"]), p(host["ws"]), p(host["fluid"]), ctypes.byref(res)) == 0
+    pres = np.empty((nz, ny, nx), dtype=np.float32)
+    assert ps.lib.shk_copy_pressure(ps.ctx, p(pres)) == 0
+    ps.close()
+    assert res.count == a["result"]["count"]
+    np.testing.assert_array_equal(pres, a["pressure"])
