@@ -106,6 +106,8 @@ void shk_destroy(shk_ctx *ctx);
  *                  (mixed precision, the counterpart of the reference's "amg" LinSolver,
  *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr applies jacobi for
  *                  any of jacobi, unet and mg.
+ *   MgOmega, MgAlpha  multigrid damped-Jacobi weight (0.8) and coarse-correction
+ *                  factor (1.5), each in (0, 2)
  *   OuterMax       refinement steps for mixed (3)
  *   InnerResidual  relative stop of each mixed inner solve (1e-6, floor)
  *   CheckInterval  iterations launched between host convergence polls (32)

@@ -493,6 +493,10 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
         }
         c->ucount = 0;
         c->ums = 0.0;
+    } else if (k == "MgOmega" || k == "MgAlpha") {
+        if (!parse_double(value, d) || !(d > 0.0) || !(d < 2.0))
+            return c->fail(SHK_BAD_ARG, "bad %s '%s' (0 < value < 2)", k.c_str(), value);
+        (k == "MgOmega" ? p.mg_omega : p.mg_alpha) = d;
     } else if (k == "UnetWeights") {
         p.unet_weights = value;
         if (c->unet) {

@@ -25,10 +25,14 @@
 
 namespace shk {
 
+constexpr float kMgOmega = 0.8f;  // default damped-Jacobi weight (MgOmega)
+constexpr float kMgAlpha = 1.5f;  // default coarse-correction factor (MgAlpha)
+
 struct MgOp {
     const int *nbr;
     const float *d, *cx, *cy, *cz;
     int n_active;
+    float omega = kMgOmega, alpha = kMgAlpha;
     // level 0 only: per-brick uniform coupling (unit diagonal, one coupling value; 0 =
     // not uniform), which lets those bricks skip the diagonal and coupling reads
     const float *kappa = nullptr;
@@ -44,11 +48,9 @@ struct MgLevelDev {
     int *parent = nullptr;  // slot of the parent brick on the next level (levels < coarsest)
     float *d = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
     float *b = nullptr, *x = nullptr, *t = nullptr;
-    MgOp op() const { return MgOp{nbr, d, cx, cy, cz, n_active}; }
+    MgOp op(float omega, float alpha) const { return MgOp{nbr, d, cx, cy, cz, n_active, omega, alpha}; }
 };
 
-constexpr float kMgOmega = 0.8f;
-constexpr float kMgAlpha = 1.5f;
 constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)
 
 // level-0 diagonal of the scaled system: 1 on unknowns (sig > 0), else 0
@@ -62,7 +64,7 @@ void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const fl
 void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
                         const SolveState *state);
 // x_f += alpha P0 x_c on the fine unknowns (levels >= 1)
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state);
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float alpha, const SolveState *state);
 // y = x + w (b - A x) / d (levels >= 1)
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
                       const SolveState *state);

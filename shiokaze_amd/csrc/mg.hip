@@ -146,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, 
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
 
-__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, const SolveState *state) {
+__global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, float alpha, const SolveState *state) {
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)f.n_active * kBrickCells) return;
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, c
     const int ps = f.parent[slot];
     if (ps < 0) return;  // cannot happen: the parent of an active brick is active
     const int cl = ((B.x & 1) * 4 + (i >> 1)) + 8 * ((B.y & 1) * 4 + (j >> 1)) + 64 * ((B.z & 1) * 4 + (k >> 1));
-    f.x[e] += kMgAlpha * c.x[(int64_t)ps * kBrickCells + cl];
+    f.x[e] += alpha * c.x[(int64_t)ps * kBrickCells + cl];
 }
 
 // y = x + w (b - A x) / d, one thread per cell
@@ -169,7 +169,7 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restri
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     const float kap = brick_kappa(op, slot);
     const float dv = diag_of(op, e, kap);
-    y[e] = dv > 0.0f ? x[e] + kMgOmega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
+    y[e] = dv > 0.0f ? x[e] + op.omega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
 }
 
 // ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
@@ -191,13 +191,14 @@ __global__ __launch_bounds__(NT) void k_mg_pre0(MgOp op, const float *__restrict
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     const float kap = brick_kappa(op, slot);
     // A (w r) = w A r by linearity
-    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - kMgOmega * apply(op, slot, l, r, kap) : 0.0f;
+    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - op.omega * apply(op, slot, l, r, kap) : 0.0f;
 }
 
 // value of x0' = w r + alpha P0 x1 at level-0 cell (slot, l) of brick B with parent ps
-__device__ inline float xprime(const float *__restrict__ r, const float *__restrict__ xc, int slot, int l, int4 B,
-                               int ps) {
-    return kMgOmega * r[(int64_t)slot * kBrickCells + l] + kMgAlpha * xc[(int64_t)ps * kBrickCells + coarse_local(B, l)];
+__device__ inline float xprime(const MgOp &op, const float *__restrict__ r, const float *__restrict__ xc, int slot,
+                               int l, int4 B, int ps) {
+    return op.omega * r[(int64_t)slot * kBrickCells + l] +
+           op.alpha * xc[(int64_t)ps * kBrickCells + coarse_local(B, l)];
 }
 
 // y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
@@ -222,7 +223,7 @@ __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict
             const int4 B = coord[slot];
             const int ps = parent[slot];
             const int *nb = op.nbr + 6 * slot;
-            const float xs = xprime(r, xc, slot, l, B, ps);
+            const float xs = xprime(op, r, xc, slot, l, B, ps);
             // neighbour (slot', l') across each face, the face coupling, and x0' there
             float sum = 0.0f;
 #pragma unroll
@@ -249,9 +250,9 @@ __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict
                 if (c == 0.0f) continue;
                 const int4 NB = ns == slot ? B : coord[ns];
                 const int nps = ns == slot ? ps : parent[ns];
-                sum += c * xprime(r, xc, ns, nl, NB, nps);
+                sum += c * xprime(op, r, xc, ns, nl, NB, nps);
             }
-            yv = xs + kMgOmega * (rv - (xs - sum));
+            yv = xs + op.omega * (rv - (xs - sum));
         }
         y[e] = yv;
         acc = (double)rv * (double)yv;
@@ -312,8 +313,9 @@ void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c
                        omega, state);
 }
 
-void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, state);
+void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float alpha, const SolveState *state) {
+    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, alpha,
+                       state);
 }
 
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,

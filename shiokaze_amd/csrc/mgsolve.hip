@@ -295,13 +295,14 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
     hipStream_t st = c->stream;
     const int L = (int)h->lv.size() - 1;
     MgLevelHost &L0 = h->lv[0];
-    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
+    const float w = (float)c->prm.mg_omega, a = (float)c->prm.mg_alpha;
+    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, w, a, c->kappa32.p};
     launch_mg_pre0(st, op0, r, L0.v.t, state);
     for (int l = 1; l < L; ++l) {
         MgLevelHost &F = h->lv[l - 1], &C = h->lv[l];
         if (!C.v.n_active) continue;
-        launch_mg_restrict(st, F.v, C.v, F.v.t, kMgOmega, state);
-        launch_mg_residual(st, C.v.op(), C.v.b, C.v.x, C.v.t, state);
+        launch_mg_restrict(st, F.v, C.v, F.v.t, w, state);
+        launch_mg_residual(st, C.v.op(w, a), C.v.b, C.v.x, C.v.t, state);
     }
     MgLevelHost &Cl = h->lv[L];
     if (Cl.v.n_active) {
@@ -311,8 +312,8 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
     for (int l = L - 1; l >= 1; --l) {
         MgLevelHost &F = h->lv[l], &C = h->lv[l + 1];
         if (!F.v.n_active) continue;
-        launch_mg_prolong(st, F.v, C.v, state);
-        launch_mg_smooth(st, F.v.op(), F.v.b, F.v.x, F.v.t, state);
+        launch_mg_prolong(st, F.v, C.v, a, state);
+        launch_mg_smooth(st, F.v.op(w, a), F.v.b, F.v.x, F.v.t, state);
         std::swap(F.v.x, F.v.t);
         std::swap(F.x.p, F.t.p);
     }
