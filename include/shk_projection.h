@@ -106,7 +106,7 @@ void shk_destroy(shk_ctx *ctx);
  *                  (mixed precision, the counterpart of the reference's "amg" LinSolver,
  *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr applies jacobi for
  *                  any of jacobi, unet and mg.
- *   MgOmega, MgAlpha  multigrid damped-Jacobi weight (0.8) and coarse-correction
+ *   MgOmega, MgAlpha  multigrid damped-Jacobi weight (0.9) and coarse-correction
  *                  factor (1.5), each in (0, 2)
  *   OuterMax       refinement steps for mixed (3)
  *   InnerResidual  relative stop of each mixed inner solve (1e-6, floor)
@@ -181,7 +181,7 @@ int shk_kernel_stats(shk_ctx *ctx, const char *kernel, int64_t *launches, double
 /* z = M r with the multigrid V-cycle of the last projection (Preconditioner=mg): dense
  * FP32 device grids (nx*ny*nz) in the Jacobi-scaled unknowns (r^ = D^-1/2 r,
  * z^ = D^1/2 z).  The hierarchy: 2x2x2 aggregates with Galerkin coarse operators,
- * one damped-Jacobi pre- and post-smoothing step (w = 0.8), coarse-correction factor
+ * one damped-Jacobi pre- and post-smoothing step (w = 0.9), coarse-correction factor
  * 1.5, dense solve on the coarsest level (<= 4 cells per axis).  A test and
  * inspection entry point; shk_project applies the same V-cycle inside CG. */
 int shk_mg_apply(shk_ctx *ctx, const float *r, float *z);

@@ -18,7 +18,7 @@ from __future__ import annotations
 
 import numpy as np
 
-OMEGA = 0.8   # damped-Jacobi weight
+OMEGA = 0.9   # damped-Jacobi weight
 ALPHA = 1.5   # coarse-correction factor
 COARSE_AXIS = 4  # coarsening stops when every axis has at most this many cells
 

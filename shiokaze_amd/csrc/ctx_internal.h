@@ -62,7 +62,7 @@ struct Params {
     int check_interval = 32;
     bool host_pointers = false;
     std::string unet_weights;
-    double mg_omega = 0.8;            // multigrid damped-Jacobi weight
+    double mg_omega = 0.9;            // multigrid damped-Jacobi weight
     double mg_alpha = 1.5;            // multigrid coarse-correction factor
 };
 

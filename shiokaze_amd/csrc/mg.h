@@ -13,7 +13,7 @@
 // V-cycle (symmetric, so M is a fixed SPD operator and plain PCG applies):
 //   x_l = w D_l^-1 b_l;  t_l = b_l - A_l x_l;  b_{l+1} = P0' t_l;  (recurse)
 //   x_l += alpha P0 x_{l+1};  x_l += w D_l^-1 (b_l - A_l x_l)
-// with w = 0.8 and alpha = 1.5 (the over-correction that unsmoothed aggregation needs).
+// with w = 0.9 and alpha = 1.5 (the over-correction that unsmoothed aggregation needs).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,7 +25,7 @@
 
 namespace shk {
 
-constexpr float kMgOmega = 0.8f;  // default damped-Jacobi weight (MgOmega)
+constexpr float kMgOmega = 0.9f;  // default damped-Jacobi weight (MgOmega)
 constexpr float kMgAlpha = 1.5f;  // default coarse-correction factor (MgAlpha)
 
 struct MgOp {
