@@ -184,6 +184,7 @@ def main():
     # ---- accuracy against an FP64 truth solve (Jacobi PCG to 1e-12 on the GPU; the
     # FP64 GPU path is checked against the CPU oracle by the parity tests)
     accuracy = {}
+    truth_x = None  # (unknown mask, FP64 truth pressure) for the extra solvers' accuracy
     if not args.no_accuracy:
         sysb = ps.export_system()
         lab = sysb["label"].bool()
@@ -200,6 +201,7 @@ def main():
             "true_relres_l2": last["true_relres_l2"],
             "truth_iters": rt["count"],
         }
+        truth_x = (lab, xt)
         truth.close()
         log(f"accuracy: {accuracy}")
 
@@ -214,6 +216,17 @@ def main():
             t1 = time.perf_counter()
             rp = step(pcg)
             extra["fp64_plain_cg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rp["count"]}
+            # its kernels against the HBM roofline (one more solve with dispatch-recorded events)
+            pcg.configure("ProfileKernels", 1)
+            step(pcg)
+            cells = rp["active_bricks"] * 512
+            for k in ("spmv_dot", "axpy_norm"):
+                cnt, ms = pcg.kernel_stats(k)
+                if cnt:
+                    bpl = kernel_bytes_per_cell(k, "fp64", "none") * cells
+                    gbs = bpl / (ms / cnt * 1e-3) / 1e9
+                    extra["fp64_plain_cg"][k] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
+                                                 "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
         pcg.close()
         if args.extra:
             pj = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "jacobi"})
@@ -224,17 +237,23 @@ def main():
             extra["mixed_jacobi_vs_fp64_plain_cg"] = round(value / extra["fp64_plain_cg"]["solves_per_s"], 2)
             pj.close()
         if args.extra:
-            # aggregation-multigrid PCG (mixed precision)
-            pm = make({"Residual": args.residual, "Preconditioner": "mg"})
-            step(pm)
-            t1 = time.perf_counter()
-            rm = step(pm)
-            extra["mg_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rm["count"],
-                               "outer": rm["outer"], "true_relres_inf": rm["true_relres_inf"]}
-            if "fp64_plain_cg" in extra:
-                extra["mg_pcg"]["vs_fp64_plain_cg"] = round(extra["mg_pcg"]["solves_per_s"]
-                                                            / extra["fp64_plain_cg"]["solves_per_s"], 2)
-            pm.close()
+            # aggregation-multigrid PCG (mixed precision): at the bench tolerance, and at a
+            # 10x tighter max-norm stop (the setting for the pressure-accuracy bar)
+            for name, tol in (("mg_pcg", args.residual), ("mg_pcg_residual_x0.1", 0.1 * args.residual)):
+                pm = make({"Residual": tol, "Preconditioner": "mg"})
+                step(pm)
+                t1 = time.perf_counter()
+                rm = step(pm)
+                e = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rm["count"],
+                     "outer": rm["outer"], "true_relres_inf": rm["true_relres_inf"]}
+                if "fp64_plain_cg" in extra:
+                    e["vs_fp64_plain_cg"] = round(e["solves_per_s"] / extra["fp64_plain_cg"]["solves_per_s"], 2)
+                if truth_x is not None:
+                    xm = pm.export_system()["x"][truth_x[0]]
+                    e["pressure_rel_l2_vs_fp64_1e-12"] = float(torch.linalg.norm(xm - truth_x[1])
+                                                               / torch.linalg.norm(truth_x[1]))
+                extra[name] = e
+                pm.close()
         if args.extra and Path(args.unet_weights).exists():
             # learned preconditioner: flexible CG with the trained network (plain PCG fallback)
             from shiokaze_amd.learned import flops_per_apply
@@ -257,6 +276,10 @@ def main():
                                  "frac": round(tf / 2500.0, 4), "flops_per_apply": fl}
             if "fp64_plain_cg" in extra:
                 entry["vs_fp64_plain_cg"] = round(entry["solves_per_s"] / extra["fp64_plain_cg"]["solves_per_s"], 2)
+            if truth_x is not None:
+                xu = pu.export_system()["x"][truth_x[0]]
+                entry["pressure_rel_l2_vs_fp64_1e-12"] = float(torch.linalg.norm(xu - truth_x[1])
+                                                               / torch.linalg.norm(truth_x[1]))
             extra["learned_fcg"] = entry
             pu.close()
         log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")

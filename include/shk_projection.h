@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SHK_ABI_VERSION 1
+#define SHK_ABI_VERSION 2
 
 typedef struct shk_ctx shk_ctx;
 
@@ -79,6 +79,8 @@ typedef struct shk_result {
     double ms_setup;        /* fractions, labels, bricks, coefficients, rhs */
     double ms_solve;        /* Krylov loop incl. refinement */
     double ms_update;       /* pressure scatter + velocity update */
+    double volume_correct_rhs; /* the constant added to every rhs row by the volume
+                                  correction (macpressuresolver3.cpp:231-235), 0 if off */
 } shk_result;
 
 /* ------------------------------------------------------------------ context */
@@ -95,8 +97,10 @@ void shk_destroy(shk_ctx *ctx);
  *   SecondOrderAccurateFluid (1), SecondOrderAccurateSolid (1)
  *                            macpressuresolver3.cpp:302-303
  *   EpsFluid (1e-2), EpsSolid (1e-2)   macutility3.cpp:502-503
- *   Gain, WarmStart          macpressuresolver3.cpp:304-305  accepted; only the
- *                            defaults (volume correction and warm start off) are built
+ *   Gain (1)                 macpressuresolver3.cpp:304, 350  volume-correction rate;
+ *                            active once shk_set_target_volume gives a nonzero target
+ *   WarmStart (0)            macpressuresolver3.cpp:305, 238-260  solve for the change
+ *                            from the previous projection's pressure
  * Keys of this library:
  *   Precision      fp64 | mixed            (fp64)
  *   Preconditioner none | jacobi | unet | mg   (jacobi)
@@ -125,13 +129,30 @@ const char *shk_last_error(const shk_ctx *ctx);
 /* ------------------------------------------------------------------ projection */
 
 /* macproject3_interface::project (macproject3_interface.h:79-87, implemented at
- * macpressuresolver3.cpp:54-299) with surface_tension = 0 and no rigid bodies.
- * u, v, w are updated in place.  solid is the nodal solid level set, fluid the
- * cell-centred liquid level set, us/vs/ws the solid face velocities.  When the
- * scene has no unknowns the velocity is left untouched (macpressuresolver3.cpp:170). */
+ * macpressuresolver3.cpp:54-299).  u, v, w are updated in place.  solid is the
+ * nodal solid level set, fluid the cell-centred liquid level set, us/vs/ws the
+ * solid face velocities.  surface_tension != 0 adds the ghost-fluid surface-tension
+ * jump to the interface faces first (macpressuresolver3.cpp:95-124), also when the
+ * scene has no unknowns; otherwise a scene without unknowns leaves the velocity
+ * untouched (macpressuresolver3.cpp:170).  The reference's rigidbodies argument is
+ * not used by its pressure solver and has no counterpart here. */
 int shk_project(shk_ctx *ctx, double dt, float *u, float *v, float *w,
                 const float *solid, const float *us, const float *vs, const float *ws,
-                const float *fluid, shk_result *result);
+                const float *fluid, double surface_tension, shk_result *result);
+
+/* macproject3_interface::set_target_volume (macproject3_interface.h:51,
+ * macpressuresolver3.cpp:46-49): the liquid volumes the next projections' PI
+ * volume correction works from (with Gain != 0 and target != 0). */
+int shk_set_target_volume(shk_ctx *ctx, double current_volume, double target_volume);
+
+/* macproject3_interface::set_solid_density (macproject3_interface.h:58,
+ * macpressuresolver3.cpp:51-53): per-face solid density (three face arrays of the
+ * u, v, w shapes; device pointers, or host pointers with HostPointers=1) that lets
+ * liquid pass solid faces: area' = density*area + (1 - density), and the updated
+ * velocity is scaled by 1 - (1 - area')*density (macpressuresolver3.cpp:75-80,
+ * 287-293).  The context keeps the pointers, as the reference keeps its pointer;
+ * all three NULL switches it off. */
+int shk_set_solid_density(shk_ctx *ctx, const float *du, const float *dv, const float *dw);
 
 /* get_pressure() (macproject3_interface.h:93): nx*ny*nz float device array of the
  * last projection (0 on non-unknowns).  Owned by the context. */

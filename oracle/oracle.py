@@ -45,6 +45,12 @@ _SIG = {
     "shk_or_project": (_i, [_i, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _i64, _i, _i, _d, _d,
                             _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_d), _vp]),
     "shk_or_set_threads": (_i, [_i]),
+    "shk_or_solid_density_blend": (None, [_i, _i, _i, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "shk_or_first_order": (None, [_i, _i, _i, _i, _i] + [_vp] * 6),
+    "shk_or_curvature": (None, [_i, _i, _i, _d, _vp, _i, _vp]),
+    "shk_or_surface_tension": (None, [_i, _i, _i, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
+    "shk_or_volume_correction": (_d, [_d, _d, _d, _d, ctypes.POINTER(_d)]),
+    "shk_or_density_scale": (None, [_i, _i, _i] + [_vp] * 6 + [_i, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -193,10 +199,71 @@ def divergence(dx, areas, vel, svel, index_map, n):
     return float(m), div[:n]
 
 
+def solid_density_blend(areas, density, real_is_double=False):
+    """macpressuresolver3.cpp:75-80 on copies of the three area arrays."""
+    out = [np.array(a, dtype=np.float64, copy=True) for a in areas]
+    dens = [_f64(d) for d in density]
+    nz, ny, nx = out[0].shape[0], out[0].shape[1], out[0].shape[2] - 1
+    load().shk_or_solid_density_blend(nx, ny, nz, *map(_p, dens), int(real_is_double), *map(_p, out))
+    return out
+
+
+def first_order(areas, rhos, second_order_solid=True, second_order_fluid=True):
+    """macpressuresolver3.cpp:83-93 on copies of the fractions."""
+    a = [np.array(x, dtype=np.float64, copy=True) for x in areas]
+    r = [np.array(x, dtype=np.float64, copy=True) for x in rhos]
+    nz, ny, nx = a[0].shape[0], a[0].shape[1], a[0].shape[2] - 1
+    load().shk_or_first_order(nx, ny, nz, int(second_order_solid), int(second_order_fluid), *map(_p, a), *map(_p, r))
+    return a, r
+
+
+def curvature(fluid, dx, real_is_double=False):
+    fluid = _f64(fluid)
+    nz, ny, nx = fluid.shape
+    out = np.empty(fluid.shape)
+    load().shk_or_curvature(nx, ny, nz, dx, _p(fluid), int(real_is_double), _p(out))
+    return out
+
+
+def surface_tension(dt, dx, kappa, fluid, rhos, vel, real_is_double=False):
+    """macpressuresolver3.cpp:95-124: returns the velocity with the jump added."""
+    fluid = _f64(fluid)
+    nz, ny, nx = fluid.shape
+    curv = curvature(fluid, dx, real_is_double)
+    vel = [np.array(v, dtype=np.float64, copy=True) for v in vel]
+    rhos = [_f64(r) for r in rhos]
+    load().shk_or_surface_tension(nx, ny, nz, dt, dx, kappa, _p(fluid), _p(curv), *map(_p, rhos),
+                                  int(real_is_double), *map(_p, vel))
+    return vel
+
+
+def volume_correction(gain, current_volume, target_volume, dt, y_prev):
+    """(rhs constant, new integral state) of macpressuresolver3.cpp:222-230."""
+    y = ctypes.c_double(y_prev)
+    c = load().shk_or_volume_correction(gain, current_volume, target_volume, dt, ctypes.byref(y))
+    return float(c), float(y.value)
+
+
+def density_scale(density, areas, vel, real_is_double=False):
+    vel = [np.array(v, dtype=np.float64, copy=True) for v in vel]
+    dens = [_f64(d) for d in density]
+    areas = [_f64(a) for a in areas]
+    nz, ny, nx = areas[0].shape[0], areas[0].shape[1], areas[0].shape[2] - 1
+    load().shk_or_density_scale(nx, ny, nz, *map(_p, dens), *map(_p, areas), int(real_is_double), *map(_p, vel))
+    return vel
+
+
 def project(scene: dict, dt=1.0, tol=1e-4, max_iterations=30000, mode=SHK_OR_REFERENCE, real_is_double=False,
-            eps_fluid=1e-2, eps_solid=1e-2):
-    """The whole project() on a scene dict (shiokaze_amd.scene layout).  Returns a
-    dict with pressure, new velocity, the system and the solver outputs."""
+            eps_fluid=1e-2, eps_solid=1e-2, surface_tension_coef=0.0, solid_density=None,
+            second_order_solid=True, second_order_fluid=True, rhs_correct=0.0, warm_prev=None):
+    """The whole project() (macpressuresolver3.cpp:54-299) on a scene dict
+    (shiokaze_amd.scene layout).  Returns a dict with pressure, new velocity, the
+    system and the solver outputs.
+
+    Branches: solid_density (three face arrays), second-order switches,
+    surface_tension_coef, rhs_correct (the volume-correction constant, see
+    volume_correction) and warm_prev (the previous pressure per cell, in double, for
+    WarmStart; the result's "prev" is the next one)."""
     fluid = _f64(scene["fluid"])
     solid = _f64(scene["solid"])
     dx = float(scene["dx"])
@@ -204,17 +271,37 @@ def project(scene: dict, dt=1.0, tol=1e-4, max_iterations=30000, mode=SHK_OR_REF
     svel = [_f64(scene[k]) for k in ("us", "vs", "ws")]
     areas = area_fraction(solid, eps_solid, real_is_double)
     rhos = fluid_fraction(fluid, eps_fluid, real_is_double)
+    if solid_density is not None:
+        areas = solid_density_blend(areas, solid_density, real_is_double)
+    if not (second_order_solid and second_order_fluid):
+        areas, rhos = first_order(areas, rhos, second_order_solid, second_order_fluid)
+    if surface_tension_coef:
+        vel = surface_tension(dt, dx, surface_tension_coef, fluid, rhos, vel, real_is_double)
     index_map, n = label(fluid, areas, rhos)
-    out = {"areas": areas, "rhos": rhos, "index_map": index_map, "n": n}
+    out = {"areas": areas, "rhos": rhos, "index_map": index_map, "n": n, "velocity_in": vel}
     if n == 0:
         out.update(pressure=np.zeros(fluid.shape), velocity=vel, iters=0, reresid=0.0, status=0)
         return out
     rowptr, cols, vals, rhs = assemble(dt, dx, fluid, areas, rhos, vel, svel, index_map, n)
+    if rhs_correct:
+        rhs = rhs + rhs_correct   # macpressuresolver3.cpp:231-233
+    w = None
+    if warm_prev is not None:
+        # macpressuresolver3.cpp:238-247: rhs' = rhs - A p_prev, the previous pressure
+        # taken per cell
+        lab = index_map >= 0
+        w = np.zeros(n)
+        w[index_map[lab]] = _f64(warm_prev)[lab]
+        rhs = rhs - spmv(rowptr, cols, vals, w)
     x, iters, reresid, status = pcg(rowptr, cols, vals, rhs, tol, max_iterations, mode)
+    if w is not None:
+        x = x + w   # macpressuresolver3.cpp:256-259
     p = scatter(index_map, x, real_is_double)
     newvel = update_velocity(dt, dx, p, areas, rhos, svel, vel, real_is_double)
+    if solid_density is not None:
+        newvel = density_scale(solid_density, areas, newvel, real_is_double)
     out.update(rowptr=rowptr, cols=cols, vals=vals, rhs=rhs, x=x, iters=iters, reresid=reresid, status=status,
-               pressure=p, velocity=newvel)
+               pressure=p, velocity=newvel, prev=scatter(index_map, x, True))
     return out
 
 

@@ -20,6 +20,18 @@
  *                   all modes agree on the iterates where they must  (test_oracle_solver)
  *   mic0            L L^T == A on the pattern up to the modification (test_oracle_solver)
  *   update/divergence  post-divergence == -(b - A p) identity        (test_oracle_system)
+ *   project() branches (test_oracle_branches):
+ *     curvature        exact on quadratics; 2/r on a sphere's distance field
+ *     surface tension  Laplace's law: a static drop projects to p = 2*sigma/R, u ~ 0
+ *     volume correction  projected divergence equals the rhs constant on every
+ *                      unknown; sign, zero and integral behaviour of the controller
+ *                      (its gains 2.3/(25*0.01) and kp^2/16 are the reference's
+ *                      constants, pinned only through those properties)
+ *     solid density    density 1 leaves areas unchanged and scales the velocity by
+ *                      the area; density 0 opens every face; affine in between
+ *     first order      nonzero fractions -> 1, zeros kept
+ *     warm start       zero previous pressure == cold start bit for bit; any previous
+ *                      pressure gives the cold solution to the solver tolerance
  */
 #include "shk_oracle.h"
 
@@ -164,6 +176,125 @@ void shk_or_fluid_fraction(int nx, int ny, int nz, const double *fluid, double e
                     out[dim][face_index(nx, ny, dim, i, j, k)] = store_real(rho, real_is_double);
                 }
             }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- project() branches */
+
+void shk_or_solid_density_blend(int nx, int ny, int nz, const double *du, const double *dv, const double *dw,
+                                int real_is_double, double *ax, double *ay, double *az) {
+    /* macpressuresolver3.cpp:75-80: area' = density*area + (1 - density).  With
+     * Real = float the product is a float product; the sum is double, stored as float. */
+    const double *dens[3] = {du, dv, dw};
+    double *areas[3] = {ax, ay, az};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int64_t nf = (int64_t)(nx + (dim == 0)) * (ny + (dim == 1)) * (nz + (dim == 2));
+        for (int64_t f = 0; f < nf; ++f) {
+            const double d = dens[dim][f], a = areas[dim][f];
+            const double prod = real_is_double ? d * a : (double)((float)d * (float)a);
+            areas[dim][f] = store_real(prod + (1.0 - d), real_is_double);
+        }
+    }
+}
+
+void shk_or_first_order(int nx, int ny, int nz, int second_order_solid, int second_order_fluid,
+                        double *ax, double *ay, double *az, double *rx, double *ry, double *rz) {
+    /* macpressuresolver3.cpp:83-93: without second-order accuracy every nonzero
+     * fraction becomes 1. */
+    double *areas[3] = {ax, ay, az}, *rhos[3] = {rx, ry, rz};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int64_t nf = (int64_t)(nx + (dim == 0)) * (ny + (dim == 1)) * (nz + (dim == 2));
+        for (int64_t f = 0; f < nf; ++f) {
+            if (!second_order_solid && areas[dim][f] != 0.0) areas[dim][f] = 1.0;
+            if (!second_order_fluid && rhos[dim][f] != 0.0) rhos[dim][f] = 1.0;
+        }
+    }
+}
+
+void shk_or_curvature(int nx, int ny, int nz, double dx, const double *fluid, int real_is_double, double *curv) {
+    /* macpressuresolver3.cpp:100-108: 7-point Laplacian of the liquid level set with
+     * clamped neighbours.  The six neighbours are summed left to right in Real
+     * (float), the centre term and the division are double, stored as Real. */
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t nb[6] = {CELL(clampi(i - 1, 0, nx - 1), j, k), CELL(clampi(i + 1, 0, nx - 1), j, k),
+                                       CELL(i, clampi(j - 1, 0, ny - 1), k), CELL(i, clampi(j + 1, 0, ny - 1), k),
+                                       CELL(i, j, clampi(k - 1, 0, nz - 1)), CELL(i, j, clampi(k + 1, 0, nz - 1))};
+                double sum;
+                if (real_is_double) {
+                    sum = fluid[nb[0]];
+                    for (int q = 1; q < 6; ++q) sum = sum + fluid[nb[q]];
+                } else {
+                    float fs = (float)fluid[nb[0]];
+                    for (int q = 1; q < 6; ++q) fs = fs + (float)fluid[nb[q]];
+                    sum = fs;
+                }
+                const double value = (sum - 6.0 * fluid[CELL(i, j, k)]) / (dx * dx);
+                curv[CELL(i, j, k)] = store_real(value, real_is_double);
+            }
+}
+
+void shk_or_surface_tension(int nx, int ny, int nz, double dt, double dx, double kappa, const double *fluid,
+                            const double *curv, const double *rx, const double *ry, const double *rz,
+                            int real_is_double, double *u, double *v, double *w) {
+    /* macpressuresolver3.cpp:110-122: on faces with 0 < rho < 1, the curvature
+     * interpolated with theta (the liquid side's fraction) gives the jump
+     * -sgn*dt/(dx*rho)*kappa*face_c, added to the face velocity (rounded to Real
+     * first by array3::increment's `const T&` parameter). */
+    const double *rhos[3] = {rx, ry, rz};
+    double *vel[3] = {u, v, w};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int fx = nx + (dim == 0), fy = ny + (dim == 1), fz = nz + (dim == 2);
+        for (int k = 0; k < fz; ++k)
+            for (int j = 0; j < fy; ++j)
+                for (int i = 0; i < fx; ++i) {
+                    const int64_t f = face_index(nx, ny, dim, i, j, k);
+                    const double rho = rhos[dim][f];
+                    if (!(rho != 0.0 && rho < 1.0)) continue;
+                    const int64_t c1 = CELL(clampi(i, 0, nx - 1), clampi(j, 0, ny - 1), clampi(k, 0, nz - 1));
+                    const int64_t c0 = CELL(clampi(i - (dim == 0), 0, nx - 1), clampi(j - (dim == 1), 0, ny - 1),
+                                            clampi(k - (dim == 2), 0, nz - 1));
+                    const double sgn = fluid[c1] < 0.0 ? -1.0 : 1.0;
+                    const double theta = sgn < 0 ? 1.0 - rho : rho;
+                    const double face_c = theta * curv[c1] + (1.0 - theta) * curv[c0];
+                    const double inc = -sgn * dt / (dx * rho) * kappa * face_c;
+                    if (real_is_double)
+                        vel[dim][f] += inc;
+                    else
+                        vel[dim][f] = (double)((float)vel[dim][f] + (float)inc);
+                }
+    }
+}
+
+double shk_or_volume_correction(double gain, double current_volume, double target_volume, double dt,
+                                double *y_prev) {
+    /* macpressuresolver3.cpp:222-230: PI control of the relative volume error x with
+     * integral y (state kept between projections); returns the constant added to
+     * every rhs row, 0 when off. */
+    if (!(gain != 0.0 && target_volume != 0.0)) return 0.0;
+    const double x = (current_volume - target_volume) / target_volume;
+    const double y = *y_prev + x * dt;
+    *y_prev = y;
+    const double kp = gain * 2.3 / (25.0 * 0.01);
+    const double ki = kp * kp / 16.0;
+    return -(kp * x + ki * y) / (x + 1.0);
+}
+
+void shk_or_density_scale(int nx, int ny, int nz, const double *du, const double *dv, const double *dw,
+                          const double *ax, const double *ay, const double *az, int real_is_double,
+                          double *u, double *v, double *w) {
+    /* macpressuresolver3.cpp:287-293: velocity *= 1 - (1 - area)*density on the faces
+     * the update left active (area != 0 and rho != 0; the others are 0 already).
+     * The scale is computed in double and stored in a Real. */
+    const double *dens[3] = {du, dv, dw}, *areas[3] = {ax, ay, az};
+    double *vel[3] = {u, v, w};
+    for (int dim = 0; dim < 3; ++dim) {
+        const int64_t nf = (int64_t)(nx + (dim == 0)) * (ny + (dim == 1)) * (nz + (dim == 2));
+        for (int64_t f = 0; f < nf; ++f) {
+            const double scale = store_real(1.0 - (1.0 - areas[dim][f]) * dens[dim][f], real_is_double);
+            vel[dim][f] = real_is_double ? scale * vel[dim][f] : (double)((float)scale * (float)vel[dim][f]);
         }
     }
 }

@@ -123,6 +123,21 @@ int shk_or_project(int nx, int ny, int nz, double dt, double dx,
 /* OpenMP threads for the fraction / assembly / update loops (the reference
  * threads those loops with NUM_THREAD = cpu count, wscript:75; the solve is
  * serial there and here).  n <= 0 only queries.  Returns the thread count. */
+/* project() branches (macpressuresolver3.cpp:75-124, 220-236, 287-293) */
+void shk_or_solid_density_blend(int nx, int ny, int nz, const double *du, const double *dv, const double *dw,
+                                int real_is_double, double *ax, double *ay, double *az);
+void shk_or_first_order(int nx, int ny, int nz, int second_order_solid, int second_order_fluid,
+                        double *ax, double *ay, double *az, double *rx, double *ry, double *rz);
+void shk_or_curvature(int nx, int ny, int nz, double dx, const double *fluid, int real_is_double, double *curv);
+void shk_or_surface_tension(int nx, int ny, int nz, double dt, double dx, double kappa, const double *fluid,
+                            const double *curv, const double *rx, const double *ry, const double *rz,
+                            int real_is_double, double *u, double *v, double *w);
+double shk_or_volume_correction(double gain, double current_volume, double target_volume, double dt,
+                                double *y_prev);
+void shk_or_density_scale(int nx, int ny, int nz, const double *du, const double *dv, const double *dw,
+                          const double *ax, const double *ay, const double *az, int real_is_double,
+                          double *u, double *v, double *w);
+
 int shk_or_set_threads(int n);
 
 #ifdef __cplusplus

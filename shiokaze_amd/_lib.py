@@ -46,6 +46,7 @@ class ShkResult(ctypes.Structure):
         ("ms_setup", ctypes.c_double),
         ("ms_solve", ctypes.c_double),
         ("ms_update", ctypes.c_double),
+        ("volume_correct_rhs", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -67,7 +68,9 @@ _SIGNATURES = {
                                   ctypes.POINTER(_vp)]),
     "shk_destroy": (None, [_vp]),
     "shk_configure": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
-    "shk_project": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 8 + [ctypes.POINTER(ShkResult)]),
+    "shk_project": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 8 + [ctypes.c_double, ctypes.POINTER(ShkResult)]),
+    "shk_set_target_volume": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
+    "shk_set_solid_density": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "shk_pressure": (_vp, [_vp]),
     "shk_copy_pressure": (ctypes.c_int, [_vp, _vp]),
     "shk_export_fractions": (ctypes.c_int, [_vp] * 7),

@@ -383,6 +383,9 @@ void shk_destroy(shk_ctx *c) {
     for (auto &b : c->area) b.release();
     for (auto &b : c->rho) b.release();
     for (auto &b : c->stage) b.release();
+    c->curvature.release();
+    c->prev64.release();
+    c->warm64.release();
     c->label.release();
     c->pressure.release();
     c->brick_flag.release();
@@ -444,7 +447,7 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
         p.gain = d;
     } else if (k == "WarmStart") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WarmStart '%s'", value);
-        if (b) return c->fail(SHK_BAD_ARG, "WarmStart=1 is not supported (only the default cold start)");
+        c->prm.warm_start = b;
     } else if (k == "EpsFluid") {
         if (!parse_double(value, d) || d < 0) return c->fail(SHK_BAD_ARG, "bad EpsFluid '%s'", value);
         p.eps_fluid = d;
@@ -509,14 +512,36 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
     return SHK_OK;
 }
 
+int shk_set_target_volume(shk_ctx *c, double current_volume, double target_volume) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    if (!std::isfinite(current_volume) || !std::isfinite(target_volume))
+        return c->fail(SHK_BAD_ARG, "volumes must be finite");
+    c->current_volume = current_volume;
+    c->target_volume = target_volume;
+    return SHK_OK;
+}
+
+int shk_set_solid_density(shk_ctx *c, const float *du, const float *dv, const float *dw) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    const int given = (du != nullptr) + (dv != nullptr) + (dw != nullptr);
+    if (given != 0 && given != 3) return c->fail(SHK_BAD_ARG, "solid density needs all three face arrays or none");
+    c->density[0] = du;
+    c->density[1] = dv;
+    c->density[2] = dw;
+    return SHK_OK;
+}
+
 int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float *solid, const float *us,
-                const float *vs, const float *ws, const float *fluid, shk_result *res) {
+                const float *vs, const float *ws, const float *fluid, double surface_tension, shk_result *res) {
     if (!c) return SHK_BAD_ARG;
     c->err.clear();
     const void *ptrs[8] = {u, v, w, solid, us, vs, ws, fluid};
     int rc = validate_fields(c, ptrs, 8);
     if (rc) return rc;
     if (!(dt == dt) || std::isinf(dt)) return c->fail(SHK_BAD_ARG, "dt is not finite");
+    if (!std::isfinite(surface_tension)) return c->fail(SHK_BAD_ARG, "surface_tension is not finite");
     HIPC(c, hipSetDevice(c->device));
     shk_result local{};
     if (!res) res = &local;
@@ -544,12 +569,27 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         dsolid = c->stage[6].p;
         dfluid = c->stage[7].p;
     }
+    const float *dens[3] = {c->density[0], c->density[1], c->density[2]};
+    if (c->prm.host_pointers && dens[0]) {
+        for (int a = 0; a < 3; ++a) {
+            HIPC(c, c->stage[10 + a].ensure(nf[a]));
+            HIPC(c, hipMemcpyAsync(c->stage[10 + a].p, dens[a], sizeof(float) * nf[a], hipMemcpyHostToDevice,
+                                   c->stream));
+            dens[a] = c->stage[10 + a].p;
+        }
+    }
     float *area[3] = {c->area[0].p, c->area[1].p, c->area[2].p};
     float *rho[3] = {c->rho[0].p, c->rho[1].p, c->rho[2].p};
+    res->volume_correct_rhs = 0.0;
 
     HIPC(c, hipEventRecord(c->ev[0], c->stream));
     launch_fractions(c->stream, d, dsolid, dfluid, c->prm.eps_solid, c->prm.eps_fluid, c->prm.so_solid,
-                     c->prm.so_fluid, area, rho);
+                     c->prm.so_fluid, dens, area, rho);
+    if (surface_tension != 0.0) {
+        // macpressuresolver3.cpp:95-124: before the labelling, whatever the unknowns
+        HIPC(c, c->curvature.ensure(G));
+        launch_surface_tension(c->stream, d, dt, c->dx, surface_tension, dfluid, c->curvature.p, rho, vel);
+    }
     HIPC(c, hipMemsetAsync(c->brick_flag.p, 0, (size_t)c->nbx * c->nby * c->nbz, c->stream));
     HIPC(c, hipMemsetAsync(c->count.p, 0, sizeof(unsigned long long), c->stream));
     launch_label(c->stream, d, dfluid, area, rho, c->label.p, c->brick_flag.p, c->nbx, c->nby, c->count.p);
@@ -561,14 +601,33 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     res->unknowns = c->unknowns;
     res->active_bricks = c->n_active;
     if (nunk == 0) {
-        // macpressuresolver3.cpp:170: nothing is solved and the velocity is untouched
+        // macpressuresolver3.cpp:170: nothing is solved; the velocity keeps only the
+        // surface-tension jump, if any
         c->have_system = false;
+        if (c->prm.host_pointers && surface_tension != 0.0)
+            for (int a = 0; a < 3; ++a)
+                HIPC(c, hipMemcpyAsync((a == 0 ? u : (a == 1 ? v : w)), vel[a], sizeof(float) * nf[a],
+                                       hipMemcpyDeviceToHost, c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         return SHK_OK;
     }
+    // volume correction, a PI controller on the relative volume error
+    // (macpressuresolver3.cpp:220-236); its integral state advances only when
+    // there is a system to correct
+    double rhs_correct = 0.0;
+    if (c->prm.gain != 0.0 && c->target_volume != 0.0) {
+        const double x = (c->current_volume - c->target_volume) / c->target_volume;
+        const double y = c->y_prev + x * dt;
+        c->y_prev = y;
+        const double kp = c->prm.gain * 2.3 / (25.0 * 0.01);
+        const double ki = kp * kp / 16.0;
+        rhs_correct = -(kp * x + ki * y) / (x + 1.0);
+        res->volume_correct_rhs = rhs_correct;
+    }
     const bool need32 = c->prm.precision == 1 || c->prm.precond >= 2;
     const bool plain32 = need32 && c->prm.precond == 0;
-    launch_coeff(c->stream, d, dt, c->dx, c->n_active, c->coord.p, area, rho, c->label.p, vel, svel, c->diag64.p,
+    launch_coeff(c->stream, d, dt, c->dx, c->n_active, c->coord.p, area, rho, c->label.p, vel, svel, rhs_correct,
+                 c->diag64.p,
                  c->cx64.p, c->cy64.p, c->cz64.p, c->b64.p, c->sig64.p, plain32 ? c->diag32.p : nullptr,
                  plain32 ? c->cx32.p : nullptr, plain32 ? c->cy32.p : nullptr, plain32 ? c->cz32.p : nullptr);
     if (c->prm.precond != 0) {
@@ -581,6 +640,18 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
                                               c->kappa64.p);
         if (s32) launch_brick_uniform<float>(c->stream, c->n_active, c->nbr.p, c->hx32.p, c->hy32.p, c->hz32.p,
                                              c->kappa32.p);
+    }
+    if (c->prm.warm_start) {
+        // macpressuresolver3.cpp:238-247: solve for the change from the previous
+        // pressure, rhs' = rhs - A p_prev.  The previous pressure is kept per cell
+        // (zero where there was no unknown), not per row number.
+        const bool fresh = c->prev64.p == nullptr;
+        HIPC(c, c->prev64.ensure(G));
+        if (fresh) HIPC(c, hipMemsetAsync(c->prev64.p, 0, sizeof(double) * G, c->stream));
+        HIPC(c, c->warm64.ensure((size_t)c->n_active * kBrickCells));
+        launch_gather_warm(c->stream, d, c->n_active, c->coord.p, c->label.p, c->prev64.p, c->warm64.p);
+        launch_residual(c->stream, c->grid, op64(c), c->b64.p, c->warm64.p, c->r64.p, c->pc.p, c->misc.p);
+        HIPC(c, hipMemcpyAsync(c->b64.p, c->r64.p, sizeof(double) * c->grid.n, hipMemcpyDeviceToDevice, c->stream));
     }
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ev[1], c->stream));
@@ -595,10 +666,13 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         rc = solve_fp64(c, res);
     }
     if (rc) return rc;
+    // macpressuresolver3.cpp:256-259: the solution is the previous pressure plus the change
+    if (c->prm.warm_start) launch_add64(c->stream, c->grid.n, c->x64.p, c->warm64.p);
     c->have_system = true;
     HIPC(c, hipEventRecord(c->ev[2], c->stream));
-    launch_scatter(c->stream, d, c->brick_slot.p, c->nbx, c->nby, c->label.p, c->x64.p, c->pressure.p);
-    launch_update_velocity(c->stream, d, dt, c->dx, c->pressure.p, area, rho, svel, vel);
+    launch_scatter(c->stream, d, c->brick_slot.p, c->nbx, c->nby, c->label.p, c->x64.p, c->pressure.p,
+                   c->prm.warm_start ? c->prev64.p : nullptr);
+    launch_update_velocity(c->stream, d, dt, c->dx, c->pressure.p, area, rho, svel, dens, vel);
     HIPC(c, hipGetLastError());
     HIPC(c, hipEventRecord(c->ev[3], c->stream));
     if (c->prm.host_pointers) {

@@ -64,6 +64,7 @@ struct Params {
     std::string unet_weights;
     double mg_omega = 0.9;            // multigrid damped-Jacobi weight
     double mg_alpha = 1.5;            // multigrid coarse-correction factor
+    bool warm_start = false;          // macpressuresolver3.cpp:354
 };
 
 }  // namespace shk
@@ -90,8 +91,15 @@ struct shk_ctx {
     DevBuf<uint8_t> brick_flag;
     DevBuf<int> brick_slot;
     DevBuf<unsigned long long> count;
-    // host-pointer staging
-    DevBuf<float> stage[10];
+    // host-pointer staging (0..7 the projection's fields, 8..9 spare, 10..12 solid density)
+    DevBuf<float> stage[13];
+    // project() branches: surface-tension curvature, solid density, volume
+    // correction state (macpressuresolver3.cpp:46-53, 365-371), warm start
+    DevBuf<float> curvature;
+    const float *density[3] = {nullptr, nullptr, nullptr};
+    double current_volume = 0.0, target_volume = 0.0, y_prev = 0.0;
+    DevBuf<double> prev64;   // dense FP64 pressure of the last warm-started projection
+    DevBuf<double> warm64;   // that pressure gathered onto the current bricks
     std::vector<float> h_pressure;
     // bricks
     int n_active = 0;

@@ -1,6 +1,7 @@
 // setup.hip -- per-projection kernels around the Krylov loop:
-//   face fractions, unknown labels, brick activity, stencil coefficients + rhs,
-//   pressure scatter, velocity update (gradient subtract), diagnostics.
+//   face fractions (with the solid-density blend), surface tension, unknown labels,
+//   brick activity, stencil coefficients + rhs (with the volume correction), warm
+//   start, pressure scatter, velocity update (gradient subtract), diagnostics.
 //
 // All floating point here rounds exactly as the reference's CPU code
 // (no contraction, IEEE division), so areas, rhos, couplings, rhs, pressure and
@@ -27,7 +28,7 @@ inline unsigned grid_for(int64_t n, int nt = kNT) {
 template <int DIM>
 __global__ void k_fractions(Dims d, const float *__restrict__ solid, const float *__restrict__ fluid,
                             double eps_solid, double eps_fluid, int second_order_solid, int second_order_fluid,
-                            float *__restrict__ area, float *__restrict__ rho) {
+                            const float *__restrict__ density, float *__restrict__ area, float *__restrict__ rho) {
 #pragma clang fp contract(off)
     const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
     const int64_t n = face_count(d, DIM);
@@ -37,8 +38,15 @@ __global__ void k_fractions(Dims d, const float *__restrict__ solid, const float
         const int k = (int)(f / ((int64_t)fx * fy));
         double a = face_area(d, solid, DIM, i, j, k, eps_solid);
         double r = face_rho(d, fluid, DIM, i, j, k, eps_fluid);
-        // first-order switches (macpressuresolver3.cpp:83-93): nonzero -> 1
         float af = (float)a, rf = (float)r;
+        if (density) {
+            // solid density (macpressuresolver3.cpp:75-80): the float product
+            // density*area, plus (1 - density) in double, stored as float
+            const float dd = density[f];
+            const float prod = dd * af;
+            af = (float)((double)prod + (1.0 - (double)dd));
+        }
+        // first-order switches (macpressuresolver3.cpp:83-93): nonzero -> 1
         if (!second_order_solid && af != 0.0f) af = 1.0f;
         if (!second_order_fluid && rf != 0.0f) rf = 1.0f;
         area[f] = af;
@@ -101,10 +109,10 @@ __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8
 __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ brick_coord, Faces fc,
                         const uint8_t *__restrict__ label, const float *__restrict__ u, const float *__restrict__ v,
                         const float *__restrict__ w, const float *__restrict__ us, const float *__restrict__ vs,
-                        const float *__restrict__ ws, double *__restrict__ diag, double *__restrict__ cx,
-                        double *__restrict__ cy, double *__restrict__ cz, double *__restrict__ rhs,
-                        double *__restrict__ sig, float *__restrict__ diag32, float *__restrict__ cx32,
-                        float *__restrict__ cy32, float *__restrict__ cz32) {
+                        const float *__restrict__ ws, double rhs_correct, double *__restrict__ diag,
+                        double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz,
+                        double *__restrict__ rhs, double *__restrict__ sig, float *__restrict__ diag32,
+                        float *__restrict__ cx32, float *__restrict__ cy32, float *__restrict__ cz32) {
 #pragma clang fp contract(off)
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
@@ -153,6 +161,8 @@ __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ b
                 b += -sgn * (1.0 - area) * usf / dx;
             }
         }
+        // volume correction: one constant added to every row (macpressuresolver3.cpp:231-233)
+        if (rhs_correct != 0.0) b += rhs_correct;
         cd = diagonal;
     }
     diag[e] = cd;
@@ -241,25 +251,27 @@ __device__ inline int64_t brick_elem(const Dims &d, const int *brick_slot, int n
 }
 
 __global__ void k_scatter(Dims d, const int *__restrict__ brick_slot, int nbx, int nby, const uint8_t *__restrict__ label,
-                          const double *__restrict__ x, float *__restrict__ pressure) {
+                          const double *__restrict__ x, float *__restrict__ pressure, double *__restrict__ prev) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
         const int i = (int)(c % d.nx);
         const int j = (int)((c / d.nx) % d.ny);
         const int k = (int)(c / ((int64_t)d.nx * d.ny));
-        float p = 0.0f;
+        double p = 0.0;
         if (label[c]) {
             const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
-            if (e >= 0) p = (float)x[e];
+            if (e >= 0) p = x[e];
         }
-        pressure[c] = p;
+        pressure[c] = (float)p;
+        if (prev) prev[c] = p;
     }
 }
 
 template <int DIM>
 __global__ void k_update_velocity(Dims d, double dt, double dx, const float *__restrict__ pressure,
                                   const float *__restrict__ area, const float *__restrict__ rho,
-                                  const float *__restrict__ svel, float *__restrict__ vel) {
+                                  const float *__restrict__ svel, const float *__restrict__ density,
+                                  float *__restrict__ vel) {
 #pragma clang fp contract(off)
     const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
     const int n = DIM == 0 ? d.nx : (DIM == 1 ? d.ny : d.nz);
@@ -270,20 +282,99 @@ __global__ void k_update_velocity(Dims d, double dt, double dx, const float *__r
         const int k = (int)(f / ((int64_t)fx * fy));
         const int pi = DIM == 0 ? i : (DIM == 1 ? j : k);
         const double r = rho[f];
+        float nv = 0.0f;
         if (area[f] != 0.0f && r != 0.0) {
             if (pi == 0 || pi == n) {
-                vel[f] = svel[f];
+                nv = svel[f];
             } else {
                 const float p1 = pressure[cell_index(d, i, j, k)];
                 const float p0 = pressure[cell_index(d, i - (DIM == 0), j - (DIM == 1), k - (DIM == 2))];
                 const float dp = p1 - p0;
                 const float inc = (float)(dt * (double)dp / (r * dx));
-                vel[f] = vel[f] - inc;
+                nv = vel[f] - inc;
             }
-        } else {
-            vel[f] = 0.0f;
+            if (density) {
+                // macpressuresolver3.cpp:287-293: scale = 1 - (1 - area)*density in
+                // double, stored as the float `scale`, times the float velocity
+                const float scale = (float)(1.0 - (1.0 - (double)area[f]) * (double)density[f]);
+                nv = scale * nv;
+            }
         }
+        vel[f] = nv;
     }
+}
+
+// ---------------------------------------------------------------- surface tension (macpressuresolver3.cpp:95-124)
+
+// Curvature as the 7-point Laplacian of the liquid level set with clamped
+// neighbours.  The six neighbour values are summed in float, left to right; the
+// centre term and the division by dx^2 are double; the result is stored as float.
+__global__ void k_curvature(Dims d, double dx, const float *__restrict__ fluid, float *__restrict__ curv) {
+#pragma clang fp contract(off)
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int j = (int)((c / d.nx) % d.ny);
+        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        float s = fluid[cell_index(d, clampi(i - 1, 0, d.nx - 1), j, k)];
+        s = s + fluid[cell_index(d, clampi(i + 1, 0, d.nx - 1), j, k)];
+        s = s + fluid[cell_index(d, i, clampi(j - 1, 0, d.ny - 1), k)];
+        s = s + fluid[cell_index(d, i, clampi(j + 1, 0, d.ny - 1), k)];
+        s = s + fluid[cell_index(d, i, j, clampi(k - 1, 0, d.nz - 1))];
+        s = s + fluid[cell_index(d, i, j, clampi(k + 1, 0, d.nz - 1))];
+        const double v = ((double)s - 6.0 * (double)fluid[c]) / (dx * dx);
+        curv[c] = (float)v;
+    }
+}
+
+// Ghost-fluid surface-tension jump on faces with 0 < rho < 1: the face curvature
+// interpolated with the liquid fraction, added to the face velocity (the increment
+// is rounded to float, then added in float as array3::increment does).
+template <int DIM>
+__global__ void k_surface_tension(Dims d, double dt, double dx, double kappa, const float *__restrict__ fluid,
+                                  const float *__restrict__ curv, const float *__restrict__ rho,
+                                  float *__restrict__ vel) {
+#pragma clang fp contract(off)
+    const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
+    const int64_t nf = face_count(d, DIM);
+    for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
+        const double r = rho[f];
+        if (!(r != 0.0 && r < 1.0)) continue;
+        const int i = (int)(f % fx);
+        const int j = (int)((f / fx) % fy);
+        const int k = (int)(f / ((int64_t)fx * fy));
+        const int64_t c1 = cell_index(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1));
+        const int64_t c0 = cell_index(d, clampi(i - (DIM == 0), 0, d.nx - 1), clampi(j - (DIM == 1), 0, d.ny - 1),
+                                      clampi(k - (DIM == 2), 0, d.nz - 1));
+        const double sgn = fluid[c1] < 0.0f ? -1.0 : 1.0;
+        const double theta = sgn < 0 ? 1.0 - r : r;
+        const double face_c = theta * (double)curv[c1] + (1.0 - theta) * (double)curv[c0];
+        const double inc = -sgn * dt / (dx * r) * kappa * face_c;
+        vel[f] = vel[f] + (float)inc;
+    }
+}
+
+// ---------------------------------------------------------------- warm start (macpressuresolver3.cpp:238-260)
+
+// The previous projection's FP64 pressure (dense) onto the current bricks, zero off
+// the unknowns.
+__global__ void k_gather_warm(Dims d, const int4 *__restrict__ brick_coord, const uint8_t *__restrict__ label,
+                              const double *__restrict__ prev, double *__restrict__ w) {
+    const int slot = blockIdx.x;
+    const int l = threadIdx.x;
+    const int4 bc = brick_coord[slot];
+    const int i = bc.x * kBrick + (l & 7), j = bc.y * kBrick + ((l >> 3) & 7), k = bc.z * kBrick + (l >> 6);
+    double v = 0.0;
+    if (i < d.nx && j < d.ny && k < d.nz) {
+        const int64_t c = cell_index(d, i, j, k);
+        if (label[c]) v = prev[c];
+    }
+    w[(int64_t)slot * kBrickCells + l] = v;
+}
+
+__global__ void k_add64(int64_t n, double *__restrict__ x, const double *__restrict__ w) {
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        x[e] = x[e] + w[e];
 }
 
 // ---------------------------------------------------------------- diagnostics
@@ -371,13 +462,36 @@ __global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, in
 // ---------------------------------------------------------------- launchers
 
 void launch_fractions(hipStream_t st, Dims d, const float *solid, const float *fluid, double eps_solid,
-                      double eps_fluid, int so_solid, int so_fluid, float *const area[3], float *const rho[3]) {
+                      double eps_fluid, int so_solid, int so_fluid, const float *const density[3],
+                      float *const area[3], float *const rho[3]) {
     hipLaunchKernelGGL(k_fractions<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, area[0], rho[0]);
+                       eps_fluid, so_solid, so_fluid, density[0], area[0], rho[0]);
     hipLaunchKernelGGL(k_fractions<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, area[1], rho[1]);
+                       eps_fluid, so_solid, so_fluid, density[1], area[1], rho[1]);
     hipLaunchKernelGGL(k_fractions<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, area[2], rho[2]);
+                       eps_fluid, so_solid, so_fluid, density[2], area[2], rho[2]);
+}
+
+void launch_surface_tension(hipStream_t st, Dims d, double dt, double dx, double kappa, const float *fluid,
+                            float *curv, float *const rho[3], float *const vel[3]) {
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_curvature, dim3(grid_for(G)), dim3(kNT), 0, st, d, dx, fluid, curv);
+    hipLaunchKernelGGL(k_surface_tension<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, dt, dx, kappa,
+                       fluid, curv, rho[0], vel[0]);
+    hipLaunchKernelGGL(k_surface_tension<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, kappa,
+                       fluid, curv, rho[1], vel[1]);
+    hipLaunchKernelGGL(k_surface_tension<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, dt, dx, kappa,
+                       fluid, curv, rho[2], vel[2]);
+}
+
+void launch_gather_warm(hipStream_t st, Dims d, int n_active, const int4 *brick_coord, const uint8_t *label,
+                        const double *prev, double *w) {
+    if (n_active <= 0) return;
+    hipLaunchKernelGGL(k_gather_warm, dim3(n_active), dim3(kBrickCells), 0, st, d, brick_coord, label, prev, w);
+}
+
+void launch_add64(hipStream_t st, int64_t n, double *x, const double *w) {
+    hipLaunchKernelGGL(k_add64, dim3(grid_for(n)), dim3(kNT), 0, st, n, x, w);
 }
 
 void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[3], float *const rho[3], uint8_t *label,
@@ -391,13 +505,13 @@ void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[
 
 void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
                   float *const area[3], float *const rho[3], const uint8_t *label, const float *const vel[3],
-                  const float *const svel[3], double *diag, double *cx, double *cy, double *cz, double *rhs,
-                  double *sig, float *diag32, float *cx32, float *cy32, float *cz32) {
+                  const float *const svel[3], double rhs_correct, double *diag, double *cx, double *cy, double *cz,
+                  double *rhs, double *sig, float *diag32, float *cx32, float *cy32, float *cz32) {
     Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
     if (n_active <= 0) return;
     hipLaunchKernelGGL(k_coeff, dim3(n_active), dim3(kBrickCells), 0, st, d, dt, dx, brick_coord, fc, label, vel[0],
-                       vel[1], vel[2], svel[0], svel[1], svel[2], diag, cx, cy, cz, rhs, sig, diag32, cx32, cy32,
-                       cz32);
+                       vel[1], vel[2], svel[0], svel[1], svel[2], rhs_correct, diag, cx, cy, cz, rhs, sig, diag32,
+                       cx32, cy32, cz32);
 }
 
 void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const double *sig, const double *cx,
@@ -420,19 +534,21 @@ template void launch_brick_uniform<float>(hipStream_t, int, const int *, const f
                                           float *);
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
-                    const double *x, float *pressure) {
+                    const double *x, float *pressure, double *prev) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, x, pressure);
+    hipLaunchKernelGGL(k_scatter, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, x, pressure,
+                       prev);
 }
 
 void launch_update_velocity(hipStream_t st, Dims d, double dt, double dx, const float *pressure, float *const area[3],
-                            float *const rho[3], const float *const svel[3], float *const vel[3]) {
+                            float *const rho[3], const float *const svel[3], const float *const density[3],
+                            float *const vel[3]) {
     hipLaunchKernelGGL(k_update_velocity<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, dt, dx, pressure,
-                       area[0], rho[0], svel[0], vel[0]);
+                       area[0], rho[0], svel[0], density[0], vel[0]);
     hipLaunchKernelGGL(k_update_velocity<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, pressure,
-                       area[1], rho[1], svel[1], vel[1]);
+                       area[1], rho[1], svel[1], density[1], vel[1]);
     hipLaunchKernelGGL(k_update_velocity<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, dt, dx, pressure,
-                       area[2], rho[2], svel[2], vel[2]);
+                       area[2], rho[2], svel[2], density[2], vel[2]);
 }
 
 int launch_divergence(hipStream_t st, Dims d, double dt, double dx, float *const area[3], float *const rho[3],

@@ -86,7 +86,25 @@ class MacPressureSolver3:
         return (nx + 1) * (ny + 1) * (nz + 1)
 
     # --- project() -----------------------------------------------------------------------
-    def project(self, dt: float, velocity, solid, solid_velocity, fluid) -> dict:
+    def set_target_volume(self, current_volume: float, target_volume: float) -> None:
+        """macproject3_interface::set_target_volume: volumes for the PI volume correction
+        of the following projections (active with Gain != 0 and a nonzero target)."""
+        _lib.check(self.lib.shk_set_target_volume(self.ctx, float(current_volume), float(target_volume)), self.ctx)
+
+    def set_solid_density(self, density) -> None:
+        """macproject3_interface::set_solid_density: (du, dv, dw) face tensors of the
+        u, v, w shapes, or None to switch it off.  The tensors must stay alive while set."""
+        if density is None:
+            _lib.check(self.lib.shk_set_solid_density(self.ctx, None, None, None), self.ctx)
+            self._density = None
+            return
+        du, dv, dw = density
+        ptrs = [_ptr(du, self.face_count(0), "du"), _ptr(dv, self.face_count(1), "dv"),
+                _ptr(dw, self.face_count(2), "dw")]
+        _lib.check(self.lib.shk_set_solid_density(self.ctx, *ptrs), self.ctx)
+        self._density = (du, dv, dw)  # keep the storage alive while the library points at it
+
+    def project(self, dt: float, velocity, solid, solid_velocity, fluid, surface_tension: float = 0.0) -> dict:
         """Projects `velocity` (u, v, w) in place; returns the solver result as a dict."""
         u, v, w = velocity
         us, vs, ws = solid_velocity
@@ -98,7 +116,7 @@ class MacPressureSolver3:
         ]
         torch.cuda.synchronize(self.device)
         res = _lib.ShkResult()
-        rc = self.lib.shk_project(self.ctx, float(dt), *ptrs, ctypes.byref(res))
+        rc = self.lib.shk_project(self.ctx, float(dt), *ptrs, float(surface_tension), ctypes.byref(res))
         _lib.check(rc, self.ctx, _lib.SOLVER_OUTCOMES)
         out = res.as_dict()
         out["status_name"] = _lib.STATUS_NAMES.get(res.status, str(res.status))

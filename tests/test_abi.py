@@ -14,7 +14,7 @@ from conftest import gpu_available
 def test_library_exports_every_header_symbol(shk_lib):
     from shiokaze_amd import _lib
     declared = set(_lib.header_symbols())
-    assert len(declared) >= 28
+    assert len(declared) >= 30
     out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
     exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
@@ -42,7 +42,7 @@ def test_argument_checks(shk_lib):
     assert shk_lib.shk_create(8, 8, 8, -1.0, 0, ctypes.byref(ctx)) == _lib.SHK_BAD_ARG
     assert shk_lib.shk_create(8, 8, 8, 0.1, 0, None) == _lib.SHK_BAD_ARG
     assert shk_lib.shk_configure(None, b"Residual", b"1e-6") == _lib.SHK_BAD_ARG
-    assert shk_lib.shk_project(None, 1.0, *([None] * 8), None) == _lib.SHK_BAD_ARG
+    assert shk_lib.shk_project(None, 1.0, *([None] * 8), 0.0, None) == _lib.SHK_BAD_ARG
     assert shk_lib.shk_unet_create(0, 0, None) == _lib.SHK_BAD_ARG
     assert shk_lib.shk_unet_num_params(None, None) == _lib.SHK_BAD_ARG
     assert shk_lib.shk_export_scaled(None, *([None] * 6)) == _lib.SHK_BAD_ARG

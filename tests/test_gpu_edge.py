@@ -59,7 +59,7 @@ def test_host_pointers_match_device_pointers(scenes):
     res = shiokaze_amd.ShkResult()
     p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     rc = lib.shk_project(ps.ctx, 1.0, p(host["u"]), p(host["v"]), p(host["w"]), p(host["solid"]), p(host["us"]),
-                         p(host["vs"]), p(host["ws"]), p(host["fluid"]), ctypes.byref(res))
+                         p(host["vs"]), p(host["ws"]), p(host["fluid"]), 0.0, ctypes.byref(res))
     assert rc == 0
     pres = np.empty((nz, ny, nx), dtype=np.float32)
     assert lib.shk_copy_pressure(ps.ctx, p(pres)) == 0
@@ -73,8 +73,8 @@ def test_host_pointers_match_device_pointers(scenes):
 def test_configure_rejects_bad_keys():
     import shiokaze_amd
     ps = shiokaze_amd.MacPressureSolver3((8, 8, 8), 1 / 8)
-    for key, val in [("NoSuchKey", "1"), ("Residual", "abc"), ("WarmStart", "1"), ("Precision", "fp16"),
-                     ("Preconditioner", "ilu")]:
+    for key, val in [("NoSuchKey", "1"), ("Residual", "abc"), ("WarmStart", "maybe"), ("Gain", "x"),
+                     ("Precision", "fp16"), ("Preconditioner", "ilu")]:
         with pytest.raises(shiokaze_amd.ShkError):
             ps.configure(key, val)
     ps.configure("PCG.Residual", "1e-8")     # scoped names as the reference's cmdparser accepts

@@ -117,7 +117,7 @@ def test_multigrid_is_deterministic_and_host_pointers_agree(scenes):
     res = shiokaze_amd.ShkResult()
     p = lambda arr: arr.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
     assert ps.lib.shk_project(ps.ctx, 1.0, p(host["u"]), p(host["v"]), p(host["w"]), p(host["solid"]),
-                              p(host["us"]), p(host["vs"]), p(host["ws"]), p(host["fluid"]), ctypes.byref(res)) == 0
+                              p(host["us"]), p(host["vs"]), p(host["ws"]), p(host["fluid"]), 0.0, ctypes.byref(res)) == 0
     pres = np.empty((nz, ny, nx), dtype=np.float32)
     assert ps.lib.shk_copy_pressure(ps.ctx, p(pres)) == 0
     ps.close()
