@@ -140,6 +140,28 @@ int shk_project(shk_ctx *ctx, double dt, float *u, float *v, float *w,
                 const float *solid, const float *us, const float *vs, const float *ws,
                 const float *fluid, double surface_tension, shk_result *result);
 
+/* ------------------------------------------------------------------ 2D projection */
+
+/* macproject2_interface::initialize(shape2, dx) for the 2D solver macpressuresolver2
+ * (src/projection/macpressuresolver2.cpp:40-300).  The context holds the nx x ny grid
+ * as nx x 1 x ny (the 2D y axis on the library's z axis) in 64x1x8 bricks.  Every
+ * context call applies, with these 2D conventions:
+ *   cell  nx*ny        index i + nx*j           (pressure, fluid level set)
+ *   u     (nx+1)*ny    index i + (nx+1)*j
+ *   v     nx*(ny+1)    index i + nx*j
+ *   solid (nx+1)*(ny+1) nodal, index i + (nx+1)*j
+ * shk_set_solid_density, shk_export_fractions and shk_divergence take the x and y
+ * face arrays in their first two slots and NULL in the third.  Preconditioner none
+ * and jacobi (either precision) are available in 2D; mg and unet are 3D only. */
+int shk_create2d(int nx, int ny, double dx, int device, shk_ctx **out);
+
+/* macproject2_interface::project (macpressuresolver2.cpp:52-281): the same steps as
+ * shk_project with 4-neighbour cells, the 2D area fraction 1 - fraction of the face's
+ * two solid nodes (macutility2.cpp:104-111) and the 5-point curvature.  u, v are
+ * updated in place. */
+int shk_project2d(shk_ctx *ctx, double dt, float *u, float *v, const float *solid, const float *us,
+                  const float *vs, const float *fluid, double surface_tension, shk_result *result);
+
 /* macproject3_interface::set_target_volume (macproject3_interface.h:51,
  * macpressuresolver3.cpp:46-49): the liquid volumes the next projections' PI
  * volume correction works from (with Gain != 0 and target != 0). */

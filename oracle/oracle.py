@@ -51,6 +51,8 @@ _SIG = {
     "shk_or_surface_tension": (None, [_i, _i, _i, _d, _d, _d, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp]),
     "shk_or_volume_correction": (_d, [_d, _d, _d, _d, ctypes.POINTER(_d)]),
     "shk_or_density_scale": (None, [_i, _i, _i] + [_vp] * 6 + [_i, _vp, _vp, _vp]),
+    "shk_or_area_fraction2d": (None, [_i, _i, _vp, _d, _i, _vp, _vp]),
+    "shk_or_curvature2d": (None, [_i, _i, _d, _vp, _i, _vp]),
 }
 
 _lib = None
@@ -328,3 +330,96 @@ def to_scipy(rowptr, cols, vals):
     import scipy.sparse as sp
     n = len(rowptr) - 1
     return sp.csr_matrix((vals, cols, rowptr), shape=(n, n))
+
+
+# ---------------------------------------------------------------- 2D (macpressuresolver2)
+
+def area_fraction2d(solid, eps_solid=1e-2, real_is_double=False):
+    """macutility2.cpp:104-111 on a (ny+1, nx+1) nodal solid: (ax (ny, nx+1), ay (ny+1, nx))."""
+    solid = _f64(solid)
+    ny, nx = solid.shape[0] - 1, solid.shape[1] - 1
+    ax = np.empty((ny, nx + 1))
+    ay = np.empty((ny + 1, nx))
+    load().shk_or_area_fraction2d(nx, ny, _p(solid), eps_solid, int(real_is_double), _p(ax), _p(ay))
+    return ax, ay
+
+
+def curvature2d(fluid, dx, real_is_double=False):
+    fluid = _f64(fluid)
+    ny, nx = fluid.shape
+    out = np.empty(fluid.shape)
+    load().shk_or_curvature2d(nx, ny, dx, _p(fluid), int(real_is_double), _p(out))
+    return out
+
+
+def project2d(scene: dict, dt=1.0, tol=1e-4, max_iterations=30000, mode=SHK_OR_REFERENCE, real_is_double=False,
+              eps_fluid=1e-2, eps_solid=1e-2, surface_tension_coef=0.0, solid_density=None,
+              second_order_solid=True, second_order_fluid=True, rhs_correct=0.0, warm_prev=None):
+    """The 2D project() (macpressuresolver2.cpp:52-281) on a 2D scene dict: fluid (ny, nx),
+    solid (ny+1, nx+1) nodal, u/us (ny, nx+1), v/vs (ny+1, nx), dx.
+
+    The 2D rows are the 3D rows of the grid held as nx x 1 x ny whose y faces are walls
+    (area 0, zero velocity): such faces add no coupling, no diagonal and only signed
+    zeros to the rhs, and every other face keeps its formula (the 2D fluid fraction,
+    labelling, assembly, update and surface-tension jump are the 3D ones with the y
+    terms removed).  Only the area fraction and the curvature differ, and they are
+    restated above.  Returns the 3D-layout result of project() plus 2D views."""
+    fluid2 = _f64(scene["fluid"])
+    ny, nx = fluid2.shape
+    dx = float(scene["dx"])
+    fluid = fluid2.reshape(ny, 1, nx)
+    u = _f64(scene["u"]).reshape(ny, 1, nx + 1)
+    w = _f64(scene["v"]).reshape(ny + 1, 1, nx)
+    us = _f64(scene["us"]).reshape(ny, 1, nx + 1)
+    ws = _f64(scene["vs"]).reshape(ny + 1, 1, nx)
+    wall = np.zeros((ny, 2, nx))
+    ax, ay = area_fraction2d(scene["solid"], eps_solid, real_is_double)
+    areas = [ax.reshape(ny, 1, nx + 1), wall.copy(), ay.reshape(ny + 1, 1, nx)]
+    rhos = fluid_fraction(fluid, eps_fluid, real_is_double)
+    rhos[1] = wall.copy()
+    density3 = None
+    if solid_density is not None:
+        du, dv = solid_density
+        density3 = [_f64(du).reshape(ny, 1, nx + 1), wall.copy(), _f64(dv).reshape(ny + 1, 1, nx)]
+        areas = solid_density_blend(areas, density3, real_is_double)
+        areas[1] = wall.copy()   # the wall faces of the embedding are not solid faces of the 2D grid
+    if not (second_order_solid and second_order_fluid):
+        areas, rhos = first_order(areas, rhos, second_order_solid, second_order_fluid)
+    vel = [u, wall.copy(), w]
+    svel = [us, wall.copy(), ws]
+    if surface_tension_coef:
+        curv = curvature2d(fluid2, dx, real_is_double).reshape(ny, 1, nx)
+        vel = [np.array(v, dtype=np.float64, copy=True) for v in vel]
+        load().shk_or_surface_tension(nx, 1, ny, dt, dx, surface_tension_coef, _p(fluid), _p(curv),
+                                      *map(_p, [_f64(r) for r in rhos]), int(real_is_double), *map(_p, vel))
+    index_map, n = label(fluid, areas, rhos)
+    out = {"areas": areas, "rhos": rhos, "index_map": index_map, "n": n, "velocity_in": vel}
+    if n == 0:
+        out.update(pressure=np.zeros(fluid.shape), velocity=vel, iters=0, reresid=0.0, status=0)
+    else:
+        rowptr, cols, vals, rhs = assemble(dt, dx, fluid, areas, rhos, vel, svel, index_map, n)
+        if rhs_correct:
+            rhs = rhs + rhs_correct
+        wv = None
+        if warm_prev is not None:
+            lab = index_map >= 0
+            wv = np.zeros(n)
+            wv[index_map[lab]] = _f64(warm_prev).reshape(fluid.shape)[lab]
+            rhs = rhs - spmv(rowptr, cols, vals, wv)
+        x, iters, reresid, status = pcg(rowptr, cols, vals, rhs, tol, max_iterations, mode)
+        if wv is not None:
+            x = x + wv
+        p = scatter(index_map, x, real_is_double)
+        newvel = update_velocity(dt, dx, p, areas, rhos, svel, vel, real_is_double)
+        if density3 is not None:
+            newvel = density_scale(density3, areas, newvel, real_is_double)
+        out.update(rowptr=rowptr, cols=cols, vals=vals, rhs=rhs, x=x, iters=iters, reresid=reresid, status=status,
+                   pressure=p, velocity=newvel, prev=scatter(index_map, x, True).reshape(ny, nx))
+    # 2D views: (u, v) faces, (ny, nx) cells
+    out["areas2"] = [out["areas"][0].reshape(ny, nx + 1), out["areas"][2].reshape(ny + 1, nx)]
+    out["rhos2"] = [out["rhos"][0].reshape(ny, nx + 1), out["rhos"][2].reshape(ny + 1, nx)]
+    out["velocity2"] = [out["velocity"][0].reshape(ny, nx + 1), out["velocity"][2].reshape(ny + 1, nx)]
+    out["velocity_in2"] = [out["velocity_in"][0].reshape(ny, nx + 1), out["velocity_in"][2].reshape(ny + 1, nx)]
+    out["pressure2"] = out["pressure"].reshape(ny, nx)
+    out["index_map2"] = index_map.reshape(ny, nx)
+    return out

@@ -32,6 +32,13 @@
  *     first order      nonzero fractions -> 1, zeros kept
  *     warm start       zero previous pressure == cold start bit for bit; any previous
  *                      pressure gives the cold solution to the solver tolerance
+ *   2D (test_oracle_2d; the 2D projection runs the functions above on the grid held
+ *   as nx x 1 x ny, whose y faces are walls):
+ *     area_fraction2d  walls, open faces, exact cuts of a straight solid boundary
+ *     curvature2d      exact on quadratics; 1/r on a circle's distance field
+ *     the embedding    its system equals a direct 4-neighbour assembly of the 2D
+ *                      rows; accuracytest2's closed form converges at second order;
+ *                      Laplace's law sigma/R for a static disc
  */
 #include "shk_oracle.h"
 
@@ -297,6 +304,56 @@ void shk_or_density_scale(int nx, int ny, int nz, const double *du, const double
             vel[dim][f] = real_is_double ? scale * vel[dim][f] : (double)((float)scale * (float)vel[dim][f]);
         }
     }
+}
+
+/* ---------------------------------------------------------------- 2D (macpressuresolver2) */
+
+void shk_or_area_fraction2d(int nx, int ny, const double *solid, double eps_solid, int real_is_double,
+                            double *ax, double *ay) {
+    /* macutility2.cpp:104-111: area = 1 - fraction of the face's two end nodes
+     * (x faces: (i,j)-(i,j+1); y faces: (i,j)-(i+1,j)), 0 on the domain walls, eps
+     * clamp.  Nodal index i + (nx+1) j. */
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+            double area;
+            if (i == 0 || i == nx)
+                area = 0.0;
+            else
+                area = 1.0 - shk_or_fraction(solid[i + (int64_t)(nx + 1) * j], solid[i + (int64_t)(nx + 1) * (j + 1)]);
+            if (area != 0.0 && area < eps_solid) area = eps_solid;
+            ax[i + (int64_t)(nx + 1) * j] = store_real(area, real_is_double);
+        }
+    for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double area;
+            if (j == 0 || j == ny)
+                area = 0.0;
+            else
+                area = 1.0 - shk_or_fraction(solid[i + (int64_t)(nx + 1) * j], solid[i + 1 + (int64_t)(nx + 1) * j]);
+            if (area != 0.0 && area < eps_solid) area = eps_solid;
+            ay[i + (int64_t)nx * j] = store_real(area, real_is_double);
+        }
+}
+
+void shk_or_curvature2d(int nx, int ny, double dx, const double *fluid, int real_is_double, double *curv) {
+    /* macpressuresolver2.cpp:93-102: clamped 5-point Laplacian, the four neighbours
+     * summed left to right in Real, centre term and division in double. */
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const int64_t nb[4] = {clampi(i - 1, 0, nx - 1) + (int64_t)nx * j, clampi(i + 1, 0, nx - 1) + (int64_t)nx * j,
+                                   i + (int64_t)nx * clampi(j - 1, 0, ny - 1), i + (int64_t)nx * clampi(j + 1, 0, ny - 1)};
+            double sum;
+            if (real_is_double) {
+                sum = fluid[nb[0]];
+                for (int q = 1; q < 4; ++q) sum = sum + fluid[nb[q]];
+            } else {
+                float fs = (float)fluid[nb[0]];
+                for (int q = 1; q < 4; ++q) fs = fs + (float)fluid[nb[q]];
+                sum = fs;
+            }
+            const double value = (sum - 4.0 * fluid[i + (int64_t)nx * j]) / (dx * dx);
+            curv[i + (int64_t)nx * j] = store_real(value, real_is_double);
+        }
 }
 
 /* ---------------------------------------------------------------- a6 */

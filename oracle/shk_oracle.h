@@ -138,6 +138,11 @@ void shk_or_density_scale(int nx, int ny, int nz, const double *du, const double
                           const double *ax, const double *ay, const double *az, int real_is_double,
                           double *u, double *v, double *w);
 
+/* 2D (macutility2.cpp:104-111, macpressuresolver2.cpp:93-102) */
+void shk_or_area_fraction2d(int nx, int ny, const double *solid, double eps_solid, int real_is_double,
+                            double *ax, double *ay);
+void shk_or_curvature2d(int nx, int ny, double dx, const double *fluid, int real_is_double, double *curv);
+
 int shk_or_set_threads(int n);
 
 #ifdef __cplusplus

@@ -12,9 +12,9 @@ host-side Python mirror of the two reference plugin interfaces it replaces:
   src/linsolver/pcg.cpp:32-82).
 """
 from ._lib import ShkError, ShkResult, load  # noqa: F401
-from .projection import MacPressureSolver3  # noqa: F401
+from .projection import MacPressureSolver2, MacPressureSolver3  # noqa: F401
 from .linsolver import PCGSolver  # noqa: F401
 from .learned import UNet  # noqa: F401
 from . import scene  # noqa: F401
 
-__all__ = ["MacPressureSolver3", "PCGSolver", "ShkError", "ShkResult", "UNet", "load", "scene"]
+__all__ = ["MacPressureSolver2", "MacPressureSolver3", "PCGSolver", "ShkError", "ShkResult", "UNet", "load", "scene"]

@@ -69,6 +69,8 @@ _SIGNATURES = {
     "shk_destroy": (None, [_vp]),
     "shk_configure": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_char_p]),
     "shk_project": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 8 + [ctypes.c_double, ctypes.POINTER(ShkResult)]),
+    "shk_create2d": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "shk_project2d": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 6 + [ctypes.c_double, ctypes.POINTER(ShkResult)]),
     "shk_set_target_volume": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
     "shk_set_solid_density": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "shk_pressure": (_vp, [_vp]),

@@ -176,11 +176,13 @@ int drive_impl(shk_ctx *c, void (*iterate)(void *), void *arg, int interval_over
 }
 
 BrickOp<double> op64(shk_ctx *c) {
-    return BrickOp<double>{c->nbr.p, c->diag64.p, c->cx64.p, c->cy64.p, c->cz64.p, c->n_active};
+    return BrickOp<double>{c->nbr.p, c->diag64.p, c->cx64.p, c->cy64.p, c->cz64.p, c->n_active, nullptr, c->dims.bx};
 }
 BrickOp<float> op32(shk_ctx *c) {
-    if (c->prm.precond == 0) return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active};
-    return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
+    if (c->prm.precond == 0)
+        return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active, nullptr,
+                              c->dims.bx};
+    return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p, c->dims.bx};
 }
 
 int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4], const double *sig, float *rs) {
@@ -205,7 +207,8 @@ int solve_fp64(shk_ctx *c, shk_result *res) {
     const Grid &g = c->grid;
     const double *sig = jac ? c->sig64.p : nullptr;
     const BrickOp<double> op =
-        jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active, c->kappa64.p} : op64(c);
+        jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active, c->kappa64.p, c->dims.bx}
+            : op64(c);
     const Reduction red = reduction(c);
     launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, red, c->prm.residual,
                                    c->prm.max_iterations);
@@ -311,7 +314,11 @@ const char *shk_build_info(void) {
 
 const char *shk_last_error(const shk_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
-int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out) {
+namespace {
+
+// A context over an nx x ny x nz grid with bricks of x extent bx (8: 8x8x8 bricks;
+// 64: 64x1x8 bricks, the layout of a 2D grid held as nx x 1 x ny).
+int create_impl(int nx, int ny, int nz, int bx, double dx, int device, shk_ctx **out) {
     if (!out) {
         g_create_error = "out is NULL";
         return SHK_BAD_ARG;
@@ -335,11 +342,12 @@ int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out) {
     }
     shk_ctx *c = new shk_ctx();
     c->device = device;
-    c->dims = Dims{nx, ny, nz};
+    c->dims = Dims{nx, ny, nz, bx};
     c->dx = dx;
-    c->nbx = (nx + kBrick - 1) / kBrick;
-    c->nby = (ny + kBrick - 1) / kBrick;
-    c->nbz = (nz + kBrick - 1) / kBrick;
+    const int ex = bx, ey = kWave / bx, ez = kBrickCells / kWave;
+    c->nbx = (nx + ex - 1) / ex;
+    c->nby = (ny + ey - 1) / ey;
+    c->nbz = (nz + ez - 1) / ez;
     auto bail = [&](hipError_t e, const char *what) {
         g_create_error = std::string(what) + ": " + hipGetErrorString(e);
         shk_destroy(c);
@@ -370,8 +378,27 @@ int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out) {
     if ((e = hipMemset(c->pressure.p, 0, sizeof(float) * G)) != hipSuccess) return bail(e, "hipMemset");
     if ((e = c->brick_flag.ensure((size_t)c->nbx * c->nby * c->nbz)) != hipSuccess) return bail(e, "hipMalloc");
     if ((e = c->brick_slot.ensure((size_t)c->nbx * c->nby * c->nbz)) != hipSuccess) return bail(e, "hipMalloc");
+    if (bx == 64) {
+        // 2D: the y faces of the nx x 1 x ny layout are walls (area 0) with zero
+        // velocity; their arrays are set once and never written
+        const int64_t nf1 = face_count(c->dims, 1);
+        if ((e = hipMemset(c->area[1].p, 0, sizeof(float) * nf1)) != hipSuccess) return bail(e, "hipMemset");
+        if ((e = hipMemset(c->rho[1].p, 0, sizeof(float) * nf1)) != hipSuccess) return bail(e, "hipMemset");
+        if ((e = c->wall_faces.ensure(nf1)) != hipSuccess) return bail(e, "hipMalloc");
+        if ((e = hipMemset(c->wall_faces.p, 0, sizeof(float) * nf1)) != hipSuccess) return bail(e, "hipMemset");
+    }
     *out = c;
     return SHK_OK;
+}
+
+}  // namespace
+
+int shk_create(int nx, int ny, int nz, double dx, int device, shk_ctx **out) {
+    return create_impl(nx, ny, nz, 8, dx, device, out);
+}
+
+int shk_create2d(int nx, int ny, double dx, int device, shk_ctx **out) {
+    return create_impl(nx, 1, ny, 64, dx, device, out);
 }
 
 void shk_destroy(shk_ctx *c) {
@@ -386,6 +413,7 @@ void shk_destroy(shk_ctx *c) {
     c->curvature.release();
     c->prev64.release();
     c->warm64.release();
+    c->wall_faces.release();
     c->label.release();
     c->pressure.release();
     c->brick_flag.release();
@@ -526,6 +554,15 @@ int shk_set_solid_density(shk_ctx *c, const float *du, const float *dv, const fl
     if (!c) return SHK_BAD_ARG;
     c->err.clear();
     const int given = (du != nullptr) + (dv != nullptr) + (dw != nullptr);
+    if (c->dims.bx == 64) {
+        // 2D: the x and y face arrays, the third NULL
+        if (dw || (given != 0 && given != 2))
+            return c->fail(SHK_BAD_ARG, "a 2D solid density takes the x and y face arrays (third NULL) or none");
+        c->density[0] = du;
+        c->density[1] = nullptr;
+        c->density[2] = dv;
+        return SHK_OK;
+    }
     if (given != 0 && given != 3) return c->fail(SHK_BAD_ARG, "solid density needs all three face arrays or none");
     c->density[0] = du;
     c->density[1] = dv;
@@ -533,15 +570,19 @@ int shk_set_solid_density(shk_ctx *c, const float *du, const float *dv, const fl
     return SHK_OK;
 }
 
-int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float *solid, const float *us,
-                const float *vs, const float *ws, const float *fluid, double surface_tension, shk_result *res) {
-    if (!c) return SHK_BAD_ARG;
-    c->err.clear();
-    const void *ptrs[8] = {u, v, w, solid, us, vs, ws, fluid};
-    int rc = validate_fields(c, ptrs, 8);
-    if (rc) return rc;
+namespace {
+
+// The whole projection.  user_vel / user_svel are the caller's face arrays per axis;
+// a 2D context (held as nx x 1 x ny) passes nullptr for axis 1, which is then the
+// context's zero wall faces.
+int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *const user_svel[3], const float *solid,
+                 const float *fluid, double surface_tension, shk_result *res) {
+    int rc;
     if (!(dt == dt) || std::isinf(dt)) return c->fail(SHK_BAD_ARG, "dt is not finite");
     if (!std::isfinite(surface_tension)) return c->fail(SHK_BAD_ARG, "surface_tension is not finite");
+    const bool two_d = c->dims.bx == 64;
+    if (two_d && c->prm.precond >= 2)
+        return c->fail(SHK_BAD_ARG, "2D projections support Preconditioner none and jacobi");
     HIPC(c, hipSetDevice(c->device));
     shk_result local{};
     if (!res) res = &local;
@@ -549,29 +590,35 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     res->precision = c->prm.precision;
     const Dims d = c->dims;
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    const int64_t nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    const int64_t nn = two_d ? (int64_t)(d.nx + 1) * (d.nz + 1) : (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
     const int64_t nf[3] = {face_count(d, 0), face_count(d, 1), face_count(d, 2)};
-    float *vel[3] = {u, v, w};
-    const float *svel[3] = {us, vs, ws};
+    float *vel[3] = {user_vel[0], user_vel[1], user_vel[2]};
+    const float *svel[3] = {user_svel[0], user_svel[1], user_svel[2]};
     const float *dsolid = solid, *dfluid = fluid;
     if (c->prm.host_pointers) {
         // stage host fields: 0..2 u v w, 3..5 us vs ws, 6 solid, 7 fluid
-        const float *src[8] = {u, v, w, us, vs, ws, solid, fluid};
+        const float *src[8] = {vel[0], vel[1], vel[2], svel[0], svel[1], svel[2], solid, fluid};
         const int64_t cnt[8] = {nf[0], nf[1], nf[2], nf[0], nf[1], nf[2], nn, G};
         for (int i = 0; i < 8; ++i) {
+            if (!src[i]) continue;
             HIPC(c, c->stage[i].ensure(cnt[i]));
             HIPC(c, hipMemcpyAsync(c->stage[i].p, src[i], sizeof(float) * cnt[i], hipMemcpyHostToDevice, c->stream));
         }
         for (int a = 0; a < 3; ++a) {
-            vel[a] = c->stage[a].p;
-            svel[a] = c->stage[3 + a].p;
+            if (vel[a]) vel[a] = c->stage[a].p;
+            if (svel[a]) svel[a] = c->stage[3 + a].p;
         }
         dsolid = c->stage[6].p;
         dfluid = c->stage[7].p;
     }
+    if (two_d) {
+        vel[1] = c->wall_faces.p;
+        svel[1] = c->wall_faces.p;
+    }
     const float *dens[3] = {c->density[0], c->density[1], c->density[2]};
     if (c->prm.host_pointers && dens[0]) {
         for (int a = 0; a < 3; ++a) {
+            if (!dens[a]) continue;
             HIPC(c, c->stage[10 + a].ensure(nf[a]));
             HIPC(c, hipMemcpyAsync(c->stage[10 + a].p, dens[a], sizeof(float) * nf[a], hipMemcpyHostToDevice,
                                    c->stream));
@@ -606,8 +653,9 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
         c->have_system = false;
         if (c->prm.host_pointers && surface_tension != 0.0)
             for (int a = 0; a < 3; ++a)
-                HIPC(c, hipMemcpyAsync((a == 0 ? u : (a == 1 ? v : w)), vel[a], sizeof(float) * nf[a],
-                                       hipMemcpyDeviceToHost, c->stream));
+                if (user_vel[a])
+                    HIPC(c, hipMemcpyAsync(user_vel[a], vel[a], sizeof(float) * nf[a], hipMemcpyDeviceToHost,
+                                           c->stream));
         HIPC(c, hipStreamSynchronize(c->stream));
         return SHK_OK;
     }
@@ -633,12 +681,12 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     if (c->prm.precond != 0) {
         const bool s64 = c->prm.precision == 0 && c->prm.precond == 1;
         const bool s32 = need32;
-        launch_scale_coeff(c->stream, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p,
+        launch_scale_coeff(c->stream, d.bx, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p,
                            s64 ? c->hx64.p : nullptr, s64 ? c->hy64.p : nullptr, s64 ? c->hz64.p : nullptr,
                            s32 ? c->hx32.p : nullptr, s32 ? c->hy32.p : nullptr, s32 ? c->hz32.p : nullptr);
-        if (s64) launch_brick_uniform<double>(c->stream, c->n_active, c->nbr.p, c->hx64.p, c->hy64.p, c->hz64.p,
+        if (s64) launch_brick_uniform<double>(c->stream, d.bx, c->n_active, c->nbr.p, c->hx64.p, c->hy64.p, c->hz64.p,
                                               c->kappa64.p);
-        if (s32) launch_brick_uniform<float>(c->stream, c->n_active, c->nbr.p, c->hx32.p, c->hy32.p, c->hz32.p,
+        if (s32) launch_brick_uniform<float>(c->stream, d.bx, c->n_active, c->nbr.p, c->hx32.p, c->hy32.p, c->hz32.p,
                                              c->kappa32.p);
     }
     if (c->prm.warm_start) {
@@ -677,8 +725,8 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     HIPC(c, hipEventRecord(c->ev[3], c->stream));
     if (c->prm.host_pointers) {
         for (int a = 0; a < 3; ++a)
-            HIPC(c, hipMemcpyAsync((a == 0 ? u : (a == 1 ? v : w)), vel[a], sizeof(float) * nf[a],
-                                   hipMemcpyDeviceToHost, c->stream));
+            if (user_vel[a])
+                HIPC(c, hipMemcpyAsync(user_vel[a], vel[a], sizeof(float) * nf[a], hipMemcpyDeviceToHost, c->stream));
         c->h_pressure.resize(G);
         HIPC(c, hipMemcpyAsync(c->h_pressure.data(), c->pressure.p, sizeof(float) * G, hipMemcpyDeviceToHost,
                                c->stream));
@@ -688,6 +736,34 @@ int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float
     if ((rc = sync_and_time(c, 1, 2, res->ms_solve))) return rc;
     if ((rc = sync_and_time(c, 2, 3, res->ms_update))) return rc;
     return res->status;
+}
+
+}  // namespace
+
+int shk_project(shk_ctx *c, double dt, float *u, float *v, float *w, const float *solid, const float *us,
+                const float *vs, const float *ws, const float *fluid, double surface_tension, shk_result *res) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    const void *ptrs[8] = {u, v, w, solid, us, vs, ws, fluid};
+    int rc = validate_fields(c, ptrs, 8);
+    if (rc) return rc;
+    if (c->dims.bx != 8) return c->fail(SHK_BAD_ARG, "shk_project needs a 3D context (use shk_project2d)");
+    float *vel[3] = {u, v, w};
+    const float *svel[3] = {us, vs, ws};
+    return project_impl(c, dt, vel, svel, solid, fluid, surface_tension, res);
+}
+
+int shk_project2d(shk_ctx *c, double dt, float *u, float *v, const float *solid, const float *us, const float *vs,
+                  const float *fluid, double surface_tension, shk_result *res) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    const void *ptrs[6] = {u, v, solid, us, vs, fluid};
+    int rc = validate_fields(c, ptrs, 6);
+    if (rc) return rc;
+    if (c->dims.bx != 64) return c->fail(SHK_BAD_ARG, "shk_project2d needs a 2D context (shk_create2d)");
+    float *vel[3] = {u, nullptr, v};
+    const float *svel[3] = {us, nullptr, vs};
+    return project_impl(c, dt, vel, svel, solid, fluid, surface_tension, res);
 }
 
 const float *shk_pressure(const shk_ctx *c) {
@@ -729,6 +805,14 @@ int shk_export_fractions(shk_ctx *c, float *au, float *av, float *aw, float *ru,
     if (!c) return SHK_BAD_ARG;
     HIPC(c, hipSetDevice(c->device));
     float *dst[6] = {au, av, aw, ru, rv, rw};
+    if (c->dims.bx == 64) {
+        // 2D: x faces, then the y faces (held as the z faces of nx x 1 x ny)
+        if (aw || rw) return c->fail(SHK_BAD_ARG, "a 2D context exports x and y faces (third arrays NULL)");
+        dst[1] = nullptr;
+        dst[2] = av;
+        dst[4] = nullptr;
+        dst[5] = rv;
+    }
     const hipMemcpyKind kind = c->prm.host_pointers ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
     for (int i = 0; i < 6; ++i) {
         if (!dst[i]) continue;
@@ -756,9 +840,11 @@ int shk_export_system(shk_ctx *c, uint8_t *label, double *diag, double *cx, doub
 int shk_divergence(shk_ctx *c, double dt, const float *u, const float *v, const float *w, const float *us,
                    const float *vs, const float *ws, int use_fp64_update, double *max_abs_div) {
     if (!c || !max_abs_div) return SHK_BAD_ARG;
-    const void *ptrs[6] = {u, v, w, us, vs, ws};
+    const bool two_d = c->dims.bx == 64;
+    const void *ptrs[6] = {u, v, two_d ? v : w, us, vs, two_d ? vs : ws};
     int rc = validate_fields(c, ptrs, 6);
     if (rc) return rc;
+    if (two_d && (w || ws)) return c->fail(SHK_BAD_ARG, "a 2D context takes x and y faces (third arrays NULL)");
     if (!c->have_system) return c->fail(SHK_BAD_ARG, "no linear system (no projection with unknowns yet)");
     if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_divergence takes device pointers");
     HIPC(c, hipSetDevice(c->device));
@@ -768,6 +854,11 @@ int shk_divergence(shk_ctx *c, double dt, const float *u, const float *v, const 
     float *rho[3] = {c->rho[0].p, c->rho[1].p, c->rho[2].p};
     const float *vel[3] = {u, v, w};
     const float *svel[3] = {us, vs, ws};
+    if (two_d) {
+        vel[1] = svel[1] = c->wall_faces.p;
+        vel[2] = v;
+        svel[2] = vs;
+    }
     const int g = launch_divergence(c->stream, c->dims, dt, c->dx, area, rho, c->label.p, vel, svel, use_fp64_update,
                                     c->brick_slot.p, c->nbx, c->nby, c->x64.p, c->pa.p, maxb);
     HIPC(c, hipGetLastError());

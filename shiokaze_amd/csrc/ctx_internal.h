@@ -100,6 +100,7 @@ struct shk_ctx {
     double current_volume = 0.0, target_volume = 0.0, y_prev = 0.0;
     DevBuf<double> prev64;   // dense FP64 pressure of the last warm-started projection
     DevBuf<double> warm64;   // that pressure gathered onto the current bricks
+    DevBuf<float> wall_faces;  // 2D contexts: zero y-face arrays of the nx x 1 x ny layout
     std::vector<float> h_pressure;
     // bricks
     int n_active = 0;

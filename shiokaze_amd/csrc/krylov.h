@@ -42,6 +42,7 @@ struct BrickOp {
     int n_active;
     // optional, unit-diagonal operators only: per-brick constant coupling (0 = not uniform)
     const C *kappa = nullptr;
+    int bx = 8;  // brick shape: 8 (8x8x8) or 64 (64x1x8, a 2D grid held as nx x 1 x ny)
 };
 struct CsrOp {
     const int64_t *rowptr, *cols;

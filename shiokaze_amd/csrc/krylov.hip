@@ -176,17 +176,19 @@ __device__ inline T lane_shfl(T v, int src) {
     return __shfl(v, src & (kWave - 1), kWave);
 }
 
-template <class T, class C, bool UNIT>
+template <class T, class C, bool UNIT, int BX>
 __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T *__restrict__ r,
                                                  const T *__restrict__ sin, T *__restrict__ sout,
                                                  T *__restrict__ q, Reduction red) {
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
     if (red.st->done) return;
     int t0, t1;
     chunk_range(op.n_active, per, t0, t1);
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
-    const int i = lane & 7, j = lane >> 3;
+    const int i = lane % X, j = lane / X;
     // brick metadata (neighbour slots, uniform coupling) does not depend on beta: the
     // first brick's is loaded before the fold, each next one while the current brick runs
     int slot = t0 + wave;
@@ -211,26 +213,31 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             if (UNIT && op.kappa) mkap = op.kappa[nslot];
         }
         const int64_t base = (int64_t)slot * NT + lane;
-        // halo planes of the four lateral neighbours, one coalesced load per plane: lane
-        // L = a + 8c holds the plane's cell (a, c) (a along the plane's in-brick axis,
-        // c = z); the edge lanes fetch them below with a shuffle from lane j + 8k / i + 8k
-        const int pa = lane & 7, pc = lane >> 3;
+        // halo planes of the lateral neighbours, one coalesced load per plane.  x plane
+        // (Y x 8 cells): lane L = a + Y c holds cell (y = a, z = c); y plane (X x 8, 3D
+        // only): lane L = a + X c holds (x = a, z = c).  The edge lanes fetch them
+        // below with a shuffle from lane j + Y k / i + X k.
+        const int xa = lane % Y, xc = lane / Y;
+        const bool xl = lane < Y * 8;  // lanes holding an x-plane cell (all in 3D, 8 in 2D)
+        const int pa = lane % X, pc = lane / X;
         T hxl = T(0), hxh = T(0), hyl = T(0), hyh = T(0);
-        if (nxm >= 0) {
-            const int64_t e = (int64_t)nxm * NT + 7 + 8 * pa + 64 * pc;
+        if (nxm >= 0 && xl) {
+            const int64_t e = (int64_t)nxm * NT + (X - 1) + X * xa + kWave * xc;
             hxl = r[e] + b * sin[e];
         }
-        if (nxp >= 0) {
-            const int64_t e = (int64_t)nxp * NT + 8 * pa + 64 * pc;
+        if (nxp >= 0 && xl) {
+            const int64_t e = (int64_t)nxp * NT + X * xa + kWave * xc;
             hxh = r[e] + b * sin[e];
         }
-        if (nym >= 0) {
-            const int64_t e = (int64_t)nym * NT + pa + 56 + 64 * pc;
-            hyl = r[e] + b * sin[e];
-        }
-        if (nyp >= 0) {
-            const int64_t e = (int64_t)nyp * NT + pa + 64 * pc;
-            hyh = r[e] + b * sin[e];
+        if constexpr (Y > 1) {
+            if (nym >= 0) {
+                const int64_t e = (int64_t)nym * NT + pa + X * (Y - 1) + kWave * pc;
+                hyl = r[e] + b * sin[e];
+            }
+            if (nyp >= 0) {
+                const int64_t e = (int64_t)nyp * NT + pa + kWave * pc;
+                hyh = r[e] + b * sin[e];
+            }
         }
         T szlo = T(0), szhi = T(0);
         if (nzm >= 0) {
@@ -255,16 +262,19 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
                 const T sk = sv[k];
                 T xm = lane_shfl(sk, lane - 1);
                 T xp = lane_shfl(sk, lane + 1);
-                T ym = lane_shfl(sk, lane - 8);
-                T yp = lane_shfl(sk, lane + 8);
-                const T hxm = lane_shfl(hxl, j + 8 * k);
-                const T hxp = lane_shfl(hxh, j + 8 * k);
-                const T hym = lane_shfl(hyl, i + 8 * k);
-                const T hyp = lane_shfl(hyh, i + 8 * k);
+                const T hxm = lane_shfl(hxl, j + Y * k);
+                const T hxp = lane_shfl(hxh, j + Y * k);
                 if (i == 0) xm = hxm;
-                if (i == 7) xp = hxp;
-                if (j == 0) ym = hym;
-                if (j == 7) yp = hyp;
+                if (i == X - 1) xp = hxp;
+                T ym = T(0), yp = T(0);
+                if constexpr (Y > 1) {
+                    ym = lane_shfl(sk, lane - X);
+                    yp = lane_shfl(sk, lane + X);
+                    const T hym = lane_shfl(hyl, i + X * k);
+                    const T hyp = lane_shfl(hyh, i + X * k);
+                    if (j == 0) ym = hym;
+                    if (j == Y - 1) yp = hyp;
+                }
                 const T zm = k > 0 ? sv[k - 1] : szlo;
                 const T zp = k < 7 ? sv[k + 1] : szhi;
                 // same association as the general path: c*a + c*b + ... summed left to right
@@ -281,39 +291,44 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
         for (int k = 0; k < 8; ++k) {
             const int64_t e = base + 64 * k;
             cxv[k] = (T)op.cx[e];
-            cyv[k] = (T)op.cy[e];
+            cyv[k] = Y > 1 ? (T)op.cy[e] : T(0);
             czv[k] = (T)op.cz[e];
             if (!UNIT) dv[k] = (T)op.diag[e];
         }
         // couplings across the +x / +y / +z faces live in the neighbour bricks
         T cxh = T(0), cyh = T(0), czhi = T(0);
-        if (nxp >= 0) cxh = (T)op.cx[(int64_t)nxp * NT + 8 * pa + 64 * pc];
-        if (nyp >= 0) cyh = (T)op.cy[(int64_t)nyp * NT + pa + 64 * pc];
+        if (nxp >= 0 && xl) cxh = (T)op.cx[(int64_t)nxp * NT + X * xa + kWave * xc];
+        if constexpr (Y > 1) {
+            if (nyp >= 0) cyh = (T)op.cy[(int64_t)nyp * NT + pa + kWave * pc];
+        }
         if (nzp >= 0) czhi = (T)op.cz[(int64_t)nzp * NT + lane];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const T sk = sv[k];
             T xm = lane_shfl(sk, lane - 1);
             T xp = lane_shfl(sk, lane + 1);
-            T ym = lane_shfl(sk, lane - 8);
-            T yp = lane_shfl(sk, lane + 8);
             T cxp = lane_shfl(cxv[k], lane + 1);
-            T cyp = lane_shfl(cyv[k], lane + 8);
-            const T hxm = lane_shfl(hxl, j + 8 * k);
-            const T hxp = lane_shfl(hxh, j + 8 * k);
-            const T hym = lane_shfl(hyl, i + 8 * k);
-            const T hyp = lane_shfl(hyh, i + 8 * k);
-            const T hcx = lane_shfl(cxh, j + 8 * k);
-            const T hcy = lane_shfl(cyh, i + 8 * k);
+            const T hxm = lane_shfl(hxl, j + Y * k);
+            const T hxp = lane_shfl(hxh, j + Y * k);
+            const T hcx = lane_shfl(cxh, j + Y * k);
             if (i == 0) xm = hxm;
-            if (i == 7) {
+            if (i == X - 1) {
                 xp = hxp;
                 cxp = hcx;
             }
-            if (j == 0) ym = hym;
-            if (j == 7) {
-                yp = hyp;
-                cyp = hcy;
+            T ym = T(0), yp = T(0), cyp = T(0);
+            if constexpr (Y > 1) {
+                ym = lane_shfl(sk, lane - X);
+                yp = lane_shfl(sk, lane + X);
+                cyp = lane_shfl(cyv[k], lane + X);
+                const T hym = lane_shfl(hyl, i + X * k);
+                const T hyp = lane_shfl(hyh, i + X * k);
+                const T hcy = lane_shfl(cyh, i + X * k);
+                if (j == 0) ym = hym;
+                if (j == Y - 1) {
+                    yp = hyp;
+                    cyp = hcy;
+                }
             }
             const T zm = k > 0 ? sv[k - 1] : szlo;
             const T zp = k < 7 ? sv[k + 1] : szhi;
@@ -412,40 +427,44 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
 
 // ---------------------------------------------------------------- true residual
 
-template <class T, class C>
+template <class T, class C, int BX>
 __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, const C *__restrict__ diag,
                                 const C *__restrict__ cx, const C *__restrict__ cy, const C *__restrict__ cz,
                                 const T *__restrict__ v) {
-    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
+    const int i = l % X, j = (l / X) % Y, k = l / kWave;
     T acc = (T)diag[e] * v[e];
     if (i > 0) {
         acc -= (T)cx[e] * v[e - 1];
     } else {
         const int nb = nb6[0];
-        if (nb >= 0) acc -= (T)cx[e] * v[(int64_t)nb * NT + l + 7];
+        if (nb >= 0) acc -= (T)cx[e] * v[(int64_t)nb * NT + l + (X - 1)];
     }
-    if (i < 7) {
+    if (i < X - 1) {
         acc -= (T)cx[e + 1] * v[e + 1];
     } else {
         const int nb = nb6[1];
         if (nb >= 0) {
-            const int64_t m = (int64_t)nb * NT + l - 7;
+            const int64_t m = (int64_t)nb * NT + l - (X - 1);
             acc -= (T)cx[m] * v[m];
         }
     }
-    if (j > 0) {
-        acc -= (T)cy[e] * v[e - 8];
-    } else {
-        const int nb = nb6[2];
-        if (nb >= 0) acc -= (T)cy[e] * v[(int64_t)nb * NT + l + 56];
-    }
-    if (j < 7) {
-        acc -= (T)cy[e + 8] * v[e + 8];
-    } else {
-        const int nb = nb6[3];
-        if (nb >= 0) {
-            const int64_t m = (int64_t)nb * NT + l - 56;
-            acc -= (T)cy[m] * v[m];
+    if constexpr (Y > 1) {
+        if (j > 0) {
+            acc -= (T)cy[e] * v[e - X];
+        } else {
+            const int nb = nb6[2];
+            if (nb >= 0) acc -= (T)cy[e] * v[(int64_t)nb * NT + l + X * (Y - 1)];
+        }
+        if (j < Y - 1) {
+            acc -= (T)cy[e + X] * v[e + X];
+        } else {
+            const int nb = nb6[3];
+            if (nb >= 0) {
+                const int64_t m = (int64_t)nb * NT + l - X * (Y - 1);
+                acc -= (T)cy[m] * v[m];
+            }
         }
     }
     if (k > 0) {
@@ -466,6 +485,7 @@ __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, c
     return acc;
 }
 
+template <int BX>
 __global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, int per, const double *__restrict__ b,
                                                  const double *__restrict__ x, double *__restrict__ r,
                                                  double *__restrict__ part, const double *__restrict__ sig,
@@ -480,7 +500,7 @@ __global__ __launch_bounds__(NT) void k_residual(BrickOp<double> op, int per, co
         double rv = 0.0, bv = 0.0;
         if (op.diag[e] != 0.0) {
             bv = b[e];
-            rv = bv - brick_apply<double, double>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, x);
+            rv = bv - brick_apply<double, double, BX>(op.nbr + 6 * slot, e, l, op.diag, op.cx, op.cy, op.cz, x);
         }
         if (r) r[e] = rv;
         if (rs) rs[e] = (float)(sig ? sig[e] * rv : rv);
@@ -621,10 +641,21 @@ void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
                      Reduction red, KEvents ev) {
-    if (op.diag)
-        SHK_LAUNCH(ev, (k_dir_spmv<T, C, false>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q, red);
-    else
-        SHK_LAUNCH(ev, (k_dir_spmv<T, C, true>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q, red);
+    if (op.diag) {
+        if (op.bx == 64)
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
+                       red);
+        else
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
+                       red);
+    } else {
+        if (op.bx == 64)
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
+                       red);
+        else
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
+                       red);
+    }
 }
 
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
@@ -645,7 +676,8 @@ void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T 
 
 void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,
                      double *r, double *part, double *out, const double *sig, float *rs) {
-    hipLaunchKernelGGL(k_residual, dim3(g.blocks), dim3(NT), 0, st, op, g.per, b, x, r, part, sig, rs);
+    auto kern = op.bx == 64 ? k_residual<64> : k_residual<8>;
+    hipLaunchKernelGGL(kern, dim3(g.blocks), dim3(NT), 0, st, op, g.per, b, x, r, part, sig, rs);
     hipLaunchKernelGGL(k_finish_residual, dim3(1), dim3(NF), 0, st, g.blocks, part, out);
 }
 

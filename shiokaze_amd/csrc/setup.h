@@ -35,13 +35,13 @@ void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, co
                   double *rhs, double *sig, float *diag32, float *cx32, float *cy32, float *cz32);
 
 // Couplings of D^-1/2 A D^-1/2 (FP64 and/or FP32 outputs; any may be nullptr).
-void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const double *sig, const double *cx,
+void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, const double *sig, const double *cx,
                         const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
                         float *hy32, float *hz32);
 
 // kappa[slot] = the one coupling value of a brick whose stencil is uniform, else 0.
 template <class C>
-void launch_brick_uniform(hipStream_t st, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
+void launch_brick_uniform(hipStream_t st, int bx, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
                           C *kappa);
 
 // prev (nullable): also the dense FP64 pressure, for the next warm start

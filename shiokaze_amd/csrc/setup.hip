@@ -25,7 +25,7 @@ inline unsigned grid_for(int64_t n, int nt = kNT) {
 
 // ---------------------------------------------------------------- fractions (a3, a4, a5)
 
-template <int DIM>
+template <int DIM, bool TWO_D>
 __global__ void k_fractions(Dims d, const float *__restrict__ solid, const float *__restrict__ fluid,
                             double eps_solid, double eps_fluid, int second_order_solid, int second_order_fluid,
                             const float *__restrict__ density, float *__restrict__ area, float *__restrict__ rho) {
@@ -36,7 +36,7 @@ __global__ void k_fractions(Dims d, const float *__restrict__ solid, const float
         const int i = (int)(f % fx);
         const int j = (int)((f / fx) % fy);
         const int k = (int)(f / ((int64_t)fx * fy));
-        double a = face_area(d, solid, DIM, i, j, k, eps_solid);
+        double a = TWO_D ? face_area_2d(d, solid, DIM, i, k, eps_solid) : face_area(d, solid, DIM, i, j, k, eps_solid);
         double r = face_rho(d, fluid, DIM, i, j, k, eps_fluid);
         float af = (float)a, rf = (float)r;
         if (density) {
@@ -77,6 +77,7 @@ __device__ inline bool is_unknown(const Dims &d, const float *fluid, const Faces
     return false;
 }
 
+template <int BX>
 __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8_t *__restrict__ label,
                         uint8_t *__restrict__ brick_flag, int nbx, int nby, unsigned long long *__restrict__ count) {
     __shared__ unsigned long long wsum[kNT / kWave];
@@ -89,7 +90,8 @@ __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8
         const bool u = is_unknown(d, fluid, fc, i, j, k);
         label[c] = u ? 1 : 0;
         if (u) {
-            brick_flag[(i / kBrick) + (int64_t)nbx * ((j / kBrick) + (int64_t)nby * (k / kBrick))] = 1;
+            using B = BrickShape<BX>;
+            brick_flag[(i / B::X) + (int64_t)nbx * ((j / B::Y) + (int64_t)nby * (k / B::Z))] = 1;
             ++local;
         }
     }
@@ -106,6 +108,7 @@ __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8
 
 // ---------------------------------------------------------------- coefficients + rhs (a7)
 
+template <int BX>
 __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ brick_coord, Faces fc,
                         const uint8_t *__restrict__ label, const float *__restrict__ u, const float *__restrict__ v,
                         const float *__restrict__ w, const float *__restrict__ us, const float *__restrict__ vs,
@@ -117,7 +120,8 @@ __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ b
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
     const int4 bc = brick_coord[slot];
-    const int i = bc.x * kBrick + (l & 7), j = bc.y * kBrick + ((l >> 3) & 7), k = bc.z * kBrick + (l >> 6);
+    using B = BrickShape<BX>;
+    const int i = bc.x * B::X + l % B::X, j = bc.y * B::Y + (l / B::X) % B::Y, k = bc.z * B::Z + l / kWave;
     const int64_t e = (int64_t)slot * kBrickCells + l;
     double cd = 0.0, c0 = 0.0, c1 = 0.0, c2 = 0.0, b = 0.0;
     if (i < d.nx && j < d.ny && k < d.nz && label[cell_index(d, i, j, k)]) {
@@ -180,15 +184,18 @@ __global__ void k_coeff(Dims d, double dt, double dx, const int4 *__restrict__ b
 }
 
 // Couplings of the Jacobi-scaled system D^-1/2 A D^-1/2: c * sig_cell * sig_low.
+template <int BX>
 __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restrict__ sig,
                               const double *__restrict__ cx, const double *__restrict__ cy,
                               const double *__restrict__ cz, double *__restrict__ hx, double *__restrict__ hy,
                               double *__restrict__ hz, float *__restrict__ hx32, float *__restrict__ hy32,
                               float *__restrict__ hz32) {
+    using B = BrickShape<BX>;
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
-    const int loc[3] = {l & 7, (l >> 3) & 7, l >> 6};
-    const int stride[3] = {1, 8, 64};
+    const int loc[3] = {l % B::X, (l / B::X) % B::Y, l / kWave};
+    const int stride[3] = {1, B::X, kWave};
+    const int ext[3] = {B::X, B::Y, B::Z};
     const int64_t e = (int64_t)slot * kBrickCells + l;
     const double *c[3] = {cx, cy, cz};
     double *h[3] = {hx, hy, hz};
@@ -203,7 +210,7 @@ __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restr
                 m = e - stride[a];
             } else {
                 const int nb = nbr[6 * slot + 2 * a];
-                m = nb >= 0 ? (int64_t)nb * kBrickCells + l + 7 * stride[a] : -1;
+                m = nb >= 0 ? (int64_t)nb * kBrickCells + l + (ext[a] - 1) * stride[a] : -1;
             }
             if (m >= 0) v = ca * se * sig[m];
         }
@@ -216,25 +223,28 @@ __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restr
 // brick's cells (its 3 low faces per cell and the far faces stored in the +x/+y/+z
 // neighbour bricks) equals one nonzero value c bit for bit, else 0.  Such a brick
 // has only unknowns and the stencil needs no coefficient loads there.
-template <class C>
+template <class C, int BX>
 __global__ void k_brick_uniform(const int *__restrict__ nbr, const C *__restrict__ cx, const C *__restrict__ cy,
                                 const C *__restrict__ cz, C *__restrict__ kappa) {
+    using B = BrickShape<BX>;
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
-    const int loc[3] = {l & 7, (l >> 3) & 7, l >> 6};
-    const int stride[3] = {1, 8, 64};
+    const int loc[3] = {l % B::X, (l / B::X) % B::Y, l / kWave};
+    const int stride[3] = {1, B::X, kWave};
+    const int ext[3] = {B::X, B::Y, B::Z};
     const C *c[3] = {cx, cy, cz};
     const int64_t e = (int64_t)slot * kBrickCells + l;
     const C k0 = cx[(int64_t)slot * kBrickCells];
     bool ok = k0 != C(0);
     for (int a = 0; a < 3; ++a) {
+        if (ext[a] == 1) continue;  // the y axis of a 2D grid has no couplings
         ok = ok && c[a][e] == k0;
         int64_t m;
-        if (loc[a] < 7) {
+        if (loc[a] < ext[a] - 1) {
             m = e + stride[a];
         } else {
             const int nb = nbr[6 * slot + 2 * a + 1];
-            m = nb >= 0 ? (int64_t)nb * kBrickCells + l - 7 * stride[a] : -1;
+            m = nb >= 0 ? (int64_t)nb * kBrickCells + l - (ext[a] - 1) * stride[a] : -1;
         }
         ok = ok && m >= 0 && c[a][m] == k0;
     }
@@ -244,12 +254,15 @@ __global__ void k_brick_uniform(const int *__restrict__ nbr, const C *__restrict
 
 // ---------------------------------------------------------------- scatter + update (a14, a15)
 
+template <int BX>
 __device__ inline int64_t brick_elem(const Dims &d, const int *brick_slot, int nbx, int nby, int i, int j, int k) {
-    const int s = brick_slot[(i / kBrick) + (int64_t)nbx * ((j / kBrick) + (int64_t)nby * (k / kBrick))];
+    using B = BrickShape<BX>;
+    const int s = brick_slot[(i / B::X) + (int64_t)nbx * ((j / B::Y) + (int64_t)nby * (k / B::Z))];
     if (s < 0) return -1;
-    return (int64_t)s * kBrickCells + (i & 7) + 8 * (j & 7) + 64 * (k & 7);
+    return (int64_t)s * kBrickCells + B::elem(i, j, k);
 }
 
+template <int BX>
 __global__ void k_scatter(Dims d, const int *__restrict__ brick_slot, int nbx, int nby, const uint8_t *__restrict__ label,
                           const double *__restrict__ x, float *__restrict__ pressure, double *__restrict__ prev) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
@@ -259,7 +272,7 @@ __global__ void k_scatter(Dims d, const int *__restrict__ brick_slot, int nbx, i
         const int k = (int)(c / ((int64_t)d.nx * d.ny));
         double p = 0.0;
         if (label[c]) {
-            const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+            const int64_t e = brick_elem<BX>(d, brick_slot, nbx, nby, i, j, k);
             if (e >= 0) p = x[e];
         }
         pressure[c] = (float)p;
@@ -327,6 +340,23 @@ __global__ void k_curvature(Dims d, double dx, const float *__restrict__ fluid, 
     }
 }
 
+// The 2D curvature of a grid held as nx x 1 x ny (macpressuresolver2.cpp:93-102):
+// the clamped 5-point Laplacian, neighbours summed in float (x-, x+, y-, y+).
+__global__ void k_curvature_2d(Dims d, double dx, const float *__restrict__ fluid, float *__restrict__ curv) {
+#pragma clang fp contract(off)
+    const int64_t G = (int64_t)d.nx * d.nz;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % d.nx);
+        const int k = (int)(c / d.nx);
+        float s = fluid[clampi(i - 1, 0, d.nx - 1) + (int64_t)d.nx * k];
+        s = s + fluid[clampi(i + 1, 0, d.nx - 1) + (int64_t)d.nx * k];
+        s = s + fluid[i + (int64_t)d.nx * clampi(k - 1, 0, d.nz - 1)];
+        s = s + fluid[i + (int64_t)d.nx * clampi(k + 1, 0, d.nz - 1)];
+        const double v = ((double)s - 4.0 * (double)fluid[c]) / (dx * dx);
+        curv[c] = (float)v;
+    }
+}
+
 // Ghost-fluid surface-tension jump on faces with 0 < rho < 1: the face curvature
 // interpolated with the liquid fraction, added to the face velocity (the increment
 // is rounded to float, then added in float as array3::increment does).
@@ -358,12 +388,14 @@ __global__ void k_surface_tension(Dims d, double dt, double dx, double kappa, co
 
 // The previous projection's FP64 pressure (dense) onto the current bricks, zero off
 // the unknowns.
+template <int BX>
 __global__ void k_gather_warm(Dims d, const int4 *__restrict__ brick_coord, const uint8_t *__restrict__ label,
                               const double *__restrict__ prev, double *__restrict__ w) {
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
     const int4 bc = brick_coord[slot];
-    const int i = bc.x * kBrick + (l & 7), j = bc.y * kBrick + ((l >> 3) & 7), k = bc.z * kBrick + (l >> 6);
+    using B = BrickShape<BX>;
+    const int i = bc.x * B::X + l % B::X, j = bc.y * B::Y + (l / B::X) % B::Y, k = bc.z * B::Z + l / kWave;
     double v = 0.0;
     if (i < d.nx && j < d.ny && k < d.nz) {
         const int64_t c = cell_index(d, i, j, k);
@@ -379,13 +411,15 @@ __global__ void k_add64(int64_t n, double *__restrict__ x, const double *__restr
 
 // ---------------------------------------------------------------- diagnostics
 
+template <int BX>
 __device__ inline double xval(const Dims &d, const int *brick_slot, int nbx, int nby, const double *x, int i, int j,
                               int k) {
     if (i < 0 || i >= d.nx || j < 0 || j >= d.ny || k < 0 || k >= d.nz) return 0.0;
-    const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+    const int64_t e = brick_elem<BX>(d, brick_slot, nbx, nby, i, j, k);
     return e >= 0 ? x[e] : 0.0;
 }
 
+template <int BX>
 __global__ void k_divergence(Dims d, double dt, double dx, Faces fc, const uint8_t *__restrict__ label,
                              const float *__restrict__ u, const float *__restrict__ v, const float *__restrict__ w,
                              const float *__restrict__ us, const float *__restrict__ vs, const float *__restrict__ ws,
@@ -418,8 +452,8 @@ __global__ void k_divergence(Dims d, double dt, double dx, Faces fc, const uint8
                     if (pi == 0 || pi == n) {
                         uf = svel[dim][f];
                     } else {
-                        const double p1 = xval(d, brick_slot, nbx, nby, x, ai, aj, ak);
-                        const double p0 = xval(d, brick_slot, nbx, nby, x, ai - (dim == 0), aj - (dim == 1),
+                        const double p1 = xval<BX>(d, brick_slot, nbx, nby, x, ai, aj, ak);
+                        const double p0 = xval<BX>(d, brick_slot, nbx, nby, x, ai - (dim == 0), aj - (dim == 1),
                                                ak - (dim == 2));
                         uf = uf - dt * (p1 - p0) / (r * dx);
                     }
@@ -436,6 +470,7 @@ __global__ void k_divergence(Dims d, double dt, double dx, Faces fc, const uint8
     if (threadIdx.x == 0) partial[blockIdx.x] = m;
 }
 
+template <int BX>
 __global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, int nby, const uint8_t *__restrict__ label,
                          const double *__restrict__ diag, const double *__restrict__ cx, const double *__restrict__ cy,
                          const double *__restrict__ cz, const double *__restrict__ rhs, const double *__restrict__ x,
@@ -446,7 +481,7 @@ __global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, in
         const int i = (int)(c % d.nx);
         const int j = (int)((c / d.nx) % d.ny);
         const int k = (int)(c / ((int64_t)d.nx * d.ny));
-        const int64_t e = brick_elem(d, brick_slot, nbx, nby, i, j, k);
+        const int64_t e = brick_elem<BX>(d, brick_slot, nbx, nby, i, j, k);
         if (o_label) o_label[c] = label[c];
         if (o_diag) o_diag[c] = e >= 0 ? diag[e] : 0.0;
         if (o_cx) o_cx[c] = e >= 0 ? cx[e] : 0.0;
@@ -464,22 +499,36 @@ __global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, in
 void launch_fractions(hipStream_t st, Dims d, const float *solid, const float *fluid, double eps_solid,
                       double eps_fluid, int so_solid, int so_fluid, const float *const density[3],
                       float *const area[3], float *const rho[3]) {
-    hipLaunchKernelGGL(k_fractions<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, density[0], area[0], rho[0]);
-    hipLaunchKernelGGL(k_fractions<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, density[1], area[1], rho[1]);
-    hipLaunchKernelGGL(k_fractions<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, solid, fluid, eps_solid,
-                       eps_fluid, so_solid, so_fluid, density[2], area[2], rho[2]);
+    if (d.bx == 64) {
+        // a 2D grid: x faces and the 2D y faces (held as z faces); the y faces of the
+        // nx x 1 x ny layout are walls whose arrays stay zero
+        hipLaunchKernelGGL((k_fractions<0, true>), dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, solid,
+                           fluid, eps_solid, eps_fluid, so_solid, so_fluid, density[0], area[0], rho[0]);
+        hipLaunchKernelGGL((k_fractions<2, true>), dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, solid,
+                           fluid, eps_solid, eps_fluid, so_solid, so_fluid, density[2], area[2], rho[2]);
+        return;
+    }
+    hipLaunchKernelGGL((k_fractions<0, false>), dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, solid, fluid,
+                       eps_solid, eps_fluid, so_solid, so_fluid, density[0], area[0], rho[0]);
+    hipLaunchKernelGGL((k_fractions<1, false>), dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, solid, fluid,
+                       eps_solid, eps_fluid, so_solid, so_fluid, density[1], area[1], rho[1]);
+    hipLaunchKernelGGL((k_fractions<2, false>), dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, solid, fluid,
+                       eps_solid, eps_fluid, so_solid, so_fluid, density[2], area[2], rho[2]);
 }
 
 void launch_surface_tension(hipStream_t st, Dims d, double dt, double dx, double kappa, const float *fluid,
                             float *curv, float *const rho[3], float *const vel[3]) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_curvature, dim3(grid_for(G)), dim3(kNT), 0, st, d, dx, fluid, curv);
+    if (d.bx == 64) {
+        hipLaunchKernelGGL(k_curvature_2d, dim3(grid_for(G)), dim3(kNT), 0, st, d, dx, fluid, curv);
+    } else {
+        hipLaunchKernelGGL(k_curvature, dim3(grid_for(G)), dim3(kNT), 0, st, d, dx, fluid, curv);
+    }
     hipLaunchKernelGGL(k_surface_tension<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, dt, dx, kappa,
                        fluid, curv, rho[0], vel[0]);
-    hipLaunchKernelGGL(k_surface_tension<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, kappa,
-                       fluid, curv, rho[1], vel[1]);
+    if (d.bx != 64)
+        hipLaunchKernelGGL(k_surface_tension<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, kappa,
+                           fluid, curv, rho[1], vel[1]);
     hipLaunchKernelGGL(k_surface_tension<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, dt, dx, kappa,
                        fluid, curv, rho[2], vel[2]);
 }
@@ -487,7 +536,10 @@ void launch_surface_tension(hipStream_t st, Dims d, double dt, double dx, double
 void launch_gather_warm(hipStream_t st, Dims d, int n_active, const int4 *brick_coord, const uint8_t *label,
                         const double *prev, double *w) {
     if (n_active <= 0) return;
-    hipLaunchKernelGGL(k_gather_warm, dim3(n_active), dim3(kBrickCells), 0, st, d, brick_coord, label, prev, w);
+    if (d.bx == 64)
+        hipLaunchKernelGGL(k_gather_warm<64>, dim3(n_active), dim3(kBrickCells), 0, st, d, brick_coord, label, prev, w);
+    else
+        hipLaunchKernelGGL(k_gather_warm<8>, dim3(n_active), dim3(kBrickCells), 0, st, d, brick_coord, label, prev, w);
 }
 
 void launch_add64(hipStream_t st, int64_t n, double *x, const double *w) {
@@ -500,7 +552,10 @@ void launch_label(hipStream_t st, Dims d, const float *fluid, float *const area[
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     unsigned g = grid_for(G);
     if (g > 4096) g = 4096;  // grid-stride: bounded number of count atomics
-    hipLaunchKernelGGL(k_label, dim3(g), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
+    if (d.bx == 64)
+        hipLaunchKernelGGL(k_label<64>, dim3(g), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
+    else
+        hipLaunchKernelGGL(k_label<8>, dim3(g), dim3(kNT), 0, st, d, fluid, fc, label, brick_flag, nbx, nby, count);
 }
 
 void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, const int4 *brick_coord,
@@ -509,35 +564,38 @@ void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, co
                   double *rhs, double *sig, float *diag32, float *cx32, float *cy32, float *cz32) {
     Faces fc{{area[0], area[1], area[2]}, {rho[0], rho[1], rho[2]}};
     if (n_active <= 0) return;
-    hipLaunchKernelGGL(k_coeff, dim3(n_active), dim3(kBrickCells), 0, st, d, dt, dx, brick_coord, fc, label, vel[0],
+    auto kern = d.bx == 64 ? k_coeff<64> : k_coeff<8>;
+    hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, d, dt, dx, brick_coord, fc, label, vel[0],
                        vel[1], vel[2], svel[0], svel[1], svel[2], rhs_correct, diag, cx, cy, cz, rhs, sig, diag32,
                        cx32, cy32, cz32);
 }
 
-void launch_scale_coeff(hipStream_t st, int n_active, const int *nbr, const double *sig, const double *cx,
+void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, const double *sig, const double *cx,
                         const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
                         float *hy32, float *hz32) {
     if (n_active <= 0) return;
-    hipLaunchKernelGGL(k_scale_coeff, dim3(n_active), dim3(kBrickCells), 0, st, nbr, sig, cx, cy, cz, hx, hy, hz, hx32,
-                       hy32, hz32);
+    auto kern = bx == 64 ? k_scale_coeff<64> : k_scale_coeff<8>;
+    hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, nbr, sig, cx, cy, cz, hx, hy, hz, hx32, hy32,
+                       hz32);
 }
 
 template <class C>
-void launch_brick_uniform(hipStream_t st, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
+void launch_brick_uniform(hipStream_t st, int bx, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
                           C *kappa) {
     if (n_active <= 0) return;
-    hipLaunchKernelGGL(k_brick_uniform<C>, dim3(n_active), dim3(kBrickCells), 0, st, nbr, cx, cy, cz, kappa);
+    auto kern = bx == 64 ? k_brick_uniform<C, 64> : k_brick_uniform<C, 8>;
+    hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, nbr, cx, cy, cz, kappa);
 }
-template void launch_brick_uniform<double>(hipStream_t, int, const int *, const double *, const double *,
+template void launch_brick_uniform<double>(hipStream_t, int, int, const int *, const double *, const double *,
                                            const double *, double *);
-template void launch_brick_uniform<float>(hipStream_t, int, const int *, const float *, const float *, const float *,
-                                          float *);
+template void launch_brick_uniform<float>(hipStream_t, int, int, const int *, const float *, const float *,
+                                          const float *, float *);
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
                     const double *x, float *pressure, double *prev) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, x, pressure,
-                       prev);
+    auto kern = d.bx == 64 ? k_scatter<64> : k_scatter<8>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, x, pressure, prev);
 }
 
 void launch_update_velocity(hipStream_t st, Dims d, double dt, double dx, const float *pressure, float *const area[3],
@@ -545,8 +603,9 @@ void launch_update_velocity(hipStream_t st, Dims d, double dt, double dx, const 
                             float *const vel[3]) {
     hipLaunchKernelGGL(k_update_velocity<0>, dim3(grid_for(face_count(d, 0))), dim3(kNT), 0, st, d, dt, dx, pressure,
                        area[0], rho[0], svel[0], density[0], vel[0]);
-    hipLaunchKernelGGL(k_update_velocity<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx, pressure,
-                       area[1], rho[1], svel[1], density[1], vel[1]);
+    if (d.bx != 64)  // the y faces of a 2D grid's nx x 1 x ny layout are walls
+        hipLaunchKernelGGL(k_update_velocity<1>, dim3(grid_for(face_count(d, 1))), dim3(kNT), 0, st, d, dt, dx,
+                           pressure, area[1], rho[1], svel[1], density[1], vel[1]);
     hipLaunchKernelGGL(k_update_velocity<2>, dim3(grid_for(face_count(d, 2))), dim3(kNT), 0, st, d, dt, dx, pressure,
                        area[2], rho[2], svel[2], density[2], vel[2]);
 }
@@ -558,8 +617,9 @@ int launch_divergence(hipStream_t st, Dims d, double dt, double dx, float *const
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     unsigned g = grid_for(G);
     if ((int)g > max_blocks) g = (unsigned)max_blocks;
-    hipLaunchKernelGGL(k_divergence, dim3(g), dim3(kNT), 0, st, d, dt, dx, fc, label, vel[0], vel[1], vel[2], svel[0],
-                       svel[1], svel[2], use_fp64_update, brick_slot, nbx, nby, x, partial);
+    auto kern = d.bx == 64 ? k_divergence<64> : k_divergence<8>;
+    hipLaunchKernelGGL(kern, dim3(g), dim3(kNT), 0, st, d, dt, dx, fc, label, vel[0], vel[1], vel[2], svel[0], svel[1],
+                       svel[2], use_fp64_update, brick_slot, nbx, nby, x, partial);
     return (int)g;
 }
 
@@ -568,7 +628,8 @@ void launch_export(hipStream_t st, Dims d, const int *brick_slot, int nbx, int n
                    const double *x, uint8_t *o_label, double *o_diag, double *o_cx, double *o_cy, double *o_cz,
                    double *o_rhs, double *o_x) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
-    hipLaunchKernelGGL(k_export, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, diag, cx, cy, cz,
+    auto kern = d.bx == 64 ? k_export<64> : k_export<8>;
+    hipLaunchKernelGGL(kern, dim3(grid_for(G)), dim3(kNT), 0, st, d, brick_slot, nbx, nby, label, diag, cx, cy, cz,
                        rhs, x, o_label, o_diag, o_cx, o_cy, o_cz, o_rhs, o_x);
 }
 

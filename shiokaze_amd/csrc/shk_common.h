@@ -9,10 +9,20 @@
 
 namespace shk {
 
-constexpr int kBrick = 8;                 // brick edge (cells)
-constexpr int kBrickCells = 512;          // 8^3 cells, one workgroup per brick
+constexpr int kBrick = 8;                 // brick edge of 3D grids (cells)
+constexpr int kBrickCells = 512;          // cells per brick, one workgroup per brick
 constexpr int kWave = 64;                 // CDNA wavefront width
 constexpr int kXcds = 8;                  // MI355X accelerator complex dies
+
+// Brick shapes: 8x8x8 for 3D grids, 64x1x8 for 2D grids (stored as nx x 1 x ny, the
+// 2D y axis on z).  Either way a brick holds 512 cells x-fastest and one wave covers
+// one z layer of 64 cells, so the stencil kernels keep a z column per lane.
+template <int BX>
+struct BrickShape {
+    static constexpr int X = BX, Y = kWave / BX, Z = kBrickCells / kWave;
+    static_assert(X * Y == kWave && Z == 8, "a brick layer is one wave");
+    __host__ __device__ static int elem(int i, int j, int k) { return (i % X) + X * ((j % Y) + Y * (k % Z)); }
+};
 
 // Krylov state kept on the device so that iterations are launched back to back
 // without a host round trip; the host polls `done` every few iterations.

@@ -67,6 +67,7 @@ __host__ __device__ inline double get_area(double q00, double q10, double q11, d
 
 struct Dims {
     int nx, ny, nz;
+    int bx = 8;  // brick x extent: 8 (8x8x8 bricks, 3D) or 64 (64x1x8 bricks, a 2D grid as nx x 1 x ny)
 };
 
 __host__ __device__ inline int64_t cell_index(const Dims &d, int i, int j, int k) {
@@ -113,6 +114,27 @@ __host__ __device__ inline double face_area(const Dims &d, const float *solid, i
             q01 = solid[node_index(d, i, j + 1, k)];
         }
         area = 1.0 - get_area(q00, q10, q11, q01);
+    }
+    if (area != 0.0 && area < eps_solid) area = eps_solid;
+    return area;
+}
+
+// Open solid area of a face of a 2D grid held as nx x 1 x ny (macutility2.cpp:104-111):
+// 0 on the domain walls, else 1 - fraction of the face's two end nodes of the 2D
+// nodal solid ((nx+1) x (ny+1), index i + (nx+1) j), raised to eps_solid when
+// positive but smaller.  dim is 0 (x faces) or 2 (the 2D y faces).
+__host__ __device__ inline double face_area_2d(const Dims &d, const float *solid2, int dim, int i, int k,
+                                               double eps_solid) {
+#pragma clang fp contract(off)
+    const int pi = dim == 0 ? i : k;
+    const int n = dim == 0 ? d.nx : d.nz;
+    double area;
+    if (pi == 0 || pi == n) {
+        area = 0.0;
+    } else {
+        const int64_t a = (int64_t)i + (int64_t)(d.nx + 1) * k;
+        const int64_t b = dim == 0 ? a + (d.nx + 1) : a + 1;
+        area = 1.0 - fraction(solid2[a], solid2[b]);
     }
     if (area != 0.0 && area < eps_solid) area = eps_solid;
     return area;

@@ -759,7 +759,7 @@ int export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy, fl
     double *tmp = nullptr, *th[3] = {nullptr, nullptr, nullptr};
     HIPC(c, dalloc(tmp, (size_t)ne));
     for (auto &t : th) HIPC(c, dalloc(t, (size_t)ne));
-    launch_scale_coeff(st, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p, th[0], th[1], th[2],
+    launch_scale_coeff(st, c->dims.bx, c->n_active, c->nbr.p, c->sig64.p, c->cx64.p, c->cy64.p, c->cz64.p, th[0], th[1], th[2],
                        nullptr, nullptr, nullptr);
     if (hx) dense(th[0], hx);
     if (hy) dense(th[1], hy);
@@ -919,6 +919,7 @@ int shk_export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy
     if (!c) return SHK_BAD_ARG;
     if (!c->have_system) return c->fail(SHK_BAD_ARG, "no linear system (no projection with unknowns yet)");
     if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_export_scaled takes device pointers");
+    if (c->dims.bx != 8) return c->fail(SHK_BAD_ARG, "shk_export_scaled needs a 3D context");
     HIPC(c, hipSetDevice(c->device));
     return export_scaled(c, rhat, mask, hx, hy, hz, xhat);
 }

@@ -193,3 +193,97 @@ class MacPressureSolver3:
         ms = ctypes.c_double(0.0)
         _lib.check(self.lib.shk_kernel_stats(self.ctx, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)), self.ctx)
         return int(n.value), float(ms.value)
+
+
+class MacPressureSolver2(MacPressureSolver3):
+    """macpressuresolver2 on the GPU (src/projection/macpressuresolver2.cpp:40-300):
+    ``project(dt, (u, v), solid, (us, vs), fluid, surface_tension)`` on an nx x ny grid.
+
+    Arrays are (ny, nx) cells, (ny, nx+1) u faces, (ny+1, nx) v faces and (ny+1, nx+1)
+    solid nodes.  The library holds the grid as nx x 1 x ny, so exported systems come
+    back as (ny, nx) arrays whose `cz` is the 2D y coupling (and `cy` is zero)."""
+
+    def __init__(self, shape, dx: float, device: int = 0, config: dict | None = None):
+        self.lib = _lib.load()
+        nx, ny = shape
+        self.shape2 = (int(nx), int(ny))
+        self.shape = (int(nx), 1, int(ny))
+        self.dx = float(dx)
+        self.device = int(device)
+        ctx = ctypes.c_void_p()
+        _lib.check(self.lib.shk_create2d(self.shape2[0], self.shape2[1], self.dx, self.device, ctypes.byref(ctx)))
+        self.ctx = ctx
+        for k, v in (config or {}).items():
+            self.configure(k, v)
+
+    def face_count2(self, dim: int) -> int:
+        nx, ny = self.shape2
+        return (nx + (dim == 0)) * (ny + (dim == 1))
+
+    @property
+    def nodes(self) -> int:
+        nx, ny = self.shape2
+        return (nx + 1) * (ny + 1)
+
+    def set_solid_density(self, density) -> None:
+        if density is None:
+            _lib.check(self.lib.shk_set_solid_density(self.ctx, None, None, None), self.ctx)
+            self._density = None
+            return
+        du, dv = density
+        _lib.check(self.lib.shk_set_solid_density(self.ctx, _ptr(du, self.face_count2(0), "du"),
+                                                  _ptr(dv, self.face_count2(1), "dv"), None), self.ctx)
+        self._density = (du, dv)
+
+    def project(self, dt: float, velocity, solid, solid_velocity, fluid, surface_tension: float = 0.0) -> dict:
+        """Projects `velocity` (u, v) in place; returns the solver result as a dict."""
+        u, v = velocity
+        us, vs = solid_velocity
+        ptrs = [_ptr(u, self.face_count2(0), "u"), _ptr(v, self.face_count2(1), "v"), _ptr(solid, self.nodes, "solid"),
+                _ptr(us, self.face_count2(0), "us"), _ptr(vs, self.face_count2(1), "vs"),
+                _ptr(fluid, self.cells, "fluid")]
+        torch.cuda.synchronize(self.device)
+        res = _lib.ShkResult()
+        rc = self.lib.shk_project2d(self.ctx, float(dt), *ptrs, float(surface_tension), ctypes.byref(res))
+        _lib.check(rc, self.ctx, _lib.SOLVER_OUTCOMES)
+        out = res.as_dict()
+        out["status_name"] = _lib.STATUS_NAMES.get(res.status, str(res.status))
+        return out
+
+    def get_pressure(self) -> torch.Tensor:
+        """Copy of the last pressure field, shape (ny, nx)."""
+        nx, ny = self.shape2
+        return super().get_pressure().reshape(ny, nx)
+
+    def export_fractions(self) -> dict:
+        nx, ny = self.shape2
+        dev = f"cuda:{self.device}"
+        shp = [(ny, nx + 1), (ny + 1, nx)]
+        out = {f"area{a}": torch.empty(shp[a], dtype=torch.float32, device=dev) for a in range(2)}
+        out.update({f"rho{a}": torch.empty(shp[a], dtype=torch.float32, device=dev) for a in range(2)})
+        torch.cuda.synchronize(self.device)
+        p = lambda k: ctypes.c_void_p(out[k].data_ptr())  # noqa: E731
+        _lib.check(self.lib.shk_export_fractions(self.ctx, p("area0"), p("area1"), None, p("rho0"), p("rho1"), None),
+                   self.ctx)
+        return out
+
+    def export_system(self) -> dict:
+        nx, ny = self.shape2
+        return {k: v.reshape(ny, nx) for k, v in super().export_system().items()}
+
+    def divergence(self, dt: float, velocity, solid_velocity, fp64_update: bool) -> float:
+        u, v = velocity
+        us, vs = solid_velocity
+        ptrs = [_ptr(u, self.face_count2(0), "u"), _ptr(v, self.face_count2(1), "v"), None,
+                _ptr(us, self.face_count2(0), "us"), _ptr(vs, self.face_count2(1), "vs"), None]
+        out = ctypes.c_double(0.0)
+        torch.cuda.synchronize(self.device)
+        _lib.check(self.lib.shk_divergence(self.ctx, float(dt), *ptrs, int(bool(fp64_update)), ctypes.byref(out)),
+                   self.ctx)
+        return float(out.value)
+
+    def export_scaled(self) -> dict:
+        raise NotImplementedError("export_scaled is defined for 3D contexts")
+
+    def mg_apply(self, r):
+        raise NotImplementedError("the multigrid preconditioner is defined for 3D contexts")
