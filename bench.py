@@ -254,6 +254,44 @@ def main():
                                                                / torch.linalg.norm(truth_x[1]))
                 extra[name] = e
                 pm.close()
+        if args.extra:
+            # the 2D projection (macpressuresolver2) on a 2048^2 pool+disc scene: mixed
+            # Jacobi PCG and plain FP64 CG, with the 2D stencil kernel's roofline
+            from shiokaze_amd import scene as S2
+            sc2 = S2.generate2d(S2.POOL_SPHERE, 2048, 0)
+            d2 = {k: torch.from_numpy(np.ascontiguousarray(sc2[k])).cuda() for k in
+                  ("fluid", "solid", "u", "v", "us", "vs")}
+            v20 = [d2["u"].clone(), d2["v"].clone()]
+            e2 = {"grid": "2048x2048 pool+disc"}
+            for name, cfg2 in (("mixed_jacobi", {"Precision": "mixed", "Preconditioner": "jacobi"}),
+                               ("fp64_plain_cg", {"Precision": "fp64", "Preconditioner": "none",
+                                                  "CheckInterval": 64})):
+                p2 = shiokaze_amd.MacPressureSolver2((2048, 2048), sc2["dx"],
+                                                     config=dict(cfg2, Residual=args.residual))
+
+                def step2():
+                    d2["u"].copy_(v20[0])
+                    d2["v"].copy_(v20[1])
+                    return p2.project(1.0, (d2["u"], d2["v"]), d2["solid"], (d2["us"], d2["vs"]), d2["fluid"])
+                step2()
+                t1 = time.perf_counter()
+                r2 = step2()
+                ent = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": r2["count"],
+                       "unknowns": r2["unknowns"], "true_relres_inf": r2["true_relres_inf"]}
+                p2.configure("ProfileKernels", 1)
+                step2()
+                cnt, ms = p2.kernel_stats("spmv_dot")
+                if cnt:
+                    prec2 = cfg2["Precision"]
+                    bpl = kernel_bytes_per_cell("spmv_dot", prec2, cfg2["Preconditioner"]) * r2["active_bricks"] * 512
+                    gbs = bpl / (ms / cnt * 1e-3) / 1e9
+                    ent["spmv_dot"] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
+                                       "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+                e2[name] = ent
+                p2.close()
+            e2["mixed_jacobi_vs_fp64_plain_cg"] = round(e2["mixed_jacobi"]["solves_per_s"]
+                                                        / e2["fp64_plain_cg"]["solves_per_s"], 2)
+            extra["projection_2d"] = e2
         if args.extra and Path(args.unet_weights).exists():
             # learned preconditioner: flexible CG with the trained network (plain PCG fallback)
             from shiokaze_amd.learned import flops_per_apply

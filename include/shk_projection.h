@@ -279,6 +279,12 @@ int shk_unet_residual(int nx, int ny, int nz, const float *cx, const float *cy, 
  * x^ = D^1/2 x (the scaled solution).  Any pointer may be NULL. */
 int shk_export_scaled(shk_ctx *ctx, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat);
 
+/* One application of the learned preconditioner of the last projection with
+ * Preconditioner=unet (features, network, prolongation and smoothing on the bricks,
+ * as inside its flexible CG): z = M r on dense nx*ny*nz float device arrays of the
+ * Jacobi-scaled unknowns.  No counterpart in the reference; for tests and tools. */
+int shk_unet_precondition(shk_ctx *ctx, const float *r, float *z);
+
 /* Layer-level access to the network's convolution kernels (device pointers):
  * x and y are channels-last BF16 grids, w is W[tap][ci][co] FP32 with
  * tap = (dz+1)*9 + (dy+1)*3 + (dx+1); cin a multiple of 8, cout of 16.

@@ -99,6 +99,7 @@ _SIGNATURES = {
     "shk_export_scaled": (ctypes.c_int, [_vp] * 7),
     "shk_unet_residual": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp] * 6),
     "shk_mg_apply": (ctypes.c_int, [_vp, _vp, _vp]),
+    "shk_unet_precondition": (ctypes.c_int, [_vp, _vp, _vp]),
     "shk_nn_conv3d": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                           _vp]),
     "shk_nn_conv3d_backward": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, ctypes.c_int] + [_vp] * 4),

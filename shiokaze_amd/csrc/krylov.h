@@ -97,6 +97,19 @@ void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, c
 void launch_residual_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *b, const double *x, double *r,
                          double *part, double *out);
 
+// Flexible CG (FCG(1)) on FP32 brick vectors of a unit-diagonal brick operator with
+// device-side scalars (fst: 3 doubles; part1 2 x blocks, part2 2 x blocks):
+//   spmv: q = A p, partials p.q and p.r;
+//   axpy: alpha = p.r / p.q; x += alpha p; r -= alpha q; partials r.r and max|r|
+//         (norm_only: the partials of r alone);
+//   dir : beta = -(z.q) / (p.q) from nz partials of z.q; p = z + beta p (first: p = z).
+void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
+                     float *q, double *part);
+void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
+                     const double *part1, double *part2, double *fst, bool norm_only);
+void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
+                    const double *fst, bool first);
+
 // x64 += sig * (double) d32 (sig may be nullptr): the refinement correction.
 void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d, const double *sig);
 // x *= sig: back from the scaled unknowns.

@@ -427,14 +427,15 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
 
 // ---------------------------------------------------------------- true residual
 
-template <class T, class C, int BX>
+// (A v)[e] for the brick operator; UNIT: unit diagonal (diag unused)
+template <class T, class C, int BX, bool UNIT = false>
 __device__ inline T brick_apply(const int *__restrict__ nb6, int64_t e, int l, const C *__restrict__ diag,
                                 const C *__restrict__ cx, const C *__restrict__ cy, const C *__restrict__ cz,
                                 const T *__restrict__ v) {
     using B = BrickShape<BX>;
     constexpr int X = B::X, Y = B::Y;
     const int i = l % X, j = (l / X) % Y, k = l / kWave;
-    T acc = (T)diag[e] * v[e];
+    T acc = UNIT ? v[e] : (T)diag[e] * v[e];
     if (i > 0) {
         acc -= (T)cx[e] * v[e - 1];
     } else {
@@ -603,6 +604,98 @@ __global__ __launch_bounds__(NT) void k_csr_scale(CsrOp op, const double *__rest
 
 }  // namespace
 
+// ---------------------------------------------------------------- flexible CG on bricks
+
+// FCG(1) (Notay 2000) on FP32 brick vectors of a unit-diagonal operator, the inner
+// solve of the learned preconditioner.  Scalars never leave the device: each kernel
+// folds the previous one's FP64 partials in one fixed order in every block.
+// fst[0] = delta = p.q, fst[1] = alpha, fst[2] = 1 if delta <= 0 or not finite.
+
+// q = A p; part[b] = p.q, part[gridDim + b] = p.r over the block's bricks.
+__global__ __launch_bounds__(NT) void k_fcg_spmv(BrickOp<float> op, int per, const float *__restrict__ p,
+                                                 const float *__restrict__ r, float *__restrict__ q,
+                                                 double *__restrict__ part) {
+    __shared__ double lds[NT / kWave];
+    int t0, t1;
+    chunk_range(op.n_active, per, t0, t1);
+    const int l = threadIdx.x;
+    double pq = 0.0, pr = 0.0;
+    for (int slot = t0; slot < t1; ++slot) {
+        const int64_t e = (int64_t)slot * NT + l;
+        const float qv = brick_apply<float, float, 8, true>(op.nbr + 6 * slot, e, l, nullptr, op.cx, op.cy, op.cz, p);
+        q[e] = qv;
+        pq += (double)p[e] * (double)qv;
+        pr += (double)p[e] * (double)r[e];
+    }
+    const double a = block_sum<NT>(pq, lds);
+    const double b = block_sum<NT>(pr, lds);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[gridDim.x + blockIdx.x] = b;
+    }
+}
+
+// alpha = (p.r) / (p.q) from spmv's partials; x += alpha p, r -= alpha q;
+// part2[b] = r.r, part2[gridDim + b] = max|r|.  norm_only: no update, just the norms.
+__global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, float *__restrict__ x,
+                                                 float *__restrict__ r, const float *__restrict__ p,
+                                                 const float *__restrict__ q, const double *__restrict__ part1,
+                                                 double *__restrict__ part2, double *__restrict__ fst, int norm_only) {
+    __shared__ double lds[NB / kWave];
+    __shared__ double bc;
+    float a = 0.0f;
+    if (!norm_only) {
+        const double delta = fold_sum<NB>(part1, gridDim.x, lds, &bc);
+        const double pr = fold_sum<NB>(part1 + gridDim.x, gridDim.x, lds, &bc);
+        const double alpha = pr / delta;
+        const bool bad = !(delta > 0.0) || !(alpha == alpha) || isinf(alpha);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            fst[0] = delta;
+            fst[1] = bad ? 0.0 : alpha;
+            fst[2] = bad ? 1.0 : 0.0;
+        }
+        a = bad ? 0.0f : (float)alpha;
+    }
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
+    double rr = 0.0, rm = 0.0;
+    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
+        float rv = r[e];
+        if (!norm_only) {
+            x[e] = x[e] + a * p[e];
+            rv = rv - a * q[e];
+            r[e] = rv;
+        }
+        rr += (double)rv * (double)rv;
+        rm = fmax(rm, fabs((double)rv));
+    }
+    const double bs = block_sum<NB>(rr, lds);
+    const double bm = block_max<NB>(rm, lds);
+    if (threadIdx.x == 0) {
+        part2[blockIdx.x] = bs;
+        part2[gridDim.x + blockIdx.x] = bm;
+    }
+}
+
+// beta = -(z.q) / delta from the preconditioner's z.q partials (nz of them);
+// p = z + beta p (first: p = z).
+__global__ __launch_bounds__(NB) void k_fcg_dir(int64_t n, int tiles, int per, const float *__restrict__ z,
+                                                float *__restrict__ p, const double *__restrict__ zq_part, int nz,
+                                                const double *__restrict__ fst, int first) {
+    __shared__ double lds[NB / kWave];
+    __shared__ double bc;
+    float b = 0.0f;
+    if (!first) {
+        const double zq = fold_sum<NB>(zq_part, nz, lds, &bc);
+        b = (float)(-zq / fst[0]);
+    }
+    int t0, t1;
+    chunk_range(tiles, per, t0, t1);
+    const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
+    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) p[e] = first ? z[e] : z[e] + b * p[e];
+}
+
 // ---------------------------------------------------------------- launchers
 
 Grid make_grid(int64_t n, int max_blocks) {
@@ -701,6 +794,23 @@ void launch_csr_diag(hipStream_t st, const CsrOp &op, double *diag, double *sig)
 
 void launch_csr_scale(hipStream_t st, const CsrOp &op, const double *sig, double *out) {
     hipLaunchKernelGGL(k_csr_scale, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, sig, out);
+}
+
+void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
+                     float *q, double *part) {
+    hipLaunchKernelGGL(k_fcg_spmv, dim3(g.blocks), dim3(NT), 0, st, op, g.per, p, r, q, part);
+}
+
+void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
+                     const double *part1, double *part2, double *fst, bool norm_only) {
+    hipLaunchKernelGGL(k_fcg_axpy, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, x, r, p, q, part1, part2,
+                       fst, norm_only ? 1 : 0);
+}
+
+void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
+                    const double *fst, bool first) {
+    hipLaunchKernelGGL(k_fcg_dir, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, z, p, zq_part, nz, fst,
+                       first ? 1 : 0);
 }
 
 template void launch_cg_init<double, double>(hipStream_t, const Grid &, const double *, const double *, double *,

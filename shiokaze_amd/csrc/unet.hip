@@ -544,18 +544,33 @@ int unet_net_load(UnetNet *n, const char *path) {
 struct UnetPrecond {
     UnetNet *net = nullptr;
     Grid3 pad{0, 0, 0};
-    float *hx = nullptr, *hy = nullptr, *hz = nullptr, *mask = nullptr;
-    float *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    int64_t cap = 0;                    // brick elements the vectors below hold
+    float *mask = nullptr, *z = nullptr;  // brick vectors: 1 on unknowns; z = M r
+    double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *fst = nullptr;
+    double *h_out = nullptr;            // pinned: part2 (2 x 1024) and fst (3)
 };
 
 void unet_destroy(UnetPrecond *u) {
     if (!u) return;
     unet_net_destroy(u->net);
-    for (float **p : {&u->hx, &u->hy, &u->hz, &u->mask, &u->b, &u->x, &u->r, &u->z, &u->p, &u->q}) dfree(*p);
+    for (float **p : {&u->mask, &u->z}) dfree(*p);
+    for (double **p : {&u->part1, &u->part2, &u->part3, &u->fst}) dfree(*p);
+    if (u->h_out) (void)hipHostFree(u->h_out);
     delete u;
 }
 
 namespace {
+
+constexpr int kPartCap = 2 * kMaxBlocks;
+
+__global__ void k_brick_mask(int nx, int ny, int nz, const int4 *__restrict__ coord, const uint8_t *__restrict__ label,
+                             float *__restrict__ mask) {
+    const int slot = blockIdx.x, l = threadIdx.x;
+    const int4 c = coord[slot];
+    const int i = c.x * 8 + (l & 7), j = c.y * 8 + ((l >> 3) & 7), k = c.z * 8 + (l >> 6);
+    mask[(int64_t)slot * 512 + l] =
+        (i < nx && j < ny && k < nz && label[((int64_t)k * ny + j) * nx + i]) ? 1.0f : 0.0f;
+}
 
 int ensure_precond(shk_ctx *c) {
     if (!c->unet) {
@@ -574,16 +589,73 @@ int ensure_precond(shk_ctx *c) {
             c->unet = nullptr;
             return c->fail(rc, "UnetWeights: %s", m.c_str());
         }
+        HIPC(c, dalloc(u->part1, (size_t)kPartCap));
+        HIPC(c, dalloc(u->part2, (size_t)kPartCap));
+        HIPC(c, dalloc(u->part3, (size_t)kPartCap));
+        HIPC(c, dalloc(u->fst, 4));
+        HIPC(c, hipHostMalloc((void **)&u->h_out, sizeof(double) * (kPartCap + 4), hipHostMallocDefault));
     }
     UnetPrecond *u = c->unet;
     u->net->stream = c->stream;
-    const Grid3 pad{(c->dims.nx + 15) / 16 * 16, (c->dims.ny + 15) / 16 * 16, (c->dims.nz + 15) / 16 * 16};
-    if (pad.nx != u->pad.nx || pad.ny != u->pad.ny || pad.nz != u->pad.nz) {
-        const size_t n = (size_t)pad.vox();
-        for (float **p : {&u->hx, &u->hy, &u->hz, &u->mask, &u->b, &u->x, &u->r, &u->z, &u->p, &u->q})
-            HIPC(c, dalloc(*p, n));
-        u->pad = pad;
+    u->pad = Grid3{(c->dims.nx + 15) / 16 * 16, (c->dims.ny + 15) / 16 * 16, (c->dims.nz + 15) / 16 * 16};
+    const int64_t n = (int64_t)c->n_active * kBrickCells;
+    if (n > u->cap) {
+        for (float **p : {&u->mask, &u->z}) HIPC(c, dalloc(*p, (size_t)n));
+        u->cap = n;
     }
+    return SHK_OK;
+}
+
+BrickView brick_view(shk_ctx *c) {
+    return BrickView{c->coord.p, c->brick_slot.p, c->nbr.p, c->n_active, c->nbx, c->nby,
+                     c->dims.nx, c->dims.ny, c->dims.nz};
+}
+
+// z = M(r) on the projection's brick vectors: features of r (all channels when
+// `full`, else the residual channel alone), the network on the half grid of the
+// padded grid, prolongation into the bricks and kSmooth damped-Jacobi steps.  With
+// q: the last step also writes the z.q partials (their count into *nz).
+int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz) {
+    UnetPrecond *u = c->unet;
+    UnetNet *n = u->net;
+    int rc;
+    const Grid3 P = u->pad;
+    const Grid3 m{P.nx / 2, P.ny / 2, P.nz / 2};
+    if ((rc = ensure_grid(n, m))) return c->fail(rc, "%s", n->err.c_str());
+    if ((rc = repack(n))) return c->fail(rc, "%s", n->err.c_str());
+    const BrickView bv = brick_view(c);
+    const double s = rms > 0 ? rms : 1.0;
+    launch_features_bricks(c->stream, m, bv, r, 1.0 / s, u->mask, c->hx32.p, c->hy32.p, c->hz32.p, !full, n->feat);
+    forward(n);
+    launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, u->z);
+    float *src = u->z, *dst = c->t32.p;
+    for (int i = 0; i < kSmooth; ++i) {
+        const bool last = i + 1 == kSmooth;
+        const int blocks = launch_smooth_bricks(c->stream, bv, n->omega, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p,
+                                                src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr);
+        if (last && nz) *nz = blocks;
+        std::swap(src, dst);
+    }
+    if (src != u->z)
+        HIPC(c, hipMemcpyAsync(u->z, src, sizeof(float) * (size_t)c->n_active * kBrickCells, hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPC(c, hipGetLastError());
+    return SHK_OK;
+}
+
+// sum and max of the 2 x blocks partials of the last launch_fcg_axpy, and fst
+int read_norms(shk_ctx *c, int blocks, double &rr, double &rmax, double fst[3]) {
+    UnetPrecond *u = c->unet;
+    HIPC(c, hipMemcpyAsync(u->h_out, u->part2, sizeof(double) * 2 * blocks, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipMemcpyAsync(u->h_out + kPartCap, u->fst, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    rr = 0.0;
+    rmax = 0.0;
+    for (int i = 0; i < blocks; ++i) {
+        rr += u->h_out[i];
+        rmax = std::max(rmax, u->h_out[blocks + i]);
+    }
+    for (int i = 0; i < 3; ++i) fst[i] = u->h_out[kPartCap + i];
     return SHK_OK;
 }
 
@@ -591,13 +663,13 @@ int ensure_precond(shk_ctx *c) {
 
 // z = M(r) inside flexible CG; with ProfileKernels=1 the application is timed with
 // events on the context's stream (read back at the next stream synchronisation).
-int timed_precondition(shk_ctx *c, UnetNet *net, Grid3 P, UnetPrecond *u, double omega, double rms) {
+int timed_precondition(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz) {
     if (c->profile && !c->uev[0]) {
         HIPC(c, hipEventCreate(&c->uev[0]));
         HIPC(c, hipEventCreate(&c->uev[1]));
     }
     if (c->profile) HIPC(c, hipEventRecord(c->uev[0], c->stream));
-    const int rc = precondition(net, P, u->r, u->mask, u->hx, u->hy, u->hz, omega, rms, u->z);
+    const int rc = precondition_bricks(c, r, rms, full, q, nz);
     if (rc) return rc;
     if (c->profile) {
         HIPC(c, hipEventRecord(c->uev[1], c->stream));
@@ -610,34 +682,25 @@ int timed_precondition(shk_ctx *c, UnetNet *net, Grid3 P, UnetPrecond *u, double
     return SHK_OK;
 }
 
-// Learned flexible CG (FCG(1), Notay 2000) on the dense padded scaled system in
-// FP32 with FP64 dots, inside the same FP64 refinement loop as the mixed solver.
-// Falls back to plain Jacobi-PCG inner solves when an inner FCG stalls (less than
-// 10% residual reduction in 20 iterations) or hits its cap; the result records the
-// fallback.
+// Learned flexible CG (FCG(1), Notay 2000) on the projection's brick vectors of the
+// Jacobi-scaled system in FP32 with FP64 dots, inside the same FP64 refinement loop
+// as the mixed solver.  The network runs on the half grid of the padded grid; every
+// vector operation, the smoothing and the prolongation work on the bricks, and the
+// scalars stay on the device (one host read per iteration, for the stop test and the
+// network's input scale).  Falls back to plain Jacobi-PCG inner solves when an inner
+// FCG stalls (less than 10% residual reduction in 20 iterations) or hits its cap; the
+// result records the fallback.
 int unet_solve(shk_ctx *c, shk_result *res) {
     int rc;
     if ((rc = ensure_precond(c))) return rc;
     UnetPrecond *u = c->unet;
-    UnetNet *net = u->net;
-    const Grid3 P = u->pad;
-    const int64_t np = P.vox();
     const Dims d = c->dims;
     hipStream_t st = c->stream;
-    const double omega = net->omega;
-    // dense scaled operator and mask
-    auto to_dense = [&](const float *src, float *dst) {
-        hipLaunchKernelGGL((k_brick_to_dense<float, float>), dim3(g1(np)), dim3(256), 0, st, d.nx, d.ny, d.nz, P.nx,
-                           P.ny, P.nz, c->brick_slot.p, c->nbx, c->nby, src, dst);
-    };
-    to_dense(c->hx32.p, u->hx);
-    to_dense(c->hy32.p, u->hy);
-    to_dense(c->hz32.p, u->hz);
-    hipLaunchKernelGGL(k_label_mask, dim3(g1(np)), dim3(256), 0, st, d.nx, d.ny, d.nz, P.nx, P.ny, P.nz, c->label.p,
+    const Grid &g = c->grid;
+    hipLaunchKernelGGL(k_brick_mask, dim3(c->n_active), dim3(512), 0, st, d.nx, d.ny, d.nz, c->coord.p, c->label.p,
                        u->mask);
     HIPC(c, hipGetLastError());
     const double nunk = (double)c->unknowns;
-    const Grid &g = c->grid;
     HIPC(c, hipMemsetAsync(c->x64.p, 0, sizeof(double) * g.n, st));
     double rr[4];
     if ((rc = true_residual(c, c->x64.p, c->r64.p, rr, c->sig64.p, c->rs32.p))) return rc;
@@ -649,40 +712,35 @@ int unet_solve(shk_ctx *c, shk_result *res) {
     int status = SHK_OK;
     bool fell_back = false;
     const int64_t fcg_cap = 400;
+    const BrickOp<float> op{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
+    float *x = c->x32.p, *r = c->r32.p, *p = c->s32.p, *q = c->q32.p;
     while (r_inf > tol_abs && outer < std::max(1, c->prm.outer_max) + (fell_back ? 2 : 0)) {
         double inner_rel = std::max(c->prm.inner_residual, 0.5 * tol_abs / r_inf);
         if (inner_rel > 0.5) inner_rel = 0.5;
         ++outer;
         if (!fell_back) {
-            // inner FCG on A^ d = r^ (dense, padded)
-            hipLaunchKernelGGL((k_brick_to_dense<float, float>), dim3(g1(np)), dim3(256), 0, st, d.nx, d.ny, d.nz, P.nx,
-                               P.ny, P.nz, c->brick_slot.p, c->nbx, c->nby, c->rs32.p, u->r);
-            HIPC(c, hipMemsetAsync(u->x, 0, sizeof(float) * np, st));
-            double v3[3];
-            if ((rc = reduce3(net, np, u->r, u->r, nullptr, nullptr, u->r, v3))) return c->fail(rc, "%s", net->err.c_str());
-            const double r0 = v3[2];
+            // inner FCG on A^ d = r^ (bricks)
+            HIPC(c, hipMemcpyAsync(r, c->rs32.p, sizeof(float) * g.n, hipMemcpyDeviceToDevice, st));
+            HIPC(c, hipMemsetAsync(x, 0, sizeof(float) * g.n, st));
+            double fst[3], r2, r0;
+            launch_fcg_axpy(st, g, x, r, p, q, nullptr, u->part2, u->fst, true);
+            if ((rc = read_norms(c, g.blocks, r2, r0, fst))) return rc;
             double rn = r0;
-            if ((rc = timed_precondition(c, net, P, u, omega, std::sqrt(v3[0] / nunk))))
-                return c->fail(rc, "%s", net->err.c_str());
-            HIPC(c, hipMemcpyAsync(u->p, u->z, sizeof(float) * np, hipMemcpyDeviceToDevice, st));
+            if ((rc = timed_precondition(c, r, std::sqrt(r2 / nunk), true, nullptr, nullptr))) return rc;
+            launch_fcg_dir(st, g, u->z, p, nullptr, 0, u->fst, true);
             int64_t it = 0;
             double best = rn;
             int64_t best_it = 0;
             bool stalled = false;
             while (rn > inner_rel * r0 && it < fcg_cap && total + it < c->prm.max_iterations) {
-                launch_dense_stencil(st, P, u->hx, u->hy, u->hz, u->p, u->q);
-                if ((rc = reduce3(net, np, u->p, u->q, u->p, u->r, nullptr, v3))) return c->fail(rc, "%s", net->err.c_str());
-                const double delta = v3[0], pr = v3[1];
-                if (!(delta > 0.0)) {
+                launch_fcg_spmv(st, g, op, p, r, q, u->part1);
+                launch_fcg_axpy(st, g, x, r, p, q, u->part1, u->part2, u->fst, false);
+                if ((rc = read_norms(c, g.blocks, r2, rn, fst))) return rc;
+                if (fst[2] != 0.0) {  // p.q <= 0: the preconditioner broke the iteration
                     stalled = true;
                     break;
                 }
-                const double alpha = pr / delta;
-                hipLaunchKernelGGL(k_axpy_f32, dim3(g1(np)), dim3(256), 0, st, np, (float)alpha, u->p, u->x);
-                hipLaunchKernelGGL(k_axpy_f32, dim3(g1(np)), dim3(256), 0, st, np, (float)-alpha, u->q, u->r);
                 ++it;
-                if ((rc = reduce3(net, np, u->r, u->r, nullptr, nullptr, u->r, v3))) return c->fail(rc, "%s", net->err.c_str());
-                rn = v3[2];
                 if (!(rn == rn)) {
                     stalled = true;
                     break;
@@ -695,21 +753,16 @@ int unet_solve(shk_ctx *c, shk_result *res) {
                     break;
                 }
                 if (rn <= inner_rel * r0) break;
-                if ((rc = timed_precondition(c, net, P, u, omega, std::sqrt(v3[0] / nunk))))
-                    return c->fail(rc, "%s", net->err.c_str());
-                if ((rc = reduce3(net, np, u->z, u->q, nullptr, nullptr, nullptr, v3))) return c->fail(rc, "%s", net->err.c_str());
-                const double beta = -v3[0] / delta;
-                hipLaunchKernelGGL(k_xpby_f32, dim3(g1(np)), dim3(256), 0, st, np, u->z, (float)beta, u->p);
+                int nz = 0;
+                if ((rc = timed_precondition(c, r, std::sqrt(r2 / nunk), false, q, &nz))) return rc;
+                launch_fcg_dir(st, g, u->z, p, u->part3, nz, u->fst, false);
             }
             total += it;
             if (stalled || it >= fcg_cap) fell_back = true;
-            // correction back to bricks and into x64 (x += sig * d)
-            hipLaunchKernelGGL(k_dense_to_brick, dim3(g1(g.n)), dim3(256), 0, st, d.nx, d.ny, d.nz, P.nx, P.ny,
-                               c->coord.p, c->n_active, u->x, c->x32.p);
-            launch_add_correction(st, g.n, c->x64.p, c->x32.p, c->sig64.p);
+            // correction into x64 (x += sig * d)
+            launch_add_correction(st, g.n, c->x64.p, x, c->sig64.p);
         } else {
             // plain Jacobi-PCG inner solve (the fallback), FP32 as in the mixed path
-            const BrickOp<float> op{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p};
             const Reduction red = reduction(c);
             launch_cg_init<float, float>(st, g, c->rs32.p, nullptr, c->x32.p, c->r32.p, c->s32.p, red, inner_rel,
                                          c->prm.max_iterations - total);
@@ -913,6 +966,34 @@ int shk_unet_residual(int nx, int ny, int nz, const float *cx, const float *cy, 
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipDeviceSynchronize();
     return (e1 == hipSuccess && e2 == hipSuccess) ? SHK_OK : SHK_HIP_ERROR;
+}
+
+int shk_unet_precondition(shk_ctx *c, const float *r, float *z) {
+    if (!c || !r || !z) return SHK_BAD_ARG;
+    if (!c->have_system || !c->unet || c->prm.precond != 2)
+        return c->fail(SHK_BAD_ARG, "no learned preconditioner (project with Preconditioner=unet first)");
+    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_unet_precondition takes device pointers");
+    HIPC(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_precond(c))) return rc;
+    const Dims d = c->dims;
+    const Grid &g = c->grid;
+    hipStream_t st = c->stream;
+    const int64_t nd = (int64_t)d.nx * d.ny * d.nz;
+    hipLaunchKernelGGL(k_brick_mask, dim3(c->n_active), dim3(512), 0, st, d.nx, d.ny, d.nz, c->coord.p, c->label.p,
+                       c->unet->mask);
+    hipLaunchKernelGGL(k_dense_to_brick, dim3(g1(g.n)), dim3(256), 0, st, d.nx, d.ny, d.nz, d.nx, d.ny, c->coord.p,
+                       c->n_active, r, c->r32.p);
+    double rr, rmax, fst[3];
+    launch_fcg_axpy(st, g, nullptr, c->r32.p, nullptr, nullptr, nullptr, c->unet->part2, c->unet->fst, true);
+    if ((rc = read_norms(c, g.blocks, rr, rmax, fst))) return rc;
+    const double rms = c->unknowns > 0 ? std::sqrt(rr / (double)c->unknowns) : 1.0;
+    if ((rc = precondition_bricks(c, c->r32.p, rms, true, nullptr, nullptr))) return rc;
+    hipLaunchKernelGGL((k_brick_to_dense<float, float>), dim3(g1(nd)), dim3(256), 0, st, d.nx, d.ny, d.nz, d.nx, d.ny,
+                       d.nz, c->brick_slot.p, c->nbx, c->nby, c->unet->z, z);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(st));
+    return SHK_OK;
 }
 
 int shk_export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat) {

@@ -76,6 +76,31 @@ void launch_features(hipStream_t st, Grid3 coarse, const float *r, double inv_sc
 // z = mask * scale * P u on the fine grid, P = cell-centred trilinear interpolation
 // (weights 3/4, 1/4 per axis, clamped at the box edges).
 void launch_prolong(hipStream_t st, Grid3 fine, const float *u, double scale, const float *mask, float *z);
+// The same three steps on the projection's 8x8x8 brick vectors (brick coordinates
+// coord[slot], slot table brick_slot over nbx x nby bricks, fine grid nx x ny x nz
+// inside the padded grid of the network): every value is the one the dense kernels
+// give on the padded grid, whose cells outside the bricks are zero.
+struct BrickView {
+    const int4 *coord;
+    const int *slot;
+    const int *nbr;
+    int n_active, nbx, nby;
+    int nx, ny, nz;
+};
+// only_r: write channel 0 (R(r)/scale) and keep the other channels
+void launch_features_bricks(hipStream_t st, Grid3 coarse, BrickView b, const float *r, double inv_scale,
+                            const float *mask, const float *cx, const float *cy, const float *cz, bool only_r,
+                            bf16 *feat);
+void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float *u, double scale, const float *mask,
+                           float *z);
+// zout = zin + omega * mask * (r - A zin), A the unit-diagonal operator with low-face
+// couplings cx, cy, cz, evaluated in the order and rounding of the dense stencil and
+// smoothing kernels.  With q and part: part[block] = sum zout * q (FP64), at most
+// 1024 blocks; returns the number of blocks.
+int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
+                         const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
+                         const float *q, double *part);
+
 // grad_u = scale * P^T (mask * g).
 void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale, float *grad_u);
 // z += omega * mask * (r - t): one weighted-Jacobi step given t = A z.

@@ -454,6 +454,116 @@ __global__ void k_dense_stencil(Grid3 g, const float *__restrict__ cx, const flo
     }
 }
 
+// ---------------------------------------------------------------- brick-layout preconditioner steps
+
+// Brick element of fine cell (i, j, k), -1 outside the bricks or the fine grid.
+__device__ inline int64_t brick_of(const BrickView &b, int i, int j, int k) {
+    if (i >= b.nx || j >= b.ny || k >= b.nz) return -1;
+    const int s = b.slot[(i >> 3) + (int64_t)b.nbx * ((j >> 3) + (int64_t)b.nby * (k >> 3))];
+    if (s < 0) return -1;
+    return (int64_t)s * 512 + (i & 7) + 8 * (j & 7) + 64 * (k & 7);
+}
+
+__global__ void k_features_bricks(Grid3 c, BrickView b, const float *__restrict__ r, double inv_scale,
+                                  const float *__restrict__ mask, const float *__restrict__ cx,
+                                  const float *__restrict__ cy, const float *__restrict__ cz, int only_r,
+                                  uint16_t *__restrict__ feat) {
+    const int64_t nv = c.vox();
+    for (int64_t V = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; V < nv; V += (int64_t)gridDim.x * blockDim.x) {
+        const int X = (int)(V % c.nx), Y = (int)((V / c.nx) % c.ny), Z = (int)(V / ((int64_t)c.nx * c.ny));
+        // the 8 children lie in one brick (2x2x2 aligned in 8x8x8)
+        const int64_t e0 = brick_of(b, 2 * X, 2 * Y, 2 * Z);
+        float s[5] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (e0 >= 0) {
+            // child a = (dx, dy, dz) bits in the dense kernel's order; +0.0f for a cell
+            // outside the fine grid, as the dense padded grid holds there
+            for (int a = 0; a < 8; ++a) {
+                const int i = 2 * X + (a & 1), j = 2 * Y + ((a >> 1) & 1), k = 2 * Z + (a >> 2);
+                const bool in = i < b.nx && j < b.ny && k < b.nz;
+                const int64_t e = e0 + (a & 1) + 8 * ((a >> 1) & 1) + 64 * (a >> 2);
+                s[0] += in ? r[e] : 0.0f;
+                if (!only_r) {
+                    s[1] += in ? mask[e] : 0.0f;
+                    s[2] += in ? cx[e] : 0.0f;
+                    s[3] += in ? cy[e] : 0.0f;
+                    s[4] += in ? cz[e] : 0.0f;
+                }
+            }
+        }
+        const uint16_t f0 = bf16_bits((float)(0.125 * s[0] * inv_scale));
+        if (only_r) {
+            feat[V * 8] = f0;
+        } else {
+            uint4 o;
+            o.x = (uint32_t)f0 | ((uint32_t)bf16_bits(0.125f * s[1]) << 16);
+            o.y = (uint32_t)bf16_bits(0.125f * s[2]) | ((uint32_t)bf16_bits(0.125f * s[3]) << 16);
+            o.z = (uint32_t)bf16_bits(0.125f * s[4]);
+            o.w = 0u;
+            *reinterpret_cast<uint4 *>(feat + V * 8) = o;
+        }
+    }
+}
+
+__global__ void k_prolong_bricks(Grid3 f, BrickView b, const float *__restrict__ u, double scale,
+                                 const float *__restrict__ mask, float *__restrict__ z) {
+    const int slot = blockIdx.x, l = threadIdx.x;
+    const int4 bc = b.coord[slot];
+    const int x = bc.x * 8 + (l & 7), y = bc.y * 8 + ((l >> 3) & 7), zc = bc.z * 8 + (l >> 6);
+    const int64_t e = (int64_t)slot * 512 + l;
+    const int cnx = f.nx / 2, cny = f.ny / 2, cnz = f.nz / 2;
+    int X[2], Y[2], Z[2];
+    taps1(x, cnx, X[0], X[1]);
+    taps1(y, cny, Y[0], Y[1]);
+    taps1(zc, cnz, Z[0], Z[1]);
+    const double w[2] = {0.75, 0.25};
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+                acc += w[c] * w[bb] * w[a] * (double)u[((int64_t)Z[c] * cny + Y[bb]) * cnx + X[a]];
+    z[e] = (float)((double)mask[e] * scale * acc);
+}
+
+__global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega, const float *__restrict__ r,
+                                                       const float *__restrict__ mask, const float *__restrict__ cx,
+                                                       const float *__restrict__ cy, const float *__restrict__ cz,
+                                                       const float *__restrict__ zin, float *__restrict__ zout,
+                                                       const float *__restrict__ q, double *__restrict__ part) {
+    __shared__ double lds[8];
+    const int l = threadIdx.x;
+    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    double acc_zq = 0.0;
+    for (int slot = blockIdx.x; slot < b.n_active; slot += gridDim.x) {
+        const int *nb = b.nbr + 6 * slot;
+        const int64_t e = (int64_t)slot * 512 + l;
+        // neighbour elements (-1: no brick there; its cells are zero with zero couplings)
+        const int64_t exm = i > 0 ? e - 1 : (nb[0] >= 0 ? (int64_t)nb[0] * 512 + l + 7 : -1);
+        const int64_t exp = i < 7 ? e + 1 : (nb[1] >= 0 ? (int64_t)nb[1] * 512 + l - 7 : -1);
+        const int64_t eym = j > 0 ? e - 8 : (nb[2] >= 0 ? (int64_t)nb[2] * 512 + l + 56 : -1);
+        const int64_t eyp = j < 7 ? e + 8 : (nb[3] >= 0 ? (int64_t)nb[3] * 512 + l - 56 : -1);
+        const int64_t ezm = k > 0 ? e - 64 : (nb[4] >= 0 ? (int64_t)nb[4] * 512 + l + 448 : -1);
+        const int64_t ezp = k < 7 ? e + 64 : (nb[5] >= 0 ? (int64_t)nb[5] * 512 + l - 448 : -1);
+        // A z in the dense stencil's order: x-, x+, y-, y+, z-, z+
+        float acc = zin[e];
+        if (exm >= 0) acc -= cx[e] * zin[exm];
+        if (exp >= 0) acc -= cx[exp] * zin[exp];
+        if (eym >= 0) acc -= cy[e] * zin[eym];
+        if (eyp >= 0) acc -= cy[eyp] * zin[eyp];
+        if (ezm >= 0) acc -= cz[e] * zin[ezm];
+        if (ezp >= 0) acc -= cz[ezp] * zin[ezp];
+        const float zn = (float)((double)zin[e] + omega * (double)mask[e] * ((double)r[e] - (double)acc));
+        zout[e] = zn;
+        if (part) acc_zq += (double)zn * (double)q[e];
+    }
+    if (part) {
+        const double t = block_sum<512>(acc_zq, lds);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
 __global__ void k_adam(int64_t n, float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
                        float *__restrict__ v, float lr, float b1, float b2, float eps, float bc1, float bc2) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -478,6 +588,28 @@ inline uint16_t *U(bf16 *p) { return reinterpret_cast<uint16_t *>(p); }
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
+
+void launch_features_bricks(hipStream_t st, Grid3 coarse, BrickView b, const float *r, double inv_scale,
+                            const float *mask, const float *cx, const float *cy, const float *cz, bool only_r,
+                            bf16 *feat) {
+    hipLaunchKernelGGL(k_features_bricks, dim3(grid1(coarse.vox(), 256)), dim3(256), 0, st, coarse, b, r, inv_scale,
+                       mask, cx, cy, cz, only_r ? 1 : 0, U(feat));
+}
+
+void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float *u, double scale, const float *mask,
+                           float *z) {
+    if (b.n_active <= 0) return;
+    hipLaunchKernelGGL(k_prolong_bricks, dim3(b.n_active), dim3(512), 0, st, fine, b, u, scale, mask, z);
+}
+
+int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
+                         const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
+                         const float *q, double *part) {
+    const int blocks = b.n_active < 1024 ? (b.n_active > 0 ? b.n_active : 1) : 1024;
+    hipLaunchKernelGGL(k_smooth_bricks, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout, q,
+                       part);
+    return blocks;
+}
 
 void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool transposed, bf16 *packed) {
     const int gin = transposed ? cout : cin;  // GEMM reduction channels

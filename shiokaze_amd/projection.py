@@ -176,6 +176,15 @@ class MacPressureSolver3:
         _lib.check(self.lib.shk_mg_apply(self.ctx, _ptr(r, self.cells, "r"), _ptr(z, self.cells, "z")), self.ctx)
         return z
 
+    def unet_precondition(self, r: torch.Tensor) -> torch.Tensor:
+        """z = M r with the learned preconditioner of the last projection
+        (Preconditioner=unet), on dense (nz, ny, nx) FP32 grids of the scaled unknowns."""
+        z = torch.empty_like(r)
+        torch.cuda.synchronize(self.device)
+        _lib.check(self.lib.shk_unet_precondition(self.ctx, _ptr(r, self.cells, "r"), _ptr(z, self.cells, "z")),
+                   self.ctx)
+        return z
+
     def divergence(self, dt: float, velocity, solid_velocity, fp64_update: bool) -> float:
         u, v, w = velocity
         us, vs, ws = solid_velocity

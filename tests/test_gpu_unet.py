@@ -282,3 +282,38 @@ def test_trained_network_gain(scenes, tmp_path):
     log(f"[gpu] gain: mean iterations jacobi {jac:.1f}, trained {trained:.1f}, untrained {raw:.1f}")
     assert trained <= 0.25 * jac
     assert trained <= 0.5 * raw
+
+
+@pytest.mark.parametrize("kind,n,seed", [(3, 32, 2), (3, 48, 7)])
+def test_brick_preconditioner_matches_dense_apply(scenes, kind, n, seed):
+    """Inside flexible CG the preconditioner runs on the projection's bricks: features
+    from brick vectors, prolongation into the bricks and brick smoothing.  One
+    application (shk_unet_precondition) equals the dense network application
+    (shk_unet_apply) on the exported scaled system: bit for bit off the unknowns (zero)
+    and to the input scale's last-bit rounding on them (the two sum r.r in different
+    orders)."""
+    import torch
+
+    import shiokaze_amd
+    from gpu_helpers import to_device
+    from shiokaze_amd import UNet
+    raw = WEIGHTS.read_bytes()   # magic, layer count, (cin, cout) per layer, nparam, omega, weights
+    nl = int(np.frombuffer(raw[8:12], dtype=np.int32)[0])
+    at = 12 + 8 * nl + 8
+    omega = float(np.frombuffer(raw[at:at + 8], dtype=np.float64)[0])
+    sc = scenes(kind, n, seed)
+    d = to_device(sc)
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=_scene_config(WEIGHTS)) as ps:
+        ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+        s = ps.export_scaled()
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        r = (torch.randn(s["rhat"].shape, device="cuda", generator=g) * s["mask"]).contiguous()
+        zb = ps.unet_precondition(r).cpu().numpy()
+    with UNet(seed=0) as net:
+        net.load(WEIGHTS)
+        zd = net.apply(r, s["mask"], s["hx"], s["hy"], s["hz"], omega=omega).cpu().numpy()
+    lab = s["mask"].cpu().numpy() > 0
+    err = np.abs(zb - zd).max() / np.abs(zd).max()
+    log(f"[gpu] brick vs dense preconditioner kind={kind} n={n}: max rel diff {err:.2e}, omega {omega}")
+    assert np.all(zb[~lab] == 0) and np.all(zd[~lab] == 0)
+    assert err <= 2e-3
