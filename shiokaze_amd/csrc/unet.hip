@@ -287,7 +287,9 @@ void wgrad(UnetNet *n, int l, Grid3 g, const bf16 *x, const bf16 *dz) {
 }
 
 // forward on the features already in n->feat; u = network output (FP32, level 0)
-void forward(UnetNet *n) {
+// train: materialise the decoder's concatenations (the backward pass reads them);
+// otherwise each decoder convolution reads [upsample(coarse), skip] directly.
+void forward(UnetNet *n, bool train) {
     const Grid3 *L = n->L;
     conv(n, 0, L[0], n->feat, n->enc0);
     launch_pool(n->stream, L[1], 16, n->enc0, n->p1);
@@ -296,6 +298,17 @@ void forward(UnetNet *n) {
     conv(n, 2, L[2], n->p2, n->enc2);
     launch_pool(n->stream, L[3], 64, n->enc2, n->p3);
     conv(n, 3, L[3], n->p3, n->bott);
+    if (!train) {
+        auto cat = [&](int l, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, bf16 *y) {
+            launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
+                                kSpec[l].cout, kSpec[l].relu, y);
+        };
+        cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
+        cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);
+        cat(6, L[0], n->dec1, 32, n->enc0, n->dec0);
+        conv(n, 7, L[0], n->dec0, nullptr, n->u);
+        return;
+    }
     launch_upsample(n->stream, L[3], 128, n->bott, 192, n->cat2);
     launch_copy_channels(n->stream, L[2].vox(), 64, n->enc2, 64, 0, n->cat2, 192, 128, false);
     conv(n, 4, L[2], n->cat2, n->dec2);
@@ -352,7 +365,7 @@ void backward(UnetNet *n) {
 
 // z = M(r) on the fine grid (fine dims multiples of 16); rms = RMS of r over the unknowns
 int precondition(UnetNet *n, Grid3 fine, const float *r, const float *mask, const float *cx, const float *cy,
-                 const float *cz, double omega, double rms, float *z) {
+                 const float *cz, double omega, double rms, float *z, bool train = false) {
     int rc;
     const Grid3 m{fine.nx / 2, fine.ny / 2, fine.nz / 2};
     if ((rc = ensure_grid(n, m))) return rc;
@@ -360,7 +373,7 @@ int precondition(UnetNet *n, Grid3 fine, const float *r, const float *mask, cons
     if ((rc = repack(n))) return rc;
     const double s = rms > 0 ? rms : 1.0;
     launch_features(n->stream, m, r, 1.0 / s, mask, cx, cy, cz, n->feat);
-    forward(n);
+    forward(n, train);
     launch_prolong(n->stream, fine, n->u, s, mask, z);
     for (int i = 0; i < kSmooth; ++i) {
         launch_dense_stencil(n->stream, fine, cx, cy, cz, z, n->ftmp);
@@ -458,7 +471,7 @@ int unet_net_train(UnetNet *n, Grid3 fine, const float *r, const float *mask, co
     const double rms = count[0] > 0 ? std::sqrt(d3[0] / count[0]) : 1.0;
     const double den = d3[1];
     if (!(den > 0.0)) return fail(n, SHK_BAD_ARG, "training sample has x*'b <= 0");
-    if ((rc = precondition(n, fine, r, mask, cx, cy, cz, omega, rms, n->fz))) return rc;
+    if ((rc = precondition(n, fine, r, mask, cx, cy, cz, omega, rms, n->fz, true))) return rc;
     // e = z - x*, Ae, num = e'Ae
     hipLaunchKernelGGL(k_sub_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, n->fz, xstar, n->fe);
     launch_dense_stencil(n->stream, fine, cx, cy, cz, n->fe, n->fae);
@@ -626,7 +639,7 @@ int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const
     const BrickView bv = brick_view(c);
     const double s = rms > 0 ? rms : 1.0;
     launch_features_bricks(c->stream, m, bv, r, 1.0 / s, u->mask, c->hx32.p, c->hy32.p, c->hz32.p, !full, n->feat);
-    forward(n);
+    forward(n, false);
     launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, u->z);
     float *src = u->z, *dst = c->t32.p;
     for (int i = 0; i < kSmooth; ++i) {

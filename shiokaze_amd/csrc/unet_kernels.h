@@ -46,6 +46,12 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
                      bool relu, bf16 *y, float *out_f32);
 
+// The same convolution of the concatenation [nearest upsampling of coarse (c1
+// channels, grid g/2), skip (cin - c1 channels, grid g)], read from the two tensors
+// directly; c1 and cin - c1 are multiples of 8.
+void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y);
+
 // dw[tap][ci][co] += sum_v x[v + tap][ci] * dz[v][co]; db[co] += sum_v dz[v][co]
 // (FP32 atomics, split over voxel chunks).
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,

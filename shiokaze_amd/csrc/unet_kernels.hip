@@ -84,10 +84,14 @@ constexpr int kTX = 16, kTY = 4, kTZ = 4;
 constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 
+// With xc: the input is the concatenation [upsample(xc), x] of the nearest upsampling
+// of the half-size grid xc (its first c1 channels) and x (the other cin - c1), read
+// straight from the two tensors while staging (no concatenated buffer).
 template <int MTB, int CC>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total,
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
-                                                  int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32) {
+                                                  int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32,
+                                                  const uint16_t *__restrict__ xc, int c1) {
     constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
     // voxel stride in uint4, odd so that 16 lanes reading 16 consecutive voxels hit 16
     // disjoint 4-bank groups of the 64-bank LDS
@@ -113,9 +117,20 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
             const int hx = vox % kHX, hy = (vox / kHX) % kHY, hz = vox / (kHX * kHY);
             const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
             uint4 v = make_uint4(0u, 0u, 0u, 0u);
-            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
-                v = *reinterpret_cast<const uint4 *>(x + (((int64_t)gz * g.ny + gy) * g.nx + gx) * cin + stage * CC +
-                                                     part * 8);
+            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+                const int ch = stage * CC + part * 8;
+                const int64_t vi = ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                const uint16_t *src;
+                if (!xc) {
+                    src = x + vi * cin + ch;
+                } else if (ch < c1) {
+                    const int64_t pv = ((int64_t)(gz >> 1) * (g.ny >> 1) + (gy >> 1)) * (g.nx >> 1) + (gx >> 1);
+                    src = xc + pv * c1 + ch;
+                } else {
+                    src = x + vi * (cin - c1) + (ch - c1);
+                }
+                v = *reinterpret_cast<const uint4 *>(src);
+            }
             lds[vox * VS + part] = v;
         }
         __syncthreads();
@@ -623,30 +638,31 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
 namespace {
 template <int MTB>
 void conv_fwd_mt(hipStream_t st, dim3 grid, Grid3 g, const bf16 *x, int cin, int mt, const uint4 *pk,
-                 const float *bias, bool relu, uint16_t *yy, float *out_f32) {
+                 const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1) {
     switch (conv_cc(cin)) {
         case 8:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 8>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32);
+                               out_f32, xc, c1);
             break;
         case 16:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 16>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32);
+                               out_f32, xc, c1);
             break;
         case 24:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 24>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32);
+                               out_f32, xc, c1);
             break;
         default:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 32>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32);
+                               out_f32, xc, c1);
             break;
     }
 }
 }  // namespace
 
-void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32) {
+namespace {
+void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
+                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1) {
     const int mt = cout / 16;
     const int bx = (g.nx + kTX - 1) / kTX, by = (g.ny + kTY - 1) / kTY, bz = (g.nz + kTZ - 1) / kTZ;
     // output tiles per block: all of them, unless the grid is too small to fill the
@@ -662,15 +678,26 @@ void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     uint16_t *yy = y ? U(y) : nullptr;
     switch (mtb) {
-        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
-        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32); break;
+        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
+        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
         default: break;  // unsupported widths are rejected by the caller
     }
+}
+}  // namespace
+
+void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
+                     bool relu, bf16 *y, float *out_f32) {
+    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0);
+}
+
+void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y) {
+    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1);
 }
 
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,
