@@ -41,7 +41,7 @@ def log(msg):
 # WRITE_SIZE x 1.0, factors from a 256 MiB copy in the same run) on the pool+sphere scene at
 # 256^3, mixed precision, Jacobi: profiles/hbm_traffic_pool_sphere_256_mixed_jacobi.md.
 # Reported as roofline.traffic only for that workload.
-MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 26.08, "axpy_norm": 24.03}
+MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 26.11, "axpy_norm": 24.03}
 
 
 def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
