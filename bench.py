@@ -81,6 +81,8 @@ def main():
                     help="trained network for the learned-preconditioner line of --extra")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")
+    ap.add_argument("--dim", type=int, default=3, choices=[2, 3],
+                    help="2: the 2D projection (shk_project2d) on an n^2 scene instead")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -92,6 +94,10 @@ def main():
 
     import shiokaze_amd
     from shiokaze_amd import scene as S
+
+    if args.dim == 2:
+        bench_2d(args, torch, shiokaze_amd, S)
+        return
 
     if args.calibrate_copy > 0:
         src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
@@ -390,6 +396,117 @@ def main():
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    ps.close()
+
+
+def bench_2d(args, torch, shiokaze_amd, S):
+    """The 2D projection (macpressuresolver2): the same step, roofline, accuracy and
+    CPU-baseline legs on an n^2 scene (pool+disc or irregular)."""
+    if args.preconditioner not in ("none", "jacobi"):
+        print("the 2D projection runs Preconditioner none or jacobi", file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    sc = S.generate2d(args.scene, n, args.seed)
+    dev = {k: torch.from_numpy(np.ascontiguousarray(sc[k])).cuda() for k in ("fluid", "solid", "u", "v", "us", "vs")}
+    vel0 = [dev["u"].clone(), dev["v"].clone()]
+
+    def make(config):
+        return shiokaze_amd.MacPressureSolver2((n, n), sc["dx"], config=config)
+
+    def step(ps):
+        dev["u"].copy_(vel0[0])
+        dev["v"].copy_(vel0[1])
+        return ps.project(1.0, (dev["u"], dev["v"]), dev["solid"], (dev["us"], dev["vs"]), dev["fluid"])
+
+    cfg = {"Residual": args.residual, "Precision": args.precision, "Preconditioner": args.preconditioner}
+    for kv in args.set:
+        k, v = kv.split("=", 1)
+        cfg[k] = v
+    ps = make(cfg)
+    for _ in range(args.warmup):
+        step(ps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        last = step(ps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    value = args.steps / elapsed
+    log(f"2D timed: {args.steps} steps in {elapsed:.3f}s -> {value:.3f} solves/s, iters={last['count']}")
+    cells = last["active_bricks"] * 512
+    kernels, roofline = {}, None
+    if not args.no_profile:
+        ps.configure("ProfileKernels", 1)
+        for _ in range(args.profile_steps):
+            step(ps)
+        for k in ("spmv_dot", "axpy_norm"):
+            cnt, ms = ps.kernel_stats(k)
+            if cnt:
+                bpl = kernel_bytes_per_cell(k, args.precision, args.preconditioner) * cells
+                gbs = bpl / (ms / cnt * 1e-3) / 1e9
+                kernels[k] = {"launches": cnt, "avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
+                              "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "total_ms": round(ms, 3)}
+        if kernels:
+            dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+            kd = kernels[dom]
+            roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": kd["frac"], "traffic": None, "bytes_per_launch": kd["bytes_per_launch"],
+                        "avg_launch_us": kd["avg_us"]}
+    accuracy = {}
+    if not args.no_accuracy:
+        sysb = ps.export_system()
+        lab = sysb["label"].bool()
+        svel = (dev["us"], dev["vs"])
+        div_before = ps.divergence(1.0, vel0, svel, fp64_update=False)
+        div_after = ps.divergence(1.0, vel0, svel, fp64_update=True)
+        truth = make({"Residual": 1e-12, "Precision": "fp64", "Preconditioner": "jacobi", "CheckInterval": 128})
+        step(truth)
+        xt = truth.export_system()["x"][lab]
+        xb = sysb["x"][lab]
+        accuracy = {"pressure_rel_l2_vs_fp64_1e-12": float(torch.linalg.norm(xb - xt) / torch.linalg.norm(xt)),
+                    "post_div_rel_fp64": div_after / div_before, "true_relres_inf": last["true_relres_inf"]}
+        truth.close()
+    # plain FP64 CG on the GPU: the reference's iteration count (for the CPU extrapolation)
+    plain = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "none", "CheckInterval": 64})
+    step(plain)
+    t1 = time.perf_counter()
+    rp = step(plain)
+    plain_sps = 1.0 / (time.perf_counter() - t1)
+    plain.close()
+    extra = {"fp64_plain_cg": {"solves_per_s": plain_sps, "iters": rp["count"]},
+             "vs_fp64_plain_cg": round(value / plain_sps, 2)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        O.set_threads(1)
+        tk = {}
+        for k in (10, 30):
+            ta = time.perf_counter()
+            O.project2d(sc, tol=args.residual, max_iterations=k, mode=O.SHK_OR_REFERENCE)
+            tk[k] = time.perf_counter() - ta
+        per_it = (tk[30] - tk[10]) / 20.0
+        base = max(tk[10] - 10 * per_it, 0.0)
+        total = base + rp["count"] * per_it
+        cpu = {"value": 1.0 / total, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": (f"{n}^2 2D scene {args.scene}: the oracle's 2D projection with the reference-faithful CG "
+                          f"(MIC(0) factor, two discarded triangular solves per iteration) run for 10 and 30 "
+                          f"iterations and extrapolated to {rp['count']} (the plain CG count); "
+                          f"{1e3 * per_it:.2f} ms/iteration, single thread"),
+               "ms_per_solve": round(1e3 * total, 1)}
+    prec_name = {"mixed": "fp32 stencil/axpy + fp64 dots/refinement", "fp64": "fp64"}[args.precision]
+    out = {
+        "metric": "pressure solves/sec of the 2D projection to 1e-6 rel. residual; PCG iters; % HBM peak",
+        "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64" if args.precision == "mixed" else "fp64", "data": "synthetic",
+        "config": {"workload": (f"2D {'pool+disc' if args.scene == 1 else 'irregular'} scene {n}^2, {prec_name}, "
+                                f"{args.preconditioner} preconditioner, to {args.residual:g} relative max-norm residual"),
+                   "n": n, "dim": 2, "scene": args.scene, "seed": args.seed, "unknowns": last["unknowns"],
+                   "active_bricks": last["active_bricks"], "pcg_iters": last["count"], "outer": last["outer"],
+                   "precision": args.precision, "preconditioner": args.preconditioner},
+        "roofline": roofline, "kernels": kernels, "accuracy": accuracy, "cpu_baseline": cpu, "extra": extra,
     }
     print(json.dumps(out), flush=True)
     ps.close()
