@@ -118,11 +118,18 @@ int build_bricks(shk_ctx *c) {
         for (DevBuf<double> *b : {&c->hx64, &c->hy64, &c->hz64}) HIPC(c, b->ensure(n));
         HIPC(c, c->kappa64.ensure(c->n_active));
     }
+    if (c->prm.precond == 0 && c->prm.precision == 0) {
+        // plain FP64 CG: uniform bricks (one coupling, one diagonal) skip their coefficients
+        HIPC(c, c->kappa64.ensure(c->n_active));
+        HIPC(c, c->kdiag64.ensure(c->n_active));
+    }
     if (c->prm.precision == 1 || c->prm.precond >= 2) {
         for (DevBuf<float> *b : {&c->x32, &c->r32, &c->s32, &c->t32, &c->q32, &c->rs32}) HIPC(c, b->ensure(n));
-        if (c->prm.precond == 0)
+        if (c->prm.precond == 0) {
             for (DevBuf<float> *b : {&c->diag32, &c->cx32, &c->cy32, &c->cz32}) HIPC(c, b->ensure(n));
-        else {
+            HIPC(c, c->kappa32.ensure(c->n_active));
+            HIPC(c, c->kdiag32.ensure(c->n_active));
+        } else {
             for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32}) HIPC(c, b->ensure(n));
             HIPC(c, c->kappa32.ensure(c->n_active));
         }
@@ -180,8 +187,8 @@ BrickOp<double> op64(shk_ctx *c) {
 }
 BrickOp<float> op32(shk_ctx *c) {
     if (c->prm.precond == 0)
-        return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active, nullptr,
-                              c->dims.bx};
+        return BrickOp<float>{c->nbr.p, c->diag32.p, c->cx32.p, c->cy32.p, c->cz32.p, c->n_active, c->kappa32.p,
+                              c->dims.bx, c->kdiag32.p};
     return BrickOp<float>{c->nbr.p, nullptr, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, c->kappa32.p, c->dims.bx};
 }
 
@@ -206,9 +213,13 @@ int solve_fp64(shk_ctx *c, shk_result *res) {
     const bool jac = c->prm.precond != 0;
     const Grid &g = c->grid;
     const double *sig = jac ? c->sig64.p : nullptr;
-    const BrickOp<double> op =
+    BrickOp<double> op =
         jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active, c->kappa64.p, c->dims.bx}
             : op64(c);
+    if (!jac) {
+        op.kappa = c->kappa64.p;
+        op.kdiag = c->kdiag64.p;
+    }
     const Reduction red = reduction(c);
     launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, red, c->prm.residual,
                                    c->prm.max_iterations);
@@ -422,10 +433,10 @@ void shk_destroy(shk_ctx *c) {
     c->coord.release();
     c->nbr.release();
     for (DevBuf<double> *b : {&c->diag64, &c->cx64, &c->cy64, &c->cz64, &c->b64, &c->sig64, &c->hx64, &c->hy64,
-                              &c->hz64, &c->kappa64, &c->x64, &c->r64, &c->s64, &c->t64, &c->q64, &c->pa, &c->pb,
+                              &c->hz64, &c->kappa64, &c->kdiag64, &c->x64, &c->r64, &c->s64, &c->t64, &c->q64, &c->pa, &c->pb,
                               &c->pc, &c->misc})
         b->release();
-    for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->kappa32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
+    for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->kappa32, &c->kdiag32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
                              &c->s32, &c->t32, &c->q32, &c->rs32})
         b->release();
     c->state.release();
@@ -688,6 +699,12 @@ int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *c
                                               c->kappa64.p);
         if (s32) launch_brick_uniform<float>(c->stream, d.bx, c->n_active, c->nbr.p, c->hx32.p, c->hy32.p, c->hz32.p,
                                              c->kappa32.p);
+    } else if (c->prm.precision == 0) {
+        launch_brick_uniform<double>(c->stream, d.bx, c->n_active, c->nbr.p, c->cx64.p, c->cy64.p, c->cz64.p,
+                                     c->kappa64.p, c->diag64.p, c->kdiag64.p);
+    } else {
+        launch_brick_uniform<float>(c->stream, d.bx, c->n_active, c->nbr.p, c->cx32.p, c->cy32.p, c->cz32.p,
+                                    c->kappa32.p, c->diag32.p, c->kdiag32.p);
     }
     if (c->prm.warm_start) {
         // macpressuresolver3.cpp:238-247: solve for the change from the previous

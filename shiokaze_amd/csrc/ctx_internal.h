@@ -118,6 +118,9 @@ struct shk_ctx {
     DevBuf<float> hx32, hy32, hz32, kappa32;
     // unscaled FP32 operator (mixed precision without preconditioner)
     DevBuf<float> diag32, cx32, cy32, cz32;
+    // plain CG: per-brick constant diagonal of uniform bricks (with kappa64 / kappa32)
+    DevBuf<double> kdiag64;
+    DevBuf<float> kdiag32;
     // Krylov vectors
     DevBuf<double> x64, r64, s64, t64, q64;
     DevBuf<float> x32, r32, s32, t32, q32, rs32;

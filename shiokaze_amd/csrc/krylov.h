@@ -40,9 +40,11 @@ struct BrickOp {
     const int *nbr;
     const C *diag, *cx, *cy, *cz;
     int n_active;
-    // optional, unit-diagonal operators only: per-brick constant coupling (0 = not uniform)
+    // optional: per-brick constant coupling (0 = not uniform); with a diagonal array the
+    // operator also needs kdiag, the brick's constant diagonal
     const C *kappa = nullptr;
     int bx = 8;  // brick shape: 8 (8x8x8) or 64 (64x1x8, a 2D grid held as nx x 1 x ny)
+    const C *kdiag = nullptr;
 };
 struct CsrOp {
     const int64_t *rowptr, *cols;

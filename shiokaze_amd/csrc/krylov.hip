@@ -193,11 +193,16 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
     // first brick's is loaded before the fold, each next one while the current brick runs
     int slot = t0 + wave;
     int mnb[6] = {-1, -1, -1, -1, -1, -1};
-    C mkap = C(0);
+    // uniform bricks: one coupling (and, with a diagonal array, one diagonal value)
+    const bool fast = op.kappa && (UNIT || op.kdiag);
+    C mkap = C(0), mkd = C(0);
     if (slot < t1) {
 #pragma unroll
         for (int f = 0; f < 6; ++f) mnb[f] = op.nbr[6 * slot + f];
-        if (UNIT && op.kappa) mkap = op.kappa[slot];
+        if (fast) {
+            mkap = op.kappa[slot];
+            if (!UNIT) mkd = op.kdiag[slot];
+        }
     }
     double beta;
     if (!direction_step(red, lds, &bc, beta)) return;
@@ -205,12 +210,15 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
     double acc = 0.0;
     for (; slot < t1; slot += NB / kWave) {
         const int nxm = mnb[0], nxp = mnb[1], nym = mnb[2], nyp = mnb[3], nzm = mnb[4], nzp = mnb[5];
-        const C kap = mkap;
+        const C kap = mkap, kdg = mkd;
         const int nslot = slot + NB / kWave;
         if (nslot < t1) {
 #pragma unroll
             for (int f = 0; f < 6; ++f) mnb[f] = op.nbr[6 * nslot + f];
-            if (UNIT && op.kappa) mkap = op.kappa[nslot];
+            if (fast) {
+                mkap = op.kappa[nslot];
+                if (!UNIT) mkd = op.kdiag[nslot];
+            }
         }
         const int64_t base = (int64_t)slot * NT + lane;
         // halo planes of the lateral neighbours, one coalesced load per plane.  x plane
@@ -254,9 +262,10 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             const int64_t e = base + 64 * k;
             sv[k] = r[e] + b * sin[e];
         }
-        if (UNIT && kap != C(0)) {
-            // uniform brick: one coupling value, no coefficient traffic
+        if (kap != C(0)) {
+            // uniform brick: one coupling (and diagonal) value, no coefficient traffic
             const T kv = (T)kap;
+            const T kd = (T)kdg;
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 const T sk = sv[k];
@@ -277,8 +286,9 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
                 }
                 const T zm = k > 0 ? sv[k - 1] : szlo;
                 const T zp = k < 7 ? sv[k + 1] : szhi;
-                // same association as the general path: c*a + c*b + ... summed left to right
-                const T qv = sk - (kv * xm + kv * xp + kv * ym + kv * yp + kv * zm + kv * zp);
+                // the general path's expression with the brick's constants: bit-identical
+                T qv = UNIT ? sk : kd * sk;
+                qv -= kv * xm + kv * xp + kv * ym + kv * yp + kv * zm + kv * zp;
                 const int64_t e = base + 64 * k;
                 sout[e] = sk;
                 q[e] = qv;

@@ -42,7 +42,7 @@ void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, co
 // kappa[slot] = the one coupling value of a brick whose stencil is uniform, else 0.
 template <class C>
 void launch_brick_uniform(hipStream_t st, int bx, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
-                          C *kappa);
+                          C *kappa, const C *diag = nullptr, C *kdiag = nullptr);
 
 // prev (nullable): also the dense FP64 pressure, for the next warm start
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,

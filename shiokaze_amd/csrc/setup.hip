@@ -221,11 +221,13 @@ __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restr
 
 // Per-brick constant coupling: kappa[slot] = c when every coupling touching the
 // brick's cells (its 3 low faces per cell and the far faces stored in the +x/+y/+z
-// neighbour bricks) equals one nonzero value c bit for bit, else 0.  Such a brick
-// has only unknowns and the stencil needs no coefficient loads there.
+// neighbour bricks) equals one nonzero value c bit for bit, and (with a diagonal
+// array) every diagonal equals one value d, then kdiag[slot] = d; else kappa = 0.
+// Such a brick has only unknowns and the stencil needs no coefficient loads there.
 template <class C, int BX>
 __global__ void k_brick_uniform(const int *__restrict__ nbr, const C *__restrict__ cx, const C *__restrict__ cy,
-                                const C *__restrict__ cz, C *__restrict__ kappa) {
+                                const C *__restrict__ cz, const C *__restrict__ diag, C *__restrict__ kappa,
+                                C *__restrict__ kdiag) {
     using B = BrickShape<BX>;
     const int slot = blockIdx.x;
     const int l = threadIdx.x;
@@ -248,8 +250,13 @@ __global__ void k_brick_uniform(const int *__restrict__ nbr, const C *__restrict
         }
         ok = ok && m >= 0 && c[a][m] == k0;
     }
+    const C d0 = diag ? diag[(int64_t)slot * kBrickCells] : C(0);
+    if (diag) ok = ok && diag[e] == d0;
     const int all = __syncthreads_and(ok ? 1 : 0);
-    if (l == 0) kappa[slot] = all ? k0 : C(0);
+    if (l == 0) {
+        kappa[slot] = all ? k0 : C(0);
+        if (kdiag) kdiag[slot] = all ? d0 : C(0);
+    }
 }
 
 // ---------------------------------------------------------------- scatter + update (a14, a15)
@@ -581,15 +588,15 @@ void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, co
 
 template <class C>
 void launch_brick_uniform(hipStream_t st, int bx, int n_active, const int *nbr, const C *cx, const C *cy, const C *cz,
-                          C *kappa) {
+                          C *kappa, const C *diag, C *kdiag) {
     if (n_active <= 0) return;
     auto kern = bx == 64 ? k_brick_uniform<C, 64> : k_brick_uniform<C, 8>;
-    hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, nbr, cx, cy, cz, kappa);
+    hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, nbr, cx, cy, cz, diag, kappa, kdiag);
 }
 template void launch_brick_uniform<double>(hipStream_t, int, int, const int *, const double *, const double *,
-                                           const double *, double *);
+                                           const double *, double *, const double *, double *);
 template void launch_brick_uniform<float>(hipStream_t, int, int, const int *, const float *, const float *,
-                                          const float *, float *);
+                                          const float *, float *, const float *, float *);
 
 void launch_scatter(hipStream_t st, Dims d, const int *brick_slot, int nbx, int nby, const uint8_t *label,
                     const double *x, float *pressure, double *prev) {
