@@ -404,8 +404,8 @@ def main():
 def bench_2d(args, torch, shiokaze_amd, S):
     """The 2D projection (macpressuresolver2): the same step, roofline, accuracy and
     CPU-baseline legs on an n^2 scene (pool+disc or irregular)."""
-    if args.preconditioner not in ("none", "jacobi"):
-        print("the 2D projection runs Preconditioner none or jacobi", file=sys.stderr)
+    if args.preconditioner not in ("none", "jacobi", "mg"):
+        print("the 2D projection runs Preconditioner none, jacobi or mg", file=sys.stderr)
         sys.exit(2)
     n = args.n
     sc = S.generate2d(args.scene, n, args.seed)

@@ -151,8 +151,9 @@ int shk_project(shk_ctx *ctx, double dt, float *u, float *v, float *w,
  *   v     nx*(ny+1)    index i + nx*j
  *   solid (nx+1)*(ny+1) nodal, index i + (nx+1)*j
  * shk_set_solid_density, shk_export_fractions and shk_divergence take the x and y
- * face arrays in their first two slots and NULL in the third.  Preconditioner none
- * and jacobi (either precision) are available in 2D; mg and unet are 3D only. */
+ * face arrays in their first two slots and NULL in the third.  Preconditioner none,
+ * jacobi (either precision) and mg (2x1x2 aggregation) are available in 2D; unet is
+ * 3D only. */
 int shk_create2d(int nx, int ny, double dx, int device, shk_ctx **out);
 
 /* macproject2_interface::project (macpressuresolver2.cpp:52-281): the same steps as

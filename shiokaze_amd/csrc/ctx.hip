@@ -592,8 +592,8 @@ int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *c
     if (!(dt == dt) || std::isinf(dt)) return c->fail(SHK_BAD_ARG, "dt is not finite");
     if (!std::isfinite(surface_tension)) return c->fail(SHK_BAD_ARG, "surface_tension is not finite");
     const bool two_d = c->dims.bx == 64;
-    if (two_d && c->prm.precond >= 2)
-        return c->fail(SHK_BAD_ARG, "2D projections support Preconditioner none and jacobi");
+    if (two_d && c->prm.precond == 2)
+        return c->fail(SHK_BAD_ARG, "the learned preconditioner is 3D only (2D: none, jacobi, mg)");
     HIPC(c, hipSetDevice(c->device));
     shk_result local{};
     if (!res) res = &local;

@@ -2,13 +2,15 @@
 // preconditioner of Preconditioner=mg.
 //
 // Level 0 is the Jacobi-scaled system A^ = D^-1/2 A D^-1/2 (unit diagonal on the
-// unknowns) in the solver's 8^3 bricks.  Level l+1 aggregates the cells of level l
-// 2x2x2 (cell (X,Y,Z) owns the cells (2X+a, 2Y+b, 2Z+c)) with the Galerkin operator
-// P0' A_l P0 of piecewise-constant prolongation P0: again a 7-point operator,
-//   couplings  c_C = sum of the 4 fine couplings across the aggregates' shared face,
-//   diagonal   d_C = sum of the 8 fine diagonals - 2 * (the 12 couplings inside C).
-// Coarsening stops when every axis has at most 4 cells; that level (<= 64 cells, one
-// brick) is solved exactly with a dense inverse computed on the host.
+// unknowns) in the solver's bricks (8x8x8, or 64x1x8 for a 2D grid held as nx x 1 x
+// ny).  Level l+1 aggregates the cells of level l 2x2x2 (2x1x2 on a 2D grid: cell
+// (X,Y,Z) owns the cells (2X+a, 2Y+b, 2Z+c)) with the Galerkin operator P0' A_l P0 of
+// piecewise-constant prolongation P0: again a 7-point (5-point) operator,
+//   couplings  c_C = sum of the fine couplings across the aggregates' shared face,
+//   diagonal   d_C = sum of the fine diagonals - 2 * (the couplings inside C).
+// Coarse levels keep the brick shape of level 0.  Coarsening stops when every axis
+// has at most 4 cells; that level (<= 64 cells, one brick) is solved exactly with a
+// dense inverse computed on the host.
 //
 // V-cycle (symmetric, so M is a fixed SPD operator and plain PCG applies):
 //   x_l = w D_l^-1 b_l;  t_l = b_l - A_l x_l;  b_{l+1} = P0' t_l;  (recurse)
@@ -36,6 +38,7 @@ struct MgOp {
     // level 0 only: per-brick uniform coupling (unit diagonal, one coupling value; 0 =
     // not uniform), which lets those bricks skip the diagonal and coupling reads
     const float *kappa = nullptr;
+    int bx = 8;  // brick shape: 8 (8x8x8) or 64 (64x1x8)
 };
 
 struct MgLevelDev {
@@ -48,7 +51,8 @@ struct MgLevelDev {
     int *parent = nullptr;  // slot of the parent brick on the next level (levels < coarsest)
     float *d = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
     float *b = nullptr, *x = nullptr, *t = nullptr;
-    MgOp op(float omega, float alpha) const { return MgOp{nbr, d, cx, cy, cz, n_active, omega, alpha}; }
+    int bx = 8;             // brick shape: 8 (8x8x8) or 64 (64x1x8)
+    MgOp op(float omega, float alpha) const { return MgOp{nbr, d, cx, cy, cz, n_active, omega, alpha, nullptr, bx}; }
 };
 
 constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)

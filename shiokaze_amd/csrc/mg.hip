@@ -21,17 +21,24 @@ __device__ inline float brick_kappa(const MgOp &op, int slot) { return op.kappa 
 __device__ inline float diag_of(const MgOp &op, int64_t e, float kap) { return kap != 0.0f ? 1.0f : op.d[e]; }
 
 // (A x)_e for the 7-point brick operator with an explicit diagonal (FP32); in a
-// uniform brick (kap != 0: unit diagonal, every face coupled with kap, all six
-// neighbours exist) A x = x - kap * (sum of the neighbours).
+// uniform brick (kap != 0: unit diagonal, every face coupled with kap, all the
+// neighbours exist) A x = x - kap * (sum of the neighbours).  A 2D grid (Y == 1) has
+// no y neighbours.
+template <int BX>
 __device__ inline float apply(const MgOp &op, int slot, int l, const float *__restrict__ x, float kap = 0.0f) {
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
     const int64_t e = (int64_t)slot * kBrickCells + l;
-    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    const int i = l % X, j = (l / X) % Y, k = l / kWave;
     const int *nb = op.nbr + 6 * slot;
     if (kap != 0.0f) {
-        const float xm = i > 0 ? x[e - 1] : x[(int64_t)nb[0] * kBrickCells + l + 7];
-        const float xp = i < 7 ? x[e + 1] : x[(int64_t)nb[1] * kBrickCells + l - 7];
-        const float ym = j > 0 ? x[e - 8] : x[(int64_t)nb[2] * kBrickCells + l + 56];
-        const float yp = j < 7 ? x[e + 8] : x[(int64_t)nb[3] * kBrickCells + l - 56];
+        const float xm = i > 0 ? x[e - 1] : x[(int64_t)nb[0] * kBrickCells + l + (X - 1)];
+        const float xp = i < X - 1 ? x[e + 1] : x[(int64_t)nb[1] * kBrickCells + l - (X - 1)];
+        float ym = 0.0f, yp = 0.0f;
+        if constexpr (Y > 1) {
+            ym = j > 0 ? x[e - X] : x[(int64_t)nb[2] * kBrickCells + l + X * (Y - 1)];
+            yp = j < Y - 1 ? x[e + X] : x[(int64_t)nb[3] * kBrickCells + l - X * (Y - 1)];
+        }
         const float zm = k > 0 ? x[e - 64] : x[(int64_t)nb[4] * kBrickCells + l + 448];
         const float zp = k < 7 ? x[e + 64] : x[(int64_t)nb[5] * kBrickCells + l - 448];
         return x[e] - kap * (xm + xp + ym + yp + zm + zp);
@@ -40,24 +47,26 @@ __device__ inline float apply(const MgOp &op, int slot, int l, const float *__re
     if (i > 0) {
         acc -= op.cx[e] * x[e - 1];
     } else if (nb[0] >= 0) {
-        acc -= op.cx[e] * x[(int64_t)nb[0] * kBrickCells + l + 7];
+        acc -= op.cx[e] * x[(int64_t)nb[0] * kBrickCells + l + (X - 1)];
     }
-    if (i < 7) {
+    if (i < X - 1) {
         acc -= op.cx[e + 1] * x[e + 1];
     } else if (nb[1] >= 0) {
-        const int64_t m = (int64_t)nb[1] * kBrickCells + l - 7;
+        const int64_t m = (int64_t)nb[1] * kBrickCells + l - (X - 1);
         acc -= op.cx[m] * x[m];
     }
-    if (j > 0) {
-        acc -= op.cy[e] * x[e - 8];
-    } else if (nb[2] >= 0) {
-        acc -= op.cy[e] * x[(int64_t)nb[2] * kBrickCells + l + 56];
-    }
-    if (j < 7) {
-        acc -= op.cy[e + 8] * x[e + 8];
-    } else if (nb[3] >= 0) {
-        const int64_t m = (int64_t)nb[3] * kBrickCells + l - 56;
-        acc -= op.cy[m] * x[m];
+    if constexpr (Y > 1) {
+        if (j > 0) {
+            acc -= op.cy[e] * x[e - X];
+        } else if (nb[2] >= 0) {
+            acc -= op.cy[e] * x[(int64_t)nb[2] * kBrickCells + l + X * (Y - 1)];
+        }
+        if (j < Y - 1) {
+            acc -= op.cy[e + X] * x[e + X];
+        } else if (nb[3] >= 0) {
+            const int64_t m = (int64_t)nb[3] * kBrickCells + l - X * (Y - 1);
+            acc -= op.cy[m] * x[m];
+        }
     }
     if (k > 0) {
         acc -= op.cz[e] * x[e - 64];
@@ -73,16 +82,48 @@ __device__ inline float apply(const MgOp &op, int slot, int l, const float *__re
     return acc;
 }
 
-// fine slot holding the 8 children of coarse cell (slot_c, l) and the children's base
+// Aggregation: 2x2x2 children per coarse cell, 2x1x2 on a 2D grid (Y == 1).
+template <int BX>
+struct Agg {
+    using B = BrickShape<BX>;
+    static constexpr int X = B::X, Y = B::Y;
+    static constexpr int AY = Y > 1 ? 2 : 1;       // aggregation factor along y
+    static constexpr int NCH = 4 * AY;             // children per coarse cell
+    // brick-local offset of child q (a along x, b along y, c along z)
+    __device__ static int child(int q) {
+        const int a = q & 1, b = AY > 1 ? (q >> 1) & 1 : 0, c = AY > 1 ? q >> 2 : q >> 1;
+        return a + X * b + 64 * c;
+    }
+    __device__ static void bits(int q, int &a, int &b, int &c) {
+        a = q & 1;
+        b = AY > 1 ? (q >> 1) & 1 : 0;
+        c = AY > 1 ? q >> 2 : q >> 1;
+    }
+};
+
+// fine slot holding the children of coarse cell (slot_c, l) and the children's base
 // local offset; -1 when the fine brick is outside the grid or inactive
+template <int BX>
 __device__ inline int children(const MgLevelDev &f, const MgLevelDev &c, int slot_c, int l, int &base) {
-    const int ci = l & 7, cj = (l >> 3) & 7, ck = l >> 6;
-    const int4 B = c.coord[slot_c];
-    if (B.x * 8 + ci >= c.nx || B.y * 8 + cj >= c.ny || B.z * 8 + ck >= c.nz) return -1;
-    const int fbx = 2 * B.x + (ci >> 2), fby = 2 * B.y + (cj >> 2), fbz = 2 * B.z + (ck >> 2);
+    using A = Agg<BX>;
+    constexpr int X = A::X, Y = A::Y, HX = X / 2, HY = Y > 1 ? Y / 2 : 1;
+    const int ci = l % X, cj = (l / X) % Y, ck = l / kWave;
+    const int4 Bc = c.coord[slot_c];
+    if (Bc.x * X + ci >= c.nx || Bc.y * Y + cj >= c.ny || Bc.z * 8 + ck >= c.nz) return -1;
+    const int fbx = 2 * Bc.x + ci / HX, fby = A::AY * Bc.y + cj / HY, fbz = 2 * Bc.z + (ck >> 2);
     if (fbx >= f.nbx || fby >= f.nby || fbz >= f.nbz) return -1;
-    base = 2 * (ci & 3) + 8 * (2 * (cj & 3)) + 64 * (2 * (ck & 3));
+    base = 2 * (ci % HX) + X * (A::AY * (cj % HY)) + 64 * (2 * (ck & 3));
     return f.slot[fbx + f.nbx * (fby + f.nby * fbz)];
+}
+
+// cell index on the next level of the cell (brick coord Bf, local l) of a fine level
+template <int BX>
+__device__ inline int coarse_local(int4 Bf, int l) {
+    using A = Agg<BX>;
+    constexpr int X = A::X, Y = A::Y, HX = X / 2, HY = Y > 1 ? Y / 2 : 1;
+    const int i = l % X, j = (l / X) % Y, k = l / kWave;
+    const int cy = A::AY > 1 ? (Bf.y & 1) * HY + j / 2 : 0;
+    return ((Bf.x & 1) * HX + i / 2) + X * cy + 64 * ((Bf.z & 1) * 4 + k / 2);
 }
 
 __global__ __launch_bounds__(NT) void k_mg_mask(int64_t n, const double *__restrict__ sig, float *__restrict__ d0) {
@@ -90,22 +131,27 @@ __global__ __launch_bounds__(NT) void k_mg_mask(int64_t n, const double *__restr
     if (e < n) d0[e] = sig[e] > 0.0 ? 1.0f : 0.0f;
 }
 
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
+    using A = Agg<BX>;
     const int64_t E = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (E >= (int64_t)c.n_active * kBrickCells) return;
     const int slot_c = (int)(E >> 9), l = (int)(E & 511);
     int base = 0;
-    const int fs = children(f, c, slot_c, l, base);
+    const int fs = children<BX>(f, c, slot_c, l, base);
     double d = 0.0, inner = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
     if (fs >= 0) {
         const int64_t fb = (int64_t)fs * kBrickCells + base;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int a = q & 1, b = (q >> 1) & 1, cc = q >> 2;
-            const int64_t e = fb + a + 8 * b + 64 * cc;
+        for (int q = 0; q < A::NCH; ++q) {
+            int a, b, cc;
+            A::bits(q, a, b, cc);
+            const int64_t e = fb + A::child(q);
             d += (double)f.d[e];
             if (a) inner += (double)f.cx[e]; else cx += (double)f.cx[e];
-            if (b) inner += (double)f.cy[e]; else cy += (double)f.cy[e];
+            if (A::AY > 1) {
+                if (b) inner += (double)f.cy[e]; else cy += (double)f.cy[e];
+            }
             if (cc) inner += (double)f.cz[e]; else cz += (double)f.cz[e];
         }
     }
@@ -117,6 +163,7 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
     c.cz[E] = (float)cz;
 }
 
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
                                                     float *__restrict__ t, const SolveState *state) {
     if (stopped(state)) return;
@@ -124,43 +171,44 @@ __global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__rest
     if (e >= (int64_t)op.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     const float kap = brick_kappa(op, slot);
-    t[e] = diag_of(op, e, kap) > 0.0f ? b[e] - apply(op, slot, l, x, kap) : 0.0f;
+    t[e] = diag_of(op, e, kap) > 0.0f ? b[e] - apply<BX>(op, slot, l, x, kap) : 0.0f;
 }
 
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, const float *__restrict__ tf,
                                                     float omega, const SolveState *state) {
+    using A = Agg<BX>;
     if (stopped(state)) return;
     const int64_t E = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (E >= (int64_t)c.n_active * kBrickCells) return;
     const int slot_c = (int)(E >> 9), l = (int)(E & 511);
     int base = 0;
-    const int fs = children(f, c, slot_c, l, base);
+    const int fs = children<BX>(f, c, slot_c, l, base);
     float s = 0.0f;
     if (fs >= 0) {
         const int64_t fb = (int64_t)fs * kBrickCells + base;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) s += tf[fb + (q & 1) + 8 * ((q >> 1) & 1) + 64 * (q >> 2)];
+        for (int q = 0; q < A::NCH; ++q) s += tf[fb + A::child(q)];
     }
     const float dc = c.d[E];
     c.b[E] = dc > 0.0f ? s : 0.0f;
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
 
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, float alpha, const SolveState *state) {
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)f.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     if (!(f.d[e] > 0.0f)) return;
-    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
-    const int4 B = f.coord[slot];
     const int ps = f.parent[slot];
     if (ps < 0) return;  // cannot happen: the parent of an active brick is active
-    const int cl = ((B.x & 1) * 4 + (i >> 1)) + 8 * ((B.y & 1) * 4 + (j >> 1)) + 64 * ((B.z & 1) * 4 + (k >> 1));
-    f.x[e] += alpha * c.x[(int64_t)ps * kBrickCells + cl];
+    f.x[e] += alpha * c.x[(int64_t)ps * kBrickCells + coarse_local<BX>(f.coord[slot], l)];
 }
 
 // y = x + w (b - A x) / d, one thread per cell
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
                                                   float *__restrict__ y, const SolveState *state) {
     if (stopped(state)) return;
@@ -169,20 +217,15 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restri
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     const float kap = brick_kappa(op, slot);
     const float dv = diag_of(op, e, kap);
-    y[e] = dv > 0.0f ? x[e] + op.omega * (b[e] - apply(op, slot, l, x, kap)) / dv : 0.0f;
+    y[e] = dv > 0.0f ? x[e] + op.omega * (b[e] - apply<BX>(op, slot, l, x, kap)) / dv : 0.0f;
 }
 
 // ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
 // unknowns and the diagonal is 1 on them, so x0 = w r needs no division and is never
 // stored: the pre-smoothing residual and the post-smoothing recompute it from r.
 
-// cell index on level 1 of the level-0 cell (brick coord B, local l)
-__device__ inline int coarse_local(int4 B, int l) {
-    return ((B.x & 1) * 4 + ((l & 7) >> 1)) + 8 * ((B.y & 1) * 4 + (((l >> 3) & 7) >> 1)) +
-           64 * ((B.z & 1) * 4 + ((l >> 6) >> 1));
-}
-
 // t0 = r - A (w r) on the unknowns (0 elsewhere)
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_pre0(MgOp op, const float *__restrict__ r, float *__restrict__ t,
                                                 const SolveState *state) {
     if (stopped(state)) return;
@@ -191,30 +234,34 @@ __global__ __launch_bounds__(NT) void k_mg_pre0(MgOp op, const float *__restrict
     const int slot = (int)(e >> 9), l = (int)(e & 511);
     const float kap = brick_kappa(op, slot);
     // A (w r) = w A r by linearity
-    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - op.omega * apply(op, slot, l, r, kap) : 0.0f;
+    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - op.omega * apply<BX>(op, slot, l, r, kap) : 0.0f;
 }
 
 // value of x0' = w r + alpha P0 x1 at level-0 cell (slot, l) of brick B with parent ps
+template <int BX>
 __device__ inline float xprime(const MgOp &op, const float *__restrict__ r, const float *__restrict__ xc, int slot,
                                int l, int4 B, int ps) {
     return op.omega * r[(int64_t)slot * kBrickCells + l] +
-           op.alpha * xc[(int64_t)ps * kBrickCells + coarse_local(B, l)];
+           op.alpha * xc[(int64_t)ps * kBrickCells + coarse_local<BX>(B, l)];
 }
 
 // y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
 // prolongation fused into the post-smoothing), one thread per cell; part[block] =
 // sum of r*y over the block's 256 cells
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict__ coord,
                                                  const int *__restrict__ parent, const float *__restrict__ r,
                                                  const float *__restrict__ xc, float *__restrict__ y,
                                                  const SolveState *state, double *__restrict__ part) {
+    using Bs = BrickShape<BX>;
+    constexpr int X = Bs::X, Y = Bs::Y;
     if (stopped(state)) return;
     __shared__ double lds[NT / kWave];
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     double acc = 0.0;
     if (e < (int64_t)op.n_active * kBrickCells) {
         const int slot = (int)(e >> 9), l = (int)(e & 511);
-        const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+        const int i = l % X, j = (l / X) % Y, k = l / kWave;
         const float kap = brick_kappa(op, slot);
         const float dv = diag_of(op, e, kap);
         float yv = 0.0f;
@@ -223,20 +270,22 @@ __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict
             const int4 B = coord[slot];
             const int ps = parent[slot];
             const int *nb = op.nbr + 6 * slot;
-            const float xs = xprime(op, r, xc, slot, l, B, ps);
+            const float xs = xprime<BX>(op, r, xc, slot, l, B, ps);
             // neighbour (slot', l') across each face, the face coupling, and x0' there
             float sum = 0.0f;
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
                 const int axis = f >> 1, hi = f & 1;
+                if (Y == 1 && axis == 1) continue;
                 const int pos = axis == 0 ? i : (axis == 1 ? j : k);
-                const int stride = axis == 0 ? 1 : (axis == 1 ? 8 : 64);
+                const int ext = axis == 0 ? X : (axis == 1 ? Y : 8);
+                const int stride = axis == 0 ? 1 : (axis == 1 ? X : kWave);
                 int ns = slot, nl;
-                if (hi ? pos < 7 : pos > 0) {
+                if (hi ? pos < ext - 1 : pos > 0) {
                     nl = hi ? l + stride : l - stride;
                 } else {
                     ns = nb[f];
-                    nl = hi ? l - 7 * stride : l + 7 * stride;
+                    nl = hi ? l - (ext - 1) * stride : l + (ext - 1) * stride;
                 }
                 if (ns < 0) continue;
                 // coupling: low face at this cell, high face at the neighbour
@@ -250,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict
                 if (c == 0.0f) continue;
                 const int4 NB = ns == slot ? B : coord[ns];
                 const int nps = ns == slot ? ps : parent[ns];
-                sum += c * xprime(op, r, xc, ns, nl, NB, nps);
+                sum += c * xprime<BX>(op, r, xc, ns, nl, NB, nps);
             }
             yv = xs + op.omega * (rv - (xs - sum));
         }
@@ -297,42 +346,46 @@ void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0) {
     hipLaunchKernelGGL(k_mg_mask, dim3(blocks(n)), dim3(NT), 0, st, n, sig, d0);
 }
 
+#define SHK_MG_LAUNCH(kernel, bx, grid, ...)                                                   \
+    do {                                                                                       \
+        if ((bx) == 64)                                                                        \
+            hipLaunchKernelGGL(kernel<64>, dim3(grid), dim3(NT), 0, __VA_ARGS__);              \
+        else                                                                                   \
+            hipLaunchKernelGGL(kernel<8>, dim3(grid), dim3(NT), 0, __VA_ARGS__);               \
+    } while (0)
+
 void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c) {
-    hipLaunchKernelGGL(k_mg_coarsen, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c);
+    SHK_MG_LAUNCH(k_mg_coarsen, c.bx, blocks((int64_t)c.n_active * kBrickCells), st, f, c);
 }
 
 void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
                         const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_residual, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x, t,
-                       state);
+    SHK_MG_LAUNCH(k_mg_residual, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, b, x, t, state);
 }
 
 void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
                         const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks((int64_t)c.n_active * kBrickCells)), dim3(NT), 0, st, f, c, tf,
-                       omega, state);
+    SHK_MG_LAUNCH(k_mg_restrict, c.bx, blocks((int64_t)c.n_active * kBrickCells), st, f, c, tf, omega, state);
 }
 
 void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float alpha, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks((int64_t)f.n_active * kBrickCells)), dim3(NT), 0, st, f, c, alpha,
-                       state);
+    SHK_MG_LAUNCH(k_mg_prolong, f.bx, blocks((int64_t)f.n_active * kBrickCells), st, f, c, alpha, state);
 }
 
 void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
                       const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_smooth, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, b, x, y,
-                       state);
+    SHK_MG_LAUNCH(k_mg_smooth, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, b, x, y, state);
 }
 
 void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state) {
-    hipLaunchKernelGGL(k_mg_pre0, dim3(blocks((int64_t)op.n_active * kBrickCells)), dim3(NT), 0, st, op, r, t, state);
+    SHK_MG_LAUNCH(k_mg_pre0, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, r, t, state);
 }
 
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
                      const float *xc, float *y, const SolveState *state, const Grid &g, double *cell_part,
                      double *part) {
     const unsigned nb = blocks((int64_t)op.n_active * kBrickCells);
-    hipLaunchKernelGGL(k_mg_post0, dim3(nb), dim3(NT), 0, st, op, coord, parent, r, xc, y, state, cell_part);
+    SHK_MG_LAUNCH(k_mg_post0, op.bx, nb, st, op, coord, parent, r, xc, y, state, cell_part);
     hipLaunchKernelGGL(k_mg_fold, dim3((g.blocks + NT - 1) / NT), dim3(NT), 0, st, (int)nb, cell_part, g.blocks, part,
                        state);
 }

@@ -5,7 +5,8 @@
 // It takes the place of the reference's "amg" linear-solver plugin
 // (src/linsolver/amg.cpp:43-166, smoothed-aggregation AMG from the vendored amgcl):
 // an algebraic-multigrid-preconditioned CG behind the same solve() contract, here
-// with geometric 2x2x2 aggregates that the brick layout makes free to build.
+// with geometric 2x2x2 aggregates (2x1x2 on a 2D grid) that the brick layout makes
+// free to build.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -71,6 +72,10 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
     L0.v.nby = c->nby;
     L0.v.nbz = c->nbz;
     L0.v.n_active = c->n_active;
+    // brick extents (8x8x8, or 64x1x8 on a 2D grid) and the y aggregation factor
+    const int bx = c->dims.bx, ex = bx, ey = kWave / bx, ez = kBrickCells / kWave;
+    const int ay = ey > 1 ? 2 : 1;
+    for (MgLevelHost &L : h->lv) L.v.bx = bx;
     if (same) return SHK_OK;
     for (size_t l = 1; l < h->lv.size(); ++l) release(h->lv[l]);
     h->lv.resize(1);
@@ -81,26 +86,29 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
         const MgLevelHost &F = h->lv.back();
         if (h->lv.size() > 1 && std::max({F.v.nx, F.v.ny, F.v.nz}) <= 4) break;
         MgLevelHost C;
+        C.v.bx = bx;
         C.v.nx = (F.v.nx + 1) / 2;
-        C.v.ny = (F.v.ny + 1) / 2;
+        C.v.ny = (F.v.ny + ay - 1) / ay;
         C.v.nz = (F.v.nz + 1) / 2;
-        C.v.nbx = (C.v.nx + 7) / 8;
-        C.v.nby = (C.v.ny + 7) / 8;
-        C.v.nbz = (C.v.nz + 7) / 8;
+        C.v.nbx = (C.v.nx + ex - 1) / ex;
+        C.v.nby = (C.v.ny + ey - 1) / ey;
+        C.v.nbz = (C.v.nz + ez - 1) / ez;
         const int64_t nb = (int64_t)C.v.nbx * C.v.nby * C.v.nbz;
         C.h_slot.assign(nb, -1);
-        for (int bz = 0; bz < C.v.nbz; ++bz)
-            for (int by = 0; by < C.v.nby; ++by)
-                for (int bx = 0; bx < C.v.nbx; ++bx) {
+        // a coarse brick covers 2 x ay x 2 fine bricks
+        for (int cz = 0; cz < C.v.nbz; ++cz)
+            for (int cy = 0; cy < C.v.nby; ++cy)
+                for (int cx = 0; cx < C.v.nbx; ++cx) {
                     bool any = false;
-                    for (int q = 0; q < 8 && !any; ++q) {
-                        const int fx = 2 * bx + (q & 1), fy = 2 * by + ((q >> 1) & 1), fz = 2 * bz + (q >> 2);
+                    for (int q = 0; q < 4 * ay && !any; ++q) {
+                        const int fx = 2 * cx + (q & 1), fy = ay * cy + (ay > 1 ? (q >> 1) & 1 : 0),
+                                  fz = 2 * cz + (ay > 1 ? q >> 2 : q >> 1);
                         if (fx < F.v.nbx && fy < F.v.nby && fz < F.v.nbz)
                             any = F.h_slot[fx + (int64_t)F.v.nbx * (fy + (int64_t)F.v.nby * fz)] >= 0;
                     }
                     if (any) {
-                        C.h_slot[bx + (int64_t)C.v.nbx * (by + (int64_t)C.v.nby * bz)] = (int)C.h_coord.size();
-                        C.h_coord.push_back(make_int4(bx, by, bz, 0));
+                        C.h_slot[cx + (int64_t)C.v.nbx * (cy + (int64_t)C.v.nby * cz)] = (int)C.h_coord.size();
+                        C.h_coord.push_back(make_int4(cx, cy, cz, 0));
                     }
                 }
         C.v.n_active = (int)C.h_coord.size();
@@ -114,7 +122,7 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
             L.h_parent.assign(std::max(L.v.n_active, 1), -1);
             for (int s = 0; s < L.v.n_active; ++s) {
                 const int4 q = L.h_coord[s];
-                L.h_parent[s] = C.h_slot[q.x / 2 + (int64_t)C.v.nbx * (q.y / 2 + (int64_t)C.v.nby * (q.z / 2))];
+                L.h_parent[s] = C.h_slot[q.x / 2 + (int64_t)C.v.nbx * (q.y / ay + (int64_t)C.v.nby * (q.z / 2))];
             }
         }
     }
@@ -157,8 +165,9 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
 
 // dense inverse (Cholesky, FP64) of the coarsest operator on its active cells;
 // the diagonal inverse when that operator is not positive definite
-void coarse_inverse(const std::vector<float> &d, const std::vector<float> &cx, const std::vector<float> &cy,
+void coarse_inverse(int bx, const std::vector<float> &d, const std::vector<float> &cx, const std::vector<float> &cy,
                     const std::vector<float> &cz, std::vector<int> &idx, std::vector<float> &inv) {
+    const int ex = bx, ey = kWave / bx;
     idx.clear();
     for (int e = 0; e < kBrickCells; ++e)
         if (d[e] > 0.0f) idx.push_back(e);
@@ -169,8 +178,8 @@ void coarse_inverse(const std::vector<float> &d, const std::vector<float> &cx, c
     for (int i = 0; i < n; ++i) {
         const int e = idx[i];
         A[(size_t)i * n + i] = d[e];
-        const int x = e & 7, y = (e >> 3) & 7, z = e >> 6;
-        const int lo[3] = {x > 0 ? e - 1 : -1, y > 0 ? e - 8 : -1, z > 0 ? e - 64 : -1};
+        const int x = e % ex, y = (e / ex) % ey, z = e / kWave;
+        const int lo[3] = {x > 0 ? e - 1 : -1, y > 0 ? e - ex : -1, z > 0 ? e - kWave : -1};
         const float cv[3] = {cx[e], cy[e], cz[e]};
         for (int a = 0; a < 3; ++a) {
             if (lo[a] < 0 || pos[lo[a]] < 0 || cv[a] == 0.0f) continue;
@@ -275,7 +284,7 @@ int mg_setup(shk_ctx *c) {
     HIPC(c, hipStreamSynchronize(st));
     std::vector<int> idx;
     std::vector<float> inv;
-    coarse_inverse(hd, hx, hy, hz, idx, inv);
+    coarse_inverse(c->dims.bx, hd, hx, hy, hz, idx, inv);
     if ((int)idx.size() > kMgCoarseMax) return c->fail(SHK_HIP_ERROR, "multigrid: coarsest level too large");
     h->ncoarse = (int)idx.size();
     HIPC(c, h->cidx.ensure(std::max<size_t>(idx.size(), 1)));
@@ -296,7 +305,7 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
     const int L = (int)h->lv.size() - 1;
     MgLevelHost &L0 = h->lv[0];
     const float w = (float)c->prm.mg_omega, a = (float)c->prm.mg_alpha;
-    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, w, a, c->kappa32.p};
+    const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, w, a, c->kappa32.p, c->dims.bx};
     launch_mg_pre0(st, op0, r, L0.v.t, state);
     for (int l = 1; l < L; ++l) {
         MgLevelHost &F = h->lv[l - 1], &C = h->lv[l];
@@ -386,23 +395,27 @@ int solve_mg(shk_ctx *c, shk_result *res) {
 
 namespace {
 
+template <int BX>
 __global__ void k_dense_to_bricks(int nx, int ny, int nz, const int4 *__restrict__ coord, int n_active,
                                   const float *__restrict__ src, float *__restrict__ dst) {
+    using B = BrickShape<BX>;
     const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (e >= (int64_t)n_active * kBrickCells) return;
     const int s = (int)(e >> 9), l = (int)(e & 511);
     const int4 q = coord[s];
-    const int i = q.x * 8 + (l & 7), j = q.y * 8 + ((l >> 3) & 7), k = q.z * 8 + (l >> 6);
+    const int i = q.x * B::X + l % B::X, j = q.y * B::Y + (l / B::X) % B::Y, k = q.z * B::Z + l / kWave;
     dst[e] = (i < nx && j < ny && k < nz) ? src[((int64_t)k * ny + j) * nx + i] : 0.0f;
 }
 
+template <int BX>
 __global__ void k_bricks_to_dense(int nx, int ny, int nz, const int *__restrict__ slot, int nbx, int nby,
                                   const float *__restrict__ src, float *__restrict__ dst) {
+    using B = BrickShape<BX>;
     const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (v >= (int64_t)nx * ny * nz) return;
     const int i = (int)(v % nx), j = (int)((v / nx) % ny), k = (int)(v / ((int64_t)nx * ny));
-    const int s = slot[(i / 8) + (int64_t)nbx * ((j / 8) + (int64_t)nby * (k / 8))];
-    dst[v] = s >= 0 ? src[(int64_t)s * kBrickCells + (i & 7) + 8 * (j & 7) + 64 * (k & 7)] : 0.0f;
+    const int s = slot[(i / B::X) + (int64_t)nbx * ((j / B::Y) + (int64_t)nby * (k / B::Z))];
+    dst[v] = s >= 0 ? src[(int64_t)s * kBrickCells + B::elem(i, j, k)] : 0.0f;
 }
 
 }  // namespace
@@ -421,11 +434,19 @@ extern "C" int shk_mg_apply(shk_ctx *c, const float *r, float *z) {
     const int64_t nd = (int64_t)d.nx * d.ny * d.nz, nb = (int64_t)c->n_active * kBrickCells;
     hipStream_t st = c->stream;
     const unsigned gb = (unsigned)((nb + 255) / 256), gd = (unsigned)((nd + 255) / 256);
-    hipLaunchKernelGGL(k_dense_to_bricks, dim3(gb), dim3(256), 0, st, d.nx, d.ny, d.nz, c->coord.p, c->n_active, r,
-                       c->r32.p);
+    if (d.bx == 64)
+        hipLaunchKernelGGL(k_dense_to_bricks<64>, dim3(gb), dim3(256), 0, st, d.nx, d.ny, d.nz, c->coord.p,
+                           c->n_active, r, c->r32.p);
+    else
+        hipLaunchKernelGGL(k_dense_to_bricks<8>, dim3(gb), dim3(256), 0, st, d.nx, d.ny, d.nz, c->coord.p,
+                           c->n_active, r, c->r32.p);
     mg_vcycle(c, c->r32.p, c->mg->z.p, nullptr, &c->grid, c->mg->pz.p);
-    hipLaunchKernelGGL(k_bricks_to_dense, dim3(gd), dim3(256), 0, st, d.nx, d.ny, d.nz, c->brick_slot.p, c->nbx,
-                       c->nby, c->mg->z.p, z);
+    if (d.bx == 64)
+        hipLaunchKernelGGL(k_bricks_to_dense<64>, dim3(gd), dim3(256), 0, st, d.nx, d.ny, d.nz, c->brick_slot.p,
+                           c->nbx, c->nby, c->mg->z.p, z);
+    else
+        hipLaunchKernelGGL(k_bricks_to_dense<8>, dim3(gd), dim3(256), 0, st, d.nx, d.ny, d.nz, c->brick_slot.p,
+                           c->nbx, c->nby, c->mg->z.p, z);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(st));
     return SHK_OK;

@@ -294,5 +294,3 @@ class MacPressureSolver2(MacPressureSolver3):
     def export_scaled(self) -> dict:
         raise NotImplementedError("export_scaled is defined for 3D contexts")
 
-    def mg_apply(self, r):
-        raise NotImplementedError("the multigrid preconditioner is defined for 3D contexts")

@@ -209,10 +209,10 @@ def test_2d_host_pointers_and_errors():
                                 p(host["vs"]), p(host["fluid"]), 0.0, ctypes.byref(res)) == 0
     np.testing.assert_array_equal(host["u"], dev["velocity"][0])
     np.testing.assert_array_equal(host["v"], dev["velocity"][1])
-    # a 3D call on a 2D context, and the 3D-only preconditioners, are refused
+    # a 3D call on a 2D context, and the 3D-only learned preconditioner, are refused
     bad = shiokaze_amd._lib.SHK_BAD_ARG
     assert ps.lib.shk_project(ps.ctx, 1.0, *([p(host["u"])] * 8), 0.0, ctypes.byref(res)) == bad
-    ps.configure("Preconditioner", "mg")
+    ps.configure("Preconditioner", "unet")
     assert ps.lib.shk_project2d(ps.ctx, 1.0, p(host["u"]), p(host["v"]), p(host["solid"]), p(host["us"]),
                                 p(host["vs"]), p(host["fluid"]), 0.0, ctypes.byref(res)) == bad
     ps.close()
@@ -231,3 +231,66 @@ def test_2d_large_converges():
             f"true {r['true_relres_inf']:.2e} solve {r['ms_solve']:.1f} ms")
         assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
         assert g["div_after_fp64"] / g["div_before"] <= 1e-6 * 1.0001
+
+
+# ---------------------------------------------------------------- multigrid in 2D
+
+def _scaled_2d(system):
+    """The Jacobi-scaled 2D system exactly as the library forms it (k_scale_coeff:
+    c * sig_cell * sig_low in double, stored as float) from the exported one."""
+    lab = system["label"].astype(bool)
+    d = system["diag"]
+    sig = np.where(lab, 1.0 / np.sqrt(np.where(lab, d, 1.0)), 0.0)
+    hx = np.zeros_like(d)
+    hz = np.zeros_like(d)
+    hx[:, 1:] = system["cx"][:, 1:] * sig[:, 1:] * sig[:, :-1]
+    hz[1:, :] = system["cz"][1:, :] * sig[1:, :] * sig[:-1, :]
+    return lab.astype(np.float64), hx.astype(np.float32).astype(np.float64), hz.astype(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("case", [(1, (64, 48), 0), (3, (96, 80), 3), (3, (70, 37), 5)], ids=_ids)
+def test_2d_mg_vcycle_matches_oracle(case):
+    """One 2D V-cycle (2x1x2 aggregation of the nx x 1 x ny layout) against the numpy
+    multigrid restatement on the same scaled system held as (ny, 1, nx)."""
+    import torch
+
+    from oracle import mg_oracle as MG
+    from shiokaze_amd import MacPressureSolver2
+    kind, shape, seed = case
+    sc = _scene(kind, shape, seed)
+    ny, nx = sc["fluid"].shape
+    d = _dev(sc)
+    with MacPressureSolver2((nx, ny), sc["dx"], config={"Residual": 1e-6, "Preconditioner": "mg"}) as ps:
+        res = ps.project(1.0, (d["u"], d["v"]), d["solid"], (d["us"], d["vs"]), d["fluid"])
+        system = {k: v.cpu().numpy() for k, v in ps.export_system().items()}
+        mask, hx, hz = _scaled_2d(system)
+        rng = np.random.default_rng(seed)
+        r = (rng.standard_normal((ny, nx)) * mask).astype(np.float32)
+        z = ps.mg_apply(torch.from_numpy(r).cuda()).cpu().numpy().astype(np.float64)
+    emb = lambda a: a.reshape(ny, 1, nx)  # noqa: E731
+    levels = MG.hierarchy(MG.scaled_level0(emb(mask), emb(hx), np.zeros((ny, 1, nx)), emb(hz)))
+    ref = MG.vcycle(levels, emb(r.astype(np.float64))).reshape(ny, nx)
+    rel = np.linalg.norm(z - ref) / np.linalg.norm(ref)
+    log(f"[gpu] 2D mg v-cycle {shape}: levels {[lv[0].shape for lv in levels]}, rel L2 {rel:.2e}, "
+        f"projection {res['count']} its")
+    assert res["status"] == 0
+    assert rel < 1e-4
+    assert np.all(z[mask == 0] == 0)
+
+
+@pytest.mark.parametrize("case", [(1, (256, 192), 0), (3, (256, 256), 2), (3, (70, 37), 5)], ids=_ids)
+def test_2d_projection_with_multigrid(oracle, case):
+    kind, shape, seed = case
+    sc = _scene(kind, shape, seed)
+    g = run2d(sc, {"Residual": 1e-6, "Preconditioner": "mg"})[0]
+    jac = run2d(sc, {"Residual": 1e-6, "Precision": "mixed", "Preconditioner": "jacobi"})[0]
+    truth = oracle.project2d(sc, tol=1e-12, mode=oracle.SHK_OR_JACOBI)
+    r, rj = g["result"], jac["result"]
+    lab = truth["index_map2"] >= 0
+    xt = _dense_2d(truth)["x"][lab]
+    rel = np.linalg.norm(g["system"]["x"][lab] - xt) / np.linalg.norm(xt)
+    log(f"[gpu] 2D mg {shape}: {r['count']} its (outer {r['outer']}) vs jacobi {rj['count']}, "
+        f"relres {r['true_relres_inf']:.2e}, x rel {rel:.2e}")
+    assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
+    assert r["count"] * 4 <= rj["count"]
+    assert rel < 1e-3
