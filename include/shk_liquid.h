@@ -1,0 +1,74 @@
+/* shk_liquid.h -- C-ABI of the velocity half of the liquid step around the pressure
+ * projection, on the same context as shk_projection.h (a 3D context from shk_create;
+ * device pointers only).
+ *
+ * macliquid3::idle (src/liquid/macliquid3.cpp:342-391) runs, per step: the time step
+ * from max |u| (:354), extend_both = velocity extrapolation + the solid constraint
+ * (:330-340), level-set advection (:366), velocity self-advection (:369-370), gravity
+ * (:373), the projection (:382).  Every call below replaces one of those steps for a
+ * caller that keeps its MAC velocity on the GPU between steps.
+ *
+ * Layouts are those of shk_projection.h: dense x-fastest float face arrays
+ * u (nx+1) ny nz, v nx (ny+1) nz, w nx ny (nz+1), and per face array one uint8
+ * activity mask of the same shape (1 = active).  The masks replace the reference's
+ * sparse activity (array3.h active()): an inactive face reads as the background 0.
+ * Results are bit-for-bit those of the reference's float/double arithmetic.
+ *
+ * Configuration keys (shk_configure): MacCormack (default 1), TrimNarrowBand
+ * (default 1), WENO (only 0 is accepted) -- macadvection3.cpp:58-60, 280-284.
+ *
+ * Every call is synchronous at return and returns an shk_status code. */
+#ifndef SHK_LIQUID_H
+#define SHK_LIQUID_H
+
+#include <stdint.h>
+
+#include "shk_projection.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* macarray_extrapolator3::extrapolate(velocity, width)
+ * (include/shiokaze/array/macarray_extrapolator3.h:43-47, array_extrapolator3.h:47-76,
+ * dilate3.h:43-97): `width` rounds; each round activates every inactive face with an
+ * active 6-neighbour and gives it their float average.  The masks are updated in place;
+ * *activated (nullable) receives the number of faces activated. */
+int shk_extrapolate_velocity(shk_ctx *ctx, float *u, float *v, float *w, uint8_t *mu, uint8_t *mv, uint8_t *mw,
+                             int width, int64_t *activated);
+
+/* macutility3_interface::constrain_velocity(solid, velocity) (src/utility/macutility3.cpp:61-92):
+ * active faces inside the nodal solid level set ((nx+1)(ny+1)(nz+1) floats, nullable
+ * for no solid) lose the velocity component into the solid; v is then clamped to
+ * the domain's floor and ceiling. */
+int shk_constrain_velocity(shk_ctx *ctx, const float *solid, float *u, float *v, float *w, const uint8_t *mu,
+                           const uint8_t *mv, const uint8_t *mw);
+
+/* macutility3_interface::compute_max_u(velocity) (macutility3.cpp:48-60): the largest
+ * cell-centred speed over cells whose six faces are active (0 if none). */
+int shk_compute_max_u(shk_ctx *ctx, const float *u, const float *v, const float *w, const uint8_t *mu,
+                      const uint8_t *mv, const uint8_t *mw, double *max_u);
+
+/* macadvection3_interface::advect_vector(u, velocity, fluid, dt) as macliquid3 calls
+ * it (macliquid3.cpp:369-370; macadvection3.cpp:46-54, 70-177): the velocity advected
+ * by itself over dt, MacCormack with the min/max limiter unless MacCormack=0, and
+ * semi-Lagrangian where the cell-centred level set `fluid` (nx ny nz) at the departed
+ * face is above -TrimNarrowBand*dx.  In place; inactive faces come back 0. */
+int shk_advect_velocity(shk_ctx *ctx, double dt, float *u, float *v, float *w, const uint8_t *mu, const uint8_t *mv,
+                        const uint8_t *mw, const float *fluid);
+
+/* `m_velocity += dt*gravity` of macliquid3::inject_external_force (macliquid3.cpp:195-210):
+ * adds (float)(dt*g) to every face and activates every face. */
+int shk_add_gravity(shk_ctx *ctx, double dt, double gx, double gy, double gz, float *u, float *v, float *w,
+                    uint8_t *mu, uint8_t *mv, uint8_t *mw);
+
+/* The activity the projection leaves (macpressuresolver3.cpp:269-283, `it.set_off()`):
+ * after an shk_project with unknowns, an active face stays active only where its area
+ * and liquid fractions of that projection are both non-zero; a projection without
+ * unknowns leaves the masks alone. */
+int shk_project_activity(shk_ctx *ctx, uint8_t *mu, uint8_t *mv, uint8_t *mw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -53,6 +53,12 @@ _SIG = {
     "shk_or_density_scale": (None, [_i, _i, _i] + [_vp] * 6 + [_i, _vp, _vp, _vp]),
     "shk_or_area_fraction2d": (None, [_i, _i, _vp, _d, _i, _vp, _vp]),
     "shk_or_curvature2d": (None, [_i, _i, _d, _vp, _i, _vp]),
+    "shk_or_extrapolate": (_i64, [_i, _i, _i, _i, _vp, _vp, _i]),
+    "shk_or_constrain_velocity": (None, [_i, _i, _i, _d, _vp] + [_vp] * 6),
+    "shk_or_compute_max_u": (_d, [_i, _i, _i] + [_vp] * 6),
+    "shk_or_advect_velocity": (None, [_i, _i, _i, _d, _d] + [_vp] * 7 + [_i, _i]),
+    "shk_or_add_gravity": (None, [_i, _i, _i, _d, _d, _d, _d] + [_vp] * 6),
+    "shk_or_project_activity": (None, [_i, _i, _i] + [_vp] * 9),
 }
 
 _lib = None
@@ -423,3 +429,93 @@ def project2d(scene: dict, dt=1.0, tol=1e-4, max_iterations=30000, mode=SHK_OR_R
     out["pressure2"] = out["pressure"].reshape(ny, nx)
     out["index_map2"] = index_map.reshape(ny, nx)
     return out
+
+
+# ---------------------------------------------------------------- liquid step (velocity half)
+# Face arrays are float32 in numpy (k, j, i) order, activity masks uint8 of the same
+# shapes; every function works on copies and returns new arrays.
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _u8(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+def _cell_dims(vel):
+    nz, ny, nx1 = np.shape(vel[0])
+    return nx1 - 1, ny, nz
+
+
+def extrapolate(vel, masks, width):
+    """macarray_extrapolator3::extrapolate(velocity, width): per face array, `width`
+    Jacobi dilation rounds (array_extrapolator3.h:47-76, dilate3.h:43-97).
+    Returns (vel, masks, newly activated count)."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    out = [_f32(v).copy() for v in vel]
+    ms = [_u8(m).copy() for m in masks]
+    total = 0
+    for d in range(3):
+        total += lib.shk_or_extrapolate(nx, ny, nz, d, _p(out[d]), _p(ms[d]), int(width))
+    return out, ms, total
+
+
+def constrain_velocity(dx, solid, vel, masks):
+    """macutility3::constrain_velocity (macutility3.cpp:61-92)."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    out = [_f32(v).copy() for v in vel]
+    ms = [_u8(m) for m in masks]
+    s = _f32(solid) if solid is not None else None
+    lib.shk_or_constrain_velocity(nx, ny, nz, float(dx), _p(s) if s is not None else None,
+                                  *[_p(v) for v in out], *[_p(m) for m in ms])
+    return out
+
+
+def compute_max_u(vel, masks):
+    """macutility3::compute_max_u (macutility3.cpp:48-60)."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    vs = [_f32(v) for v in vel]
+    ms = [_u8(m) for m in masks]
+    return float(lib.shk_or_compute_max_u(nx, ny, nz, *[_p(v) for v in vs], *[_p(m) for m in ms]))
+
+
+def advect_velocity(dx, dt, vel, masks, fluid, maccormack=True, trim=1):
+    """macadvection3::advect_vector (macadvection3.cpp:46-54, 70-177), the velocity
+    advecting itself; inactive faces come back 0."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    out = [_f32(v).copy() for v in vel]
+    ms = [_u8(m) for m in masks]
+    fl = _f32(fluid)
+    lib.shk_or_advect_velocity(nx, ny, nz, float(dx), float(dt), *[_p(v) for v in out], *[_p(m) for m in ms],
+                               _p(fl), 1 if maccormack else 0, int(trim))
+    return out
+
+
+def add_gravity(dt, gravity, vel, masks):
+    """macliquid3::inject_external_force's `m_velocity += dt*gravity`
+    (macliquid3.cpp:207-208); every face comes back active."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    out = [_f32(v).copy() for v in vel]
+    ms = [_u8(m).copy() for m in masks]
+    lib.shk_or_add_gravity(nx, ny, nz, float(dt), *[float(g) for g in gravity], *[_p(v) for v in out],
+                           *[_p(m) for m in ms])
+    return out, ms
+
+
+def project_activity(areas, rhos, masks):
+    """The projection's set_off (macpressuresolver3.cpp:269-283): a face stays active
+    only where area and rho are both non-zero."""
+    lib = load()
+    nz, ny, nx1 = np.shape(areas[0])
+    a = [_f64(x) for x in areas]
+    r = [_f64(x) for x in rhos]
+    ms = [_u8(m).copy() for m in masks]
+    lib.shk_or_project_activity(nx1 - 1, ny, nz, *[_p(x) for x in a], *[_p(x) for x in r], *[_p(m) for m in ms])
+    return ms

@@ -145,6 +145,22 @@ void shk_or_curvature2d(int nx, int ny, double dx, const double *fluid, int real
 
 int shk_or_set_threads(int n);
 
+/* the velocity half of the liquid step around the projection (shk_liquid_oracle.c;
+ * macliquid3.cpp:342-391): dense float face arrays with one uint8 activity mask each */
+int64_t shk_or_extrapolate(int nx, int ny, int nz, int dim, float *a, uint8_t *m, int count);
+void shk_or_constrain_velocity(int nx, int ny, int nz, double dx, const float *solid, float *u, float *v,
+                               float *w, const uint8_t *mu, const uint8_t *mv, const uint8_t *mw);
+double shk_or_compute_max_u(int nx, int ny, int nz, const float *u, const float *v, const float *w,
+                            const uint8_t *mu, const uint8_t *mv, const uint8_t *mw);
+void shk_or_advect_velocity(int nx, int ny, int nz, double dx, double dt, float *u, float *v, float *w,
+                            const uint8_t *mu, const uint8_t *mv, const uint8_t *mw, const float *fluid,
+                            int maccormack, int trim);
+void shk_or_add_gravity(int nx, int ny, int nz, double dt, double gx, double gy, double gz, float *u, float *v,
+                        float *w, uint8_t *mu, uint8_t *mv, uint8_t *mw);
+void shk_or_project_activity(int nx, int ny, int nz, const double *au, const double *av, const double *aw,
+                             const double *ru, const double *rv, const double *rw, uint8_t *mu, uint8_t *mv,
+                             uint8_t *mw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -103,14 +103,21 @@ _SIGNATURES = {
     "shk_nn_conv3d": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                           _vp]),
     "shk_nn_conv3d_backward": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, ctypes.c_int] + [_vp] * 4),
+    # include/shk_liquid.h
+    "shk_extrapolate_velocity": (ctypes.c_int, [_vp] * 7 + [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
+    "shk_constrain_velocity": (ctypes.c_int, [_vp] * 8),
+    "shk_compute_max_u": (ctypes.c_int, [_vp] * 7 + [ctypes.POINTER(ctypes.c_double)]),
+    "shk_advect_velocity": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 7),
+    "shk_add_gravity": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),
+    "shk_project_activity": (ctypes.c_int, [_vp] * 4),
 }
 
 _lib = None
 
 
 def header_symbols() -> list[str]:
-    """Function names declared in include/shk_projection.h."""
-    text = HEADER.read_text()
+    """Function names declared in the public headers include/*.h."""
+    text = "".join(h.read_text() for h in sorted(HEADER.parent.glob("*.h")))
     return sorted(set(re.findall(r"\b(shk_[a-z_0-9]+)\s*\(", text)))
 
 

@@ -418,6 +418,7 @@ void shk_destroy(shk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->unet) unet_destroy(c->unet);
     if (c->mg) mg_destroy(c->mg);
+    if (c->liquid) liquid_destroy(c->liquid);
     for (auto &b : c->area) b.release();
     for (auto &b : c->rho) b.release();
     for (auto &b : c->stage) b.release();
@@ -487,6 +488,15 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
     } else if (k == "WarmStart") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WarmStart '%s'", value);
         c->prm.warm_start = b;
+    } else if (k == "MacCormack") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad MacCormack '%s'", value);
+        p.maccormack = b;
+    } else if (k == "TrimNarrowBand") {
+        if (!parse_int(value, i) || i < 0 || i > 1000) return c->fail(SHK_BAD_ARG, "bad TrimNarrowBand '%s'", value);
+        p.trim_narrowband = (int)i;
+    } else if (k == "WENO") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WENO '%s'", value);
+        if (b) return c->fail(SHK_BAD_ARG, "WENO interpolation is not built (trilinear only)");
     } else if (k == "EpsFluid") {
         if (!parse_double(value, d) || d < 0) return c->fail(SHK_BAD_ARG, "bad EpsFluid '%s'", value);
         p.eps_fluid = d;
@@ -595,6 +605,7 @@ int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *c
     if (two_d && c->prm.precond == 2)
         return c->fail(SHK_BAD_ARG, "the learned preconditioner is 3D only (2D: none, jacobi, mg)");
     HIPC(c, hipSetDevice(c->device));
+    c->velocity_updated = false;
     shk_result local{};
     if (!res) res = &local;
     *res = shk_result{};
@@ -739,6 +750,7 @@ int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *c
                    c->prm.warm_start ? c->prev64.p : nullptr);
     launch_update_velocity(c->stream, d, dt, c->dx, c->pressure.p, area, rho, svel, dens, vel);
     HIPC(c, hipGetLastError());
+    c->velocity_updated = true;
     HIPC(c, hipEventRecord(c->ev[3], c->stream));
     if (c->prm.host_pointers) {
         for (int a = 0; a < 3; ++a)

@@ -17,6 +17,7 @@
 namespace shk {
 struct UnetPrecond;
 struct MgHierarchy;
+struct LiquidWork;
 }
 
 namespace shk {
@@ -65,6 +66,8 @@ struct Params {
     double mg_omega = 0.9;            // multigrid damped-Jacobi weight
     double mg_alpha = 1.5;            // multigrid coarse-correction factor
     bool warm_start = false;          // macpressuresolver3.cpp:354
+    bool maccormack = true;           // macadvection3.cpp:281
+    int trim_narrowband = 1;          // macadvection3.cpp:283
 };
 
 }  // namespace shk
@@ -134,6 +137,10 @@ struct shk_ctx {
     shk::UnetPrecond *unet = nullptr;
     // multigrid preconditioner hierarchy
     shk::MgHierarchy *mg = nullptr;
+    // scratch of the liquid-step kernels (liquid.hip)
+    shk::LiquidWork *liquid = nullptr;
+    // the last projection ran its velocity update (it had unknowns)
+    bool velocity_updated = false;
     // per-kernel timing of the Krylov loop (ProfileKernels=1)
     bool profile = false;
     std::vector<hipEvent_t> pev;
@@ -195,6 +202,7 @@ int solve_fp64(shk_ctx *c, shk_result *res);
 // learned preconditioner (unet.hip)
 void unet_destroy(UnetPrecond *u);
 void mg_destroy(MgHierarchy *h);
+void liquid_destroy(LiquidWork *w);
 int solve_mg(shk_ctx *c, shk_result *res);
 int unet_solve(shk_ctx *c, shk_result *res);
 int export_scaled(shk_ctx *c, float *rhat, float *mask, float *hx, float *hy, float *hz, float *xhat);

@@ -1,0 +1,658 @@
+// liquid.hip -- the velocity half of the liquid step around the pressure projection
+// (src/liquid/macliquid3.cpp:342-391) on gfx950: velocity extrapolation, the solid
+// constraint, max |u| for the time step, MacCormack self-advection, gravity and the
+// activity left by the projection.
+//
+// The reference keeps the MAC velocity in sparse arrays with an activity flag per
+// face.  Here the faces are dense float arrays in the caller's x-fastest layout with
+// one uint8 activity mask per axis (1 = active): HBM holds every face of a 512^3 grid
+// several times over, so a dense pass with masked reads replaces the tile bookkeeping,
+// and every kernel is one thread per face (or cell), coalesced along x.  An inactive
+// face reads as the background value 0, exactly like a sparse array's unset cell.
+//
+// Every expression rounds as the reference's (Real = float fields, vec3r float
+// vectors, float interpolation accumulators; vec.h:208-296, array_interpolator3.h:
+// 50-100): fp contraction is off and float quotients are taken in double and rounded
+// once, which equals the correctly rounded float quotient.  The results are bit-for-
+// bit those of the reference's arithmetic on the same inputs.
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+
+#include "ctx_internal.h"
+#include "shk_common.h"
+
+namespace shk {
+
+struct LiquidWork {
+    DevBuf<float> a[3], b[3], c[3], lo[3], hi[3];
+    DevBuf<uint8_t> m[3], band[3];
+    DevBuf<unsigned long long> count;
+    DevBuf<double> part;
+    std::vector<double> h_part;
+    void release() {
+        for (int d = 0; d < 3; ++d) {
+            a[d].release();
+            b[d].release();
+            c[d].release();
+            lo[d].release();
+            hi[d].release();
+            m[d].release();
+            band[d].release();
+        }
+        count.release();
+        part.release();
+    }
+};
+
+void liquid_destroy(LiquidWork *w) {
+    if (!w) return;
+    w->release();
+    delete w;
+}
+
+namespace {
+
+constexpr int kNT = 256;
+constexpr int kMaxPartials = 2048;
+
+struct FS {  // extents of one face (or cell) array
+    int nx, ny, nz;
+    __host__ __device__ int64_t n() const { return (int64_t)nx * ny * nz; }
+    __host__ __device__ int64_t at(int i, int j, int k) const {
+        return (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k);
+    }
+};
+
+__host__ __device__ inline FS face_fs(const Dims &d, int dim) {
+    return FS{d.nx + (dim == 0), d.ny + (dim == 1), d.nz + (dim == 2)};
+}
+
+struct Faces {  // the three face arrays of one MAC field and their masks
+    float *v[3];
+    const uint8_t *m[3];
+};
+struct CFaces {
+    const float *v[3];
+    const uint8_t *m[3];
+};
+
+__device__ inline float rd(const float *a, const uint8_t *m, int64_t q) { return m[q] ? a[q] : 0.0f; }
+
+__device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+inline unsigned blocks_for(int64_t n) {
+    int64_t g = (n + kNT - 1) / kNT;
+    return (unsigned)(g > 0 ? g : 1);
+}
+
+// array_interpolator3::interpolate_coef (array_interpolator3.h:50-72)
+__device__ inline void interp_coef(FS s, const double p[3], int &i, int &j, int &k, double coef[8]) {
+#pragma clang fp contract(off)
+    const double x = fmax(0.0, fmin(s.nx - 1., p[0]));
+    const double y = fmax(0.0, fmin(s.ny - 1., p[1]));
+    const double z = fmax(0.0, fmin(s.nz - 1., p[2]));
+    i = (int)fmin(x, s.nx - 2.);
+    j = (int)fmin(y, s.ny - 2.);
+    k = (int)fmin(z, s.nz - 2.);
+    coef[0] = (k + 1 - z) * (i + 1 - x) * (j + 1 - y);
+    coef[1] = (k + 1 - z) * (x - i) * (j + 1 - y);
+    coef[2] = (k + 1 - z) * (i + 1 - x) * (y - j);
+    coef[3] = (k + 1 - z) * (x - i) * (y - j);
+    coef[4] = (z - k) * (i + 1 - x) * (j + 1 - y);
+    coef[5] = (z - k) * (x - i) * (j + 1 - y);
+    coef[6] = (z - k) * (i + 1 - x) * (y - j);
+    coef[7] = (z - k) * (x - i) * (y - j);
+}
+
+__device__ inline int64_t corner(FS s, int i, int j, int k, int n) {
+    return s.at(i + (n & 1), j + ((n >> 1) & 1), k + ((n >> 2) & 1));
+}
+
+// array_interpolator3::interpolate<float>: float accumulation of the non-zero terms
+// (array_interpolator3.h:86-99); m == nullptr reads every value as stored
+__device__ inline float interp(const float *a, const uint8_t *m, FS s, const double p[3]) {
+#pragma clang fp contract(off)
+    int i, j, k;
+    double coef[8];
+    interp_coef(s, p, i, j, k, coef);
+    float value = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+        if (coef[n] != 0.0) {
+            const int64_t q = corner(s, i, j, k, n);
+            const float v = m ? rd(a, m, q) : a[q];
+            value = (float)((double)value + (double)v * coef[n]);
+        }
+    return value;
+}
+
+// macarray3::convert_to_full for one face (macarray3.h:372-393): its own component
+// and the clamped 4-face averages of the other two, each stored as float
+__device__ inline void full_velocity(const Dims &d, const CFaces &f, int dim, int i, int j, int k, float out[3]) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int ud = 0; ud < 3; ++ud) {
+        const FS s = face_fs(d, ud);
+        if (ud == dim) {
+            out[ud] = rd(f.v[ud], f.m[ud], s.at(i, j, k));
+            continue;
+        }
+        double acc = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj) {
+                int q[3] = {i - (dim == 0), j - (dim == 1), k - (dim == 2)};
+                q[dim] += ii;
+                q[ud] += jj;
+                acc += rd(f.v[ud], f.m[ud], s.at(clampi(q[0], 0, s.nx - 1), clampi(q[1], 0, s.ny - 1),
+                                                 clampi(q[2], 0, s.nz - 1)));
+            }
+        acc /= 4.0;
+        out[ud] = (float)acc;
+    }
+}
+
+// ---------------------------------------------------------------- extrapolation
+
+// One dilation round of array_extrapolator3::extrapolate (array_extrapolator3.h:47-76,
+// dilate3.h:43-97): an inactive face with active in-bounds 6-neighbours takes their
+// float average, summed in the order +x, -x, +y, -y, -z, +z; every round reads only
+// the previous round (src) and writes dst.  blockIdx.y is the axis.
+__global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, CFaces src, Faces dst, uint8_t *m0, uint8_t *m1,
+                                                            uint8_t *m2, unsigned long long *count) {
+#pragma clang fp contract(off)
+    const int dim = blockIdx.y;
+    const FS s = face_fs(d, dim);
+    const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    uint8_t *mo = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
+    __shared__ unsigned int block_count;
+    if (threadIdx.x == 0) block_count = 0;
+    __syncthreads();
+    if (q < s.n()) {
+        const float *a = src.v[dim];
+        const uint8_t *m = src.m[dim];
+        float val = a[q];
+        uint8_t act = m[q];
+        if (!act) {
+            const int i = (int)(q % s.nx), j = (int)((q / s.nx) % s.ny), k = (int)(q / ((int64_t)s.nx * s.ny));
+            float sum = 0.0f;
+            int weight = 0;
+            const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                const int ii = i + di[t], jj = j + dj[t], kk = k + dk[t];
+                if (ii < 0 || jj < 0 || kk < 0 || ii >= s.nx || jj >= s.ny || kk >= s.nz) continue;
+                const int64_t e = s.at(ii, jj, kk);
+                if (m[e]) {
+                    sum += a[e];
+                    ++weight;
+                }
+            }
+            if (weight) {
+                val = (float)((double)sum / (double)weight);
+                act = 1;
+                atomicAdd(&block_count, 1u);
+            }
+        }
+        dst.v[dim][q] = val;
+        mo[q] = act;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && block_count) atomicAdd(count, (unsigned long long)block_count);
+}
+
+// ---------------------------------------------------------------- max |u|
+
+// macutility3::compute_max_u (macutility3.cpp:48-60): over cells with all six faces
+// active, the squared length of the float cell velocity (float squares summed in
+// double, vec.h:272-276); one partial max per block, the host takes the sqrt
+__global__ __launch_bounds__(kNT) void k_max_u2(Dims d, CFaces f, double *part) {
+#pragma clang fp contract(off)
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    double best = 0.0;
+    for (int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x; c < G; c += (int64_t)gridDim.x * kNT) {
+        const int i = (int)(c % d.nx), j = (int)((c / d.nx) % d.ny), k = (int)(c / ((int64_t)d.nx * d.ny));
+        double n2 = 0.0;
+        bool valid = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const FS s = face_fs(d, a);
+            const int64_t lo = s.at(i, j, k), hi = s.at(i + (a == 0), j + (a == 1), k + (a == 2));
+            if (!(f.m[a][lo] && f.m[a][hi])) {
+                valid = false;
+                break;
+            }
+            double value = 0.0;
+            value += f.v[a][lo];
+            value += f.v[a][hi];
+            const float cv = (float)(0.5 * value);
+            n2 += (double)(cv * cv);
+        }
+        if (valid) best = fmax(best, n2);
+    }
+    __shared__ double lds[kNT / 64];
+    for (int o = 32; o > 0; o >>= 1) best = fmax(best, __shfl_down(best, o));
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = lds[0];
+        for (int w = 1; w < kNT / 64; ++w) b = fmax(b, lds[w]);
+        part[blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------- solid constraint
+
+// macutility3::constrain_velocity (macutility3.cpp:61-92) for the faces of axis DIM:
+// an active face inside the solid whose interpolated velocity (from the snapshot
+// `save`) points into it loses the normal part; then the y axis's floor and ceiling
+// rule.  `solid` is the nodal level set ((nx+1)(ny+1)(nz+1)).
+template <int DIM>
+__global__ __launch_bounds__(kNT) void k_constrain(Dims d, const float *__restrict__ solid, double dx, CFaces save,
+                                                   float *__restrict__ vel, const uint8_t *__restrict__ mask) {
+#pragma clang fp contract(off)
+    const FS s = face_fs(d, DIM);
+    const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (q >= s.n() || !mask[q]) return;
+    const int i = (int)(q % s.nx), j = (int)((q / s.nx) % s.ny), k = (int)(q / ((int64_t)s.nx * s.ny));
+    float out = vel[q];
+    bool changed = false;
+    if (solid) {
+        const FS ns{d.nx + 1, d.ny + 1, d.nz + 1};
+        const double p[3] = {i + 0.5 * (DIM != 0), j + 0.5 * (DIM != 1), k + 0.5 * (DIM != 2)};
+        if (interp(solid, nullptr, ns, p) < 0.0f) {
+            // array_derivative3::derivative<float>: the trilinear gradient in index units
+            int ci, cj, ck;
+            double w[8];
+            interp_coef(ns, p, ci, cj, ck, w);  // clamps and cell as derivative_interpolate_coef
+            const double x = fmax(0.0, fmin(ns.nx - 1., p[0]));
+            const double y = fmax(0.0, fmin(ns.ny - 1., p[1]));
+            const double z = fmax(0.0, fmin(ns.nz - 1., p[2]));
+            const double gx[8] = {-(ck + 1 - z) * (cj + 1 - y), (ck + 1 - z) * (cj + 1 - y),
+                                  -(ck + 1 - z) * (y - cj),     (ck + 1 - z) * (y - cj),
+                                  -(z - ck) * (cj + 1 - y),     (z - ck) * (cj + 1 - y),
+                                  -(z - ck) * (y - cj),         (z - ck) * (y - cj)};
+            const double gy[8] = {-(ck + 1 - z) * (ci + 1 - x), -(ck + 1 - z) * (x - ci),
+                                  (ck + 1 - z) * (ci + 1 - x),  (ck + 1 - z) * (x - ci),
+                                  -(z - ck) * (ci + 1 - x),     -(z - ck) * (x - ci),
+                                  (z - ck) * (ci + 1 - x),      (z - ck) * (x - ci)};
+            const double gz[8] = {-(ci + 1 - x) * (cj + 1 - y), -(x - ci) * (cj + 1 - y),
+                                  -(ci + 1 - x) * (y - cj),     -(x - ci) * (y - cj),
+                                  (ci + 1 - x) * (cj + 1 - y),  (x - ci) * (cj + 1 - y),
+                                  (ci + 1 - x) * (y - cj),      (x - ci) * (y - cj)};
+            float g[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const double v = (double)solid[corner(ns, ci, cj, ck, n)];
+                g[0] = (float)((double)g[0] + gx[n] * v);
+                g[1] = (float)((double)g[1] + gy[n] * v);
+                g[2] = (float)((double)g[2] + gz[n] * v);
+            }
+            const double nrm[3] = {(double)g[0] / dx, (double)g[1] / dx, (double)g[2] / dx};
+            double n2 = 0.0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) n2 += nrm[a] * nrm[a];
+            if (n2 != 0.0) {
+                double uu[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double pp[3] = {p[0] - 0.5 * (a != 0), p[1] - 0.5 * (a != 1), p[2] - 0.5 * (a != 2)};
+                    uu[a] = (double)interp(save.v[a], save.m[a], face_fs(d, a), pp);
+                }
+                double un = 0.0;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) un += uu[a] * nrm[a];
+                if (un < 0.0) {
+                    out = (float)(uu[DIM] - nrm[DIM] * un);
+                    changed = true;
+                }
+            }
+        }
+    }
+    if (DIM == 1) {
+        if (j == 0 && out < 0.0f) {
+            out = 0.0f;
+            changed = true;
+        }
+        if (j == d.ny && out > 0.0f) {
+            out = 0.0f;
+            changed = true;
+        }
+    }
+    if (changed) vel[q] = out;
+}
+
+// ---------------------------------------------------------------- advection
+
+// Semi-Lagrangian advection of the faces of axis DIM (macadvection3.cpp:70-138) with
+// the departure velocity of `src` itself: out = interpolation of src at the departure
+// point (the vec3r arithmetic of `dt*u/m_dx` rounds to float twice).  With lo != 0 it
+// also keeps the MacCormack limiter's bounds (min and max of the 8 corners of the
+// departure point computed in double; the max starts at DBL_MIN, which rounds to
+// 0.0f) and the narrow-band flag (fluid at the departed face centre above -trim dx).
+// With fin != 0 it is the backward pass and combines the result straight into the
+// final MacCormack value (macadvection3.cpp:150-163).
+template <int DIM>
+__global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, double dx, double dt, CFaces src, float *out,
+                                                         float *lo, float *hi, uint8_t *band, const float *fluid,
+                                                         int trim, const float *fin_vin, const float *fin_lo,
+                                                         const float *fin_hi, const uint8_t *fin_band,
+                                                         const float *fin_v0, float *fin_out) {
+#pragma clang fp contract(off)
+    const FS s = face_fs(d, DIM);
+    const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (q >= s.n()) return;
+    if (!src.m[DIM][q]) {
+        if (out) out[q] = 0.0f;
+        if (fin_out) fin_out[q] = 0.0f;
+        return;
+    }
+    const int i = (int)(q % s.nx), j = (int)((q / s.nx) % s.ny), k = (int)(q / ((int64_t)s.nx * s.ny));
+    float uf[3];
+    full_velocity(d, src, DIM, i, j, k, uf);
+    const bool empty = uf[0] == 0.0f && uf[1] == 0.0f && uf[2] == 0.0f;
+    const float here = src.v[DIM][q];
+    float value = here;
+    if (!empty) {
+        const int ijk[3] = {i, j, k};
+        double p[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float t = (float)((double)uf[a] * dt);
+            t = (float)((double)t / dx);
+            p[a] = (double)ijk[a] - (double)t;
+        }
+        value = interp(src.v[DIM], src.m[DIM], s, p);
+    }
+    if (out) out[q] = value;
+    if (lo) {
+        double face_p[3] = {i + 0.5 * (DIM != 0), j + 0.5 * (DIM != 1), k + 0.5 * (DIM != 2)};
+        if (!empty) {
+            const int ijk[3] = {i, j, k};
+            double p[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const double disp = dt * (double)uf[a] / dx;
+                p[a] = (double)ijk[a] - disp;
+                face_p[a] = face_p[a] - disp;
+            }
+            int ci, cj, ck;
+            double coef[8];
+            interp_coef(s, p, ci, cj, ck, coef);
+            double mn = DBL_MAX, mx = DBL_MIN;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                const double v = rd(src.v[DIM], src.m[DIM], corner(s, ci, cj, ck, n));
+                mn = fmin(mn, v);
+                mx = fmax(mx, v);
+            }
+            lo[q] = (float)mn;
+            hi[q] = (float)mx;
+        } else {
+            lo[q] = here;
+            hi[q] = here;
+        }
+        const double fp[3] = {face_p[0] - 0.5, face_p[1] - 0.5, face_p[2] - 0.5};
+        band[q] = (double)interp(fluid, nullptr, FS{d.nx, d.ny, d.nz}, fp) > -dx * (double)trim;
+    }
+    if (fin_out) {
+        float r;
+        if (fin_band[q]) {
+            r = fin_v0[q];
+        } else {
+            const double mn = fin_lo[q], mx = fin_hi[q];
+            const double vel0 = fin_v0[q];
+            const double corr = 0.5 * ((double)fin_vin[q] - (double)value);
+            if (vel0 + corr < mn)
+                r = (float)mn;
+            else if (vel0 + corr > mx)
+                r = (float)mx;
+            else
+                r = (float)(vel0 + corr);
+        }
+        fin_out[q] = r;
+    }
+}
+
+// ---------------------------------------------------------------- gravity, activity
+
+// `m_velocity += dt*gravity` (macliquid3.cpp:207-208): array3::operator+= touches every
+// cell and array3::increment activates an inactive one with background + value
+// (array3.h:647-655, 917-921); the vec3d increment is converted to float first
+__global__ __launch_bounds__(kNT) void k_gravity(Dims d, float g0, float g1, float g2, Faces f, uint8_t *m0,
+                                                 uint8_t *m1, uint8_t *m2) {
+#pragma clang fp contract(off)
+    const int dim = blockIdx.y;
+    const FS s = face_fs(d, dim);
+    const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (q >= s.n()) return;
+    const float add = dim == 0 ? g0 : (dim == 1 ? g1 : g2);
+    uint8_t *m = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
+    float *v = f.v[dim];
+    v[q] = m[q] ? v[q] + add : 0.0f + add;
+    m[q] = 1;
+}
+
+// The projection's set_off (macpressuresolver3.cpp:269-283): an active face stays
+// active only where both its area and liquid fractions are non-zero.
+__global__ __launch_bounds__(kNT) void k_project_activity(Dims d, const float *a0, const float *a1, const float *a2,
+                                                          const float *r0, const float *r1, const float *r2,
+                                                          uint8_t *m0, uint8_t *m1, uint8_t *m2) {
+    const int dim = blockIdx.y;
+    const FS s = face_fs(d, dim);
+    const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (q >= s.n()) return;
+    const float *a = dim == 0 ? a0 : (dim == 1 ? a1 : a2);
+    const float *r = dim == 0 ? r0 : (dim == 1 ? r1 : r2);
+    uint8_t *m = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
+    if (m[q] && !(a[q] != 0.0f && r[q] != 0.0f)) m[q] = 0;
+}
+
+int64_t max_faces(const Dims &d) {
+    int64_t n = 0;
+    for (int a = 0; a < 3; ++a) n = face_fs(d, a).n() > n ? face_fs(d, a).n() : n;
+    return n;
+}
+
+int check3d(shk_ctx *c, const char *fn) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    if (c->dims.bx != 8) return c->fail(SHK_BAD_ARG, "%s needs a 3D context", fn);
+    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "%s takes device pointers", fn);
+    // trilinear interpolation needs two samples per axis (array_interpolator3.h:54-56)
+    if (c->dims.nx < 2 || c->dims.ny < 2 || c->dims.nz < 2)
+        return c->fail(SHK_BAD_ARG, "%s needs at least 2 cells per axis", fn);
+    if (hipSetDevice(c->device) != hipSuccess) return c->fail(SHK_HIP_ERROR, "hipSetDevice failed");
+    return SHK_OK;
+}
+
+LiquidWork *work(shk_ctx *c) {
+    if (!c->liquid) c->liquid = new LiquidWork();
+    return c->liquid;
+}
+
+}  // namespace
+}  // namespace shk
+
+using namespace shk;
+
+extern "C" {
+
+int shk_extrapolate_velocity(shk_ctx *c, float *u, float *v, float *w, uint8_t *mu, uint8_t *mv, uint8_t *mw,
+                             int width, int64_t *activated) {
+    int rc;
+    if ((rc = check3d(c, "shk_extrapolate_velocity"))) return rc;
+    if (!u || !v || !w || !mu || !mv || !mw || width < 0) return c->fail(SHK_BAD_ARG, "null array or negative width");
+    const Dims d = c->dims;
+    LiquidWork *lw = work(c);
+    float *user[3] = {u, v, w};
+    uint8_t *um[3] = {mu, mv, mw};
+    if (width > 0)
+        for (int a = 0; a < 3; ++a) {
+            HIPC(c, lw->a[a].ensure(face_fs(d, a).n()));
+            HIPC(c, lw->m[a].ensure(face_fs(d, a).n()));
+        }
+    HIPC(c, lw->count.ensure(1));
+    HIPC(c, hipMemsetAsync(lw->count.p, 0, sizeof(unsigned long long), c->stream));
+    for (int r = 0; r < width; ++r) {
+        const bool from_user = (r % 2) == 0;
+        CFaces src;
+        Faces dst;
+        uint8_t *dm[3];
+        for (int a = 0; a < 3; ++a) {
+            src.v[a] = from_user ? user[a] : lw->a[a].p;
+            src.m[a] = from_user ? um[a] : lw->m[a].p;
+            dst.v[a] = from_user ? lw->a[a].p : user[a];
+            dm[a] = from_user ? lw->m[a].p : um[a];
+            dst.m[a] = dm[a];
+        }
+        hipLaunchKernelGGL(k_extrapolate_round, dim3(blocks_for(max_faces(d)), 3), dim3(kNT), 0, c->stream, d, src,
+                           dst, dm[0], dm[1], dm[2], lw->count.p);
+        HIPC(c, hipGetLastError());
+    }
+    if (width % 2 == 1)
+        for (int a = 0; a < 3; ++a) {
+            const size_t n = (size_t)face_fs(d, a).n();
+            HIPC(c, hipMemcpyAsync(user[a], lw->a[a].p, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+            HIPC(c, hipMemcpyAsync(um[a], lw->m[a].p, n, hipMemcpyDeviceToDevice, c->stream));
+        }
+    unsigned long long cnt = 0;
+    HIPC(c, hipMemcpyAsync(&cnt, lw->count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    if (activated) *activated = (int64_t)cnt;
+    return SHK_OK;
+}
+
+int shk_constrain_velocity(shk_ctx *c, const float *solid, float *u, float *v, float *w, const uint8_t *mu,
+                           const uint8_t *mv, const uint8_t *mw) {
+    int rc;
+    if ((rc = check3d(c, "shk_constrain_velocity"))) return rc;
+    if (!u || !v || !w || !mu || !mv || !mw) return c->fail(SHK_BAD_ARG, "null array");
+    const Dims d = c->dims;
+    LiquidWork *lw = work(c);
+    float *user[3] = {u, v, w};
+    const uint8_t *um[3] = {mu, mv, mw};
+    CFaces save;
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = (size_t)face_fs(d, a).n();
+        HIPC(c, lw->a[a].ensure(n));
+        HIPC(c, hipMemcpyAsync(lw->a[a].p, user[a], sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+        save.v[a] = lw->a[a].p;
+        save.m[a] = um[a];
+    }
+    hipLaunchKernelGGL(k_constrain<0>, dim3(blocks_for(face_fs(d, 0).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+                       save, u, mu);
+    hipLaunchKernelGGL(k_constrain<1>, dim3(blocks_for(face_fs(d, 1).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+                       save, v, mv);
+    hipLaunchKernelGGL(k_constrain<2>, dim3(blocks_for(face_fs(d, 2).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+                       save, w, mw);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_compute_max_u(shk_ctx *c, const float *u, const float *v, const float *w, const uint8_t *mu,
+                      const uint8_t *mv, const uint8_t *mw, double *max_u) {
+    int rc;
+    if ((rc = check3d(c, "shk_compute_max_u"))) return rc;
+    if (!u || !v || !w || !mu || !mv || !mw || !max_u) return c->fail(SHK_BAD_ARG, "null argument");
+    const Dims d = c->dims;
+    LiquidWork *lw = work(c);
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    const int blocks = (int)((G + kNT - 1) / kNT < kMaxPartials ? (G + kNT - 1) / kNT : kMaxPartials);
+    HIPC(c, lw->part.ensure(kMaxPartials));
+    lw->h_part.resize(blocks);
+    const CFaces f{{u, v, w}, {mu, mv, mw}};
+    hipLaunchKernelGGL(k_max_u2, dim3(blocks), dim3(kNT), 0, c->stream, d, f, lw->part.p);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipMemcpyAsync(lw->h_part.data(), lw->part.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    double best = 0.0;
+    for (double p : lw->h_part) best = p > best ? p : best;
+    *max_u = std::sqrt(best);
+    return SHK_OK;
+}
+
+int shk_advect_velocity(shk_ctx *c, double dt, float *u, float *v, float *w, const uint8_t *mu, const uint8_t *mv,
+                        const uint8_t *mw, const float *fluid) {
+    int rc;
+    if ((rc = check3d(c, "shk_advect_velocity"))) return rc;
+    if (!u || !v || !w || !mu || !mv || !mw || !fluid) return c->fail(SHK_BAD_ARG, "null array");
+    if (!std::isfinite(dt)) return c->fail(SHK_BAD_ARG, "dt must be finite");
+    const Dims d = c->dims;
+    LiquidWork *lw = work(c);
+    const bool mc = c->prm.maccormack;
+    float *user[3] = {u, v, w};
+    const uint8_t *um[3] = {mu, mv, mw};
+    CFaces vin, v0;
+    for (int a = 0; a < 3; ++a) {
+        const size_t n = (size_t)face_fs(d, a).n();
+        HIPC(c, lw->a[a].ensure(n));
+        HIPC(c, hipMemcpyAsync(lw->a[a].p, user[a], sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
+        vin.v[a] = lw->a[a].p;
+        vin.m[a] = um[a];
+        if (mc) {
+            HIPC(c, lw->b[a].ensure(n));
+            HIPC(c, lw->lo[a].ensure(n));
+            HIPC(c, lw->hi[a].ensure(n));
+            HIPC(c, lw->band[a].ensure(n));
+        }
+        v0.v[a] = mc ? lw->b[a].p : user[a];
+        v0.m[a] = um[a];
+    }
+    const int trim = c->prm.trim_narrowband;
+#define SHK_SL(DIM, SRC, OUT, LO, HI, BAND, DT, FV, FL, FH, FB, F0, FO)                                            \
+    hipLaunchKernelGGL(k_semi_lagrangian<DIM>, dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, c->stream, d, \
+                       c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO)
+    // forward pass: v0 (the result itself without MacCormack) and the limiter data
+    SHK_SL(0, vin, const_cast<float *>(v0.v[0]), mc ? lw->lo[0].p : nullptr, mc ? lw->hi[0].p : nullptr,
+           mc ? lw->band[0].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    SHK_SL(1, vin, const_cast<float *>(v0.v[1]), mc ? lw->lo[1].p : nullptr, mc ? lw->hi[1].p : nullptr,
+           mc ? lw->band[1].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    SHK_SL(2, vin, const_cast<float *>(v0.v[2]), mc ? lw->lo[2].p : nullptr, mc ? lw->hi[2].p : nullptr,
+           mc ? lw->band[2].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (mc) {
+        // backward pass of v0 with its own velocity and -dt, combined into the result
+        SHK_SL(0, v0, nullptr, nullptr, nullptr, nullptr, -dt, vin.v[0], lw->lo[0].p, lw->hi[0].p, lw->band[0].p,
+               v0.v[0], u);
+        SHK_SL(1, v0, nullptr, nullptr, nullptr, nullptr, -dt, vin.v[1], lw->lo[1].p, lw->hi[1].p, lw->band[1].p,
+               v0.v[1], v);
+        SHK_SL(2, v0, nullptr, nullptr, nullptr, nullptr, -dt, vin.v[2], lw->lo[2].p, lw->hi[2].p, lw->band[2].p,
+               v0.v[2], w);
+    }
+#undef SHK_SL
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_add_gravity(shk_ctx *c, double dt, double gx, double gy, double gz, float *u, float *v, float *w,
+                    uint8_t *mu, uint8_t *mv, uint8_t *mw) {
+    int rc;
+    if ((rc = check3d(c, "shk_add_gravity"))) return rc;
+    if (!u || !v || !w || !mu || !mv || !mw) return c->fail(SHK_BAD_ARG, "null array");
+    const Dims d = c->dims;
+    const Faces f{{u, v, w}, {mu, mv, mw}};
+    hipLaunchKernelGGL(k_gravity, dim3(blocks_for(max_faces(d)), 3), dim3(kNT), 0, c->stream, d, (float)(dt * gx),
+                       (float)(dt * gy), (float)(dt * gz), f, mu, mv, mw);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_project_activity(shk_ctx *c, uint8_t *mu, uint8_t *mv, uint8_t *mw) {
+    int rc;
+    if ((rc = check3d(c, "shk_project_activity"))) return rc;
+    if (!mu || !mv || !mw) return c->fail(SHK_BAD_ARG, "null mask");
+    if (!c->velocity_updated) return SHK_OK;  // no unknowns: the projection left the velocity alone
+    const Dims d = c->dims;
+    hipLaunchKernelGGL(k_project_activity, dim3(blocks_for(max_faces(d)), 3), dim3(kNT), 0, c->stream, d,
+                       c->area[0].p, c->area[1].p, c->area[2].p, c->rho[0].p, c->rho[1].p, c->rho[2].p, mu, mv, mw);
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+}  // extern "C"

@@ -97,6 +97,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     // disjoint 4-bank groups of the 64-bank LDS
     constexpr int VS = (G8 & 1) ? G8 : G8 + 1;
     constexpr int KPC = (27 * CC + 31) / 32;
+    constexpr int PF = MTB <= 2 ? 4 : (MTB <= 4 ? 2 : 1);
     __shared__ uint4 lds[kHalo * VS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
@@ -134,6 +135,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
             lds[vox * VS + part] = v;
         }
         __syncthreads();
+        // weight fragments run PF chunks ahead of the MFMAs that use them
+        const uint4 *abase = packed + ((int64_t)(mg * MTB) * kchunks + (int64_t)stage * KPC) * 64 + lane;
+        Frag ar[PF][MTB];
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+#pragma unroll
+            for (int mt = 0; mt < MTB; ++mt)
+                if (p < KPC) ar[p][mt].u = abase[((int64_t)mt * kchunks + p) * 64];
+#pragma unroll
         for (int kc = 0; kc < KPC; ++kc) {
             const int k0 = kc * 32 + kq * 8;
             Frag b[4];
@@ -148,15 +158,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt) b[nt].u = make_uint4(0u, 0u, 0u, 0u);
             }
-            const int kcg = stage * KPC + kc;
+            Frag a[MTB];
 #pragma unroll
             for (int mt = 0; mt < MTB; ++mt) {
-                Frag a;
-                a.u = packed[((int64_t)(mg * MTB + mt) * kchunks + kcg) * 64 + lane];
+                a[mt] = ar[kc % PF][mt];
+                if (kc + PF < KPC) ar[kc % PF][mt].u = abase[((int64_t)mt * kchunks + kc + PF) * 64];
+            }
+#pragma unroll
+            for (int mt = 0; mt < MTB; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < 4; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b[nt].v, acc[mt][nt], 0, 0, 0);
-            }
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[mt].v, b[nt].v, acc[mt][nt], 0, 0, 0);
         }
     }
     // lane holds D[co = (mg*MTB + mt)*16 + 4*kq + j][voxel n of row nt]

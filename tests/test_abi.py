@@ -1,5 +1,5 @@
 """The C-ABI library without a GPU: it loads, exports every function that
-include/shk_projection.h declares, and its argument checks and device probe fail
+include/*.h declares, and its argument checks and device probe fail
 cleanly (no compute call is made here)."""
 from __future__ import annotations
 

@@ -1,0 +1,264 @@
+"""GPU parity of the velocity half of the liquid step (include/shk_liquid.h; the
+reference's macliquid3.cpp:342-391) through the C-ABI against the CPU oracle
+(oracle/shk_liquid_oracle.c).  Every stage is bit-exact: the same float/double
+rounding points, no reductions whose order matters (max |u| is a max)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from gpu_helpers import log
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype=None):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a if dtype is None else a.astype(dtype))).cuda()
+
+
+def _h(t):
+    return t.cpu().numpy()
+
+
+def _solver(n, dx, config=None):
+    from shiokaze_amd import MacLiquidVelocity, MacPressureSolver3
+    ps = MacPressureSolver3(n, dx, device=0, config=config or {})
+    return ps, MacLiquidVelocity(ps)
+
+
+def _active_masks(sc, O):
+    """The activity a projection leaves: faces with non-zero area and liquid fraction."""
+    areas = O.area_fraction(sc["solid"])
+    rhos = O.fluid_fraction(sc["fluid"])
+    return [((a != 0) & (r != 0)).astype(np.uint8) for a, r in zip(areas, rhos)]
+
+
+def _velocity(shapes, rng, scale=1.0):
+    return [(rng.uniform(-scale, scale, s)).astype(np.float32) for s in shapes]
+
+
+def _scene(scenes, kind, n, seed=0):
+    from shiokaze_amd import scene as S
+    return scenes(getattr(S, kind), n, seed)
+
+
+CASES = [("POOL_SPHERE", 24, 0), ("IRREGULAR", 32, 1), ("IRREGULAR", (40, 28, 36), 3)]
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES)
+@pytest.mark.parametrize("width", [1, 2, 4])
+def test_extrapolate_matches_oracle(scenes, oracle, kind, n, seed, width):
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(seed + 10 * width)
+    masks = _active_masks(sc, O)
+    vel = _velocity([m.shape for m in masks], rng)
+    want, want_m, want_n = O.extrapolate(vel, masks, width)
+    ps, lv = _solver(shape, sc["dx"])
+    with ps:
+        dv = [_t(v) for v in vel]
+        dm = [_t(m) for m in masks]
+        got_n = lv.extrapolate(dv, dm, width)
+        log(f"[gpu] extrapolate {kind} {shape} width {width}: activated {got_n} (oracle {want_n})")
+        assert got_n == want_n
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dm[d]), want_m[d])
+            act = want_m[d] == 1
+            np.testing.assert_array_equal(_h(dv[d])[act], want[d][act])
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES)
+def test_constrain_matches_oracle(scenes, oracle, kind, n, seed):
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(seed + 1)
+    masks = [(rng.random(m.shape) < 0.9).astype(np.uint8) for m in _active_masks(sc, O)]   # walls included
+    vel = _velocity([m.shape for m in masks], rng)
+    want = O.constrain_velocity(sc["dx"], sc["solid"], vel, masks)
+    ps, lv = _solver(shape, sc["dx"])
+    with ps:
+        dv = [_t(v) for v in vel]
+        dm = [_t(m) for m in masks]
+        lv.constrain(_t(sc["solid"]), dv, dm)
+        changed = 0
+        for d in range(3):
+            got = _h(dv[d])
+            act = masks[d] == 1
+            np.testing.assert_array_equal(got[act], want[d][act])
+            np.testing.assert_array_equal(got[~act], vel[d][~act])   # inactive faces are not touched
+            changed += int((want[d] != vel[d]).sum())
+        log(f"[gpu] constrain {kind} {shape}: {changed} faces changed, bit-exact")
+        assert changed > 0
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES)
+def test_compute_max_u_matches_oracle(scenes, oracle, kind, n, seed):
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(seed + 2)
+    masks = _active_masks(sc, O)
+    vel = _velocity([m.shape for m in masks], rng, 3.0)
+    want = O.compute_max_u(vel, masks)
+    ps, lv = _solver(shape, sc["dx"])
+    with ps:
+        got = lv.compute_max_u([_t(v) for v in vel], [_t(m) for m in masks])
+        log(f"[gpu] max_u {kind} {shape}: {got!r} vs {want!r}")
+        assert got == want
+        none = lv.compute_max_u([_t(v) for v in vel], [_t(np.zeros_like(m)) for m in masks])
+        assert none == 0.0
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES)
+@pytest.mark.parametrize("maccormack,trim", [(True, 1), (False, 1), (True, 3)])
+def test_advect_matches_oracle(scenes, oracle, kind, n, seed, maccormack, trim):
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(seed + 3)
+    masks = _active_masks(sc, O)
+    _, masks, _ = O.extrapolate(_velocity([m.shape for m in masks], rng), masks, 3)
+    vel = _velocity([m.shape for m in masks], rng, 2.0)
+    dt = 1.5 * sc["dx"] / 2.0   # departures up to 1.5 cells
+    want = O.advect_velocity(sc["dx"], dt, vel, masks, sc["fluid"], maccormack=maccormack, trim=trim)
+    ps, lv = _solver(shape, sc["dx"], {"MacCormack": maccormack, "TrimNarrowBand": trim})
+    with ps:
+        dv = [_t(v) for v in vel]
+        lv.advect(dt, dv, [_t(m) for m in masks], _t(sc["fluid"]))
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dv[d]), want[d])
+        log(f"[gpu] advect {kind} {shape} maccormack={maccormack} trim={trim}: bit-exact")
+
+
+def test_gravity_and_projection_activity_match_oracle(scenes, oracle):
+    sc = _scene(scenes, "IRREGULAR", 32, 1)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(7)
+    masks = [(rng.random(m.shape) < 0.5).astype(np.uint8) for m in _active_masks(sc, O)]
+    vel = _velocity([m.shape for m in masks], rng)
+    dt = 0.004
+    want, want_m = O.add_gravity(dt, (0.0, -9.8, 0.0), vel, masks)
+    ps, lv = _solver(shape, sc["dx"], {"Residual": 1e-6})
+    with ps:
+        dv = [_t(v) for v in vel]
+        dm = [_t(m) for m in masks]
+        lv.add_gravity(dt, dv, dm)
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dv[d]), want[d])
+            np.testing.assert_array_equal(_h(dm[d]), want_m[d])
+        svel = [_t(np.zeros(m.shape, np.float32)) for m in masks]
+        res = ps.project(dt, dv, _t(sc["solid"]), svel, _t(sc["fluid"]))
+        assert res["unknowns"] > 0
+        lv.project_activity(dm)
+        fr = ps.export_fractions()
+        areas = [fr[f"area{a}"].cpu().numpy().astype(np.float64) for a in range(3)]
+        rhos = [fr[f"rho{a}"].cpu().numpy().astype(np.float64) for a in range(3)]
+        want_act = O.project_activity(areas, rhos, want_m)
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dm[d]), want_act[d])
+        log(f"[gpu] gravity + projection activity: {sum(int(_h(m).sum()) for m in dm)} active faces")
+
+
+def test_liquid_steps_stagewise_against_oracle(scenes, oracle):
+    """Three steps of macliquid3's velocity half with the projection; every stage of
+    every step is checked bit-for-bit against the oracle applied to the GPU's state."""
+    from shiokaze_amd import scene as S
+    O = oracle
+    sc = scenes(S.POOL_SPHERE, 32, 0)
+    shape = tuple(sc["fluid"].shape[::-1])
+    dx = sc["dx"]
+    masks = _active_masks(sc, O)
+    rng = np.random.default_rng(11)
+    vel = [np.where(m == 1, v, 0).astype(np.float32) for m, v in zip(masks, _velocity([m.shape for m in masks], rng))]
+    ps, lv = _solver(shape, dx, {"Residual": 1e-6, "Preconditioner": "mg", "Precision": "mixed"})
+    with ps:
+        dv = [_t(v) for v in vel]
+        dm = [_t(m) for m in masks]
+        solid, fluid = _t(sc["solid"]), _t(sc["fluid"])
+        svel = [_t(np.zeros(m.shape, np.float32)) for m in masks]
+        for step in range(3):
+            hv, hm = [_h(v) for v in dv], [_h(m) for m in dm]
+            max_u = lv.compute_max_u(dv, dm)
+            assert max_u == O.compute_max_u(hv, hm)
+            dt = lv.timestepper.advance(max_u, dx)
+            width = int(lv.extend_width + lv.timestepper.current_cfl)
+            ev, em, _ = O.extrapolate(hv, hm, width)
+            lv.extrapolate(dv, dm, width)
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dm[d]), em[d])
+                np.testing.assert_array_equal(_h(dv[d])[em[d] == 1], ev[d][em[d] == 1])
+            hv, hm = [_h(v) for v in dv], [_h(m) for m in dm]
+            cv = O.constrain_velocity(dx, sc["solid"], hv, hm)
+            lv.constrain(solid, dv, dm)
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d])[hm[d] == 1], cv[d][hm[d] == 1])
+            hv = [_h(v) for v in dv]
+            av = O.advect_velocity(dx, dt, hv, hm, sc["fluid"])
+            lv.advect(dt, dv, dm, fluid)
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d]), av[d])
+            gv, gm = O.add_gravity(dt, lv.gravity, av, hm)
+            lv.add_gravity(dt, dv, dm)
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d]), gv[d])
+                np.testing.assert_array_equal(_h(dm[d]), gm[d])
+            res = ps.project(dt, dv, solid, svel, fluid)
+            lv.project_activity(dm)
+            log(f"[gpu] step {step}: dt {dt:.4g} CFL {lv.timestepper.current_cfl:.3f} width {width} "
+                f"its {res['count']} relres {res['true_relres_inf']:.2e}")
+            assert res["status"] == 0 and res["true_relres_inf"] < 1e-5
+            assert max(int(_h(m).sum()) for m in dm) > 0
+
+
+def test_step_method_runs_and_keeps_the_activity_consistent(scenes, oracle):
+    from shiokaze_amd import scene as S
+    O = oracle
+    sc = scenes(S.IRREGULAR, 32, 1)
+    shape = tuple(sc["fluid"].shape[::-1])
+    masks = _active_masks(sc, O)
+    ps, lv = _solver(shape, sc["dx"], {"Residual": 1e-6, "Precision": "mixed", "Preconditioner": "mg"})
+    with ps:
+        dv = [_t(np.zeros(m.shape, np.float32)) for m in masks]
+        dm = [_t(m) for m in masks]
+        svel = [_t(np.zeros(m.shape, np.float32)) for m in masks]
+        for step in range(4):
+            res = lv.step(dv, dm, _t(sc["solid"]), svel, _t(sc["fluid"]))
+            log(f"[gpu] step() {step}: dt {res['dt']:.4g} max_u {res['max_u']:.3f} width {res['extend_width']} "
+                f"extended {res['extended']} its {res['count']}")
+            assert res["status"] == 0
+        fr = ps.export_fractions()
+        for d in range(3):
+            a = fr[f"area{d}"].cpu().numpy()
+            r = fr[f"rho{d}"].cpu().numpy()
+            m = _h(dm[d]).astype(bool)
+            assert np.all(m == ((a != 0) & (r != 0)))   # after gravity every face was active
+            assert np.all(_h(dv[d])[~m] == 0.0)         # and the projection zeroed what it switched off
+
+
+def test_liquid_calls_refuse_2d_and_host_pointers():
+    import ctypes
+
+    import torch
+
+    from shiokaze_amd import MacLiquidVelocity, MacPressureSolver2, MacPressureSolver3, ShkError
+    from shiokaze_amd import _lib
+    ps2 = MacPressureSolver2((16, 16), 1.0 / 16, device=0)
+    with ps2:
+        buf = torch.zeros(64, device="cuda")
+        p = ctypes.c_void_p(buf.data_ptr())
+        assert ps2.lib.shk_add_gravity(ps2.ctx, 0.01, 0.0, -9.8, 0.0, p, p, p, p, p, p) == _lib.SHK_BAD_ARG
+    ps = MacPressureSolver3((8, 8, 8), 1.0 / 8, device=0, config={"HostPointers": True})
+    with ps:
+        lv = MacLiquidVelocity(ps)
+        u = [torch.zeros(ps.face_count(d), device="cuda") for d in range(3)]
+        m = [torch.zeros(ps.face_count(d), device="cuda", dtype=torch.uint8) for d in range(3)]
+        with pytest.raises(ShkError):
+            lv.add_gravity(0.01, u, m)
+    ps = MacPressureSolver3((8, 8, 8), 1.0 / 8, device=0)
+    with ps:
+        with pytest.raises(ShkError):
+            ps.configure("WENO", True)
