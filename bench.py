@@ -58,6 +58,103 @@ def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
     raise ValueError(kernel)
 
 
+# Algorithmic HBM bytes per launch of the liquid-step kernels, in units of G = n^3
+# (every face array holds ~G faces): value + mask read and written once each.
+LIQUID_BYTES_PER_CELL = {
+    "k_extrapolate_round": 30,     # 3 axes x (4+1 read, 4+1 written)
+    "k_semi_lagrangian_fwd": 32,   # u, v, w + masks read (15), fluid (4), v0, lo, hi, band written (13)
+    "k_semi_lagrangian_bwd": 32,   # v0 x3 + masks (15), vin, lo, hi, band (13), result written (4)
+    "k_constrain": 5,              # per axis: the mask and one solid node per face (the snapshot is read only
+                                   # inside solids, and only changed faces are written)
+    "k_max_u2": 15,                # u, v, w + masks read once
+    "k_project_activity": 30,      # 3 axes x (mask read and written, area and rho read)
+    "k_gravity": 30,               # 3 axes x (4+1 read, 4+1 written)
+}
+
+
+def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
+    """Times the velocity half of macliquid3's step with the projection at the bench
+    size on the bench scene, stage by stage (each library call is synchronous), and
+    the CPU oracle of the same stages on a (up to) 128^3 sample of the same scene."""
+    n = args.n
+    shape = (n, n, n)
+    ps = shk.MacPressureSolver3(shape, sc["dx"], config=cfg)
+    lv = shk.MacLiquidVelocity(ps)
+    vel = [dev[k].clone() for k in ("u", "v", "w")]
+    svel = [torch.zeros_like(v) for v in vel]
+    masks = [torch.ones(v.shape, dtype=torch.uint8, device="cuda") for v in vel]
+    ps.project(1.0 / 256, vel, dev["solid"], svel, dev["fluid"])   # a divergence-free start
+    lv.project_activity(masks)
+    lv.step(vel, masks, dev["solid"], svel, dev["fluid"])            # warm-up step
+    stages = {"max_u": 0.0, "extrapolate": 0.0, "constrain": 0.0, "advect": 0.0, "gravity": 0.0, "project": 0.0}
+    widths = []
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t = time.perf_counter()
+        max_u = lv.compute_max_u(vel, masks)
+        stages["max_u"] += time.perf_counter() - t
+        dt = lv.timestepper.advance(max_u, ps.dx)
+        width = int(lv.extend_width + lv.timestepper.current_cfl)
+        widths.append(width)
+        t = time.perf_counter()
+        lv.extrapolate(vel, masks, width)
+        stages["extrapolate"] += time.perf_counter() - t
+        t = time.perf_counter()
+        lv.constrain(dev["solid"], vel, masks)
+        stages["constrain"] += time.perf_counter() - t
+        t = time.perf_counter()
+        lv.advect(dt, vel, masks, dev["fluid"])
+        stages["advect"] += time.perf_counter() - t
+        t = time.perf_counter()
+        lv.add_gravity(dt, vel, masks)
+        stages["gravity"] += time.perf_counter() - t
+        t = time.perf_counter()
+        res = ps.project(dt, vel, dev["solid"], svel, dev["fluid"])
+        lv.project_activity(masks)
+        stages["project"] += time.perf_counter() - t
+    total = time.perf_counter() - t_all
+    ps.close()
+    velocity_half = sum(v for k, v in stages.items() if k != "project")
+    out = {"steps_per_s": round(steps / total, 3), "velocity_half_ms": round(1e3 * velocity_half / steps, 3),
+           "stage_ms": {k: round(1e3 * v / steps, 3) for k, v in stages.items()}, "extend_widths": widths,
+           "last_projection_iters": res["count"], "preconditioner": cfg.get("Preconditioner"),
+           "advect_algorithmic_bytes": int((LIQUID_BYTES_PER_CELL["k_semi_lagrangian_fwd"]
+                                            + LIQUID_BYTES_PER_CELL["k_semi_lagrangian_bwd"]) * 3 * n ** 3)}
+    if not args.no_cpu_baseline:
+        # the same stages in the CPU oracle (one core) on a 64^3 sample of the scene
+        from oracle import oracle as O
+        m = min(n, 128)
+        small = S.generate(args.scene, m, args.seed)
+        v = [np.ascontiguousarray(small[k]) for k in ("u", "v", "w")]
+        areas, rhos = O.area_fraction(small["solid"]), O.fluid_fraction(small["fluid"])
+        msk = [((a != 0) & (r != 0)).astype(np.uint8) for a, r in zip(areas, rhos)]   # after a projection
+        tc = {}
+        t = time.perf_counter()
+        O.compute_max_u(v, msk)
+        tc["max_u"] = time.perf_counter() - t
+        t = time.perf_counter()
+        v, msk, _ = O.extrapolate(v, msk, 4)
+        tc["extrapolate"] = time.perf_counter() - t
+        t = time.perf_counter()
+        v = O.constrain_velocity(small["dx"], small["solid"], v, msk)
+        tc["constrain"] = time.perf_counter() - t
+        t = time.perf_counter()
+        v = O.advect_velocity(small["dx"], 0.5 * small["dx"], v, msk, small["fluid"])
+        tc["advect"] = time.perf_counter() - t
+        t = time.perf_counter()
+        O.add_gravity(0.001, (0.0, -9.8, 0.0), v, msk)
+        tc["gravity"] = time.perf_counter() - t
+        cpu_half = sum(tc.values())
+        scale = (n / m) ** 3
+        out["cpu_oracle"] = {"sample": f"{m}^3 scene {args.scene}, every stage once, one host core",
+                             "velocity_half_ms_sample": round(1e3 * cpu_half, 2),
+                             "velocity_half_ms_extrapolated_to_n": round(1e3 * cpu_half * scale, 1),
+                             "stage_ms_sample": {k: round(1e3 * x, 2) for k, x in tc.items()}}
+        out["velocity_half_gpu_vs_cpu"] = round(cpu_half * scale / (velocity_half / steps), 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +178,9 @@ def main():
                     help="trained network for the learned-preconditioner line of --extra")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")
+    ap.add_argument("--liquid", action="store_true",
+                    help="also time the liquid step around the projection (extrapolation, constraint, "
+                         "MacCormack advection, gravity, projection) and its CPU oracle on a 128^3 sample")
     ap.add_argument("--dim", type=int, default=3, choices=[2, 3],
                     help="2: the 2D projection (shk_project2d) on an n^2 scene instead")
     args = ap.parse_args()
@@ -327,6 +427,9 @@ def main():
             extra["learned_fcg"] = entry
             pu.close()
         log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")
+    if args.liquid or args.extra:
+        extra["liquid_step"] = liquid_leg(args, torch, shiokaze_amd, S, sc, dev, cfg)
+        log(f"liquid step: {extra['liquid_step']}")
 
     cpu = None
     if not args.no_cpu_baseline and n <= 96:
