@@ -61,7 +61,8 @@ def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
 # Algorithmic HBM bytes per launch of the liquid-step kernels, in units of G = n^3
 # (every face array holds ~G faces): value + mask read and written once each.
 LIQUID_BYTES_PER_CELL = {
-    "k_extrapolate_round": 30,     # 3 axes x (4+1 read, 4+1 written)
+    "k_extrapolate_round": 3,      # 3 axes x the mask read once (in place: only the activation shell's
+                                   # neighbour values are read and its faces written, a few % of the grid)
     "k_semi_lagrangian_fwd": 32,   # u, v, w + masks read (15), fluid (4), v0, lo, hi, band written (13)
     "k_semi_lagrangian_bwd": 32,   # v0 x3 + masks (15), vin, lo, hi, band (13), result written (4)
     "k_constrain": 5,              # per axis: the mask and one solid node per face (the snapshot is read only

@@ -26,22 +26,22 @@
 namespace shk {
 
 struct LiquidWork {
-    DevBuf<float> a[3], b[3], c[3], lo[3], hi[3];
-    DevBuf<uint8_t> m[3], band[3];
+    DevBuf<float> a[3], b[3], lo[3], hi[3];
+    DevBuf<uint8_t> band[3];
     DevBuf<unsigned long long> count;
+    DevBuf<unsigned int> slots;
     DevBuf<double> part;
     std::vector<double> h_part;
     void release() {
         for (int d = 0; d < 3; ++d) {
             a[d].release();
             b[d].release();
-            c[d].release();
             lo[d].release();
             hi[d].release();
-            m[d].release();
             band[d].release();
         }
         count.release();
+        slots.release();
         part.release();
     }
 };
@@ -106,8 +106,8 @@ inline unsigned blocks_for(int64_t n) {
 // XCD-aware block order for the extrapolation stencil.  Workgroup b of a launch goes
 // to XCD b % 8, each XCD with its own L2; the z-neighbours of a block's faces lie about
 // one plane (nx*ny faces, hundreds of blocks) away, so the launch gives each XCD one
-// contiguous range of blocks instead (379 -> 259 us median per round at 256^3; the
-// advection's short gathers measured slower with it and keep the plain order).
+// contiguous range of blocks instead (the advection's short gathers measured slower
+// with it and keep the plain order).
 // Launch xcd_blocks(n) blocks; a block whose logical index is past the last one exits.
 constexpr int kXcds = 8;
 inline unsigned xcd_blocks(int64_t n) {
@@ -190,60 +190,101 @@ __device__ inline void full_velocity(const Dims &d, const CFaces &f, int dim, in
 // ---------------------------------------------------------------- extrapolation
 
 // One dilation round of array_extrapolator3::extrapolate (array_extrapolator3.h:47-76,
-// dilate3.h:43-97): an inactive face with active in-bounds 6-neighbours takes their
-// float average, summed in the order +x, -x, +y, -y, -z, +z; every round reads only
-// the previous round (src) and writes dst.  blockIdx.y is the axis.
-__global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, CFaces src, Faces dst, uint8_t *m0, uint8_t *m1,
-                                                            uint8_t *m2, unsigned long long *count) {
+// dilate3.h:43-97), in place: an inactive face with active in-bounds 6-neighbours
+// takes their float average, summed in the order +x, -x, +y, -y, -z, +z.  The round
+// must see only what was active before it (the reference collects every candidate and
+// sets them together), so a face activated in round r stores r + 1 in its mask and a
+// neighbour counts in round r when its mask is in [1, r]; a face that the same round
+// activates reads as inactive whether its write has landed or not, and the values
+// read are final.  Only inactive faces do work; active ones read one byte.  A last
+// pass turns the round numbers back into 1.  blockIdx.y is the axis.
+__global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Faces f, uint8_t *m0, uint8_t *m1, uint8_t *m2,
+                                                            int round, unsigned int *slots) {
 #pragma clang fp contract(off)
     const int dim = blockIdx.y;
     const FS s = face_fs(d, dim);
     const int64_t q = xcd_block() * kNT + threadIdx.x;
-    uint8_t *mo = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
-    __shared__ unsigned int block_count;
-    if (threadIdx.x == 0) block_count = 0;
-    __syncthreads();
+    uint8_t *m = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
+    bool activated = false;
     if (q < s.n()) {
-        const float *a = src.v[dim];
-        const uint8_t *m = src.m[dim];
+        float *a = f.v[dim];
         int i, j, k;
         decompose(q, s.nx, s.ny, i, j, k);
-        // the face and its six neighbours (clamped addresses, validity flags), all loads
-        // issued before any is used
+        const uint8_t own = m[q];   // issued together with the neighbour loads below
         const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
-        float nv[6];
-        uint8_t nm[6];
-        bool in[6];
+        int64_t e[6];
+        bool use[6];
+        bool any = false;
 #pragma unroll
-        for (int t = 0; t < 6; ++t) {
+        for (int t = 0; t < 6; ++t) {   // the six neighbour masks, loaded together
             const int ii = i + di[t], jj = j + dj[t], kk = k + dk[t];
-            in[t] = ii >= 0 && jj >= 0 && kk >= 0 && ii < s.nx && jj < s.ny && kk < s.nz;
-            const int64_t e = s.at(clampi(ii, 0, s.nx - 1), clampi(jj, 0, s.ny - 1), clampi(kk, 0, s.nz - 1));
-            nv[t] = a[e];
-            nm[t] = m[e];
+            const bool in = ii >= 0 && jj >= 0 && kk >= 0 && ii < s.nx && jj < s.ny && kk < s.nz;
+            e[t] = s.at(clampi(ii, 0, s.nx - 1), clampi(jj, 0, s.ny - 1), clampi(kk, 0, s.nz - 1));
+            const uint8_t nm = m[e[t]];
+            use[t] = in && nm != 0 && nm <= round;
+            any = any || use[t];
         }
-        float val = a[q];
-        uint8_t act = m[q];
-        if (!act) {
-            float sum = 0.0f;
-            int weight = 0;
+        float sum = 0.0f;
+        int weight = 0;
+        if (own == 0 && any) {   // values only on the activation shell
 #pragma unroll
             for (int t = 0; t < 6; ++t)
-                if (in[t] && nm[t]) {
-                    sum += nv[t];
+                if (use[t]) {
+                    sum += a[e[t]];
                     ++weight;
                 }
-            if (weight) {
-                val = (float)((double)sum / (double)weight);
-                act = 1;
-                atomicAdd(&block_count, 1u);
-            }
         }
-        dst.v[dim][q] = val;
-        mo[q] = act;
+        if (own == 0 && weight) {
+            a[q] = (float)((double)sum / (double)weight);
+            m[q] = (uint8_t)(round + 1);
+            activated = true;
+        }
     }
+    // the block's count goes to its own slot (rounds run one after another, so no two
+    // threads ever update a slot at once): no contended atomics on one counter
+    __shared__ unsigned int wave_count[kNT / 64];
+    const unsigned long long ballot = __ballot(activated);
+    if ((threadIdx.x & 63) == 0) wave_count[threadIdx.x >> 6] = (unsigned int)__popcll(ballot);
     __syncthreads();
-    if (threadIdx.x == 0 && block_count) atomicAdd(count, (unsigned long long)block_count);
+    if (threadIdx.x == 0) {
+        unsigned int c = 0;
+        for (int w = 0; w < kNT / 64; ++w) c += wave_count[w];
+        if (c) slots[blockIdx.y * gridDim.x + blockIdx.x] += c;
+    }
+}
+
+// total of the per-block slots: grid-stride partial sums, one atomic per block
+// (*total starts at 0)
+__global__ __launch_bounds__(kNT) void k_sum_slots(int64_t n, const unsigned int *slots,
+                                                   unsigned long long *total) {
+    unsigned long long c = 0;
+    for (int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x; q < n; q += (int64_t)gridDim.x * kNT) c += slots[q];
+    __shared__ unsigned long long lds[kNT];
+    lds[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = kNT / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) lds[threadIdx.x] += lds[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && lds[0]) atomicAdd(total, lds[0]);
+}
+
+// masks to 0 / 1 (before the rounds: any nonzero input is active; after them: the
+// round tags), four per thread, uchar4 when aligned
+__global__ __launch_bounds__(kNT) void k_mask_normalise(Dims d, uint8_t *m0, uint8_t *m1, uint8_t *m2, int vec) {
+    const int dim = blockIdx.y;
+    const int64_t n = face_fs(d, dim).n();
+    const int64_t q4 = (blockIdx.x * (int64_t)kNT + threadIdx.x) * 4;
+    if (q4 >= n) return;
+    uint8_t *m = dim == 0 ? m0 : (dim == 1 ? m1 : m2);
+    if (vec && q4 + 4 <= n) {
+        const uchar4 v = *reinterpret_cast<const uchar4 *>(m + q4);
+        if (v.x <= 1 && v.y <= 1 && v.z <= 1 && v.w <= 1) return;
+        *reinterpret_cast<uchar4 *>(m + q4) = make_uchar4(v.x != 0, v.y != 0, v.z != 0, v.w != 0);
+    } else {
+        for (int64_t q = q4; q < n; ++q)
+            if (m[q] > 1) m[q] = 1;
+    }
 }
 
 // ---------------------------------------------------------------- max |u|
@@ -580,39 +621,34 @@ int shk_extrapolate_velocity(shk_ctx *c, float *u, float *v, float *w, uint8_t *
     int rc;
     if ((rc = check3d(c, "shk_extrapolate_velocity"))) return rc;
     if (!u || !v || !w || !mu || !mv || !mw || width < 0) return c->fail(SHK_BAD_ARG, "null array or negative width");
+    // the round number lives in the uint8 mask (round r stores r + 1)
+    if (width > 254) return c->fail(SHK_BAD_ARG, "width %d is above 254", width);
     const Dims d = c->dims;
     LiquidWork *lw = work(c);
-    float *user[3] = {u, v, w};
-    uint8_t *um[3] = {mu, mv, mw};
-    if (width > 0)
-        for (int a = 0; a < 3; ++a) {
-            HIPC(c, lw->a[a].ensure(face_fs(d, a).n()));
-            HIPC(c, lw->m[a].ensure(face_fs(d, a).n()));
-        }
+    const unsigned grid = xcd_blocks(max_faces(d));
     HIPC(c, lw->count.ensure(1));
-    HIPC(c, hipMemsetAsync(lw->count.p, 0, sizeof(unsigned long long), c->stream));
-    for (int r = 0; r < width; ++r) {
-        const bool from_user = (r % 2) == 0;
-        CFaces src;
-        Faces dst;
-        uint8_t *dm[3];
-        for (int a = 0; a < 3; ++a) {
-            src.v[a] = from_user ? user[a] : lw->a[a].p;
-            src.m[a] = from_user ? um[a] : lw->m[a].p;
-            dst.v[a] = from_user ? lw->a[a].p : user[a];
-            dm[a] = from_user ? lw->m[a].p : um[a];
-            dst.m[a] = dm[a];
-        }
-        hipLaunchKernelGGL(k_extrapolate_round, dim3(xcd_blocks(max_faces(d)), 3), dim3(kNT), 0, c->stream, d, src,
-                           dst, dm[0], dm[1], dm[2], lw->count.p);
+    HIPC(c, lw->slots.ensure((size_t)grid * 3));
+    HIPC(c, hipMemsetAsync(lw->slots.p, 0, sizeof(unsigned int) * (size_t)grid * 3, c->stream));
+    const Faces f{{u, v, w}, {mu, mv, mw}};
+    const int vec = aligned16({mu, mv, mw}) ? 1 : 0;
+    if (width > 0) {
+        // any nonzero mask is active: make them 1 so that the rounds can tag theirs
+        hipLaunchKernelGGL(k_mask_normalise, dim3(blocks_for((max_faces(d) + 3) / 4), 3), dim3(kNT), 0, c->stream, d,
+                           mu, mv, mw, vec);
+    }
+    for (int r = 1; r <= width; ++r) {
+        hipLaunchKernelGGL(k_extrapolate_round, dim3(grid, 3), dim3(kNT), 0, c->stream, d, f, mu, mv, mw, r,
+                           lw->slots.p);
         HIPC(c, hipGetLastError());
     }
-    if (width % 2 == 1)
-        for (int a = 0; a < 3; ++a) {
-            const size_t n = (size_t)face_fs(d, a).n();
-            HIPC(c, hipMemcpyAsync(user[a], lw->a[a].p, sizeof(float) * n, hipMemcpyDeviceToDevice, c->stream));
-            HIPC(c, hipMemcpyAsync(um[a], lw->m[a].p, n, hipMemcpyDeviceToDevice, c->stream));
-        }
+    if (width > 0) {
+        hipLaunchKernelGGL(k_mask_normalise, dim3(blocks_for((max_faces(d) + 3) / 4), 3), dim3(kNT), 0, c->stream, d,
+                           mu, mv, mw, vec);
+        HIPC(c, hipGetLastError());
+    }
+    HIPC(c, hipMemsetAsync(lw->count.p, 0, sizeof(unsigned long long), c->stream));
+    hipLaunchKernelGGL(k_sum_slots, dim3(256), dim3(kNT), 0, c->stream, (int64_t)grid * 3, lw->slots.p, lw->count.p);
+    HIPC(c, hipGetLastError());
     unsigned long long cnt = 0;
     HIPC(c, hipMemcpyAsync(&cnt, lw->count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));

@@ -69,6 +69,24 @@ def test_extrapolate_matches_oracle(scenes, oracle, kind, n, seed, width):
             np.testing.assert_array_equal(_h(dv[d])[act], want[d][act])
 
 
+def test_extrapolate_takes_any_nonzero_mask_as_active(scenes, oracle):
+    sc = _scene(scenes, "IRREGULAR", 32, 1)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    rng = np.random.default_rng(5)
+    masks = _active_masks(sc, O)
+    vel = _velocity([m.shape for m in masks], rng)
+    want, want_m, want_n = O.extrapolate(vel, masks, 3)
+    ps, lv = _solver(shape, sc["dx"])
+    with ps:
+        dv = [_t(v) for v in vel]
+        dm = [_t((m * rng.integers(1, 256, m.shape)).astype(np.uint8)) for m in masks]   # 1..255 where active
+        assert lv.extrapolate(dv, dm, 3) == want_n
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dm[d]), want_m[d])
+            np.testing.assert_array_equal(_h(dv[d])[want_m[d] == 1], want[d][want_m[d] == 1])
+
+
 @pytest.mark.parametrize("kind,n,seed", CASES)
 def test_constrain_matches_oracle(scenes, oracle, kind, n, seed):
     sc = _scene(scenes, kind, n, seed)
