@@ -70,6 +70,8 @@ LIQUID_BYTES_PER_CELL = {
     "k_max_u2": 15,                # u, v, w + masks read once
     "k_project_activity": 30,      # 3 axes x (mask read and written, area and rho read)
     "k_gravity": 30,               # 3 axes x (4+1 read, 4+1 written)
+    "k_advect_cell": 1,            # per pass: the band mask; only band cells (a few % of the grid) read
+                                   # the level set and six faces and write their results
 }
 
 
@@ -87,7 +89,13 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
     ps.project(1.0 / 256, vel, dev["solid"], svel, dev["fluid"])   # a divergence-free start
     lv.project_activity(masks)
     lv.step(vel, masks, dev["solid"], svel, dev["fluid"])            # warm-up step
-    stages = {"max_u": 0.0, "extrapolate": 0.0, "constrain": 0.0, "advect": 0.0, "gravity": 0.0, "project": 0.0}
+    # the level set's narrow band (half width 3 dx, array3::set_as_levelset) for the
+    # level-set advection stage; the projection keeps using the scene's level set
+    bw = 3.0 * ps.dx
+    band = (dev["fluid"].abs() < bw).to(torch.uint8)
+    phi0 = dev["fluid"].clamp(-bw, bw)
+    stages = {"max_u": 0.0, "extrapolate": 0.0, "constrain": 0.0, "advect_levelset": 0.0, "advect": 0.0,
+              "gravity": 0.0, "project": 0.0}
     widths = []
     torch.cuda.synchronize()
     t_all = time.perf_counter()
@@ -104,6 +112,11 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
         t = time.perf_counter()
         lv.constrain(dev["solid"], vel, masks)
         stages["constrain"] += time.perf_counter() - t
+        phi = phi0.clone()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        lv.advect_levelset(dt, phi, band, vel, masks, bw)
+        stages["advect_levelset"] += time.perf_counter() - t
         t = time.perf_counter()
         lv.advect(dt, vel, masks, dev["fluid"])
         stages["advect"] += time.perf_counter() - t
@@ -140,6 +153,11 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
         t = time.perf_counter()
         v = O.constrain_velocity(small["dx"], small["solid"], v, msk)
         tc["constrain"] = time.perf_counter() - t
+        bws = 3.0 * small["dx"]
+        t = time.perf_counter()
+        O.advect_levelset(small["dx"], 0.5 * small["dx"], np.clip(small["fluid"], -bws, bws),
+                          (np.abs(small["fluid"]) < bws).astype(np.uint8), v, msk, bws)
+        tc["advect_levelset"] = time.perf_counter() - t
         t = time.perf_counter()
         v = O.advect_velocity(small["dx"], 0.5 * small["dx"], v, msk, small["fluid"])
         tc["advect"] = time.perf_counter() - t

@@ -57,6 +57,19 @@ int shk_compute_max_u(shk_ctx *ctx, const float *u, const float *v, const float 
 int shk_advect_velocity(shk_ctx *ctx, double dt, float *u, float *v, float *w, const uint8_t *mu, const uint8_t *mv,
                         const uint8_t *mw, const float *fluid);
 
+/* macadvection3_interface::advect_scalar(fluid, velocity, fluid_save, dt) as the level-set
+ * tracker calls it (src/surfacetracker/maclevelsetsurfacetracker3.cpp:49-52;
+ * macadvection3.cpp:37-45, 178-277): the cells of the cell-centred level set phi
+ * (nx ny nz) whose mask mphi is set (the narrow band) are advected by the MAC velocity,
+ * MacCormack unless MacCormack=0, semi-Lagrangian where the level set before the step
+ * is above -TrimNarrowBand*dx at the departure point; the other cells keep the values
+ * they hold (the sparse array's fill and background reads).  `background` is the level
+ * set's background value (its half bandwidth, array3.h:261-266): the MacCormack
+ * backward pass reads it outside the band (macadvection3.cpp:248).  Redistancing is not
+ * part of this call. */
+int shk_advect_levelset(shk_ctx *ctx, double dt, float *phi, const uint8_t *mphi, const float *u, const float *v,
+                        const float *w, const uint8_t *mu, const uint8_t *mv, const uint8_t *mw, float background);
+
 /* `m_velocity += dt*gravity` of macliquid3::inject_external_force (macliquid3.cpp:195-210):
  * adds (float)(dt*g) to every face and activates every face. */
 int shk_add_gravity(shk_ctx *ctx, double dt, double gx, double gy, double gz, float *u, float *v, float *w,

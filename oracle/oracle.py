@@ -58,6 +58,7 @@ _SIG = {
     "shk_or_compute_max_u": (_d, [_i, _i, _i] + [_vp] * 6),
     "shk_or_advect_velocity": (None, [_i, _i, _i, _d, _d] + [_vp] * 7 + [_i, _i]),
     "shk_or_add_gravity": (None, [_i, _i, _i, _d, _d, _d, _d] + [_vp] * 6),
+    "shk_or_advect_levelset": (None, [_i, _i, _i, _d, _d, _vp, _vp] + [_vp] * 6 + [ctypes.c_float, _i, _i]),
     "shk_or_project_activity": (None, [_i, _i, _i] + [_vp] * 9),
 }
 
@@ -519,3 +520,21 @@ def project_activity(areas, rhos, masks):
     ms = [_u8(m).copy() for m in masks]
     lib.shk_or_project_activity(nx1 - 1, ny, nz, *[_p(x) for x in a], *[_p(x) for x in r], *[_p(m) for m in ms])
     return ms
+
+
+def advect_levelset(dx, dt, phi, mphi, vel, masks, background, maccormack=True, trim=1):
+    """macadvection3::advect_scalar(fluid, velocity, fluid_save, dt) as the level-set
+    tracker calls it (maclevelsetsurfacetracker3.cpp:49-52): the active (narrow-band)
+    cells of the cell-centred level set `phi` advected by the MAC velocity; inactive
+    cells keep their values.  `background` is the level set's background value (the
+    half bandwidth, array3.h:261-266), which the MacCormack backward pass reads outside
+    the band.  Returns the new level set."""
+    lib = load()
+    nx, ny, nz = _cell_dims(vel)
+    out = _f32(phi).copy()
+    mp = _u8(mphi)
+    vs = [_f32(v) for v in vel]
+    ms = [_u8(m) for m in masks]
+    lib.shk_or_advect_levelset(nx, ny, nz, float(dx), float(dt), _p(out), _p(mp), *[_p(v) for v in vs],
+                               *[_p(m) for m in ms], float(background), 1 if maccormack else 0, int(trim))
+    return out

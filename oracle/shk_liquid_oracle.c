@@ -24,6 +24,10 @@
  *   shk_or_compute_max_u       macutility3.cpp:48-60, macarray3.h:327-365, vec.h:272-296
  *   shk_or_advect_velocity     macadvection3.cpp:46-54, 70-177 (MacCormack default on,
  *                              WENO off, TrimNarrowBand 1: :280-284), macarray3.h:372-393
+ *   shk_or_advect_levelset     macadvection3::advect_scalar as the level-set tracker calls it
+ *                              (maclevelsetsurfacetracker3.cpp:49-52; macadvection3.cpp:37-45,
+ *                              178-277; macarray3.h:327-365): active (narrow-band) cells only,
+ *                              inactive cells keep what they read
  *   shk_or_add_gravity         macliquid3.cpp:195-210 (`m_velocity += dt*gravity`
  *                              activates every face: array3.h:647-655, 917-921, 1152)
  *   shk_or_project_activity    macpressuresolver3.cpp:269-283 (set_off where area*rho == 0)
@@ -419,6 +423,115 @@ void shk_or_advect_velocity(int nx, int ny, int nz, double dx, double dt, float 
         free(mx[d]);
         free(band[d]);
     }
+}
+
+/* ------------------------------------------------------------------ level-set advection */
+
+/* macarray3::convert_to_full, cell version (macarray3.h:327-365): the float cell
+ * velocity of a cell whose six faces are active, else the empty (zero) vector */
+static int cell_velocity(const float *const vel[3], const uint8_t *const m[3], int nx, int ny, int nz, int i, int j,
+                         int k, double out[3]) {
+    int valid = 0;
+    for (int d = 0; d < 3; ++d) {
+        const Shape s = face_shape(nx, ny, nz, d);
+        const int64_t lo = at(s, i, j, k), hi = at(s, i + (d == 0), j + (d == 1), k + (d == 2));
+        if (m[d][lo] && m[d][hi]) {
+            double value = 0.0;
+            value += vel[d][lo];
+            value += vel[d][hi];
+            out[d] = (double)(float)(0.5 * value);
+            valid++;
+        }
+    }
+    if (valid != 3) {
+        out[0] = out[1] = out[2] = 0.0;
+        return 0;
+    }
+    return out[0] != 0.0 || out[1] != 0.0 || out[2] != 0.0;
+}
+
+/* advect_semiLagrangian_cell (macadvection3.cpp:178-241) over the active cells of mq:
+ * out = q interpolated at the departure point (vec3d arithmetic), with the limiter's
+ * bounds (min from FLT_MAX, max from FLT_MIN) and the narrow-band flag when lo != 0 */
+static void semi_lagrangian_cell(int nx, int ny, int nz, double dx, double dt, const float *q, const uint8_t *mq,
+                                 const float *const vel[3], const uint8_t *const m[3], const float *fluid,
+                                 int trim, float *out, float *lo, float *hi, uint8_t *band) {
+    const Shape cs = {nx, ny, nz};
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = at(cs, i, j, k);
+                if (!mq[c]) continue;
+                double u[3];
+                const int moving = cell_velocity(vel, m, nx, ny, nz, i, j, k, u);
+                double p[3] = {(double)i, (double)j, (double)k};
+                if (moving) {
+                    for (int d = 0; d < 3; ++d) p[d] = p[d] - dt * u[d] / dx;
+                    out[c] = interp(q, NULL, cs, p);
+                } else {
+                    out[c] = q[c];
+                }
+                if (!lo) continue;
+                if (moving) {
+                    int idx[8][3];
+                    double coef[8];
+                    interp_coef(cs, p, idx, coef);
+                    double mn = FLT_MAX, mx = FLT_MIN;
+                    for (int n = 0; n < 8; ++n) {
+                        const double v = q[at(cs, idx[n][0], idx[n][1], idx[n][2])];
+                        mn = fmin(mn, v);
+                        mx = fmax(mx, v);
+                    }
+                    lo[c] = (float)mn;
+                    hi[c] = (float)mx;
+                    band[c] = (double)interp(fluid, NULL, cs, p) > -dx * (double)trim;
+                } else {
+                    lo[c] = hi[c] = q[c];
+                    band[c] = (double)fluid[c] > -dx * (double)trim;
+                }
+            }
+}
+
+void shk_or_advect_levelset(int nx, int ny, int nz, double dx, double dt, float *phi, const uint8_t *mphi,
+                            const float *u, const float *v, const float *w, const uint8_t *mu, const uint8_t *mv,
+                            const uint8_t *mw, float background, int maccormack, int trim) {
+    const float *vel[3] = {u, v, w};
+    const uint8_t *m[3] = {mu, mv, mw};
+    const size_t n = (size_t)nx * ny * nz;
+    float *q0 = (float *)malloc(sizeof(float) * n), *q1 = (float *)malloc(sizeof(float) * n);
+    float *lo = (float *)malloc(sizeof(float) * n), *hi = (float *)malloc(sizeof(float) * n);
+    float *qin = (float *)malloc(sizeof(float) * n);
+    uint8_t *band = (uint8_t *)malloc(n);
+    memcpy(qin, phi, sizeof(float) * n); /* scalar0 and fluid_save: the level set before the step */
+    /* q_0 is a new array of the level set's type (macadvection3.cpp:248): its inactive
+     * cells read the background value, not the fill value the interior reads in phi */
+    for (size_t c = 0; c < n; ++c) q0[c] = mphi[c] ? phi[c] : background;
+    semi_lagrangian_cell(nx, ny, nz, dx, dt, qin, mphi, vel, m, qin, trim, q0, maccormack ? lo : NULL, hi, band);
+    if (maccormack) {
+        memcpy(q1, q0, sizeof(float) * n);
+        semi_lagrangian_cell(nx, ny, nz, dx, -dt, q0, mphi, vel, m, qin, trim, q1, NULL, NULL, NULL);
+        for (size_t c = 0; c < n; ++c) {
+            if (!mphi[c]) continue;
+            if (band[c]) {
+                phi[c] = q0[c];
+            } else {
+                const double mn = lo[c], mx = hi[c], q = q0[c];
+                const double corr = 0.5 * ((double)qin[c] - (double)q1[c]);
+                if (q + corr < mn) phi[c] = (float)mn;
+                else if (q + corr > mx) phi[c] = (float)mx;
+                else phi[c] = (float)(q + corr);
+            }
+        }
+    } else {
+        for (size_t c = 0; c < n; ++c)
+            if (mphi[c]) phi[c] = q0[c];
+    }
+    free(q0);
+    free(q1);
+    free(lo);
+    free(hi);
+    free(qin);
+    free(band);
 }
 
 /* ------------------------------------------------------------------ gravity, activity */

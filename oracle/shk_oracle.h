@@ -155,6 +155,9 @@ double shk_or_compute_max_u(int nx, int ny, int nz, const float *u, const float 
 void shk_or_advect_velocity(int nx, int ny, int nz, double dx, double dt, float *u, float *v, float *w,
                             const uint8_t *mu, const uint8_t *mv, const uint8_t *mw, const float *fluid,
                             int maccormack, int trim);
+void shk_or_advect_levelset(int nx, int ny, int nz, double dx, double dt, float *phi, const uint8_t *mphi,
+                            const float *u, const float *v, const float *w, const uint8_t *mu, const uint8_t *mv,
+                            const uint8_t *mw, float background, int maccormack, int trim);
 void shk_or_add_gravity(int nx, int ny, int nz, double dt, double gx, double gy, double gz, float *u, float *v,
                         float *w, uint8_t *mu, uint8_t *mv, uint8_t *mw);
 void shk_or_project_activity(int nx, int ny, int nz, const double *au, const double *av, const double *aw,

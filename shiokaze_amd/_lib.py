@@ -109,6 +109,7 @@ _SIGNATURES = {
     "shk_compute_max_u": (ctypes.c_int, [_vp] * 7 + [ctypes.POINTER(ctypes.c_double)]),
     "shk_advect_velocity": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 7),
     "shk_add_gravity": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),
+    "shk_advect_levelset": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 8 + [ctypes.c_float]),
     "shk_project_activity": (ctypes.c_int, [_vp] * 4),
 }
 

@@ -518,6 +518,113 @@ __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, double dx, doub
     }
 }
 
+// ---------------------------------------------------------------- level-set advection
+
+// macarray3::convert_to_full, cell version (macarray3.h:327-365): the float cell
+// velocity of a cell whose six faces are active, else the empty (zero) vector;
+// returns whether it moves
+__device__ inline bool cell_velocity(const Dims &d, const CFaces &f, int i, int j, int k, double u[3]) {
+#pragma clang fp contract(off)
+    float lo_v[3], hi_v[3];
+    bool valid = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const FS s = face_fs(d, a);
+        const int64_t lo = s.at(i, j, k), hi = s.at(i + (a == 0), j + (a == 1), k + (a == 2));
+        lo_v[a] = f.v[a][lo];
+        hi_v[a] = f.v[a][hi];
+        valid &= (f.m[a][lo] != 0) & (f.m[a][hi] != 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double value = 0.0;
+        value += lo_v[a];
+        value += hi_v[a];
+        u[a] = valid ? (double)(float)(0.5 * value) : 0.0;
+    }
+    return u[0] != 0.0 || u[1] != 0.0 || u[2] != 0.0;
+}
+
+// MacCormack advection of the active cells of a cell-centred level set by the MAC
+// velocity (macadvection3.cpp:178-277, advect_scalar as the level-set tracker calls
+// it).  Forward pass (lo != 0): out = semi-Lagrangian value (vec3d departure), the
+// limiter's bounds (min from FLT_MAX, max from FLT_MIN) and the narrow-band flag of
+// the level set before the step at the departure point.  Backward pass (fin_out !=
+// 0): the semi-Lagrangian value of q0 over -dt, combined into the final value.
+// Inactive cells of the output are left alone (they keep what they read).
+__global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, double dx, double dt, const float *q, const uint8_t *mq,
+                                                     CFaces vel, const float *fluid, int trim, float *out, float *lo,
+                                                     float *hi, uint8_t *band, const float *fin_qin,
+                                                     const float *fin_lo, const float *fin_hi,
+                                                     const uint8_t *fin_band, const float *fin_q0, float *fin_out,
+                                                     int fill_inactive, float background) {
+#pragma clang fp contract(off)
+    const FS cs{d.nx, d.ny, d.nz};
+    const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (c >= cs.n()) return;
+    if (!mq[c]) {
+        // q_0 is a new array of the level set's type (macadvection3.cpp:248): outside
+        // the band it reads the background value
+        if (fill_inactive) out[c] = background;
+        return;
+    }
+    int i, j, k;
+    decompose(c, d.nx, d.ny, i, j, k);
+    double u[3];
+    const bool moving = cell_velocity(d, vel, i, j, k, u);
+    double p[3] = {(double)i, (double)j, (double)k};
+    float value = q[c];
+    int ci = 0, cj = 0, ck = 0;
+    float vals[8];
+    if (moving) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) p[a] = p[a] - dt * u[a] / dx;
+        double coef[8];
+        interp_coef(cs, p, ci, cj, ck, coef);
+        float acc = 0.0f;
+#pragma unroll
+        for (int n = 0; n < 8; ++n) {
+            vals[n] = q[corner(cs, ci, cj, ck, n)];
+            if (coef[n] != 0.0) acc = (float)((double)acc + (double)vals[n] * coef[n]);
+        }
+        value = acc;
+    }
+    if (out) out[c] = value;
+    if (lo) {
+        if (moving) {
+            double mn = FLT_MAX, mx = FLT_MIN;
+#pragma unroll
+            for (int n = 0; n < 8; ++n) {
+                mn = fmin(mn, (double)vals[n]);
+                mx = fmax(mx, (double)vals[n]);
+            }
+            lo[c] = (float)mn;
+            hi[c] = (float)mx;
+            band[c] = (double)interp(fluid, nullptr, cs, p) > -dx * (double)trim;
+        } else {
+            lo[c] = q[c];
+            hi[c] = q[c];
+            band[c] = (double)fluid[c] > -dx * (double)trim;
+        }
+    }
+    if (fin_out) {
+        float r;
+        if (fin_band[c]) {
+            r = fin_q0[c];
+        } else {
+            const double mn = fin_lo[c], mx = fin_hi[c], q0 = fin_q0[c];
+            const double corr = 0.5 * ((double)fin_qin[c] - (double)value);
+            if (q0 + corr < mn)
+                r = (float)mn;
+            else if (q0 + corr > mx)
+                r = (float)mx;
+            else
+                r = (float)(q0 + corr);
+        }
+        fin_out[c] = r;
+    }
+}
+
 // ---------------------------------------------------------------- gravity, activity
 
 // `m_velocity += dt*gravity` (macliquid3.cpp:207-208): array3::operator+= touches every
@@ -754,6 +861,40 @@ int shk_advect_velocity(shk_ctx *c, double dt, float *u, float *v, float *w, con
                v0.v[2], w);
     }
 #undef SHK_SL
+    HIPC(c, hipGetLastError());
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return SHK_OK;
+}
+
+int shk_advect_levelset(shk_ctx *c, double dt, float *phi, const uint8_t *mphi, const float *u, const float *v,
+                        const float *w, const uint8_t *mu, const uint8_t *mv, const uint8_t *mw, float background) {
+    int rc;
+    if ((rc = check3d(c, "shk_advect_levelset"))) return rc;
+    if (!phi || !mphi || !u || !v || !w || !mu || !mv || !mw) return c->fail(SHK_BAD_ARG, "null array");
+    if (!std::isfinite(dt)) return c->fail(SHK_BAD_ARG, "dt must be finite");
+    const Dims d = c->dims;
+    LiquidWork *lw = work(c);
+    const bool mc = c->prm.maccormack;
+    const int64_t G = (int64_t)d.nx * d.ny * d.nz;
+    // scratch: the level set before the step (a[0]), q0 (b[0]), limiter bounds, band
+    HIPC(c, lw->a[0].ensure(G));
+    HIPC(c, hipMemcpyAsync(lw->a[0].p, phi, sizeof(float) * G, hipMemcpyDeviceToDevice, c->stream));
+    if (mc) {
+        HIPC(c, lw->b[0].ensure(G));
+        HIPC(c, lw->lo[0].ensure(G));
+        HIPC(c, lw->hi[0].ensure(G));
+        HIPC(c, lw->band[0].ensure(G));
+    }
+    const CFaces vel{{u, v, w}, {mu, mv, mw}};
+    const float *qin = lw->a[0].p;
+    hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, c->dx, dt, qin, mphi, vel, qin,
+                       c->prm.trim_narrowband, mc ? lw->b[0].p : phi, mc ? lw->lo[0].p : nullptr,
+                       mc ? lw->hi[0].p : nullptr, mc ? lw->band[0].p : nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, mc ? 1 : 0, background);
+    if (mc)
+        hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, c->dx, -dt, lw->b[0].p,
+                           mphi, vel, qin, c->prm.trim_narrowband, nullptr, nullptr, nullptr, nullptr, qin,
+                           lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
     return SHK_OK;

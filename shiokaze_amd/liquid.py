@@ -5,8 +5,9 @@
 342-391): the time step from max |u| (``TimeStepper``, timestepper.cpp:48-126), the
 velocity extension of ``extend_both`` (:330-340), self-advection (:369-370), gravity
 (:207-208) and the projection (:382), with the face activity the reference's sparse
-arrays carry held in uint8 masks.  The level set is an input here: its advection and
-redistancing (maclevelsetsurfacetracker3.cpp:46-63) are not part of this path.
+arrays carry held in uint8 masks.  The level set's advection on its narrow band
+(``advect_levelset``) is here too; its redistancing (maclevelsetsurfacetracker3.cpp:
+54-62) is not, so ``step`` takes the level set as given.
 Fields are torch tensors on the GPU (device memory only); the work runs in HIP kernels.
 """
 from __future__ import annotations
@@ -96,6 +97,15 @@ class MacLiquidVelocity:
         p, m = self._faces(velocity, masks)
         fp = _ptr(fluid, self.solver.cells, "fluid")
         self._check(self.lib.shk_advect_velocity(self.solver.ctx, float(dt), *p, *m, fp))
+
+    def advect_levelset(self, dt: float, phi, band, velocity, masks, background: float) -> None:
+        """macadvection3::advect_scalar(fluid, velocity, fluid_save, dt) on the narrow
+        band `band` (uint8 cell mask) of the level set `phi`, in place; `background` is
+        the level set's background value (its half bandwidth, 3 dx by default)."""
+        p, m = self._faces(velocity, masks)
+        self._check(self.lib.shk_advect_levelset(self.solver.ctx, float(dt), _ptr(phi, self.solver.cells, "phi"),
+                                                 _ptr(band, self.solver.cells, "band", torch.uint8), *p, *m,
+                                                 float(background)))
 
     def add_gravity(self, dt: float, velocity, masks) -> None:
         """`m_velocity += dt*gravity` (activates every face)."""

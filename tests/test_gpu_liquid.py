@@ -151,6 +151,32 @@ def test_advect_matches_oracle(scenes, oracle, kind, n, seed, maccormack, trim):
         log(f"[gpu] advect {kind} {shape} maccormack={maccormack} trim={trim}: bit-exact")
 
 
+@pytest.mark.parametrize("kind,n,seed", CASES)
+@pytest.mark.parametrize("maccormack,trim", [(True, 1), (False, 1), (True, 2)])
+def test_advect_levelset_matches_oracle(scenes, oracle, kind, n, seed, maccormack, trim):
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    dx = sc["dx"]
+    rng = np.random.default_rng(seed + 4)
+    masks = _active_masks(sc, O)
+    _, masks, _ = O.extrapolate(_velocity([m.shape for m in masks], rng), masks, 3)
+    vel = _velocity([m.shape for m in masks], rng, 2.0)
+    bw = 3 * dx
+    phi = np.clip(sc["fluid"], -bw, bw).astype(np.float32)     # the band values and the fill / background reads
+    band = (np.abs(sc["fluid"]) < bw).astype(np.uint8)
+    dt = 1.5 * dx / 2.0
+    want = O.advect_levelset(dx, dt, phi, band, vel, masks, bw, maccormack=maccormack, trim=trim)
+    ps, lv = _solver(shape, dx, {"MacCormack": maccormack, "TrimNarrowBand": trim})
+    with ps:
+        dphi = _t(phi)
+        lv.advect_levelset(dt, dphi, _t(band), [_t(v) for v in vel], [_t(m) for m in masks], bw)
+        got = _h(dphi)
+        np.testing.assert_array_equal(got, want)
+        log(f"[gpu] advect_levelset {kind} {shape} maccormack={maccormack} trim={trim}: "
+            f"{int((got != phi).sum())} of {int(band.sum())} band cells moved, bit-exact")
+
+
 def test_gravity_and_projection_activity_match_oracle(scenes, oracle):
     sc = _scene(scenes, "IRREGULAR", 32, 1)
     O = oracle

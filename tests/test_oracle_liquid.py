@@ -259,3 +259,60 @@ def test_constrain_tilted_plane_removes_the_normal_component():
         if d == 1:   # the y walls still apply to v
             keep = keep[(f[1][outside] > 0) & (f[1][outside] < ny)]
         np.testing.assert_array_equal(keep, np.float32(U[d]))
+
+
+def _uniform(n, U):
+    nx, ny, nz = n
+    shapes = [(nz, ny, nx + 1), (nz, ny + 1, nx), (nz + 1, ny, nx)]
+    return [np.full(s, U[d], np.float32) for d, s in enumerate(shapes)], [np.ones(s, np.uint8) for s in shapes]
+
+
+@pytest.mark.parametrize("maccormack", [True, False])
+def test_levelset_linear_field_shifts_exactly(maccormack):
+    n, dx, dt = (20, 18, 16), 1.0 / 20, 0.01
+    U = (0.7, -0.4, 0.3)
+    vel, masks = _uniform(n, U)
+    k, j, i = np.indices((n[2], n[1], n[0])).astype(np.float64)
+    phi = (0.3 * i + 0.5 * j - 0.2 * k - 4.0) * dx
+    band = np.ones(phi.shape, np.uint8)
+    out = O.advect_levelset(dx, dt, phi.astype(np.float32), band, vel, masks, background=3 * dx,
+                            maccormack=maccormack)
+    want = ((0.3 * (i - dt * U[0] / dx) + 0.5 * (j - dt * U[1] / dx) - 0.2 * (k - dt * U[2] / dx)) - 4.0) * dx
+    inner = (slice(1, -1),) * 3
+    np.testing.assert_allclose(out[inner], want[inner], rtol=0, atol=2e-6)
+
+
+def test_levelset_still_cells_and_inactive_cells_keep_their_values():
+    rng = np.random.default_rng(8)
+    n, dx, dt = (10, 9, 8), 1.0 / 10, 0.02
+    vel, shapes = _faces(n, rng)
+    masks = [np.ones(s, np.uint8) for s in shapes]
+    masks[0][3, 4, :] = 0    # the cells next to these faces do not move
+    phi = rng.uniform(-0.3, 0.3, (n[2], n[1], n[0])).astype(np.float32)
+    band = (rng.random(phi.shape) < 0.6).astype(np.uint8)
+    out = O.advect_levelset(dx, dt, phi, band, vel, masks, background=0.3)
+    np.testing.assert_array_equal(out[band == 0], phi[band == 0])
+    still = np.zeros(phi.shape, bool)
+    still[3, 4, :] = True
+    np.testing.assert_array_equal(out[still & (band == 1)], phi[still & (band == 1)])
+    zero = [np.zeros_like(v) for v in vel]
+    np.testing.assert_array_equal(O.advect_levelset(dx, dt, phi, band, zero, masks, background=0.3), phi)
+
+
+def test_levelset_backward_pass_reads_the_background_outside_the_band():
+    rng = np.random.default_rng(9)
+    n, dx, dt = (12, 12, 12), 1.0 / 12, 0.04
+    vel, shapes = _faces(n, rng)
+    masks = [np.ones(s, np.uint8) for s in shapes]
+    phi = rng.uniform(-0.3, 0.3, (n[2], n[1], n[0])).astype(np.float32)
+    band = np.zeros(phi.shape, np.uint8)
+    band[:, :, 4:8] = 1
+    a = O.advect_levelset(dx, dt, phi, band, vel, masks, background=0.25)
+    b = O.advect_levelset(dx, dt, phi, band, vel, masks, background=-0.25)
+    differs = a != b
+    assert differs.any()
+    assert not differs[:, :, 5:7].all()   # only cells whose backward departure leaves the band
+    assert np.all(band[differs] == 1)
+    sa = O.advect_levelset(dx, dt, phi, band, vel, masks, background=0.25, maccormack=False)
+    sb = O.advect_levelset(dx, dt, phi, band, vel, masks, background=-0.25, maccormack=False)
+    np.testing.assert_array_equal(sa, sb)   # the semi-Lagrangian pass never reads it
