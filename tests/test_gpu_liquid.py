@@ -306,3 +306,54 @@ def test_liquid_calls_refuse_2d_and_host_pointers():
     with ps:
         with pytest.raises(ShkError):
             ps.configure("WENO", True)
+
+
+def test_liquid_edge_cases(scenes, oracle):
+    """Width 0, dt = 0, all-inactive masks and the smallest grid (2 cells per axis),
+    each against the oracle."""
+    O = oracle
+    rng = np.random.default_rng(12)
+    for shape in ((2, 2, 2), (9, 5, 3)):
+        nx, ny, nz = shape
+        dx = 1.0 / nx
+        fshape = [(nz, ny, nx + 1), (nz, ny + 1, nx), (nz + 1, ny, nx)]
+        vel = [rng.uniform(-1, 1, s).astype(np.float32) for s in fshape]
+        masks = [(rng.random(s) < 0.5).astype(np.uint8) for s in fshape]
+        fluid = rng.uniform(-0.2, 0.2, (nz, ny, nx)).astype(np.float32)
+        solid = rng.uniform(-0.2, 0.2, (nz + 1, ny + 1, nx + 1)).astype(np.float32)
+        ps, lv = _solver(shape, dx)
+        with ps:
+            dv, dm = [_t(v) for v in vel], [_t(m) for m in masks]
+            assert lv.extrapolate(dv, dm, 0) == 0
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d]), vel[d])
+                np.testing.assert_array_equal(_h(dm[d]), masks[d])
+            for dt in (0.0, 0.3 * dx):
+                want = O.advect_velocity(dx, dt, vel, masks, fluid)
+                dv = [_t(v) for v in vel]
+                lv.advect(dt, dv, dm, _t(fluid))
+                for d in range(3):
+                    np.testing.assert_array_equal(_h(dv[d]), want[d])
+            want = O.constrain_velocity(dx, solid, vel, masks)
+            dv = [_t(v) for v in vel]
+            lv.constrain(_t(solid), dv, dm)
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d])[masks[d] == 1], want[d][masks[d] == 1])
+            none = [np.zeros(s, np.uint8) for s in fshape]
+            ev, em, n = O.extrapolate(vel, none, 3)
+            dv, dn = [_t(v) for v in vel], [_t(m) for m in none]
+            assert lv.extrapolate(dv, dn, 3) == n == 0
+            assert lv.compute_max_u(dv, dn) == 0.0
+            want = O.advect_velocity(dx, 0.1, vel, none, fluid)
+            lv.advect(0.1, dv, dn, _t(fluid))
+            for d in range(3):
+                np.testing.assert_array_equal(_h(dv[d]), want[d])
+    # one cell per axis is refused (trilinear interpolation needs two)
+    ps, lv = _solver((1, 4, 4), 0.25)
+    with ps:
+        import torch
+        from shiokaze_amd import ShkError
+        v = [torch.zeros(ps.face_count(d), device="cuda") for d in range(3)]
+        m = [torch.zeros(ps.face_count(d), device="cuda", dtype=torch.uint8) for d in range(3)]
+        with pytest.raises(ShkError):
+            lv.add_gravity(0.01, v, m)
