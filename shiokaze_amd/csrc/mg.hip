@@ -310,6 +310,166 @@ __global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// ---------------------------------------------------------------- level 0, 8^3 bricks, one wave per brick
+//
+// The two level-0 passes for 8x8x8 bricks: lane = i + 8 j holds one (x, y) column
+// and walks the brick's 8 z-planes.  In-brick x and y neighbours come by lane
+// shuffles (taken by all 64 lanes, outside any lane-dependent branch), z neighbours
+// from the lane's own registers, and the cells across the six faces from one
+// coalesced load per neighbour plane (the neighbour brick's slot, coordinates and
+// parent are wave-uniform).  The arithmetic is that of k_mg_pre0 / k_mg_post0 term by
+// term, in the same face order.
+
+__device__ inline float lane_get(float v, int src) { return __shfl(v, src & 63, 64); }
+
+// the cells across the faces of brick `slot` as six plane registers (0 where the
+// neighbour brick does not exist): x planes hold (y = a, z = c) at lane a + 8c, y planes
+// (x = a, z = c), z planes (x, y) at the lane itself
+template <class V>
+__device__ inline void halo_planes(const int nb[6], int lane, V value, float h[6]) {
+    const int a = lane & 7, c = lane >> 3;
+    h[0] = nb[0] >= 0 ? value(nb[0], 7 + 8 * a + 64 * c) : 0.0f;
+    h[1] = nb[1] >= 0 ? value(nb[1], 8 * a + 64 * c) : 0.0f;
+    h[2] = nb[2] >= 0 ? value(nb[2], a + 56 + 64 * c) : 0.0f;
+    h[3] = nb[3] >= 0 ? value(nb[3], a + 64 * c) : 0.0f;
+    h[4] = nb[4] >= 0 ? value(nb[4], lane + 448) : 0.0f;
+    h[5] = nb[5] >= 0 ? value(nb[5], lane) : 0.0f;
+}
+
+// the six neighbour values of cell (i, j, k) in face order x-, x+, y-, y+, z-, z+
+__device__ inline void neighbours6(const float *v, const float h[6], int lane, int k, float n[6]) {
+    const int i = lane & 7, j = lane >> 3;
+    const float x0 = v[k];
+    float xm = lane_get(x0, lane - 1), xp = lane_get(x0, lane + 1);
+    float ym = lane_get(x0, lane - 8), yp = lane_get(x0, lane + 8);
+    const float hxm = lane_get(h[0], j + 8 * k), hxp = lane_get(h[1], j + 8 * k);
+    const float hym = lane_get(h[2], i + 8 * k), hyp = lane_get(h[3], i + 8 * k);
+    if (i == 0) xm = hxm;
+    if (i == 7) xp = hxp;
+    if (j == 0) ym = hym;
+    if (j == 7) yp = hyp;
+    n[0] = xm;
+    n[1] = xp;
+    n[2] = ym;
+    n[3] = yp;
+    n[4] = k > 0 ? v[k - 1] : h[4];
+    n[5] = k < 7 ? v[k + 1] : h[5];
+}
+
+// the couplings of cell e across its six faces (low faces live at e, high faces at the
+// neighbour cell) and whether each neighbour exists; a uniform brick has kap on all
+__device__ inline void couplings6(const MgOp &op, int64_t e, const int nb[6], int lane, int k, float kap, float c[6],
+                                  bool has[6]) {
+    const int i = lane & 7, j = lane >> 3;
+    has[0] = i > 0 || nb[0] >= 0;
+    has[1] = i < 7 || nb[1] >= 0;
+    has[2] = j > 0 || nb[2] >= 0;
+    has[3] = j < 7 || nb[3] >= 0;
+    has[4] = k > 0 || nb[4] >= 0;
+    has[5] = k < 7 || nb[5] >= 0;
+    if (kap != 0.0f) {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) c[f] = kap;
+        return;
+    }
+    c[0] = op.cx[e];
+    c[2] = op.cy[e];
+    c[4] = op.cz[e];
+    c[1] = i < 7 ? op.cx[e + 1] : (nb[1] >= 0 ? op.cx[(int64_t)nb[1] * kBrickCells + lane - 7 + 64 * k] : 0.0f);
+    c[3] = j < 7 ? op.cy[e + 8] : (nb[3] >= 0 ? op.cy[(int64_t)nb[3] * kBrickCells + lane - 56 + 64 * k] : 0.0f);
+    c[5] = k < 7 ? op.cz[e + 64] : (nb[5] >= 0 ? op.cz[(int64_t)nb[5] * kBrickCells + lane] : 0.0f);
+}
+
+// t = w r - ... : k_mg_pre0 (t = r - w A r on the unknowns, 0 elsewhere)
+__global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restrict__ r, float *__restrict__ t,
+                                                  const SolveState *state) {
+    if (stopped(state)) return;
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    if (slot >= op.n_active) return;   // no block-wide synchronisation below
+    int nb[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = op.nbr[6 * slot + f];
+    const float kap = brick_kappa(op, slot);
+    const int64_t base = (int64_t)slot * kBrickCells + lane;
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = r[base + 64 * k];
+    float h[6];
+    halo_planes(nb, lane, [&](int ns, int l) { return r[(int64_t)ns * kBrickCells + l]; }, h);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t e = base + 64 * k;
+        float n[6];
+        neighbours6(v, h, lane, k, n);
+        float ax;
+        if (kap != 0.0f) {
+            ax = v[k] - kap * (n[0] + n[1] + n[2] + n[3] + n[4] + n[5]);
+        } else {
+            float c[6];
+            bool has[6];
+            couplings6(op, e, nb, lane, k, kap, c, has);
+            ax = op.d[e] * v[k];
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+                if (has[f]) ax -= c[f] * n[f];
+        }
+        t[e] = diag_of(op, e, kap) > 0.0f ? v[k] - op.omega * ax : 0.0f;
+    }
+}
+
+// y = x0' + w (r - A x0'), x0' = w r + alpha P0 x1: k_mg_post0; part[block] = sum r*y
+// over the block's bricks
+__global__ __launch_bounds__(NT) void k_mg_post0_w(MgOp op, const int4 *__restrict__ coord,
+                                                   const int *__restrict__ parent, const float *__restrict__ r,
+                                                   const float *__restrict__ xc, float *__restrict__ y,
+                                                   const SolveState *state, double *__restrict__ part) {
+    if (stopped(state)) return;
+    __shared__ double lds[NT / kWave];
+    const int lane = threadIdx.x & 63;
+    const int slot = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+    double acc = 0.0;
+    if (slot < op.n_active) {
+        int nb[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) nb[f] = op.nbr[6 * slot + f];
+        const float kap = brick_kappa(op, slot);
+        const int4 B = coord[slot];
+        const int ps = parent[slot];
+        const int64_t base = (int64_t)slot * kBrickCells + lane;
+        float rv[8], xs[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            rv[k] = r[base + 64 * k];
+            xs[k] = xprime<8>(op, r, xc, slot, lane + 64 * k, B, ps);
+        }
+        float h[6];
+        halo_planes(nb, lane, [&](int ns, int l) { return xprime<8>(op, r, xc, ns, l, coord[ns], parent[ns]); }, h);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t e = base + 64 * k;
+            float yv = 0.0f;
+            // every lane takes part in the shuffles, whether its cell is an unknown or not
+            float n[6];
+            neighbours6(xs, h, lane, k, n);
+            if (diag_of(op, e, kap) > 0.0f) {
+                float c[6];
+                bool has[6];
+                couplings6(op, e, nb, lane, k, kap, c, has);
+                float sum = 0.0f;
+#pragma unroll
+                for (int f = 0; f < 6; ++f)
+                    if (has[f] && c[f] != 0.0f) sum += c[f] * n[f];
+                yv = xs[k] + op.omega * (rv[k] - (xs[k] - sum));
+            }
+            y[e] = yv;
+            acc += (double)rv[k] * (double)yv;
+        }
+    }
+    const double s = block_sum<NT>(acc, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
 // out[t] = sum of in[t*per .. (t+1)*per) for t < n_out, in a fixed order (the
 // per-block partials of k_mg_post0 folded to the Krylov grid's partial count)
 __global__ __launch_bounds__(NT) void k_mg_fold(int n_in, const double *__restrict__ in, int n_out,
@@ -378,14 +538,24 @@ void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const floa
 }
 
 void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state) {
+    if (op.bx == 8) {
+        hipLaunchKernelGGL(k_mg_pre0_w, dim3(blocks((int64_t)op.n_active * 64)), dim3(NT), 0, st, op, r, t, state);
+        return;
+    }
     SHK_MG_LAUNCH(k_mg_pre0, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, r, t, state);
 }
 
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
                      const float *xc, float *y, const SolveState *state, const Grid &g, double *cell_part,
                      double *part) {
-    const unsigned nb = blocks((int64_t)op.n_active * kBrickCells);
-    SHK_MG_LAUNCH(k_mg_post0, op.bx, nb, st, op, coord, parent, r, xc, y, state, cell_part);
+    unsigned nb;
+    if (op.bx == 8) {
+        nb = blocks((int64_t)op.n_active * 64);   // one wave per brick
+        hipLaunchKernelGGL(k_mg_post0_w, dim3(nb), dim3(NT), 0, st, op, coord, parent, r, xc, y, state, cell_part);
+    } else {
+        nb = blocks((int64_t)op.n_active * kBrickCells);
+        SHK_MG_LAUNCH(k_mg_post0, op.bx, nb, st, op, coord, parent, r, xc, y, state, cell_part);
+    }
     hipLaunchKernelGGL(k_mg_fold, dim3((g.blocks + NT - 1) / NT), dim3(NT), 0, st, (int)nb, cell_part, g.blocks, part,
                        state);
 }

@@ -33,9 +33,8 @@ __global__ void k_fractions(Dims d, const float *__restrict__ solid, const float
     const int fx = d.nx + (DIM == 0), fy = d.ny + (DIM == 1);
     const int64_t n = face_count(d, DIM);
     for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < n; f += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(f % fx);
-        const int j = (int)((f / fx) % fy);
-        const int k = (int)(f / ((int64_t)fx * fy));
+        int i, j, k;
+        split_index(f, fx, fy, i, j, k);
         double a = TWO_D ? face_area_2d(d, solid, DIM, i, k, eps_solid) : face_area(d, solid, DIM, i, j, k, eps_solid);
         double r = face_rho(d, fluid, DIM, i, j, k, eps_fluid);
         float af = (float)a, rf = (float)r;
@@ -84,9 +83,8 @@ __global__ void k_label(Dims d, const float *__restrict__ fluid, Faces fc, uint8
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     unsigned long long local = 0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % d.nx);
-        const int j = (int)((c / d.nx) % d.ny);
-        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        int i, j, k;
+        split_index(c, d.nx, d.ny, i, j, k);
         const bool u = is_unknown(d, fluid, fc, i, j, k);
         label[c] = u ? 1 : 0;
         if (u) {
@@ -274,9 +272,8 @@ __global__ void k_scatter(Dims d, const int *__restrict__ brick_slot, int nbx, i
                           const double *__restrict__ x, float *__restrict__ pressure, double *__restrict__ prev) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % d.nx);
-        const int j = (int)((c / d.nx) % d.ny);
-        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        int i, j, k;
+        split_index(c, d.nx, d.ny, i, j, k);
         double p = 0.0;
         if (label[c]) {
             const int64_t e = brick_elem<BX>(d, brick_slot, nbx, nby, i, j, k);
@@ -297,9 +294,8 @@ __global__ void k_update_velocity(Dims d, double dt, double dx, const float *__r
     const int n = DIM == 0 ? d.nx : (DIM == 1 ? d.ny : d.nz);
     const int64_t nf = face_count(d, DIM);
     for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(f % fx);
-        const int j = (int)((f / fx) % fy);
-        const int k = (int)(f / ((int64_t)fx * fy));
+        int i, j, k;
+        split_index(f, fx, fy, i, j, k);
         const int pi = DIM == 0 ? i : (DIM == 1 ? j : k);
         const double r = rho[f];
         float nv = 0.0f;
@@ -333,9 +329,8 @@ __global__ void k_curvature(Dims d, double dx, const float *__restrict__ fluid, 
 #pragma clang fp contract(off)
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % d.nx);
-        const int j = (int)((c / d.nx) % d.ny);
-        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        int i, j, k;
+        split_index(c, d.nx, d.ny, i, j, k);
         float s = fluid[cell_index(d, clampi(i - 1, 0, d.nx - 1), j, k)];
         s = s + fluid[cell_index(d, clampi(i + 1, 0, d.nx - 1), j, k)];
         s = s + fluid[cell_index(d, i, clampi(j - 1, 0, d.ny - 1), k)];
@@ -353,8 +348,9 @@ __global__ void k_curvature_2d(Dims d, double dx, const float *__restrict__ flui
 #pragma clang fp contract(off)
     const int64_t G = (int64_t)d.nx * d.nz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % d.nx);
-        const int k = (int)(c / d.nx);
+        int i, jj, k;
+        split_index(c, d.nx, 1, i, jj, k);
+        (void)jj;
         float s = fluid[clampi(i - 1, 0, d.nx - 1) + (int64_t)d.nx * k];
         s = s + fluid[clampi(i + 1, 0, d.nx - 1) + (int64_t)d.nx * k];
         s = s + fluid[i + (int64_t)d.nx * clampi(k - 1, 0, d.nz - 1)];
@@ -377,9 +373,8 @@ __global__ void k_surface_tension(Dims d, double dt, double dx, double kappa, co
     for (int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; f < nf; f += (int64_t)gridDim.x * blockDim.x) {
         const double r = rho[f];
         if (!(r != 0.0 && r < 1.0)) continue;
-        const int i = (int)(f % fx);
-        const int j = (int)((f / fx) % fy);
-        const int k = (int)(f / ((int64_t)fx * fy));
+        int i, j, k;
+        split_index(f, fx, fy, i, j, k);
         const int64_t c1 = cell_index(d, clampi(i, 0, d.nx - 1), clampi(j, 0, d.ny - 1), clampi(k, 0, d.nz - 1));
         const int64_t c0 = cell_index(d, clampi(i - (DIM == 0), 0, d.nx - 1), clampi(j - (DIM == 1), 0, d.ny - 1),
                                       clampi(k - (DIM == 2), 0, d.nz - 1));
@@ -440,9 +435,8 @@ __global__ void k_divergence(Dims d, double dt, double dx, Faces fc, const uint8
     double best = 0.0;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
         if (!label[c]) continue;
-        const int i = (int)(c % d.nx);
-        const int j = (int)((c / d.nx) % d.ny);
-        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        int i, j, k;
+        split_index(c, d.nx, d.ny, i, j, k);
         double acc = 0.0;
         for (int nq = 0; nq < 6; ++nq) {
             int qi, qj, qk, fi, fj, fk, dim, sgn;
@@ -485,9 +479,8 @@ __global__ void k_export(Dims d, const int *__restrict__ brick_slot, int nbx, in
                          double *o_x) {
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < G; c += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(c % d.nx);
-        const int j = (int)((c / d.nx) % d.ny);
-        const int k = (int)(c / ((int64_t)d.nx * d.ny));
+        int i, j, k;
+        split_index(c, d.nx, d.ny, i, j, k);
         const int64_t e = brick_elem<BX>(d, brick_slot, nbx, nby, i, j, k);
         if (o_label) o_label[c] = label[c];
         if (o_diag) o_diag[c] = e >= 0 ? diag[e] : 0.0;

@@ -33,42 +33,68 @@ __host__ __device__ inline double fraction(double phi0, double phi1) {
 // q10 (1,0), q11 (1,1), q01 (0,1) (include/shiokaze/utility/utility.h:191-226):
 // negative corners and linear edge crossings are visited counter-clockwise and
 // the shoelace formula is applied to that point list.
+// The shoelace terms are added as the points are generated (each term needs only the
+// previous point; the closing term the first and last): the same terms in the same
+// order as a sum over the stored point list, without a per-thread point array (which
+// a GPU keeps in slow scratch memory when it is indexed at run time).
+struct ShoelaceSum {
+    double fx = 0.0, fy = 0.0, lx = 0.0, ly = 0.0, sum = 0.0;
+    int np = 0;
+    __host__ __device__ inline void add(double x, double y) {
+#pragma clang fp contract(off)
+        if (np == 0) {
+            fx = x;
+            fy = y;
+        } else {
+            sum += lx * y - ly * x;
+        }
+        lx = x;
+        ly = y;
+        ++np;
+    }
+    __host__ __device__ inline double close() {
+#pragma clang fp contract(off)
+        if (np > 0) sum += lx * fy - ly * fx;
+        return sum;
+    }
+};
+
 __host__ __device__ inline double get_area(double q00, double q10, double q11, double q01) {
 #pragma clang fp contract(off)
     const double v[4] = {q00, q10, q11, q01};
     const double cx[4] = {0.0, 1.0, 1.0, 0.0};
     const double cy[4] = {0.0, 0.0, 1.0, 1.0};
-    double px[8], py[8];
-    int np = 0;
+    ShoelaceSum s;
+#pragma unroll
     for (int n = 0; n < 4; ++n) {
         const int m = (n + 1) & 3;
-        if (v[n] < 0.0) {
-            px[np] = cx[n];
-            py[np] = cy[n];
-            ++np;
-        }
+        if (v[n] < 0.0) s.add(cx[n], cy[n]);
         if (v[n] * v[m] < 0) {
             const double y0 = v[n], y1 = v[m];
             if (y0 - y1 != 0.0) {
                 const double a = y0 / (y0 - y1);
-                px[np] = (1.0 - a) * cx[n] + a * cx[m];
-                py[np] = (1.0 - a) * cy[n] + a * cy[m];
-                ++np;
+                s.add((1.0 - a) * cx[n] + a * cx[m], (1.0 - a) * cy[n] + a * cy[m]);
             }
         }
     }
-    double sum = 0.0;
-    for (int m = 0; m < np; ++m) {
-        const int m1 = (m + 1 == np) ? 0 : m + 1;
-        sum += px[m] * py[m1] - py[m] * px[m1];
-    }
-    return 0.5 * sum;
+    return 0.5 * s.close();
 }
 
 struct Dims {
     int nx, ny, nz;
     int bx = 8;  // brick x extent: 8 (8x8x8 bricks, 3D) or 64 (64x1x8 bricks, a 2D grid as nx x 1 x ny)
 };
+
+// (i, j, k) of the linear index q of an nx x ny x (any) array with 32-bit unsigned
+// arithmetic: the grids hold at most 2^31 cells, so every cell, face or node index
+// fits in 32 bits, and a 64-bit division per thread would cost several times more
+__host__ __device__ inline void split_index(int64_t q, int nx, int ny, int &i, int &j, int &k) {
+    const uint32_t u = (uint32_t)q;
+    const uint32_t t = u / (uint32_t)nx;
+    i = (int)(u - t * (uint32_t)nx);
+    k = (int)(t / (uint32_t)ny);
+    j = (int)(t - (uint32_t)k * (uint32_t)ny);
+}
 
 __host__ __device__ inline int64_t cell_index(const Dims &d, int i, int j, int k) {
     return (int64_t)i + (int64_t)d.nx * ((int64_t)j + (int64_t)d.ny * (int64_t)k);
