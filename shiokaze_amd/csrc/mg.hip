@@ -224,19 +224,6 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restri
 // unknowns and the diagonal is 1 on them, so x0 = w r needs no division and is never
 // stored: the pre-smoothing residual and the post-smoothing recompute it from r.
 
-// t0 = r - A (w r) on the unknowns (0 elsewhere)
-template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_pre0(MgOp op, const float *__restrict__ r, float *__restrict__ t,
-                                                const SolveState *state) {
-    if (stopped(state)) return;
-    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e >= (int64_t)op.n_active * kBrickCells) return;
-    const int slot = (int)(e >> 9), l = (int)(e & 511);
-    const float kap = brick_kappa(op, slot);
-    // A (w r) = w A r by linearity
-    t[e] = diag_of(op, e, kap) > 0.0f ? r[e] - op.omega * apply<BX>(op, slot, l, r, kap) : 0.0f;
-}
-
 // value of x0' = w r + alpha P0 x1 at level-0 cell (slot, l) of brick B with parent ps
 template <int BX>
 __device__ inline float xprime(const MgOp &op, const float *__restrict__ r, const float *__restrict__ xc, int slot,
@@ -245,109 +232,61 @@ __device__ inline float xprime(const MgOp &op, const float *__restrict__ r, cons
            op.alpha * xc[(int64_t)ps * kBrickCells + coarse_local<BX>(B, l)];
 }
 
-// y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
-// prolongation fused into the post-smoothing), one thread per cell; part[block] =
-// sum of r*y over the block's 256 cells
-template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_post0(MgOp op, const int4 *__restrict__ coord,
-                                                 const int *__restrict__ parent, const float *__restrict__ r,
-                                                 const float *__restrict__ xc, float *__restrict__ y,
-                                                 const SolveState *state, double *__restrict__ part) {
-    using Bs = BrickShape<BX>;
-    constexpr int X = Bs::X, Y = Bs::Y;
-    if (stopped(state)) return;
-    __shared__ double lds[NT / kWave];
-    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    double acc = 0.0;
-    if (e < (int64_t)op.n_active * kBrickCells) {
-        const int slot = (int)(e >> 9), l = (int)(e & 511);
-        const int i = l % X, j = (l / X) % Y, k = l / kWave;
-        const float kap = brick_kappa(op, slot);
-        const float dv = diag_of(op, e, kap);
-        float yv = 0.0f;
-        const float rv = r[e];
-        if (dv > 0.0f) {
-            const int4 B = coord[slot];
-            const int ps = parent[slot];
-            const int *nb = op.nbr + 6 * slot;
-            const float xs = xprime<BX>(op, r, xc, slot, l, B, ps);
-            // neighbour (slot', l') across each face, the face coupling, and x0' there
-            float sum = 0.0f;
-#pragma unroll
-            for (int f = 0; f < 6; ++f) {
-                const int axis = f >> 1, hi = f & 1;
-                if (Y == 1 && axis == 1) continue;
-                const int pos = axis == 0 ? i : (axis == 1 ? j : k);
-                const int ext = axis == 0 ? X : (axis == 1 ? Y : 8);
-                const int stride = axis == 0 ? 1 : (axis == 1 ? X : kWave);
-                int ns = slot, nl;
-                if (hi ? pos < ext - 1 : pos > 0) {
-                    nl = hi ? l + stride : l - stride;
-                } else {
-                    ns = nb[f];
-                    nl = hi ? l - (ext - 1) * stride : l + (ext - 1) * stride;
-                }
-                if (ns < 0) continue;
-                // coupling: low face at this cell, high face at the neighbour
-                float c;
-                if (kap != 0.0f) {
-                    c = kap;
-                } else {
-                    const float *cc = axis == 0 ? op.cx : (axis == 1 ? op.cy : op.cz);
-                    c = hi ? cc[(int64_t)ns * kBrickCells + nl] : cc[e];
-                }
-                if (c == 0.0f) continue;
-                const int4 NB = ns == slot ? B : coord[ns];
-                const int nps = ns == slot ? ps : parent[ns];
-                sum += c * xprime<BX>(op, r, xc, ns, nl, NB, nps);
-            }
-            yv = xs + op.omega * (rv - (xs - sum));
-        }
-        y[e] = yv;
-        acc = (double)rv * (double)yv;
-    }
-    const double s = block_sum<NT>(acc, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = s;
-}
-
-// ---------------------------------------------------------------- level 0, 8^3 bricks, one wave per brick
+// ---------------------------------------------------------------- level 0, one wave per brick
 //
-// The two level-0 passes for 8x8x8 bricks: lane = i + 8 j holds one (x, y) column
+// The two level-0 passes with one wave per brick: lane = i + X j holds one (x, y)
+// column (X x Y = 64: 8 x 8 for 3D bricks, 64 x 1 for the 64x1x8 bricks of 2D grids)
 // and walks the brick's 8 z-planes.  In-brick x and y neighbours come by lane
 // shuffles (taken by all 64 lanes, outside any lane-dependent branch), z neighbours
-// from the lane's own registers, and the cells across the six faces from one
-// coalesced load per neighbour plane (the neighbour brick's slot, coordinates and
-// parent are wave-uniform).  The arithmetic is that of k_mg_pre0 / k_mg_post0 term by
-// term, in the same face order.
+// from the lane's own registers, and the cells across the faces from one coalesced
+// load per neighbour plane (the neighbour brick's slot, coordinates and parent are
+// wave-uniform).  Each cell's arithmetic is that of one thread per cell applying
+// apply<BX> (pre) or summing c * x0' over its faces (post), term by term in the same
+// face order.
 
 __device__ inline float lane_get(float v, int src) { return __shfl(v, src & 63, 64); }
 
-// the cells across the faces of brick `slot` as six plane registers (0 where the
-// neighbour brick does not exist): x planes hold (y = a, z = c) at lane a + 8c, y planes
-// (x = a, z = c), z planes (x, y) at the lane itself
-template <class V>
+// the cells across the faces of a brick as plane registers (0 where the neighbour
+// brick does not exist): x planes hold (y = a, z = c) at lane a + Y c (the first 8 Y
+// lanes), y planes (x = a, z = c) at lane a + X c, z planes (x, y) at the lane itself
+template <int BX, class V>
 __device__ inline void halo_planes(const int nb[6], int lane, V value, float h[6]) {
-    const int a = lane & 7, c = lane >> 3;
-    h[0] = nb[0] >= 0 ? value(nb[0], 7 + 8 * a + 64 * c) : 0.0f;
-    h[1] = nb[1] >= 0 ? value(nb[1], 8 * a + 64 * c) : 0.0f;
-    h[2] = nb[2] >= 0 ? value(nb[2], a + 56 + 64 * c) : 0.0f;
-    h[3] = nb[3] >= 0 ? value(nb[3], a + 64 * c) : 0.0f;
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
+    const int xa = lane % Y, xc = lane / Y;
+    const bool xl = lane < Y * 8;
+    h[0] = nb[0] >= 0 && xl ? value(nb[0], (X - 1) + X * xa + 64 * xc) : 0.0f;
+    h[1] = nb[1] >= 0 && xl ? value(nb[1], X * xa + 64 * xc) : 0.0f;
+    h[2] = h[3] = 0.0f;
+    if constexpr (Y > 1) {
+        const int pa = lane % X, pc = lane / X;
+        h[2] = nb[2] >= 0 ? value(nb[2], pa + X * (Y - 1) + 64 * pc) : 0.0f;
+        h[3] = nb[3] >= 0 ? value(nb[3], pa + 64 * pc) : 0.0f;
+    }
     h[4] = nb[4] >= 0 ? value(nb[4], lane + 448) : 0.0f;
     h[5] = nb[5] >= 0 ? value(nb[5], lane) : 0.0f;
 }
 
 // the six neighbour values of cell (i, j, k) in face order x-, x+, y-, y+, z-, z+
+// (y values 0 without a y axis)
+template <int BX>
 __device__ inline void neighbours6(const float *v, const float h[6], int lane, int k, float n[6]) {
-    const int i = lane & 7, j = lane >> 3;
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
+    const int i = lane % X, j = lane / X;
     const float x0 = v[k];
     float xm = lane_get(x0, lane - 1), xp = lane_get(x0, lane + 1);
-    float ym = lane_get(x0, lane - 8), yp = lane_get(x0, lane + 8);
-    const float hxm = lane_get(h[0], j + 8 * k), hxp = lane_get(h[1], j + 8 * k);
-    const float hym = lane_get(h[2], i + 8 * k), hyp = lane_get(h[3], i + 8 * k);
+    const float hxm = lane_get(h[0], j + Y * k), hxp = lane_get(h[1], j + Y * k);
     if (i == 0) xm = hxm;
-    if (i == 7) xp = hxp;
-    if (j == 0) ym = hym;
-    if (j == 7) yp = hyp;
+    if (i == X - 1) xp = hxp;
+    float ym = 0.0f, yp = 0.0f;
+    if constexpr (Y > 1) {
+        ym = lane_get(x0, lane - X);
+        yp = lane_get(x0, lane + X);
+        const float hym = lane_get(h[2], i + X * k), hyp = lane_get(h[3], i + X * k);
+        if (j == 0) ym = hym;
+        if (j == Y - 1) yp = hyp;
+    }
     n[0] = xm;
     n[1] = xp;
     n[2] = ym;
@@ -356,15 +295,18 @@ __device__ inline void neighbours6(const float *v, const float h[6], int lane, i
     n[5] = k < 7 ? v[k + 1] : h[5];
 }
 
-// the couplings of cell e across its six faces (low faces live at e, high faces at the
+// the couplings of cell e across its faces (low faces live at e, high faces at the
 // neighbour cell) and whether each neighbour exists; a uniform brick has kap on all
+template <int BX>
 __device__ inline void couplings6(const MgOp &op, int64_t e, const int nb[6], int lane, int k, float kap, float c[6],
                                   bool has[6]) {
-    const int i = lane & 7, j = lane >> 3;
+    using B = BrickShape<BX>;
+    constexpr int X = B::X, Y = B::Y;
+    const int i = lane % X, j = lane / X;
     has[0] = i > 0 || nb[0] >= 0;
-    has[1] = i < 7 || nb[1] >= 0;
-    has[2] = j > 0 || nb[2] >= 0;
-    has[3] = j < 7 || nb[3] >= 0;
+    has[1] = i < X - 1 || nb[1] >= 0;
+    has[2] = Y > 1 && (j > 0 || nb[2] >= 0);
+    has[3] = Y > 1 && (j < Y - 1 || nb[3] >= 0);
     has[4] = k > 0 || nb[4] >= 0;
     has[5] = k < 7 || nb[5] >= 0;
     if (kap != 0.0f) {
@@ -373,16 +315,23 @@ __device__ inline void couplings6(const MgOp &op, int64_t e, const int nb[6], in
         return;
     }
     c[0] = op.cx[e];
-    c[2] = op.cy[e];
+    c[1] = i < X - 1 ? op.cx[e + 1]
+                     : (nb[1] >= 0 ? op.cx[(int64_t)nb[1] * kBrickCells + lane - (X - 1) + 64 * k] : 0.0f);
+    c[2] = c[3] = 0.0f;
+    if constexpr (Y > 1) {
+        c[2] = op.cy[e];
+        c[3] = j < Y - 1 ? op.cy[e + X]
+                         : (nb[3] >= 0 ? op.cy[(int64_t)nb[3] * kBrickCells + lane - X * (Y - 1) + 64 * k] : 0.0f);
+    }
     c[4] = op.cz[e];
-    c[1] = i < 7 ? op.cx[e + 1] : (nb[1] >= 0 ? op.cx[(int64_t)nb[1] * kBrickCells + lane - 7 + 64 * k] : 0.0f);
-    c[3] = j < 7 ? op.cy[e + 8] : (nb[3] >= 0 ? op.cy[(int64_t)nb[3] * kBrickCells + lane - 56 + 64 * k] : 0.0f);
     c[5] = k < 7 ? op.cz[e + 64] : (nb[5] >= 0 ? op.cz[(int64_t)nb[5] * kBrickCells + lane] : 0.0f);
 }
 
-// t = w r - ... : k_mg_pre0 (t = r - w A r on the unknowns, 0 elsewhere)
+// t0 = r - A (w r) = r - w A r on the unknowns (0 elsewhere), by linearity
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restrict__ r, float *__restrict__ t,
                                                   const SolveState *state) {
+    constexpr int Y = BrickShape<BX>::Y;
     if (stopped(state)) return;
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
@@ -396,19 +345,20 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = r[base + 64 * k];
     float h[6];
-    halo_planes(nb, lane, [&](int ns, int l) { return r[(int64_t)ns * kBrickCells + l]; }, h);
+    halo_planes<BX>(nb, lane, [&](int ns, int l) { return r[(int64_t)ns * kBrickCells + l]; }, h);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int64_t e = base + 64 * k;
         float n[6];
-        neighbours6(v, h, lane, k, n);
+        neighbours6<BX>(v, h, lane, k, n);
         float ax;
         if (kap != 0.0f) {
-            ax = v[k] - kap * (n[0] + n[1] + n[2] + n[3] + n[4] + n[5]);
+            // the y terms are 0.0f without a y axis, as in apply<BX>
+            ax = v[k] - kap * (n[0] + n[1] + (Y > 1 ? n[2] : 0.0f) + (Y > 1 ? n[3] : 0.0f) + n[4] + n[5]);
         } else {
             float c[6];
             bool has[6];
-            couplings6(op, e, nb, lane, k, kap, c, has);
+            couplings6<BX>(op, e, nb, lane, k, kap, c, has);
             ax = op.d[e] * v[k];
 #pragma unroll
             for (int f = 0; f < 6; ++f)
@@ -418,8 +368,10 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
     }
 }
 
-// y = x0' + w (r - A x0'), x0' = w r + alpha P0 x1: k_mg_post0; part[block] = sum r*y
-// over the block's bricks
+// y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 computed on the fly (the
+// prolongation fused into the post-smoothing); part[block] = sum r*y over the block's
+// bricks
+template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_post0_w(MgOp op, const int4 *__restrict__ coord,
                                                    const int *__restrict__ parent, const float *__restrict__ r,
                                                    const float *__restrict__ xc, float *__restrict__ y,
@@ -441,21 +393,22 @@ __global__ __launch_bounds__(NT) void k_mg_post0_w(MgOp op, const int4 *__restri
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             rv[k] = r[base + 64 * k];
-            xs[k] = xprime<8>(op, r, xc, slot, lane + 64 * k, B, ps);
+            xs[k] = xprime<BX>(op, r, xc, slot, lane + 64 * k, B, ps);
         }
         float h[6];
-        halo_planes(nb, lane, [&](int ns, int l) { return xprime<8>(op, r, xc, ns, l, coord[ns], parent[ns]); }, h);
+        halo_planes<BX>(nb, lane,
+                        [&](int ns, int l) { return xprime<BX>(op, r, xc, ns, l, coord[ns], parent[ns]); }, h);
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int64_t e = base + 64 * k;
             float yv = 0.0f;
             // every lane takes part in the shuffles, whether its cell is an unknown or not
             float n[6];
-            neighbours6(xs, h, lane, k, n);
+            neighbours6<BX>(xs, h, lane, k, n);
             if (diag_of(op, e, kap) > 0.0f) {
                 float c[6];
                 bool has[6];
-                couplings6(op, e, nb, lane, k, kap, c, has);
+                couplings6<BX>(op, e, nb, lane, k, kap, c, has);
                 float sum = 0.0f;
 #pragma unroll
                 for (int f = 0; f < 6; ++f)
@@ -471,7 +424,7 @@ __global__ __launch_bounds__(NT) void k_mg_post0_w(MgOp op, const int4 *__restri
 }
 
 // out[t] = sum of in[t*per .. (t+1)*per) for t < n_out, in a fixed order (the
-// per-block partials of k_mg_post0 folded to the Krylov grid's partial count)
+// per-block partials of k_mg_post0_w folded to the Krylov grid's partial count)
 __global__ __launch_bounds__(NT) void k_mg_fold(int n_in, const double *__restrict__ in, int n_out,
                                                 double *__restrict__ out, const SolveState *state) {
     if (stopped(state)) return;
@@ -538,24 +491,14 @@ void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const floa
 }
 
 void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state) {
-    if (op.bx == 8) {
-        hipLaunchKernelGGL(k_mg_pre0_w, dim3(blocks((int64_t)op.n_active * 64)), dim3(NT), 0, st, op, r, t, state);
-        return;
-    }
-    SHK_MG_LAUNCH(k_mg_pre0, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, r, t, state);
+    SHK_MG_LAUNCH(k_mg_pre0_w, op.bx, blocks((int64_t)op.n_active * 64), st, op, r, t, state);
 }
 
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,
                      const float *xc, float *y, const SolveState *state, const Grid &g, double *cell_part,
                      double *part) {
-    unsigned nb;
-    if (op.bx == 8) {
-        nb = blocks((int64_t)op.n_active * 64);   // one wave per brick
-        hipLaunchKernelGGL(k_mg_post0_w, dim3(nb), dim3(NT), 0, st, op, coord, parent, r, xc, y, state, cell_part);
-    } else {
-        nb = blocks((int64_t)op.n_active * kBrickCells);
-        SHK_MG_LAUNCH(k_mg_post0, op.bx, nb, st, op, coord, parent, r, xc, y, state, cell_part);
-    }
+    const unsigned nb = blocks((int64_t)op.n_active * 64);   // one wave per brick
+    SHK_MG_LAUNCH(k_mg_post0_w, op.bx, nb, st, op, coord, parent, r, xc, y, state, cell_part);
     hipLaunchKernelGGL(k_mg_fold, dim3((g.blocks + NT - 1) / NT), dim3(NT), 0, st, (int)nb, cell_part, g.blocks, part,
                        state);
 }
