@@ -52,7 +52,9 @@ struct MgLevelDev {
     float *d = nullptr, *cx = nullptr, *cy = nullptr, *cz = nullptr;
     float *b = nullptr, *x = nullptr, *t = nullptr;
     int bx = 8;             // brick shape: 8 (8x8x8) or 64 (64x1x8)
-    MgOp op(float omega, float alpha) const { return MgOp{nbr, d, cx, cy, cz, n_active, omega, alpha, nullptr, bx}; }
+    __host__ __device__ MgOp op(float omega, float alpha) const {
+        return MgOp{nbr, d, cx, cy, cz, n_active, omega, alpha, nullptr, bx};
+    }
 };
 
 constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)
@@ -61,20 +63,22 @@ constexpr int kMgCoarseMax = 64;  // cells of the coarsest level (<= 4 per axis)
 void launch_mg_mask(hipStream_t st, int64_t n, const double *sig, float *d0);
 // coarse operator of level `c` from level `f`
 void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c);
-// t = b - A x
-void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
-                        const SolveState *state);
-// b_c = P0' t_f, x_c = w b_c / d_c (or 0 with w = 0)
-void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
-                        const SolveState *state);
+// b_c = P0' (b_f - A_f x_f), x_c = w b_c / d_c (or 0 with w = 0): the residual of a
+// level >= 1 and its restriction in one pass, the residual never stored
+void launch_mg_residual_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float omega,
+                                 const SolveState *state);
 // x_f += alpha P0 x_c on the fine unknowns (levels >= 1)
 void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float alpha, const SolveState *state);
-// y = x + w (b - A x) / d (levels >= 1)
-void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
+// c.t = c.x + w (c.b - A c.x) / d on a level >= 1; with `fine` (a level >= 1 above c)
+// also fine->x += alpha P0 c.t
+void launch_mg_smooth(hipStream_t st, const MgLevelDev &c, const MgLevelDev *fine, float omega, float alpha,
                       const SolveState *state);
 // level 0 (scaled system, unit diagonal on the unknowns, vectors zero elsewhere):
-// t = r - A (w r), the residual after the pre-smoothing step x0 = w r
-void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state);
+// the residual t = r - A (w r) after the pre-smoothing step x0 = w r, restricted at
+// once to level 1 c1 (c1.b = P0' t, c1.x = w_c c1.b / c1.d; t is not stored); coord
+// and parent are level 0's brick coordinates and parent slots
+void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, const int4 *coord, const int *parent,
+                    const MgLevelDev &c1, float omega_c, const SolveState *state);
 // y = x0' + w (r - A x0') with x0' = w r + alpha P0 x1 (prolongation fused into the
 // post-smoothing), one thread per cell; the r.y sums of its blocks (cell_part, one per
 // 256 cells) are folded in order into g.blocks partials `part` for the Krylov kernels

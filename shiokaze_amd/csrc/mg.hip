@@ -20,64 +20,48 @@ __device__ inline float brick_kappa(const MgOp &op, int slot) { return op.kappa 
 // diagonal of cell e: 1 in a uniform brick
 __device__ inline float diag_of(const MgOp &op, int64_t e, float kap) { return kap != 0.0f ? 1.0f : op.d[e]; }
 
-// (A x)_e for the 7-point brick operator with an explicit diagonal (FP32); in a
-// uniform brick (kap != 0: unit diagonal, every face coupled with kap, all the
-// neighbours exist) A x = x - kap * (sum of the neighbours).  A 2D grid (Y == 1) has
-// no y neighbours.
-template <int BX>
-__device__ inline float apply(const MgOp &op, int slot, int l, const float *__restrict__ x, float kap = 0.0f) {
+// (A v)_e for the 7-point brick operator of a level >= 1 (explicit diagonal and
+// couplings, FP32), where v(s, m) is the value at local cell m of slot s; a missing
+// neighbour brick adds nothing.  A 2D grid (Y == 1) has no y neighbours.
+template <int BX, class V>
+__device__ inline float apply_v(const MgOp &op, int slot, int l, V v) {
     using B = BrickShape<BX>;
     constexpr int X = B::X, Y = B::Y;
     const int64_t e = (int64_t)slot * kBrickCells + l;
     const int i = l % X, j = (l / X) % Y, k = l / kWave;
     const int *nb = op.nbr + 6 * slot;
-    if (kap != 0.0f) {
-        const float xm = i > 0 ? x[e - 1] : x[(int64_t)nb[0] * kBrickCells + l + (X - 1)];
-        const float xp = i < X - 1 ? x[e + 1] : x[(int64_t)nb[1] * kBrickCells + l - (X - 1)];
-        float ym = 0.0f, yp = 0.0f;
-        if constexpr (Y > 1) {
-            ym = j > 0 ? x[e - X] : x[(int64_t)nb[2] * kBrickCells + l + X * (Y - 1)];
-            yp = j < Y - 1 ? x[e + X] : x[(int64_t)nb[3] * kBrickCells + l - X * (Y - 1)];
-        }
-        const float zm = k > 0 ? x[e - 64] : x[(int64_t)nb[4] * kBrickCells + l + 448];
-        const float zp = k < 7 ? x[e + 64] : x[(int64_t)nb[5] * kBrickCells + l - 448];
-        return x[e] - kap * (xm + xp + ym + yp + zm + zp);
-    }
-    float acc = op.d[e] * x[e];
+    float acc = op.d[e] * v(slot, l);
     if (i > 0) {
-        acc -= op.cx[e] * x[e - 1];
+        acc -= op.cx[e] * v(slot, l - 1);
     } else if (nb[0] >= 0) {
-        acc -= op.cx[e] * x[(int64_t)nb[0] * kBrickCells + l + (X - 1)];
+        acc -= op.cx[e] * v(nb[0], l + (X - 1));
     }
     if (i < X - 1) {
-        acc -= op.cx[e + 1] * x[e + 1];
+        acc -= op.cx[e + 1] * v(slot, l + 1);
     } else if (nb[1] >= 0) {
-        const int64_t m = (int64_t)nb[1] * kBrickCells + l - (X - 1);
-        acc -= op.cx[m] * x[m];
+        acc -= op.cx[(int64_t)nb[1] * kBrickCells + l - (X - 1)] * v(nb[1], l - (X - 1));
     }
     if constexpr (Y > 1) {
         if (j > 0) {
-            acc -= op.cy[e] * x[e - X];
+            acc -= op.cy[e] * v(slot, l - X);
         } else if (nb[2] >= 0) {
-            acc -= op.cy[e] * x[(int64_t)nb[2] * kBrickCells + l + X * (Y - 1)];
+            acc -= op.cy[e] * v(nb[2], l + X * (Y - 1));
         }
         if (j < Y - 1) {
-            acc -= op.cy[e + X] * x[e + X];
+            acc -= op.cy[e + X] * v(slot, l + X);
         } else if (nb[3] >= 0) {
-            const int64_t m = (int64_t)nb[3] * kBrickCells + l - X * (Y - 1);
-            acc -= op.cy[m] * x[m];
+            acc -= op.cy[(int64_t)nb[3] * kBrickCells + l - X * (Y - 1)] * v(nb[3], l - X * (Y - 1));
         }
     }
     if (k > 0) {
-        acc -= op.cz[e] * x[e - 64];
+        acc -= op.cz[e] * v(slot, l - 64);
     } else if (nb[4] >= 0) {
-        acc -= op.cz[e] * x[(int64_t)nb[4] * kBrickCells + l + 448];
+        acc -= op.cz[e] * v(nb[4], l + 448);
     }
     if (k < 7) {
-        acc -= op.cz[e + 64] * x[e + 64];
+        acc -= op.cz[e + 64] * v(slot, l + 64);
     } else if (nb[5] >= 0) {
-        const int64_t m = (int64_t)nb[5] * kBrickCells + l - 448;
-        acc -= op.cz[m] * x[m];
+        acc -= op.cz[(int64_t)nb[5] * kBrickCells + l - 448] * v(nb[5], l - 448);
     }
     return acc;
 }
@@ -161,35 +145,41 @@ __global__ __launch_bounds__(NT) void k_mg_coarsen(MgLevelDev f, MgLevelDev c) {
     c.cx[E] = (float)cx;
     c.cy[E] = (float)cy;
     c.cz[E] = (float)cz;
+    // k_mg_residual_restrict does not write the cells without a fine brick
+    c.b[E] = 0.0f;
+    c.x[E] = 0.0f;
 }
 
+// b_c = P0' (b_f - A_f x_f), x_c = w b_c / d_c (0 with w = 0) from a level >= 1, one
+// thread per fine cell: each cell's residual goes to LDS and is never stored; the
+// first child of each aggregate sums its children in the order q (the children of an
+// aggregate lie in one 256-cell half brick, the z pairs (2m, 2m+1) never straddle
+// the halves).  Coarse cells without a fine brick are not written: their b and x
+// hold the zeros k_mg_coarsen writes, and nothing else writes there.
 template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_residual(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
-                                                    float *__restrict__ t, const SolveState *state) {
-    if (stopped(state)) return;
-    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e >= (int64_t)op.n_active * kBrickCells) return;
-    const int slot = (int)(e >> 9), l = (int)(e & 511);
-    const float kap = brick_kappa(op, slot);
-    t[e] = diag_of(op, e, kap) > 0.0f ? b[e] - apply<BX>(op, slot, l, x, kap) : 0.0f;
-}
-
-template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_restrict(MgLevelDev f, MgLevelDev c, const float *__restrict__ tf,
-                                                    float omega, const SolveState *state) {
+__global__ __launch_bounds__(NT) void k_mg_residual_restrict(MgLevelDev f, MgLevelDev c, float omega,
+                                                             const SolveState *state) {
     using A = Agg<BX>;
+    constexpr int X = A::X, Y = A::Y;
+    static_assert(kBrickCells % NT == 0 && NT % (2 * kWave) == 0, "a block holds whole z pairs of one brick");
     if (stopped(state)) return;
-    const int64_t E = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (E >= (int64_t)c.n_active * kBrickCells) return;
-    const int slot_c = (int)(E >> 9), l = (int)(E & 511);
-    int base = 0;
-    const int fs = children<BX>(f, c, slot_c, l, base);
+    __shared__ float ts[NT];
+    const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
+    if (e >= (int64_t)f.n_active * kBrickCells) return;  // whole blocks: n_active * 512 is a multiple of NT
+    const int slot = (int)(e >> 9), l = (int)(e & 511);
+    const MgOp op = f.op(0.0f, 0.0f);
+    const float *__restrict__ xf = f.x;
+    const auto v = [xf](int sl, int m) { return xf[(int64_t)sl * kBrickCells + m]; };
+    ts[threadIdx.x] = f.d[e] > 0.0f ? f.b[e] - apply_v<BX>(op, slot, l, v) : 0.0f;
+    __syncthreads();
+    const int i = l % X, j = (l / X) % Y, k = l / kWave;
+    if ((i & 1) || (A::AY > 1 && (j & 1)) || (k & 1)) return;
+    const int ps = f.parent[slot];
+    if (ps < 0) return;  // cannot happen: the parent of an active brick is active
     float s = 0.0f;
-    if (fs >= 0) {
-        const int64_t fb = (int64_t)fs * kBrickCells + base;
 #pragma unroll
-        for (int q = 0; q < A::NCH; ++q) s += tf[fb + A::child(q)];
-    }
+    for (int q = 0; q < A::NCH; ++q) s += ts[threadIdx.x + A::child(q)];
+    const int64_t E = (int64_t)ps * kBrickCells + coarse_local<BX>(f.coord[slot], l);
     const float dc = c.d[E];
     c.b[E] = dc > 0.0f ? s : 0.0f;
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
@@ -204,20 +194,37 @@ __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, f
     if (!(f.d[e] > 0.0f)) return;
     const int ps = f.parent[slot];
     if (ps < 0) return;  // cannot happen: the parent of an active brick is active
-    f.x[e] += alpha * c.x[(int64_t)ps * kBrickCells + coarse_local<BX>(f.coord[slot], l)];
+    f.x[e] = fmaf(alpha, c.x[(int64_t)ps * kBrickCells + coarse_local<BX>(f.coord[slot], l)], f.x[e]);
 }
 
-// y = x + w (b - A x) / d, one thread per cell
+// c.t = y = x + w (b - A x) / d on a level >= 1, one thread per cell; with a finer
+// level f >= 1 (prolong) the same thread also adds alpha y to the unknowns among its
+// children there (f.x += alpha P0 y, the next coarse correction, which k_mg_prolong
+// would make from the stored y)
 template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_smooth(MgOp op, const float *__restrict__ b, const float *__restrict__ x,
-                                                  float *__restrict__ y, const SolveState *state) {
+__global__ __launch_bounds__(NT) void k_mg_smooth(MgLevelDev c, MgLevelDev f, bool prolong, float omega, float alpha,
+                                                  const SolveState *state) {
+    using A = Agg<BX>;
     if (stopped(state)) return;
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
-    if (e >= (int64_t)op.n_active * kBrickCells) return;
+    if (e >= (int64_t)c.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
-    const float kap = brick_kappa(op, slot);
-    const float dv = diag_of(op, e, kap);
-    y[e] = dv > 0.0f ? x[e] + op.omega * (b[e] - apply<BX>(op, slot, l, x, kap)) / dv : 0.0f;
+    const MgOp op = c.op(omega, alpha);
+    const float dv = c.d[e];
+    const float *__restrict__ x = c.x;
+    const auto v = [x](int sl, int m) { return x[(int64_t)sl * kBrickCells + m]; };
+    const float y = dv > 0.0f ? x[e] + omega * (c.b[e] - apply_v<BX>(op, slot, l, v)) / dv : 0.0f;
+    c.t[e] = y;
+    if (!prolong) return;
+    int base = 0;
+    const int fs = children<BX>(f, c, slot, l, base);
+    if (fs < 0) return;
+    const int64_t fb = (int64_t)fs * kBrickCells + base;
+#pragma unroll
+    for (int q = 0; q < A::NCH; ++q) {
+        const int64_t ef = fb + A::child(q);
+        if (f.d[ef] > 0.0f) f.x[ef] = fmaf(alpha, y, f.x[ef]);  // one rounding, as k_mg_prolong
+    }
 }
 
 // ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
@@ -240,9 +247,9 @@ __device__ inline float xprime(const MgOp &op, const float *__restrict__ r, cons
 // shuffles (taken by all 64 lanes, outside any lane-dependent branch), z neighbours
 // from the lane's own registers, and the cells across the faces from one coalesced
 // load per neighbour plane (the neighbour brick's slot, coordinates and parent are
-// wave-uniform).  Each cell's arithmetic is that of one thread per cell applying
-// apply<BX> (pre) or summing c * x0' over its faces (post), term by term in the same
-// face order.
+// wave-uniform).  Each cell's arithmetic takes its terms in the face order of
+// apply_v<BX> (-x, +x, -y, +y, -z, +z); a uniform brick sums its six neighbours
+// first and scales the sum by its one coupling.
 
 __device__ inline float lane_get(float v, int src) { return __shfl(v, src & 63, 64); }
 
@@ -327,11 +334,13 @@ __device__ inline void couplings6(const MgOp &op, int64_t e, const int nb[6], in
     c[5] = k < 7 ? op.cz[e + 64] : (nb[5] >= 0 ? op.cz[(int64_t)nb[5] * kBrickCells + lane] : 0.0f);
 }
 
-// t0 = r - A (w r) = r - w A r on the unknowns (0 elsewhere), by linearity
+// t0 = r - A (w r) = r - w A r on the unknowns (0 elsewhere), by linearity, restricted
+// at once to level 1 (b_1 = P0' t0, x_1 = w_c b_1 / d_1)
 template <int BX>
-__global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restrict__ r, float *__restrict__ t,
+__global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restrict__ r, const int4 *__restrict__ coord,
+                                                  const int *__restrict__ parent, MgLevelDev c, float omega_c,
                                                   const SolveState *state) {
-    constexpr int Y = BrickShape<BX>::Y;
+    constexpr int X = BrickShape<BX>::X, Y = BrickShape<BX>::Y;
     if (stopped(state)) return;
     const int lane = threadIdx.x & 63;
     const int slot = blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
@@ -341,11 +350,14 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
     for (int f = 0; f < 6; ++f) nb[f] = op.nbr[6 * slot + f];
     const float kap = brick_kappa(op, slot);
     const int64_t base = (int64_t)slot * kBrickCells + lane;
+    const int ps = parent[slot];
+    const int4 Bf = coord[slot];
     float v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = r[base + 64 * k];
     float h[6];
     halo_planes<BX>(nb, lane, [&](int ns, int l) { return r[(int64_t)ns * kBrickCells + l]; }, h);
+    float t[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int64_t e = base + 64 * k;
@@ -353,7 +365,7 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
         neighbours6<BX>(v, h, lane, k, n);
         float ax;
         if (kap != 0.0f) {
-            // the y terms are 0.0f without a y axis, as in apply<BX>
+            // the y terms are 0.0f without a y axis
             ax = v[k] - kap * (n[0] + n[1] + (Y > 1 ? n[2] : 0.0f) + (Y > 1 ? n[3] : 0.0f) + n[4] + n[5]);
         } else {
             float c[6];
@@ -364,7 +376,44 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
             for (int f = 0; f < 6; ++f)
                 if (has[f]) ax -= c[f] * n[f];
         }
-        t[e] = diag_of(op, e, kap) > 0.0f ? v[k] - op.omega * ax : 0.0f;
+        t[k] = diag_of(op, e, kap) > 0.0f ? v[k] - op.omega * ax : 0.0f;
+    }
+    // b_1 = P0' t and x_1 = w_c b_1 / d_1 on the coarse cells whose children are in
+    // this brick: a cell of even (i, j, k) sums the children (i+a, j+b, k+c) in the
+    // order a, b, c (that of Agg<BX>::child), taken from lanes +1, +X, +X+1 and its
+    // own next register; t is not stored
+    float sum[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        float q[8];
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            const float tv = t[2 * m + cz];
+            q[4 * cz] = tv;
+            q[4 * cz + 1] = lane_get(tv, lane + 1);
+            q[4 * cz + 2] = Y > 1 ? lane_get(tv, lane + X) : 0.0f;
+            q[4 * cz + 3] = Y > 1 ? lane_get(tv, lane + X + 1) : 0.0f;
+        }
+        float sm = 0.0f;
+#pragma unroll
+        for (int cz = 0; cz < 2; ++cz) {
+            sm += q[4 * cz];
+            sm += q[4 * cz + 1];
+            if constexpr (Y > 1) {
+                sm += q[4 * cz + 2];
+                sm += q[4 * cz + 3];
+            }
+        }
+        sum[m] = sm;
+    }
+    const int i = lane % X, j = lane / X;
+    if ((i & 1) || (Y > 1 && (j & 1)) || ps < 0) return;  // ps < 0 cannot happen for an active brick
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const int64_t E = (int64_t)ps * kBrickCells + coarse_local<BX>(Bf, lane + 64 * (2 * m));
+        const float dc = c.d[E];
+        c.b[E] = dc > 0.0f ? sum[m] : 0.0f;
+        c.x[E] = dc > 0.0f ? omega_c * sum[m] / dc : 0.0f;
     }
 }
 
@@ -471,27 +520,25 @@ void launch_mg_coarsen(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c)
     SHK_MG_LAUNCH(k_mg_coarsen, c.bx, blocks((int64_t)c.n_active * kBrickCells), st, f, c);
 }
 
-void launch_mg_residual(hipStream_t st, const MgOp &op, const float *b, const float *x, float *t,
-                        const SolveState *state) {
-    SHK_MG_LAUNCH(k_mg_residual, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, b, x, t, state);
-}
-
-void launch_mg_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, const float *tf, float omega,
-                        const SolveState *state) {
-    SHK_MG_LAUNCH(k_mg_restrict, c.bx, blocks((int64_t)c.n_active * kBrickCells), st, f, c, tf, omega, state);
+void launch_mg_residual_restrict(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float omega,
+                                 const SolveState *state) {
+    SHK_MG_LAUNCH(k_mg_residual_restrict, f.bx, blocks((int64_t)f.n_active * kBrickCells), st, f, c, omega, state);
 }
 
 void launch_mg_prolong(hipStream_t st, const MgLevelDev &f, const MgLevelDev &c, float alpha, const SolveState *state) {
     SHK_MG_LAUNCH(k_mg_prolong, f.bx, blocks((int64_t)f.n_active * kBrickCells), st, f, c, alpha, state);
 }
 
-void launch_mg_smooth(hipStream_t st, const MgOp &op, const float *b, const float *x, float *y,
+void launch_mg_smooth(hipStream_t st, const MgLevelDev &c, const MgLevelDev *fine, float omega, float alpha,
                       const SolveState *state) {
-    SHK_MG_LAUNCH(k_mg_smooth, op.bx, blocks((int64_t)op.n_active * kBrickCells), st, op, b, x, y, state);
+    const MgLevelDev none{};
+    SHK_MG_LAUNCH(k_mg_smooth, c.bx, blocks((int64_t)c.n_active * kBrickCells), st, c, fine ? *fine : none,
+                  fine != nullptr, omega, alpha, state);
 }
 
-void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, float *t, const SolveState *state) {
-    SHK_MG_LAUNCH(k_mg_pre0_w, op.bx, blocks((int64_t)op.n_active * 64), st, op, r, t, state);
+void launch_mg_pre0(hipStream_t st, const MgOp &op, const float *r, const int4 *coord, const int *parent,
+                    const MgLevelDev &c1, float omega_c, const SolveState *state) {
+    SHK_MG_LAUNCH(k_mg_pre0_w, op.bx, blocks((int64_t)op.n_active * 64), st, op, r, coord, parent, c1, omega_c, state);
 }
 
 void launch_mg_post0(hipStream_t st, const MgOp &op, const int4 *coord, const int *parent, const float *r,

@@ -149,9 +149,8 @@ int build_tables(shk_ctx *c, MgHierarchy *h) {
         } else {
             HIPC(c, L.d.ensure(n));
         }
-        if (l > 0)
-            for (DevBuf<float> *b : {&L.b, &L.x}) HIPC(c, b->ensure(n));
-        HIPC(c, L.t.ensure(n));  // level 0 keeps only the pre-smoothing residual
+        if (l > 0)  // level 0 keeps no vectors: its residual is restricted where it is computed
+            for (DevBuf<float> *b : {&L.b, &L.x, &L.t}) HIPC(c, b->ensure(n));
         if (!L.h_parent.empty()) {
             HIPC(c, L.parent.ensure(L.h_parent.size()));
             HIPC(c, hipMemcpyAsync(L.parent.p, L.h_parent.data(), sizeof(int) * L.h_parent.size(),
@@ -306,23 +305,25 @@ void mg_vcycle(shk_ctx *c, const float *r, float *z, const SolveState *state, co
     MgLevelHost &L0 = h->lv[0];
     const float w = (float)c->prm.mg_omega, a = (float)c->prm.mg_alpha;
     const MgOp op0{c->nbr.p, L0.d.p, c->hx32.p, c->hy32.p, c->hz32.p, c->n_active, w, a, c->kappa32.p, c->dims.bx};
-    launch_mg_pre0(st, op0, r, L0.v.t, state);
+    // down: level 1 from the level-0 residual in the pre-smoothing pass, every later
+    // level from the residual of the level above, computed where it is restricted
+    // (x_L = 0: w = 0)
+    launch_mg_pre0(st, op0, r, c->coord.p, L0.parent.p, h->lv[1].v, L > 1 ? w : 0.0f, state);
     for (int l = 1; l < L; ++l) {
-        MgLevelHost &F = h->lv[l - 1], &C = h->lv[l];
+        MgLevelHost &F = h->lv[l], &C = h->lv[l + 1];
         if (!C.v.n_active) continue;
-        launch_mg_restrict(st, F.v, C.v, F.v.t, w, state);
-        launch_mg_residual(st, C.v.op(w, a), C.v.b, C.v.x, C.v.t, state);
+        launch_mg_residual_restrict(st, F.v, C.v, l + 1 < L ? w : 0.0f, state);
     }
     MgLevelHost &Cl = h->lv[L];
-    if (Cl.v.n_active) {
-        launch_mg_restrict(st, h->lv[L - 1].v, Cl.v, h->lv[L - 1].v.t, 0.0f, state);
-        launch_mg_coarse(st, h->ncoarse, h->cidx.p, h->cinv.p, Cl.v.b, Cl.v.x, kBrickCells, state);
-    }
+    if (Cl.v.n_active) launch_mg_coarse(st, h->ncoarse, h->cidx.p, h->cinv.p, Cl.v.b, Cl.v.x, kBrickCells, state);
+    // up: the correction from the coarsest level, then the post-smoothing of each level,
+    // which also applies the correction of the level above it (level 0's is applied
+    // in its post-smoothing)
+    if (L > 1 && h->lv[L - 1].v.n_active) launch_mg_prolong(st, h->lv[L - 1].v, Cl.v, a, state);
     for (int l = L - 1; l >= 1; --l) {
-        MgLevelHost &F = h->lv[l], &C = h->lv[l + 1];
+        MgLevelHost &F = h->lv[l];
         if (!F.v.n_active) continue;
-        launch_mg_prolong(st, F.v, C.v, a, state);
-        launch_mg_smooth(st, F.v.op(w, a), F.v.b, F.v.x, F.v.t, state);
+        launch_mg_smooth(st, F.v, l > 1 ? &h->lv[l - 1].v : nullptr, w, a, state);
         std::swap(F.v.x, F.v.t);
         std::swap(F.x.p, F.t.p);
     }
