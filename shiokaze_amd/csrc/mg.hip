@@ -167,20 +167,22 @@ __global__ __launch_bounds__(NT) void k_mg_residual_restrict(MgLevelDev f, MgLev
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)f.n_active * kBrickCells) return;  // whole blocks: n_active * 512 is a multiple of NT
     const int slot = (int)(e >> 9), l = (int)(e & 511);
+    // every load is issued before the barrier, none waits on the residual: the parent
+    // cell's diagonal, then the residual's operands (all in bounds whatever d is)
+    const int ps = f.parent[slot];  // >= 0: the parent of an active brick is active
+    const int64_t E = (int64_t)(ps > 0 ? ps : 0) * kBrickCells + coarse_local<BX>(f.coord[slot], l);
+    const float dc = c.d[E];
     const MgOp op = f.op(0.0f, 0.0f);
     const float *__restrict__ xf = f.x;
     const auto v = [xf](int sl, int m) { return xf[(int64_t)sl * kBrickCells + m]; };
-    ts[threadIdx.x] = f.d[e] > 0.0f ? f.b[e] - apply_v<BX>(op, slot, l, v) : 0.0f;
+    const float ax = apply_v<BX>(op, slot, l, v);
+    ts[threadIdx.x] = f.d[e] > 0.0f ? f.b[e] - ax : 0.0f;
     __syncthreads();
     const int i = l % X, j = (l / X) % Y, k = l / kWave;
-    if ((i & 1) || (A::AY > 1 && (j & 1)) || (k & 1)) return;
-    const int ps = f.parent[slot];
-    if (ps < 0) return;  // cannot happen: the parent of an active brick is active
+    if ((i & 1) || (A::AY > 1 && (j & 1)) || (k & 1) || ps < 0) return;
     float s = 0.0f;
 #pragma unroll
     for (int q = 0; q < A::NCH; ++q) s += ts[threadIdx.x + A::child(q)];
-    const int64_t E = (int64_t)ps * kBrickCells + coarse_local<BX>(f.coord[slot], l);
-    const float dc = c.d[E];
     c.b[E] = dc > 0.0f ? s : 0.0f;
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
@@ -209,22 +211,26 @@ __global__ __launch_bounds__(NT) void k_mg_smooth(MgLevelDev c, MgLevelDev f, bo
     const int64_t e = blockIdx.x * (int64_t)NT + threadIdx.x;
     if (e >= (int64_t)c.n_active * kBrickCells) return;
     const int slot = (int)(e >> 9), l = (int)(e & 511);
-    const MgOp op = c.op(omega, alpha);
-    const float dv = c.d[e];
-    const float *__restrict__ x = c.x;
-    const auto v = [x](int sl, int m) { return x[(int64_t)sl * kBrickCells + m]; };
-    const float y = dv > 0.0f ? x[e] + omega * (c.b[e] - apply_v<BX>(op, slot, l, v)) / dv : 0.0f;
-    c.t[e] = y;
-    if (!prolong) return;
+    // the children's d and x are loaded with the smoothing's operands, not after them
     int base = 0;
-    const int fs = children<BX>(f, c, slot, l, base);
-    if (fs < 0) return;
-    const int64_t fb = (int64_t)fs * kBrickCells + base;
+    const int fs = prolong ? children<BX>(f, c, slot, l, base) : -1;
+    const int64_t fb = (int64_t)(fs > 0 ? fs : 0) * kBrickCells + base;
+    float fd[A::NCH], fx[A::NCH];
 #pragma unroll
     for (int q = 0; q < A::NCH; ++q) {
-        const int64_t ef = fb + A::child(q);
-        if (f.d[ef] > 0.0f) f.x[ef] = fmaf(alpha, y, f.x[ef]);  // one rounding, as k_mg_prolong
+        fd[q] = fs >= 0 ? f.d[fb + A::child(q)] : 0.0f;
+        fx[q] = fs >= 0 ? f.x[fb + A::child(q)] : 0.0f;
     }
+    const MgOp op = c.op(omega, alpha);
+    const float *__restrict__ x = c.x;
+    const auto v = [x](int sl, int m) { return x[(int64_t)sl * kBrickCells + m]; };
+    const float dv = c.d[e], xe = x[e], be = c.b[e];
+    const float ax = apply_v<BX>(op, slot, l, v);  // all loads in bounds whatever d is
+    const float y = dv > 0.0f ? xe + omega * (be - ax) / dv : 0.0f;
+    c.t[e] = y;
+#pragma unroll
+    for (int q = 0; q < A::NCH; ++q)
+        if (fd[q] > 0.0f) f.x[fb + A::child(q)] = fmaf(alpha, y, fx[q]);  // one rounding, as k_mg_prolong
 }
 
 // ---- level 0, fused.  On the scaled system every level-0 vector is zero off the
@@ -350,8 +356,11 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
     for (int f = 0; f < 6; ++f) nb[f] = op.nbr[6 * slot + f];
     const float kap = brick_kappa(op, slot);
     const int64_t base = (int64_t)slot * kBrickCells + lane;
-    const int ps = parent[slot];
+    const int ps = parent[slot];  // >= 0: the parent of an active brick is active
     const int4 Bf = coord[slot];
+    float dcm[4];  // level-1 diagonals of the cells this lane may restrict to, loaded early
+#pragma unroll
+    for (int m = 0; m < 4; ++m) dcm[m] = c.d[(int64_t)(ps > 0 ? ps : 0) * kBrickCells + coarse_local<BX>(Bf, lane + 128 * m)];
     float v[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) v[k] = r[base + 64 * k];
@@ -411,7 +420,7 @@ __global__ __launch_bounds__(NT) void k_mg_pre0_w(MgOp op, const float *__restri
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
         const int64_t E = (int64_t)ps * kBrickCells + coarse_local<BX>(Bf, lane + 64 * (2 * m));
-        const float dc = c.d[E];
+        const float dc = dcm[m];
         c.b[E] = dc > 0.0f ? sum[m] : 0.0f;
         c.x[E] = dc > 0.0f ? omega_c * sum[m] / dc : 0.0f;
     }
