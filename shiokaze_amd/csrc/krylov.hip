@@ -415,16 +415,50 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
     chunk_range(tiles, per, t0, t1);
     const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
     double rr = 0.0, rm = 0.0;
-    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
-        x[e] += a * s[e];
-        const T rv = r[e] - a * q[e];
-        r[e] = rv;
-        rr += (double)rv * (double)rv;
-        if (SCALED) {
-            const double sg = sig[e];
-            if (sg > 0.0) rm = fmax(rm, fabs((double)rv) / sg);
-        } else {
-            rm = fmax(rm, fabs((double)rv));
+    if constexpr (sizeof(T) == 4 && !SCALED) {
+        // FP32 vectors: four consecutive elements per thread in 16-byte accesses (tile
+        // boundaries are multiples of 4, the arrays 256-byte aligned; a remainder of
+        // fewer than 4 elements at the end goes through the scalar loop); each
+        // element's arithmetic is that of the scalar loop below
+        using T4 = HIP_vector_type<T, 4>;
+        T4 *__restrict__ x4 = reinterpret_cast<T4 *>(x);
+        T4 *__restrict__ r4 = reinterpret_cast<T4 *>(r);
+        const T4 *__restrict__ s4 = reinterpret_cast<const T4 *>(s);
+        const T4 *__restrict__ q4 = reinterpret_cast<const T4 *>(q);
+        for (int64_t e = (int64_t)t0 * (NT / 4) + threadIdx.x; e < end / 4; e += NB) {
+            T4 xv = x4[e], rv = r4[e];
+            const T4 sv = s4[e], qv = q4[e];
+            xv.x += a * sv.x;
+            xv.y += a * sv.y;
+            xv.z += a * sv.z;
+            xv.w += a * sv.w;
+            rv.x = rv.x - a * qv.x;
+            rv.y = rv.y - a * qv.y;
+            rv.z = rv.z - a * qv.z;
+            rv.w = rv.w - a * qv.w;
+            x4[e] = xv;
+            r4[e] = rv;
+            rr += (double)rv.x * (double)rv.x;
+            rr += (double)rv.y * (double)rv.y;
+            rr += (double)rv.z * (double)rv.z;
+            rr += (double)rv.w * (double)rv.w;
+            rm = fmax(rm, fmax(fmax(fabs((double)rv.x), fabs((double)rv.y)), fmax(fabs((double)rv.z), fabs((double)rv.w))));
+        }
+    }
+    {
+        const int64_t first = (int64_t)t0 * NT;  // a block past the last tile has end <= first
+        const int64_t start = (sizeof(T) == 4 && !SCALED) ? (end / 4) * 4 > first ? (end / 4) * 4 : first : first;
+        for (int64_t e = start + threadIdx.x; e < end; e += NB) {
+            x[e] += a * s[e];
+            const T rv = r[e] - a * q[e];
+            r[e] = rv;
+            rr += (double)rv * (double)rv;
+            if (SCALED) {
+                const double sg = sig[e];
+                if (sg > 0.0) rm = fmax(rm, fabs((double)rv) / sg);
+            } else {
+                rm = fmax(rm, fabs((double)rv));
+            }
         }
     }
     const double bs = block_sum<NB>(rr, lds);
