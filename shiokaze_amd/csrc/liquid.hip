@@ -210,7 +210,9 @@ __global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Faces f, uint
         float *a = f.v[dim];
         int i, j, k;
         decompose(q, s.nx, s.ny, i, j, k);
-        const uint8_t own = m[q];   // issued together with the neighbour loads below
+        // an active face loads nothing more: the round is bound by the rate of byte
+        // loads, not by the latency of the two-step chain
+        const uint8_t own = m[q];
         const int di[6] = {1, -1, 0, 0, 0, 0}, dj[6] = {0, 0, 1, -1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
         int64_t e[6];
         bool use[6];
@@ -220,9 +222,17 @@ __global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Faces f, uint
             const int ii = i + di[t], jj = j + dj[t], kk = k + dk[t];
             const bool in = ii >= 0 && jj >= 0 && kk >= 0 && ii < s.nx && jj < s.ny && kk < s.nz;
             e[t] = s.at(clampi(ii, 0, s.nx - 1), clampi(jj, 0, s.ny - 1), clampi(kk, 0, s.nz - 1));
-            const uint8_t nm = m[e[t]];
-            use[t] = in && nm != 0 && nm <= round;
-            any = any || use[t];
+            use[t] = false;
+        }
+        if (own == 0) {
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                const int ii = i + di[t], jj = j + dj[t], kk = k + dk[t];
+                const bool in = ii >= 0 && jj >= 0 && kk >= 0 && ii < s.nx && jj < s.ny && kk < s.nz;
+                const uint8_t nm = m[e[t]];
+                use[t] = in && nm != 0 && nm <= round;
+                any = any || use[t];
+            }
         }
         float sum = 0.0f;
         int weight = 0;
