@@ -187,6 +187,8 @@ __global__ __launch_bounds__(NT) void k_mg_residual_restrict(MgLevelDev f, MgLev
     c.x[E] = dc > 0.0f ? omega * s / dc : 0.0f;
 }
 
+// x_f += alpha P0 x_c on the unknowns of a level f >= 1, one thread per cell (the
+// level above the coarsest; the others get theirs from k_mg_smooth)
 template <int BX>
 __global__ __launch_bounds__(NT) void k_mg_prolong(MgLevelDev f, MgLevelDev c, float alpha, const SolveState *state) {
     if (stopped(state)) return;
