@@ -88,12 +88,38 @@ __device__ inline float rd(const float *a, const uint8_t *m, int64_t q) {
 
 // (i, j, k) of a linear index with 32-bit unsigned arithmetic (shk_create bounds the
 // cells by 2^31, so every face array has fewer than 2^32 entries)
-__device__ inline void decompose(int64_t q, int nx, int ny, int &i, int &j, int &k) {
+// Exact division of a 32-bit index by an extent n without a divide sequence:
+// floor(q / n) = floor(m q / 2^64) with m = ceil(2^64 / n) for every 32-bit q and
+// 2 <= n < 2^32 (Lemire, Kaser and Kurz, "Faster remainder by direct computation",
+// 2019).  m is computed on the host; n = 1 takes q itself.
+struct Div32 {
+    uint32_t n;
+    uint64_t m;
+};
+inline Div32 div32(int n) { return Div32{(uint32_t)n, n > 1 ? ~0ull / (uint64_t)n + 1 : 0ull}; }
+__device__ inline uint32_t quot(uint32_t q, Div32 d) {
+    return d.n > 1 ? (uint32_t)__umul64hi(d.m, (uint64_t)q) : q;
+}
+
+// the x and y extents of an index space (a face or cell array), for decompose
+struct Split {
+    Div32 x, y;
+};
+inline Split split_of(FS s) { return Split{div32(s.nx), div32(s.ny)}; }
+struct Split3 {  // the three face arrays
+    Split a[3];
+};
+inline Split cells_split(const Dims &d) { return split_of(FS{d.nx, d.ny, d.nz}); }
+inline Split3 faces_split(const Dims &d) {
+    return Split3{{split_of(face_fs(d, 0)), split_of(face_fs(d, 1)), split_of(face_fs(d, 2))}};
+}
+
+__device__ inline void decompose(int64_t q, Split sp, int &i, int &j, int &k) {
     const uint32_t u = (uint32_t)q;
-    const uint32_t t = u / (uint32_t)nx;
-    i = (int)(u - t * (uint32_t)nx);
-    k = (int)(t / (uint32_t)ny);
-    j = (int)(t - (uint32_t)k * (uint32_t)ny);
+    const uint32_t t = quot(u, sp.x);
+    i = (int)(u - t * sp.x.n);
+    k = (int)quot(t, sp.y);
+    j = (int)(t - (uint32_t)k * sp.y.n);
 }
 
 __device__ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -198,7 +224,7 @@ __device__ inline void full_velocity(const Dims &d, const CFaces &f, int dim, in
 // activates reads as inactive whether its write has landed or not, and the values
 // read are final.  Only inactive faces do work; active ones read one byte.  A last
 // pass turns the round numbers back into 1.  blockIdx.y is the axis.
-__global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Faces f, uint8_t *m0, uint8_t *m1, uint8_t *m2,
+__global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Split3 sp, Faces f, uint8_t *m0, uint8_t *m1, uint8_t *m2,
                                                             int round, unsigned int *slots) {
 #pragma clang fp contract(off)
     const int dim = blockIdx.y;
@@ -209,7 +235,7 @@ __global__ __launch_bounds__(kNT) void k_extrapolate_round(Dims d, Faces f, uint
     if (q < s.n()) {
         float *a = f.v[dim];
         int i, j, k;
-        decompose(q, s.nx, s.ny, i, j, k);
+        decompose(q, sp.a[dim], i, j, k);
         // an active face loads nothing more: the round is bound by the rate of byte
         // loads, not by the latency of the two-step chain
         const uint8_t own = m[q];
@@ -302,13 +328,13 @@ __global__ __launch_bounds__(kNT) void k_mask_normalise(Dims d, uint8_t *m0, uin
 // macutility3::compute_max_u (macutility3.cpp:48-60): over cells with all six faces
 // active, the squared length of the float cell velocity (float squares summed in
 // double, vec.h:272-276); one partial max per block, the host takes the sqrt
-__global__ __launch_bounds__(kNT) void k_max_u2(Dims d, CFaces f, double *part) {
+__global__ __launch_bounds__(kNT) void k_max_u2(Dims d, Split sp, CFaces f, double *part) {
 #pragma clang fp contract(off)
     const int64_t G = (int64_t)d.nx * d.ny * d.nz;
     double best = 0.0;
     for (int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x; c < G; c += (int64_t)gridDim.x * kNT) {
         int i, j, k;
-        decompose(c, d.nx, d.ny, i, j, k);
+        decompose(c, sp, i, j, k);
         float lo_v[3], hi_v[3];
         bool valid = true;
 #pragma unroll
@@ -348,14 +374,14 @@ __global__ __launch_bounds__(kNT) void k_max_u2(Dims d, CFaces f, double *part) 
 // `save`) points into it loses the normal part; then the y axis's floor and ceiling
 // rule.  `solid` is the nodal level set ((nx+1)(ny+1)(nz+1)).
 template <int DIM>
-__global__ __launch_bounds__(kNT) void k_constrain(Dims d, const float *__restrict__ solid, double dx, CFaces save,
+__global__ __launch_bounds__(kNT) void k_constrain(Dims d, Split sp, const float *__restrict__ solid, double dx, CFaces save,
                                                    float *__restrict__ vel, const uint8_t *__restrict__ mask) {
 #pragma clang fp contract(off)
     const FS s = face_fs(d, DIM);
     const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
     if (q >= s.n() || !mask[q]) return;
     int i, j, k;
-    decompose(q, s.nx, s.ny, i, j, k);
+    decompose(q, sp, i, j, k);
     float out = vel[q];
     bool changed = false;
     if (solid) {
@@ -434,7 +460,7 @@ __global__ __launch_bounds__(kNT) void k_constrain(Dims d, const float *__restri
 // With fin != 0 it is the backward pass and combines the result straight into the
 // final MacCormack value (macadvection3.cpp:150-163).
 template <int DIM>
-__global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, double dx, double dt, CFaces src, float *out,
+__global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, Split sp, double dx, double dt, CFaces src, float *out,
                                                          float *lo, float *hi, uint8_t *band, const float *fluid,
                                                          int trim, const float *fin_vin, const float *fin_lo,
                                                          const float *fin_hi, const uint8_t *fin_band,
@@ -449,7 +475,7 @@ __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, double dx, doub
         return;
     }
     int i, j, k;
-    decompose(q, s.nx, s.ny, i, j, k);
+    decompose(q, sp, i, j, k);
     float uf[3];
     full_velocity(d, src, DIM, i, j, k, uf);
     const bool empty = uf[0] == 0.0f && uf[1] == 0.0f && uf[2] == 0.0f;
@@ -562,7 +588,7 @@ __device__ inline bool cell_velocity(const Dims &d, const CFaces &f, int i, int 
 // the level set before the step at the departure point.  Backward pass (fin_out !=
 // 0): the semi-Lagrangian value of q0 over -dt, combined into the final value.
 // Inactive cells of the output are left alone (they keep what they read).
-__global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, double dx, double dt, const float *q, const uint8_t *mq,
+__global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, Split sp, double dx, double dt, const float *q, const uint8_t *mq,
                                                      CFaces vel, const float *fluid, int trim, float *out, float *lo,
                                                      float *hi, uint8_t *band, const float *fin_qin,
                                                      const float *fin_lo, const float *fin_hi,
@@ -579,7 +605,7 @@ __global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, double dx, double d
         return;
     }
     int i, j, k;
-    decompose(c, d.nx, d.ny, i, j, k);
+    decompose(c, sp, i, j, k);
     double u[3];
     const bool moving = cell_velocity(d, vel, i, j, k, u);
     double p[3] = {(double)i, (double)j, (double)k};
@@ -754,7 +780,7 @@ int shk_extrapolate_velocity(shk_ctx *c, float *u, float *v, float *w, uint8_t *
                            mu, mv, mw, vec);
     }
     for (int r = 1; r <= width; ++r) {
-        hipLaunchKernelGGL(k_extrapolate_round, dim3(grid, 3), dim3(kNT), 0, c->stream, d, f, mu, mv, mw, r,
+        hipLaunchKernelGGL(k_extrapolate_round, dim3(grid, 3), dim3(kNT), 0, c->stream, d, faces_split(d), f, mu, mv, mw, r,
                            lw->slots.p);
         HIPC(c, hipGetLastError());
     }
@@ -790,11 +816,14 @@ int shk_constrain_velocity(shk_ctx *c, const float *solid, float *u, float *v, f
         save.v[a] = lw->a[a].p;
         save.m[a] = um[a];
     }
-    hipLaunchKernelGGL(k_constrain<0>, dim3(blocks_for(face_fs(d, 0).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+    hipLaunchKernelGGL(k_constrain<0>, dim3(blocks_for(face_fs(d, 0).n())), dim3(kNT), 0, c->stream, d,
+                       split_of(face_fs(d, 0)), solid, c->dx,
                        save, u, mu);
-    hipLaunchKernelGGL(k_constrain<1>, dim3(blocks_for(face_fs(d, 1).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+    hipLaunchKernelGGL(k_constrain<1>, dim3(blocks_for(face_fs(d, 1).n())), dim3(kNT), 0, c->stream, d,
+                       split_of(face_fs(d, 1)), solid, c->dx,
                        save, v, mv);
-    hipLaunchKernelGGL(k_constrain<2>, dim3(blocks_for(face_fs(d, 2).n())), dim3(kNT), 0, c->stream, d, solid, c->dx,
+    hipLaunchKernelGGL(k_constrain<2>, dim3(blocks_for(face_fs(d, 2).n())), dim3(kNT), 0, c->stream, d,
+                       split_of(face_fs(d, 2)), solid, c->dx,
                        save, w, mw);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -813,7 +842,7 @@ int shk_compute_max_u(shk_ctx *c, const float *u, const float *v, const float *w
     HIPC(c, lw->part.ensure(kMaxPartials));
     lw->h_part.resize(blocks);
     const CFaces f{{u, v, w}, {mu, mv, mw}};
-    hipLaunchKernelGGL(k_max_u2, dim3(blocks), dim3(kNT), 0, c->stream, d, f, lw->part.p);
+    hipLaunchKernelGGL(k_max_u2, dim3(blocks), dim3(kNT), 0, c->stream, d, cells_split(d), f, lw->part.p);
     HIPC(c, hipGetLastError());
     HIPC(c, hipMemcpyAsync(lw->h_part.data(), lw->part.p, sizeof(double) * blocks, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
@@ -853,7 +882,7 @@ int shk_advect_velocity(shk_ctx *c, double dt, float *u, float *v, float *w, con
     const int trim = c->prm.trim_narrowband;
 #define SHK_SL(DIM, SRC, OUT, LO, HI, BAND, DT, FV, FL, FH, FB, F0, FO)                                            \
     hipLaunchKernelGGL(k_semi_lagrangian<DIM>, dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, c->stream, d, \
-                       c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO)
+                       split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO)
     // forward pass: v0 (the result itself without MacCormack) and the limiter data
     SHK_SL(0, vin, const_cast<float *>(v0.v[0]), mc ? lw->lo[0].p : nullptr, mc ? lw->hi[0].p : nullptr,
            mc ? lw->band[0].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -897,12 +926,12 @@ int shk_advect_levelset(shk_ctx *c, double dt, float *phi, const uint8_t *mphi, 
     }
     const CFaces vel{{u, v, w}, {mu, mv, mw}};
     const float *qin = lw->a[0].p;
-    hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, c->dx, dt, qin, mphi, vel, qin,
+    hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, dt, qin, mphi, vel, qin,
                        c->prm.trim_narrowband, mc ? lw->b[0].p : phi, mc ? lw->lo[0].p : nullptr,
                        mc ? lw->hi[0].p : nullptr, mc ? lw->band[0].p : nullptr, nullptr, nullptr, nullptr, nullptr,
                        nullptr, nullptr, mc ? 1 : 0, background);
     if (mc)
-        hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, c->dx, -dt, lw->b[0].p,
+        hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, -dt, lw->b[0].p,
                            mphi, vel, qin, c->prm.trim_narrowband, nullptr, nullptr, nullptr, nullptr, qin,
                            lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background);
     HIPC(c, hipGetLastError());
