@@ -12,8 +12,17 @@ algorithmic bytes per launch / its average launch duration, timed with HIP event
 on the library's stream (ProfileKernels=1) over --profile-steps further steps of
 the same context right after the timed ones (the events add a little time per
 launch, so the timed steps themselves run without them).
-`cpu_baseline` times the oracle (the FP64 CPU restatement of the reference's
-CG, with its discarded MIC(0) work) on a bounded sample of the same workload.
+`cpu_baseline` is the oracle (the FP64 CPU restatement of the reference's path) on the
+host: the whole projection with the CG-equivalent solve to the same stop, timed in full
+on the host threads, beside the reference-faithful serial CG (its discarded MIC(0) work
+included) extrapolated from 10 and 30 timed iterations.  The same leg solves the system
+to 1e-12 with the oracle's FP64 Jacobi-PCG: the truth of `accuracy`.
+`accuracy` holds the pressure rel-L2 against that truth and the FP64 post-projection
+divergence, with `meets_bar` for the bars (1e-5, 1e-6) and `bar_point`, the loosest
+max-norm stop at which the solver meets both bars and its solves/s.
+`north_star` times plain FP64 GPU CG (the reference's iteration), multigrid PCG and
+the learned preconditioner in flexible CG in the same run, with their ratios to plain
+FP64 CG, their accuracy and their rooflines.
 """
 from __future__ import annotations
 
@@ -174,6 +183,89 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
     return out
 
 
+PRESSURE_BAR = 1e-5   # pressure rel-L2 against the FP64 solve to 1e-12
+DIVERGENCE_BAR = 1e-6  # FP64 post-projection divergence relative to the initial one
+# candidate max-norm stops for the loosest setting that meets both bars
+BAR_STOPS = (1e-6, 5e-7, 2e-7, 1e-7, 5e-8, 2e-8, 1e-8, 5e-9, 2e-9, 1e-9, 5e-10, 2e-10, 1e-10)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Host threads for the oracle's threaded solves: the process's CPU share (at most 16,
+    the share of one GPU on the box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_leg(args, sc, residual):
+    """The oracle on the host: the system (fractions, labels, CSR assembly), the FP64
+    truth (Jacobi-PCG to 1e-12 on T threads), the CG-equivalent solve to the bench
+    tolerance timed in full on T threads, and the reference-faithful CG (MIC(0) factor and
+    two discarded triangular solves per iteration, one thread) timed for 10 iterations
+    and extrapolated to the CG count."""
+    from oracle import oracle as O
+    T = cpu_threads()
+    O.set_threads(T)
+    sy = O.system(sc)
+    log(f"cpu: system N={sy['n']} in {sy['seconds']:.2f}s on {T} threads")
+    xt, it_t, rr_t, st_t, sec_t = O.pcg_threaded(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], 1e-12, 100000,
+                                                 O.SHK_OR_JACOBI)
+    log(f"cpu: truth Jacobi-PCG to 1e-12: {it_t} iterations in {sec_t:.1f}s (status {st_t})")
+    xc, it_c, rr_c, st_c, sec_c = O.pcg_threaded(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], residual, 100000,
+                                                 O.SHK_OR_CG)
+    log(f"cpu: CG to {residual:g}: {it_c} iterations in {sec_c:.1f}s")
+    # the scatter and velocity update of the same projection (threaded in the oracle)
+    t0 = time.perf_counter()
+    p = O.scatter(sy["index_map"], xc)
+    areas, rhos = O.area_fraction(sc["solid"]), O.fluid_fraction(sc["fluid"])
+    O.update_velocity(1.0, sc["dx"], p, areas, rhos, [sc["us"], sc["vs"], sc["ws"]], [sc["u"], sc["v"], sc["w"]])
+    sec_u = time.perf_counter() - t0
+    total = sy["seconds"] + sec_c + sec_u
+    # reference-faithful single-thread sample
+    O.set_threads(1)
+    t10 = O.pcg(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], residual, 10, O.SHK_OR_REFERENCE)
+    ta = time.perf_counter()
+    O.pcg(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], residual, 10, O.SHK_OR_REFERENCE)
+    t10s = time.perf_counter() - ta
+    ta = time.perf_counter()
+    O.pcg(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], residual, 30, O.SHK_OR_REFERENCE)
+    t30s = time.perf_counter() - ta
+    del t10
+    per_it = (t30s - t10s) / 20.0
+    fixed = max(t10s - 10 * per_it, 0.0)
+    ref_total = sy["seconds"] + fixed + it_c * per_it + sec_u
+    O.set_threads(T)
+    lab = sy["index_map"] >= 0
+    truth = np.zeros(sy["index_map"].shape)
+    truth[lab] = xt[sy["index_map"][lab]]
+    cg_dense = np.zeros(sy["index_map"].shape)
+    cg_dense[lab] = xc[sy["index_map"][lab]]
+    cpu = {"value": 1.0 / total, "unit": "solves/s", "cores": T, "kind": "port",
+           "sample": (f"{args.n}^3 scene {args.scene}: the whole projection in the oracle (fractions, labels, CSR "
+                      f"assembly, CG to {residual:g} with the reference's max-norm stop, scatter, velocity update), "
+                      f"timed in full on {T} host threads: {it_c} iterations, solve {sec_c:.2f} s"),
+           "ms_per_solve": round(1e3 * total, 1), "iters": it_c, "cpu_model": cpu_model(),
+           "reference_faithful_1_core": {
+               "solves_per_s_extrapolated": 1.0 / ref_total, "ms_per_iteration": round(1e3 * per_it, 2),
+               "note": (f"the reference's serial PCG (MIC(0) factor, two discarded triangular solves per iteration) "
+                        f"on one thread, timed for 10 and 30 iterations and extrapolated to {it_c} iterations")},
+           "truth": {"mode": "Jacobi-PCG", "residual": 1e-12, "iters": it_t, "seconds": round(sec_t, 2),
+                     "status": st_t}}
+    return cpu, truth, cg_dense
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,13 +280,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--extra", action="store_true", help="also time FP64 Jacobi and plain CG solves")
+    ap.add_argument("--no-north-star", action="store_true",
+                    help="skip the FP64 plain CG, multigrid and learned-preconditioner lines")
+    ap.add_argument("--extra", action="store_true", help="also time FP64 Jacobi, the 2D projection and the liquid step")
     ap.add_argument("--profile-steps", type=int, default=2,
                     help="steps after the timed ones with per-kernel HIP event timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a known-size kernel for HBM counter scaling)")
     ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128_256.bin"),
-                    help="trained network for the learned-preconditioner line of --extra")
+                    help="trained network for the learned-preconditioner line")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")
     ap.add_argument("--liquid", action="store_true",
@@ -244,6 +338,14 @@ def main():
             vel[a].copy_(vel0[a])
         return ps.project(1.0, vel, dev["solid"], svel, dev["fluid"])
 
+    def timed(ps, k):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(k):
+            r = step(ps)
+        torch.cuda.synchronize()
+        return k / (time.perf_counter() - t1), r
+
     cfg = {"Residual": args.residual, "Precision": args.precision, "Preconditioner": args.preconditioner}
     if args.preconditioner == "unet":
         cfg["UnetWeights"] = args.unet_weights
@@ -284,9 +386,6 @@ def main():
             if cnt == 0:
                 continue
             avg_us = 1e3 * ms / cnt
-            if k.startswith("finish"):
-                kernels[k] = {"launches": cnt, "avg_us": round(avg_us, 3), "total_ms": round(ms, 3)}
-                continue
             bpc = kernel_bytes_per_cell(k, args.precision, args.preconditioner)
             traffic = bpc * n_cells
             gbs = traffic / (avg_us * 1e-6) / 1e9
@@ -306,189 +405,163 @@ def main():
             solve_ms = sum(kernels[k]["total_ms"] for k in kernels)
             roofline["krylov_loop_GBs"] = round(solve_bytes / (solve_ms * 1e-3) / 1e9, 1)
 
-    # ---- accuracy against an FP64 truth solve (Jacobi PCG to 1e-12 on the GPU; the
-    # FP64 GPU path is checked against the CPU oracle by the parity tests)
+    # ---- the CPU leg: the oracle's FP64 truth, its CG-equivalent solve timed in full and
+    # the reference-faithful serial CG (see cpu_leg)
+    cpu = None
+    truth_dense = cpu_cg_dense = None
+    if not args.no_cpu_baseline:
+        cpu, truth_dense, cpu_cg_dense = cpu_leg(args, sc, args.residual)
+        if n <= 96:
+            # small grids (config 1): the whole reference-faithful projection on one host
+            # core, and the GPU pressure compared with the CPU one of the same run
+            from oracle import oracle as O
+            O.set_threads(1)
+            rr = O.project_timed(sc, tol=args.residual, mode=O.SHK_OR_REFERENCE)
+            total_ms = float(sum(rr["t_ms"][:3]))
+            p_gpu = ps.get_pressure().cpu().numpy().astype(np.float64)
+            p_cpu = np.asarray(rr["pressure"], dtype=np.float64)
+            cpu["reference_faithful_1_core"] = {
+                "solves_per_s": 1e3 / total_ms, "iters": rr["iters"],
+                "note": "the whole reference-faithful projection timed on one thread (not extrapolated)",
+                "pressure_rel_l2_gpu_vs_cpu": float(np.linalg.norm(p_gpu - p_cpu)
+                                                    / max(np.linalg.norm(p_cpu), 1e-300))}
+            O.set_threads(cpu_threads())
+        log(f"cpu baseline: {cpu}")
+
+    # ---- accuracy against the FP64 truth (the oracle's Jacobi-PCG to 1e-12 on the host;
+    # without the CPU leg, the GPU's FP64 Jacobi-PCG to 1e-12)
     accuracy = {}
-    truth_x = None  # (unknown mask, FP64 truth pressure) for the extra solvers' accuracy
+    truth_x = None
     if not args.no_accuracy:
         sysb = ps.export_system()
         lab = sysb["label"].bool()
-        xb = sysb["x"][lab]
+        if truth_dense is not None:
+            xt = torch.from_numpy(truth_dense).cuda().reshape(lab.shape)[lab]
+            truth_src = "cpu oracle FP64 Jacobi-PCG to 1e-12"
+        else:
+            truth = make({"Residual": 1e-12, "Precision": "fp64", "Preconditioner": "jacobi", "CheckInterval": 128})
+            step(truth)
+            xt = truth.export_system()["x"][lab]
+            truth.close()
+            truth_src = "gpu FP64 Jacobi-PCG to 1e-12"
+        truth_x = (lab, xt)
+
+        def perr(ctx):
+            x = ctx.export_system()["x"][lab]
+            return float(torch.linalg.norm(x - xt) / torch.linalg.norm(xt))
+
         div_before = ps.divergence(1.0, vel0, svel, fp64_update=False)
         div_after = ps.divergence(1.0, vel0, svel, fp64_update=True)
-        truth = make({"Residual": 1e-12, "Precision": "fp64", "Preconditioner": "jacobi", "CheckInterval": 128})
-        rt = step(truth)
-        xt = truth.export_system()["x"][lab]
+        p_rel = perr(ps)
         accuracy = {
-            "pressure_rel_l2_vs_fp64_1e-12": float(torch.linalg.norm(xb - xt) / torch.linalg.norm(xt)),
+            "truth": truth_src,
+            "pressure_rel_l2": p_rel,
             "post_div_rel_fp64": div_after / div_before,
             "true_relres_inf": last["true_relres_inf"],
             "true_relres_l2": last["true_relres_l2"],
-            "truth_iters": rt["count"],
+            "bars": {"pressure_rel_l2": PRESSURE_BAR, "post_div_rel_fp64": DIVERGENCE_BAR},
+            "meets_bar": bool(p_rel <= PRESSURE_BAR and div_after / div_before <= DIVERGENCE_BAR),
         }
-        truth_x = (lab, xt)
-        truth.close()
+
+        def bar_point(base_cfg, stops=BAR_STOPS, k=3):
+            """The loosest max-norm stop of `stops` at which the solver meets both bars,
+            with its solves/s (1 warm-up + k timed solves)."""
+            for tol in stops:
+                ctx = make(dict(base_cfg, Residual=tol))
+                r = step(ctx)
+                e = perr(ctx)
+                if e <= PRESSURE_BAR and r["true_relres_inf"] <= DIVERGENCE_BAR and r["status"] == 0:
+                    sps, r = timed(ctx, k)
+                    out = {"residual": tol, "solves_per_s": sps, "iters": r["count"], "outer": r["outer"],
+                           "pressure_rel_l2": perr(ctx), "post_div_rel_fp64": r["true_relres_inf"],
+                           "meets_bar": True}
+                    ctx.close()
+                    return out
+                ctx.close()
+            return {"meets_bar": False, "stops_tried": list(stops)}
+
+        accuracy["bar_point"] = bar_point(cfg)
         log(f"accuracy: {accuracy}")
 
-    extra = {}
-    plain_cg_iters = None
-    if args.extra or (not args.no_cpu_baseline and n > 96):
-        # plain FP64 CG (the reference's effective iteration) for the CPU extrapolation
+    # ---- the north star's comparison: plain FP64 GPU CG (the reference's iteration),
+    # multigrid PCG and the learned preconditioner in flexible CG, each at the bench
+    # tolerance and at the loosest stop meeting the accuracy bars
+    north = {}
+    plain_sps = None
+    if not args.no_north_star:
         pcg = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "none", "CheckInterval": 64})
-        rp = step(pcg)
-        plain_cg_iters = rp["count"]
-        if args.extra:
-            t1 = time.perf_counter()
-            rp = step(pcg)
-            extra["fp64_plain_cg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rp["count"]}
-            # its kernels against the HBM roofline (one more solve with dispatch-recorded events)
-            pcg.configure("ProfileKernels", 1)
-            step(pcg)
-            cells = rp["active_bricks"] * 512
-            for k in ("spmv_dot", "axpy_norm"):
-                cnt, ms = pcg.kernel_stats(k)
-                if cnt:
-                    bpl = kernel_bytes_per_cell(k, "fp64", "none") * cells
-                    gbs = bpl / (ms / cnt * 1e-3) / 1e9
-                    extra["fp64_plain_cg"][k] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
-                                                 "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-        pcg.close()
-        if args.extra:
-            pj = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "jacobi"})
-            step(pj)
-            t1 = time.perf_counter()
-            rj = step(pj)
-            extra["fp64_jacobi_pcg"] = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rj["count"]}
-            extra["mixed_jacobi_vs_fp64_plain_cg"] = round(value / extra["fp64_plain_cg"]["solves_per_s"], 2)
-            pj.close()
-        if args.extra:
-            # aggregation-multigrid PCG (mixed precision): at the bench tolerance, and at a
-            # 10x tighter max-norm stop (the setting for the pressure-accuracy bar)
-            for name, tol in (("mg_pcg", args.residual), ("mg_pcg_residual_x0.1", 0.1 * args.residual)):
-                pm = make({"Residual": tol, "Preconditioner": "mg"})
-                step(pm)
-                t1 = time.perf_counter()
-                rm = step(pm)
-                e = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": rm["count"],
-                     "outer": rm["outer"], "true_relres_inf": rm["true_relres_inf"]}
-                if "fp64_plain_cg" in extra:
-                    e["vs_fp64_plain_cg"] = round(e["solves_per_s"] / extra["fp64_plain_cg"]["solves_per_s"], 2)
-                if truth_x is not None:
-                    xm = pm.export_system()["x"][truth_x[0]]
-                    e["pressure_rel_l2_vs_fp64_1e-12"] = float(torch.linalg.norm(xm - truth_x[1])
-                                                               / torch.linalg.norm(truth_x[1]))
-                extra[name] = e
-                pm.close()
-        if args.extra:
-            # the 2D projection (macpressuresolver2) on a 2048^2 pool+disc scene: mixed
-            # Jacobi PCG and plain FP64 CG, with the 2D stencil kernel's roofline
-            from shiokaze_amd import scene as S2
-            sc2 = S2.generate2d(S2.POOL_SPHERE, 2048, 0)
-            d2 = {k: torch.from_numpy(np.ascontiguousarray(sc2[k])).cuda() for k in
-                  ("fluid", "solid", "u", "v", "us", "vs")}
-            v20 = [d2["u"].clone(), d2["v"].clone()]
-            e2 = {"grid": "2048x2048 pool+disc"}
-            for name, cfg2 in (("mixed_jacobi", {"Precision": "mixed", "Preconditioner": "jacobi"}),
-                               ("fp64_plain_cg", {"Precision": "fp64", "Preconditioner": "none",
-                                                  "CheckInterval": 64})):
-                p2 = shiokaze_amd.MacPressureSolver2((2048, 2048), sc2["dx"],
-                                                     config=dict(cfg2, Residual=args.residual))
-
-                def step2():
-                    d2["u"].copy_(v20[0])
-                    d2["v"].copy_(v20[1])
-                    return p2.project(1.0, (d2["u"], d2["v"]), d2["solid"], (d2["us"], d2["vs"]), d2["fluid"])
-                step2()
-                t1 = time.perf_counter()
-                r2 = step2()
-                ent = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": r2["count"],
-                       "unknowns": r2["unknowns"], "true_relres_inf": r2["true_relres_inf"]}
-                p2.configure("ProfileKernels", 1)
-                step2()
-                cnt, ms = p2.kernel_stats("spmv_dot")
-                if cnt:
-                    prec2 = cfg2["Precision"]
-                    bpl = kernel_bytes_per_cell("spmv_dot", prec2, cfg2["Preconditioner"]) * r2["active_bricks"] * 512
-                    gbs = bpl / (ms / cnt * 1e-3) / 1e9
-                    ent["spmv_dot"] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
-                                       "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
-                e2[name] = ent
-                p2.close()
-            e2["mixed_jacobi_vs_fp64_plain_cg"] = round(e2["mixed_jacobi"]["solves_per_s"]
-                                                        / e2["fp64_plain_cg"]["solves_per_s"], 2)
-            extra["projection_2d"] = e2
-        if args.extra and Path(args.unet_weights).exists():
-            # learned preconditioner: flexible CG with the trained network (plain PCG fallback)
-            from shiokaze_amd.learned import flops_per_apply
-            pu = make({"Residual": args.residual, "Preconditioner": "unet", "UnetWeights": args.unet_weights})
-            step(pu)
-            t1 = time.perf_counter()
-            ru = step(pu)
-            t_solve = time.perf_counter() - t1
-            pu.configure("ProfileKernels", 1)
-            step(pu)
-            cnt, ms = pu.kernel_stats("unet_apply")
-            entry = {"solves_per_s": 1.0 / t_solve, "iters": ru["count"], "outer": ru["outer"],
-                     "fallback": ru["fallback"], "true_relres_inf": ru["true_relres_inf"],
-                     "weights": Path(args.unet_weights).name}
+        step(pcg)
+        plain_sps, rp = timed(pcg, 2)
+        ent = {"solves_per_s": plain_sps, "iters": rp["count"], "true_relres_inf": rp["true_relres_inf"]}
+        pcg.configure("ProfileKernels", 1)
+        step(pcg)
+        cells = rp["active_bricks"] * 512
+        for k in ("spmv_dot", "axpy_norm"):
+            cnt, ms = pcg.kernel_stats(k)
             if cnt:
-                fl = flops_per_apply((n, n, n))
-                tf = fl / (ms / cnt * 1e-3) / 1e12
-                entry["net_apply_ms"] = round(ms / cnt, 4)
-                entry["mfma"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
-                                 "frac": round(tf / 2500.0, 4), "flops_per_apply": fl}
-            if "fp64_plain_cg" in extra:
-                entry["vs_fp64_plain_cg"] = round(entry["solves_per_s"] / extra["fp64_plain_cg"]["solves_per_s"], 2)
+                bpl = kernel_bytes_per_cell(k, "fp64", "none") * cells
+                gbs = bpl / (ms / cnt * 1e-3) / 1e9
+                ent[k] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
+                          "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if truth_x is not None:
+            ent["pressure_rel_l2"] = perr(pcg)
+        if cpu_cg_dense is not None:
+            # the same algorithm and stop on the CPU (the oracle's CG, as the reference runs it)
+            lab0 = truth_x[0] if truth_x is not None else pcg.export_system()["label"].bool()
+            xg = pcg.export_system()["x"][lab0]
+            xc = torch.from_numpy(cpu_cg_dense).cuda().reshape(lab0.shape)[lab0]
+            ent["pressure_rel_l2_vs_cpu_cg_same_stop"] = float(torch.linalg.norm(xg - xc) / torch.linalg.norm(xc))
+            ent["cpu_cg_iters"] = cpu["iters"]
+        pcg.close()
+        north["fp64_plain_cg"] = ent
+        north["headline_vs_fp64_plain_cg"] = round(value / plain_sps, 2)
+        solvers = [("mg_pcg", {"Precision": "mixed", "Preconditioner": "mg"})]
+        if Path(args.unet_weights).exists():
+            solvers.append(("learned_fcg", {"Precision": "mixed", "Preconditioner": "unet",
+                                            "UnetWeights": args.unet_weights}))
+        for name, scfg in solvers:
+            ctx = make(dict(scfg, Residual=args.residual))
+            step(ctx)
+            sps, r = timed(ctx, 3)
+            e = {"solves_per_s": sps, "iters": r["count"], "outer": r["outer"], "fallback": r["fallback"],
+                 "true_relres_inf": r["true_relres_inf"], "vs_fp64_plain_cg": round(sps / plain_sps, 2)}
             if truth_x is not None:
-                xu = pu.export_system()["x"][truth_x[0]]
-                entry["pressure_rel_l2_vs_fp64_1e-12"] = float(torch.linalg.norm(xu - truth_x[1])
-                                                               / torch.linalg.norm(truth_x[1]))
-            extra["learned_fcg"] = entry
-            pu.close()
-        log(f"plain FP64 CG iterations on the GPU: {plain_cg_iters}; extra={extra}")
+                e["pressure_rel_l2"] = perr(ctx)
+                e["meets_bar"] = bool(e["pressure_rel_l2"] <= PRESSURE_BAR and r["true_relres_inf"] <= DIVERGENCE_BAR)
+            if name == "learned_fcg":
+                from shiokaze_amd.learned import flops_per_apply
+                e["weights"] = Path(args.unet_weights).name
+                ctx.configure("ProfileKernels", 1)
+                step(ctx)
+                cnt, ms = ctx.kernel_stats("unet_apply")
+                if cnt:
+                    fl = flops_per_apply((n, n, n))
+                    tf = fl / (ms / cnt * 1e-3) / 1e12
+                    e["net_apply_ms"] = round(ms / cnt, 4)
+                    e["mfma"] = {"bound": "mfma", "achieved": round(tf, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                                 "frac": round(tf / 2500.0, 4), "flops_per_apply": fl}
+            ctx.close()
+            if truth_x is not None:
+                bp = bar_point(scfg)
+                if bp.get("meets_bar"):
+                    bp["vs_fp64_plain_cg"] = round(bp["solves_per_s"] / plain_sps, 2)
+                e["bar_point"] = bp
+            north[name] = e
+        if accuracy.get("bar_point", {}).get("meets_bar"):
+            accuracy["bar_point"]["vs_fp64_plain_cg"] = round(accuracy["bar_point"]["solves_per_s"] / plain_sps, 2)
+        log(f"north star: {north}")
+
+    extra = {}
+    if args.extra:
+        pj = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "jacobi"})
+        step(pj)
+        sps, rj = timed(pj, 2)
+        extra["fp64_jacobi_pcg"] = {"solves_per_s": sps, "iters": rj["count"]}
+        pj.close()
+        extra["projection_2d"] = bench_2d_extra(args, torch, shiokaze_amd)
     if args.liquid or args.extra:
         extra["liquid_step"] = liquid_leg(args, torch, shiokaze_amd, S, sc, dev, cfg)
         log(f"liquid step: {extra['liquid_step']}")
-
-    cpu = None
-    if not args.no_cpu_baseline and n <= 96:
-        # small grids (config 1): the whole reference-faithful solve on one host core,
-        # and the GPU pressure compared with the CPU one of the same run
-        from oracle import oracle as O
-        O.set_threads(1)
-        rr = O.project_timed(sc, tol=args.residual, mode=O.SHK_OR_REFERENCE)
-        total_ms = float(sum(rr["t_ms"][:3]))
-        p_gpu = ps.get_pressure().cpu().numpy().astype(np.float64)
-        p_cpu = np.asarray(rr["pressure"], dtype=np.float64)
-        p_rel = float(np.linalg.norm(p_gpu - p_cpu) / max(np.linalg.norm(p_cpu), 1e-300))
-        cpu = {"value": 1e3 / total_ms, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": (f"{n}^3 scene {args.scene}: the whole oracle projection, reference-faithful CG (MIC(0) "
-                          f"factor and two discarded triangular solves per iteration) to {args.residual:g}, "
-                          f"{rr['iters']} iterations, single thread"),
-               "ms_per_solve": round(total_ms, 1), "iters": rr["iters"],
-               "pressure_rel_l2_gpu_vs_cpu": p_rel}
-        log(f"cpu baseline: {cpu}")
-    elif not args.no_cpu_baseline:
-        from oracle import oracle as O
-        O.set_threads(1)
-        t_runs = {}
-        for k in (10, 30):
-            rr = O.project_timed(sc, tol=args.residual, max_iterations=k, mode=O.SHK_OR_REFERENCE)
-            t_runs[k] = rr["t_ms"]
-            log(f"cpu oracle {k} its: setup={rr['t_ms'][0]:.0f}ms solve={rr['t_ms'][1]:.0f}ms "
-                f"update={rr['t_ms'][2]:.0f}ms")
-        per_it = (t_runs[30][1] - t_runs[10][1]) / 20.0
-        factor = max(t_runs[10][1] - 10 * per_it, 0.0)
-        setup = min(t_runs[10][0], t_runs[30][0])
-        update = min(t_runs[10][2], t_runs[30][2])
-        total_ms = setup + factor + plain_cg_iters * per_it + update
-        cpu = {"value": 1e3 / total_ms, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": (f"{n}^3 scene {args.scene}: oracle reference-faithful CG (MIC(0) factor and two discarded "
-                          f"triangular solves per iteration) timed for 10 and 30 iterations, extrapolated to "
-                          f"{plain_cg_iters} iterations (the plain CG count on this scene); "
-                          f"{per_it:.1f} ms/iteration, setup {setup:.0f} ms, single thread"),
-               "ms_per_solve": round(total_ms, 1)}
-        log(f"cpu baseline: {cpu}")
 
     prec_name = {"mixed": "fp32 stencil/axpy + fp64 dots/refinement", "fp64": "fp64"}[args.precision]
     out = {
@@ -517,10 +590,47 @@ def main():
         "kernels": kernels,
         "accuracy": accuracy,
         "cpu_baseline": cpu,
+        "north_star": north,
         "extra": extra,
     }
     print(json.dumps(out), flush=True)
     ps.close()
+
+
+def bench_2d_extra(args, torch, shiokaze_amd):
+    """The 2D projection (macpressuresolver2) on a 2048^2 pool+disc scene: mixed Jacobi
+    PCG and plain FP64 CG, with the 2D stencil kernel's roofline."""
+    from shiokaze_amd import scene as S2
+    sc2 = S2.generate2d(S2.POOL_SPHERE, 2048, 0)
+    d2 = {k: torch.from_numpy(np.ascontiguousarray(sc2[k])).cuda() for k in ("fluid", "solid", "u", "v", "us", "vs")}
+    v20 = [d2["u"].clone(), d2["v"].clone()]
+    e2 = {"grid": "2048x2048 pool+disc"}
+    for name, cfg2 in (("mixed_jacobi", {"Precision": "mixed", "Preconditioner": "jacobi"}),
+                       ("fp64_plain_cg", {"Precision": "fp64", "Preconditioner": "none", "CheckInterval": 64})):
+        p2 = shiokaze_amd.MacPressureSolver2((2048, 2048), sc2["dx"], config=dict(cfg2, Residual=args.residual))
+
+        def step2():
+            d2["u"].copy_(v20[0])
+            d2["v"].copy_(v20[1])
+            return p2.project(1.0, (d2["u"], d2["v"]), d2["solid"], (d2["us"], d2["vs"]), d2["fluid"])
+        step2()
+        t1 = time.perf_counter()
+        r2 = step2()
+        ent = {"solves_per_s": 1.0 / (time.perf_counter() - t1), "iters": r2["count"],
+               "unknowns": r2["unknowns"], "true_relres_inf": r2["true_relres_inf"]}
+        p2.configure("ProfileKernels", 1)
+        step2()
+        cnt, ms = p2.kernel_stats("spmv_dot")
+        if cnt:
+            bpl = kernel_bytes_per_cell("spmv_dot", cfg2["Precision"], cfg2["Preconditioner"]) * r2["active_bricks"] * 512
+            gbs = bpl / (ms / cnt * 1e-3) / 1e9
+            ent["spmv_dot"] = {"avg_us": round(1e3 * ms / cnt, 3), "bytes_per_launch": bpl,
+                               "GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        e2[name] = ent
+        p2.close()
+    e2["mixed_jacobi_vs_fp64_plain_cg"] = round(e2["mixed_jacobi"]["solves_per_s"]
+                                                / e2["fp64_plain_cg"]["solves_per_s"], 2)
+    return e2
 
 
 def bench_2d(args, torch, shiokaze_amd, S):

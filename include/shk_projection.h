@@ -108,8 +108,8 @@ void shk_destroy(shk_ctx *ctx);
  *                  jacobi: diag(A) as symmetric scaling; unet: flexible CG with the
  *                  learned network below; mg: CG with an aggregation-multigrid V-cycle
  *                  (mixed precision, the counterpart of the reference's "amg" LinSolver,
- *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr applies jacobi for
- *                  any of jacobi, unet and mg.
+ *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr takes none and
+ *                  jacobi; unet and mg return SHK_BAD_ARG there.
  *   MgOmega, MgAlpha  multigrid damped-Jacobi weight (0.9) and coarse-correction
  *                  factor (1.5), each in (0, 2)
  *   OuterMax       refinement steps for mixed (3)

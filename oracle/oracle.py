@@ -37,6 +37,8 @@ _SIG = {
     "shk_or_assemble": (_i64, [_i, _i, _i, _d, _d, _vp] + [_vp] * 6 + [_vp] * 6 + [_vp, _i64] + [_vp] * 4),
     "shk_or_pcg": (_i, [_i64, _vp, _vp, _vp, _vp, _vp, _d, _i64, _i, _d, _d, ctypes.POINTER(_i64),
                         ctypes.POINTER(_d)]),
+    "shk_or_pcg_threaded": (_i, [_i64, _vp, _vp, _vp, _vp, _vp, _d, _i64, _i, ctypes.POINTER(_i64),
+                                 ctypes.POINTER(_d)]),
     "shk_or_spmv": (None, [_i64, _vp, _vp, _vp, _vp, _vp]),
     "shk_or_mic0": (_i64, [_i64, _vp, _vp, _vp, _d, _d, _vp, _vp, _vp, _vp]),
     "shk_or_scatter": (None, [_i, _i, _i, _vp, _vp, _i, _vp]),
@@ -159,6 +161,42 @@ def pcg(rowptr, cols, vals, b, tol=1e-4, max_iterations=30000, mode=SHK_OR_REFER
     status = load().shk_or_pcg(n, _p(rowptr), _p(cols), _p(vals), _p(b), _p(x), tol, max_iterations, mode, omega,
                                min_diag_ratio, ctypes.byref(iters), ctypes.byref(reresid))
     return x[:n], int(iters.value), float(reresid.value), int(status)
+
+
+def pcg_threaded(rowptr, cols, vals, b, tol=1e-4, max_iterations=30000, mode=SHK_OR_CG):
+    """shk_or_pcg_threaded: CG (mode SHK_OR_CG) or Jacobi-PCG (SHK_OR_JACOBI) on all
+    oracle threads (set_threads).  Returns (x, iters, reresid, status, seconds)."""
+    import time
+    n = len(b)
+    x = np.empty(max(n, 1))
+    iters = ctypes.c_int64(0)
+    reresid = ctypes.c_double(0.0)
+    b = _f64(b)
+    t0 = time.perf_counter()
+    status = load().shk_or_pcg_threaded(n, _p(rowptr), _p(cols), _p(vals), _p(b), _p(x), tol, max_iterations, mode,
+                                        ctypes.byref(iters), ctypes.byref(reresid))
+    sec = time.perf_counter() - t0
+    if status < 0:
+        raise ValueError("pcg_threaded takes SHK_OR_CG or SHK_OR_JACOBI and fewer than 2^31 rows")
+    return x[:n], int(iters.value), float(reresid.value), int(status), sec
+
+
+def system(scene: dict, dt=1.0):
+    """Fractions, labels and the assembled CSR system of a scene (the steps of
+    project() before the solve, macpressuresolver3.cpp:71-216) with their time."""
+    import time
+    t0 = time.perf_counter()
+    fluid = _f64(scene["fluid"])
+    solid = _f64(scene["solid"])
+    dx = float(scene["dx"])
+    vel = [_f64(scene[k]) for k in ("u", "v", "w")]
+    svel = [_f64(scene[k]) for k in ("us", "vs", "ws")]
+    areas = area_fraction(solid)
+    rhos = fluid_fraction(fluid)
+    index_map, n = label(fluid, areas, rhos)
+    rowptr, cols, vals, rhs = assemble(dt, dx, fluid, areas, rhos, vel, svel, index_map, n)
+    return {"index_map": index_map, "n": n, "rowptr": rowptr, "cols": cols, "vals": vals, "rhs": rhs,
+            "seconds": time.perf_counter() - t0}
 
 
 def spmv(rowptr, cols, vals, x):

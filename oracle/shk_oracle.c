@@ -736,6 +736,149 @@ done:
     return status;
 }
 
+/* ---------------------------------------------------------------- threaded CG */
+
+/* The same Krylov recurrences as shk_or_pcg in modes SHK_OR_CG and SHK_OR_JACOBI
+ * (pcg_solver.h:246-295 with z = r, or z = r / diag(A)), spread over the OpenMP
+ * threads: the CG-equivalent CPU baseline on T cores, and the FP64 truth solves of
+ * large grids.  Deterministic for a fixed thread count: every thread reduces its own
+ * static row range and the partials are summed in thread order.  Each iteration is
+ * three passes over the rows: q = A s with s.q; x += a s, r -= a q, z, z.r and max|r|;
+ * s = z + b s.  Column indices are narrowed to 32 bits once at the start (N < 2^31). */
+int shk_or_pcg_threaded(int64_t N, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                        const double *b, double *x, double tol, int64_t max_iterations, int mode,
+                        int64_t *iters, double *reresid) {
+    if (mode != SHK_OR_CG && mode != SHK_OR_JACOBI) return -1;
+    if (N >= ((int64_t)1 << 31)) return -1;
+    if (tol < 1e-30) tol = 1e-30;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    const int64_t nnz = N > 0 ? rowptr[N] : 0;
+    int32_t *c32 = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+    const size_t nb = sizeof(double) * (size_t)(N > 0 ? N : 1);
+    double *r = (double *)malloc(nb), *z = (double *)malloc(nb), *s = (double *)malloc(nb),
+           *q = (double *)malloc(nb), *inv = (double *)malloc(nb); /* diag(A) */
+    double *part = (double *)calloc((size_t)(3 * nt), sizeof(double));
+    int status = 1;
+    double res0 = 0.0, res = 0.0, rho = 0.0;
+#pragma omp parallel
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        const int64_t lo = N * t / nt, hi = N * (t + 1) / nt;
+        double m = 0.0, zr = 0.0;
+        for (int64_t i = lo; i < hi; ++i) {
+            double d = 0.0;
+            for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) {
+                c32[p] = (int32_t)cols[p];
+                if (cols[p] == i) d = vals[p];
+            }
+            inv[i] = d;
+            x[i] = 0.0;
+            r[i] = b[i];
+            z[i] = mode == SHK_OR_JACOBI ? b[i] / d : b[i];
+            s[i] = z[i];
+            zr += z[i] * r[i];
+            if (fabs(r[i]) > m) m = fabs(r[i]);
+        }
+        part[3 * t] = m;
+        part[3 * t + 1] = zr;
+    }
+    for (int t = 0; t < nt; ++t) {
+        if (part[3 * t] > res0) res0 = part[3 * t];
+        rho += part[3 * t + 1];
+    }
+    res = res0;
+    int64_t it = 0;
+    if (res0 == 0.0) {
+        status = 0;
+        goto done;
+    }
+    if (rho == 0.0 || rho != rho) {
+        status = rho != rho ? 2 : 3;
+        goto done;
+    }
+    {
+        const double tol_abs = tol * res0;
+        for (it = 0; it < max_iterations; ++it) {
+            double sq = 0.0;
+#pragma omp parallel
+            {
+                int t = 0;
+#ifdef _OPENMP
+                t = omp_get_thread_num();
+#endif
+                const int64_t lo = N * t / nt, hi = N * (t + 1) / nt;
+                double acc_sq = 0.0;
+                for (int64_t i = lo; i < hi; ++i) {
+                    double acc = 0.0;
+                    for (int64_t p = rowptr[i]; p < rowptr[i + 1]; ++p) acc += vals[p] * s[c32[p]];
+                    q[i] = acc;
+                    acc_sq += s[i] * acc;
+                }
+                part[3 * t] = acc_sq;
+            }
+            for (int t = 0; t < nt; ++t) sq += part[3 * t];
+            const double alpha = rho / sq;
+            double rho_new = 0.0;
+            res = 0.0;
+#pragma omp parallel
+            {
+                int t = 0;
+#ifdef _OPENMP
+                t = omp_get_thread_num();
+#endif
+                const int64_t lo = N * t / nt, hi = N * (t + 1) / nt;
+                double m = 0.0, zr = 0.0;
+                for (int64_t i = lo; i < hi; ++i) {
+                    x[i] += alpha * s[i];
+                    const double ri = r[i] - alpha * q[i];
+                    r[i] = ri;
+                    const double zi = mode == SHK_OR_JACOBI ? ri / inv[i] : ri;
+                    z[i] = zi;
+                    zr += zi * ri;
+                    if (fabs(ri) > m) m = fabs(ri);
+                }
+                part[3 * t] = m;
+                part[3 * t + 1] = zr;
+            }
+            for (int t = 0; t < nt; ++t) {
+                if (part[3 * t] > res) res = part[3 * t];
+                rho_new += part[3 * t + 1];
+            }
+            if (res != res) {
+                status = 2;
+                goto done;
+            }
+            if (res <= tol_abs) {
+                ++it;
+                status = 0;
+                goto done;
+            }
+            const double beta = rho_new / rho;
+            rho = rho_new;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < N; ++i) s[i] = z[i] + beta * s[i];
+        }
+        status = 1;
+    }
+done:
+    *iters = it;
+    *reresid = res0 > 0 ? res / res0 : res;
+    free(c32);
+    free(r);
+    free(z);
+    free(s);
+    free(q);
+    free(inv);
+    free(part);
+    return status;
+}
+
 /* ---------------------------------------------------------------- a14, a15 */
 
 void shk_or_scatter(int nx, int ny, int nz, const int64_t *index_map, const double *x,

@@ -79,6 +79,14 @@ int shk_or_pcg(int64_t N, const int64_t *rowptr, const int64_t *cols, const doub
 void shk_or_spmv(int64_t N, const int64_t *rowptr, const int64_t *cols, const double *vals,
                  const double *x, double *y);
 
+/* The CG and Jacobi-PCG modes of shk_or_pcg on all OpenMP threads (three fused
+ * passes per iteration, deterministic per thread count): the CG-equivalent CPU
+ * baseline on T cores and the FP64 truth solves of large grids.  Returns -1 for
+ * other modes or N >= 2^31. */
+int shk_or_pcg_threaded(int64_t N, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                        const double *b, double *x, double tol, int64_t max_iterations, int mode,
+                        int64_t *iters, double *reresid);
+
 /* MIC(0) factor in CSC lower form (pcg_solver.h:88-189 restated): colstart[N+1],
  * rowidx/value sized nnz; invdiag[N].  Exposed for property tests. */
 int64_t shk_or_mic0(int64_t N, const int64_t *rowptr, const int64_t *cols, const double *vals,

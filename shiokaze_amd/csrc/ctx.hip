@@ -913,6 +913,11 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     *res = shk_result{};
     res->unknowns = rows;
     res->outer = 1;
+    if (c->prm.precond == 2)
+        return c->fail(SHK_BAD_ARG, "Preconditioner=unet needs the projection's grid; shk_solve_csr takes none, "
+                                    "jacobi or mg");
+    if (c->prm.precond == 3)
+        return c->fail(SHK_BAD_ARG, "Preconditioner=mg is not available for CSR input");
     if (rows == 0) return SHK_OK;
     int rc = ensure_partials(c);
     if (rc) return rc;

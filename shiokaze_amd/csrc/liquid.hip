@@ -318,7 +318,9 @@ __global__ __launch_bounds__(kNT) void k_mask_normalise(Dims d, uint8_t *m0, uin
         if (v.x <= 1 && v.y <= 1 && v.z <= 1 && v.w <= 1) return;
         *reinterpret_cast<uchar4 *>(m + q4) = make_uchar4(v.x != 0, v.y != 0, v.z != 0, v.w != 0);
     } else {
-        for (int64_t q = q4; q < n; ++q)
+        // this thread's own four faces (fewer at the end of the array)
+        const int64_t end = q4 + 4 < n ? q4 + 4 : n;
+        for (int64_t q = q4; q < end; ++q)
             if (m[q] > 1) m[q] = 1;
     }
 }
@@ -688,7 +690,10 @@ __global__ __launch_bounds__(kNT) void k_gravity(Dims d, float g0, float g1, flo
         *reinterpret_cast<float4 *>(v + q4) = val;
         *reinterpret_cast<uchar4 *>(m + q4) = make_uchar4(1, 1, 1, 1);
     } else {
-        for (int64_t q = q4; q < n; ++q) {
+        // this thread's own four faces (fewer at the end of the array): every face
+        // is incremented exactly once
+        const int64_t end = q4 + 4 < n ? q4 + 4 : n;
+        for (int64_t q = q4; q < end; ++q) {
             v[q] = m[q] ? v[q] + add : 0.0f + add;
             m[q] = 1;
         }
@@ -718,7 +723,8 @@ __global__ __launch_bounds__(kNT) void k_project_activity(Dims d, const float *a
         act.w = act.w && av.w != 0.0f && rv.w != 0.0f;
         *reinterpret_cast<uchar4 *>(m + q4) = act;
     } else {
-        for (int64_t q = q4; q < n; ++q)
+        const int64_t end = q4 + 4 < n ? q4 + 4 : n;
+        for (int64_t q = q4; q < end; ++q)
             if (m[q] && !(a[q] != 0.0f && r[q] != 0.0f)) m[q] = 0;
     }
 }

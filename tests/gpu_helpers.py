@@ -113,3 +113,49 @@ def oracle_dense_system(O, ora: dict):
     if "x" in ora:
         x[cell_of_row] = ora["x"]
     return {k: v.reshape(shape) for k, v in dict(diag=diag, cx=cx, cy=cy, cz=cz, rhs=rhs, x=x).items()}
+
+
+# ---------------------------------------------------------------- the pressure bar
+
+# Pressure relative L2 against the oracle's FP64 solve to 1e-12 (FP64 Jacobi-PCG on
+# the host) that every solver path must meet.
+PRESSURE_BAR = 1e-5
+# The max-norm stop at which each solver path meets it.  A stop bounds the residual,
+# and the error is bounded by the conditioning times the residual, so each solver needs
+# a stop below the reference's 1e-6: Jacobi-type CG leaves a smooth residual (large
+# error per unit residual), multigrid and the learned preconditioner a rough one.
+BAR_STOP = {"none": 1e-8, "jacobi": 1e-8, "mg": 1e-7, "unet": 1e-7}
+BAR_STOP_2D = {"none": 1e-9, "jacobi": 1e-9, "mg": 1e-8}
+
+_TRUTH = {}
+
+
+def oracle_truth(O, sc, key=None):
+    """Dense (k, j, i) FP64 truth pressure of a scene and its unknown mask: the oracle's
+    system solved by its Jacobi-PCG to a 1e-12 max-norm residual (threaded)."""
+    if key is not None and key in _TRUTH:
+        return _TRUTH[key]
+    sy = O.system(sc)
+    x, it, rr, st, _ = O.pcg_threaded(sy["rowptr"], sy["cols"], sy["vals"], sy["rhs"], 1e-12, 200000,
+                                      O.SHK_OR_JACOBI)
+    assert st == 0, st
+    lab = sy["index_map"] >= 0
+    dense = np.zeros(sy["index_map"].shape)
+    dense[lab] = x[sy["index_map"][lab]]
+    out = (dense, lab)
+    if key is not None:
+        _TRUTH[key] = out
+    return out
+
+
+def pressure_rel_l2(x_dense, truth) -> float:
+    dense, lab = truth
+    xt = dense[lab]
+    return float(np.linalg.norm(np.asarray(x_dense).reshape(dense.shape)[lab] - xt) / np.linalg.norm(xt))
+
+
+def assert_pressure_bar(x_dense, truth, what: str) -> float:
+    rel = pressure_rel_l2(x_dense, truth)
+    log(f"[gpu] {what}: pressure rel-L2 vs oracle 1e-12 = {rel:.3e} (bar {PRESSURE_BAR:g})")
+    assert rel <= PRESSURE_BAR, (what, rel)
+    return rel

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_helpers import iteration_spread, log, within_spread
+from gpu_helpers import BAR_STOP_2D, PRESSURE_BAR, iteration_spread, log, within_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -136,18 +136,20 @@ def test_2d_parity(oracle, case):
 
 @pytest.mark.parametrize("precond,precision", [("jacobi", "fp64"), ("jacobi", "mixed"), ("none", "mixed")])
 def test_2d_solvers(oracle, precond, precision):
+    """Each 2D solver path reaches the reference's stop, and at the 2D bar stop meets
+    the pressure bar against the oracle's FP64 solve to 1e-12."""
     sc = _scene(3, (96, 80), 3)
-    g = run2d(sc, {"Residual": 1e-6, "Preconditioner": precond, "Precision": precision})[0]
     truth = oracle.project2d(sc, tol=1e-12, mode=oracle.SHK_OR_JACOBI)
     lab = truth["index_map2"] >= 0
-    x = g["system"]["x"][lab]
     xt = _dense_2d(truth)["x"][lab]
-    rel = np.linalg.norm(x - xt) / np.linalg.norm(xt)
-    r = g["result"]
-    log(f"[gpu] 2D {precond}/{precision}: iters {r['count']} outer {r['outer']} true {r['true_relres_inf']:.2e} "
-        f"x rel {rel:.2e}")
-    assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
-    assert rel < 1e-3
+    for stop in (1e-6, BAR_STOP_2D[precond]):
+        g = run2d(sc, {"Residual": stop, "Preconditioner": precond, "Precision": precision})[0]
+        rel = np.linalg.norm(g["system"]["x"][lab] - xt) / np.linalg.norm(xt)
+        r = g["result"]
+        log(f"[gpu] 2D {precond}/{precision} stop {stop:g}: iters {r['count']} outer {r['outer']} "
+            f"true {r['true_relres_inf']:.2e} x rel {rel:.2e}")
+        assert r["status"] == 0 and r["true_relres_inf"] <= stop * 1.0001
+    assert rel <= PRESSURE_BAR
 
 
 def test_2d_branches(oracle):
@@ -170,14 +172,17 @@ def test_2d_branches(oracle):
     vel = oracle.density_scale(d3, o["areas"], vel)
     np.testing.assert_array_equal(g["velocity"][0].astype(np.float64), vel[0].reshape(ny, nx + 1))
     np.testing.assert_array_equal(g["velocity"][1].astype(np.float64), vel[2].reshape(ny + 1, nx))
-    # warm start: the first call is the cold start, the second matches the truth
+    # warm start: the first call is the cold start, the second (which solves for the
+    # change from the first pressure) meets the pressure bar
     cold = run2d(sc, {"Residual": 1e-6, "Preconditioner": "jacobi"})[0]
     w1, w2 = run2d(sc, {"Residual": 1e-6, "Preconditioner": "jacobi", "WarmStart": 1}, repeat=2)
     np.testing.assert_array_equal(w1["system"]["x"], cold["system"]["x"])
     truth = oracle.project2d(sc, tol=1e-12, mode=oracle.SHK_OR_JACOBI)
     lab = truth["index_map2"] >= 0
     xt = _dense_2d(truth)["x"][lab]
-    assert np.linalg.norm(w2["system"]["x"][lab] - xt) / np.linalg.norm(xt) < 1e-3
+    rel = np.linalg.norm(w2["system"]["x"][lab] - xt) / np.linalg.norm(xt)
+    log(f"[gpu] 2D warm start: second solve pressure rel-L2 vs 1e-12 = {rel:.2e}")
+    assert rel <= PRESSURE_BAR
 
 
 def test_2d_laplace_law():
@@ -274,7 +279,8 @@ def test_2d_mg_vcycle_matches_oracle(case):
     log(f"[gpu] 2D mg v-cycle {shape}: levels {[lv[0].shape for lv in levels]}, rel L2 {rel:.2e}, "
         f"projection {res['count']} its")
     assert res["status"] == 0
-    assert rel < 1e-4
+    # FP32 V-cycle against the FP64 restatement
+    assert rel < 1e-5
     assert np.all(z[mask == 0] == 0)
 
 
@@ -293,4 +299,9 @@ def test_2d_projection_with_multigrid(oracle, case):
         f"relres {r['true_relres_inf']:.2e}, x rel {rel:.2e}")
     assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
     assert r["count"] * 4 <= rj["count"]
-    assert rel < 1e-3
+    # at the 2D bar stop the pressure meets the bar
+    gb = run2d(sc, {"Residual": BAR_STOP_2D["mg"], "Preconditioner": "mg"})[0]
+    relb = np.linalg.norm(gb["system"]["x"][lab] - xt) / np.linalg.norm(xt)
+    log(f"[gpu] 2D mg {shape} at {BAR_STOP_2D['mg']:g}: {gb['result']['count']} its, x rel {relb:.2e}")
+    assert gb["result"]["status"] == 0
+    assert relb <= PRESSURE_BAR

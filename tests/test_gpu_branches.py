@@ -284,5 +284,8 @@ def test_warm_start(scenes, oracle, precond, precision):
     truth = oracle.project(sc, tol=1e-12, mode=oracle.SHK_OR_JACOBI)
     xt = truth["x"][truth["index_map"][lab]]
     rel = np.linalg.norm(w2["system"]["x"][lab] - xt) / np.linalg.norm(xt)
-    assert rel < 1e-4
+    log(f"[gpu] warm start {precond}/{precision}: second solve pressure rel-L2 vs 1e-12 = {rel:.2e}")
+    # the second call solves for the change from the first pressure, to 1e-6 of the
+    # change's own rhs: its pressure meets the bar
+    assert rel <= 1e-5
     assert w2["result"]["status"] == 0

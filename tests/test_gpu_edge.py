@@ -7,7 +7,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_helpers import iteration_spread, log, run_projection, to_device, within_spread
+from gpu_helpers import BAR_STOP, PRESSURE_BAR, iteration_spread, log, run_projection, to_device, within_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -102,5 +102,36 @@ def test_csr_solve_parity(scenes, oracle, precond):
     assert res["status"] == 0
     assert within_spread(res["count"], spread)
     assert res["true_relres_inf"] <= 1e-6
+    # the oracle's iterate after the same number of iterations: equal up to rounding order
+    xk, k, _, _ = oracle.pcg(o["rowptr"], o["cols"], o["vals"], o["rhs"], 1e-30, res["count"], mode)
     xg = x.cpu().numpy()
-    assert np.linalg.norm(xg - o["x"]) / np.linalg.norm(o["x"]) < 1e-4
+    rel = np.linalg.norm(xg - xk) / np.linalg.norm(xk)
+    log(f"[gpu] csr {precond}: x vs the oracle's iterate {k}: {rel:.2e}")
+    assert rel <= 1e-7
+    # and at the bar stop, the pressure bar against the oracle's FP64 solve to 1e-12
+    solver = shiokaze_amd.PCGSolver(config={"Residual": BAR_STOP[precond], "Preconditioner": precond})
+    xb, rb = solver.solve(t(o["rowptr"], torch.int64), t(o["cols"], torch.int64), t(o["vals"], torch.float64),
+                          t(o["rhs"], torch.float64))
+    solver.close()
+    assert rb["status"] == 0
+    xt, _, _, st, _ = oracle.pcg_threaded(o["rowptr"], o["cols"], o["vals"], o["rhs"], 1e-12, 100000,
+                                         oracle.SHK_OR_JACOBI)
+    relb = np.linalg.norm(xb.cpu().numpy() - xt) / np.linalg.norm(xt)
+    log(f"[gpu] csr {precond} at {BAR_STOP[precond]:g}: {rb['count']} its, pressure rel-L2 vs 1e-12 {relb:.2e}")
+    assert relb <= PRESSURE_BAR
+
+
+@pytest.mark.parametrize("precond", ["unet", "mg"])
+def test_csr_solve_refuses_grid_preconditioners(scenes, oracle, precond):
+    """The learned and the brick multigrid preconditioners need the projection's grid;
+    shk_solve_csr refuses them with SHK_BAD_ARG instead of running another method."""
+    import torch
+
+    import shiokaze_amd
+    o = oracle.project(scenes(1, 16, 0), tol=1e-6, mode=oracle.SHK_OR_CG)
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dt).cuda()  # noqa: E731
+    solver = shiokaze_amd.PCGSolver(config={"Residual": 1e-6, "Preconditioner": precond})
+    with pytest.raises(shiokaze_amd.ShkError):
+        solver.solve(t(o["rowptr"], torch.int64), t(o["cols"], torch.int64), t(o["vals"], torch.float64),
+                     t(o["rhs"], torch.float64))
+    solver.close()

@@ -1,0 +1,116 @@
+"""The 512^3 workload (134M cells, irregular geometry from the seeded generator of
+random solid boxes and spheres and liquid spheres over a pool): mixed-precision
+Jacobi-PCG and the learned preconditioner in flexible CG against a tight FP64 GPU
+solve.  At this size the oracle's FP64 truth would take minutes on the host, so the
+reference pressure is the GPU's FP64 Jacobi-PCG to 1e-11, the path whose parity with
+the oracle the smaller tests establish; the size-independent properties (the FP64 true
+residual, the post-projection divergence identity) are checked directly."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from gpu_helpers import PRESSURE_BAR, log, to_device
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = Path(__file__).resolve().parents[1] / "shiokaze_amd" / "weights" / "unet_irregular_128_256.bin"
+N = 512
+# max-norm stops at which each solver meets the pressure bar at 512^3 (the error per
+# unit residual grows with the conditioning, ~n^2)
+STOP_JACOBI = 1e-9
+STOP_UNET = 1e-8
+
+
+@pytest.fixture(scope="module")
+def scene512():
+    from shiokaze_amd import scene as S
+    log(f"[gpu] generating the irregular {N}^3 scene (seed 1)")
+    return S.generate(S.IRREGULAR, N, 1)
+
+
+@pytest.fixture(scope="module")
+def dev512(scene512):
+    d = to_device(scene512)
+    vel0 = [d[k].clone() for k in ("u", "v", "w")]
+    return d, vel0
+
+
+def _project(dev512, scene512, config):
+    import torch
+
+    import shiokaze_amd
+    d, vel0 = dev512
+    for a, k in enumerate(("u", "v", "w")):
+        d[k].copy_(vel0[a])
+    with shiokaze_amd.MacPressureSolver3((N, N, N), scene512["dx"], config=config) as ps:
+        res = ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+        sysd = ps.export_system()
+        lab = sysd["label"].bool()
+        x = sysd["x"][lab].clone()
+        del sysd
+        svel = (d["us"], d["vs"], d["ws"])
+        div_before = ps.divergence(1.0, vel0, svel, fp64_update=False)
+        div_after = ps.divergence(1.0, vel0, svel, fp64_update=True)
+        torch.cuda.synchronize()
+    return res, lab, x, div_after / div_before
+
+
+@pytest.fixture(scope="module")
+def reference512(dev512, scene512):
+    res, lab, x, div = _project(dev512, scene512, {"Residual": 1e-11, "Precision": "fp64",
+                                                   "Preconditioner": "jacobi", "CheckInterval": 128})
+    log(f"[gpu] {N}^3 FP64 Jacobi-PCG to 1e-11: {res['unknowns']} unknowns, {res['count']} its, "
+        f"true relres {res['true_relres_inf']:.2e}")
+    assert res["status"] == 0
+    return lab, x
+
+
+def _rel(x, ref):
+    import torch
+    lab, xr = ref
+    return float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr))
+
+
+def test_512_mixed_jacobi(dev512, scene512, reference512):
+    res, lab, x, div = _project(dev512, scene512, {"Residual": 1e-6, "Precision": "mixed",
+                                                   "Preconditioner": "jacobi"})
+    log(f"[gpu] {N}^3 mixed Jacobi at 1e-6: {res['count']} its ({res['outer']} outer), true relres "
+        f"{res['true_relres_inf']:.2e}, post-divergence {div:.2e}, pressure vs FP64 1e-11 {_rel(x, reference512):.2e}, "
+        f"solve {res['ms_solve']:.1f} ms")
+    assert res["status"] == 0 and res["true_relres_inf"] <= 1e-6
+    assert div == pytest.approx(res["true_relres_inf"], rel=1e-6, abs=1e-12)
+    res, lab, x, div = _project(dev512, scene512, {"Residual": STOP_JACOBI, "Precision": "mixed",
+                                                   "Preconditioner": "jacobi"})
+    rel = _rel(x, reference512)
+    log(f"[gpu] {N}^3 mixed Jacobi at {STOP_JACOBI:g}: {res['count']} its, pressure vs FP64 1e-11 {rel:.2e}")
+    assert res["status"] == 0 and div <= 1e-6
+    assert rel <= PRESSURE_BAR
+
+
+@pytest.mark.skipif(not WEIGHTS.exists(), reason="no trained network weights")
+def test_512_learned_fcg(dev512, scene512, reference512):
+    cfg = {"Residual": 1e-6, "Preconditioner": "unet", "UnetWeights": str(WEIGHTS)}
+    res, lab, x, div = _project(dev512, scene512, cfg)
+    log(f"[gpu] {N}^3 learned FCG at 1e-6: {res['count']} its ({res['outer']} outer), fallback {res['fallback']}, "
+        f"true relres {res['true_relres_inf']:.2e}, pressure vs FP64 1e-11 {_rel(x, reference512):.2e}, "
+        f"solve {res['ms_solve']:.1f} ms")
+    assert res["status"] == 0 and res["true_relres_inf"] <= 1e-6
+    assert res["fallback"] == 0
+    res, lab, x, div = _project(dev512, scene512, dict(cfg, Residual=STOP_UNET))
+    rel = _rel(x, reference512)
+    log(f"[gpu] {N}^3 learned FCG at {STOP_UNET:g}: {res['count']} its, fallback {res['fallback']}, "
+        f"pressure vs FP64 1e-11 {rel:.2e}")
+    assert res["status"] == 0 and res["fallback"] == 0 and div <= 1e-6
+    assert rel <= PRESSURE_BAR
+
+
+def test_512_multigrid(dev512, scene512, reference512):
+    res, lab, x, div = _project(dev512, scene512, {"Residual": 1e-7, "Preconditioner": "mg"})
+    rel = _rel(x, reference512)
+    log(f"[gpu] {N}^3 multigrid at 1e-7: {res['count']} its ({res['outer']} outer), pressure vs FP64 1e-11 "
+        f"{rel:.2e}, solve {res['ms_solve']:.1f} ms")
+    assert res["status"] == 0 and div <= 1e-6
+    assert rel <= PRESSURE_BAR

@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from gpu_helpers import log, run_projection, to_device
+from gpu_helpers import BAR_STOP, assert_pressure_bar, log, oracle_truth, run_projection, to_device
 from oracle import mg_oracle as MG
 
 pytestmark = pytest.mark.gpu
@@ -41,7 +41,8 @@ def test_vcycle_matches_oracle(scenes, kind, n, seed):
     ref = MG.vcycle(levels, r.astype(np.float64))
     rel = np.linalg.norm(z - ref) / np.linalg.norm(ref)
     log(f"[gpu]   levels {[lv[0].shape for lv in levels]}: rel L2 error {rel:.2e}")
-    assert rel < 1e-4
+    # FP32 V-cycle against the FP64 restatement: a few float roundings per level
+    assert rel < 1e-5
     assert np.abs(z - ref).max() <= 1e-3 * np.abs(ref).max()
     assert np.all(z[s["mask"] == 0] == 0)
 
@@ -68,7 +69,7 @@ def test_vcycle_after_another_scene_on_the_same_context(scenes):
     ref = MG.vcycle(MG.hierarchy(MG.scaled_level0(s["mask"], s["hx"], s["hy"], s["hz"])), r.astype(np.float64))
     rel = np.linalg.norm(z - ref) / np.linalg.norm(ref)
     log(f"[gpu] mg v-cycle after another scene: rel L2 error {rel:.2e}")
-    assert rel < 1e-4
+    assert rel < 1e-5
     assert np.all(z[s["mask"] == 0] == 0)
 
 
@@ -91,21 +92,25 @@ def test_vcycle_is_symmetric(scenes):
 
 
 @pytest.mark.parametrize("kind,n,seed", [(1, 32, 0), (3, (24, 40, 16), 3), (3, 64, 5), (1, (13, 21, 9), 0)])
-def test_projection_with_multigrid(scenes, kind, n, seed):
+def test_projection_with_multigrid(scenes, oracle, kind, n, seed):
+    """Multigrid PCG reaches the reference's stop in at most a quarter of Jacobi-PCG's
+    iterations, and at its bar stop meets the pressure bar against the oracle."""
     log(f"[gpu] mg projection kind={kind} n={n} seed={seed}")
     sc = scenes(kind, n, seed)
     g = run_projection(sc, {"Residual": 1e-6, "Preconditioner": "mg"}, want_system=False)
     jac = run_projection(sc, {"Residual": 1e-6, "Precision": "mixed", "Preconditioner": "jacobi"}, want_system=False)
-    ref = run_projection(sc, {"Residual": 1e-10, "Preconditioner": "jacobi"}, want_system=False)
     r, rj = g["result"], jac["result"]
     log(f"[gpu]   mg {r['count']} it (outer {r['outer']}), jacobi {rj['count']} it, relres {r['true_relres_inf']:.2e}")
     assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
     assert r["count"] * 4 <= rj["count"]
-    pr = ref["pressure"]
-    assert np.abs(g["pressure"] - pr).max() <= 1e-3 * np.abs(pr).max()
     # the velocity update is the same code path as every other preconditioner
     for a in range(3):
         assert np.isfinite(g["velocity"][a]).all()
+    gb = run_projection(sc, {"Residual": BAR_STOP["mg"], "Preconditioner": "mg"})
+    rb = gb["result"]
+    assert rb["status"] == 0 and gb["div_after_fp64"] / gb["div_before"] <= 1e-6
+    truth = oracle_truth(oracle, sc, ("3d", str(kind), str(n), str(seed)))
+    assert_pressure_bar(gb["system"]["x"], truth, f"mg kind={kind} n={n} at {BAR_STOP['mg']:g} ({rb['count']} its)")
 
 
 def test_multigrid_reports_nan(scenes):

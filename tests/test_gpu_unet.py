@@ -7,7 +7,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from gpu_helpers import log, run_projection
+from gpu_helpers import BAR_STOP, assert_pressure_bar, log, oracle_truth, run_projection
 from oracle import unet_oracle as U
 from unet_cases import init_params, training_sample
 
@@ -98,6 +98,13 @@ def _sample_dev(shape, seed):
 
 
 def test_network_loss_and_gradients():
+    """The GPU network's loss and parameter gradients against the FP64 oracle with BF16
+    rounding emulated where the GPU stores BF16 (activations and their gradients).
+    Error model: each stored BF16 value carries a relative rounding of at most 2^-9; the
+    FP32 MFMA accumulation over up to 27*128 terms adds ~1e-6 relative; so a layer's
+    gradient, an FP32 sum over the voxels of BF16 products whose rounding the oracle
+    reproduces only at the stored points, differs by O(1e-3) relative L2 (observed
+    <= 1.2e-3).  The per-layer bound is 5e-3."""
     from shiokaze_amd import UNet
     log("[gpu] network loss/gradients 32^3")
     (r, mask, cx, cy, cz, xs), dev = _sample_dev((32, 32, 32), 5)
@@ -119,7 +126,7 @@ def test_network_loss_and_gradients():
         seg = slice(wo, bo + cout)
         rel = np.linalg.norm(g[seg] - rg[seg]) / np.linalg.norm(rg[seg])
         log(f"[gpu]   layer {l}: rel L2 grad error {rel:.3e}")
-        assert rel < 5e-2
+        assert rel < 5e-3
     assert cos > 0.998
 
 
@@ -136,7 +143,9 @@ def test_apply_matches_oracle_and_is_homogeneous():
                               bf16=True)
     rel = np.linalg.norm(z - rz) / np.linalg.norm(rz)
     log(f"[gpu]   rel L2 error {rel:.3e}")
-    assert rel < 1e-2
+    # BF16 storage emulated in the oracle; the remaining FP32 accumulation-order error
+    # is ~1e-5 (observed <= 1e-4)
+    assert rel < 1e-3
     assert np.all(z[mask == 0] == 0)
     # the RMS normalisation makes M positively homogeneous (exactly, up to FP32 scaling)
     np.testing.assert_allclose(z2, 4.0 * z, rtol=1e-5, atol=1e-6 * np.abs(z).max())
@@ -188,9 +197,10 @@ def _scene_config(weights):
 
 
 @pytest.mark.parametrize("kind,n,seed", [(1, 32, 0), (3, (24, 40, 16), 3), (3, 48, 5)])
-def test_projection_with_learned_preconditioner(tmp_path, scenes, kind, n, seed):
+def test_projection_with_learned_preconditioner(tmp_path, scenes, oracle, kind, n, seed):
     """Flexible CG with the (untrained) network converges to the residual asked for,
-    falling back to Jacobi-PCG if it stalls; the pressure matches the plain solver's."""
+    falling back to Jacobi-PCG if it stalls; at the bar stop its pressure meets the
+    pressure bar against the oracle's FP64 solve to 1e-12."""
     from shiokaze_amd import UNet
     log(f"[gpu] unet projection kind={kind} n={n} seed={seed}")
     path = tmp_path / "net.bin"
@@ -198,15 +208,16 @@ def test_projection_with_learned_preconditioner(tmp_path, scenes, kind, n, seed)
         net.save(path)
     sc = scenes(kind, n, seed)
     g = run_projection(sc, _scene_config(path), want_system=False)
-    ref = run_projection(sc, {"Residual": 1e-10, "Preconditioner": "jacobi"}, want_system=False)
     r = g["result"]
     log(f"[gpu]   status={r['status_name']} count={r['count']} outer={r['outer']} fallback={r['fallback']} "
         f"relres={r['true_relres_inf']:.2e}")
     assert r["status"] == 0
     assert r["true_relres_inf"] <= 1e-6
-    pr = ref["pressure"]
-    scale = np.abs(pr).max()
-    assert np.abs(g["pressure"] - pr).max() <= 1e-2 * scale
+    gb = run_projection(sc, dict(_scene_config(path), Residual=BAR_STOP["unet"]))
+    assert gb["result"]["status"] == 0
+    assert gb["div_after_fp64"] / gb["div_before"] <= 1e-6
+    truth = oracle_truth(oracle, sc, ("3d", str(kind), str(n), str(seed)))
+    assert_pressure_bar(gb["system"]["x"], truth, f"untrained unet kind={kind} n={n} at {BAR_STOP['unet']:g}")
 
 
 def test_unet_config_errors(scenes, tmp_path):
@@ -253,7 +264,7 @@ WEIGHTS = __import__("pathlib").Path(__file__).resolve().parents[1] / "shiokaze_
     "unet_irregular_128_256.bin"
 
 
-def test_trained_network_gain(scenes, tmp_path):
+def test_trained_network_gain(scenes, oracle, tmp_path):
     """What training buys: on held-out irregular scenes at 128^3 (seeds after the
     training range), flexible CG with the trained network needs a fraction of the
     iterations of Jacobi-PCG and of the same network untrained, never falls back,
@@ -275,9 +286,13 @@ def test_trained_network_gain(scenes, tmp_path):
         for r in (p, t, u):
             assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
         assert t["fallback"] == 0
-        scale = np.abs(plain["pressure"]).max()
-        assert np.abs(tr["pressure"] - plain["pressure"]).max() <= 1e-3 * scale
         rows.append((p["count"], t["count"], u["count"]))
+        if seed == 2000:
+            # the trained network at the bar stop against the oracle's FP64 truth
+            gb = run_projection(sc, dict(_scene_config(WEIGHTS), Residual=BAR_STOP["unet"]))
+            assert gb["result"]["status"] == 0 and gb["result"]["fallback"] == 0
+            truth = oracle_truth(oracle, sc, ("3d", "3", "128", str(seed)))
+            assert_pressure_bar(gb["system"]["x"], truth, f"trained unet 128^3 seed {seed} at {BAR_STOP['unet']:g}")
     jac, trained, raw = (np.mean([r[i] for r in rows]) for i in range(3))
     log(f"[gpu] gain: mean iterations jacobi {jac:.1f}, trained {trained:.1f}, untrained {raw:.1f}")
     assert trained <= 0.25 * jac
