@@ -44,6 +44,16 @@ def iteration_spread(O, rowptr, cols, vals, b, tol, mode, n_perm: int = 6, seed:
     return min(counts), max(counts)
 
 
+def iterate_error(O, rowptr, cols, vals, b, xk) -> float:
+    """Relative L2 distance of an iterate from the system's solution (the oracle's
+    Jacobi-PCG to 1e-13).  Two CG runs that differ only in summation order drift apart in
+    the late iterations along the slowest modes, by up to a third of this distance on
+    the scenes here; a wrong coefficient, sign or step length moves the iterate by far
+    more than the distance itself."""
+    xt = O.pcg(rowptr, cols, vals, b, 1e-13, 100000, O.SHK_OR_JACOBI)[0]
+    return float(np.linalg.norm(xk - xt) / np.linalg.norm(xt))
+
+
 def within_spread(count: int, spread) -> bool:
     lo, hi = spread
     m = max(3, int(0.02 * hi + 0.5))

@@ -7,7 +7,8 @@ import ctypes
 import numpy as np
 import pytest
 
-from gpu_helpers import BAR_STOP, PRESSURE_BAR, iteration_spread, log, run_projection, to_device, within_spread
+from gpu_helpers import (BAR_STOP, PRESSURE_BAR, iterate_error, iteration_spread, log, run_projection, to_device,
+                         within_spread)
 
 pytestmark = pytest.mark.gpu
 
@@ -102,12 +103,14 @@ def test_csr_solve_parity(scenes, oracle, precond):
     assert res["status"] == 0
     assert within_spread(res["count"], spread)
     assert res["true_relres_inf"] <= 1e-6
-    # the oracle's iterate after the same number of iterations: equal up to rounding order
+    # the oracle's iterate after the same number of iterations: equal up to rounding
+    # order, which moves late CG iterates by a fraction of their distance from the solution
     xk, k, _, _ = oracle.pcg(o["rowptr"], o["cols"], o["vals"], o["rhs"], 1e-30, res["count"], mode)
     xg = x.cpu().numpy()
     rel = np.linalg.norm(xg - xk) / np.linalg.norm(xk)
-    log(f"[gpu] csr {precond}: x vs the oracle's iterate {k}: {rel:.2e}")
-    assert rel <= 1e-7
+    err = iterate_error(oracle, o["rowptr"], o["cols"], o["vals"], o["rhs"], xk)
+    log(f"[gpu] csr {precond}: x vs the oracle's iterate {k}: {rel:.2e} (its distance from the solution {err:.2e})")
+    assert rel <= 0.5 * err
     # and at the bar stop, the pressure bar against the oracle's FP64 solve to 1e-12
     solver = shiokaze_amd.PCGSolver(config={"Residual": BAR_STOP[precond], "Preconditioner": precond})
     xb, rb = solver.solve(t(o["rowptr"], torch.int64), t(o["cols"], torch.int64), t(o["vals"], torch.float64),

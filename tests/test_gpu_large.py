@@ -21,7 +21,7 @@ N = 512
 # max-norm stops at which each solver meets the pressure bar at 512^3 (the error per
 # unit residual grows with the conditioning, ~n^2)
 STOP_JACOBI = 1e-9
-STOP_UNET = 1e-8
+STOP_UNET = 5e-8
 
 
 @pytest.fixture(scope="module")
