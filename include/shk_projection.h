@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define SHK_ABI_VERSION 2
+#define SHK_ABI_VERSION 3
 
 typedef struct shk_ctx shk_ctx;
 
@@ -81,6 +81,18 @@ typedef struct shk_result {
     double ms_update;       /* pressure scatter + velocity update */
     double volume_correct_rhs; /* the constant added to every rhs row by the volume
                                   correction (macpressuresolver3.cpp:231-235), 0 if off */
+    /* RCMatrix_utility::report of the assembled matrix (RCMatrix_utility.h:98-146,
+     * called at macpressuresolver3.cpp:236), filled by shk_project / shk_project2d
+     * when Report=1 (report_valid = 1), else zero. */
+    int64_t report_nnz;              /* nonzero entries */
+    int32_t report_min_row;          /* fewest nonzeros in a row */
+    int32_t report_max_row;          /* most nonzeros in a row */
+    double report_min_diag;          /* smallest diagonal entry */
+    double report_max_diag;          /* largest diagonal entry */
+    double report_max_offdiag_ratio; /* worst max|off-diagonal| / diagonal of a row */
+    double report_symmetry_error;    /* max |A_ij - A_ji| (0: one coupling per face) */
+    int32_t report_has_nan;          /* 1 if a matrix entry is NaN */
+    int32_t report_valid;
 } shk_result;
 
 /* ------------------------------------------------------------------ context */
@@ -117,6 +129,8 @@ void shk_destroy(shk_ctx *ctx);
  *   CheckInterval  iterations launched between host convergence polls (32)
  *   HostPointers   0 | 1                   (0)
  *   UnetWeights    path of a trained preconditioner blob (for Preconditioner=unet)
+ *   Report         0 | 1: RCMatrix_utility::report statistics of every assembled
+ *                  system into shk_result.report_* (RCMatrix_utility.h:98-146) (0)
  *   ProfileKernels 0 | 1: time every Krylov kernel launch with HIP events on the
  *                  context's stream (read with shk_kernel_stats; resets the counters)
  * Unknown keys and unparsable values return SHK_BAD_ARG. */

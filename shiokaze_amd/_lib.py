@@ -47,6 +47,15 @@ class ShkResult(ctypes.Structure):
         ("ms_solve", ctypes.c_double),
         ("ms_update", ctypes.c_double),
         ("volume_correct_rhs", ctypes.c_double),
+        ("report_nnz", ctypes.c_int64),
+        ("report_min_row", ctypes.c_int32),
+        ("report_max_row", ctypes.c_int32),
+        ("report_min_diag", ctypes.c_double),
+        ("report_max_diag", ctypes.c_double),
+        ("report_max_offdiag_ratio", ctypes.c_double),
+        ("report_symmetry_error", ctypes.c_double),
+        ("report_has_nan", ctypes.c_int32),
+        ("report_valid", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:

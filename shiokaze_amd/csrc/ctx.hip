@@ -440,6 +440,8 @@ void shk_destroy(shk_ctx *c) {
     for (DevBuf<float> *b : {&c->hx32, &c->hy32, &c->hz32, &c->kappa32, &c->kdiag32, &c->diag32, &c->cx32, &c->cy32, &c->cz32, &c->x32, &c->r32,
                              &c->s32, &c->t32, &c->q32, &c->rs32})
         b->release();
+    for (auto &b : c->csr_i) b.release();
+    for (auto &b : c->csr_d) b.release();
     c->state.release();
     if (c->h_state) (void)hipHostFree(c->h_state);
     if (c->h_misc) (void)hipHostFree(c->h_misc);
@@ -488,6 +490,9 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
     } else if (k == "WarmStart") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WarmStart '%s'", value);
         c->prm.warm_start = b;
+    } else if (k == "Report") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad Report '%s'", value);
+        p.report = b;
     } else if (k == "MacCormack") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad MacCormack '%s'", value);
         p.maccormack = b;
@@ -717,6 +722,38 @@ int project_impl(shk_ctx *c, double dt, float *const user_vel[3], const float *c
         launch_brick_uniform<float>(c->stream, d.bx, c->n_active, c->nbr.p, c->cx32.p, c->cy32.p, c->cz32.p,
                                     c->kappa32.p, c->diag32.p, c->kdiag32.p);
     }
+    if (c->prm.report) {
+        // RCMatrix_utility::report(Lhs) (macpressuresolver3.cpp:236, RCMatrix_utility.h:98-146)
+        const int blocks = std::min(c->n_active, 512);
+        launch_report(c->stream, d.bx, c->n_active, blocks, c->nbr.p, c->diag64.p, c->cx64.p, c->cy64.p, c->cz64.p,
+                      c->pc.p);
+        HIPC(c, hipGetLastError());
+        std::vector<double> part(8 * (size_t)blocks);
+        HIPC(c, hipMemcpyAsync(part.data(), c->pc.p, sizeof(double) * part.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+        double nnz = 0.0, rmin = 1e300, rmax = 0.0, dmin = 1e300, dmax = -1e300, ratio = 0.0, nan = 0.0;
+        for (int b = 0; b < blocks; ++b) {
+            const double *q = &part[8 * (size_t)b];
+            if (q[7] == 0.0) continue;  // a block without rows
+            nnz += q[0];
+            rmin = std::min(rmin, q[1]);
+            rmax = std::max(rmax, q[2]);
+            dmin = std::min(dmin, q[3]);
+            dmax = std::max(dmax, q[4]);
+            ratio = std::max(ratio, q[5]);
+            nan = std::max(nan, q[6]);
+        }
+        res->report_valid = 1;
+        res->report_nnz = (int64_t)nnz;
+        res->report_min_row = (int32_t)rmin;
+        res->report_max_row = (int32_t)rmax;
+        res->report_min_diag = dmin <= dmax ? dmin : 0.0;
+        res->report_max_diag = dmin <= dmax ? dmax : 0.0;
+        res->report_max_offdiag_ratio = ratio;
+        // one coupling is stored per face and read by both of its rows
+        res->report_symmetry_error = 0.0;
+        res->report_has_nan = nan != 0.0 ? 1 : 0;
+    }
     if (c->prm.warm_start) {
         // macpressuresolver3.cpp:238-247: solve for the change from the previous
         // pressure, rhs' = rhs - A p_prev.  The previous pressure is kept per cell
@@ -900,14 +937,13 @@ int shk_divergence(shk_ctx *c, double dt, const float *u, const float *v, const 
     return SHK_OK;
 }
 
-int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t *cols, const double *vals,
-                  const double *b, double *x, shk_result *res) {
-    if (!c) return SHK_BAD_ARG;
-    c->err.clear();
-    if (rows < 0 || rows > ((int64_t)1 << 40)) return c->fail(SHK_BAD_ARG, "bad row count");
-    if (!rowptr || !cols || !vals || !b || !x) return c->fail(SHK_BAD_ARG, "NULL CSR/vector pointer");
-    if (c->prm.host_pointers) return c->fail(SHK_BAD_ARG, "shk_solve_csr takes device pointers");
-    HIPC(c, hipSetDevice(c->device));
+}  // extern "C"
+
+namespace {
+
+// shk_solve_csr on device pointers
+int solve_csr_device(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                     const double *b, double *x, shk_result *res) {
     shk_result local{};
     if (!res) res = &local;
     *res = shk_result{};
@@ -972,6 +1008,39 @@ int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t
     if ((rc = sync_and_time(c, 0, 1, res->ms_solve))) return rc;
     c->have_system = false;
     return res->status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int shk_solve_csr(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t *cols, const double *vals,
+                  const double *b, double *x, shk_result *res) {
+    if (!c) return SHK_BAD_ARG;
+    c->err.clear();
+    if (rows < 0 || rows > ((int64_t)1 << 40)) return c->fail(SHK_BAD_ARG, "bad row count");
+    if (!rowptr || !cols || !vals || !b || !x) return c->fail(SHK_BAD_ARG, "NULL CSR/vector pointer");
+    HIPC(c, hipSetDevice(c->device));
+    if (!c->prm.host_pointers) return solve_csr_device(c, rows, rowptr, cols, vals, b, x, res);
+    // host arrays: stage the matrix and rhs in HBM, solve, copy x back (the copies are
+    // part of the call, as for shk_project with HostPointers=1)
+    const int64_t nnz = rowptr[rows];
+    if (nnz < 0 || rowptr[0] != 0) return c->fail(SHK_BAD_ARG, "rowptr must start at 0 and end at nnz >= 0");
+    HIPC(c, c->csr_i[0].ensure((size_t)rows + 1));
+    HIPC(c, c->csr_i[1].ensure((size_t)std::max<int64_t>(nnz, 1)));
+    HIPC(c, c->csr_d[0].ensure((size_t)std::max<int64_t>(nnz, 1)));
+    HIPC(c, c->csr_d[1].ensure((size_t)std::max<int64_t>(rows, 1)));
+    HIPC(c, c->csr_d[2].ensure((size_t)std::max<int64_t>(rows, 1)));
+    HIPC(c, hipMemcpyAsync(c->csr_i[0].p, rowptr, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->csr_i[1].p, cols, sizeof(int64_t) * nnz, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->csr_d[0].p, vals, sizeof(double) * nnz, hipMemcpyHostToDevice, c->stream));
+    HIPC(c, hipMemcpyAsync(c->csr_d[1].p, b, sizeof(double) * rows, hipMemcpyHostToDevice, c->stream));
+    const int rc = solve_csr_device(c, rows, c->csr_i[0].p, c->csr_i[1].p, c->csr_d[0].p, c->csr_d[1].p,
+                                    c->csr_d[2].p, res);
+    if (rc != SHK_OK && rc != SHK_NOT_CONVERGED && rc != SHK_NAN) return rc;
+    HIPC(c, hipMemcpyAsync(x, c->csr_d[2].p, sizeof(double) * rows, hipMemcpyDeviceToHost, c->stream));
+    HIPC(c, hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 }  // extern "C"

@@ -66,6 +66,7 @@ struct Params {
     double mg_omega = 0.9;            // multigrid damped-Jacobi weight
     double mg_alpha = 1.5;            // multigrid coarse-correction factor
     bool warm_start = false;          // macpressuresolver3.cpp:354
+    bool report = false;              // RCMatrix_utility::report of each assembled system
     bool maccormack = true;           // macadvection3.cpp:281
     int trim_narrowband = 1;          // macadvection3.cpp:283
 };
@@ -128,6 +129,9 @@ struct shk_ctx {
     DevBuf<double> x64, r64, s64, t64, q64;
     DevBuf<float> x32, r32, s32, t32, q32, rs32;
     DevBuf<double> pa, pb, pc, misc;
+    // shk_solve_csr with HostPointers=1: device copies of rowptr, cols and of vals, b, x
+    DevBuf<int64_t> csr_i[2];
+    DevBuf<double> csr_d[3];
     shk::Grid grid;
     DevBuf<SolveState> state;
     SolveState *h_state = nullptr;

@@ -35,6 +35,10 @@ void launch_coeff(hipStream_t st, Dims d, double dt, double dx, int n_active, co
                   double *rhs, double *sig, float *diag32, float *cx32, float *cy32, float *cz32);
 
 // Couplings of D^-1/2 A D^-1/2 (FP64 and/or FP32 outputs; any may be nullptr).
+// RCMatrix_utility::report statistics of the brick system into part[8 * blocks]
+// (blocks <= n_active or any positive count; each block folds its strided bricks)
+void launch_report(hipStream_t st, int bx, int n_active, int blocks, const int *nbr, const double *diag,
+                   const double *cx, const double *cy, const double *cz, double *part);
 void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, const double *sig, const double *cx,
                         const double *cy, const double *cz, double *hx, double *hy, double *hz, float *hx32,
                         float *hy32, float *hz32);

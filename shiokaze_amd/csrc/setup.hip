@@ -217,6 +217,83 @@ __global__ void k_scale_coeff(const int *__restrict__ nbr, const double *__restr
     }
 }
 
+// RCMatrix_utility::report (RCMatrix_utility.h:98-146) of the assembled brick system:
+// per block (grid-strided over bricks, one thread per brick cell) the row statistics
+// of the matrix diag - sum c neighbour, whose rows are the cells with a nonzero
+// diagonal.  A row's entries are its diagonal and its nonzero couplings: the three
+// stored at the cell (low faces) and the three stored at its +x/+y/+z neighbours
+// (high faces).  part[8*block + {0..7}] = {nonzeros, min row, max row, min diag,
+// max diag, max |coupling|/diag, NaN present, rows}.
+template <int BX>
+__global__ __launch_bounds__(kBrickCells) void k_report(int n_active, const int *__restrict__ nbr,
+                                                        const double *__restrict__ diag,
+                                                        const double *__restrict__ cx,
+                                                        const double *__restrict__ cy,
+                                                        const double *__restrict__ cz, double *__restrict__ part) {
+    using B = BrickShape<BX>;
+    __shared__ double red[8][kBrickCells / kWave];
+    const int l = threadIdx.x;
+    const int loc[3] = {l % B::X, (l / B::X) % B::Y, l / kWave};
+    const int stride[3] = {1, B::X, kWave};
+    const int ext[3] = {B::X, B::Y, B::Z};
+    const double *c[3] = {cx, cy, cz};
+    double nnz = 0.0, rmin = 1e300, rmax = 0.0, dmin = 1e300, dmax = -1e300, ratio = 0.0, nan = 0.0, rows = 0.0;
+    for (int slot = blockIdx.x; slot < n_active; slot += gridDim.x) {
+        const int64_t e = (int64_t)slot * kBrickCells + l;
+        const double d = diag[e];
+        if (d == 0.0) continue;  // not an unknown (NaN is not == 0 and counts as a row)
+        int cnt = 1;
+        double off = 0.0;
+        bool bad = d != d;
+        for (int a = 0; a < 3; ++a) {
+            const double lo = c[a][e];
+            double hi = 0.0;
+            if (loc[a] < ext[a] - 1) {
+                hi = c[a][e + stride[a]];
+            } else {
+                const int nb = nbr[6 * slot + 2 * a + 1];
+                if (nb >= 0) hi = c[a][(int64_t)nb * kBrickCells + l - (ext[a] - 1) * stride[a]];
+            }
+            const double lohi[2] = {lo, hi};
+            for (int t = 0; t < 2; ++t) {
+                if (lohi[t] != 0.0) {
+                    ++cnt;
+                    off = fmax(off, fabs(lohi[t]));
+                }
+                bad = bad || lohi[t] != lohi[t];
+            }
+        }
+        rows += 1.0;
+        nnz += cnt;
+        rmin = fmin(rmin, (double)cnt);
+        rmax = fmax(rmax, (double)cnt);
+        if (d == d) {
+            dmin = fmin(dmin, d);
+            dmax = fmax(dmax, d);
+            if (d != 0.0) ratio = fmax(ratio, off / d);
+        }
+        if (bad) nan = 1.0;
+    }
+    double v[8] = {nnz, rmin, rmax, dmin, dmax, ratio, nan, rows};
+    for (int k = 0; k < 8; ++k) {
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const double w = __shfl_xor(v[k], o);
+            v[k] = (k == 0 || k == 7) ? v[k] + w : ((k == 1 || k == 3) ? fmin(v[k], w) : fmax(v[k], w));
+        }
+    }
+    if ((l & (kWave - 1)) == 0)
+        for (int k = 0; k < 8; ++k) red[k][l / kWave] = v[k];
+    __syncthreads();
+    if (l < 8) {
+        double a = red[l][0];
+        for (int wv = 1; wv < kBrickCells / kWave; ++wv) {
+            const double w = red[l][wv];
+            a = (l == 0 || l == 7) ? a + w : ((l == 1 || l == 3) ? fmin(a, w) : fmax(a, w));
+        }
+        part[8 * (int64_t)blockIdx.x + l] = a;
+    }
+}
+
 // Per-brick constant coupling: kappa[slot] = c when every coupling touching the
 // brick's cells (its 3 low faces per cell and the far faces stored in the +x/+y/+z
 // neighbour bricks) equals one nonzero value c bit for bit, and (with a diagonal
@@ -577,6 +654,12 @@ void launch_scale_coeff(hipStream_t st, int bx, int n_active, const int *nbr, co
     auto kern = bx == 64 ? k_scale_coeff<64> : k_scale_coeff<8>;
     hipLaunchKernelGGL(kern, dim3(n_active), dim3(kBrickCells), 0, st, nbr, sig, cx, cy, cz, hx, hy, hz, hx32, hy32,
                        hz32);
+}
+
+void launch_report(hipStream_t st, int bx, int n_active, int blocks, const int *nbr, const double *diag,
+                   const double *cx, const double *cy, const double *cz, double *part) {
+    auto kern = bx == 64 ? k_report<64> : k_report<8>;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBrickCells), 0, st, n_active, nbr, diag, cx, cy, cz, part);
 }
 
 template <class C>

@@ -138,3 +138,72 @@ def test_csr_solve_refuses_grid_preconditioners(scenes, oracle, precond):
         solver.solve(t(o["rowptr"], torch.int64), t(o["cols"], torch.int64), t(o["vals"], torch.float64),
                      t(o["rhs"], torch.float64))
     solver.close()
+
+
+@pytest.mark.parametrize("precond", ["none", "jacobi"])
+def test_csr_host_pointers_match_device_pointers(scenes, oracle, precond):
+    """shk_solve_csr with HostPointers=1 (the linear-solver plugin's mode: the matrix
+    and vectors in host memory) stages them and returns the device-pointer result."""
+    import torch
+
+    import shiokaze_amd
+    from shiokaze_amd import _lib
+    o = oracle.system(scenes(3, 24, 3))
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dtype=dt).cuda()  # noqa: E731
+    cfg = {"Residual": 1e-6, "Preconditioner": precond}
+    dsol = shiokaze_amd.PCGSolver(config=cfg)
+    xd, rd = dsol.solve(t(o["rowptr"], torch.int64), t(o["cols"], torch.int64), t(o["vals"], torch.float64),
+                        t(o["rhs"], torch.float64))
+    dsol.close()
+    hsol = shiokaze_amd.PCGSolver(config=dict(cfg, HostPointers=1))
+    host = [np.ascontiguousarray(o["rowptr"], np.int64), np.ascontiguousarray(o["cols"], np.int64),
+            np.ascontiguousarray(o["vals"], np.float64), np.ascontiguousarray(o["rhs"], np.float64)]
+    xh = np.full(len(o["rhs"]), np.nan)
+    res = _lib.ShkResult()
+    rc = hsol.lib.shk_solve_csr(hsol.ctx, len(xh), *[a.ctypes.data_as(ctypes.c_void_p) for a in (*host, xh)],
+                                ctypes.byref(res))
+    hsol.close()
+    log(f"[gpu] csr host pointers {precond}: rc {rc}, {res.count} its (device pointers {rd['count']})")
+    assert rc == 0 and res.count == rd["count"]
+    np.testing.assert_array_equal(xh, xd.cpu().numpy())
+
+
+@pytest.mark.parametrize("case", [(3, 24, 3), (1, (13, 21, 9), 0)])
+def test_report_matches_oracle_matrix(scenes, oracle, case):
+    """Report=1 fills RCMatrix_utility::report's statistics (RCMatrix_utility.h:98-146)
+    of the assembled matrix; the oracle's CSR (exact zeros dropped, as
+    RCMatrix::add_to_element does) gives the same figures."""
+    import scipy.sparse as sp
+    sc = scenes(*case)
+    g = run_projection(sc, {"Residual": 1e-6, "Report": 1}, want_system=False)
+    r = g["result"]
+    o = oracle.system(sc)
+    n = len(o["rhs"])
+    A = sp.csr_matrix((o["vals"], o["cols"], o["rowptr"]), shape=(n, n))
+    d = A.diagonal()
+    off = abs(A - sp.diags(d)).max(axis=1).toarray().ravel()
+    rowlen = np.diff(o["rowptr"])
+    log(f"[gpu] report {case}: nnz {r['report_nnz']} rows {r['report_min_row']}..{r['report_max_row']} "
+        f"diag {r['report_min_diag']:.4g}..{r['report_max_diag']:.4g} ratio {r['report_max_offdiag_ratio']:.4g}")
+    assert r["report_valid"] == 1 and r["report_has_nan"] == 0
+    assert r["report_nnz"] == len(o["vals"]) == A.nnz
+    assert (r["report_min_row"], r["report_max_row"]) == (rowlen.min(), rowlen.max())
+    assert (r["report_min_diag"], r["report_max_diag"]) == (d.min(), d.max())
+    assert r["report_max_offdiag_ratio"] == pytest.approx((off / d).max(), rel=1e-15)
+    assert r["report_symmetry_error"] == 0.0 == abs(A - A.T).max()
+
+
+def test_report_flags_nan_in_the_matrix(scenes, oracle):
+    """A NaN in the liquid level set next to the surface makes a face's fluid fraction,
+    hence matrix entries, NaN (so does the oracle's assembly): report_has_nan says so
+    (RCMatrix_utility.h:116)."""
+    sc = dict(scenes(3, 16, 2))
+    sc["fluid"] = sc["fluid"].copy()
+    sc["fluid"][0, 6, 1] = np.nan
+    assert np.isnan(oracle.system(sc)["vals"]).any()
+    g = run_projection(sc, {"Residual": 1e-6, "Report": 1}, want_system=False)
+    r = g["result"]
+    log(f"[gpu] report with a NaN solid node: has_nan {r['report_has_nan']} status {r['status']}")
+    assert r["report_valid"] == 1 and r["report_has_nan"] == 1
+    g = run_projection(scenes(3, 16, 2), {"Residual": 1e-6}, want_system=False)
+    assert g["result"]["report_valid"] == 0
