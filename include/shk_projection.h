@@ -228,6 +228,21 @@ int shk_divergence(shk_ctx *ctx, double dt, const float *u, const float *v, cons
 int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64_t *cols,
                   const double *vals, const double *b, double *x, shk_result *result);
 
+/* No counterpart in the reference.  shk_solve_csr for callers that cannot allocate
+ * device memory during a solve (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and result, with every piece of
+ * per-call device workspace (partials, the five CG vectors, the diagonal and the
+ * Jacobi-scaled values) carved from one static per-device array of
+ * shk_noalloc_scratch_bytes() bytes (128 MiB) instead of context buffers.  It calls
+ * no hipMalloc or hipFree; the context itself is allocated by shk_create as usual.
+ * Preconditioner none or jacobi; a system whose workspace
+ * (6 * rows + nnz [jacobi] + 7,360 doubles) exceeds the array returns SHK_BAD_ARG, as
+ * do ProfileKernels=1 and HostPointers=1.  Every call on one device shares that
+ * device's array: calls that overlap on one GPU (threads, contexts or streams) race. */
+int shk_solve_csr_noalloc(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64_t *cols,
+                          const double *vals, const double *b, double *x, shk_result *result);
+size_t shk_noalloc_scratch_bytes(void);
+
 /* Accumulated launch count and total milliseconds of one Krylov kernel
  * ("spmv_dot": direction update fused with the stencil and s.As, or "axpy_norm")
  * since ProfileKernels was last set.

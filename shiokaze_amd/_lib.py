@@ -89,6 +89,8 @@ _SIGNATURES = {
     "shk_divergence": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 6 + [ctypes.c_int,
                                                                           ctypes.POINTER(ctypes.c_double)]),
     "shk_solve_csr": (ctypes.c_int, [_vp, ctypes.c_int64] + [_vp] * 5 + [ctypes.POINTER(ShkResult)]),
+    "shk_solve_csr_noalloc": (ctypes.c_int, [_vp, ctypes.c_int64] + [_vp] * 5 + [ctypes.POINTER(ShkResult)]),
+    "shk_noalloc_scratch_bytes": (ctypes.c_size_t, []),
     "shk_kernel_stats": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64),
                                         ctypes.POINTER(ctypes.c_double)]),
     "shk_scene_generate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,

@@ -366,6 +366,8 @@ int create_impl(int nx, int ny, int nz, int bx, double dx, int device, shk_ctx *
     };
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return bail(e, "hipSetDevice");
+    // the device address of this device's allocation-free solve scratch (noalloc.hip)
+    if ((e = shk::scratch_bind(device)) != hipSuccess) return bail(e, "hipGetSymbolAddress(scratch)");
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess)
         return bail(e, "hipStreamCreate");
     for (auto &ev : c->ev)

@@ -203,6 +203,11 @@ int true_residual(shk_ctx *c, const double *x, double *r_out, double out[4], con
 void fill_true_residual(shk_result *res, const double rr[4]);
 int solve_fp64(shk_ctx *c, shk_result *res);
 
+// allocation-free CSR solve (noalloc.hip): looks up, once per device and with that
+// device current, the address of the device's static scratch array
+hipError_t scratch_bind(int device);
+size_t scratch_capacity_bytes();
+
 // learned preconditioner (unet.hip)
 void unet_destroy(UnetPrecond *u);
 void mg_destroy(MgHierarchy *h);

@@ -38,7 +38,10 @@ class PCGSolver:
         except Exception:
             pass
 
-    def solve(self, rowptr: torch.Tensor, cols: torch.Tensor, vals: torch.Tensor, b: torch.Tensor):
+    def solve(self, rowptr: torch.Tensor, cols: torch.Tensor, vals: torch.Tensor, b: torch.Tensor,
+              noalloc: bool = False):
+        """noalloc=True runs shk_solve_csr_noalloc: the same solve with its device
+        workspace in the library's static per-device scratch array (no hipMalloc)."""
         rows = b.numel()
         for t, dt, name in ((rowptr, torch.int64, "rowptr"), (cols, torch.int64, "cols"),
                             (vals, torch.float64, "vals"), (b, torch.float64, "b")):
@@ -49,7 +52,8 @@ class PCGSolver:
         x = torch.empty_like(b)
         res = _lib.ShkResult()
         torch.cuda.synchronize(self.device)
-        rc = self.lib.shk_solve_csr(self.ctx, rows, *[ctypes.c_void_p(t.data_ptr()) for t in (rowptr, cols, vals, b, x)],
+        fn = self.lib.shk_solve_csr_noalloc if noalloc else self.lib.shk_solve_csr
+        rc = fn(self.ctx, rows, *[ctypes.c_void_p(t.data_ptr()) for t in (rowptr, cols, vals, b, x)],
                                     ctypes.byref(res))
         _lib.check(rc, self.ctx, _lib.SOLVER_OUTCOMES)
         out = res.as_dict()
