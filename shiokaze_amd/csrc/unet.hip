@@ -180,6 +180,9 @@ struct UnetNet {
     int64_t step = 0;
     double omega = 0.8;  // smoothing weight the network was trained with (saved with it)
     bf16 *pk[kLayers] = {}, *pkt[kLayers] = {};
+    uint32_t *head_pk = nullptr;  // the head's output-channel-0 weights as BF16 pairs (inference)
+    float *split_ws = nullptr;    // K-split partial sums of the small-grid convolutions (inference)
+    int64_t split_cap = 0;
     bool packed_dirty = true;
     // activations for the current half-resolution grid
     Grid3 L[4] = {};
@@ -220,6 +223,7 @@ int repack(UnetNet *n) {
         if (kSpec[l].need_dgrad)
             launch_conv_pack(n->stream, n->param + n->woff[l], kSpec[l].cin, kSpec[l].cout, true, n->pkt[l]);
     }
+    launch_head_pack(n->stream, n->param + n->woff[kLayers - 1], kSpec[kLayers - 1].cout, n->head_pk);
     UHIP(n, hipGetLastError());
     n->packed_dirty = false;
     return SHK_OK;
@@ -243,6 +247,14 @@ int ensure_grid(UnetNet *n, Grid3 m) {
     for (auto &x : a) UHIP(n, dalloc(*x.p, (size_t)x.n));
     UHIP(n, dalloc(n->u, (size_t)v0));
     UHIP(n, dalloc(n->du, (size_t)v0));
+    // the K-split workspace of the inference convolutions on this grid
+    int64_t cap = 0;
+    for (int l = 0; l < kLayers - 1; ++l) {
+        const int lev = l < 4 ? l : kLayers - 2 - l;  // enc0..bott on levels 0..3, dec2..dec0 on 2..0
+        cap = std::max(cap, conv_split_floats(n->L[lev], kSpec[l].cin, kSpec[l].cout));
+    }
+    if (cap > 0) UHIP(n, dalloc(n->split_ws, (size_t)cap));
+    n->split_cap = cap;
     return SHK_OK;
 }
 
@@ -291,22 +303,31 @@ void wgrad(UnetNet *n, int l, Grid3 g, const bf16 *x, const bf16 *dz) {
 // otherwise each decoder convolution reads [upsample(coarse), skip] directly.
 void forward(UnetNet *n, bool train) {
     const Grid3 *L = n->L;
-    conv(n, 0, L[0], n->feat, n->enc0);
+    // inference splits the K loop of the small grids' convolutions (training keeps the
+    // unsplit kernels its gradient checks were made with)
+    float *ws = train ? nullptr : n->split_ws;
+    const int64_t wc = train ? 0 : n->split_cap;
+    auto enc = [&](int l, Grid3 g, const bf16 *x, bf16 *y) {
+        launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout, kSpec[l].relu,
+                        y, nullptr, ws, wc);
+    };
+    enc(0, L[0], n->feat, n->enc0);
     launch_pool(n->stream, L[1], 16, n->enc0, n->p1);
-    conv(n, 1, L[1], n->p1, n->enc1);
+    enc(1, L[1], n->p1, n->enc1);
     launch_pool(n->stream, L[2], 32, n->enc1, n->p2);
-    conv(n, 2, L[2], n->p2, n->enc2);
+    enc(2, L[2], n->p2, n->enc2);
     launch_pool(n->stream, L[3], 64, n->enc2, n->p3);
-    conv(n, 3, L[3], n->p3, n->bott);
+    enc(3, L[3], n->p3, n->bott);
     if (!train) {
         auto cat = [&](int l, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, bf16 *y) {
             launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                kSpec[l].cout, kSpec[l].relu, y);
+                                kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap);
         };
         cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
         cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);
         cat(6, L[0], n->dec1, 32, n->enc0, n->dec0);
-        conv(n, 7, L[0], n->dec0, nullptr, n->u);
+        // only output channel 0 of the head is used: the vector-ALU head kernel
+        launch_conv_head(n->stream, L[0], n->dec0, n->head_pk, n->param + n->boff[7], n->u);
         return;
     }
     launch_upsample(n->stream, L[3], 128, n->bott, 192, n->cat2);
@@ -396,6 +417,8 @@ void unet_net_destroy(UnetNet *n) {
         dfree(n->pk[l]);
         dfree(n->pkt[l]);
     }
+    dfree(n->head_pk);
+    dfree(n->split_ws);
     bf16 **acts[] = {&n->feat, &n->enc0, &n->p1,   &n->enc1, &n->p2,  &n->enc2,  &n->p3,   &n->bott,
                      &n->cat2, &n->dec2, &n->cat1, &n->dec1, &n->cat0, &n->dec0, &n->dout, &n->d16,
                      &n->dcat0, &n->dd1, &n->dcat1, &n->dd2, &n->dcat2, &n->dbott, &n->dp3, &n->de2,
@@ -437,6 +460,7 @@ int unet_net_create(uint64_t seed, hipStream_t stream, UnetNet **out) {
             (e = dalloc(n->pkt[l], (size_t)conv_packed_elems(kSpec[l].cout, kSpec[l].cin))) != hipSuccess)
             return bail(e);
     }
+    if ((e = dalloc(n->head_pk, 27 * 8)) != hipSuccess) return bail(e);
     if ((e = dalloc(n->part, 3 * kRedBlocks)) != hipSuccess) return bail(e);
     if ((e = hipHostMalloc((void **)&n->h_part, sizeof(double) * 3 * kRedBlocks, hipHostMallocDefault)) != hipSuccess)
         return bail(e);

@@ -43,14 +43,25 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
 
 // y[v][co] = act(bias[co] + sum_k w * x); act = ReLU when relu.  out_f32 (nullable)
 // receives channel 0 in FP32 instead of bf16 output (the network's final layer).
+// ws (ws_floats floats, may be null): the workspace of K splits on grids too small to
+// fill the chip (conv_split_floats); without it the convolution runs unsplit.
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32);
+                     bool relu, bf16 *y, float *out_f32, float *ws = nullptr, int64_t ws_floats = 0);
+// workspace floats a BF16-output convolution of this shape splits into (0: unsplit)
+int64_t conv_split_floats(Grid3 g, int cin, int cout);
 
 // The same convolution of the concatenation [nearest upsampling of coarse (c1
 // channels, grid g/2), skip (cin - c1 channels, grid g)], read from the two tensors
 // directly; c1 and cin - c1 are multiples of 8.
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
-                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y);
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws = nullptr,
+                         int64_t ws_floats = 0);
+
+// y[v] = bias[0] + sum_{tap, ci} W[tap][ci][0] x[v + tap][ci] for a 16-channel BF16 input
+// (the network's head, of which only output channel 0 is used); wpk from
+// launch_head_pack (27 * 8 BF16 pairs)
+void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y);
+void launch_head_pack(hipStream_t st, const float *w, int cout, uint32_t *wpk);
 
 // dw[tap][ci][co] += sum_v x[v + tap][ci] * dz[v][co]; db[co] += sum_v dz[v][co]
 // (FP32 atomics, split over voxel chunks).

@@ -87,11 +87,18 @@ constexpr int kHalo = kHX * kHY * kHZ;
 // With xc: the input is the concatenation [upsample(xc), x] of the nearest upsampling
 // of the half-size grid xc (its first c1 channels) and x (the other cin - c1), read
 // straight from the two tensors while staging (no concatenated buffer).
+//
+// Split K (small grids, split_out != nullptr): blockIdx.z also indexes one of nsplit splits of
+// the stages x KPC weight chunks; a split stages only the halos of the stages its chunk
+// range touches, and writes its raw FP32 sums to part[split][voxel][cout] for
+// k_conv_split_finish, which adds the splits in order, the bias and the ReLU
+// (split_out == nullptr: one split, the epilogue applies them).
 template <int MTB, int CC>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total,
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
                                                   int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32,
-                                                  const uint16_t *__restrict__ xc, int c1) {
+                                                  const uint16_t *__restrict__ xc, int c1, int nsplit,
+                                                  float *__restrict__ split_out) {
     constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
     // voxel stride in uint4, odd so that 16 lanes reading 16 consecutive voxels hit 16
     // disjoint 4-bank groups of the 64-bank LDS
@@ -102,16 +109,21 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int mgroups = mt_total / MTB;
-    const int ztile = blockIdx.z / mgroups, mg = blockIdx.z - ztile * mgroups;
+    const int zs = blockIdx.z / nsplit, sp = blockIdx.z - zs * nsplit;
+    const int ztile = zs / mgroups, mg = zs - ztile * mgroups;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = ztile * kTZ;
     const int stages = cin / CC;
     const int kchunks = stages * KPC;
+    // this block's weight chunks [c_lo, c_hi) of the stages x KPC (all of them unsplit)
+    const int cper = (kchunks + nsplit - 1) / nsplit;
+    const int c_lo = sp * cper, c_hi = c_lo + cper < kchunks ? c_lo + cper : kchunks;
     f32x4 acc[MTB][4];
 #pragma unroll
     for (int mt = 0; mt < MTB; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int stage = 0; stage < stages; ++stage) {
+        if ((stage + 1) * KPC <= c_lo || stage * KPC >= c_hi) continue;  // block-uniform
         __syncthreads();  // the previous stage's reads are done
         for (int idx = threadIdx.x; idx < kHalo * G8; idx += 256) {
             const int vox = idx / G8, part = idx - vox * G8;
@@ -146,8 +158,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
 #pragma unroll
         for (int kc = 0; kc < KPC; ++kc) {
             const int k0 = kc * 32 + kq * 8;
+            const int gc = stage * KPC + kc;
+            const bool on = gc >= c_lo && gc < c_hi;  // block-uniform: this split's chunk
             Frag b[4];
-            if (k0 < 27 * CC) {
+            if (on && k0 < 27 * CC) {
                 const int tap = k0 / CC, grp = (k0 - tap * CC) / 8;
                 int dx, dy, dz;
                 tap_offset(tap, dx, dy, dz);
@@ -164,6 +178,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
                 a[mt] = ar[kc % PF][mt];
                 if (kc + PF < KPC) ar[kc % PF][mt].u = abase[((int64_t)mt * kchunks + kc + PF) * 64];
             }
+            if (!on) continue;
 #pragma unroll
             for (int mt = 0; mt < MTB; ++mt)
 #pragma unroll
@@ -183,6 +198,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
 #pragma unroll
         for (int mt = 0; mt < MTB; ++mt) {
             const int co = (mg * MTB + mt) * 16 + 4 * kq;
+            if (split_out) {
+                *reinterpret_cast<float4 *>(split_out + ((int64_t)sp * g.vox() + vb) * cout + co) =
+                    make_float4(acc[mt][nt][0], acc[mt][nt][1], acc[mt][nt][2], acc[mt][nt][3]);
+                continue;
+            }
             float o[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -199,6 +219,94 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
             }
         }
     }
+}
+
+// ---------------------------------------------------------------- head (16 -> 1 channel)
+
+// The network's last convolution has one used output channel: y[v] = b + sum over the
+// 27 taps and 16 input channels of W[tap][ci] x[v + tap][ci].  As an MFMA GEMM it would
+// compute 16 output rows to keep one, so it runs on the vector ALUs with packed BF16
+// channel-pair dot products (v_dot2c_f32_bf16, FP32 accumulation).  Block: 256 threads
+// over a 16 x 8 x 8 voxel tile, each thread 4 consecutive x voxels; the 18 x 10 x 10
+// input halo is staged in LDS as two arrays of 8 channels (one 16-byte row per voxel,
+// rows of 19 voxels so that the 16 lanes of one LDS phase, 4 x-quads by 4 y rows, hit 16
+// disjoint bank groups).  The weights, packed as BF16 pairs wpk[tap * 8 + pair], are
+// wave-uniform and live in scalar registers.
+constexpr int kHeadTX = 16, kHeadTY = 8, kHeadTZ = 8;
+constexpr int kHeadHX = 19, kHeadHY = kHeadTY + 2, kHeadHZ = kHeadTZ + 2;
+constexpr int kHeadRows = kHeadHX * kHeadHY * kHeadHZ;
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline float dot2(uint32_t a, uint32_t w, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a), __builtin_bit_cast(bf16x2, w), acc, false);
+}
+
+__global__ __launch_bounds__(256) void k_conv_head(Grid3 g, const uint16_t *__restrict__ x,
+                                                   const uint32_t *__restrict__ wpk, const float *__restrict__ bias,
+                                                   float *__restrict__ y) {
+    __shared__ uint4 lds[2][kHeadRows];
+    const int t = threadIdx.x;
+    const int x0 = blockIdx.x * kHeadTX, y0 = blockIdx.y * kHeadTY, z0 = blockIdx.z * kHeadTZ;
+    // stage the halo (hx 0..17 used of the 19-voxel rows)
+    for (int idx = t; idx < 18 * kHeadHY * kHeadHZ * 2; idx += 256) {
+        const int half = idx & 1, vox = idx >> 1;
+        const int hx = vox % 18, hy = (vox / 18) % kHeadHY, hz = vox / (18 * kHeadHY);
+        const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
+            v = *reinterpret_cast<const uint4 *>(x + (((int64_t)gz * g.ny + gy) * g.nx + gx) * 16 + half * 8);
+        lds[half][(hz * kHeadHY + hy) * kHeadHX + hx] = v;
+    }
+    __syncthreads();
+    const int xq = t & 3, ty = (t >> 2) & 7, tz = t >> 5;
+    const float b0 = bias[0];
+    float acc[4] = {b0, b0, b0, b0};
+    // one (dz, dy) row of taps at a time keeps 12 staged rows live, not all 108
+#pragma unroll 1
+    for (int dzy = 0; dzy < 9; ++dzy) {
+            const int dz = dzy / 3, dy = dzy - 3 * dz;
+            const int row = ((tz + dz) * kHeadHY + (ty + dy)) * kHeadHX + 4 * xq;
+#pragma unroll
+            for (int half = 0; half < 2; ++half) {
+                uint4 in[6];
+#pragma unroll
+                for (int p = 0; p < 6; ++p) in[p] = lds[half][row + p];
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const uint32_t *w = wpk + ((dz * 3 + dy) * 3 + dx) * 8 + half * 4;
+                    const uint32_t w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const uint4 v = in[j + dx];
+                        float a = acc[j];
+                        a = dot2(v.x, w0, a);
+                        a = dot2(v.y, w1, a);
+                        a = dot2(v.z, w2, a);
+                        a = dot2(v.w, w3, a);
+                        acc[j] = a;
+                    }
+                }
+            }
+        }
+    const int gy = y0 + ty, gz = z0 + tz;
+    if (gy >= g.ny || gz >= g.nz) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int gx = x0 + 4 * xq + j;
+        if (gx < g.nx) y[((int64_t)gz * g.ny + gy) * g.nx + gx] = acc[j];
+    }
+}
+
+// wpk[tap * 8 + pair] = (bf16(W[tap][2 pair][0]), bf16(W[tap][2 pair + 1][0])) from the
+// FP32 weights W[tap][ci][co] of a 16 -> cout convolution (output channel 0)
+__global__ void k_head_pack(const float *__restrict__ w, int cout, uint32_t *__restrict__ wpk) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 27 * 8) return;
+    const int tap = i / 8, pair = i % 8;
+    const float lo = w[((int64_t)tap * 16 + 2 * pair) * cout];
+    const float hi = w[((int64_t)tap * 16 + 2 * pair + 1) * cout];
+    wpk[i] = (uint32_t)bf16_bits(lo) | ((uint32_t)bf16_bits(hi) << 16);
 }
 
 // ---------------------------------------------------------------- weight gradient
@@ -647,69 +755,142 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
                        conv_cc(gin), conv_kpc(gin), kchunks, U(packed), total);
 }
 
+void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y) {
+    const dim3 grid((unsigned)((g.nx + kHeadTX - 1) / kHeadTX), (unsigned)((g.ny + kHeadTY - 1) / kHeadTY),
+                    (unsigned)((g.nz + kHeadTZ - 1) / kHeadTZ));
+    hipLaunchKernelGGL(k_conv_head, grid, dim3(256), 0, st, g, U(x), wpk, bias, y);
+}
+
+void launch_head_pack(hipStream_t st, const float *w, int cout, uint32_t *wpk) {
+    hipLaunchKernelGGL(k_head_pack, dim3(1), dim3(256), 0, st, w, cout, wpk);
+}
+
 namespace {
 template <int MTB>
 void conv_fwd_mt(hipStream_t st, dim3 grid, Grid3 g, const bf16 *x, int cin, int mt, const uint4 *pk,
-                 const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1) {
+                 const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1, int nsplit,
+                 float *split_out) {
     switch (conv_cc(cin)) {
         case 8:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 8>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1);
+                               out_f32, xc, c1, nsplit, split_out);
             break;
         case 16:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 16>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1);
+                               out_f32, xc, c1, nsplit, split_out);
             break;
         case 24:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 24>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1);
+                               out_f32, xc, c1, nsplit, split_out);
             break;
         default:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 32>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1);
+                               out_f32, xc, c1, nsplit, split_out);
             break;
     }
 }
+
+// y[v][c] = relu(bias[c] + sum over the splits of part[split][v][c]), in split order
+__global__ void k_conv_split_finish(int64_t n, int cout, int nsplit, const float *__restrict__ part,
+                                    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // 4 channels of one voxel
+    if (q * 4 >= n) return;
+    float4 s = *reinterpret_cast<const float4 *>(part + q * 4);
+    for (int k = 1; k < nsplit; ++k) {
+        const float4 t = *reinterpret_cast<const float4 *>(part + (int64_t)k * n + q * 4);
+        s.x += t.x;
+        s.y += t.y;
+        s.z += t.z;
+        s.w += t.w;
+    }
+    const int c = (int)((q * 4) % cout);
+    float o[4] = {s.x, s.y, s.z, s.w};
+    for (int j = 0; j < 4; ++j) {
+        o[j] = o[j] + (bias ? bias[c + j] : 0.0f);
+        if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
+    }
+    uint2 po;
+    po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+    po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+    *reinterpret_cast<uint2 *>(y + q * 4) = po;
+}
+
+struct ConvPlan {
+    int bx, by, bz, mtb, nsplit;
+};
+
+// Blocks of the convolution: output tiles per block (all of them, unless the grid is
+// too small to fill the chip, then the largest divisor of mt that gives >= 1024
+// blocks, or 1), and, when even that leaves fewer than 1024 blocks, K splits of at
+// least 4 weight chunks each up to about 1024 blocks.
+ConvPlan conv_plan(Grid3 g, int cin, int cout) {
+    ConvPlan p;
+    const int mt = cout / 16;
+    p.bx = (g.nx + kTX - 1) / kTX;
+    p.by = (g.ny + kTY - 1) / kTY;
+    p.bz = (g.nz + kTZ - 1) / kTZ;
+    const int64_t spatial = (int64_t)p.bx * p.by * p.bz;
+    p.mtb = mt;
+    for (int d = mt; d >= 1; --d) {
+        if (mt % d) continue;
+        p.mtb = d;
+        if (spatial * (mt / d) >= 1024) break;
+    }
+    const int64_t blocks = spatial * (mt / p.mtb);
+    const int chunks = (cin / conv_cc(cin)) * conv_kpc(cin);
+    p.nsplit = 1;
+    if (blocks < 1024) {
+        const int64_t want = (1024 + blocks - 1) / blocks;
+        const int most = chunks / 4;
+        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(want, most));
+    }
+    return p;
+}
 }  // namespace
+
+int64_t conv_split_floats(Grid3 g, int cin, int cout) {
+    const ConvPlan p = conv_plan(g, cin, cout);
+    return p.nsplit > 1 ? (int64_t)p.nsplit * g.vox() * cout : 0;
+}
 
 namespace {
 void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1) {
+                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1, float *ws, int64_t ws_floats) {
     const int mt = cout / 16;
-    const int bx = (g.nx + kTX - 1) / kTX, by = (g.ny + kTY - 1) / kTY, bz = (g.nz + kTZ - 1) / kTZ;
-    // output tiles per block: all of them, unless the grid is too small to fill the
-    // chip, then the largest divisor of mt that gives >= 1024 blocks (or 1)
-    const int64_t spatial = (int64_t)bx * by * bz;
-    int mtb = mt;
-    for (int d = mt; d >= 1; --d) {
-        if (mt % d) continue;
-        mtb = d;
-        if (spatial * (mt / d) >= 1024) break;
-    }
-    const dim3 grid((unsigned)bx, (unsigned)by, (unsigned)(bz * (mt / mtb)));
+    ConvPlan p = conv_plan(g, cin, cout);
+    // K splits need a workspace of nsplit x voxels x cout floats and a BF16 output
+    if (p.nsplit > 1 && (!ws || out_f32 || !y || (int64_t)p.nsplit * g.vox() * cout > ws_floats)) p.nsplit = 1;
+    const dim3 grid((unsigned)p.bx, (unsigned)p.by, (unsigned)(p.bz * (mt / p.mtb) * p.nsplit));
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     uint16_t *yy = y ? U(y) : nullptr;
-    switch (mtb) {
-        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1); break;
-        default: break;  // unsupported widths are rejected by the caller
+    float *so = p.nsplit > 1 ? ws : nullptr;
+    switch (p.mtb) {
+        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        default: return;  // unsupported widths are rejected by the caller
+    }
+    if (so) {
+        const int64_t n = g.vox() * cout;
+        hipLaunchKernelGGL(k_conv_split_finish, dim3((unsigned)((n / 4 + 255) / 256)), dim3(256), 0, st, n, cout,
+                           p.nsplit, so, bias, relu ? 1 : 0, yy);
     }
 }
 }  // namespace
 
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32) {
-    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0);
+                     bool relu, bf16 *y, float *out_f32, float *ws, int64_t ws_floats) {
+    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0, ws, ws_floats);
 }
 
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
-                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y) {
-    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1);
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws,
+                         int64_t ws_floats) {
+    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, ws, ws_floats);
 }
 
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,

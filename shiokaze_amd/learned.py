@@ -101,12 +101,14 @@ class UNet:
         return z
 
 
-# (cin, cout, level) of the network's 3x3x3 convolutions; level l runs on the fine grid / 2^(l+1)
-LAYERS = [(8, 16, 0), (16, 32, 1), (32, 64, 2), (64, 128, 3), (192, 64, 2), (96, 32, 1), (48, 16, 0), (16, 16, 0)]
+# (cin, cout, level) of the network's 3x3x3 convolutions as inference computes them; level
+# l runs on the fine grid / 2^(l+1).  The head has 16 outputs but only channel 0 is used,
+# and inference computes that one alone (on the vector ALUs).
+LAYERS = [(8, 16, 0), (16, 32, 1), (32, 64, 2), (64, 128, 3), (192, 64, 2), (96, 32, 1), (48, 16, 0), (16, 1, 0)]
 
 
 def flops_per_apply(shape) -> float:
-    """Dense MFMA FLOPs of one network forward for a fine grid (nx, ny, nz), padded to
+    """FLOPs of one network forward (inference) for a fine grid (nx, ny, nz), padded to
     multiples of 16 as the solver pads it: 2 * 27 * cin * cout per output voxel."""
     pad = [(int(d) + 15) // 16 * 16 for d in shape]
     total = 0.0

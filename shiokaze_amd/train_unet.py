@@ -139,7 +139,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--sizes", default=None,
-                    help="comma-separated grid sizes to train on in turn (default: --n); seed s uses sizes[s % k]")
+                    help="comma-separated grid sizes to train on in turn (default: --n); seed s uses sizes[s %% k]")
     ap.add_argument("--eval-sizes", default=None, help="comma-separated grid sizes to evaluate on (default: --n)")
     ap.add_argument("--kind", type=int, default=3)
     ap.add_argument("--scenes", type=int, default=2000, help="training scenes (seeds 0..scenes-1)")
@@ -155,10 +155,26 @@ def main(argv=None):
     ap.add_argument("--eval-residual", type=float, default=1e-6)
     ap.add_argument("--time-budget", type=float, default=600.0, help="seconds of training before evaluation")
     ap.add_argument("--init", default=None, help="weights file to start from")
-    ap.add_argument("--out", required=True)
+    ap.add_argument("--out", default=None, help="weights file to write (required unless --eval-only)")
+    ap.add_argument("--eval-only", default=None, metavar="WEIGHTS",
+                    help="skip training: evaluate this weights file on the held-out scenes")
     ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--report", default=None, help="JSON file for the training curve and evaluation")
     args = ap.parse_args(argv)
+    if args.eval_only:
+        report = {"weights": args.eval_only, "eval": {}}
+        for n in ([int(v) for v in args.eval_sizes.split(",")] if args.eval_sizes else [args.n]):
+            count = args.eval if n <= 128 else max(1, args.eval // 4)
+            ev = evaluate(args.eval_only, n, args.kind, range(args.eval_seed0, args.eval_seed0 + count),
+                          args.eval_residual, workers=args.workers)
+            log(f"eval {n}: " + json.dumps(ev))
+            report["eval"][str(n)] = ev
+        if args.report:
+            with open(args.report, "w") as f:
+                json.dump(report, f, indent=1)
+        return report
+    if not args.out:
+        ap.error("--out is required when training")
 
     rng = np.random.default_rng(args.seed)
     net = UNet(seed=args.seed)

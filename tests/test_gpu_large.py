@@ -16,7 +16,7 @@ from gpu_helpers import PRESSURE_BAR, log, to_device
 
 pytestmark = pytest.mark.gpu
 
-WEIGHTS = Path(__file__).resolve().parents[1] / "shiokaze_amd" / "weights" / "unet_irregular_128_256.bin"
+WEIGHTS = Path(__file__).resolve().parents[1] / "shiokaze_amd" / "weights" / "unet_irregular_128.bin"
 N = 512
 # max-norm stops at which each solver meets the pressure bar at 512^3 (the error per
 # unit residual grows with the conditioning, ~n^2)

@@ -261,7 +261,7 @@ def test_export_scaled_matches_system(scenes):
 
 
 WEIGHTS = __import__("pathlib").Path(__file__).resolve().parents[1] / "shiokaze_amd" / "weights" / \
-    "unet_irregular_128_256.bin"
+    "unet_irregular_128.bin"
 
 
 def test_trained_network_gain(scenes, oracle, tmp_path):
