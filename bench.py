@@ -287,7 +287,7 @@ def main():
                     help="steps after the timed ones with per-kernel HIP event timing")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one device copy of MIB MiB first (a known-size kernel for HBM counter scaling)")
-    ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128_256.bin"),
+    ap.add_argument("--unet-weights", default=str(ROOT / "shiokaze_amd" / "weights" / "unet_irregular_128.bin"),
                     help="trained network for the learned-preconditioner line")
     ap.add_argument("--set", action="append", default=[], metavar="KEY=VALUE",
                     help="extra shk_configure keys for the timed context")
@@ -559,6 +559,8 @@ def main():
         extra["fp64_jacobi_pcg"] = {"solves_per_s": sps, "iters": rj["count"]}
         pj.close()
         extra["projection_2d"] = bench_2d_extra(args, torch, shiokaze_amd)
+        extra["csr_solve"] = csr_leg(args, torch, shiokaze_amd, make, step)
+        log(f"csr solve: {extra['csr_solve']}")
     if args.liquid or args.extra:
         extra["liquid_step"] = liquid_leg(args, torch, shiokaze_amd, S, sc, dev, cfg)
         log(f"liquid step: {extra['liquid_step']}")
@@ -595,6 +597,65 @@ def main():
     }
     print(json.dumps(out), flush=True)
     ps.close()
+
+
+def system_csr(torch, sysd):
+    """The projection's FP64 system as CSR on the device (rows = unknowns in cell order):
+    the diagonal and -c for each coupling between two unknowns, columns sorted."""
+    lab = sysd["label"].bool().reshape(-1)
+    nz, ny, nx = sysd["label"].shape
+    row = torch.full((lab.numel(),), -1, dtype=torch.int64, device=lab.device)
+    nrow = int(lab.sum())
+    row[lab] = torch.arange(nrow, device=lab.device)
+    cells = torch.nonzero(lab).reshape(-1)
+    rs, cs, vs = [row[cells]], [row[cells]], [sysd["diag"].reshape(-1)[cells]]
+    coord = (cells % nx, (cells // nx) % ny, cells // (nx * ny))
+    for key, stride, q in (("cx", 1, coord[0]), ("cy", nx, coord[1]), ("cz", nx * ny, coord[2])):
+        c = sysd[key].reshape(-1)
+        hi = cells[q >= 1]  # the low face's coupling is stored at its high cell
+        lo = hi - stride
+        ok = (c[hi] != 0) & lab[lo]
+        hi, lo = hi[ok], lo[ok]
+        v = -c[hi]
+        rs += [row[hi], row[lo]]
+        cs += [row[lo], row[hi]]
+        vs += [v, v]
+    r, cidx, v = torch.cat(rs), torch.cat(cs), torch.cat(vs)
+    order = torch.argsort(r * nrow + cidx)
+    r, cidx, v = r[order], cidx[order], v[order]
+    rowptr = torch.zeros(nrow + 1, dtype=torch.int64, device=lab.device)
+    rowptr[1:] = torch.cumsum(torch.bincount(r, minlength=nrow), 0)
+    b = sysd["rhs"].reshape(-1)[cells].contiguous()
+    return rowptr, cidx.contiguous(), v.contiguous(), b
+
+
+def csr_leg(args, torch, shiokaze_amd, make, step):
+    """shk_solve_csr (the solve() entry for callers holding a CSR matrix) on the bench
+    scene's FP64 system: Jacobi-PCG and the aggregation-multigrid PCG (its host setup
+    included, as it runs on every solve), each to the bench tolerance."""
+    ps = make({"Residual": args.residual, "Precision": "fp64", "Preconditioner": "jacobi"})
+    step(ps)
+    sysd = ps.export_system()
+    rowptr, cols, vals, b = system_csr(torch, sysd)
+    del sysd
+    ps.close()
+    out = {"rows": int(b.numel()), "nnz": int(vals.numel())}
+    for name in ("jacobi", "mg"):
+        s = shiokaze_amd.PCGSolver(config={"Residual": args.residual, "Preconditioner": name})
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.solve(rowptr, cols, vals, b)  # mg: builds the hierarchy
+        torch.cuda.synchronize()
+        first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, r = s.solve(rowptr, cols, vals, b)  # mg: the same matrix, the hierarchy reused
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        s.close()
+        out[name] = {"solves_per_s": round(1.0 / dt, 3), "iters": r["count"], "ms": round(1e3 * dt, 2),
+                     "first_solve_ms": round(1e3 * first, 2), "true_relres_inf": r["true_relres_inf"],
+                     "status": r["status"]}
+    return out
 
 
 def bench_2d_extra(args, torch, shiokaze_amd):

@@ -15,7 +15,8 @@
  * Results are bit-for-bit those of the reference's float/double arithmetic.
  *
  * Configuration keys (shk_configure): MacCormack (default 1), TrimNarrowBand
- * (default 1), WENO (only 0 is accepted) -- macadvection3.cpp:58-60, 280-284.
+ * (default 1), WENO (default 0: trilinear; 1: the semi-Lagrangian values by 6th-order
+ * WENO interpolation, WENO3.h:92-122) -- macadvection3.cpp:58-60, 280-286.
  *
  * Every call is synchronous at return and returns an shk_status code. */
 #ifndef SHK_LIQUID_H
@@ -74,6 +75,18 @@ int shk_advect_levelset(shk_ctx *ctx, double dt, float *phi, const uint8_t *mphi
  * adds (float)(dt*g) to every face and activates every face. */
 int shk_add_gravity(shk_ctx *ctx, double dt, double gx, double gy, double gz, float *u, float *v, float *w,
                     uint8_t *mu, uint8_t *mv, uint8_t *mw);
+
+/* redistancer3_interface::redistance(phi, width) of the pderedistancer3 plugin
+ * (src/redistancer/pderedistancer3.cpp:39-183) on the cell-centred level set phi
+ * (nx*ny*nz floats) and its narrow-band mask (uint8, 1 = active), both in place:
+ * the band is trimmed to the cells next to a sign change (gridutility3.cpp:184-216),
+ * dilated `width` times (new cells at -/+dx), evolved by ceil(width / IntegrationRate)
+ * steps of the redistancing PDE (Euler, or RK2 with RedistTemporalScheme=RK2) and cut at
+ * |phi| > width*dx; every cell off the final band is set to -width*dx where its value is
+ * negative and +width*dx elsewhere (the level-set fill that the reference's flood fill
+ * assigns; the dense values carry each cell's side).  *active receives the final band's
+ * cell count (may be NULL).  Keys: IntegrationRate (0.75), RedistTemporalScheme (Euler). */
+int shk_redistance(shk_ctx *ctx, float *phi, uint8_t *band, int width, int64_t *active);
 
 /* The activity the projection leaves (macpressuresolver3.cpp:269-283, `it.set_off()`):
  * after an shk_project with unknowns, an active face stays active only where its area

@@ -120,8 +120,9 @@ void shk_destroy(shk_ctx *ctx);
  *                  jacobi: diag(A) as symmetric scaling; unet: flexible CG with the
  *                  learned network below; mg: CG with an aggregation-multigrid V-cycle
  *                  (mixed precision, the counterpart of the reference's "amg" LinSolver,
- *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr takes none and
- *                  jacobi; unet and mg return SHK_BAD_ARG there.
+ *                  src/linsolver/amg.cpp:43-166).  shk_solve_csr takes none, jacobi
+ *                  and mg (pairwise algebraic aggregation of the CSR matrix); unet
+ *                  returns SHK_BAD_ARG there.
  *   MgOmega, MgAlpha  multigrid damped-Jacobi weight (0.9) and coarse-correction
  *                  factor (1.5), each in (0, 2)
  *   OuterMax       refinement steps for mixed (3)
@@ -182,6 +183,18 @@ int shk_project2d(shk_ctx *ctx, double dt, float *u, float *v, const float *soli
  * volume correction works from (with Gain != 0 and target != 0). */
 int shk_set_target_volume(shk_ctx *ctx, double current_volume, double target_volume);
 
+/* macpressuresolver3::serialize / initialize(const filestream &)
+ * (macpressuresolver3.cpp:319-344): the projection's state that a resumed run needs,
+ * in the reference's field order: shape, dx, target and current volume, the PI
+ * controller's integral and, after a warm-started projection, the previous pressure
+ * (here per cell, FP64).  shk_state_bytes gives the size, shk_save_state writes it to a
+ * host buffer of at least that size, shk_load_state restores it into a context of the
+ * same shape and dx (SHK_BAD_ARG otherwise).  The bytes are what a plugin writes through
+ * its filestream. */
+int shk_state_bytes(const shk_ctx *ctx, size_t *bytes);
+int shk_save_state(shk_ctx *ctx, void *buf, size_t capacity);
+int shk_load_state(shk_ctx *ctx, const void *buf, size_t bytes);
+
 /* macproject3_interface::set_solid_density (macproject3_interface.h:58,
  * macpressuresolver3.cpp:51-53): per-face solid density (three face arrays of the
  * u, v, w shapes; device pointers, or host pointers with HostPointers=1) that lets
@@ -224,7 +237,13 @@ int shk_divergence(shk_ctx *ctx, double dt, const float *u, const float *v, cons
 
 /* RCMatrix_solver_interface<size_t,double>::solve (RCMatrix_solver.h:77) as the
  * pcg plugin does it (pcg.cpp:45-73): x0 = 0, stop when max|r| <= Residual*max|b|.
- * CSR with sorted int64 columns, rows = number of rows; device pointers. */
+ * CSR with sorted int64 columns, rows = number of rows; device pointers (host pointers
+ * with HostPointers=1).  Preconditioner=mg is the amg plugin's solve (amg.cpp:43-166):
+ * CG preconditioned by a V-cycle over a pairwise-aggregation hierarchy built on the
+ * host from the matrix and kept for the next call with the same matrix (a content
+ * hash of rowptr, cols and vals decides); a matrix whose coarsest level (where the
+ * aggregation stops shrinking) keeps more than 1,024 rows returns SHK_BAD_ARG, one whose
+ * coarsest matrix is not positive definite SHK_BREAKDOWN. */
 int shk_solve_csr(shk_ctx *ctx, int64_t rows, const int64_t *rowptr, const int64_t *cols,
                   const double *vals, const double *b, double *x, shk_result *result);
 

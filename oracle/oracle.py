@@ -56,6 +56,8 @@ _SIG = {
     "shk_or_area_fraction2d": (None, [_i, _i, _vp, _d, _i, _vp, _vp]),
     "shk_or_curvature2d": (None, [_i, _i, _d, _vp, _i, _vp]),
     "shk_or_extrapolate": (_i64, [_i, _i, _i, _i, _vp, _vp, _i]),
+    "shk_or_redistance": (_i64, [_i, _i, _i, _d, _vp, _vp, _i, _d, _i]),
+    "shk_or_set_weno": (None, [_i]),
     "shk_or_constrain_velocity": (None, [_i, _i, _i, _d, _vp] + [_vp] * 6),
     "shk_or_compute_max_u": (_d, [_i, _i, _i] + [_vp] * 6),
     "shk_or_advect_velocity": (None, [_i, _i, _i, _d, _d] + [_vp] * 7 + [_i, _i]),
@@ -502,6 +504,18 @@ def extrapolate(vel, masks, width):
     return out, ms, total
 
 
+def redistance(dx, phi, band, width, rate=0.75, rk2=False):
+    """pderedistancer3::redistance(phi, width) (pderedistancer3.cpp:39-183) on a dense
+    (nz, ny, nx) level set with its activity mask.  Returns (phi, band, active count):
+    the band after the trim, `width` dilations and the PDE, cells off it at -/+ width*dx."""
+    lib = load()
+    nz, ny, nx = phi.shape
+    out = _f32(phi).copy()
+    m = _u8(band).copy()
+    cnt = lib.shk_or_redistance(nx, ny, nz, float(dx), _p(out), _p(m), int(width), float(rate), 1 if rk2 else 0)
+    return out, m, cnt
+
+
 def constrain_velocity(dx, solid, vel, masks):
     """macutility3::constrain_velocity (macutility3.cpp:61-92)."""
     lib = load()
@@ -523,16 +537,21 @@ def compute_max_u(vel, masks):
     return float(lib.shk_or_compute_max_u(nx, ny, nz, *[_p(v) for v in vs], *[_p(m) for m in ms]))
 
 
-def advect_velocity(dx, dt, vel, masks, fluid, maccormack=True, trim=1):
+def advect_velocity(dx, dt, vel, masks, fluid, maccormack=True, trim=1, weno=False):
     """macadvection3::advect_vector (macadvection3.cpp:46-54, 70-177), the velocity
-    advecting itself; inactive faces come back 0."""
+    advecting itself; inactive faces come back 0.  weno: WENO=1 (WENO3 interpolation of
+    the semi-Lagrangian values)."""
     lib = load()
     nx, ny, nz = _cell_dims(vel)
     out = [_f32(v).copy() for v in vel]
     ms = [_u8(m) for m in masks]
     fl = _f32(fluid)
-    lib.shk_or_advect_velocity(nx, ny, nz, float(dx), float(dt), *[_p(v) for v in out], *[_p(m) for m in ms],
-                               _p(fl), 1 if maccormack else 0, int(trim))
+    lib.shk_or_set_weno(1 if weno else 0)
+    try:
+        lib.shk_or_advect_velocity(nx, ny, nz, float(dx), float(dt), *[_p(v) for v in out], *[_p(m) for m in ms],
+                                   _p(fl), 1 if maccormack else 0, int(trim))
+    finally:
+        lib.shk_or_set_weno(0)
     return out
 
 
@@ -560,7 +579,7 @@ def project_activity(areas, rhos, masks):
     return ms
 
 
-def advect_levelset(dx, dt, phi, mphi, vel, masks, background, maccormack=True, trim=1):
+def advect_levelset(dx, dt, phi, mphi, vel, masks, background, maccormack=True, trim=1, weno=False):
     """macadvection3::advect_scalar(fluid, velocity, fluid_save, dt) as the level-set
     tracker calls it (maclevelsetsurfacetracker3.cpp:49-52): the active (narrow-band)
     cells of the cell-centred level set `phi` advected by the MAC velocity; inactive
@@ -573,6 +592,10 @@ def advect_levelset(dx, dt, phi, mphi, vel, masks, background, maccormack=True, 
     mp = _u8(mphi)
     vs = [_f32(v) for v in vel]
     ms = [_u8(m) for m in masks]
-    lib.shk_or_advect_levelset(nx, ny, nz, float(dx), float(dt), _p(out), _p(mp), *[_p(v) for v in vs],
-                               *[_p(m) for m in ms], float(background), 1 if maccormack else 0, int(trim))
+    lib.shk_or_set_weno(1 if weno else 0)
+    try:
+        lib.shk_or_advect_levelset(nx, ny, nz, float(dx), float(dt), _p(out), _p(mp), *[_p(v) for v in vs],
+                                   *[_p(m) for m in ms], float(background), 1 if maccormack else 0, int(trim))
+    finally:
+        lib.shk_or_set_weno(0)
     return out

@@ -31,6 +31,16 @@
  *   shk_or_add_gravity         macliquid3.cpp:195-210 (`m_velocity += dt*gravity`
  *                              activates every face: array3.h:647-655, 917-921, 1152)
  *   shk_or_project_activity    macpressuresolver3.cpp:269-283 (set_off where area*rho == 0)
+ *   shk_or_set_weno            WENO=1 of macadvection3 (macadvection3.cpp:59, 85-86, 193-194):
+ *                              the semi-Lagrangian values by WENO3::interpolate (order 6,
+ *                              WENO3.h:92-122, WENO.h interp6: Macdonald & Ruuth 2008)
+ *                              instead of trilinear interpolation; the limiter bounds stay
+ *                              the trilinear corners
+ *   shk_or_redistance          pderedistancer3.cpp:39-183 (gridutility3.cpp:184-216 for the
+ *                              trim, dilate3.h:43-97 for the dilation, array3.h:878-951 for
+ *                              the float arithmetic of the Euler and RK2 updates); cells
+ *                              off the final band take -/+ width*dx by the sign of their
+ *                              value (the flood fill's inside test, `value < 0`)
  *
  * Pins (tests/test_oracle_liquid.py): extrapolation against scipy's binary dilation
  * for the active set, the half-space closed form, the maximum principle and a brute
@@ -67,6 +77,70 @@ static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 static float rd(const float *a, const uint8_t *m, Shape s, int i, int j, int k) {
     const int64_t q = at(s, i, j, k);
     return m[q] ? a[q] : 0.0f;
+}
+
+/* ------------------------------------------------------------------ WENO interpolation */
+
+static int g_weno = 0;
+
+void shk_or_set_weno(int on) { g_weno = on != 0; }
+
+/* the 6th-order WENO interpolation at x in [0, 1] of samples f at -2, -1, 0, 1, 2, 3
+ * (Macdonald and Ruuth, J. Sci. Comput. 35 (2008) 219-240; include/shiokaze/math/WENO.h
+ * interp6), each operation in the order written there */
+static double weno6(double x, const double f[6]) {
+    const double fm2 = f[0], fm1 = f[1], f0 = f[2], f1 = f[3], f2 = f[4], f3 = f[5];
+    const double c[3] = {(2.0 - x) * (3.0 - x) / 20.0, (3.0 - x) * (x - -2.0) / 10.0,
+                         (x - -2.0) * (x - -1.0) / 20.0};
+    const double s[3] = {
+        ((814. * (f1 * f1)) + (4326. * (f0 * f0)) + (2976. * (fm1 * fm1)) + (244. * (fm2 * fm2)) - (3579. * f0 * f1) -
+         (6927. * f0 * fm1) + (1854. * f0 * fm2) + (2634. * f1 * fm1) - (683. * f1 * fm2) - (1659. * fm1 * fm2)) /
+            180.0,
+        ((1986. * (f1 * f1)) + (1986. * (f0 * f0)) + (244. * (fm1 * fm1)) + (244. * (f2 * f2)) + (1074. * f0 * f2) -
+         (3777. * f0 * f1) - (1269. * f0 * fm1) + (1074. * f1 * fm1) - (1269. * f2 * f1) - (293. * f2 * fm1)) /
+            180.0,
+        ((814. * (f0 * f0)) + (4326. * (f1 * f1)) + (2976. * (f2 * f2)) + (244. * (f3 * f3)) - (683. * f0 * f3) +
+         (2634. * f0 * f2) - (3579. * f0 * f1) - (6927. * f1 * f2) + (1854. * f1 * f3) - (1659. * f2 * f3)) /
+            180.0};
+    const double p[3] = {
+        fm2 + (fm1 - fm2) * (x - -2.0) + (f0 - 2. * fm1 + fm2) * (x - -2.0) * (x - -1.0) / 2.0 +
+            (f1 - 3. * f0 + 3. * fm1 - fm2) * (x - -2.0) * (x - -1.0) * (x - 0.0) / 6.0,
+        fm1 + (f0 - fm1) * (x - -1.0) + (f1 - 2. * f0 + fm1) * (x - -1.0) * (x - 0.0) / 2.0 +
+            (f2 - 3. * f1 + 3. * f0 - fm1) * (x - -1.0) * (x - 0.0) * (x - 1.0) / 6.0,
+        f0 + (f1 - f0) * (x - 0.0) + (f2 - 2. * f1 + f0) * (x - 0.0) * (x - 1.0) / 2.0 +
+            (f3 - 3. * f2 + 3. * f1 - f0) * (x - 0.0) * (x - 1.0) * (x - 2.0) / 6.0};
+    double a[3], sum = 0.0;
+    for (int q = 0; q < 3; ++q) {
+        a[q] = c[q] / (DBL_EPSILON + s[q] * s[q]);
+        sum += a[q];
+    }
+    double w[3];
+    for (int q = 0; q < 3; ++q) w[q] = a[q] / sum;
+    return w[0] * p[0] + w[1] * p[1] + w[2] * p[2];
+}
+
+/* WENO3::interpolate(array, p) with order 6: the position clamped into the array, then
+ * interp6 along x for the 6 x 6 rows, along y, along z (reads clamped to the array) */
+static double weno3(const float *a, const uint8_t *m, Shape s, const double p[3]) {
+    const double x = fmax(0.0, fmin(s.nx - 1., p[0]));
+    const double y = fmax(0.0, fmin(s.ny - 1., p[1]));
+    const double z = fmax(0.0, fmin(s.nz - 1., p[2]));
+    const int i = (int)fmin(x, s.nx - 2.), j = (int)fmin(y, s.ny - 2.), k = (int)fmin(z, s.nz - 2.);
+    double vvv[6];
+    for (int kk = 0; kk < 6; ++kk) {
+        double vv[6];
+        for (int jj = 0; jj < 6; ++jj) {
+            double v[6];
+            for (int ii = 0; ii < 6; ++ii) {
+                const int64_t q = at(s, clampi(i + ii - 2, 0, s.nx - 1), clampi(j + jj - 2, 0, s.ny - 1),
+                                     clampi(k + kk - 2, 0, s.nz - 1));
+                v[ii] = (m && !m[q]) ? 0.0 : (double)a[q];
+            }
+            vv[jj] = weno6(x - i, v);
+        }
+        vvv[kk] = weno6(y - j, vv);
+    }
+    return weno6(z - k, vvv);
 }
 
 /* ------------------------------------------------------------------ extrapolation */
@@ -332,7 +406,7 @@ static void semi_lagrangian(float *const src[3], const uint8_t *const m[3], cons
                         t = (float)((double)t / dx);
                         p[d] = (double)ijk[d] - (double)t;
                     }
-                    out[f] = interp(src[dim], m[dim], s, p);
+                    out[f] = g_weno ? (float)weno3(src[dim], m[dim], s, p) : interp(src[dim], m[dim], s, p);
                 } else {
                     out[f] = here;
                 }
@@ -467,7 +541,7 @@ static void semi_lagrangian_cell(int nx, int ny, int nz, double dx, double dt, c
                 double p[3] = {(double)i, (double)j, (double)k};
                 if (moving) {
                     for (int d = 0; d < 3; ++d) p[d] = p[d] - dt * u[d] / dx;
-                    out[c] = interp(q, NULL, cs, p);
+                    out[c] = g_weno ? (float)weno3(q, NULL, cs, p) : interp(q, NULL, cs, p);
                 } else {
                     out[c] = q[c];
                 }
@@ -563,4 +637,181 @@ void shk_or_project_activity(int nx, int ny, int nz, const double *au, const dou
         for (int64_t q = 0; q < n; ++q)
             if (m[d][q] && !(a[d][q] != 0.0 && r[d][q] != 0.0)) m[d][q] = 0;
     }
+}
+
+/* ------------------------------------------------------------------ redistancing */
+
+/* pderedistancer3::derivative (pderedistancer3.cpp:86-134): sgn0 (1 - |grad phi|) with
+ * the upwind one-sided differences that switch to the interface distance where the
+ * initial level set changes sign along the axis */
+static float redist_derivative(Shape s, double dx, const float *phi, const float *phi0, const float *sgn,
+                               const uint8_t *m, int i, int j, int k) {
+    const int64_t c = at(s, i, j, k);
+    const double sgn0 = sgn[c];
+    const double p0 = phi0[c];
+    const double p = phi[c];
+    const int ijk[3] = {i, j, k};
+    const int ext[3] = {s.nx, s.ny, s.nz};
+    double grad[3];
+    for (int dim = 0; dim < 3; ++dim) {
+        const int ox = dim == 0, oy = dim == 1, oz = dim == 2;
+        int sel = 0;
+        const double tmp = sgn0 * p;
+        double pb = 0.0, pf = 0.0, g = 0.0;
+        if (ijk[dim] > 0 && m[at(s, i - ox, j - oy, k - oz)]) {
+            pb = phi[at(s, i - ox, j - oy, k - oz)];
+            if (sgn0 * pb < tmp) sel = -1;
+        }
+        if (ijk[dim] < ext[dim] - 1 && m[at(s, i + ox, j + oy, k + oz)]) {
+            pf = phi[at(s, i + ox, j + oy, k + oz)];
+            if (sgn0 * pf < tmp) {
+                if (sel == 0)
+                    sel = 1;
+                else if (sgn0 * pf < sgn0 * pb)
+                    sel = 1;
+            }
+        }
+        if (sel == -1) {
+            const double pb0 = phi0[at(s, i - ox, j - oy, k - oz)];
+            const double frac = shk_or_fraction(p0, pb0);
+            if (frac == 1.0 || frac == 0.0)
+                g = (p - pb) / dx;
+            else if (sgn0 < 0.0)
+                g = p / (dx * frac);
+            else
+                g = p / (dx * (1.0 - frac));
+        } else if (sel == 1) {
+            const double pf0 = phi0[at(s, i + ox, j + oy, k + oz)];
+            const double frac = shk_or_fraction(p0, pf0);
+            if (frac == 1.0 || frac == 0.0)
+                g = (pf - p) / dx;
+            else if (sgn0 < 0.0)
+                g = -p / (dx * frac);
+            else
+                g = -p / (dx * (1.0 - frac));
+        }
+        grad[dim] = g;
+    }
+    const double len = sqrt(grad[0] * grad[0] + grad[1] * grad[1] + grad[2] * grad[2]);
+    return (float)(sgn0 * (1.0 - len));
+}
+
+static void redist_rate(Shape s, double dx, const float *phi, const float *phi0, const float *sgn, const uint8_t *m,
+                        float *d) {
+    for (int k = 0; k < s.nz; ++k)
+        for (int j = 0; j < s.ny; ++j)
+            for (int i = 0; i < s.nx; ++i) {
+                const int64_t c = at(s, i, j, k);
+                d[c] = m[c] ? redist_derivative(s, dx, phi, phi0, sgn, m, i, j, k) : 0.0f;
+            }
+}
+
+/* phi (n cells, x fastest) and band (1 = active) in place; returns the active count */
+int64_t shk_or_redistance(int nx, int ny, int nz, double dx, float *phi, uint8_t *band, int width, double rate,
+                          int rk2) {
+    const Shape s = {nx, ny, nz};
+    const int64_t n = (int64_t)nx * ny * nz;
+    uint8_t *m = (uint8_t *)malloc((size_t)n), *nm = (uint8_t *)malloc((size_t)n);
+    float *np_ = (float *)malloc(sizeof(float) * (size_t)n), *phi0 = (float *)malloc(sizeof(float) * (size_t)n);
+    float *sgn = (float *)malloc(sizeof(float) * (size_t)n), *d0 = (float *)malloc(sizeof(float) * (size_t)n);
+    float *d1 = (float *)malloc(sizeof(float) * (size_t)n), *tmp = (float *)malloc(sizeof(float) * (size_t)n);
+    static const int q[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    /* trim_narrowband (gridutility3.cpp:184-216): keep active cells with an active axis
+     * neighbour of the opposite sign, decided on the activity before the trim */
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = at(s, i, j, k);
+                int keep = 0;
+                if (band[c]) {
+                    const double p = phi[c];
+                    for (int t = 0; t < 6 && !keep; ++t) {
+                        const int ii = i + q[t][0], jj = j + q[t][1], kk = k + q[t][2];
+                        if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
+                        const int64_t e = at(s, ii, jj, kk);
+                        if (band[e] && p * (double)phi[e] < 0.0) keep = 1;
+                    }
+                }
+                m[c] = (uint8_t)keep;
+            }
+    /* width dilations (pderedistancer3.cpp:50-68, dilate3.h:43-97): a new cell takes
+     * -dx or +dx by the sign of its first active neighbour in the order -x, +x, -y, +y,
+     * -z, +z, every round read from the state before it */
+    for (int round = 0; round < width; ++round) {
+        memcpy(nm, m, (size_t)n);
+        memcpy(np_, phi, sizeof(float) * (size_t)n);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i) {
+                    const int64_t c = at(s, i, j, k);
+                    if (m[c]) continue;
+                    for (int t = 0; t < 6; ++t) {
+                        const int ii = i + q[t][0], jj = j + q[t][1], kk = k + q[t][2];
+                        if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
+                        const int64_t e = at(s, ii, jj, kk);
+                        if (m[e]) {
+                            np_[c] = phi[e] < 0.0f ? (float)fmin(0.0, 0.0 - dx) : (float)fmax(0.0, 0.0 + dx);
+                            nm[c] = 1;
+                            break;
+                        }
+                    }
+                }
+        memcpy(m, nm, (size_t)n);
+        memcpy(phi, np_, sizeof(float) * (size_t)n);
+    }
+    /* the initial level set and its smoothed sign (pderedistancer3.cpp:70-78) */
+    for (int64_t c = 0; c < n; ++c) {
+        phi0[c] = phi[c];
+        const double v = phi[c];
+        sgn[c] = m[c] ? (float)(v / sqrt(v * v + dx * dx)) : 0.0f;
+    }
+    /* the PDE in pseudo-time (pderedistancer3.cpp:137-174), on the active cells, in the
+     * float arithmetic of array3's *= (by the float dt) and += */
+    const float dtf = (float)(rate * dx);
+    const float hdtf = (float)(0.5 * (rate * dx));
+    const int iters = (int)ceil((double)width / rate);
+    for (int it = 0; it < iters; ++it) {
+        redist_rate(s, dx, phi, phi0, sgn, m, d0);
+        if (!rk2) {
+            for (int64_t c = 0; c < n; ++c)
+                if (m[c]) {
+                    const float t = d0[c] * dtf;
+                    phi[c] = t + phi[c];
+                }
+        } else {
+            for (int64_t c = 0; c < n; ++c) {
+                tmp[c] = phi[c];
+                if (m[c]) {
+                    const float t = d0[c] * dtf;
+                    tmp[c] = t + phi[c];
+                }
+            }
+            redist_rate(s, dx, tmp, phi0, sgn, m, d1);
+            for (int64_t c = 0; c < n; ++c)
+                if (m[c]) {
+                    const float sum = d0[c] + d1[c];
+                    const float t = sum * hdtf;
+                    phi[c] = phi[c] + t;
+                }
+        }
+    }
+    /* off the band beyond width*dx, then every inactive cell reads the level-set
+     * fill of its side (set_as_levelset, flood_fill: pderedistancer3.cpp:176-182) */
+    const double bw = dx * (double)width;
+    int64_t active = 0;
+    for (int64_t c = 0; c < n; ++c) {
+        if (m[c] && fabs((double)phi[c]) > bw) m[c] = 0;
+        if (!m[c]) phi[c] = phi[c] < 0.0f ? (float)(-bw) : (float)bw;
+        band[c] = m[c];
+        active += m[c];
+    }
+    free(m);
+    free(nm);
+    free(np_);
+    free(phi0);
+    free(sgn);
+    free(d0);
+    free(d1);
+    free(tmp);
+    return active;
 }

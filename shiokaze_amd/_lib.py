@@ -82,6 +82,9 @@ _SIGNATURES = {
     "shk_project2d": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 6 + [ctypes.c_double, ctypes.POINTER(ShkResult)]),
     "shk_set_target_volume": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double]),
     "shk_set_solid_density": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "shk_state_bytes": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_size_t)]),
+    "shk_save_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "shk_load_state": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
     "shk_pressure": (_vp, [_vp]),
     "shk_copy_pressure": (ctypes.c_int, [_vp, _vp]),
     "shk_export_fractions": (ctypes.c_int, [_vp] * 7),
@@ -122,6 +125,7 @@ _SIGNATURES = {
     "shk_add_gravity": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),
     "shk_advect_levelset": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 8 + [ctypes.c_float]),
     "shk_project_activity": (ctypes.c_int, [_vp] * 4),
+    "shk_redistance": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 _lib = None

@@ -420,6 +420,8 @@ void shk_destroy(shk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->unet) unet_destroy(c->unet);
     if (c->mg) mg_destroy(c->mg);
+    if (c->amg) amg_destroy(c->amg);
+    if (c->redist) redist_destroy(c->redist);
     if (c->liquid) liquid_destroy(c->liquid);
     for (auto &b : c->area) b.release();
     for (auto &b : c->rho) b.release();
@@ -495,6 +497,18 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
     } else if (k == "Report") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad Report '%s'", value);
         p.report = b;
+    } else if (k == "IntegrationRate") {
+        if (!parse_double(value, d) || !(d > 0.0) || d > 1.0)
+            return c->fail(SHK_BAD_ARG, "bad IntegrationRate '%s' (0 < rate <= 1)", value);
+        p.redist_rate = d;
+    } else if (k == "RedistTemporalScheme") {
+        const std::string v(value);
+        if (v == "Euler")
+            p.redist_rk2 = false;
+        else if (v == "RK2")
+            p.redist_rk2 = true;
+        else
+            return c->fail(SHK_BAD_ARG, "RedistTemporalScheme must be Euler or RK2, got '%s'", value);
     } else if (k == "MacCormack") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad MacCormack '%s'", value);
         p.maccormack = b;
@@ -503,7 +517,7 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
         p.trim_narrowband = (int)i;
     } else if (k == "WENO") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad WENO '%s'", value);
-        if (b) return c->fail(SHK_BAD_ARG, "WENO interpolation is not built (trilinear only)");
+        p.weno = b;
     } else if (k == "EpsFluid") {
         if (!parse_double(value, d) || d < 0) return c->fail(SHK_BAD_ARG, "bad EpsFluid '%s'", value);
         p.eps_fluid = d;
@@ -575,6 +589,77 @@ int shk_set_target_volume(shk_ctx *c, double current_volume, double target_volum
         return c->fail(SHK_BAD_ARG, "volumes must be finite");
     c->current_volume = current_volume;
     c->target_volume = target_volume;
+    return SHK_OK;
+}
+
+// Projection state in the field order of macpressuresolver3::serialize
+// (macpressuresolver3.cpp:332-344): magic, shape (3 x uint32), dx, target volume,
+// current volume, the PI integral y_prev, then a uint64 cell count and the warm-start
+// pressure per cell (FP64), the count 0 when there is none.
+namespace {
+constexpr char kStateMagic[8] = {'S', 'H', 'K', 'P', 'S', 'T', '0', '1'};
+constexpr size_t kStateHeader = 8 + 3 * 4 + 4 + 4 * 8 + 8;  // magic, shape, pad, 4 doubles, count
+}  // namespace
+
+int shk_state_bytes(const shk_ctx *c, size_t *bytes) {
+    if (!c || !bytes) return SHK_BAD_ARG;
+    const size_t G = (size_t)c->dims.nx * c->dims.ny * c->dims.nz;
+    *bytes = kStateHeader + (c->prev64.p ? G * sizeof(double) : 0);
+    return SHK_OK;
+}
+
+int shk_save_state(shk_ctx *c, void *buf, size_t capacity) {
+    if (!c || !buf) return SHK_BAD_ARG;
+    c->err.clear();
+    size_t need = 0;
+    shk_state_bytes(c, &need);
+    if (capacity < need) return c->fail(SHK_BAD_ARG, "state buffer holds %zu bytes, %zu needed", capacity, need);
+    HIPC(c, hipSetDevice(c->device));
+    unsigned char *p = static_cast<unsigned char *>(buf);
+    std::memcpy(p, kStateMagic, 8);
+    const uint32_t shape[4] = {(uint32_t)c->dims.nx, (uint32_t)c->dims.ny, (uint32_t)c->dims.nz, 0u};
+    std::memcpy(p + 8, shape, sizeof shape);
+    const double v[4] = {c->dx, c->target_volume, c->current_volume, c->y_prev};
+    std::memcpy(p + 24, v, sizeof v);
+    const uint64_t G = c->prev64.p ? (uint64_t)c->dims.nx * c->dims.ny * c->dims.nz : 0;
+    std::memcpy(p + 56, &G, 8);
+    if (G) {
+        HIPC(c, hipMemcpyAsync(p + kStateHeader, c->prev64.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+    }
+    return SHK_OK;
+}
+
+int shk_load_state(shk_ctx *c, const void *buf, size_t bytes) {
+    if (!c || !buf) return SHK_BAD_ARG;
+    c->err.clear();
+    const unsigned char *p = static_cast<const unsigned char *>(buf);
+    if (bytes < kStateHeader || std::memcmp(p, kStateMagic, 8) != 0)
+        return c->fail(SHK_BAD_ARG, "not a projection state (bad magic or size)");
+    uint32_t shape[4];
+    std::memcpy(shape, p + 8, sizeof shape);
+    double v[4];
+    std::memcpy(v, p + 24, sizeof v);
+    uint64_t G = 0;
+    std::memcpy(&G, p + 56, 8);
+    if ((int)shape[0] != c->dims.nx || (int)shape[1] != c->dims.ny || (int)shape[2] != c->dims.nz || v[0] != c->dx)
+        return c->fail(SHK_BAD_ARG, "the state was saved for another shape or dx");
+    const uint64_t cells = (uint64_t)c->dims.nx * c->dims.ny * c->dims.nz;
+    if ((G != 0 && G != cells) || bytes != kStateHeader + G * sizeof(double))
+        return c->fail(SHK_BAD_ARG, "truncated or inconsistent projection state");
+    for (int i = 1; i < 4; ++i)
+        if (!std::isfinite(v[i])) return c->fail(SHK_BAD_ARG, "non-finite volume in the state");
+    HIPC(c, hipSetDevice(c->device));
+    c->target_volume = v[1];
+    c->current_volume = v[2];
+    c->y_prev = v[3];
+    if (G) {
+        HIPC(c, c->prev64.ensure(G));
+        HIPC(c, hipMemcpyAsync(c->prev64.p, p + kStateHeader, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+        HIPC(c, hipStreamSynchronize(c->stream));
+    } else {
+        c->prev64.release();  // the next warm-started projection starts from zero
+    }
     return SHK_OK;
 }
 
@@ -954,8 +1039,6 @@ int solve_csr_device(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int6
     if (c->prm.precond == 2)
         return c->fail(SHK_BAD_ARG, "Preconditioner=unet needs the projection's grid; shk_solve_csr takes none, "
                                     "jacobi or mg");
-    if (c->prm.precond == 3)
-        return c->fail(SHK_BAD_ARG, "Preconditioner=mg is not available for CSR input");
     if (rows == 0) return SHK_OK;
     int rc = ensure_partials(c);
     if (rc) return rc;
@@ -966,9 +1049,13 @@ int solve_csr_device(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int6
     HIPC(c, c->t64.ensure(rows));
     HIPC(c, c->q64.ensure(rows));
     const CsrOp op{rowptr, cols, vals, rows};
-    const bool jac = c->prm.precond != 0;
+    const bool jac = c->prm.precond == 1;
+    // Preconditioner=mg: CG with the aggregation-multigrid V-cycle, the hierarchy built
+    // from this matrix (the reference's amg plugin builds its own per solve too)
+    const bool amg = c->prm.precond == 3;
     const Grid g = make_grid(rows);
     HIPC(c, hipEventRecord(c->ev[0], c->stream));
+    if (amg && (rc = amg_setup(c, rows, rowptr, cols, vals))) return rc;
     launch_csr_diag(c->stream, op, c->diag64.p, c->sig64.p);
     CsrOp sop = op;
     if (jac) {
@@ -983,18 +1070,27 @@ int solve_csr_device(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int6
         sop.vals = c->hx64.p;
     }
     const double *sig = jac ? c->sig64.p : nullptr;
-    const Reduction red = reduction(c);
+    Reduction red = reduction(c);
+    if (amg) red.pz = amg_pz(c->amg);
+    // the vector the next direction is formed from: r, or z = M r
+    const double *dsrc = amg ? amg_z(c->amg) : c->r64.p;
     launch_cg_init<double, double>(c->stream, g, b, sig, x, c->r64.p, c->s64.p, red, c->prm.residual,
                                    c->prm.max_iterations);
     double *S[2] = {c->s64.p, c->t64.p};
     int m = 0;
-    launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[0], S[1], c->q64.p, red);
+    if (amg && (rc = amg_vcycle(c, c->r64.p, g, c->state.p))) return rc;
+    launch_dir_spmv_csr(c->stream, g, sop, dsrc, S[0], S[1], c->q64.p, red);
     ++m;
-    rc = drive(c, [&]() {
-        launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, S[m & 1], c->q64.p, red, c->next_events());
-        launch_dir_spmv_csr(c->stream, g, sop, c->r64.p, S[m & 1], S[(m + 1) & 1], c->q64.p, red, c->next_events());
-        ++m;
-    });
+    rc = drive(
+        c,
+        [&]() {
+            launch_axpy_norm<double>(c->stream, g, sig, x, c->r64.p, S[m & 1], c->q64.p, red, c->next_events());
+            if (amg) (void)amg_vcycle(c, c->r64.p, g, c->state.p);
+            launch_dir_spmv_csr(c->stream, g, sop, dsrc, S[m & 1], S[(m + 1) & 1], c->q64.p, red,
+                                c->next_events());
+            ++m;
+        },
+        amg ? std::min(std::max(1, c->prm.check_interval), 4) : 0);
     if (rc) return rc;
     if (jac) launch_unscale(c->stream, rows, x, c->sig64.p);
     HIPC(c, hipEventRecord(c->ev[1], c->stream));

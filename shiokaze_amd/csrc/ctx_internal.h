@@ -18,6 +18,8 @@ namespace shk {
 struct UnetPrecond;
 struct MgHierarchy;
 struct LiquidWork;
+struct CsrAmg;
+struct RedistWork;
 }
 
 namespace shk {
@@ -68,7 +70,10 @@ struct Params {
     bool warm_start = false;          // macpressuresolver3.cpp:354
     bool report = false;              // RCMatrix_utility::report of each assembled system
     bool maccormack = true;           // macadvection3.cpp:281
+    bool weno = false;                // macadvection3.cpp:286 (WENO interpolation)
     int trim_narrowband = 1;          // macadvection3.cpp:283
+    double redist_rate = 0.75;        // pderedistancer3.cpp:206 (IntegrationRate)
+    bool redist_rk2 = false;          // pderedistancer3.cpp:207 (RedistTemporalScheme Euler)
 };
 
 }  // namespace shk
@@ -141,6 +146,10 @@ struct shk_ctx {
     shk::UnetPrecond *unet = nullptr;
     // multigrid preconditioner hierarchy
     shk::MgHierarchy *mg = nullptr;
+    // aggregation multigrid of the last shk_solve_csr with Preconditioner=mg
+    shk::CsrAmg *amg = nullptr;
+    // work arrays of the PDE redistancer (redist.hip)
+    shk::RedistWork *redist = nullptr;
     // scratch of the liquid-step kernels (liquid.hip)
     shk::LiquidWork *liquid = nullptr;
     // the last projection ran its velocity update (it had unknowns)
@@ -211,6 +220,14 @@ size_t scratch_capacity_bytes();
 // learned preconditioner (unet.hip)
 void unet_destroy(UnetPrecond *u);
 void mg_destroy(MgHierarchy *h);
+// aggregation multigrid for CSR input (amg.hip): setup from the device CSR arrays,
+// then z = M r per V-cycle into c->amg (z, and the r.z partials of the CG grid g)
+void amg_destroy(CsrAmg *h);
+void redist_destroy(RedistWork *w);
+int amg_setup(shk_ctx *c, int64_t rows, const int64_t *rowptr, const int64_t *cols, const double *vals);
+int amg_vcycle(shk_ctx *c, const double *r, const Grid &g, const SolveState *state);
+const double *amg_z(const CsrAmg *h);
+double *amg_pz(CsrAmg *h);
 void liquid_destroy(LiquidWork *w);
 int solve_mg(shk_ctx *c, shk_result *res);
 int unet_solve(shk_ctx *c, shk_result *res);

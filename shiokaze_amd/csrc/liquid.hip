@@ -186,6 +186,68 @@ __device__ inline float interp(const float *a, const uint8_t *m, FS s, const dou
     return value;
 }
 
+// The 6th-order WENO interpolation at x in [0, 1] of samples f at -2, -1, 0, 1, 2, 3
+// (Macdonald and Ruuth, J. Sci. Comput. 35 (2008) 219-240; the reference's WENO::interp6,
+// include/shiokaze/math/WENO.h), each operation in the order written there.
+__device__ inline double weno6(double x, const double f[6]) {
+#pragma clang fp contract(off)
+    const double fm2 = f[0], fm1 = f[1], f0 = f[2], f1 = f[3], f2 = f[4], f3 = f[5];
+    const double c[3] = {(2.0 - x) * (3.0 - x) / 20.0, (3.0 - x) * (x - -2.0) / 10.0,
+                         (x - -2.0) * (x - -1.0) / 20.0};
+    const double s[3] = {
+        ((814. * (f1 * f1)) + (4326. * (f0 * f0)) + (2976. * (fm1 * fm1)) + (244. * (fm2 * fm2)) - (3579. * f0 * f1) -
+         (6927. * f0 * fm1) + (1854. * f0 * fm2) + (2634. * f1 * fm1) - (683. * f1 * fm2) - (1659. * fm1 * fm2)) /
+            180.0,
+        ((1986. * (f1 * f1)) + (1986. * (f0 * f0)) + (244. * (fm1 * fm1)) + (244. * (f2 * f2)) + (1074. * f0 * f2) -
+         (3777. * f0 * f1) - (1269. * f0 * fm1) + (1074. * f1 * fm1) - (1269. * f2 * f1) - (293. * f2 * fm1)) /
+            180.0,
+        ((814. * (f0 * f0)) + (4326. * (f1 * f1)) + (2976. * (f2 * f2)) + (244. * (f3 * f3)) - (683. * f0 * f3) +
+         (2634. * f0 * f2) - (3579. * f0 * f1) - (6927. * f1 * f2) + (1854. * f1 * f3) - (1659. * f2 * f3)) /
+            180.0};
+    const double p[3] = {
+        fm2 + (fm1 - fm2) * (x - -2.0) + (f0 - 2. * fm1 + fm2) * (x - -2.0) * (x - -1.0) / 2.0 +
+            (f1 - 3. * f0 + 3. * fm1 - fm2) * (x - -2.0) * (x - -1.0) * (x - 0.0) / 6.0,
+        fm1 + (f0 - fm1) * (x - -1.0) + (f1 - 2. * f0 + fm1) * (x - -1.0) * (x - 0.0) / 2.0 +
+            (f2 - 3. * f1 + 3. * f0 - fm1) * (x - -1.0) * (x - 0.0) * (x - 1.0) / 6.0,
+        f0 + (f1 - f0) * (x - 0.0) + (f2 - 2. * f1 + f0) * (x - 0.0) * (x - 1.0) / 2.0 +
+            (f3 - 3. * f2 + 3. * f1 - f0) * (x - 0.0) * (x - 1.0) * (x - 2.0) / 6.0};
+    double a[3], sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        a[q] = c[q] / (DBL_EPSILON + s[q] * s[q]);
+        sum += a[q];
+    }
+    return a[0] / sum * p[0] + a[1] / sum * p[1] + a[2] / sum * p[2];
+}
+
+// WENO3::interpolate(array, p), order 6 (WENO3.h:92-122): the position clamped into the
+// array, then interp6 along x for each of the 6 x 6 rows around it, along y, along z,
+// every read clamped to the array (m == nullptr reads every value as stored)
+__device__ inline double weno3(const float *a, const uint8_t *m, FS s, const double p[3]) {
+#pragma clang fp contract(off)
+    const double x = fmax(0.0, fmin(s.nx - 1., p[0]));
+    const double y = fmax(0.0, fmin(s.ny - 1., p[1]));
+    const double z = fmax(0.0, fmin(s.nz - 1., p[2]));
+    const int i = (int)fmin(x, s.nx - 2.), j = (int)fmin(y, s.ny - 2.), k = (int)fmin(z, s.nz - 2.);
+    double vvv[6];
+    for (int kk = 0; kk < 6; ++kk) {
+        const int kc = min(max(k + kk - 2, 0), s.nz - 1);
+        double vv[6];
+        for (int jj = 0; jj < 6; ++jj) {
+            const int jc = min(max(j + jj - 2, 0), s.ny - 1);
+            double v[6];
+#pragma unroll
+            for (int ii = 0; ii < 6; ++ii) {
+                const int64_t q = s.at(min(max(i + ii - 2, 0), s.nx - 1), jc, kc);
+                v[ii] = m ? (double)rd(a, m, q) : (double)a[q];
+            }
+            vv[jj] = weno6(x - i, v);
+        }
+        vvv[kk] = weno6(y - j, vv);
+    }
+    return weno6(z - k, vvv);
+}
+
 // macarray3::convert_to_full for one face (macarray3.h:372-393): its own component
 // and the clamped 4-face averages of the other two, each stored as float
 __device__ inline void full_velocity(const Dims &d, const CFaces &f, int dim, int i, int j, int k, float out[3]) {
@@ -466,7 +528,7 @@ __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, Split sp, doubl
                                                          float *lo, float *hi, uint8_t *band, const float *fluid,
                                                          int trim, const float *fin_vin, const float *fin_lo,
                                                          const float *fin_hi, const uint8_t *fin_band,
-                                                         const float *fin_v0, float *fin_out) {
+                                                         const float *fin_v0, float *fin_out, int weno) {
 #pragma clang fp contract(off)
     const FS s = face_fs(d, DIM);
     const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
@@ -503,7 +565,9 @@ __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, Split sp, doubl
             vals[n] = rd(src.v[DIM], src.m[DIM], corner(s, ci, cj, ck, n));
             if (coef[n] != 0.0) acc = (float)((double)acc + (double)vals[n] * coef[n]);
         }
-        value = acc;
+        // WENO=1: the value by WENO3 interpolation (macadvection3.cpp:85-86); the corners
+        // above still bound the MacCormack limiter
+        value = weno ? (float)weno3(src.v[DIM], src.m[DIM], s, p) : acc;
     }
     if (out) out[q] = value;
     if (lo) {
@@ -595,7 +659,7 @@ __global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, Split sp, double dx
                                                      float *hi, uint8_t *band, const float *fin_qin,
                                                      const float *fin_lo, const float *fin_hi,
                                                      const uint8_t *fin_band, const float *fin_q0, float *fin_out,
-                                                     int fill_inactive, float background) {
+                                                     int fill_inactive, float background, int weno) {
 #pragma clang fp contract(off)
     const FS cs{d.nx, d.ny, d.nz};
     const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
@@ -625,7 +689,7 @@ __global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, Split sp, double dx
             vals[n] = q[corner(cs, ci, cj, ck, n)];
             if (coef[n] != 0.0) acc = (float)((double)acc + (double)vals[n] * coef[n]);
         }
-        value = acc;
+        value = weno ? (float)weno3(q, nullptr, cs, p) : acc;  // WENO=1 (macadvection3.cpp:193-194)
     }
     if (out) out[c] = value;
     if (lo) {
@@ -886,9 +950,11 @@ int shk_advect_velocity(shk_ctx *c, double dt, float *u, float *v, float *w, con
         v0.m[a] = um[a];
     }
     const int trim = c->prm.trim_narrowband;
+    const int weno = c->prm.weno ? 1 : 0;
 #define SHK_SL(DIM, SRC, OUT, LO, HI, BAND, DT, FV, FL, FH, FB, F0, FO)                                            \
     hipLaunchKernelGGL(k_semi_lagrangian<DIM>, dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, c->stream, d, \
-                       split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO)
+                       split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO, \
+                       weno)
     // forward pass: v0 (the result itself without MacCormack) and the limiter data
     SHK_SL(0, vin, const_cast<float *>(v0.v[0]), mc ? lw->lo[0].p : nullptr, mc ? lw->hi[0].p : nullptr,
            mc ? lw->band[0].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -935,11 +1001,12 @@ int shk_advect_levelset(shk_ctx *c, double dt, float *phi, const uint8_t *mphi, 
     hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, dt, qin, mphi, vel, qin,
                        c->prm.trim_narrowband, mc ? lw->b[0].p : phi, mc ? lw->lo[0].p : nullptr,
                        mc ? lw->hi[0].p : nullptr, mc ? lw->band[0].p : nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, mc ? 1 : 0, background);
+                       nullptr, nullptr, mc ? 1 : 0, background, c->prm.weno ? 1 : 0);
     if (mc)
         hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, -dt, lw->b[0].p,
                            mphi, vel, qin, c->prm.trim_narrowband, nullptr, nullptr, nullptr, nullptr, qin,
-                           lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background);
+                           lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background,
+                           c->prm.weno ? 1 : 0);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
     return SHK_OK;

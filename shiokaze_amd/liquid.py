@@ -107,6 +107,17 @@ class MacLiquidVelocity:
                                                  _ptr(band, self.solver.cells, "band", torch.uint8), *p, *m,
                                                  float(background)))
 
+    def redistance(self, phi, band, width: int = 3) -> int:
+        """redistancer3_interface::redistance(phi, width) (the pderedistancer3 plugin) on
+        the level set `phi` and its narrow band `band` (uint8 cell mask), in place;
+        returns the final band's cell count.  Cells off the band are set to -/+ width*dx
+        by the sign of their value."""
+        active = ctypes.c_int64()
+        self._check(self.lib.shk_redistance(self.solver.ctx, _ptr(phi, self.solver.cells, "phi"),
+                                            _ptr(band, self.solver.cells, "band", torch.uint8), int(width),
+                                            ctypes.byref(active)))
+        return int(active.value)
+
     def add_gravity(self, dt: float, velocity, masks) -> None:
         """`m_velocity += dt*gravity` (activates every face)."""
         p, m = self._faces(velocity, masks)

@@ -91,6 +91,19 @@ class MacPressureSolver3:
         of the following projections (active with Gain != 0 and a nonzero target)."""
         _lib.check(self.lib.shk_set_target_volume(self.ctx, float(current_volume), float(target_volume)), self.ctx)
 
+    def save_state(self) -> bytes:
+        """macpressuresolver3::serialize (macpressuresolver3.cpp:332-344): shape, dx,
+        volumes, the PI integral and the warm-start pressure, as bytes."""
+        n = ctypes.c_size_t()
+        _lib.check(self.lib.shk_state_bytes(self.ctx, ctypes.byref(n)), self.ctx)
+        buf = ctypes.create_string_buffer(n.value)
+        _lib.check(self.lib.shk_save_state(self.ctx, buf, n.value), self.ctx)
+        return buf.raw
+
+    def load_state(self, state: bytes) -> None:
+        """macpressuresolver3::initialize(const filestream &) (macpressuresolver3.cpp:319-330)."""
+        _lib.check(self.lib.shk_load_state(self.ctx, state, len(state)), self.ctx)
+
     def set_solid_density(self, density) -> None:
         """macproject3_interface::set_solid_density: (du, dv, dw) face tensors of the
         u, v, w shapes, or None to switch it off.  The tensors must stay alive while set."""

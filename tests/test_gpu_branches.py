@@ -289,3 +289,51 @@ def test_warm_start(scenes, oracle, precond, precision):
     # change's own rhs: its pressure meets the bar
     assert rel <= 1e-5
     assert w2["result"]["status"] == 0
+
+
+# ---------------------------------------------------------------- serialized state
+
+def test_saved_state_resumes_bit_for_bit(scenes):
+    """macpressuresolver3::serialize / initialize(filestream) (macpressuresolver3.cpp:
+    319-344): a context that loads the state another context saved after a warm-started,
+    volume-corrected projection continues exactly as that context does (same volume
+    correction constant, iterations, pressure and velocity)."""
+    import torch
+
+    import shiokaze_amd
+    from gpu_helpers import to_device
+    sc1, sc2 = scenes(3, 24, 4), scenes(3, 24, 5)
+    cfg = {"Residual": 1e-6, "WarmStart": 1, "Gain": 1.0}
+
+    def run(ps, sc):
+        d = to_device(sc)
+        res = ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+        torch.cuda.synchronize()
+        return res, ps.get_pressure().cpu().numpy(), [d[k].cpu().numpy() for k in ("u", "v", "w")]
+
+    with shiokaze_amd.MacPressureSolver3(sc1["shape"], sc1["dx"], config=cfg) as a, \
+            shiokaze_amd.MacPressureSolver3(sc1["shape"], sc1["dx"], config=cfg) as b:
+        a.set_target_volume(0.95, 1.0)
+        run(a, sc1)
+        state = a.save_state()
+        b.load_state(state)
+        assert b.save_state() == state
+        a.set_target_volume(0.97, 1.0)
+        b.set_target_volume(0.97, 1.0)
+        ra, pa, va = run(a, sc2)
+        rb, pb, vb = run(b, sc2)
+        log(f"[gpu] resumed state ({len(state)} bytes): rhs constant {rb['volume_correct_rhs']:.6g}, "
+            f"{rb['count']} its (uninterrupted {ra['count']})")
+        assert ra["volume_correct_rhs"] == rb["volume_correct_rhs"] != 0.0
+        assert ra["count"] == rb["count"]
+        np.testing.assert_array_equal(pa, pb)
+        for x, y in zip(va, vb):
+            np.testing.assert_array_equal(x, y)
+        # a state of another shape, or a damaged one, is refused
+        with shiokaze_amd.MacPressureSolver3((16, 24, 24), sc1["dx"], config=cfg) as c:
+            with pytest.raises(shiokaze_amd.ShkError):
+                c.load_state(state)
+        with pytest.raises(shiokaze_amd.ShkError):
+            b.load_state(state[:-8])
+        with pytest.raises(shiokaze_amd.ShkError):
+            b.load_state(b"XXXXXXXX" + state[8:])

@@ -124,10 +124,10 @@ def test_csr_solve_parity(scenes, oracle, precond):
     assert relb <= PRESSURE_BAR
 
 
-@pytest.mark.parametrize("precond", ["unet", "mg"])
+@pytest.mark.parametrize("precond", ["unet"])
 def test_csr_solve_refuses_grid_preconditioners(scenes, oracle, precond):
-    """The learned and the brick multigrid preconditioners need the projection's grid;
-    shk_solve_csr refuses them with SHK_BAD_ARG instead of running another method."""
+    """The learned preconditioner needs the projection's grid; shk_solve_csr refuses it
+    with SHK_BAD_ARG instead of running another method."""
     import torch
 
     import shiokaze_amd

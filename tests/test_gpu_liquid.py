@@ -305,7 +305,7 @@ def test_liquid_calls_refuse_2d_and_host_pointers():
     ps = MacPressureSolver3((8, 8, 8), 1.0 / 8, device=0)
     with ps:
         with pytest.raises(ShkError):
-            ps.configure("WENO", True)
+            ps.configure("WENO", "maybe")
 
 
 def test_liquid_edge_cases(scenes, oracle):
@@ -422,3 +422,88 @@ def test_unaligned_arrays_match_oracle(scenes, oracle, shape):
         elapsed = time.perf_counter() - t0
     log(f"[gpu] unaligned views {shape}: gravity, normalisation and activity bit-exact in {elapsed:.2f} s")
     assert elapsed < 30.0
+
+
+# ---------------------------------------------------------------- PDE redistancing
+
+def _distorted_levelset(n, seed):
+    """A sphere level set with its distance distorted (slopes 0.5..2 by region), its
+    narrow band at 3 cells, and the background values outside it (-/+ 3 dx)."""
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    c = rng.uniform(0.35, 0.65, 3)
+    d = np.sqrt((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) - rng.uniform(0.15, 0.3)
+    d = np.minimum(d, Y - rng.uniform(0.2, 0.35))
+    phi = (d * (1.25 + 0.75 * np.sin(7 * X + 3 * Z))).astype(np.float32)
+    band = (np.abs(phi) < 3 * dx).astype(np.uint8)
+    phi = np.where(band > 0, phi, np.where(phi < 0, -3 * dx, 3 * dx)).astype(np.float32)
+    return dx, phi, band
+
+
+@pytest.mark.parametrize("n,seed,width", [(24, 0, 3), (40, 1, 3), (33, 2, 5), (17, 3, 1)])
+@pytest.mark.parametrize("scheme", ["Euler", "RK2"])
+def test_redistance_matches_oracle(oracle, n, seed, width, scheme):
+    """shk_redistance against the oracle's pderedistancer3 restatement: band, values
+    and count bit for bit."""
+    dx, phi, band = _distorted_levelset(n, seed)
+    ps, lv = _solver((n, n, n), dx, {"RedistTemporalScheme": scheme})
+    dphi, dband = _t(phi), _t(band)
+    cnt = lv.redistance(dphi, dband, width)
+    ps.close()
+    ophi, oband, ocnt = oracle.redistance(dx, phi, band, width, rk2=scheme == "RK2")
+    log(f"[gpu] redistance n={n} seed={seed} width={width} {scheme}: band {cnt} (oracle {ocnt})")
+    assert cnt == ocnt
+    np.testing.assert_array_equal(_h(dband), oband)
+    np.testing.assert_array_equal(_h(dphi), ophi)
+
+
+def test_redistance_rate_and_refusals(oracle):
+    import shiokaze_amd
+    dx, phi, band = _distorted_levelset(20, 4)
+    ps, lv = _solver((20, 20, 20), dx, {"IntegrationRate": 0.5})
+    dphi, dband = _t(phi), _t(band)
+    lv.redistance(dphi, dband, 2)
+    ophi, oband, _ = oracle.redistance(dx, phi, band, 2, rate=0.5)
+    np.testing.assert_array_equal(_h(dphi), ophi)
+    for key, val in (("IntegrationRate", "0"), ("IntegrationRate", "1.5"), ("RedistTemporalScheme", "RK4")):
+        with pytest.raises(shiokaze_amd.ShkError):
+            ps.configure(key, val)
+    with pytest.raises(shiokaze_amd.ShkError):
+        lv.redistance(dphi, dband, 0)
+    ps.close()
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES[:2])
+@pytest.mark.parametrize("maccormack", [True, False])
+def test_advect_weno_matches_oracle(scenes, oracle, kind, n, seed, maccormack):
+    """WENO=1: the semi-Lagrangian values of the velocity and of the level set by 6th-order
+    WENO interpolation (macadvection3.cpp:85-86, 193-194), bit for bit with the oracle."""
+    sc = _scene(scenes, kind, n, seed)
+    O = oracle
+    shape = tuple(sc["fluid"].shape[::-1])
+    dx = sc["dx"]
+    rng = np.random.default_rng(seed + 7)
+    masks = _active_masks(sc, O)
+    _, masks, _ = O.extrapolate(_velocity([m.shape for m in masks], rng), masks, 3)
+    vel = _velocity([m.shape for m in masks], rng, 2.0)
+    dt = 1.5 * dx / 2.0
+    want = O.advect_velocity(dx, dt, vel, masks, sc["fluid"], maccormack=maccormack, weno=True)
+    bw = 3 * dx
+    phi = np.clip(sc["fluid"], -bw, bw).astype(np.float32)
+    band = (np.abs(sc["fluid"]) < bw).astype(np.uint8)
+    want_phi = O.advect_levelset(dx, dt, phi, band, vel, masks, bw, maccormack=maccormack, weno=True)
+    plain = O.advect_velocity(dx, dt, vel, masks, sc["fluid"], maccormack=maccormack)
+    ps, lv = _solver(shape, dx, {"MacCormack": maccormack, "WENO": 1})
+    with ps:
+        dv = [_t(v) for v in vel]
+        lv.advect(dt, dv, [_t(m) for m in masks], _t(sc["fluid"]))
+        for d in range(3):
+            np.testing.assert_array_equal(_h(dv[d]), want[d])
+        dphi = _t(phi)
+        lv.advect_levelset(dt, dphi, _t(band), [_t(v) for v in vel], [_t(m) for m in masks], bw)
+        np.testing.assert_array_equal(_h(dphi), want_phi)
+    moved = sum(int((w != p).sum()) for w, p in zip(want, plain))
+    log(f"[gpu] advect WENO {kind} {shape} maccormack={maccormack}: bit-exact ({moved} faces differ from trilinear)")
+    assert moved > 0
