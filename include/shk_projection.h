@@ -340,6 +340,13 @@ int shk_unet_precondition(shk_ctx *ctx, const float *r, float *z);
  * y = act(conv3x3x3(x, w) + bias) with zero padding (act = ReLU when relu). */
 int shk_nn_conv3d(int nx, int ny, int nz, const void *x, int cin, const float *w, const float *bias, int cout,
                   int relu, void *y);
+/* The same convolution on the input [upsample(coarse), x] (coarse: the half-size grid
+ * with c1 channels, nearest upsampling; x: cin - c1 channels; coarse = NULL and c1 = 0:
+ * x alone with cin channels), as the decoder layers read it, by kernel 0 (the choice the
+ * network's inference makes for this grid), 1 (the tiled kernel the training uses) or
+ * 2 (the row-reuse kernel; cout 16 or 32). */
+int shk_nn_conv3d_cat(int nx, int ny, int nz, const void *coarse, int c1, const void *x, int cin, const float *w,
+                      const float *bias, int cout, int relu, int kernel, void *y);
 /* dx = input gradient (BF16, NULL to skip), dw / db += weight / bias gradients (FP32). */
 int shk_nn_conv3d_backward(int nx, int ny, int nz, const void *x, int cin, const float *w, int cout, const void *dz,
                            void *dx, float *dw, float *db);

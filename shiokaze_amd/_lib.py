@@ -116,10 +116,13 @@ _SIGNATURES = {
     "shk_unet_precondition": (ctypes.c_int, [_vp, _vp, _vp]),
     "shk_nn_conv3d": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_int,
                                                           _vp]),
+    "shk_nn_conv3d_cat": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp,
+                                                              ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     "shk_nn_conv3d_backward": (ctypes.c_int, [ctypes.c_int] * 3 + [_vp, ctypes.c_int, _vp, ctypes.c_int] + [_vp] * 4),
     # include/shk_liquid.h
     "shk_extrapolate_velocity": (ctypes.c_int, [_vp] * 7 + [ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]),
     "shk_constrain_velocity": (ctypes.c_int, [_vp] * 8),
+    "shk_extrapolate_levelset": (ctypes.c_int, [_vp] * 5 + [ctypes.c_float, ctypes.POINTER(ctypes.c_int64)]),
     "shk_compute_max_u": (ctypes.c_int, [_vp] * 7 + [ctypes.POINTER(ctypes.c_double)]),
     "shk_advect_velocity": (ctypes.c_int, [_vp, ctypes.c_double] + [_vp] * 7),
     "shk_add_gravity": (ctypes.c_int, [_vp] + [ctypes.c_double] * 4 + [_vp] * 6),

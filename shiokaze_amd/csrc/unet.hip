@@ -180,6 +180,7 @@ struct UnetNet {
     int64_t step = 0;
     double omega = 0.8;  // smoothing weight the network was trained with (saved with it)
     bf16 *pk[kLayers] = {}, *pkt[kLayers] = {};
+    bf16 *pkr[kLayers] = {};      // row-reuse packing of the 16/32-output layers (inference)
     uint32_t *head_pk = nullptr;  // the head's output-channel-0 weights as BF16 pairs (inference)
     float *split_ws = nullptr;    // K-split partial sums of the small-grid convolutions (inference)
     int64_t split_cap = 0;
@@ -222,6 +223,7 @@ int repack(UnetNet *n) {
         launch_conv_pack(n->stream, n->param + n->woff[l], kSpec[l].cin, kSpec[l].cout, false, n->pk[l]);
         if (kSpec[l].need_dgrad)
             launch_conv_pack(n->stream, n->param + n->woff[l], kSpec[l].cin, kSpec[l].cout, true, n->pkt[l]);
+        if (n->pkr[l]) launch_conv_pack_rows(n->stream, n->param + n->woff[l], kSpec[l].cin, kSpec[l].cout, n->pkr[l]);
     }
     launch_head_pack(n->stream, n->param + n->woff[kLayers - 1], kSpec[kLayers - 1].cout, n->head_pk);
     UHIP(n, hipGetLastError());
@@ -307,9 +309,15 @@ void forward(UnetNet *n, bool train) {
     // unsplit kernels its gradient checks were made with)
     float *ws = train ? nullptr : n->split_ws;
     const int64_t wc = train ? 0 : n->split_cap;
+    // inference runs the row-reuse kernel where a layer's grid fills the chip
+    auto rows = [&](int l, Grid3 g) { return !train && n->pkr[l] && conv_rows_ok(g, kSpec[l].cin, kSpec[l].cout); };
     auto enc = [&](int l, Grid3 g, const bf16 *x, bf16 *y) {
-        launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout, kSpec[l].relu,
-                        y, nullptr, ws, wc);
+        if (rows(l, g))
+            launch_conv_rows(n->stream, g, x, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
+                             kSpec[l].relu, y);
+        else
+            launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
+                            kSpec[l].relu, y, nullptr, ws, wc);
     };
     enc(0, L[0], n->feat, n->enc0);
     launch_pool(n->stream, L[1], 16, n->enc0, n->p1);
@@ -320,8 +328,12 @@ void forward(UnetNet *n, bool train) {
     enc(3, L[3], n->p3, n->bott);
     if (!train) {
         auto cat = [&](int l, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, bf16 *y) {
-            launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap);
+            if (rows(l, g))
+                launch_conv_rows(n->stream, g, skip, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
+                                 kSpec[l].relu, y, coarse, c1);
+            else
+                launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
+                                    kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap);
         };
         cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
         cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);
@@ -416,6 +428,7 @@ void unet_net_destroy(UnetNet *n) {
     for (int l = 0; l < kLayers; ++l) {
         dfree(n->pk[l]);
         dfree(n->pkt[l]);
+        dfree(n->pkr[l]);
     }
     dfree(n->head_pk);
     dfree(n->split_ws);
@@ -458,6 +471,9 @@ int unet_net_create(uint64_t seed, hipStream_t stream, UnetNet **out) {
         if ((e = dalloc(n->pk[l], (size_t)conv_packed_elems(kSpec[l].cin, kSpec[l].cout))) != hipSuccess) return bail(e);
         if (kSpec[l].need_dgrad &&
             (e = dalloc(n->pkt[l], (size_t)conv_packed_elems(kSpec[l].cout, kSpec[l].cin))) != hipSuccess)
+            return bail(e);
+        if (l < kLayers - 1 && (kSpec[l].cout == 16 || kSpec[l].cout == 32) &&
+            (e = dalloc(n->pkr[l], (size_t)conv_rows_packed_elems(kSpec[l].cin, kSpec[l].cout))) != hipSuccess)
             return bail(e);
     }
     if ((e = dalloc(n->head_pk, 27 * 8)) != hipSuccess) return bail(e);
@@ -1053,6 +1069,39 @@ int shk_nn_conv3d(int nx, int ny, int nz, const void *x, int cin, const float *w
     launch_conv_pack(nullptr, w, cin, cout, false, pk);
     launch_conv_fwd(nullptr, Grid3{nx, ny, nz}, static_cast<const bf16 *>(x), cin, pk, bias, cout, relu != 0,
                     static_cast<bf16 *>(y), nullptr);
+    const hipError_t e1 = hipGetLastError();
+    const hipError_t e2 = hipDeviceSynchronize();
+    (void)hipFree(pk);
+    return (e1 == hipSuccess && e2 == hipSuccess) ? SHK_OK : SHK_HIP_ERROR;
+}
+
+int shk_nn_conv3d_cat(int nx, int ny, int nz, const void *coarse, int c1, const void *x, int cin, const float *w,
+                      const float *bias, int cout, int relu, int kernel, void *y) {
+    const int mt = cout / 16;
+    if (!x || !w || !y || nx <= 0 || ny <= 0 || nz <= 0 || cin <= 0 || cin % 8 || cout <= 0 || cout % 16 ||
+        kernel < 0 || kernel > 2 || (coarse && (c1 <= 0 || c1 % 8 || c1 >= cin || nx % 2 || ny % 2 || nz % 2)) ||
+        (!coarse && c1 != 0))
+        return SHK_BAD_ARG;
+    const Grid3 g{nx, ny, nz};
+    const bool rows = kernel == 2 || (kernel == 0 && conv_rows_ok(g, cin, cout));
+    if (rows && cout != 16 && cout != 32) return SHK_BAD_ARG;
+    if (!rows && mt != 1 && mt != 2 && mt != 3 && mt != 4 && mt != 6 && mt != 8 && mt != 12) return SHK_BAD_ARG;
+    const int64_t elems = rows ? conv_rows_packed_elems(cin, cout) : conv_packed_elems(cin, cout);
+    bf16 *pk = nullptr;
+    if (hipMalloc((void **)&pk, sizeof(bf16) * elems) != hipSuccess) return SHK_OUT_OF_MEMORY;
+    const bf16 *xc = static_cast<const bf16 *>(coarse);
+    const bf16 *xx = static_cast<const bf16 *>(x);
+    bf16 *yy = static_cast<bf16 *>(y);
+    if (rows) {
+        launch_conv_pack_rows(nullptr, w, cin, cout, pk);
+        launch_conv_rows(nullptr, g, xx, cin, pk, bias, cout, relu != 0, yy, xc, c1);
+    } else {
+        launch_conv_pack(nullptr, w, cin, cout, false, pk);
+        if (xc)
+            launch_conv_fwd_cat(nullptr, g, xc, c1, xx, cin, pk, bias, cout, relu != 0, yy);
+        else
+            launch_conv_fwd(nullptr, g, xx, cin, pk, bias, cout, relu != 0, yy, nullptr);
+    }
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipDeviceSynchronize();
     (void)hipFree(pk);

@@ -57,6 +57,34 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
                          const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws = nullptr,
                          int64_t ws_floats = 0);
 
+// Row-reuse forward convolution for grids that fill the chip (inference).  The
+// reduction runs by 32-channel stages of cin (the last one 8, 16 or 24 wide: G groups
+// of 8 channels); within a stage, for each (dz, dy), the 3G entries (dx, group) in the
+// order e = dx*G + group are cut into chunks of four (one per 16-lane quarter of the
+// MFMA's K = 32, the last chunk padded with zero entries).  All entries of a chunk share
+// (dz, dy), so one B fragment (a 16-voxel row) feeds the up-to-9 output rows it is a
+// (dz, dy) neighbour of.
+inline int conv_rows_chunks(int cin) {
+    int q = 0;
+    for (int ch0 = 0; ch0 < cin; ch0 += 32) {
+        const int g = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
+        q += (3 * g + 3) / 4;
+    }
+    return q;
+}
+// Packed A fragments: [chunk][dz*3 + dy][16-output tile][lane][8] BF16.
+inline int64_t conv_rows_packed_elems(int cin, int cout) {
+    return (int64_t)conv_rows_chunks(cin) * 9 * (cout / 16) * 64 * 8;
+}
+void launch_conv_pack_rows(hipStream_t st, const float *w, int cin, int cout, bf16 *packed);
+// Whether launch_conv_rows takes a layer: 1 or 2 output tiles of 16 channels, cin a
+// multiple of 8, and at least 512 tiles of 16 x 8 x 4 voxels.
+bool conv_rows_ok(Grid3 g, int cin, int cout);
+// y = act(conv(x) + bias) as launch_conv_fwd (coarse != null: the input is
+// [upsample(coarse) (c1 channels), x (cin - c1)] as in launch_conv_fwd_cat).
+void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
+                      int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0);
+
 // y[v] = bias[0] + sum_{tap, ci} W[tap][ci][0] x[v + tap][ci] for a 16-channel BF16 input
 // (the network's head, of which only output channel 0 is used); wpk from
 // launch_head_pack (27 * 8 BF16 pairs)

@@ -221,6 +221,229 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     }
 }
 
+// ---------------------------------------------------------------- row-reuse forward
+
+// i / g for the 1..4 channel groups of a stage, without an integer division
+__device__ inline int div_groups(int i, int g) {
+    return g == 4 ? i >> 2 : g == 2 ? i >> 1 : g == 1 ? i : (int)(((uint32_t)i * 0xAAABu) >> 17);
+}
+
+// Block: 256 threads (4 waves, 2 in y by 2 in z) computing a 16 x 2RY x 2RZ voxel tile
+// for all MT output tiles of 16 channels; wave (wy, wz) owns RY x RZ rows of 16 voxels
+// (one MFMA N-tile each).  A stage's input halo (18 x (2RY+2) x (2RZ+2) voxels, G groups
+// of 8 channels) sits in LDS group-major, lds[group * HP + voxel], so the 16 lanes of a
+// quarter read 16 consecutive voxels of one group (no bank conflicts; HP = 4 mod 16
+// spreads the groups of one voxel over the banks for the staging writes).  For each
+// chunk (four (dx, group) entries sharing (dz, dy)) the wave reads each of its
+// (RY+2) x (RZ+2) input rows once and feeds it to the output rows it neighbours: 9 RY RZ
+// MT MFMAs from (RY+2)(RZ+2) LDS reads.  A fragments (the chunk's 9 (dz, dy) taps x MT)
+// come from global memory (L2: every block reads the same weights).  Tiles go to blocks
+// XCD by XCD, so
+// neighbouring tiles (which share halo rows) run on one L2.
+template <int MT, int RY, int RZ, bool AHEAD, bool PREFA>
+__global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *__restrict__ x, int cin,
+                                                      const uint4 *__restrict__ packed, int chunks,
+                                                      const float *__restrict__ bias, int relu,
+                                                      uint16_t *__restrict__ y, const uint16_t *__restrict__ xc,
+                                                      int c1, int tx, int ty) {
+    constexpr int TY = 2 * RY, TZ = 2 * RZ;
+    constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2;
+    constexpr int HV = HX * HY * HZ;
+    constexpr int HP = (HV + 11) / 16 * 16 + 4;  // >= HV, = 4 (mod 16)
+    __shared__ uint4 lds[4 * HP];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int wy = w & 1, wz = w >> 1;
+    const int t = xcd_slot((int)blockIdx.x, (int)gridDim.x);
+    const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
+    f32x4 acc[RZ][RY][MT];
+#pragma unroll
+    for (int oz = 0; oz < RZ; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < RY; ++oy)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) acc[oz][oy][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // With MT == 1 the registers allow the next stage's halo loads to be issued before
+    // this stage's MFMAs, to land while they run; with MT == 2 they are issued at the
+    // stage's start.
+    constexpr bool kAhead = AHEAD;
+    Frag a[9][MT], an[9][MT];
+    if (PREFA) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) an[k][mt].u = packed[(k * MT + mt) * 64 + lane];
+    }
+    // a stage's halo: up to NLD 16-byte loads per thread, all issued before any of them
+    // is written to LDS
+    constexpr int NLD = (HV * 4 + 255) / 256;
+    uint4 st[NLD];
+    auto fetch = [&](int c0) {
+        const int G = (cin - c0) / 8 < 4 ? (cin - c0) / 8 : 4;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            st[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (idx >= HV * G) continue;
+            const int vox = div_groups(idx, G), grp = idx - vox * G;
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
+                const int ch = c0 + grp * 8;
+                const int64_t vi = ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                const uint16_t *src;
+                if (!xc) {
+                    src = x + vi * cin + ch;
+                } else if (ch < c1) {
+                    const int64_t pv = ((int64_t)(gz >> 1) * (g.ny >> 1) + (gy >> 1)) * (g.nx >> 1) + (gx >> 1);
+                    src = xc + pv * c1 + ch;
+                } else {
+                    src = x + vi * (cin - c1) + (ch - c1);
+                }
+                st[i] = *reinterpret_cast<const uint4 *>(src);
+            }
+        }
+    };
+    auto put = [&](int G) {
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int idx = threadIdx.x + 256 * i;
+            if (idx < HV * G) {
+                const int vox = div_groups(idx, G), grp = idx - vox * G;
+                lds[grp * HP + vox] = st[i];
+            }
+        }
+    };
+    if (kAhead) fetch(0);
+    int q = 0;
+    for (int ch0 = 0; ch0 < cin; ch0 += 32) {
+        const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
+        if (!kAhead) fetch(ch0);
+        __syncthreads();  // the previous stage's reads are done
+        put(G);
+        __syncthreads();
+        if (kAhead && ch0 + 32 < cin) fetch(ch0 + 32);
+        const int nch = (3 * G + 3) / 4;
+        for (int c = 0; c < nch; ++c, ++q) {
+            if (PREFA) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) a[k][mt] = an[k][mt];
+                if (q + 1 < chunks) {
+                    const uint4 *nb = packed + (int64_t)(q + 1) * 9 * MT * 64 + lane;
+#pragma unroll
+                    for (int k = 0; k < 9; ++k)
+#pragma unroll
+                        for (int mt = 0; mt < MT; ++mt) an[k][mt].u = nb[(k * MT + mt) * 64];
+                }
+            } else {
+                const uint4 *cb = packed + (int64_t)q * 9 * MT * 64 + lane;
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt) a[k][mt].u = cb[(k * MT + mt) * 64];
+            }
+            // this quarter's entry of the chunk: voxel offset dx, channel group grp
+            const int e = 4 * c + kq;
+            const bool valid = e < 3 * G;
+            const int dx = valid ? e / G : 0;
+            const int grp = valid ? e - dx * G : 0;
+            const uint4 *lb = lds + grp * HP + ((wz * RZ) * HY + wy * RY) * HX + n + dx;
+#pragma unroll
+            for (int iz = 0; iz < RZ + 2; ++iz)
+#pragma unroll
+                for (int iy = 0; iy < RY + 2; ++iy) {
+                    Frag b;
+                    b.u = lb[(iz * HY + iy) * HX];
+                    if (!valid) b.u = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int dz = 0; dz < 3; ++dz) {
+                        const int oz = iz - dz;
+                        if (oz < 0 || oz >= RZ) continue;
+#pragma unroll
+                        for (int dy = 0; dy < 3; ++dy) {
+                            const int oy = iy - dy;
+                            if (oy < 0 || oy >= RY) continue;
+#pragma unroll
+                            for (int mt = 0; mt < MT; ++mt)
+                                acc[oz][oy][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                    a[dz * 3 + dy][mt].v, b.v, acc[oz][oy][mt], 0, 0, 0);
+                        }
+                    }
+                }
+        }
+    }
+    // lane holds D[co = mt*16 + 4*kq + j][voxel n of row (oy, oz)]
+    constexpr int cout = MT * 16;
+    const int gx = x0 + n;
+    if (gx >= g.nx) return;
+#pragma unroll
+    for (int oz = 0; oz < RZ; ++oz) {
+        const int gz = z0 + wz * RZ + oz;
+        if (gz >= g.nz) continue;
+#pragma unroll
+        for (int oy = 0; oy < RY; ++oy) {
+            const int gy = y0 + wy * RY + oy;
+            if (gy >= g.ny) continue;
+            const int64_t vb = ((int64_t)gz * g.ny + gy) * g.nx + gx;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int co = mt * 16 + 4 * kq;
+                float o[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    o[j] = acc[oz][oy][mt][j] + (bias ? bias[co + j] : 0.0f);
+                    if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
+                }
+                uint2 po;
+                po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+                po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+                *reinterpret_cast<uint2 *>(y + vb * cout + co) = po;
+            }
+        }
+    }
+}
+
+// A fragments of k_conv_rows: element idx = (((q * 9 + dz*3 + dy) * MT + mt) * 64 + lane) * 8 + j
+// is W[tap = dz*9 + dy*3 + dx][ci][co] with co = mt*16 + lane % 16 and, for the chunk's
+// entry e = 4c + lane/16 of its stage (ch0, G groups), dx = e / G and
+// ci = ch0 + 8 (e % G) + j; 0 for the padding entries (e >= 3G).
+__global__ void k_conv_pack_rows(const float *__restrict__ w, int cin, int cout, uint16_t *__restrict__ packed,
+                                 int64_t total) {
+    const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int mtn = cout / 16;
+    const int j = (int)(idx & 7);
+    const int lane = (int)((idx >> 3) & 63);
+    int64_t rest = idx >> 9;
+    const int mt = (int)(rest % mtn);
+    rest /= mtn;
+    const int t9 = (int)(rest % 9);
+    int q = (int)(rest / 9);
+    // the stage and chunk of global chunk q
+    int ch0 = 0, G = 0, c = 0;
+    for (ch0 = 0; ch0 < cin; ch0 += 32) {
+        G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
+        const int nch = (3 * G + 3) / 4;
+        if (q < nch) {
+            c = q;
+            break;
+        }
+        q -= nch;
+    }
+    const int e = 4 * c + (lane >> 4);
+    float val = 0.0f;
+    if (ch0 < cin && e < 3 * G) {
+        const int dx = e / G, grp = e - dx * G;
+        const int tap = t9 * 3 + dx;  // (dz*3 + dy)*3 + dx
+        const int ci = ch0 + 8 * grp + j;
+        const int co = mt * 16 + (lane & 15);
+        val = w[((int64_t)tap * cin + ci) * cout + co];
+    }
+    packed[idx] = bf16_bits(val);
+}
+
 // ---------------------------------------------------------------- head (16 -> 1 channel)
 
 // The network's last convolution has one used output channel: y[v] = b + sum over the
@@ -891,6 +1114,38 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
                          const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws,
                          int64_t ws_floats) {
     conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, ws, ws_floats);
+}
+
+namespace {
+constexpr int kRowsRY = 4, kRowsRZ = 2;  // rows per wave in y and z: tiles of 16 x 8 x 4
+}  // namespace
+
+void launch_conv_pack_rows(hipStream_t st, const float *w, int cin, int cout, bf16 *packed) {
+    const int64_t total = conv_rows_packed_elems(cin, cout);
+    hipLaunchKernelGGL(k_conv_pack_rows, dim3(grid1(total, 256)), dim3(256), 0, st, w, cin, cout, U(packed), total);
+}
+
+bool conv_rows_ok(Grid3 g, int cin, int cout) {
+    if (cin <= 0 || cin % 8 || (cout != 16 && cout != 32)) return false;
+    const int64_t tiles = (int64_t)((g.nx + 15) / 16) * ((g.ny + 2 * kRowsRY - 1) / (2 * kRowsRY)) *
+                          ((g.nz + 2 * kRowsRZ - 1) / (2 * kRowsRZ));
+    return tiles >= 512 && tiles < (1ll << 31);
+}
+
+void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
+                      int cout, bool relu, bf16 *y, const bf16 *coarse, int c1) {
+    const int tx = (g.nx + 15) / 16, ty = (g.ny + 2 * kRowsRY - 1) / (2 * kRowsRY);
+    const int tz = (g.nz + 2 * kRowsRZ - 1) / (2 * kRowsRZ);
+    const unsigned blocks = (unsigned)((int64_t)tx * ty * tz);
+    const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
+    const uint16_t *xc = coarse ? U(coarse) : nullptr;
+    const int chunks = conv_rows_chunks(cin);
+    if (cout == 16)
+        hipLaunchKernelGGL((k_conv_rows<1, kRowsRY, kRowsRZ, true, false>), dim3(blocks), dim3(256), 0, st, g, U(x),
+                           cin, pk, chunks, bias, relu ? 1 : 0, U(y), xc, c1, tx, ty);
+    else
+        hipLaunchKernelGGL((k_conv_rows<2, kRowsRY, kRowsRZ, false, false>), dim3(blocks), dim3(256), 0, st, g,
+                           U(x), cin, pk, chunks, bias, relu ? 1 : 0, U(y), xc, c1, tx, ty);
 }
 
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,

@@ -149,6 +149,29 @@ def conv3d(x: torch.Tensor, w: torch.Tensor, bias: torch.Tensor | None, relu: bo
     return y
 
 
+def conv3d_cat(coarse: torch.Tensor | None, x: torch.Tensor, w: torch.Tensor, bias: torch.Tensor | None,
+               relu: bool, kernel: int = 0) -> torch.Tensor:
+    """conv3d of the concatenation [upsample(coarse), x] (coarse (nz/2, ny/2, nx/2, c1)
+    bf16 or None) by kernel 0 (the network's inference choice for this grid), 1 (the
+    tiled kernel) or 2 (the row-reuse kernel, 16 or 32 outputs)."""
+    lib = _lib.load()
+    nz, ny, nx, cx = x.shape
+    c1 = 0 if coarse is None else coarse.shape[3]
+    cin = c1 + cx
+    cout = w.shape[2]
+    if w.shape[1] != cin:
+        raise ValueError(f"weights take {w.shape[1]} input channels, the input has {cin}")
+    y = torch.empty((nz, ny, nx, cout), dtype=torch.bfloat16, device=x.device)
+    nv = nx * ny * nz
+    cp = None if coarse is None else _ptr(coarse, (nv // 8) * c1, "coarse", torch.bfloat16)
+    bp = _ptr(bias, cout, "bias") if bias is not None else None
+    torch.cuda.synchronize(x.device)
+    _lib.check(lib.shk_nn_conv3d_cat(nx, ny, nz, cp, c1, _ptr(x, nv * cx, "x", torch.bfloat16), cin,
+                                     _ptr(w, 27 * cin * cout, "w"), bp, cout, int(bool(relu)), int(kernel),
+                                     _ptr(y, nv * cout, "y", torch.bfloat16)))
+    return y
+
+
 def conv3d_backward(x: torch.Tensor, w: torch.Tensor, dz: torch.Tensor, need_dx: bool = True):
     """(dx bf16 or None, dw fp32, db fp32) of conv3d for the output gradient dz."""
     lib = _lib.load()

@@ -63,6 +63,62 @@ def test_conv3d_forward(shape, cin, cout, relu):
     assert np.all(err <= tol)
 
 
+CAT_CASES = [  # (nz, ny, nx), c1 (upsampled coarse channels), cin, cout, relu
+    ((10, 20, 40), 0, 8, 16, True),      # partial tiles in x, y and z (tiles of 16 x 8 x 4)
+    ((12, 16, 32), 0, 16, 32, True),
+    ((8, 24, 48), 32, 48, 16, True),     # the last decoder layer: 32 + 16 channels
+    ((6, 10, 34), 64, 96, 32, False),    # 64 + 32 channels; the last stage starts in the skip input
+    ((4, 8, 16), 0, 24, 16, False),      # one 24-channel stage (3 groups: 9 entries in 3 chunks)
+    ((8, 8, 16), 0, 40, 32, True),       # stages of 32 and 8 channels
+]
+
+
+@pytest.mark.parametrize("shape,c1,cin,cout,relu", CAT_CASES)
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_conv3d_concatenated_input(shape, c1, cin, cout, relu, kernel):
+    """Both convolution kernels (the tiled one and the row-reuse one) on the decoder's
+    input [nearest upsampling of a half-size grid, skip] against the oracle's
+    convolution of the explicitly concatenated FP64 array."""
+    from shiokaze_amd import learned
+    log(f"[gpu] conv3d cat {shape} c1={c1} {cin}->{cout} relu={relu} kernel={kernel}")
+    rng = np.random.default_rng(cin * 100 + cout + c1)
+    nz, ny, nx = shape
+    skip = U.bf16_round(rng.standard_normal(shape + (cin - c1,)))
+    coarse = U.bf16_round(rng.standard_normal((nz // 2, ny // 2, nx // 2, c1))) if c1 else None
+    w = rng.standard_normal((27, cin, cout)) * np.sqrt(2.0 / (27 * cin))
+    b = rng.standard_normal(cout) * 0.1
+    y = _host(learned.conv3d_cat(None if coarse is None else _bf16_tensor(coarse), _bf16_tensor(skip),
+                                 _dev(w.astype(np.float32)), _dev(b.astype(np.float32)), relu, kernel))
+    if c1:
+        up = coarse.repeat(2, axis=0).repeat(2, axis=1).repeat(2, axis=2)
+        x = np.concatenate([up, skip], axis=3)
+    else:
+        x = skip
+    wq = U.bf16_round(w)
+    ref = U.conv3d(x, wq, b.astype(np.float32).astype(np.float64))
+    if relu:
+        ref = np.maximum(ref, 0.0)
+    scale = U.conv3d(np.abs(x), np.abs(wq))
+    tol = 2.0 ** -8 * np.abs(ref) + 1e-5 * scale + 1e-30
+    err = np.abs(y - ref)
+    log(f"[gpu]   max err/tol = {np.max(err / tol):.3f}")
+    assert np.all(err <= tol)
+
+
+def test_conv3d_kernel_choice():
+    """Kernel 0 takes the row-reuse kernel on a grid of >= 512 tiles (bit-identical to
+    kernel 2) and the tiled kernel below (bit-identical to kernel 1)."""
+    import torch
+    from shiokaze_amd import learned
+    rng = np.random.default_rng(7)
+    w = _dev((rng.standard_normal((27, 16, 16)) * 0.1).astype(np.float32))
+    for shape, pick in (((64, 64, 64), 2), ((32, 32, 32), 1)):
+        x = _bf16_tensor(rng.standard_normal(shape + (16,)))
+        y0 = learned.conv3d_cat(None, x, w, None, True, 0)
+        yp = learned.conv3d_cat(None, x, w, None, True, pick)
+        assert torch.equal(y0, yp), shape
+
+
 @pytest.mark.parametrize("shape,cin,cout", [((5, 6, 7), 16, 16), ((4, 5, 6), 32, 64), ((4, 4, 4), 192, 64),
                                             ((3, 4, 5), 8, 16), ((4, 4, 3), 96, 32)])
 def test_conv3d_backward(shape, cin, cout):
