@@ -88,6 +88,28 @@ int shk_add_gravity(shk_ctx *ctx, double dt, double gx, double gy, double gz, fl
  * cell count (may be NULL).  Keys: IntegrationRate (0.75), RedistTemporalScheme (Euler). */
 int shk_redistance(shk_ctx *ctx, float *phi, uint8_t *band, int width, int64_t *active);
 
+/* gridutility3::extrapolate_levelset(solid, fluid) as the level-set tracker calls it after
+ * redistancing (maclevelsetsurfacetracker3.cpp:59-61; gridutility3.cpp:87-146) on the
+ * cell-centred level set phi (nx*ny*nz floats) and its band mask, in place, with the
+ * nodal solid ((nx+1)(ny+1)(nz+1) floats) and its node activity solid_band (uint8, NULL:
+ * every node active).  Nothing changes unless an active solid node is below 0.  Band
+ * cells whose interpolated solid value s is below 0 take the level set interpolated at
+ * the cell moved by -s/dx along the normalised solid gradient (with
+ * HorizontalSolidWallExtrapolation, and a normal whose y part is below sin(pi/4), the
+ * minimum of that and the value along the normal without its y part); the band is
+ * dilated once (new cells >= 0 next to a cell below 0 become dx); then, as
+ * combine_levelset(solid, fluid, combined, dx) (:66-86), the band becomes its union with
+ * the cells that have an active solid node, each with max(phi, -dx - mean of its 8
+ * solid nodes).  Cells off the final band take -background where that expression is
+ * negative and +background elsewhere (the flood fill's side), `background` being the
+ * level set's half bandwidth.  Arrays are read as given off their bands: pass what the
+ * reference reads there (the background with the side's sign, as shk_redistance
+ * leaves it).  *active (nullable) receives the final band's cell count.  Keys:
+ * SolidWallExtrapolation (1; 0: band cells inside the solid take s),
+ * HorizontalSolidWallExtrapolation (1) -- gridutility3.cpp:394-395, 413-414. */
+int shk_extrapolate_levelset(shk_ctx *ctx, const float *solid, const uint8_t *solid_band, float *phi, uint8_t *band,
+                             float background, int64_t *active);
+
 /* The activity the projection leaves (macpressuresolver3.cpp:269-283, `it.set_off()`):
  * after an shk_project with unknowns, an active face stays active only where its area
  * and liquid fractions of that projection are both non-zero; a projection without

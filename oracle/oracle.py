@@ -57,6 +57,7 @@ _SIG = {
     "shk_or_curvature2d": (None, [_i, _i, _d, _vp, _i, _vp]),
     "shk_or_extrapolate": (_i64, [_i, _i, _i, _i, _vp, _vp, _i]),
     "shk_or_redistance": (_i64, [_i, _i, _i, _d, _vp, _vp, _i, _d, _i]),
+    "shk_or_extrapolate_levelset": (_i64, [_i, _i, _i, _d, _vp, _vp, _vp, _vp, ctypes.c_float, _i, _i]),
     "shk_or_set_weno": (None, [_i]),
     "shk_or_constrain_velocity": (None, [_i, _i, _i, _d, _vp] + [_vp] * 6),
     "shk_or_compute_max_u": (_d, [_i, _i, _i] + [_vp] * 6),
@@ -513,6 +514,21 @@ def redistance(dx, phi, band, width, rate=0.75, rk2=False):
     out = _f32(phi).copy()
     m = _u8(band).copy()
     cnt = lib.shk_or_redistance(nx, ny, nz, float(dx), _p(out), _p(m), int(width), float(rate), 1 if rk2 else 0)
+    return out, m, cnt
+
+
+def extrapolate_levelset(dx, solid, phi, band, background, solid_band=None, wall=True, horizontal=True):
+    """gridutility3::extrapolate_levelset(solid, fluid) (gridutility3.cpp:87-146) on a
+    dense (nz, ny, nx) level set with its band mask and the nodal solid (nz+1, ny+1,
+    nx+1) with its node activity (None: every node).  Returns (phi, band, active count)."""
+    lib = load()
+    nz, ny, nx = phi.shape
+    out = _f32(phi).copy()
+    m = _u8(band).copy()
+    sol = _f32(solid)
+    sb = None if solid_band is None else _u8(solid_band)
+    cnt = lib.shk_or_extrapolate_levelset(nx, ny, nz, float(dx), _p(sol), None if sb is None else _p(sb), _p(out),
+                                          _p(m), float(background), 1 if wall else 0, 1 if horizontal else 0)
     return out, m, cnt
 
 

@@ -41,6 +41,11 @@
  *                              the float arithmetic of the Euler and RK2 updates); cells
  *                              off the final band take -/+ width*dx by the sign of their
  *                              value (the flood fill's inside test, `value < 0`)
+ *   shk_or_extrapolate_levelset  gridutility3.cpp:87-146 (extrapolation towards the solid
+ *                              walls, one dilation, combine_levelset :66-86 with
+ *                              convert_to_cell :48-65), array_utility3.h:112-122,
+ *                              vec.h:272-320; cells off the final band take -/+ the
+ *                              background by the sign of their combined value
  *
  * Pins (tests/test_oracle_liquid.py): extrapolation against scipy's binary dilation
  * for the active set, the half-space closed form, the maximum principle and a brute
@@ -813,5 +818,147 @@ int64_t shk_or_redistance(int nx, int ny, int nz, double dx, float *phi, uint8_t
     free(d0);
     free(d1);
     free(tmp);
+    return active;
+}
+
+/* ------------------------------------------------------------------ level set into solids */
+
+/* vec::normalize (vec.h:311-320): divide by the length when it is not zero */
+static void normalize3(double v[3]) {
+    double n2 = 0.0;
+    for (int d = 0; d < 3; ++d) n2 += v[d] * v[d];
+    const double len = sqrt(n2);
+    if (len != 0.0)
+        for (int d = 0; d < 3; ++d) v[d] /= len;
+}
+static double norm2_3(const double v[3]) {
+    double n2 = 0.0;
+    for (int d = 0; d < 3; ++d) n2 += v[d] * v[d];
+    return n2;
+}
+
+/* gridutility3::extrapolate_levelset(solid, fluid, threshold = 0) (gridutility3.cpp:87-146)
+ * on the cell-centred level set phi (nx ny nz) with its band mask, both in place, and the
+ * nodal solid ((nx+1)(ny+1)(nz+1)) with its node activity solid_band (NULL: every node).
+ * `background` is the fluid level set's background value (its half bandwidth).
+ *  1. levelset_exist(solid) (array_utility3.h:112-122): an active node below 0, else no
+ *     change at all.
+ *  2. every band cell whose interpolated solid value s < 0 takes, from a copy of phi
+ *     (SolidWallExtrapolation = wall), the value interpolated at the cell moved by -s/dx
+ *     along the normalised solid gradient (array_derivative3::derivative in index units),
+ *     and with HorizontalSolidWallExtrapolation (horizontal) and normal_y < sin(pi/4) the
+ *     minimum of that and the value along the normal with its y part removed; a zero
+ *     gradient gives `background`; without wall the cell takes s.
+ *  3. one dilation of the band: a new cell whose value is >= 0 next to an in-bounds cell
+ *     below 0 becomes dx; the other new cells stay off the band.
+ *  4. combine_levelset(solid, fluid, combined, dx) (:66-86, convert_to_cell :48-65): the
+ *     cell solid value is the mean of its 8 nodes on cells with an active node (elsewhere
+ *     -/+ background by its sign); the band becomes the union of the fluid band and those
+ *     cells, each with max(phi, -dx - solid_cell) in float; cells off it take -/+
+ *     background by the sign of that same expression (the flood fill's inside test).
+ * Returns the final band's cell count. */
+int64_t shk_or_extrapolate_levelset(int nx, int ny, int nz, double dx, const float *solid, const uint8_t *solid_band,
+                                    float *phi, uint8_t *band, float background, int wall, int horizontal) {
+    const Shape s = {nx, ny, nz}, ns = {nx + 1, ny + 1, nz + 1};
+    const int64_t n = (int64_t)nx * ny * nz, nn = (int64_t)ns.nx * ns.ny * ns.nz;
+    int exist = 0;
+    for (int64_t q = 0; q < nn && !exist; ++q)
+        if ((!solid_band || solid_band[q]) && solid[q] < 0.0f) exist = 1;
+    int64_t active = 0;
+    if (!exist) {
+        for (int64_t c = 0; c < n; ++c) active += band[c] != 0;
+        return active;
+    }
+    float *old = (float *)malloc(sizeof(float) * (size_t)n);
+    uint8_t *m = (uint8_t *)malloc((size_t)n);
+    memcpy(old, phi, sizeof(float) * (size_t)n);
+    memcpy(m, band, (size_t)n);
+    const double limit_y = sin(3.14159265358979323846 / 4.0); /* sin(M_PI/4) */
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = at(s, i, j, k);
+                if (!band[c]) continue;
+                const double p[3] = {i + 0.5, j + 0.5, k + 0.5};
+                const double sl = (double)interp(solid, NULL, ns, p);
+                if (!(sl < 0.0)) continue;
+                if (!wall) {
+                    phi[c] = (float)sl;
+                    continue;
+                }
+                float g[3];
+                derivative(solid, ns, p, g);
+                double nrm[3] = {g[0], g[1], g[2]};
+                normalize3(nrm);
+                if (norm2_3(nrm) == 0.0) {
+                    phi[c] = background;
+                    continue;
+                }
+                const double t = -sl / dx;
+                const double pn[3] = {i + t * nrm[0], j + t * nrm[1], k + t * nrm[2]};
+                float value = interp(old, NULL, s, pn);
+                if (horizontal && nrm[1] < limit_y) {
+                    double h[3] = {nrm[0], 0.0, nrm[2]};
+                    normalize3(h);
+                    if (norm2_3(h) != 0.0) {
+                        const double ph[3] = {i + t * h[0], j + t * h[1], k + t * h[2]};
+                        const float vh = interp(old, NULL, s, ph);
+                        if (vh < value) value = vh;
+                    }
+                }
+                phi[c] = value;
+            }
+    /* one dilation, decided on the state after the extrapolation (the cells it sets
+     * become dx >= 0, so no sign test of this pass depends on another) */
+    static const int q6[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}};
+    memcpy(old, phi, sizeof(float) * (size_t)n);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = at(s, i, j, k);
+                if (band[c]) continue;
+                int next = 0, below = 0;
+                for (int t = 0; t < 6; ++t) {
+                    const int ii = i + q6[t][0], jj = j + q6[t][1], kk = k + q6[t][2];
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= nx || jj >= ny || kk >= nz) continue;
+                    const int64_t e = at(s, ii, jj, kk);
+                    if (band[e]) next = 1;
+                    if (old[e] < 0.0f) below = 1;
+                }
+                if (next && old[c] >= 0.0f && below) {
+                    phi[c] = (float)dx;
+                    m[c] = 1;
+                }
+            }
+    /* combine_levelset with the solid offset dx */
+    const float off = (float)dx;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = at(s, i, j, k);
+                double v = 0.0;
+                int any = 0;
+                for (int ii = 0; ii < 2; ++ii)
+                    for (int jj = 0; jj < 2; ++jj)
+                        for (int kk = 0; kk < 2; ++kk) {
+                            const int64_t e = at(ns, i + ii, j + jj, k + kk);
+                            v += (double)solid[e];
+                            if (!solid_band || solid_band[e]) any = 1;
+                        }
+                v = v / 8.0;
+                const float sc = any ? (float)v : (v < 0.0 ? -background : background);
+                const float neg = -off - sc;
+                const float comb = phi[c] < neg ? neg : phi[c];
+                if (m[c] || any) {
+                    phi[c] = comb;
+                    band[c] = 1;
+                    ++active;
+                } else {
+                    phi[c] = comb < 0.0f ? -background : background;
+                    band[c] = 0;
+                }
+            }
+    free(old);
+    free(m);
     return active;
 }

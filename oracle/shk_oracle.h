@@ -159,6 +159,8 @@ int64_t shk_or_extrapolate(int nx, int ny, int nz, int dim, float *a, uint8_t *m
 void shk_or_set_weno(int on);
 int64_t shk_or_redistance(int nx, int ny, int nz, double dx, float *phi, uint8_t *band, int width, double rate,
                           int rk2);
+int64_t shk_or_extrapolate_levelset(int nx, int ny, int nz, double dx, const float *solid, const uint8_t *solid_band,
+                                    float *phi, uint8_t *band, float background, int wall, int horizontal);
 void shk_or_constrain_velocity(int nx, int ny, int nz, double dx, const float *solid, float *u, float *v,
                                float *w, const uint8_t *mu, const uint8_t *mv, const uint8_t *mw);
 double shk_or_compute_max_u(int nx, int ny, int nz, const float *u, const float *v, const float *w,

@@ -509,6 +509,12 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
             p.redist_rk2 = true;
         else
             return c->fail(SHK_BAD_ARG, "RedistTemporalScheme must be Euler or RK2, got '%s'", value);
+    } else if (k == "SolidWallExtrapolation") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad SolidWallExtrapolation '%s'", value);
+        p.solid_wall_extrap = b;
+    } else if (k == "HorizontalSolidWallExtrapolation") {
+        if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad HorizontalSolidWallExtrapolation '%s'", value);
+        p.horizontal_extrap = b;
     } else if (k == "MacCormack") {
         if (!parse_bool(value, b)) return c->fail(SHK_BAD_ARG, "bad MacCormack '%s'", value);
         p.maccormack = b;

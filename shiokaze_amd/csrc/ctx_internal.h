@@ -74,6 +74,8 @@ struct Params {
     int trim_narrowband = 1;          // macadvection3.cpp:283
     double redist_rate = 0.75;        // pderedistancer3.cpp:206 (IntegrationRate)
     bool redist_rk2 = false;          // pderedistancer3.cpp:207 (RedistTemporalScheme Euler)
+    bool solid_wall_extrap = true;    // gridutility3.cpp:394, 413 (SolidWallExtrapolation)
+    bool horizontal_extrap = true;    // gridutility3.cpp:395, 414 (HorizontalSolidWallExtrapolation)
 };
 
 }  // namespace shk

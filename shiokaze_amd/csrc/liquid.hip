@@ -799,6 +799,191 @@ int64_t max_faces(const Dims &d) {
     return n;
 }
 
+// ---------------------------------------------------------------- level set into solids
+// gridutility3::extrapolate_levelset(solid, fluid) (gridutility3.cpp:87-146), in four
+// passes over the cells: whether the solid has an active node below 0
+// (array_utility3.h:112-122), the extrapolation of the band cells inside the solid from
+// a snapshot of phi, one dilation of the band, and combine_levelset with the offset dx
+// (:66-86; convert_to_cell :48-65).
+
+__global__ void k_exls_exist(int64_t nn, const float *__restrict__ solid, const uint8_t *__restrict__ sband,
+                             unsigned long long *__restrict__ flag) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nn; q += (int64_t)gridDim.x * blockDim.x)
+        if ((!sband || sband[q]) && solid[q] < 0.0f) *flag = 1ull;  // every writer stores the same value
+}
+
+// array_derivative3::derivative<float> (array_derivative3.h:51-122): the trilinear
+// gradient in index units, float accumulation per axis
+__device__ inline void derivative_idx(const float *a, FS s, const double p[3], float g[3]) {
+#pragma clang fp contract(off)
+    int ci, cj, ck;
+    double w[8];
+    interp_coef(s, p, ci, cj, ck, w);  // clamps and cell as derivative_interpolate_coef
+    const double x = fmax(0.0, fmin(s.nx - 1., p[0]));
+    const double y = fmax(0.0, fmin(s.ny - 1., p[1]));
+    const double z = fmax(0.0, fmin(s.nz - 1., p[2]));
+    const double gx[8] = {-(ck + 1 - z) * (cj + 1 - y), (ck + 1 - z) * (cj + 1 - y),
+                          -(ck + 1 - z) * (y - cj),     (ck + 1 - z) * (y - cj),
+                          -(z - ck) * (cj + 1 - y),     (z - ck) * (cj + 1 - y),
+                          -(z - ck) * (y - cj),         (z - ck) * (y - cj)};
+    const double gy[8] = {-(ck + 1 - z) * (ci + 1 - x), -(ck + 1 - z) * (x - ci),
+                          (ck + 1 - z) * (ci + 1 - x),  (ck + 1 - z) * (x - ci),
+                          -(z - ck) * (ci + 1 - x),     -(z - ck) * (x - ci),
+                          (z - ck) * (ci + 1 - x),      (z - ck) * (x - ci)};
+    const double gz[8] = {-(ci + 1 - x) * (cj + 1 - y), -(x - ci) * (cj + 1 - y),
+                          -(ci + 1 - x) * (y - cj),     -(x - ci) * (y - cj),
+                          (ci + 1 - x) * (cj + 1 - y),  (x - ci) * (cj + 1 - y),
+                          (ci + 1 - x) * (y - cj),      (x - ci) * (y - cj)};
+    g[0] = g[1] = g[2] = 0.0f;
+#pragma unroll
+    for (int n = 0; n < 8; ++n) {
+        const double v = (double)a[corner(s, ci, cj, ck, n)];
+        g[0] = (float)((double)g[0] + gx[n] * v);
+        g[1] = (float)((double)g[1] + gy[n] * v);
+        g[2] = (float)((double)g[2] + gz[n] * v);
+    }
+}
+
+// vec::normalize (vec.h:311-320) and norm2 (vec.h:272-276)
+__device__ inline double norm2_3(const double v[3]) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) r += v[d] * v[d];
+    return r;
+}
+__device__ inline void normalize3(double v[3]) {
+#pragma clang fp contract(off)
+    const double len = sqrt(norm2_3(v));
+    if (len != 0.0)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[d] /= len;
+}
+
+// pass 2: the band cells inside the solid, from the snapshot `old`
+__global__ __launch_bounds__(kNT) void k_exls_wall(Dims d, Split sp, double dx, const float *__restrict__ solid,
+                                                   const float *__restrict__ old, const uint8_t *__restrict__ band,
+                                                   float *__restrict__ phi, float background, int wall,
+                                                   int horizontal) {
+#pragma clang fp contract(off)
+    const FS s{d.nx, d.ny, d.nz}, ns{d.nx + 1, d.ny + 1, d.nz + 1};
+    const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (c >= s.n() || !band[c]) return;
+    int i, j, k;
+    decompose(c, sp, i, j, k);
+    const double p[3] = {i + 0.5, j + 0.5, k + 0.5};
+    const double sl = (double)interp(solid, nullptr, ns, p);
+    if (!(sl < 0.0)) return;
+    if (!wall) {
+        phi[c] = (float)sl;
+        return;
+    }
+    float g[3];
+    derivative_idx(solid, ns, p, g);
+    double nrm[3] = {(double)g[0], (double)g[1], (double)g[2]};
+    normalize3(nrm);
+    if (norm2_3(nrm) == 0.0) {
+        phi[c] = background;
+        return;
+    }
+    const double t = -sl / dx;
+    const double pn[3] = {i + t * nrm[0], j + t * nrm[1], k + t * nrm[2]};
+    float value = interp(old, nullptr, s, pn);
+    // sin(pi/4) as the double the reference computes (gridutility3.cpp:93)
+    const double limit_y = 0.70710678118654746;
+    if (horizontal && nrm[1] < limit_y) {
+        double h[3] = {nrm[0], 0.0, nrm[2]};
+        normalize3(h);
+        if (norm2_3(h) != 0.0) {
+            const double ph[3] = {i + t * h[0], j + t * h[1], k + t * h[2]};
+            const float vh = interp(old, nullptr, s, ph);
+            if (vh < value) value = vh;
+        }
+    }
+    phi[c] = value;
+}
+
+// pass 3: one dilation; a new cell >= 0 next to an in-bounds cell below 0 becomes dx
+__global__ __launch_bounds__(kNT) void k_exls_dilate(Dims d, Split sp, double dx, const float *__restrict__ phi,
+                                                     const uint8_t *__restrict__ band, float *__restrict__ phi2,
+                                                     uint8_t *__restrict__ band2) {
+    const FS s{d.nx, d.ny, d.nz};
+    const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    if (c >= s.n()) return;
+    float v = phi[c];
+    uint8_t m = band[c];
+    if (!m) {
+        int i, j, k;
+        decompose(c, sp, i, j, k);
+        const int o[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, -1}, {0, 0, 1}};
+        bool next = false, below = false;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+            const int ii = i + o[t][0], jj = j + o[t][1], kk = k + o[t][2];
+            if (ii < 0 || jj < 0 || kk < 0 || ii >= s.nx || jj >= s.ny || kk >= s.nz) continue;
+            const int64_t e = s.at(ii, jj, kk);
+            next = next || band[e];
+            below = below || phi[e] < 0.0f;
+        }
+        if (next && v >= 0.0f && below) {
+            v = (float)dx;
+            m = 1;
+        }
+    }
+    phi2[c] = v;
+    band2[c] = m;
+}
+
+// pass 4: combine_levelset(solid, fluid, combined, dx); off the union band the cells take
+// -/+ background by the sign of the combined value
+__global__ __launch_bounds__(kNT) void k_exls_combine(Dims d, Split sp, double dx, const float *__restrict__ solid,
+                                                      const uint8_t *__restrict__ sband,
+                                                      const float *__restrict__ phi2,
+                                                      const uint8_t *__restrict__ band2, float background,
+                                                      float *__restrict__ phi, uint8_t *__restrict__ band,
+                                                      unsigned long long *__restrict__ count) {
+#pragma clang fp contract(off)
+    const FS s{d.nx, d.ny, d.nz}, ns{d.nx + 1, d.ny + 1, d.nz + 1};
+    const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    bool act = false;
+    if (c < s.n()) {
+        int i, j, k;
+        decompose(c, sp, i, j, k);
+        double v = 0.0;
+        bool any = false;
+#pragma unroll
+        for (int ii = 0; ii < 2; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int kk = 0; kk < 2; ++kk) {
+                    const int64_t e = ns.at(i + ii, j + jj, k + kk);
+                    v += (double)solid[e];
+                    any = any || !sband || sband[e];
+                }
+        v = v / 8.0;
+        const float sc = any ? (float)v : (v < 0.0 ? -background : background);
+        const float neg = -(float)dx - sc;
+        const float f = phi2[c];
+        const float comb = f < neg ? neg : f;
+        act = band2[c] || any;
+        phi[c] = act ? comb : (comb < 0.0f ? -background : background);
+        band[c] = act ? 1 : 0;
+    }
+    const unsigned long long wave_act = __popcll(__ballot(act));
+    if ((threadIdx.x & 63) == 0 && wave_act) atomicAdd(count, wave_act);
+}
+
+__global__ void k_exls_fill(unsigned long long *p) { p[0] = 0ull; }
+
+__global__ __launch_bounds__(kNT) void k_count_band(int64_t n, const uint8_t *__restrict__ band,
+                                                    unsigned long long *__restrict__ count) {
+    const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
+    const bool act = c < n && band[c];
+    const unsigned long long wave_act = __popcll(__ballot(act));
+    if ((threadIdx.x & 63) == 0 && wave_act) atomicAdd(count, wave_act);
+}
+
 bool aligned16(std::initializer_list<const void *> ps) {
     for (const void *p : ps)
         if (reinterpret_cast<uintptr_t>(p) % 16) return false;
@@ -866,6 +1051,50 @@ int shk_extrapolate_velocity(shk_ctx *c, float *u, float *v, float *w, uint8_t *
     HIPC(c, hipMemcpyAsync(&cnt, lw->count.p, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIPC(c, hipStreamSynchronize(c->stream));
     if (activated) *activated = (int64_t)cnt;
+    return SHK_OK;
+}
+
+int shk_extrapolate_levelset(shk_ctx *c, const float *solid, const uint8_t *solid_band, float *phi, uint8_t *band,
+                             float background, int64_t *active) {
+    int rc;
+    if ((rc = check3d(c, "shk_extrapolate_levelset"))) return rc;
+    if (!solid || !phi || !band) return c->fail(SHK_BAD_ARG, "null array");
+    const Dims d = c->dims;
+    const FS s{d.nx, d.ny, d.nz};
+    const int64_t n = s.n(), nn = (int64_t)(d.nx + 1) * (d.ny + 1) * (d.nz + 1);
+    LiquidWork *lw = work(c);
+    HIPC(c, lw->count.ensure(2));
+    HIPC(c, lw->a[0].ensure((size_t)n));
+    HIPC(c, lw->a[1].ensure((size_t)n));
+    HIPC(c, lw->band[0].ensure((size_t)n));
+    hipStream_t st = c->stream;
+    unsigned long long *flag = lw->count.p, *cnt_d = lw->count.p + 1;
+    hipLaunchKernelGGL(k_exls_fill, dim3(1), dim3(1), 0, st, flag);
+    hipLaunchKernelGGL(k_exls_fill, dim3(1), dim3(1), 0, st, cnt_d);
+    hipLaunchKernelGGL(k_exls_exist, dim3(1024), dim3(kNT), 0, st, nn, solid, solid_band, flag);
+    HIPC(c, hipGetLastError());
+    unsigned long long exist = 0;
+    HIPC(c, hipMemcpyAsync(&exist, flag, sizeof exist, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    const Split sp = cells_split(d);
+    const unsigned gb = blocks_for(n);
+    if (exist) {
+        HIPC(c, hipMemcpyAsync(lw->a[0].p, phi, sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_exls_wall, dim3(gb), dim3(kNT), 0, st, d, sp, c->dx, solid, lw->a[0].p, band, phi,
+                           background, c->prm.solid_wall_extrap ? 1 : 0, c->prm.horizontal_extrap ? 1 : 0);
+        hipLaunchKernelGGL(k_exls_dilate, dim3(gb), dim3(kNT), 0, st, d, sp, c->dx, phi, band, lw->a[1].p,
+                           lw->band[0].p);
+        hipLaunchKernelGGL(k_exls_combine, dim3(gb), dim3(kNT), 0, st, d, sp, c->dx, solid, solid_band, lw->a[1].p,
+                           lw->band[0].p, background, phi, band, cnt_d);
+    } else {
+        // no solid: nothing changes; count the band
+        hipLaunchKernelGGL(k_count_band, dim3(gb), dim3(kNT), 0, st, n, band, cnt_d);
+    }
+    HIPC(c, hipGetLastError());
+    unsigned long long cnt = 0;
+    HIPC(c, hipMemcpyAsync(&cnt, cnt_d, sizeof cnt, hipMemcpyDeviceToHost, st));
+    HIPC(c, hipStreamSynchronize(st));
+    if (active) *active = (int64_t)cnt;
     return SHK_OK;
 }
 

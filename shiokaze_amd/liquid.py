@@ -118,6 +118,20 @@ class MacLiquidVelocity:
                                             ctypes.byref(active)))
         return int(active.value)
 
+    def extrapolate_levelset(self, solid, phi, band, background: float, solid_band=None) -> int:
+        """gridutility3::extrapolate_levelset(solid, fluid) on the level set `phi` and its
+        band (uint8 cell mask), in place, with the nodal solid and its node activity
+        (None: every node); `background` is the level set's half bandwidth.  Returns the
+        final band's cell count."""
+        nn = (self.solver.shape[0] + 1) * (self.solver.shape[1] + 1) * (self.solver.shape[2] + 1)
+        active = ctypes.c_int64()
+        sb = None if solid_band is None else _ptr(solid_band, nn, "solid_band", torch.uint8)
+        self._check(self.lib.shk_extrapolate_levelset(self.solver.ctx, _ptr(solid, nn, "solid"), sb,
+                                                      _ptr(phi, self.solver.cells, "phi"),
+                                                      _ptr(band, self.solver.cells, "band", torch.uint8),
+                                                      float(background), ctypes.byref(active)))
+        return int(active.value)
+
     def add_gravity(self, dt: float, velocity, masks) -> None:
         """`m_velocity += dt*gravity` (activates every face)."""
         p, m = self._faces(velocity, masks)

@@ -507,3 +507,65 @@ def test_advect_weno_matches_oracle(scenes, oracle, kind, n, seed, maccormack):
     moved = sum(int((w != p).sum()) for w, p in zip(want, plain))
     log(f"[gpu] advect WENO {kind} {shape} maccormack={maccormack}: bit-exact ({moved} faces differ from trilinear)")
     assert moved > 0
+
+
+def _tracker_inputs(scenes, oracle, kind, n, seed, width=3):
+    """The level set as the tracker holds it before extrapolate_levelset: the scene's liquid
+    redistanced by the oracle (band and -/+ width*dx off it), the scene's nodal solid and
+    its band |solid| < 3 dx with -/+ 3 dx off it (as the reference's solid array reads)."""
+    sc = _scene(scenes, kind, n, seed)
+    dx = sc["dx"]
+    bw = width * dx
+    phi = np.clip(sc["fluid"], -bw, bw).astype(np.float32)
+    band = (np.abs(sc["fluid"]) < bw).astype(np.uint8)
+    phi, band, _ = oracle.redistance(dx, phi, band, width)
+    solid = sc["solid"].astype(np.float32)
+    sband = (np.abs(solid) < 3 * dx).astype(np.uint8)
+    solid = np.where(sband > 0, solid, np.where(solid < 0, -3 * dx, 3 * dx)).astype(np.float32)
+    return sc, dx, phi, band, solid, sband, np.float32(bw)
+
+
+@pytest.mark.parametrize("kind,n,seed", CASES)
+@pytest.mark.parametrize("keys", [{}, {"HorizontalSolidWallExtrapolation": 0}, {"SolidWallExtrapolation": 0}])
+def test_extrapolate_levelset_matches_oracle(scenes, oracle, kind, n, seed, keys):
+    """shk_extrapolate_levelset against the oracle's gridutility3::extrapolate_levelset:
+    values, band and count bit for bit (the level set redistanced first, as the tracker
+    runs them)."""
+    sc, dx, phi, band, solid, sband, bg = _tracker_inputs(scenes, oracle, kind, n, seed)
+    shape = tuple(phi.shape[::-1])
+    wall = bool(keys.get("SolidWallExtrapolation", 1))
+    horizontal = bool(keys.get("HorizontalSolidWallExtrapolation", 1))
+    ophi, oband, ocnt = oracle.extrapolate_levelset(dx, solid, phi, band, bg, solid_band=sband, wall=wall,
+                                                    horizontal=horizontal)
+    ps, lv = _solver(shape, dx, keys)
+    with ps:
+        dphi, dband = _t(phi), _t(band)
+        cnt = lv.extrapolate_levelset(_t(solid), dphi, dband, float(bg), solid_band=_t(sband))
+        log(f"[gpu] extrapolate_levelset {kind} {n} seed={seed} {keys}: band {cnt} (oracle {ocnt}), "
+            f"{int((ophi != phi).sum())} values changed")
+        assert cnt == ocnt
+        np.testing.assert_array_equal(_h(dband), oband)
+        np.testing.assert_array_equal(_h(dphi), ophi)
+
+
+def test_extrapolate_levelset_without_solid_and_all_nodes_active(scenes, oracle):
+    """No active solid node below 0: nothing changes.  A NULL node mask: every node is
+    active (every cell joins the band)."""
+    sc, dx, phi, band, solid, sband, bg = _tracker_inputs(scenes, oracle, "POOL_SPHERE", 24, 0)
+    shape = tuple(phi.shape[::-1])
+    ps, lv = _solver(shape, dx)
+    with ps:
+        dphi, dband = _t(phi), _t(band)
+        cnt = lv.extrapolate_levelset(_t(np.abs(solid) + np.float32(dx)), dphi, dband, float(bg))
+        assert cnt == int(band.sum())
+        np.testing.assert_array_equal(_h(dphi), phi)
+        np.testing.assert_array_equal(_h(dband), band)
+        sc3, dx3, phi3, band3, solid3, _, bg3 = _tracker_inputs(scenes, oracle, "IRREGULAR", 32, 1)
+    ophi, oband, ocnt = oracle.extrapolate_levelset(dx3, solid3, phi3, band3, bg3)
+    ps, lv = _solver(tuple(phi3.shape[::-1]), dx3)
+    with ps:
+        dphi, dband = _t(phi3), _t(band3)
+        cnt = lv.extrapolate_levelset(_t(solid3), dphi, dband, float(bg3))
+        assert cnt == ocnt == phi3.size
+        np.testing.assert_array_equal(_h(dphi), ophi)
+        np.testing.assert_array_equal(_h(dband), oband)
