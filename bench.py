@@ -315,9 +315,21 @@ def main():
     if args.calibrate_copy > 0:
         src = torch.ones(args.calibrate_copy * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        dst.copy_(src)  # first touch of the destination pages
         torch.cuda.synchronize()
-        log(f"calibration copy: {args.calibrate_copy} MiB read + {args.calibrate_copy} MiB written")
+        # three timed copies: the device-to-device copy rate of this box (a measured HBM
+        # ceiling for streams that read and write)
+        best = float("inf")
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            dst.copy_(src)
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+        mib = args.calibrate_copy
+        log(f"calibration copy: {mib} MiB read + {mib} MiB written; best of 3: {best * 1e6:.1f} us = "
+            f"{2 * mib * (1 << 20) / best / 1e9:.1f} GB/s")
         del src, dst
 
     n = args.n

@@ -176,7 +176,25 @@ __device__ inline T lane_shfl(T v, int src) {
     return __shfl(v, src & (kWave - 1), kWave);
 }
 
-template <class T, class C, bool UNIT, int BX>
+// Stores (and loads) of the vector streams with the non-temporal hint when NTS: past
+// the 256 MB Infinity Cache the streams are touched once per kernel and streaming
+// accesses measure faster; a working set that fits keeps the cached ones.
+template <bool NTS, class V>
+__device__ inline void st_stream(V v, V *p) {
+    if constexpr (NTS)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+template <bool NTS, class V>
+__device__ inline V ld_stream(const V *p) {
+    if constexpr (NTS)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <class T, class C, bool UNIT, int BX, bool NTS>
 __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T *__restrict__ r,
                                                  const T *__restrict__ sin, T *__restrict__ sout,
                                                  T *__restrict__ q, Reduction red) {
@@ -290,8 +308,8 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
                 T qv = UNIT ? sk : kd * sk;
                 qv -= kv * xm + kv * xp + kv * ym + kv * yp + kv * zm + kv * zp;
                 const int64_t e = base + 64 * k;
-                sout[e] = sk;
-                q[e] = qv;
+                st_stream<NTS>(sk, &sout[e]);
+                st_stream<NTS>(qv, &q[e]);
                 acc += (double)sk * (double)qv;
             }
             continue;
@@ -346,8 +364,8 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
             T qv = UNIT ? sk : dv[k] * sk;
             qv -= cxv[k] * xm + cxp * xp + cyv[k] * ym + cyp * yp + czv[k] * zm + czp * zp;
             const int64_t e = base + 64 * k;
-            sout[e] = sk;
-            q[e] = qv;
+            st_stream<NTS>(sk, &sout[e]);
+            st_stream<NTS>(qv, &q[e]);
             acc += (double)sk * (double)qv;
         }
     }
@@ -384,7 +402,7 @@ __global__ __launch_bounds__(NB) void k_dir_spmv_csr(CsrOp op, int per, const do
 
 // ---------------------------------------------------------------- axpy + norm
 
-template <class T, bool SCALED>
+template <class T, bool SCALED, bool NTS>
 __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per, const double *__restrict__ sig,
                                                   T *__restrict__ x, T *__restrict__ r, const T *__restrict__ s,
                                                   const T *__restrict__ q, Reduction red) {
@@ -415,39 +433,40 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
     chunk_range(tiles, per, t0, t1);
     const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
     double rr = 0.0, rm = 0.0;
-    if constexpr (sizeof(T) == 4 && !SCALED) {
-        // FP32 vectors: four consecutive elements per thread in 16-byte accesses (tile
-        // boundaries are multiples of 4, the arrays 256-byte aligned; a remainder of
-        // fewer than 4 elements at the end goes through the scalar loop); each
-        // element's arithmetic is that of the scalar loop below
-        using T4 = HIP_vector_type<T, 4>;
-        T4 *__restrict__ x4 = reinterpret_cast<T4 *>(x);
-        T4 *__restrict__ r4 = reinterpret_cast<T4 *>(r);
-        const T4 *__restrict__ s4 = reinterpret_cast<const T4 *>(s);
-        const T4 *__restrict__ q4 = reinterpret_cast<const T4 *>(q);
-        for (int64_t e = (int64_t)t0 * (NT / 4) + threadIdx.x; e < end / 4; e += NB) {
-            T4 xv = x4[e], rv = r4[e];
-            const T4 sv = s4[e], qv = q4[e];
-            xv.x += a * sv.x;
-            xv.y += a * sv.y;
-            xv.z += a * sv.z;
-            xv.w += a * sv.w;
-            rv.x = rv.x - a * qv.x;
-            rv.y = rv.y - a * qv.y;
-            rv.z = rv.z - a * qv.z;
-            rv.w = rv.w - a * qv.w;
-            x4[e] = xv;
-            r4[e] = rv;
-            rr += (double)rv.x * (double)rv.x;
-            rr += (double)rv.y * (double)rv.y;
-            rr += (double)rv.z * (double)rv.z;
-            rr += (double)rv.w * (double)rv.w;
-            rm = fmax(rm, fmax(fmax(fabs((double)rv.x), fabs((double)rv.y)), fmax(fabs((double)rv.z), fabs((double)rv.w))));
+    constexpr int V = 16 / (int)sizeof(T);  // elements per 16-byte access
+    if constexpr (!SCALED) {
+        // 16-byte vectors: V consecutive elements per thread (tile boundaries are
+        // multiples of 4, the arrays 256-byte aligned; a remainder of fewer than V
+        // elements at the end goes through the scalar loop); each element's arithmetic
+        // is that of the scalar loop below
+        typedef T TV __attribute__((ext_vector_type(V)));
+        TV *__restrict__ xv4 = reinterpret_cast<TV *>(x);
+        TV *__restrict__ rv4 = reinterpret_cast<TV *>(r);
+        const TV *__restrict__ sv4 = reinterpret_cast<const TV *>(s);
+        const TV *__restrict__ qv4 = reinterpret_cast<const TV *>(q);
+        for (int64_t e = (int64_t)t0 * (NT / V) + threadIdx.x; e < end / V; e += NB) {
+            TV xv = ld_stream<NTS>(&xv4[e]), rv = ld_stream<NTS>(&rv4[e]);
+            const TV sv = ld_stream<NTS>(&sv4[e]), qv = ld_stream<NTS>(&qv4[e]);
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                xv[u] += a * sv[u];
+                rv[u] = rv[u] - a * qv[u];
+            }
+            st_stream<NTS>(xv, &xv4[e]);
+            st_stream<NTS>(rv, &rv4[e]);
+#pragma unroll
+            for (int u = 0; u < V; ++u) rr += (double)rv[u] * (double)rv[u];
+            if constexpr (V == 4) {
+                rm = fmax(rm, fmax(fmax(fabs((double)rv[0]), fabs((double)rv[1])),
+                                   fmax(fabs((double)rv[2]), fabs((double)rv[3]))));
+            } else {
+                rm = fmax(rm, fmax(fabs((double)rv[0]), fabs((double)rv[1])));
+            }
         }
     }
     {
         const int64_t first = (int64_t)t0 * NT;  // a block past the last tile has end <= first
-        const int64_t start = (sizeof(T) == 4 && !SCALED) ? (end / 4) * 4 > first ? (end / 4) * 4 : first : first;
+        const int64_t start = !SCALED ? (end / V) * V > first ? (end / V) * V : first : first;
         for (int64_t e = start + threadIdx.x; e < end; e += NB) {
             x[e] += a * s[e];
             const T rv = r[e] - a * q[e];
@@ -775,24 +794,41 @@ void launch_cg_init(hipStream_t st, const Grid &g, const TB *rhs, const double *
             hipLaunchKernelGGL(kernel, grid, block, 0, st, __VA_ARGS__);                            \
     } while (0)
 
+// the vector streams bypass the caches (non-temporal accesses) when the Krylov working
+// set (about six vectors) exceeds the 256 MB Infinity Cache
+constexpr int64_t kStreamBytes = 256ll << 20;
+template <class T>
+bool streaming(const Grid &g) {
+    return g.n * 6 * (int64_t)sizeof(T) > kStreamBytes;
+}
+
+template <class T, class C, bool NTS>
+void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out,
+                        T *q, Reduction red, KEvents ev) {
+    if (op.diag) {
+        if (op.bx == 64)
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+                       s_out, q, red);
+        else
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+                       s_out, q, red);
+    } else {
+        if (op.bx == 64)
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+                       s_out, q, red);
+        else
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+                       s_out, q, red);
+    }
+}
+
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
                      Reduction red, KEvents ev) {
-    if (op.diag) {
-        if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
-                       red);
-        else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
-                       red);
-    } else {
-        if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
-                       red);
-        else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in, s_out, q,
-                       red);
-    }
+    if (streaming<T>(g))
+        launch_dir_spmv_nt<T, C, true>(st, g, op, r, s_in, s_out, q, red, ev);
+    else
+        launch_dir_spmv_nt<T, C, false>(st, g, op, r, s_in, s_out, q, red, ev);
 }
 
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
@@ -803,12 +839,16 @@ void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const d
 template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
                       Reduction red, KEvents ev) {
+    const bool nts = streaming<T>(g);
     if (sig)
-        SHK_LAUNCH(ev, (k_axpy_norm<T, true>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s, q,
-                   red);
+        SHK_LAUNCH(ev, (k_axpy_norm<T, true, false>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s,
+                   q, red);
+    else if (nts)
+        SHK_LAUNCH(ev, (k_axpy_norm<T, false, true>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s,
+                   q, red);
     else
-        SHK_LAUNCH(ev, (k_axpy_norm<T, false>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s, q,
-                   red);
+        SHK_LAUNCH(ev, (k_axpy_norm<T, false, false>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s,
+                   q, red);
 }
 
 void launch_residual(hipStream_t st, const Grid &g, const BrickOp<double> &op, const double *b, const double *x,

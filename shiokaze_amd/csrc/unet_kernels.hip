@@ -230,19 +230,22 @@ __device__ inline int div_groups(int i, int g) {
 
 // Block: 256 threads (4 waves, 2 in y by 2 in z) computing a 16 x 2RY x 2RZ voxel tile
 // for all MT output tiles of 16 channels; wave (wy, wz) owns RY x RZ rows of 16 voxels
-// (one MFMA N-tile each).  A stage's input halo (18 x (2RY+2) x (2RZ+2) voxels, G groups
-// of 8 channels) sits in LDS group-major, lds[group * HP + voxel], so the 16 lanes of a
-// quarter read 16 consecutive voxels of one group (no bank conflicts; HP = 4 mod 16
-// spreads the groups of one voxel over the banks for the staging writes).  For each
-// chunk (four (dx, group) entries sharing (dz, dy)) the wave reads each of its
+// (one MFMA N-tile each).  A stage's input halo (18 x (2RY+2) x (2RZ+2) voxels, up to 4
+// groups of 8 channels) sits in LDS group-major, lds[group * HP + voxel], so the 16
+// lanes of a quarter read 16 consecutive voxels of one group (no bank conflicts).  For
+// each chunk (four (dx, group) entries sharing (dz, dy)) the wave reads each of its
 // (RY+2) x (RZ+2) input rows once and feeds it to the output rows it neighbours: 9 RY RZ
 // MT MFMAs from (RY+2)(RZ+2) LDS reads.  A fragments (the chunk's 9 (dz, dy) taps x MT)
 // come from global memory (L2: every block reads the same weights).  Tiles go to blocks
-// XCD by XCD, so
-// neighbouring tiles (which share halo rows) run on one L2.
-template <int MT, int RY, int RZ, bool AHEAD, bool PREFA>
+// XCD by XCD, so neighbouring tiles (which share halo rows) run on one L2.
+//
+// Staging: thread t always loads channel group t % 4 of the halo voxels t / 4 + 64 i, so
+// its source tensor, channel offset and voxel list are fixed per stage and the address
+// of each load is a few integer operations; all of a stage's loads are issued before
+// any is written to LDS.
+template <int MT, int RY, int RZ>
 __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *__restrict__ x, int cin,
-                                                      const uint4 *__restrict__ packed, int chunks,
+                                                      const uint4 *__restrict__ packed,
                                                       const float *__restrict__ bias, int relu,
                                                       uint16_t *__restrict__ y, const uint16_t *__restrict__ xc,
                                                       int c1, int tx, int ty) {
@@ -250,6 +253,7 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
     constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2;
     constexpr int HV = HX * HY * HZ;
     constexpr int HP = (HV + 11) / 16 * 16 + 4;  // >= HV, = 4 (mod 16)
+    constexpr int NLD = (HV + 63) / 64;          // halo voxels per thread
     __shared__ uint4 lds[4 * HP];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
@@ -263,91 +267,51 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
         for (int oy = 0; oy < RY; ++oy)
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) acc[oz][oy][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    // With MT == 1 the registers allow the next stage's halo loads to be issued before
-    // this stage's MFMAs, to land while they run; with MT == 2 they are issued at the
-    // stage's start.
-    constexpr bool kAhead = AHEAD;
-    Frag a[9][MT], an[9][MT];
-    if (PREFA) {
-#pragma unroll
-        for (int k = 0; k < 9; ++k)
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) an[k][mt].u = packed[(k * MT + mt) * 64 + lane];
-    }
-    // a stage's halo: up to NLD 16-byte loads per thread, all issued before any of them
-    // is written to LDS
-    constexpr int NLD = (HV * 4 + 255) / 256;
+    Frag a[9][MT];
+    const int tgrp = threadIdx.x & 3, tv0 = threadIdx.x >> 2;
+    const int hnx = g.nx >> 1, hny = g.ny >> 1;
     uint4 st[NLD];
-    auto fetch = [&](int c0) {
-        const int G = (cin - c0) / 8 < 4 ? (cin - c0) / 8 : 4;
-#pragma unroll
-        for (int i = 0; i < NLD; ++i) {
-            const int idx = threadIdx.x + 256 * i;
-            st[i] = make_uint4(0u, 0u, 0u, 0u);
-            if (idx >= HV * G) continue;
-            const int vox = div_groups(idx, G), grp = idx - vox * G;
-            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz) {
-                const int ch = c0 + grp * 8;
-                const int64_t vi = ((int64_t)gz * g.ny + gy) * g.nx + gx;
-                const uint16_t *src;
-                if (!xc) {
-                    src = x + vi * cin + ch;
-                } else if (ch < c1) {
-                    const int64_t pv = ((int64_t)(gz >> 1) * (g.ny >> 1) + (gy >> 1)) * (g.nx >> 1) + (gx >> 1);
-                    src = xc + pv * c1 + ch;
-                } else {
-                    src = x + vi * (cin - c1) + (ch - c1);
-                }
-                st[i] = *reinterpret_cast<const uint4 *>(src);
-            }
-        }
-    };
-    auto put = [&](int G) {
-#pragma unroll
-        for (int i = 0; i < NLD; ++i) {
-            const int idx = threadIdx.x + 256 * i;
-            if (idx < HV * G) {
-                const int vox = div_groups(idx, G), grp = idx - vox * G;
-                lds[grp * HP + vox] = st[i];
-            }
-        }
-    };
-    if (kAhead) fetch(0);
     int q = 0;
     for (int ch0 = 0; ch0 < cin; ch0 += 32) {
         const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
-        if (!kAhead) fetch(ch0);
+        // this thread's source for the stage: 8 channels of x, or of the coarse grid xc
+        // read at the voxel's parent (the nearest upsampling)
+        const int ch = ch0 + 8 * tgrp;
+        const bool on = tgrp < G;
+        const bool coarse = xc && ch < c1;
+        const uint16_t *base = coarse ? xc + ch : (xc ? x + (ch - c1) : x + ch);
+        const int64_t stride = coarse ? c1 : (xc ? cin - c1 : cin);
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int vox = tv0 + 64 * i;
+            st[i] = make_uint4(0u, 0u, 0u, 0u);
+            if (!on || vox >= HV) continue;
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            if ((unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz) {
+                const int64_t vi = coarse ? ((int64_t)(gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1)
+                                          : ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                st[i] = *reinterpret_cast<const uint4 *>(base + vi * stride);
+            }
+        }
         __syncthreads();  // the previous stage's reads are done
-        put(G);
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int vox = tv0 + 64 * i;
+            if (on && vox < HV) lds[tgrp * HP + vox] = st[i];
+        }
         __syncthreads();
-        if (kAhead && ch0 + 32 < cin) fetch(ch0 + 32);
         const int nch = (3 * G + 3) / 4;
         for (int c = 0; c < nch; ++c, ++q) {
-            if (PREFA) {
+            const uint4 *cb = packed + (int64_t)q * 9 * MT * 64 + lane;
 #pragma unroll
-                for (int k = 0; k < 9; ++k)
+            for (int k = 0; k < 9; ++k)
 #pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) a[k][mt] = an[k][mt];
-                if (q + 1 < chunks) {
-                    const uint4 *nb = packed + (int64_t)(q + 1) * 9 * MT * 64 + lane;
-#pragma unroll
-                    for (int k = 0; k < 9; ++k)
-#pragma unroll
-                        for (int mt = 0; mt < MT; ++mt) an[k][mt].u = nb[(k * MT + mt) * 64];
-                }
-            } else {
-                const uint4 *cb = packed + (int64_t)q * 9 * MT * 64 + lane;
-#pragma unroll
-                for (int k = 0; k < 9; ++k)
-#pragma unroll
-                    for (int mt = 0; mt < MT; ++mt) a[k][mt].u = cb[(k * MT + mt) * 64];
-            }
+                for (int mt = 0; mt < MT; ++mt) a[k][mt].u = cb[(k * MT + mt) * 64];
             // this quarter's entry of the chunk: voxel offset dx, channel group grp
             const int e = 4 * c + kq;
             const bool valid = e < 3 * G;
-            const int dx = valid ? e / G : 0;
+            const int dx = valid ? div_groups(e, G) : 0;
             const int grp = valid ? e - dx * G : 0;
             const uint4 *lb = lds + grp * HP + ((wz * RZ) * HY + wy * RY) * HX + n + dx;
 #pragma unroll
@@ -471,15 +435,25 @@ __global__ __launch_bounds__(256) void k_conv_head(Grid3 g, const uint16_t *__re
     __shared__ uint4 lds[2][kHeadRows];
     const int t = threadIdx.x;
     const int x0 = blockIdx.x * kHeadTX, y0 = blockIdx.y * kHeadTY, z0 = blockIdx.z * kHeadTZ;
-    // stage the halo (hx 0..17 used of the 19-voxel rows)
-    for (int idx = t; idx < 18 * kHeadHY * kHeadHZ * 2; idx += 256) {
-        const int half = idx & 1, vox = idx >> 1;
+    // stage the halo (hx 0..17 used of the 19-voxel rows): thread t loads half t % 2 of
+    // the halo voxels t / 2 + 128 i, all loads issued before any LDS write
+    constexpr int kHV = 18 * kHeadHY * kHeadHZ, kNL = (kHV + 127) / 128;
+    const int half = t & 1, v0 = t >> 1;
+    uint4 st[kNL];
+#pragma unroll
+    for (int i = 0; i < kNL; ++i) {
+        const int vox = v0 + 128 * i;
         const int hx = vox % 18, hy = (vox / 18) % kHeadHY, hz = vox / (18 * kHeadHY);
         const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-        uint4 v = make_uint4(0u, 0u, 0u, 0u);
-        if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny && gz >= 0 && gz < g.nz)
-            v = *reinterpret_cast<const uint4 *>(x + (((int64_t)gz * g.ny + gy) * g.nx + gx) * 16 + half * 8);
-        lds[half][(hz * kHeadHY + hy) * kHeadHX + hx] = v;
+        st[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (vox < kHV && (unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz)
+            st[i] = *reinterpret_cast<const uint4 *>(x + (((int64_t)gz * g.ny + gy) * g.nx + gx) * 16 + half * 8);
+    }
+#pragma unroll
+    for (int i = 0; i < kNL; ++i) {
+        const int vox = v0 + 128 * i;
+        const int hx = vox % 18, hy = (vox / 18) % kHeadHY, hz = vox / (18 * kHeadHY);
+        if (vox < kHV) lds[half][(hz * kHeadHY + hy) * kHeadHX + hx] = st[i];
     }
     __syncthreads();
     const int xq = t & 3, ty = (t >> 2) & 7, tz = t >> 5;
@@ -634,6 +608,36 @@ __global__ void k_pool(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint1
         float s = 0.0f;
         for (int a = 0; a < 8; ++a) s += bits_f32(fine[child(c, X, Y, Z, a) * ch + cc]);
         out[i] = bf16_bits(0.125f * s);
+    }
+}
+
+// k_pool for 8 channels per thread (16-byte loads and stores; the same per-channel sums)
+__global__ void k_pool8(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint16_t *__restrict__ out) {
+    const int groups = ch >> 3;
+    const int64_t total = c.vox() * groups;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int cg = (int)(i % groups);
+        const int64_t V = i / groups;
+        const int X = (int)(V % c.nx), Y = (int)((V / c.nx) % c.ny), Z = (int)(V / ((int64_t)c.nx * c.ny));
+        uint4 v[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) v[a] = *reinterpret_cast<const uint4 *>(fine + child(c, X, Y, Z, a) * ch + cg * 8);
+        float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const uint32_t w[4] = {v[a].x, v[a].y, v[a].z, v[a].w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                s[2 * h] += bits_f32((uint16_t)(w[h] & 0xffffu));
+                s[2 * h + 1] += bits_f32((uint16_t)(w[h] >> 16));
+            }
+        }
+        uint4 o;
+        o.x = (uint32_t)bf16_bits(0.125f * s[0]) | ((uint32_t)bf16_bits(0.125f * s[1]) << 16);
+        o.y = (uint32_t)bf16_bits(0.125f * s[2]) | ((uint32_t)bf16_bits(0.125f * s[3]) << 16);
+        o.z = (uint32_t)bf16_bits(0.125f * s[4]) | ((uint32_t)bf16_bits(0.125f * s[5]) << 16);
+        o.w = (uint32_t)bf16_bits(0.125f * s[6]) | ((uint32_t)bf16_bits(0.125f * s[7]) << 16);
+        *reinterpret_cast<uint4 *>(out + V * ch + cg * 8) = o;
     }
 }
 
@@ -1126,7 +1130,9 @@ void launch_conv_pack_rows(hipStream_t st, const float *w, int cin, int cout, bf
 }
 
 bool conv_rows_ok(Grid3 g, int cin, int cout) {
-    if (cin <= 0 || cin % 8 || (cout != 16 && cout != 32)) return false;
+    // an 8-channel input fills a quarter of the staged groups and a chunk three quarters:
+    // the tiled kernel is faster there (measured 37 us against 62 us for 8 -> 16 at 128^3)
+    if (cin < 16 || cin % 8 || (cout != 16 && cout != 32)) return false;
     const int64_t tiles = (int64_t)((g.nx + 15) / 16) * ((g.ny + 2 * kRowsRY - 1) / (2 * kRowsRY)) *
                           ((g.nz + 2 * kRowsRZ - 1) / (2 * kRowsRZ));
     return tiles >= 512 && tiles < (1ll << 31);
@@ -1139,13 +1145,12 @@ void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf1
     const unsigned blocks = (unsigned)((int64_t)tx * ty * tz);
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     const uint16_t *xc = coarse ? U(coarse) : nullptr;
-    const int chunks = conv_rows_chunks(cin);
     if (cout == 16)
-        hipLaunchKernelGGL((k_conv_rows<1, kRowsRY, kRowsRZ, true, false>), dim3(blocks), dim3(256), 0, st, g, U(x),
-                           cin, pk, chunks, bias, relu ? 1 : 0, U(y), xc, c1, tx, ty);
+        hipLaunchKernelGGL((k_conv_rows<1, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
+                           relu ? 1 : 0, U(y), xc, c1, tx, ty);
     else
-        hipLaunchKernelGGL((k_conv_rows<2, kRowsRY, kRowsRZ, false, false>), dim3(blocks), dim3(256), 0, st, g,
-                           U(x), cin, pk, chunks, bias, relu ? 1 : 0, U(y), xc, c1, tx, ty);
+        hipLaunchKernelGGL((k_conv_rows<2, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
+                           relu ? 1 : 0, U(y), xc, c1, tx, ty);
 }
 
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,
@@ -1165,7 +1170,11 @@ void launch_relu_bwd(hipStream_t st, int64_t n, const bf16 *y, bf16 *dy) {
 }
 
 void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out) {
-    hipLaunchKernelGGL(k_pool, dim3(grid1(coarse.vox() * c, 256)), dim3(256), 0, st, coarse, c, U(fine), U(out));
+    if (c % 8 == 0)
+        hipLaunchKernelGGL(k_pool8, dim3(grid1(coarse.vox() * (c / 8), 256)), dim3(256), 0, st, coarse, c, U(fine),
+                           U(out));
+    else
+        hipLaunchKernelGGL(k_pool, dim3(grid1(coarse.vox() * c, 256)), dim3(256), 0, st, coarse, c, U(fine), U(out));
 }
 
 void launch_pool_bwd(hipStream_t st, Grid3 coarse, int c, const bf16 *coarse_grad, bf16 *fine_grad) {
