@@ -434,7 +434,7 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
     const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
     double rr = 0.0, rm = 0.0;
     constexpr int V = 16 / (int)sizeof(T);  // elements per 16-byte access
-    if constexpr (!SCALED) {
+    {
         // 16-byte vectors: V consecutive elements per thread (tile boundaries are
         // multiples of 4, the arrays 256-byte aligned; a remainder of fewer than V
         // elements at the end goes through the scalar loop); each element's arithmetic
@@ -456,7 +456,18 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
             st_stream<NTS>(rv, &rv4[e]);
 #pragma unroll
             for (int u = 0; u < V; ++u) rr += (double)rv[u] * (double)rv[u];
-            if constexpr (V == 4) {
+            if constexpr (SCALED) {
+                // max |r| / sig over the unknowns (sig > 0), sig read as double pairs
+                typedef double D2 __attribute__((ext_vector_type(2)));
+                const D2 *sg2 = reinterpret_cast<const D2 *>(sig) + e * (V / 2);
+#pragma unroll
+                for (int h = 0; h < V / 2; ++h) {
+                    const D2 sg = ld_stream<NTS>(&sg2[h]);
+#pragma unroll
+                    for (int u = 0; u < 2; ++u)
+                        if (sg[u] > 0.0) rm = fmax(rm, fabs((double)rv[2 * h + u]) / sg[u]);
+                }
+            } else if constexpr (V == 4) {
                 rm = fmax(rm, fmax(fmax(fabs((double)rv[0]), fabs((double)rv[1])),
                                    fmax(fabs((double)rv[2]), fabs((double)rv[3]))));
             } else {
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(NB) void k_axpy_norm(int64_t n, int tiles, int per,
     }
     {
         const int64_t first = (int64_t)t0 * NT;  // a block past the last tile has end <= first
-        const int64_t start = !SCALED ? (end / V) * V > first ? (end / V) * V : first : first;
+        const int64_t start = (end / V) * V > first ? (end / V) * V : first;
         for (int64_t e = start + threadIdx.x; e < end; e += NB) {
             x[e] += a * s[e];
             const T rv = r[e] - a * q[e];
@@ -840,7 +851,10 @@ template <class T>
 void launch_axpy_norm(hipStream_t st, const Grid &g, const double *sig, T *x, T *r, const T *s, const T *q,
                       Reduction red, KEvents ev) {
     const bool nts = streaming<T>(g);
-    if (sig)
+    if (sig && nts)
+        SHK_LAUNCH(ev, (k_axpy_norm<T, true, true>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s,
+                   q, red);
+    else if (sig)
         SHK_LAUNCH(ev, (k_axpy_norm<T, true, false>), dim3(g.blocks), dim3(NB), st, g.n, g.tiles, g.per, sig, x, r, s,
                    q, red);
     else if (nts)

@@ -78,7 +78,7 @@ inline int64_t conv_rows_packed_elems(int cin, int cout) {
 }
 void launch_conv_pack_rows(hipStream_t st, const float *w, int cin, int cout, bf16 *packed);
 // Whether launch_conv_rows takes a layer: 1 or 2 output tiles of 16 channels, cin a
-// multiple of 8 and at least 16, and at least 512 tiles of 16 x 8 x 4 voxels.
+// multiple of 8 and at least 16, and at least 512 tiles of 16 x 4 x 8 voxels.
 bool conv_rows_ok(Grid3 g, int cin, int cout);
 // y = act(conv(x) + bias) as launch_conv_fwd (coarse != null: the input is
 // [upsample(coarse) (c1 channels), x (cin - c1)] as in launch_conv_fwd_cat).

@@ -11,6 +11,8 @@
 // consecutive output channels of one voxel (one 8-byte store).
 #include "unet_kernels.h"
 
+#include <type_traits>
+
 #include "shk_common.h"
 
 namespace shk {
@@ -255,11 +257,24 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
     constexpr int HP = (HV + 11) / 16 * 16 + 4;  // >= HV, = 4 (mod 16)
     constexpr int NLD = (HV + 63) / 64;          // halo voxels per thread
     __shared__ uint4 lds[4 * HP];
+    // element index of every halo voxel in x (fine) and in xc (its parent), -1 outside
+    // the grid: computed once per block, read by every stage's loads
+    __shared__ int off_f[HV], off_c[HV];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int wy = w & 1, wz = w >> 1;
     const int t = xcd_slot((int)blockIdx.x, (int)gridDim.x);
     const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
+    {
+        const int hnx = g.nx >> 1, hny = g.ny >> 1;
+        for (int vox = threadIdx.x; vox < HV; vox += 256) {
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            const bool in = (unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz;
+            off_f[vox] = in ? (gz * g.ny + gy) * g.nx + gx : -1;
+            off_c[vox] = in ? ((gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1) : -1;
+        }
+    }
     f32x4 acc[RZ][RY][MT];
 #pragma unroll
     for (int oz = 0; oz < RZ; ++oz)
@@ -269,7 +284,6 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
             for (int mt = 0; mt < MT; ++mt) acc[oz][oy][mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     Frag a[9][MT];
     const int tgrp = threadIdx.x & 3, tv0 = threadIdx.x >> 2;
-    const int hnx = g.nx >> 1, hny = g.ny >> 1;
     uint4 st[NLD];
     int q = 0;
     for (int ch0 = 0; ch0 < cin; ch0 += 32) {
@@ -281,18 +295,15 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
         const bool coarse = xc && ch < c1;
         const uint16_t *base = coarse ? xc + ch : (xc ? x + (ch - c1) : x + ch);
         const int64_t stride = coarse ? c1 : (xc ? cin - c1 : cin);
+        if (ch0 == 0) __syncthreads();  // the offset tables are complete
+        const int *offs = coarse ? off_c : off_f;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) {
             const int vox = tv0 + 64 * i;
             st[i] = make_uint4(0u, 0u, 0u, 0u);
             if (!on || vox >= HV) continue;
-            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            if ((unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz) {
-                const int64_t vi = coarse ? ((int64_t)(gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1)
-                                          : ((int64_t)gz * g.ny + gy) * g.nx + gx;
-                st[i] = *reinterpret_cast<const uint4 *>(base + vi * stride);
-            }
+            const int o = offs[vox];
+            if (o >= 0) st[i] = *reinterpret_cast<const uint4 *>(base + (int64_t)o * stride);
         }
         __syncthreads();  // the previous stage's reads are done
 #pragma unroll
@@ -314,28 +325,36 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
             const int dx = valid ? div_groups(e, G) : 0;
             const int grp = valid ? e - dx * G : 0;
             const uint4 *lb = lds + grp * HP + ((wz * RZ) * HY + wy * RY) * HX + n + dx;
+            auto rows = [&](auto masked) {
 #pragma unroll
-            for (int iz = 0; iz < RZ + 2; ++iz)
+                for (int iz = 0; iz < RZ + 2; ++iz)
 #pragma unroll
-                for (int iy = 0; iy < RY + 2; ++iy) {
-                    Frag b;
-                    b.u = lb[(iz * HY + iy) * HX];
-                    if (!valid) b.u = make_uint4(0u, 0u, 0u, 0u);
+                    for (int iy = 0; iy < RY + 2; ++iy) {
+                        Frag b;
+                        b.u = lb[(iz * HY + iy) * HX];
+                        if (decltype(masked)::value && !valid) b.u = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-                    for (int dz = 0; dz < 3; ++dz) {
-                        const int oz = iz - dz;
-                        if (oz < 0 || oz >= RZ) continue;
+                        for (int dz = 0; dz < 3; ++dz) {
+                            const int oz = iz - dz;
+                            if (oz < 0 || oz >= RZ) continue;
 #pragma unroll
-                        for (int dy = 0; dy < 3; ++dy) {
-                            const int oy = iy - dy;
-                            if (oy < 0 || oy >= RY) continue;
+                            for (int dy = 0; dy < 3; ++dy) {
+                                const int oy = iy - dy;
+                                if (oy < 0 || oy >= RY) continue;
 #pragma unroll
-                            for (int mt = 0; mt < MT; ++mt)
-                                acc[oz][oy][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                                    a[dz * 3 + dy][mt].v, b.v, acc[oz][oy][mt], 0, 0, 0);
+                                for (int mt = 0; mt < MT; ++mt)
+                                    acc[oz][oy][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                                        a[dz * 3 + dy][mt].v, b.v, acc[oz][oy][mt], 0, 0, 0);
+                            }
                         }
                     }
-                }
+            };
+            // only a stage's last chunk can hold padding entries (zero weights): zero
+            // their inputs there, so that no Inf or NaN of a neighbour reaches a sum
+            if (4 * c + 4 <= 3 * G)
+                rows(std::false_type{});
+            else
+                rows(std::true_type{});
         }
     }
     // lane holds D[co = mt*16 + 4*kq + j][voxel n of row (oy, oz)]
@@ -1121,7 +1140,9 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
 }
 
 namespace {
-constexpr int kRowsRY = 4, kRowsRZ = 2;  // rows per wave in y and z: tiles of 16 x 8 x 4
+// rows per wave in y and z: tiles of 16 x 4 x 8 (measured 3% faster than 16 x 8 x 4 on
+// every layer; 16 x 4 x 4 and 16 x 8 x 2 tiles re-read more halo and were slower)
+constexpr int kRowsRY = 2, kRowsRZ = 4;
 }  // namespace
 
 void launch_conv_pack_rows(hipStream_t st, const float *w, int cin, int cout, bf16 *packed) {
@@ -1135,7 +1156,7 @@ bool conv_rows_ok(Grid3 g, int cin, int cout) {
     if (cin < 16 || cin % 8 || (cout != 16 && cout != 32)) return false;
     const int64_t tiles = (int64_t)((g.nx + 15) / 16) * ((g.ny + 2 * kRowsRY - 1) / (2 * kRowsRY)) *
                           ((g.nz + 2 * kRowsRZ - 1) / (2 * kRowsRZ));
-    return tiles >= 512 && tiles < (1ll << 31);
+    return tiles >= 512 && g.vox() < (1ll << 31);  // 32-bit voxel offsets
 }
 
 void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
