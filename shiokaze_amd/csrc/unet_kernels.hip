@@ -908,36 +908,110 @@ __global__ void k_prolong_bricks(Grid3 f, BrickView b, const float *__restrict__
     z[e] = (float)((double)mask[e] * scale * acc);
 }
 
+// One thread per 4 consecutive x cells of a brick row (float4 accesses), 128 threads per
+// brick, 4 bricks per 512-thread block.  Each cell's arithmetic is the dense stencil's:
+// z - sum of coupling * neighbour in the order x-, x+, y-, y+, z-, z+, missing
+// neighbours (no brick there) skipped, then the damped-Jacobi update in double.
 __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega, const float *__restrict__ r,
                                                        const float *__restrict__ mask, const float *__restrict__ cx,
                                                        const float *__restrict__ cy, const float *__restrict__ cz,
                                                        const float *__restrict__ zin, float *__restrict__ zout,
                                                        const float *__restrict__ q, double *__restrict__ part) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
     __shared__ double lds[8];
-    const int l = threadIdx.x;
-    const int i = l & 7, j = (l >> 3) & 7, k = l >> 6;
+    const int l = threadIdx.x & 127, sub = threadIdx.x >> 7;
+    const int h = l & 1, j = (l >> 1) & 7, k = l >> 4;
+    const int off = 4 * h + 8 * j + 64 * k;  // first of the thread's 4 cells in its brick
+    auto v4 = [](const float *p, int64_t e) { return *reinterpret_cast<const f4 *>(p + e); };
     double acc_zq = 0.0;
-    for (int slot = blockIdx.x; slot < b.n_active; slot += gridDim.x) {
+    for (int slot = blockIdx.x * 4 + sub; slot < b.n_active; slot += gridDim.x * 4) {
         const int *nb = b.nbr + 6 * slot;
-        const int64_t e = (int64_t)slot * 512 + l;
-        // neighbour elements (-1: no brick there; its cells are zero with zero couplings)
-        const int64_t exm = i > 0 ? e - 1 : (nb[0] >= 0 ? (int64_t)nb[0] * 512 + l + 7 : -1);
-        const int64_t exp = i < 7 ? e + 1 : (nb[1] >= 0 ? (int64_t)nb[1] * 512 + l - 7 : -1);
-        const int64_t eym = j > 0 ? e - 8 : (nb[2] >= 0 ? (int64_t)nb[2] * 512 + l + 56 : -1);
-        const int64_t eyp = j < 7 ? e + 8 : (nb[3] >= 0 ? (int64_t)nb[3] * 512 + l - 56 : -1);
-        const int64_t ezm = k > 0 ? e - 64 : (nb[4] >= 0 ? (int64_t)nb[4] * 512 + l + 448 : -1);
-        const int64_t ezp = k < 7 ? e + 64 : (nb[5] >= 0 ? (int64_t)nb[5] * 512 + l - 448 : -1);
-        // A z in the dense stencil's order: x-, x+, y-, y+, z-, z+
-        float acc = zin[e];
-        if (exm >= 0) acc -= cx[e] * zin[exm];
-        if (exp >= 0) acc -= cx[exp] * zin[exp];
-        if (eym >= 0) acc -= cy[e] * zin[eym];
-        if (eyp >= 0) acc -= cy[eyp] * zin[eyp];
-        if (ezm >= 0) acc -= cz[e] * zin[ezm];
-        if (ezp >= 0) acc -= cz[ezp] * zin[ezp];
-        const float zn = (float)((double)zin[e] + omega * (double)mask[e] * ((double)r[e] - (double)acc));
-        zout[e] = zn;
-        if (part) acc_zq += (double)zn * (double)q[e];
+        const int64_t e0 = (int64_t)slot * 512 + off;
+        const f4 z = v4(zin, e0), cxv = v4(cx, e0), cyv = v4(cy, e0), czv = v4(cz, e0);
+        // x: the neighbours outside the quad (the other quad, or the x neighbour brick)
+        bool hxm = true, hxp = true;
+        float zxm = 0.0f, zxp = 0.0f, cxp3 = 0.0f;
+        if (h == 1) {
+            zxm = zin[e0 - 1];
+        } else if (nb[0] >= 0) {
+            zxm = zin[(int64_t)nb[0] * 512 + 7 + 8 * j + 64 * k];
+        } else {
+            hxm = false;
+        }
+        if (h == 0) {
+            zxp = zin[e0 + 4];
+            cxp3 = cx[e0 + 4];
+        } else if (nb[1] >= 0) {
+            const int64_t e = (int64_t)nb[1] * 512 + 8 * j + 64 * k;
+            zxp = zin[e];
+            cxp3 = cx[e];
+        } else {
+            hxp = false;
+        }
+        // y and z: whole neighbouring rows
+        bool hym = true, hyp = true, hzm = true, hzp = true;
+        f4 zym = {0, 0, 0, 0}, zyp = {0, 0, 0, 0}, cyp = {0, 0, 0, 0};
+        f4 zzm = {0, 0, 0, 0}, zzp = {0, 0, 0, 0}, czp = {0, 0, 0, 0};
+        if (j > 0)
+            zym = v4(zin, e0 - 8);
+        else if (nb[2] >= 0)
+            zym = v4(zin, (int64_t)nb[2] * 512 + off + 56);
+        else
+            hym = false;
+        if (j < 7) {
+            zyp = v4(zin, e0 + 8);
+            cyp = v4(cy, e0 + 8);
+        } else if (nb[3] >= 0) {
+            const int64_t e = (int64_t)nb[3] * 512 + off - 56;
+            zyp = v4(zin, e);
+            cyp = v4(cy, e);
+        } else {
+            hyp = false;
+        }
+        if (k > 0)
+            zzm = v4(zin, e0 - 64);
+        else if (nb[4] >= 0)
+            zzm = v4(zin, (int64_t)nb[4] * 512 + off + 448);
+        else
+            hzm = false;
+        if (k < 7) {
+            zzp = v4(zin, e0 + 64);
+            czp = v4(cz, e0 + 64);
+        } else if (nb[5] >= 0) {
+            const int64_t e = (int64_t)nb[5] * 512 + off - 448;
+            zzp = v4(zin, e);
+            czp = v4(cz, e);
+        } else {
+            hzp = false;
+        }
+        const f4 rv = v4(r, e0), mv = v4(mask, e0);
+        f4 zn;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            // A z in the dense stencil's order: x-, x+, y-, y+, z-, z+
+            float acc = z[u];
+            if (u > 0) {
+                acc -= cxv[u] * z[u - 1];
+            } else if (hxm) {
+                acc -= cxv[0] * zxm;
+            }
+            if (u < 3) {
+                acc -= cxv[u + 1] * z[u + 1];
+            } else if (hxp) {
+                acc -= cxp3 * zxp;
+            }
+            if (hym) acc -= cyv[u] * zym[u];
+            if (hyp) acc -= cyp[u] * zyp[u];
+            if (hzm) acc -= czv[u] * zzm[u];
+            if (hzp) acc -= czp[u] * zzp[u];
+            zn[u] = (float)((double)z[u] + omega * (double)mv[u] * ((double)rv[u] - (double)acc));
+        }
+        *reinterpret_cast<f4 *>(zout + e0) = zn;
+        if (part) {
+            const f4 qv = v4(q, e0);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc_zq += (double)zn[u] * (double)qv[u];
+        }
     }
     if (part) {
         const double t = block_sum<512>(acc_zq, lds);
@@ -986,7 +1060,8 @@ void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float 
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
                          const float *q, double *part) {
-    const int blocks = b.n_active < 1024 ? (b.n_active > 0 ? b.n_active : 1) : 1024;
+    const int want = (b.n_active + 3) / 4;  // 4 bricks per block and pass
+    const int blocks = want < 1024 ? (want > 0 ? want : 1) : 1024;
     hipLaunchKernelGGL(k_smooth_bricks, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout, q,
                        part);
     return blocks;
