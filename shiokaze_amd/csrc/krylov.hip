@@ -194,8 +194,15 @@ __device__ inline V ld_stream(const V *p) {
         return *p;
 }
 
+// Brick order: with ilv == 0 workgroup g walks its own contiguous range of bricks (after
+// the XCD remap, so neighbouring bricks share an L2); with ilv != 0 the workgroups walk
+// the bricks together, wave w of workgroup g taking bricks 4g + w, 4g + w + 4G, ...
+// (G workgroups), so that at any time the whole chip works on about 4G consecutive
+// bricks (one z layer of bricks at 512^3) and a brick's y and z neighbours were read a
+// layer before or are read now: their halo planes come from the Infinity Cache, not
+// from HBM, when the vectors themselves do not fit in it.
 template <class T, class C, bool UNIT, int BX, bool NTS>
-__global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T *__restrict__ r,
+__global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, int ilv, const T *__restrict__ r,
                                                  const T *__restrict__ sin, T *__restrict__ sout,
                                                  T *__restrict__ q, Reduction red) {
     using B = BrickShape<BX>;
@@ -203,8 +210,15 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
     if (red.st->done) return;
-    int t0, t1;
-    chunk_range(op.n_active, per, t0, t1);
+    int t0, t1, step;
+    if (ilv) {
+        t0 = (int)blockIdx.x * (NB / kWave);
+        t1 = op.n_active;
+        step = (int)gridDim.x * (NB / kWave);
+    } else {
+        chunk_range(op.n_active, per, t0, t1);
+        step = NB / kWave;
+    }
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     const int i = lane % X, j = lane / X;
     // brick metadata (neighbour slots, uniform coupling) does not depend on beta: the
@@ -226,10 +240,10 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, const T
     if (!direction_step(red, lds, &bc, beta)) return;
     const T b = (T)beta;
     double acc = 0.0;
-    for (; slot < t1; slot += NB / kWave) {
+    for (; slot < t1; slot += step) {
         const int nxm = mnb[0], nxp = mnb[1], nym = mnb[2], nyp = mnb[3], nzm = mnb[4], nzp = mnb[5];
         const C kap = mkap, kdg = mkd;
-        const int nslot = slot + NB / kWave;
+        const int nslot = slot + step;
         if (nslot < t1) {
 #pragma unroll
             for (int f = 0; f < 6; ++f) mnb[f] = op.nbr[6 * nslot + f];
@@ -814,21 +828,21 @@ bool streaming(const Grid &g) {
 }
 
 template <class T, class C, bool NTS>
-void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out,
-                        T *q, Reduction red, KEvents ev) {
+void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, int ilv, const T *r, const T *s_in,
+                        T *s_out, T *q, Reduction red, KEvents ev) {
     if (op.diag) {
         if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
                        s_out, q, red);
         else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
                        s_out, q, red);
     } else {
         if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
                        s_out, q, red);
         else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, r, s_in,
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
                        s_out, q, red);
     }
 }
@@ -836,10 +850,11 @@ void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, con
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
                      Reduction red, KEvents ev) {
+    const int ilv = streaming<T>(g) ? 1 : 0;
     if (streaming<T>(g))
-        launch_dir_spmv_nt<T, C, true>(st, g, op, r, s_in, s_out, q, red, ev);
+        launch_dir_spmv_nt<T, C, true>(st, g, op, ilv, r, s_in, s_out, q, red, ev);
     else
-        launch_dir_spmv_nt<T, C, false>(st, g, op, r, s_in, s_out, q, red, ev);
+        launch_dir_spmv_nt<T, C, false>(st, g, op, ilv, r, s_in, s_out, q, red, ev);
 }
 
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
