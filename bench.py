@@ -47,10 +47,12 @@ def log(msg):
 
 
 # HBM bytes per brick cell of one launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, factors from a 256 MiB copy in the same run) on the pool+sphere scene at
-# 256^3, mixed precision, Jacobi: profiles/hbm_traffic_pool_sphere_256_mixed_jacobi.md.
-# Reported as roofline.traffic only for that workload.
-MEASURED_TRAFFIC_PER_CELL = {"spmv_dot": 26.11, "axpy_norm": 24.03}
+# WRITE_SIZE x 1.0, factors from a 256 MiB copy in the same run), mixed precision, Jacobi,
+# pool+sphere scene: at 256^3 profiles/hbm_traffic_pool_sphere_256_mixed_jacobi.md, at 512^3
+# profiles/krylov_pool_sphere_512_mixed_fp64.md.  Reported as roofline.traffic only for
+# those workloads.
+MEASURED_TRAFFIC_PER_CELL = {256: {"spmv_dot": 26.11, "axpy_norm": 24.03},
+                             512: {"spmv_dot": 25.42, "axpy_norm": 24.01}}
 
 
 def kernel_bytes_per_cell(kernel: str, precision: str, precond: str) -> int:
@@ -407,9 +409,9 @@ def main():
         if kernels:
             dom = max((k for k in kernels if "GBs" in kernels[k]), key=lambda k: kernels[k]["total_ms"])
             kd = kernels[dom]
-            measured = (args.scene == 1 and n == 256 and args.precision == "mixed"
+            measured = (args.scene == 1 and n in MEASURED_TRAFFIC_PER_CELL and args.precision == "mixed"
                         and args.preconditioner == "jacobi" and not args.set)
-            traffic = round(MEASURED_TRAFFIC_PER_CELL[dom] * n_cells) if measured else None
+            traffic = round(MEASURED_TRAFFIC_PER_CELL[n][dom] * n_cells) if measured else None
             roofline = {"bound": "hbm", "kernel": dom, "achieved": kd["GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": kd["frac"], "traffic": traffic,
                         "bytes_per_launch": kd["bytes_per_launch"], "avg_launch_us": kd["avg_us"]}

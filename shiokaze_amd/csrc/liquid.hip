@@ -523,12 +523,14 @@ __global__ __launch_bounds__(kNT) void k_constrain(Dims d, Split sp, const float
 // 0.0f) and the narrow-band flag (fluid at the departed face centre above -trim dx).
 // With fin != 0 it is the backward pass and combines the result straight into the
 // final MacCormack value (macadvection3.cpp:150-163).
-template <int DIM>
+// WENO is a template parameter: the trilinear instantiation keeps the register budget
+// (and occupancy) it had without the WENO code
+template <int DIM, bool WENO>
 __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, Split sp, double dx, double dt, CFaces src, float *out,
                                                          float *lo, float *hi, uint8_t *band, const float *fluid,
                                                          int trim, const float *fin_vin, const float *fin_lo,
                                                          const float *fin_hi, const uint8_t *fin_band,
-                                                         const float *fin_v0, float *fin_out, int weno) {
+                                                         const float *fin_v0, float *fin_out) {
 #pragma clang fp contract(off)
     const FS s = face_fs(d, DIM);
     const int64_t q = blockIdx.x * (int64_t)kNT + threadIdx.x;
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(kNT) void k_semi_lagrangian(Dims d, Split sp, doubl
         }
         // WENO=1: the value by WENO3 interpolation (macadvection3.cpp:85-86); the corners
         // above still bound the MacCormack limiter
-        value = weno ? (float)weno3(src.v[DIM], src.m[DIM], s, p) : acc;
+        value = WENO ? (float)weno3(src.v[DIM], src.m[DIM], s, p) : acc;
     }
     if (out) out[q] = value;
     if (lo) {
@@ -654,12 +656,13 @@ __device__ inline bool cell_velocity(const Dims &d, const CFaces &f, int i, int 
 // the level set before the step at the departure point.  Backward pass (fin_out !=
 // 0): the semi-Lagrangian value of q0 over -dt, combined into the final value.
 // Inactive cells of the output are left alone (they keep what they read).
+template <bool WENO>
 __global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, Split sp, double dx, double dt, const float *q, const uint8_t *mq,
                                                      CFaces vel, const float *fluid, int trim, float *out, float *lo,
                                                      float *hi, uint8_t *band, const float *fin_qin,
                                                      const float *fin_lo, const float *fin_hi,
                                                      const uint8_t *fin_band, const float *fin_q0, float *fin_out,
-                                                     int fill_inactive, float background, int weno) {
+                                                     int fill_inactive, float background) {
 #pragma clang fp contract(off)
     const FS cs{d.nx, d.ny, d.nz};
     const int64_t c = blockIdx.x * (int64_t)kNT + threadIdx.x;
@@ -689,7 +692,7 @@ __global__ __launch_bounds__(kNT) void k_advect_cell(Dims d, Split sp, double dx
             vals[n] = q[corner(cs, ci, cj, ck, n)];
             if (coef[n] != 0.0) acc = (float)((double)acc + (double)vals[n] * coef[n]);
         }
-        value = weno ? (float)weno3(q, nullptr, cs, p) : acc;  // WENO=1 (macadvection3.cpp:193-194)
+        value = WENO ? (float)weno3(q, nullptr, cs, p) : acc;  // WENO=1 (macadvection3.cpp:193-194)
     }
     if (out) out[c] = value;
     if (lo) {
@@ -1179,11 +1182,18 @@ int shk_advect_velocity(shk_ctx *c, double dt, float *u, float *v, float *w, con
         v0.m[a] = um[a];
     }
     const int trim = c->prm.trim_narrowband;
-    const int weno = c->prm.weno ? 1 : 0;
-#define SHK_SL(DIM, SRC, OUT, LO, HI, BAND, DT, FV, FL, FH, FB, F0, FO)                                            \
-    hipLaunchKernelGGL(k_semi_lagrangian<DIM>, dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, c->stream, d, \
-                       split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, FV, FL, FH, FB, F0, FO, \
-                       weno)
+    const bool weno = c->prm.weno;
+#define SHK_SL(DIM, SRC, OUT, LO, HI, BAND, DT, FV, FL, FH, FB, F0, FO)                                             \
+    do {                                                                                                            \
+        if (weno)                                                                                                   \
+            hipLaunchKernelGGL((k_semi_lagrangian<DIM, true>), dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, \
+                               c->stream, d, split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, \
+                               FV, FL, FH, FB, F0, FO);                                                            \
+        else                                                                                                        \
+            hipLaunchKernelGGL((k_semi_lagrangian<DIM, false>), dim3(blocks_for(face_fs(d, DIM).n())), dim3(kNT), 0, \
+                               c->stream, d, split_of(face_fs(d, DIM)), c->dx, DT, SRC, OUT, LO, HI, BAND, fluid, trim, \
+                               FV, FL, FH, FB, F0, FO);                                                            \
+    } while (0)
     // forward pass: v0 (the result itself without MacCormack) and the limiter data
     SHK_SL(0, vin, const_cast<float *>(v0.v[0]), mc ? lw->lo[0].p : nullptr, mc ? lw->hi[0].p : nullptr,
            mc ? lw->band[0].p : nullptr, dt, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -1227,15 +1237,15 @@ int shk_advect_levelset(shk_ctx *c, double dt, float *phi, const uint8_t *mphi, 
     }
     const CFaces vel{{u, v, w}, {mu, mv, mw}};
     const float *qin = lw->a[0].p;
-    hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, dt, qin, mphi, vel, qin,
-                       c->prm.trim_narrowband, mc ? lw->b[0].p : phi, mc ? lw->lo[0].p : nullptr,
+    auto kern = c->prm.weno ? k_advect_cell<true> : k_advect_cell<false>;
+    hipLaunchKernelGGL(kern, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, dt, qin, mphi, vel,
+                       qin, c->prm.trim_narrowband, mc ? lw->b[0].p : phi, mc ? lw->lo[0].p : nullptr,
                        mc ? lw->hi[0].p : nullptr, mc ? lw->band[0].p : nullptr, nullptr, nullptr, nullptr, nullptr,
-                       nullptr, nullptr, mc ? 1 : 0, background, c->prm.weno ? 1 : 0);
+                       nullptr, nullptr, mc ? 1 : 0, background);
     if (mc)
-        hipLaunchKernelGGL(k_advect_cell, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, -dt, lw->b[0].p,
-                           mphi, vel, qin, c->prm.trim_narrowband, nullptr, nullptr, nullptr, nullptr, qin,
-                           lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background,
-                           c->prm.weno ? 1 : 0);
+        hipLaunchKernelGGL(kern, dim3(blocks_for(G)), dim3(kNT), 0, c->stream, d, cells_split(d), c->dx, -dt,
+                           lw->b[0].p, mphi, vel, qin, c->prm.trim_narrowband, nullptr, nullptr, nullptr, nullptr, qin,
+                           lw->lo[0].p, lw->hi[0].p, lw->band[0].p, lw->b[0].p, phi, 0, background);
     HIPC(c, hipGetLastError());
     HIPC(c, hipStreamSynchronize(c->stream));
     return SHK_OK;
