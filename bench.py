@@ -87,9 +87,11 @@ LIQUID_BYTES_PER_CELL = {
 
 
 def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
-    """Times the velocity half of macliquid3's step with the projection at the bench
-    size on the bench scene, stage by stage (each library call is synchronous), and
-    the CPU oracle of the same stages on a (up to) 128^3 sample of the same scene."""
+    """Times macliquid3's step at the bench size on the bench scene, stage by stage (each
+    library call is synchronous): the velocity half, the level-set tracker (advection,
+    redistancing to width 3, extrapolation into the solid; maclevelsetsurfacetracker3.cpp:
+    46-62) and the projection; and the CPU oracle of the same stages on a (up to) 128^3
+    sample of the same scene."""
     n = args.n
     shape = (n, n, n)
     ps = shk.MacPressureSolver3(shape, sc["dx"], config=cfg)
@@ -105,8 +107,11 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
     bw = 3.0 * ps.dx
     band = (dev["fluid"].abs() < bw).to(torch.uint8)
     phi0 = dev["fluid"].clamp(-bw, bw)
-    stages = {"max_u": 0.0, "extrapolate": 0.0, "constrain": 0.0, "advect_levelset": 0.0, "advect": 0.0,
-              "gravity": 0.0, "project": 0.0}
+    stages = {"max_u": 0.0, "extrapolate": 0.0, "constrain": 0.0, "advect_levelset": 0.0, "redistance": 0.0,
+              "extrapolate_levelset": 0.0, "advect": 0.0, "gravity": 0.0, "project": 0.0}
+    # the solid's node band as the reference keeps it (|phi_s| < 3 dx, -/+ 3 dx off it)
+    sband = (dev["solid"].abs() < bw).to(torch.uint8)
+    solid_b = torch.where(sband > 0, dev["solid"], torch.where(dev["solid"] < 0, -bw, bw)).contiguous()
     widths = []
     torch.cuda.synchronize()
     t_all = time.perf_counter()
@@ -126,8 +131,15 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
         phi = phi0.clone()
         torch.cuda.synchronize()
         t = time.perf_counter()
-        lv.advect_levelset(dt, phi, band, vel, masks, bw)
+        b = band.clone()
+        lv.advect_levelset(dt, phi, b, vel, masks, bw)
         stages["advect_levelset"] += time.perf_counter() - t
+        t = time.perf_counter()
+        lv.redistance(phi, b, 3)
+        stages["redistance"] += time.perf_counter() - t
+        t = time.perf_counter()
+        lv.extrapolate_levelset(solid_b, phi, b, bw, solid_band=sband)
+        stages["extrapolate_levelset"] += time.perf_counter() - t
         t = time.perf_counter()
         lv.advect(dt, vel, masks, dev["fluid"])
         stages["advect"] += time.perf_counter() - t
@@ -140,8 +152,11 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
         stages["project"] += time.perf_counter() - t
     total = time.perf_counter() - t_all
     ps.close()
-    velocity_half = sum(v for k, v in stages.items() if k != "project")
+    tracker_keys = ("advect_levelset", "redistance", "extrapolate_levelset")
+    tracker = sum(stages[k] for k in tracker_keys)
+    velocity_half = sum(v for k, v in stages.items() if k != "project" and k not in tracker_keys)
     out = {"steps_per_s": round(steps / total, 3), "velocity_half_ms": round(1e3 * velocity_half / steps, 3),
+           "levelset_tracker_ms": round(1e3 * tracker / steps, 3),
            "stage_ms": {k: round(1e3 * v / steps, 3) for k, v in stages.items()}, "extend_widths": widths,
            "last_projection_iters": res["count"], "preconditioner": cfg.get("Preconditioner"),
            "advect_algorithmic_bytes": int((LIQUID_BYTES_PER_CELL["k_semi_lagrangian_fwd"]
@@ -166,22 +181,30 @@ def liquid_leg(args, torch, shk, S, sc, dev, cfg, steps=3):
         tc["constrain"] = time.perf_counter() - t
         bws = 3.0 * small["dx"]
         t = time.perf_counter()
-        O.advect_levelset(small["dx"], 0.5 * small["dx"], np.clip(small["fluid"], -bws, bws),
-                          (np.abs(small["fluid"]) < bws).astype(np.uint8), v, msk, bws)
+        sb = (np.abs(small["fluid"]) < bws).astype(np.uint8)
+        ph = O.advect_levelset(small["dx"], 0.5 * small["dx"], np.clip(small["fluid"], -bws, bws), sb, v, msk, bws)
         tc["advect_levelset"] = time.perf_counter() - t
+        t = time.perf_counter()
+        ph, sb, _ = O.redistance(small["dx"], ph, sb, 3)
+        tc["redistance"] = time.perf_counter() - t
+        snb = (np.abs(small["solid"]) < bws).astype(np.uint8)
+        sol = np.where(snb > 0, small["solid"], np.where(small["solid"] < 0, -bws, bws)).astype(np.float32)
+        t = time.perf_counter()
+        O.extrapolate_levelset(small["dx"], sol, ph, sb, bws, solid_band=snb)
+        tc["extrapolate_levelset"] = time.perf_counter() - t
         t = time.perf_counter()
         v = O.advect_velocity(small["dx"], 0.5 * small["dx"], v, msk, small["fluid"])
         tc["advect"] = time.perf_counter() - t
         t = time.perf_counter()
         O.add_gravity(0.001, (0.0, -9.8, 0.0), v, msk)
         tc["gravity"] = time.perf_counter() - t
-        cpu_half = sum(tc.values())
+        cpu_all = sum(tc.values())
         scale = (n / m) ** 3
         out["cpu_oracle"] = {"sample": f"{m}^3 scene {args.scene}, every stage once, one host core",
-                             "velocity_half_ms_sample": round(1e3 * cpu_half, 2),
-                             "velocity_half_ms_extrapolated_to_n": round(1e3 * cpu_half * scale, 1),
+                             "stages_ms_sample": round(1e3 * cpu_all, 2),
+                             "stages_ms_extrapolated_to_n": round(1e3 * cpu_all * scale, 1),
                              "stage_ms_sample": {k: round(1e3 * x, 2) for k, x in tc.items()}}
-        out["velocity_half_gpu_vs_cpu"] = round(cpu_half * scale / (velocity_half / steps), 1)
+        out["stages_gpu_vs_cpu"] = round(cpu_all * scale / ((velocity_half + tracker) / steps), 1)
     return out
 
 
