@@ -343,8 +343,9 @@ int shk_nn_conv3d(int nx, int ny, int nz, const void *x, int cin, const float *w
 /* The same convolution on the input [upsample(coarse), x] (coarse: the half-size grid
  * with c1 channels, nearest upsampling; x: cin - c1 channels; coarse = NULL and c1 = 0:
  * x alone with cin channels), as the decoder layers read it, by kernel 0 (the choice the
- * network's inference makes for this grid), 1 (the tiled kernel the training uses) or
- * 2 (the row-reuse kernel; cout 16 or 32). */
+ * network's inference makes for this grid), 1 (the tiled kernel the training uses),
+ * 2 (the row-reuse kernel; cout 16 or 32) or 3 (the direct-load kernel of the small
+ * grids). */
 int shk_nn_conv3d_cat(int nx, int ny, int nz, const void *coarse, int c1, const void *x, int cin, const float *w,
                       const float *bias, int cout, int relu, int kernel, void *y);
 /* dx = input gradient (BF16, NULL to skip), dw / db += weight / bias gradients (FP32). */

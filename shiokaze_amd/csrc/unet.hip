@@ -311,10 +311,15 @@ void forward(UnetNet *n, bool train) {
     const int64_t wc = train ? 0 : n->split_cap;
     // inference runs the row-reuse kernel where a layer's grid fills the chip
     auto rows = [&](int l, Grid3 g) { return !train && n->pkr[l] && conv_rows_ok(g, kSpec[l].cin, kSpec[l].cout); };
+    // and the direct-load kernel on the grids too small for the tiled kernel
+    auto direct = [&](int l, Grid3 g) { return !train && conv_direct_ok(g, kSpec[l].cin, kSpec[l].cout); };
     auto enc = [&](int l, Grid3 g, const bf16 *x, bf16 *y) {
         if (rows(l, g))
             launch_conv_rows(n->stream, g, x, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
                              kSpec[l].relu, y);
+        else if (direct(l, g))
+            launch_conv_direct(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
+                               kSpec[l].relu, y);
         else
             launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
                             kSpec[l].relu, y, nullptr, ws, wc);
@@ -331,6 +336,9 @@ void forward(UnetNet *n, bool train) {
             if (rows(l, g))
                 launch_conv_rows(n->stream, g, skip, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
                                  kSpec[l].relu, y, coarse, c1);
+            else if (direct(l, g))
+                launch_conv_direct(n->stream, g, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
+                                   kSpec[l].cout, kSpec[l].relu, y, coarse, c1);
             else
                 launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
                                     kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap);
@@ -1079,13 +1087,16 @@ int shk_nn_conv3d_cat(int nx, int ny, int nz, const void *coarse, int c1, const 
                       const float *bias, int cout, int relu, int kernel, void *y) {
     const int mt = cout / 16;
     if (!x || !w || !y || nx <= 0 || ny <= 0 || nz <= 0 || cin <= 0 || cin % 8 || cout <= 0 || cout % 16 ||
-        kernel < 0 || kernel > 2 || (coarse && (c1 <= 0 || c1 % 8 || c1 >= cin || nx % 2 || ny % 2 || nz % 2)) ||
+        kernel < 0 || kernel > 3 || (coarse && (c1 <= 0 || c1 % 8 || c1 >= cin || nx % 2 || ny % 2 || nz % 2)) ||
         (!coarse && c1 != 0))
         return SHK_BAD_ARG;
     const Grid3 g{nx, ny, nz};
     const bool rows = kernel == 2 || (kernel == 0 && conv_rows_ok(g, cin, cout));
+    const bool direct = !rows && (kernel == 3 || (kernel == 0 && conv_direct_ok(g, cin, cout)));
     if (rows && cout != 16 && cout != 32) return SHK_BAD_ARG;
-    if (!rows && mt != 1 && mt != 2 && mt != 3 && mt != 4 && mt != 6 && mt != 8 && mt != 12) return SHK_BAD_ARG;
+    if (!rows && !direct && mt != 1 && mt != 2 && mt != 3 && mt != 4 && mt != 6 && mt != 8 && mt != 12)
+        return SHK_BAD_ARG;
+    if (direct && g.vox() >= (1ll << 31)) return SHK_BAD_ARG;
     const int64_t elems = rows ? conv_rows_packed_elems(cin, cout) : conv_packed_elems(cin, cout);
     bf16 *pk = nullptr;
     if (hipMalloc((void **)&pk, sizeof(bf16) * elems) != hipSuccess) return SHK_OUT_OF_MEMORY;
@@ -1095,6 +1106,9 @@ int shk_nn_conv3d_cat(int nx, int ny, int nz, const void *coarse, int c1, const 
     if (rows) {
         launch_conv_pack_rows(nullptr, w, cin, cout, pk);
         launch_conv_rows(nullptr, g, xx, cin, pk, bias, cout, relu != 0, yy, xc, c1);
+    } else if (direct) {
+        launch_conv_pack(nullptr, w, cin, cout, false, pk);
+        launch_conv_direct(nullptr, g, xx, cin, pk, bias, cout, relu != 0, yy, xc, c1);
     } else {
         launch_conv_pack(nullptr, w, cin, cout, false, pk);
         if (xc)

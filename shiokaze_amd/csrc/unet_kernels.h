@@ -57,6 +57,14 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
                          const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws = nullptr,
                          int64_t ws_floats = 0);
 
+// Direct-load forward convolution for grids too small for the tiled kernel to fill the
+// chip (the grids on which it splits K over blocks): same packed weights as
+// launch_conv_fwd, K split over the waves of a block and added in LDS, BF16 output
+// (coarse != null: the input is [upsample(coarse) (c1 channels), x (cin - c1)]).
+bool conv_direct_ok(Grid3 g, int cin, int cout);
+void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
+                        int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0);
+
 // Row-reuse forward convolution for grids that fill the chip (inference).  The
 // reduction runs by 32-channel stages of cin (the last one 8, 16 or 24 wide: G groups
 // of 8 channels); within a stage, for each (dz, dy), the 3G entries (dx, group) in the

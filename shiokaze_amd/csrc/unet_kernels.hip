@@ -223,6 +223,126 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     }
 }
 
+// ---------------------------------------------------------------- direct forward (small grids)
+
+// For grids of 8^3 to 32^3, whose tiles cannot fill the chip: no LDS staging, and the
+// K loop split over the waves of one block.  Block: S = blockDim.x / 64 waves, all on the
+// same outputs, MW of the output tiles of 16 channels (blockIdx.y selects which) times
+// NW N-tiles of 16 consecutive voxels in grid order (blockIdx.x, XCD-grouped so that
+// neighbouring voxels, which share their inputs, run on one L2).  Wave w takes the
+// weight chunks [w C / S, (w+1) C / S) of the C = stages x KPC chunks of the packed
+// A fragments of launch_conv_pack (chunk (stage, kc), quarter kq: tap = k / CC and
+// channel stage*CC + k % CC for k = 32 kc + 8 kq), and loads each B fragment (8
+// channels of one shifted voxel per lane, one 16-byte load) straight from global
+// memory, PF chunks ahead of its MFMAs.  The waves' FP32 partial sums meet in LDS and
+// are added in wave order; bias, ReLU and the BF16 store follow.
+template <int MW, int NW>
+constexpr int direct_max_threads() {
+    return 64 * (64 / (MW * NW) < 16 ? 64 / (MW * NW) : 16);
+}
+
+template <int MW, int NW, int CC>
+__global__ __launch_bounds__((direct_max_threads<MW, NW>())) void k_conv_direct(
+    Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total, const uint4 *__restrict__ packed,
+    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y, const uint16_t *__restrict__ xc, int c1) {
+    constexpr int KPC = (27 * CC + 31) / 32;
+    constexpr int PF = MW + NW <= 2 ? 4 : (MW + NW <= 4 ? 3 : 2);
+    constexpr int F = MW * NW;
+    extern __shared__ f32x4 red[];  // [S][F][64]
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, S = blockDim.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int nb = xcd_slot((int)blockIdx.x, (int)gridDim.x), mg = blockIdx.y;
+    const int nxy = g.nx * g.ny, vox = nxy * g.nz;
+    const int kchunks = (cin / CC) * KPC;
+    const int lo = (int)((int64_t)w * kchunks / S), hi = (int)((int64_t)(w + 1) * kchunks / S);
+    // this lane's voxel in each N-tile
+    int vx[NW], vy[NW], vz[NW], vv[NW];
+#pragma unroll
+    for (int t = 0; t < NW; ++t) {
+        const int v = (nb * NW + t) * 16 + n;
+        vv[t] = v < vox ? v : -1;
+        const int vc = v < vox ? v : 0;
+        vz[t] = vc / nxy;
+        vy[t] = (vc - vz[t] * nxy) / g.nx;
+        vx[t] = vc - vz[t] * nxy - vy[t] * g.nx;
+    }
+    const int hnx = g.nx >> 1, hny = g.ny >> 1;
+    const int fstride = xc ? cin - c1 : cin;
+    const uint4 *abase = packed + (int64_t)(mg * MW) * kchunks * 64 + lane;
+    auto load = [&](int gc, Frag *a, Frag *b) {
+#pragma unroll
+        for (int m = 0; m < MW; ++m) a[m].u = abase[((int64_t)m * kchunks + gc) * 64];
+        const int stage = gc / KPC, k = (gc - stage * KPC) * 32 + kq * 8;
+        const int tap = k / CC, c = stage * CC + (k - tap * CC);
+        const bool kin = k < 27 * CC;  // padding entries of a stage's last chunk: zero
+        int dx, dy, dz;
+        tap_offset(kin ? tap : 13, dx, dy, dz);
+        const bool coarse = xc && c < c1;
+#pragma unroll
+        for (int t = 0; t < NW; ++t) {
+            const int gx = vx[t] + dx, gy = vy[t] + dy, gz = vz[t] + dz;
+            b[t].u = make_uint4(0u, 0u, 0u, 0u);
+            if (kin && vv[t] >= 0 && (unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny &&
+                (unsigned)gz < (unsigned)g.nz) {
+                const uint16_t *src =
+                    coarse ? xc + (int64_t)(((gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1)) * c1 + c
+                           : x + (int64_t)(vv[t] + dz * nxy + dy * g.nx + dx) * fstride + (c - (xc ? c1 : 0));
+                b[t].u = *reinterpret_cast<const uint4 *>(src);
+            }
+        }
+    };
+    f32x4 acc[MW][NW];
+#pragma unroll
+    for (int m = 0; m < MW; ++m)
+#pragma unroll
+        for (int t = 0; t < NW; ++t) acc[m][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    Frag fa[PF][MW], fb[PF][NW];
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+        if (lo + p < hi) load(lo + p, fa[p], fb[p]);
+    for (int base = lo; base < hi; base += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            const int gc = base + p;
+            if (gc >= hi) break;  // wave-uniform
+#pragma unroll
+            for (int m = 0; m < MW; ++m)
+#pragma unroll
+                for (int t = 0; t < NW; ++t)
+                    acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[p][m].v, fb[p][t].v, acc[m][t], 0, 0, 0);
+            if (gc + PF < hi) load(gc + PF, fa[p], fb[p]);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MW; ++m)
+#pragma unroll
+        for (int t = 0; t < NW; ++t) red[(w * F + m * NW + t) * 64 + lane] = acc[m][t];
+    __syncthreads();
+    // lane holds D[co = (mg*MW + m)*16 + 4*kq + j][voxel n of N-tile t]
+    const int cout = mt_total * 16;
+    for (int f = w; f < F; f += S) {
+        const int m = f / NW, t = f - m * NW;
+        f32x4 s = red[f * 64 + lane];
+        for (int q = 1; q < S; ++q) s += red[(q * F + f) * 64 + lane];
+        int v = -1;
+#pragma unroll
+        for (int u = 0; u < NW; ++u)
+            if (u == t) v = vv[u];
+        if (v < 0) continue;
+        const int co = (mg * MW + m) * 16 + 4 * kq;
+        float o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            o[j] = s[j] + (bias ? bias[co + j] : 0.0f);
+            if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
+        }
+        uint2 po;
+        po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+        po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+        *reinterpret_cast<uint2 *>(y + (int64_t)v * cout + co) = po;
+    }
+}
+
 // ---------------------------------------------------------------- row-reuse forward
 
 // i / g for the 1..4 channel groups of a stage, without an integer division
@@ -1202,6 +1322,83 @@ void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *p
     }
 }
 }  // namespace
+
+namespace {
+struct DirectPlan {
+    int mw, nw, s;
+};
+
+// Output tile per block: the largest of 4x4, 4x2, 2x4, 2x2, 4x1, 2x1, 1x1 (output tiles x
+// N-tiles) that still gives >= 256 blocks; waves per block: enough for about 4096 waves
+// on the chip, at least 2 weight chunks per wave, at most 64 KB of partial sums in LDS
+// (waves x tile <= 64) and 16 waves.
+DirectPlan direct_plan(Grid3 g, int cin, int cout) {
+    const int mt = cout / 16;
+    const int64_t ntiles = (g.vox() + 15) / 16;
+    static const int cand[7][2] = {{4, 4}, {4, 2}, {2, 4}, {2, 2}, {4, 1}, {2, 1}, {1, 1}};
+    DirectPlan p{1, 1, 1};
+    for (const auto &c : cand) {
+        if (mt % c[0]) continue;
+        p.mw = c[0];
+        p.nw = c[1];
+        if ((ntiles + c[1] - 1) / c[1] * (mt / c[0]) >= 256) break;
+    }
+    const int64_t blocks = (ntiles + p.nw - 1) / p.nw * (mt / p.mw);
+    const int chunks = (cin / conv_cc(cin)) * conv_kpc(cin);
+    const int cap = std::min(16, 64 / (p.mw * p.nw));
+    while (p.s < cap && blocks * p.s < 4096 && 2 * p.s * 2 <= chunks) p.s *= 2;
+    return p;
+}
+
+template <int MW, int NW>
+void conv_direct_mw(hipStream_t st, const DirectPlan &p, Grid3 g, const uint16_t *x, int cin, int mt,
+                    const uint4 *pk, const float *bias, int relu, uint16_t *y, const uint16_t *xc, int c1) {
+    const dim3 grid((unsigned)((g.vox() + 16 * NW - 1) / (16 * NW)), (unsigned)(mt / MW));
+    const dim3 block((unsigned)(64 * p.s));
+    const size_t lds = sizeof(float) * 4 * 64 * MW * NW * p.s;
+    switch (conv_cc(cin)) {
+        case 8:
+            hipLaunchKernelGGL((k_conv_direct<MW, NW, 8>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y, xc,
+                               c1);
+            break;
+        case 16:
+            hipLaunchKernelGGL((k_conv_direct<MW, NW, 16>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
+                               xc, c1);
+            break;
+        case 24:
+            hipLaunchKernelGGL((k_conv_direct<MW, NW, 24>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
+                               xc, c1);
+            break;
+        default:
+            hipLaunchKernelGGL((k_conv_direct<MW, NW, 32>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
+                               xc, c1);
+            break;
+    }
+}
+}  // namespace
+
+bool conv_direct_ok(Grid3 g, int cin, int cout) {
+    if (cin % 8 || cout % 16 || cout <= 0 || g.vox() >= (1ll << 31)) return false;
+    return conv_plan(g, cin, cout).nsplit > 1;  // the tiled kernel cannot fill the chip
+}
+
+void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
+                        int cout, bool relu, bf16 *y, const bf16 *coarse, int c1) {
+    const DirectPlan p = direct_plan(g, cin, cout);
+    const int mt = cout / 16, r = relu ? 1 : 0;
+    const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
+    const uint16_t *xc = coarse ? U(coarse) : nullptr;
+    const int key = p.mw * 8 + p.nw;
+    switch (key) {
+        case 4 * 8 + 4: conv_direct_mw<4, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 4 * 8 + 2: conv_direct_mw<4, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 2 * 8 + 4: conv_direct_mw<2, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 2 * 8 + 2: conv_direct_mw<2, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 4 * 8 + 1: conv_direct_mw<4, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 2 * 8 + 1: conv_direct_mw<2, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        default: conv_direct_mw<1, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+    }
+}
 
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
                      bool relu, bf16 *y, float *out_f32, float *ws, int64_t ws_floats) {

@@ -153,7 +153,8 @@ def conv3d_cat(coarse: torch.Tensor | None, x: torch.Tensor, w: torch.Tensor, bi
                relu: bool, kernel: int = 0) -> torch.Tensor:
     """conv3d of the concatenation [upsample(coarse), x] (coarse (nz/2, ny/2, nx/2, c1)
     bf16 or None) by kernel 0 (the network's inference choice for this grid), 1 (the
-    tiled kernel) or 2 (the row-reuse kernel, 16 or 32 outputs)."""
+    tiled kernel), 2 (the row-reuse kernel, 16 or 32 outputs) or 3 (the direct-load
+    kernel of the small grids)."""
     lib = _lib.load()
     nz, ny, nx, cx = x.shape
     c1 = 0 if coarse is None else coarse.shape[3]

@@ -73,12 +73,40 @@ CAT_CASES = [  # (nz, ny, nx), c1 (upsampled coarse channels), cin, cout, relu
 ]
 
 
+# the direct-load kernel on the network's small-grid layers and on ragged shapes:
+# cout of 48 (3 output tiles), 64 and 128; stages of 8, 16, 24 and 32 channels; the
+# last N-tile partly outside the grid
+DIRECT_CASES = [  # (nz, ny, nx), c1, cin, cout, relu
+    ((16, 16, 16), 0, 16, 32, True),     # encoder level 1 at 16^3
+    ((8, 8, 8), 0, 32, 64, True),        # encoder level 2 at 8^3
+    ((8, 8, 8), 0, 64, 128, True),       # bottleneck at 8^3
+    ((4, 4, 4), 0, 64, 128, True),       # bottleneck at 4^3 (4 N-tiles)
+    ((16, 16, 16), 128, 192, 64, True),  # decoder level 2 at 16^3
+    ((32, 32, 32), 64, 96, 32, True),    # decoder level 1 at 32^3
+    ((5, 6, 7), 0, 8, 16, True),         # 210 voxels: a partial N-tile
+    ((3, 5, 4), 0, 32, 48, False),
+    ((4, 9, 6), 0, 48, 16, False),       # stages of 24 channels
+    ((8, 8, 16), 0, 40, 32, True),       # stages of 8 channels
+]
+
+
+@pytest.mark.parametrize("shape,c1,cin,cout,relu", CAT_CASES[:4] + DIRECT_CASES)
+def test_conv3d_direct_kernel(shape, c1, cin, cout, relu):
+    """The direct-load kernel (K split over the waves of a block, partial sums added in
+    LDS) against the oracle, on concatenated and plain inputs."""
+    _check_conv_cat(shape, c1, cin, cout, relu, 3)
+
+
 @pytest.mark.parametrize("shape,c1,cin,cout,relu", CAT_CASES)
 @pytest.mark.parametrize("kernel", [1, 2])
 def test_conv3d_concatenated_input(shape, c1, cin, cout, relu, kernel):
     """Both convolution kernels (the tiled one and the row-reuse one) on the decoder's
     input [nearest upsampling of a half-size grid, skip] against the oracle's
     convolution of the explicitly concatenated FP64 array."""
+    _check_conv_cat(shape, c1, cin, cout, relu, kernel)
+
+
+def _check_conv_cat(shape, c1, cin, cout, relu, kernel):
     from shiokaze_amd import learned
     log(f"[gpu] conv3d cat {shape} c1={c1} {cin}->{cout} relu={relu} kernel={kernel}")
     rng = np.random.default_rng(cin * 100 + cout + c1)
@@ -107,13 +135,14 @@ def test_conv3d_concatenated_input(shape, c1, cin, cout, relu, kernel):
 
 def test_conv3d_kernel_choice():
     """Kernel 0 takes the row-reuse kernel on a grid of >= 512 tiles (bit-identical to
-    kernel 2) and the tiled kernel below (bit-identical to kernel 1)."""
+    kernel 2), the direct-load kernel where the tiled kernel would have fewer than 1024
+    blocks (bit-identical to kernel 3) and the tiled kernel otherwise (kernel 1)."""
     import torch
     from shiokaze_amd import learned
     rng = np.random.default_rng(7)
-    w = _dev((rng.standard_normal((27, 16, 16)) * 0.1).astype(np.float32))
-    for shape, pick in (((64, 64, 64), 2), ((32, 32, 32), 1)):
-        x = _bf16_tensor(rng.standard_normal(shape + (16,)))
+    for shape, cin, pick in (((64, 64, 64), 16, 2), ((32, 32, 32), 16, 3), ((64, 64, 64), 8, 1)):
+        w = _dev((rng.standard_normal((27, cin, 16)) * 0.1).astype(np.float32))
+        x = _bf16_tensor(rng.standard_normal(shape + (cin,)))
         y0 = learned.conv3d_cat(None, x, w, None, True, 0)
         yp = learned.conv3d_cat(None, x, w, None, True, pick)
         assert torch.equal(y0, yp), shape
