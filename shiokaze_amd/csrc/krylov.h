@@ -105,12 +105,17 @@ void launch_residual_csr(hipStream_t st, const Grid &g, const CsrOp &op, const d
 //   axpy: alpha = p.r / p.q; x += alpha p; r -= alpha q; partials r.r and max|r|
 //         (norm_only: the partials of r alone);
 //   dir : beta = -(z.q) / (p.q) from nz partials of z.q; p = z + beta p (first: p = z).
+// With fs: nothing happens once fs->stop is set (the stop rules are applied between
+// the axpy and the next application of the network: fcg_decide).
 void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
-                     float *q, double *part);
+                     float *q, double *part, const FcgState *fs = nullptr);
 void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
-                     const double *part1, double *part2, double *fst, bool norm_only);
+                     const double *part1, double *part2, double *fst, bool norm_only,
+                     const FcgState *fs = nullptr);
 void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
-                    const double *fst, bool first);
+                    const double *fst, bool first, const FcgState *fs = nullptr);
+// *fs = v (one thread: the value travels as a kernel argument)
+void launch_fcg_state_init(hipStream_t st, FcgState *fs, const FcgState &v);
 
 // x64 += sig * (double) d32 (sig may be nullptr): the refinement correction.
 void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d, const double *sig);

@@ -702,8 +702,9 @@ __global__ __launch_bounds__(NT) void k_csr_scale(CsrOp op, const double *__rest
 // q = A p; part[b] = p.q, part[gridDim + b] = p.r over the block's bricks.
 __global__ __launch_bounds__(NT) void k_fcg_spmv(BrickOp<float> op, int per, const float *__restrict__ p,
                                                  const float *__restrict__ r, float *__restrict__ q,
-                                                 double *__restrict__ part) {
+                                                 double *__restrict__ part, const FcgState *__restrict__ fs) {
     __shared__ double lds[NT / kWave];
+    if (fs && fs->stop) return;
     int t0, t1;
     chunk_range(op.n_active, per, t0, t1);
     const int l = threadIdx.x;
@@ -728,9 +729,11 @@ __global__ __launch_bounds__(NT) void k_fcg_spmv(BrickOp<float> op, int per, con
 __global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, float *__restrict__ x,
                                                  float *__restrict__ r, const float *__restrict__ p,
                                                  const float *__restrict__ q, const double *__restrict__ part1,
-                                                 double *__restrict__ part2, double *__restrict__ fst, int norm_only) {
+                                                 double *__restrict__ part2, double *__restrict__ fst, int norm_only,
+                                                 const FcgState *__restrict__ fs) {
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
+    if (fs && fs->stop) return;
     float a = 0.0f;
     if (!norm_only) {
         const double delta = fold_sum<NB>(part1, gridDim.x, lds, &bc);
@@ -770,9 +773,11 @@ __global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, 
 // p = z + beta p (first: p = z).
 __global__ __launch_bounds__(NB) void k_fcg_dir(int64_t n, int tiles, int per, const float *__restrict__ z,
                                                 float *__restrict__ p, const double *__restrict__ zq_part, int nz,
-                                                const double *__restrict__ fst, int first) {
+                                                const double *__restrict__ fst, int first,
+                                                const FcgState *__restrict__ fs) {
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
+    if (fs && fs->stop) return;
     float b = 0.0f;
     if (!first) {
         const double zq = fold_sum<NB>(zq_part, nz, lds, &bc);
@@ -782,6 +787,10 @@ __global__ __launch_bounds__(NB) void k_fcg_dir(int64_t n, int tiles, int per, c
     chunk_range(tiles, per, t0, t1);
     const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
     for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) p[e] = first ? z[e] : z[e] + b * p[e];
+}
+
+__global__ void k_fcg_state_init(FcgState *fs, FcgState v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *fs = v;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -910,20 +919,24 @@ void launch_csr_scale(hipStream_t st, const CsrOp &op, const double *sig, double
 }
 
 void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
-                     float *q, double *part) {
-    hipLaunchKernelGGL(k_fcg_spmv, dim3(g.blocks), dim3(NT), 0, st, op, g.per, p, r, q, part);
+                     float *q, double *part, const FcgState *fs) {
+    hipLaunchKernelGGL(k_fcg_spmv, dim3(g.blocks), dim3(NT), 0, st, op, g.per, p, r, q, part, fs);
 }
 
 void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
-                     const double *part1, double *part2, double *fst, bool norm_only) {
+                     const double *part1, double *part2, double *fst, bool norm_only, const FcgState *fs) {
     hipLaunchKernelGGL(k_fcg_axpy, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, x, r, p, q, part1, part2,
-                       fst, norm_only ? 1 : 0);
+                       fst, norm_only ? 1 : 0, fs);
 }
 
 void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
-                    const double *fst, bool first) {
+                    const double *fst, bool first, const FcgState *fs) {
     hipLaunchKernelGGL(k_fcg_dir, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, z, p, zq_part, nz, fst,
-                       first ? 1 : 0);
+                       first ? 1 : 0, fs);
+}
+
+void launch_fcg_state_init(hipStream_t st, FcgState *fs, const FcgState &v) {
+    hipLaunchKernelGGL(k_fcg_state_init, dim3(1), dim3(64), 0, st, fs, v);
 }
 
 template void launch_cg_init<double, double>(hipStream_t, const Grid &, const double *, const double *, double *,

@@ -43,6 +43,57 @@ struct SolveState {
     int32_t status;   // shk_status
 };
 
+// Device-side control of one inner flexible-CG solve (the learned preconditioner's):
+// the stop rules run on the device between an update and the next application of the
+// network, the following kernels read the result (a set stop makes them return at
+// once) and the host reads it an iteration later.
+struct FcgState {
+    double scale;     // rms of r (1 if not positive): the network's input scale
+    double rn;        // max|r| after the last update
+    double target;    // converged once rn <= target
+    double best;      // last rn that improved on the one before by 10%
+    int64_t it;       // completed iterations
+    int64_t best_it;  // the iteration of best
+    int64_t cap;      // stop after this many iterations
+    int32_t stop;     // 0 running, 1 converged or capped, 2 stalled, broke down or not finite
+    int32_t pad;
+};
+
+// The stop rules after an update (r2 = r.r, rn = max|r|, bad: p.q <= 0 or a non-finite
+// step, so the update was skipped; nunk: unknowns, for the rms).
+__host__ __device__ inline void fcg_decide(FcgState &v, double r2, double rn, bool bad, double nunk) {
+    if (bad) {  // the preconditioner broke the iteration
+        v.stop = 2;
+        return;
+    }
+    v.it += 1;
+    if (!(rn == rn)) {
+        v.stop = 2;
+    } else if (rn < 0.9 * v.best) {
+        v.best = rn;
+        v.best_it = v.it;
+    } else if (v.it - v.best_it >= 20) {  // less than 10% progress in 20 iterations
+        v.stop = 2;
+    }
+    if (v.stop == 0 && rn <= v.target) v.stop = 1;
+    if (v.stop == 0 && v.it >= v.cap) v.stop = 1;
+    v.rn = rn;
+    const double rms = sqrt(r2 / nunk);
+    v.scale = rms > 0.0 ? rms : 1.0;
+}
+
+// What the first kernel of a network application needs to apply fcg_decide: the
+// axpy's 2 x nparts partials (r.r, max|r|), its flags fst (fst[2] != 0: bad), the
+// state before (in) and where the state after goes (out, written by block 0).
+struct FcgCtl {
+    const double *part2;
+    const double *fst;
+    int nparts;
+    double nunk;
+    const FcgState *in;
+    FcgState *out;
+};
+
 // Workgroup -> slot mapping that hands each XCD one contiguous range of slots.
 // Workgroups are dispatched to the 8 XCDs round-robin by id, and each XCD has its
 // own L2, so contiguous slots (spatial neighbours share halo cells) should land

@@ -183,6 +183,7 @@ struct UnetNet {
     bf16 *pkr[kLayers] = {};      // row-reuse packing of the 16/32-output layers (inference)
     uint32_t *head_pk = nullptr;  // the head's output-channel-0 weights as BF16 pairs (inference)
     float *split_ws = nullptr;    // K-split partial sums of the small-grid convolutions (inference)
+    const int32_t *stop = nullptr;  // inference kernels return at once while *stop is set
     int64_t split_cap = 0;
     bool packed_dirty = true;
     // activations for the current half-resolution grid
@@ -316,38 +317,38 @@ void forward(UnetNet *n, bool train) {
     auto enc = [&](int l, Grid3 g, const bf16 *x, bf16 *y) {
         if (rows(l, g))
             launch_conv_rows(n->stream, g, x, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
-                             kSpec[l].relu, y);
+                             kSpec[l].relu, y, nullptr, 0, n->stop);
         else if (direct(l, g))
             launch_conv_direct(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
-                               kSpec[l].relu, y);
+                               kSpec[l].relu, y, nullptr, 0, n->stop);
         else
             launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
-                            kSpec[l].relu, y, nullptr, ws, wc);
+                            kSpec[l].relu, y, nullptr, ws, wc, n->stop);
     };
     enc(0, L[0], n->feat, n->enc0);
-    launch_pool(n->stream, L[1], 16, n->enc0, n->p1);
+    launch_pool(n->stream, L[1], 16, n->enc0, n->p1, n->stop);
     enc(1, L[1], n->p1, n->enc1);
-    launch_pool(n->stream, L[2], 32, n->enc1, n->p2);
+    launch_pool(n->stream, L[2], 32, n->enc1, n->p2, n->stop);
     enc(2, L[2], n->p2, n->enc2);
-    launch_pool(n->stream, L[3], 64, n->enc2, n->p3);
+    launch_pool(n->stream, L[3], 64, n->enc2, n->p3, n->stop);
     enc(3, L[3], n->p3, n->bott);
     if (!train) {
         auto cat = [&](int l, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, bf16 *y) {
             if (rows(l, g))
                 launch_conv_rows(n->stream, g, skip, kSpec[l].cin, n->pkr[l], n->param + n->boff[l], kSpec[l].cout,
-                                 kSpec[l].relu, y, coarse, c1);
+                                 kSpec[l].relu, y, coarse, c1, n->stop);
             else if (direct(l, g))
                 launch_conv_direct(n->stream, g, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                   kSpec[l].cout, kSpec[l].relu, y, coarse, c1);
+                                   kSpec[l].cout, kSpec[l].relu, y, coarse, c1, n->stop);
             else
                 launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                    kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap);
+                                    kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap, n->stop);
         };
         cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
         cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);
         cat(6, L[0], n->dec1, 32, n->enc0, n->dec0);
         // only output channel 0 of the head is used: the vector-ALU head kernel
-        launch_conv_head(n->stream, L[0], n->dec0, n->head_pk, n->param + n->boff[7], n->u);
+        launch_conv_head(n->stream, L[0], n->dec0, n->head_pk, n->param + n->boff[7], n->u, n->stop);
         return;
     }
     launch_upsample(n->stream, L[3], 128, n->bott, 192, n->cat2);
@@ -609,6 +610,10 @@ struct UnetPrecond {
     float *mask = nullptr, *z = nullptr;  // brick vectors: 1 on unknowns; z = M r
     double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *fst = nullptr;
     double *h_out = nullptr;            // pinned: part2 (2 x 1024) and fst (3)
+    static constexpr int kRing = 2;     // inner-solve states in flight to the host
+    FcgState *fs = nullptr;             // the inner solve's device-side state (two: before, after)
+    FcgState *h_fs = nullptr;           // pinned: kRing copies of it
+    hipEvent_t fev[kRing] = {};         // each copy's completion
 };
 
 void unet_destroy(UnetPrecond *u) {
@@ -617,6 +622,10 @@ void unet_destroy(UnetPrecond *u) {
     for (float **p : {&u->mask, &u->z}) dfree(*p);
     for (double **p : {&u->part1, &u->part2, &u->part3, &u->fst}) dfree(*p);
     if (u->h_out) (void)hipHostFree(u->h_out);
+    if (u->fs) (void)hipFree(u->fs);
+    if (u->h_fs) (void)hipHostFree(u->h_fs);
+    for (hipEvent_t e : u->fev)
+        if (e) (void)hipEventDestroy(e);
     delete u;
 }
 
@@ -655,6 +664,9 @@ int ensure_precond(shk_ctx *c) {
         HIPC(c, dalloc(u->part3, (size_t)kPartCap));
         HIPC(c, dalloc(u->fst, 4));
         HIPC(c, hipHostMalloc((void **)&u->h_out, sizeof(double) * (kPartCap + 4), hipHostMallocDefault));
+        HIPC(c, hipMalloc((void **)&u->fs, 2 * sizeof(FcgState)));
+        HIPC(c, hipHostMalloc((void **)&u->h_fs, sizeof(FcgState) * UnetPrecond::kRing, hipHostMallocDefault));
+        for (hipEvent_t &e : u->fev) HIPC(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     UnetPrecond *u = c->unet;
     u->net->stream = c->stream;
@@ -675,8 +687,11 @@ BrickView brick_view(shk_ctx *c) {
 // z = M(r) on the projection's brick vectors: features of r (all channels when
 // `full`, else the residual channel alone), the network on the half grid of the
 // padded grid, prolongation into the bricks and kSmooth damped-Jacobi steps.  With
-// q: the last step also writes the z.q partials (their count into *nz).
-int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz) {
+// q: the last step also writes the z.q partials (their count into *nz).  ctl (not
+// with full): the inner solve's stop rules run first, and the input scale and the
+// stop flag come from its state.
+int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz,
+                        const FcgCtl *ctl = nullptr) {
     UnetPrecond *u = c->unet;
     UnetNet *n = u->net;
     int rc;
@@ -686,14 +701,19 @@ int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const
     if ((rc = repack(n))) return c->fail(rc, "%s", n->err.c_str());
     const BrickView bv = brick_view(c);
     const double s = rms > 0 ? rms : 1.0;
-    launch_features_bricks(c->stream, m, bv, r, 1.0 / s, u->mask, c->hx32.p, c->hy32.p, c->hz32.p, !full, n->feat);
+    const double *dscale = ctl ? &ctl->out->scale : nullptr;
+    const int32_t *stop = ctl ? &ctl->out->stop : nullptr;
+    launch_features_bricks(c->stream, m, bv, r, 1.0 / s, u->mask, c->hx32.p, c->hy32.p, c->hz32.p, !full, n->feat,
+                           full ? nullptr : ctl);
+    n->stop = stop;
     forward(n, false);
-    launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, u->z);
+    n->stop = nullptr;
+    launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, u->z, dscale, stop);
     float *src = u->z, *dst = c->t32.p;
     for (int i = 0; i < kSmooth; ++i) {
         const bool last = i + 1 == kSmooth;
         const int blocks = launch_smooth_bricks(c->stream, bv, n->omega, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p,
-                                                src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr);
+                                                src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr, stop);
         if (last && nz) *nz = blocks;
         std::swap(src, dst);
     }
@@ -724,13 +744,14 @@ int read_norms(shk_ctx *c, int blocks, double &rr, double &rmax, double fst[3]) 
 
 // z = M(r) inside flexible CG; with ProfileKernels=1 the application is timed with
 // events on the context's stream (read back at the next stream synchronisation).
-int timed_precondition(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz) {
+int timed_precondition(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz,
+                       const FcgCtl *ctl = nullptr) {
     if (c->profile && !c->uev[0]) {
         HIPC(c, hipEventCreate(&c->uev[0]));
         HIPC(c, hipEventCreate(&c->uev[1]));
     }
     if (c->profile) HIPC(c, hipEventRecord(c->uev[0], c->stream));
-    const int rc = precondition_bricks(c, r, rms, full, q, nz);
+    const int rc = precondition_bricks(c, r, rms, full, q, nz, ctl);
     if (rc) return rc;
     if (c->profile) {
         HIPC(c, hipEventRecord(c->uev[1], c->stream));
@@ -747,10 +768,14 @@ int timed_precondition(shk_ctx *c, const float *r, double rms, bool full, const 
 // Jacobi-scaled system in FP32 with FP64 dots, inside the same FP64 refinement loop
 // as the mixed solver.  The network runs on the half grid of the padded grid; every
 // vector operation, the smoothing and the prolongation work on the bricks, and the
-// scalars stay on the device (one host read per iteration, for the stop test and the
-// network's input scale).  Falls back to plain Jacobi-PCG inner solves when an inner
-// FCG stalls (less than 10% residual reduction in 20 iterations) or hits its cap; the
-// result records the fallback.
+// scalars stay on the device: the stop rules and the network's input scale are
+// applied by the first kernel of every network application (each of its blocks folds
+// the update's partials; the state is double-buffered, read from one copy and written
+// to the other), a stop makes every kernel of the following iterations return at
+// once, and the host reads the state one iteration late, so the stream never waits
+// for it.  Falls back to plain Jacobi-PCG inner solves when an
+// inner FCG stalls (less than 10% residual reduction in 20 iterations) or hits its
+// cap; the result records the fallback.
 int unet_solve(shk_ctx *c, shk_result *res) {
     int rc;
     if ((rc = ensure_precond(c))) return rc;
@@ -789,34 +814,46 @@ int unet_solve(shk_ctx *c, shk_result *res) {
             double rn = r0;
             if ((rc = timed_precondition(c, r, std::sqrt(r2 / nunk), true, nullptr, nullptr))) return rc;
             launch_fcg_dir(st, g, u->z, p, nullptr, 0, u->fst, true);
+            FcgState s0{};
+            s0.scale = 1.0;
+            s0.rn = rn;
+            s0.target = inner_rel * r0;
+            s0.best = rn;
+            s0.cap = std::min<int64_t>(fcg_cap, c->prm.max_iterations - total);
+            launch_fcg_state_init(st, u->fs, s0);
             int64_t it = 0;
-            double best = rn;
-            int64_t best_it = 0;
             bool stalled = false;
-            while (rn > inner_rel * r0 && it < fcg_cap && total + it < c->prm.max_iterations) {
-                launch_fcg_spmv(st, g, op, p, r, q, u->part1);
-                launch_fcg_axpy(st, g, x, r, p, q, u->part1, u->part2, u->fst, false);
-                if ((rc = read_norms(c, g.blocks, r2, rn, fst))) return rc;
-                if (fst[2] != 0.0) {  // p.q <= 0: the preconditioner broke the iteration
-                    stalled = true;
-                    break;
+            if (rn > s0.target && s0.cap > 0) {
+                constexpr int R = UnetPrecond::kRing;
+                int64_t enq = 0;
+                for (;;) {
+                    // the state before this iteration's update, and after it
+                    FcgState *fin = u->fs + (enq & 1), *fout = u->fs + ((enq + 1) & 1);
+                    launch_fcg_spmv(st, g, op, p, r, q, u->part1, fin);
+                    launch_fcg_axpy(st, g, x, r, p, q, u->part1, u->part2, u->fst, false, fin);
+                    const FcgCtl ctl{u->part2, u->fst, g.blocks, nunk, fin, fout};
+                    int nz = 0;
+                    if ((rc = timed_precondition(c, r, 1.0, false, q, &nz, &ctl))) return rc;
+                    launch_fcg_dir(st, g, u->z, p, u->part3, nz, u->fst, false, fout);
+                    HIPC(c, hipGetLastError());
+                    const int slot = (int)(enq % R);
+                    HIPC(c, hipMemcpyAsync(u->h_fs + slot, fout, sizeof(FcgState), hipMemcpyDeviceToHost, st));
+                    HIPC(c, hipEventRecord(u->fev[slot], st));
+                    ++enq;
+                    // the state after the previous iteration (this one is still running)
+                    if (enq >= R) {
+                        const int prev = (int)((enq - R) % R);
+                        HIPC(c, hipEventSynchronize(u->fev[prev]));
+                        if (u->h_fs[prev].stop) break;
+                    }
+                    // every iteration either stops or counts: a device state that does
+                    // neither within the cap is an error, not a reason to spin
+                    if (enq > s0.cap + R) return c->fail(SHK_HIP_ERROR, "inner FCG state did not advance");
                 }
-                ++it;
-                if (!(rn == rn)) {
-                    stalled = true;
-                    break;
-                }
-                if (rn < 0.9 * best) {
-                    best = rn;
-                    best_it = it;
-                } else if (it - best_it >= 20) {  // less than 10% progress in 20 iterations
-                    stalled = true;
-                    break;
-                }
-                if (rn <= inner_rel * r0) break;
-                int nz = 0;
-                if ((rc = timed_precondition(c, r, std::sqrt(r2 / nunk), false, q, &nz))) return rc;
-                launch_fcg_dir(st, g, u->z, p, u->part3, nz, u->fst, false);
+                HIPC(c, hipStreamSynchronize(st));
+                const FcgState &last = u->h_fs[(enq - 1) % R];
+                it = last.it;
+                stalled = last.stop == 2;
             }
             total += it;
             if (stalled || it >= fcg_cap) fell_back = true;

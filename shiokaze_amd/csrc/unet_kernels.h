@@ -12,6 +12,8 @@
 
 #include <cstdint>
 
+#include "shk_common.h"
+
 namespace shk {
 
 using bf16 = __hip_bfloat16;
@@ -46,7 +48,8 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
 // ws (ws_floats floats, may be null): the workspace of K splits on grids too small to
 // fill the chip (conv_split_floats); without it the convolution runs unsplit.
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32, float *ws = nullptr, int64_t ws_floats = 0);
+                     bool relu, bf16 *y, float *out_f32, float *ws = nullptr, int64_t ws_floats = 0,
+                     const int32_t *stop = nullptr);
 // workspace floats a BF16-output convolution of this shape splits into (0: unsplit)
 int64_t conv_split_floats(Grid3 g, int cin, int cout);
 
@@ -55,7 +58,7 @@ int64_t conv_split_floats(Grid3 g, int cin, int cout);
 // directly; c1 and cin - c1 are multiples of 8.
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
                          const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws = nullptr,
-                         int64_t ws_floats = 0);
+                         int64_t ws_floats = 0, const int32_t *stop = nullptr);
 
 // Direct-load forward convolution for grids too small for the tiled kernel to fill the
 // chip (the grids on which it splits K over blocks): same packed weights as
@@ -63,7 +66,8 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
 // (coarse != null: the input is [upsample(coarse) (c1 channels), x (cin - c1)]).
 bool conv_direct_ok(Grid3 g, int cin, int cout);
 void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
-                        int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0);
+                        int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0,
+                        const int32_t *stop = nullptr);
 
 // Row-reuse forward convolution for grids that fill the chip (inference).  The
 // reduction runs by 32-channel stages of cin (the last one 8, 16 or 24 wide: G groups
@@ -91,12 +95,14 @@ bool conv_rows_ok(Grid3 g, int cin, int cout);
 // y = act(conv(x) + bias) as launch_conv_fwd (coarse != null: the input is
 // [upsample(coarse) (c1 channels), x (cin - c1)] as in launch_conv_fwd_cat).
 void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
-                      int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0);
+                      int cout, bool relu, bf16 *y, const bf16 *coarse = nullptr, int c1 = 0,
+                      const int32_t *stop = nullptr);
 
 // y[v] = bias[0] + sum_{tap, ci} W[tap][ci][0] x[v + tap][ci] for a 16-channel BF16 input
 // (the network's head, of which only output channel 0 is used); wpk from
 // launch_head_pack (27 * 8 BF16 pairs)
-void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y);
+void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y,
+                      const int32_t *stop = nullptr);
 void launch_head_pack(hipStream_t st, const float *w, int cout, uint32_t *wpk);
 
 // dw[tap][ci][co] += sum_v x[v + tap][ci] * dz[v][co]; db[co] += sum_v dz[v][co]
@@ -107,7 +113,7 @@ void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf
 // dz = dy * (y > 0) in place on dy (ReLU backward), n elements.
 void launch_relu_bwd(hipStream_t st, int64_t n, const bf16 *y, bf16 *dy);
 // coarse[V][c] = mean of the 8 fine children (fine grid is 2x in each axis).
-void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out);
+void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out, const int32_t *stop = nullptr);
 // fine_grad[v][c] = coarse_grad[parent(v)][c] / 8 (pool backward).
 void launch_pool_bwd(hipStream_t st, Grid3 coarse, int c, const bf16 *coarse_grad, bf16 *fine_grad);
 // out[v][c_off + c] = coarse[parent(v)][c] for c < c_coarse (nearest upsampling into
@@ -140,19 +146,24 @@ struct BrickView {
     int n_active, nbx, nby;
     int nx, ny, nz;
 };
-// only_r: write channel 0 (R(r)/scale) and keep the other channels
+// only_r: write channel 0 (R(r)/scale) of the coarse voxels over active bricks and keep
+// everything else (a full pass wrote the rest); with ctl (only_r), the inner solve's
+// stop rules are applied first (fcg_decide) and the scale is the state's.  In the
+// prolongation, dscale (nullable) is the scale read from device memory instead of the
+// argument; stop (nullable, there and in the network's inference kernels): the kernel
+// returns at once when *stop is set.
 void launch_features_bricks(hipStream_t st, Grid3 coarse, BrickView b, const float *r, double inv_scale,
                             const float *mask, const float *cx, const float *cy, const float *cz, bool only_r,
-                            bf16 *feat);
+                            bf16 *feat, const FcgCtl *ctl = nullptr);
 void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float *u, double scale, const float *mask,
-                           float *z);
+                           float *z, const double *dscale = nullptr, const int32_t *stop = nullptr);
 // zout = zin + omega * mask * (r - A zin), A the unit-diagonal operator with low-face
 // couplings cx, cy, cz, evaluated in the order and rounding of the dense stencil and
 // smoothing kernels.  With q and part: part[block] = sum zout * q (FP64), at most
 // 1024 blocks; returns the number of blocks.
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
-                         const float *q, double *part);
+                         const float *q, double *part, const int32_t *stop = nullptr);
 
 // grad_u = scale * P^T (mask * g).
 void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale, float *grad_u);

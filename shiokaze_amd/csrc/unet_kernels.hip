@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
                                                   int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32,
                                                   const uint16_t *__restrict__ xc, int c1, int nsplit,
-                                                  float *__restrict__ split_out) {
+                                                  float *__restrict__ split_out, const int32_t *__restrict__ stop) {
     constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
     // voxel stride in uint4, odd so that 16 lanes reading 16 consecutive voxels hit 16
     // disjoint 4-bank groups of the 64-bank LDS
@@ -108,6 +108,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     constexpr int KPC = (27 * CC + 31) / 32;
     constexpr int PF = MTB <= 2 ? 4 : (MTB <= 4 ? 2 : 1);
     __shared__ uint4 lds[kHalo * VS];
+    if (stop && *stop) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int mgroups = mt_total / MTB;
@@ -244,11 +245,13 @@ constexpr int direct_max_threads() {
 template <int MW, int NW, int CC>
 __global__ __launch_bounds__((direct_max_threads<MW, NW>())) void k_conv_direct(
     Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total, const uint4 *__restrict__ packed,
-    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y, const uint16_t *__restrict__ xc, int c1) {
+    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y, const uint16_t *__restrict__ xc, int c1,
+    const int32_t *__restrict__ stop) {
     constexpr int KPC = (27 * CC + 31) / 32;
     constexpr int PF = MW + NW <= 2 ? 4 : (MW + NW <= 4 ? 3 : 2);
     constexpr int F = MW * NW;
     extern __shared__ f32x4 red[];  // [S][F][64]
+    if (stop && *stop) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, S = blockDim.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int nb = xcd_slot((int)blockIdx.x, (int)gridDim.x), mg = blockIdx.y;
@@ -370,7 +373,7 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
                                                       const uint4 *__restrict__ packed,
                                                       const float *__restrict__ bias, int relu,
                                                       uint16_t *__restrict__ y, const uint16_t *__restrict__ xc,
-                                                      int c1, int tx, int ty) {
+                                                      int c1, int tx, int ty, const int32_t *__restrict__ stop) {
     constexpr int TY = 2 * RY, TZ = 2 * RZ;
     constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2;
     constexpr int HV = HX * HY * HZ;
@@ -380,6 +383,7 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
     // element index of every halo voxel in x (fine) and in xc (its parent), -1 outside
     // the grid: computed once per block, read by every stage's loads
     __shared__ int off_f[HV], off_c[HV];
+    if (stop && *stop) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int wy = w & 1, wz = w >> 1;
@@ -570,8 +574,9 @@ __device__ inline float dot2(uint32_t a, uint32_t w, float acc) {
 
 __global__ __launch_bounds__(256) void k_conv_head(Grid3 g, const uint16_t *__restrict__ x,
                                                    const uint32_t *__restrict__ wpk, const float *__restrict__ bias,
-                                                   float *__restrict__ y) {
+                                                   float *__restrict__ y, const int32_t *__restrict__ stop) {
     __shared__ uint4 lds[2][kHeadRows];
+    if (stop && *stop) return;
     const int t = threadIdx.x;
     const int x0 = blockIdx.x * kHeadTX, y0 = blockIdx.y * kHeadTY, z0 = blockIdx.z * kHeadTZ;
     // stage the halo (hx 0..17 used of the 19-voxel rows): thread t loads half t % 2 of
@@ -738,7 +743,9 @@ __device__ inline int64_t child(const Grid3 &c, int X, int Y, int Z, int a) {
     return ((int64_t)(2 * Z + (a >> 2)) * fy + (2 * Y + ((a >> 1) & 1))) * fx + (2 * X + (a & 1));
 }
 
-__global__ void k_pool(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint16_t *__restrict__ out) {
+__global__ void k_pool(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint16_t *__restrict__ out,
+                       const int32_t *__restrict__ stop) {
+    if (stop && *stop) return;
     const int64_t total = c.vox() * ch;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         const int cc = (int)(i % ch);
@@ -751,7 +758,9 @@ __global__ void k_pool(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint1
 }
 
 // k_pool for 8 channels per thread (16-byte loads and stores; the same per-channel sums)
-__global__ void k_pool8(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint16_t *__restrict__ out) {
+__global__ void k_pool8(Grid3 c, int ch, const uint16_t *__restrict__ fine, uint16_t *__restrict__ out,
+                        const int32_t *__restrict__ stop) {
+    if (stop && *stop) return;
     const int groups = ch >> 3;
     const int64_t total = c.vox() * groups;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
@@ -966,10 +975,13 @@ __device__ inline int64_t brick_of(const BrickView &b, int i, int j, int k) {
 }
 
 __global__ void k_features_bricks(Grid3 c, BrickView b, const float *__restrict__ r, double inv_scale,
-                                  const float *__restrict__ mask, const float *__restrict__ cx,
-                                  const float *__restrict__ cy, const float *__restrict__ cz, int only_r,
-                                  uint16_t *__restrict__ feat) {
+                                  const double *__restrict__ dscale, const float *__restrict__ mask,
+                                  const float *__restrict__ cx, const float *__restrict__ cy,
+                                  const float *__restrict__ cz, int only_r, uint16_t *__restrict__ feat,
+                                  const int32_t *__restrict__ stop) {
+    if (stop && *stop) return;
     const int64_t nv = c.vox();
+    if (dscale) inv_scale = 1.0 / *dscale;
     for (int64_t V = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; V < nv; V += (int64_t)gridDim.x * blockDim.x) {
         const int X = (int)(V % c.nx), Y = (int)((V / c.nx) % c.ny), Z = (int)(V / ((int64_t)c.nx * c.ny));
         // the 8 children lie in one brick (2x2x2 aligned in 8x8x8)
@@ -1005,9 +1017,56 @@ __global__ void k_features_bricks(Grid3 c, BrickView b, const float *__restrict_
     }
 }
 
+// Channel 0 of k_features_bricks for the coarse voxels whose children lie in an active
+// brick: 8 bricks per 512-thread block, thread t -> coarse voxel t % 64 of brick t / 64.
+// With ctl.in: every block first applies the inner solve's stop rules to the axpy's
+// partials (one fixed fold order, so every block gets the same state; block 0 stores
+// it in ctl.out) and takes the input scale from there.
+__global__ __launch_bounds__(512) void k_features_bricks_r(Grid3 c, BrickView b, const float *__restrict__ r,
+                                                           double inv_scale, FcgCtl ctl, uint16_t *__restrict__ feat) {
+    __shared__ double lds[8];
+    __shared__ double sc;
+    __shared__ int stp;
+    if (ctl.in) {
+        const FcgState v0 = *ctl.in;
+        if (v0.stop) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) *ctl.out = v0;
+            return;
+        }
+        const double r2 = reduce_partials_sum<512>(ctl.part2, ctl.nparts, lds);
+        const double rn = reduce_partials_max<512>(ctl.part2 + ctl.nparts, ctl.nparts, lds);
+        if (threadIdx.x == 0) {
+            FcgState v = v0;
+            fcg_decide(v, r2, rn, ctl.fst[2] != 0.0, ctl.nunk);
+            if (blockIdx.x == 0) *ctl.out = v;
+            sc = v.scale;
+            stp = v.stop;
+        }
+        __syncthreads();
+        if (stp) return;
+        inv_scale = 1.0 / sc;
+    }
+    const int slot = blockIdx.x * 8 + (threadIdx.x >> 6);
+    if (slot >= b.n_active) return;
+    const int v = threadIdx.x & 63, ci = v & 3, cj = (v >> 2) & 3, ck = v >> 4;
+    const int4 bc = b.coord[slot];
+    const int X = bc.x * 4 + ci, Y = bc.y * 4 + cj, Z = bc.z * 4 + ck;
+    const int64_t e0 = (int64_t)slot * 512 + 2 * ci + 16 * cj + 128 * ck;
+    float s0 = 0.0f;
+    for (int a = 0; a < 8; ++a) {
+        const int i = 2 * X + (a & 1), j = 2 * Y + ((a >> 1) & 1), k = 2 * Z + (a >> 2);
+        const bool in = i < b.nx && j < b.ny && k < b.nz;
+        s0 += in ? r[e0 + (a & 1) + 8 * ((a >> 1) & 1) + 64 * (a >> 2)] : 0.0f;
+    }
+    feat[(((int64_t)Z * c.ny + Y) * c.nx + X) * 8] = bf16_bits((float)(0.125 * s0 * inv_scale));
+}
+
 __global__ void k_prolong_bricks(Grid3 f, BrickView b, const float *__restrict__ u, double scale,
-                                 const float *__restrict__ mask, float *__restrict__ z) {
+                                 const double *__restrict__ dscale, const float *__restrict__ mask,
+                                 float *__restrict__ z, const int32_t *__restrict__ stop) {
+    if (stop && *stop) return;
     const int slot = blockIdx.x, l = threadIdx.x;
+    if (dscale) scale = *dscale;
     const int4 bc = b.coord[slot];
     const int x = bc.x * 8 + (l & 7), y = bc.y * 8 + ((l >> 3) & 7), zc = bc.z * 8 + (l >> 6);
     const int64_t e = (int64_t)slot * 512 + l;
@@ -1036,9 +1095,11 @@ __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega
                                                        const float *__restrict__ mask, const float *__restrict__ cx,
                                                        const float *__restrict__ cy, const float *__restrict__ cz,
                                                        const float *__restrict__ zin, float *__restrict__ zout,
-                                                       const float *__restrict__ q, double *__restrict__ part) {
+                                                       const float *__restrict__ q, double *__restrict__ part,
+                                                       const int32_t *__restrict__ stop) {
     typedef float f4 __attribute__((ext_vector_type(4)));
     __shared__ double lds[8];
+    if (stop && *stop) return;
     const int l = threadIdx.x & 127, sub = threadIdx.x >> 7;
     const int h = l & 1, j = (l >> 1) & 7, k = l >> 4;
     const int off = 4 * h + 8 * j + 64 * k;  // first of the thread's 4 cells in its brick
@@ -1166,24 +1227,33 @@ inline uint16_t *U(bf16 *p) { return reinterpret_cast<uint16_t *>(p); }
 
 void launch_features_bricks(hipStream_t st, Grid3 coarse, BrickView b, const float *r, double inv_scale,
                             const float *mask, const float *cx, const float *cy, const float *cz, bool only_r,
-                            bf16 *feat) {
+                            bf16 *feat, const FcgCtl *ctl) {
+    if (only_r) {
+        // channel 0 of the coarse voxels over the active bricks (4^3 per brick); the
+        // others hold the zero a full pass wrote there and keep it
+        const FcgCtl none{nullptr, nullptr, 0, 1.0, nullptr, nullptr};
+        if (b.n_active > 0)
+            hipLaunchKernelGGL(k_features_bricks_r, dim3((unsigned)((b.n_active + 7) / 8)), dim3(512), 0, st, coarse,
+                               b, r, inv_scale, ctl ? *ctl : none, U(feat));
+        return;
+    }
     hipLaunchKernelGGL(k_features_bricks, dim3(grid1(coarse.vox(), 256)), dim3(256), 0, st, coarse, b, r, inv_scale,
-                       mask, cx, cy, cz, only_r ? 1 : 0, U(feat));
+                       nullptr, mask, cx, cy, cz, 0, U(feat), nullptr);
 }
 
 void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float *u, double scale, const float *mask,
-                           float *z) {
+                           float *z, const double *dscale, const int32_t *stop) {
     if (b.n_active <= 0) return;
-    hipLaunchKernelGGL(k_prolong_bricks, dim3(b.n_active), dim3(512), 0, st, fine, b, u, scale, mask, z);
+    hipLaunchKernelGGL(k_prolong_bricks, dim3(b.n_active), dim3(512), 0, st, fine, b, u, scale, dscale, mask, z, stop);
 }
 
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
-                         const float *q, double *part) {
+                         const float *q, double *part, const int32_t *stop) {
     const int want = (b.n_active + 3) / 4;  // 4 bricks per block and pass
     const int blocks = want < 1024 ? (want > 0 ? want : 1) : 1024;
     hipLaunchKernelGGL(k_smooth_bricks, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout, q,
-                       part);
+                       part, stop);
     return blocks;
 }
 
@@ -1196,10 +1266,11 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
                        conv_cc(gin), conv_kpc(gin), kchunks, U(packed), total);
 }
 
-void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y) {
+void launch_conv_head(hipStream_t st, Grid3 g, const bf16 *x, const uint32_t *wpk, const float *bias, float *y,
+                      const int32_t *stop) {
     const dim3 grid((unsigned)((g.nx + kHeadTX - 1) / kHeadTX), (unsigned)((g.ny + kHeadTY - 1) / kHeadTY),
                     (unsigned)((g.nz + kHeadTZ - 1) / kHeadTZ));
-    hipLaunchKernelGGL(k_conv_head, grid, dim3(256), 0, st, g, U(x), wpk, bias, y);
+    hipLaunchKernelGGL(k_conv_head, grid, dim3(256), 0, st, g, U(x), wpk, bias, y, stop);
 }
 
 void launch_head_pack(hipStream_t st, const float *w, int cout, uint32_t *wpk) {
@@ -1210,30 +1281,32 @@ namespace {
 template <int MTB>
 void conv_fwd_mt(hipStream_t st, dim3 grid, Grid3 g, const bf16 *x, int cin, int mt, const uint4 *pk,
                  const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1, int nsplit,
-                 float *split_out) {
+                 float *split_out, const int32_t *stop) {
     switch (conv_cc(cin)) {
         case 8:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 8>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out);
+                               out_f32, xc, c1, nsplit, split_out, stop);
             break;
         case 16:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 16>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out);
+                               out_f32, xc, c1, nsplit, split_out, stop);
             break;
         case 24:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 24>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out);
+                               out_f32, xc, c1, nsplit, split_out, stop);
             break;
         default:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 32>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out);
+                               out_f32, xc, c1, nsplit, split_out, stop);
             break;
     }
 }
 
 // y[v][c] = relu(bias[c] + sum over the splits of part[split][v][c]), in split order
 __global__ void k_conv_split_finish(int64_t n, int cout, int nsplit, const float *__restrict__ part,
-                                    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y) {
+                                    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y,
+                                    const int32_t *__restrict__ stop) {
+    if (stop && *stop) return;
     const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // 4 channels of one voxel
     if (q * 4 >= n) return;
     float4 s = *reinterpret_cast<const float4 *>(part + q * 4);
@@ -1296,7 +1369,8 @@ int64_t conv_split_floats(Grid3 g, int cin, int cout) {
 
 namespace {
 void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1, float *ws, int64_t ws_floats) {
+                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1, float *ws, int64_t ws_floats,
+                  const int32_t *stop) {
     const int mt = cout / 16;
     ConvPlan p = conv_plan(g, cin, cout);
     // K splits need a workspace of nsplit x voxels x cout floats and a BF16 output
@@ -1306,19 +1380,19 @@ void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *p
     uint16_t *yy = y ? U(y) : nullptr;
     float *so = p.nsplit > 1 ? ws : nullptr;
     switch (p.mtb) {
-        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
-        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so); break;
+        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
         default: return;  // unsupported widths are rejected by the caller
     }
     if (so) {
         const int64_t n = g.vox() * cout;
         hipLaunchKernelGGL(k_conv_split_finish, dim3((unsigned)((n / 4 + 255) / 256)), dim3(256), 0, st, n, cout,
-                           p.nsplit, so, bias, relu ? 1 : 0, yy);
+                           p.nsplit, so, bias, relu ? 1 : 0, yy, stop);
     }
 }
 }  // namespace
@@ -1352,26 +1426,27 @@ DirectPlan direct_plan(Grid3 g, int cin, int cout) {
 
 template <int MW, int NW>
 void conv_direct_mw(hipStream_t st, const DirectPlan &p, Grid3 g, const uint16_t *x, int cin, int mt,
-                    const uint4 *pk, const float *bias, int relu, uint16_t *y, const uint16_t *xc, int c1) {
+                    const uint4 *pk, const float *bias, int relu, uint16_t *y, const uint16_t *xc, int c1,
+                    const int32_t *stop) {
     const dim3 grid((unsigned)((g.vox() + 16 * NW - 1) / (16 * NW)), (unsigned)(mt / MW));
     const dim3 block((unsigned)(64 * p.s));
     const size_t lds = sizeof(float) * 4 * 64 * MW * NW * p.s;
     switch (conv_cc(cin)) {
         case 8:
             hipLaunchKernelGGL((k_conv_direct<MW, NW, 8>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y, xc,
-                               c1);
+                               c1, stop);
             break;
         case 16:
             hipLaunchKernelGGL((k_conv_direct<MW, NW, 16>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
-                               xc, c1);
+                               xc, c1, stop);
             break;
         case 24:
             hipLaunchKernelGGL((k_conv_direct<MW, NW, 24>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
-                               xc, c1);
+                               xc, c1, stop);
             break;
         default:
             hipLaunchKernelGGL((k_conv_direct<MW, NW, 32>), grid, block, lds, st, g, x, cin, mt, pk, bias, relu, y,
-                               xc, c1);
+                               xc, c1, stop);
             break;
     }
 }
@@ -1383,32 +1458,32 @@ bool conv_direct_ok(Grid3 g, int cin, int cout) {
 }
 
 void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
-                        int cout, bool relu, bf16 *y, const bf16 *coarse, int c1) {
+                        int cout, bool relu, bf16 *y, const bf16 *coarse, int c1, const int32_t *stop) {
     const DirectPlan p = direct_plan(g, cin, cout);
     const int mt = cout / 16, r = relu ? 1 : 0;
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     const uint16_t *xc = coarse ? U(coarse) : nullptr;
     const int key = p.mw * 8 + p.nw;
     switch (key) {
-        case 4 * 8 + 4: conv_direct_mw<4, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        case 4 * 8 + 2: conv_direct_mw<4, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        case 2 * 8 + 4: conv_direct_mw<2, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        case 2 * 8 + 2: conv_direct_mw<2, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        case 4 * 8 + 1: conv_direct_mw<4, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        case 2 * 8 + 1: conv_direct_mw<2, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
-        default: conv_direct_mw<1, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1); break;
+        case 4 * 8 + 4: conv_direct_mw<4, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        case 4 * 8 + 2: conv_direct_mw<4, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        case 2 * 8 + 4: conv_direct_mw<2, 4>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        case 2 * 8 + 2: conv_direct_mw<2, 2>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        case 4 * 8 + 1: conv_direct_mw<4, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        case 2 * 8 + 1: conv_direct_mw<2, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
+        default: conv_direct_mw<1, 1>(st, p, g, U(x), cin, mt, pk, bias, r, U(y), xc, c1, stop); break;
     }
 }
 
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32, float *ws, int64_t ws_floats) {
-    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0, ws, ws_floats);
+                     bool relu, bf16 *y, float *out_f32, float *ws, int64_t ws_floats, const int32_t *stop) {
+    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0, ws, ws_floats, stop);
 }
 
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
                          const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws,
-                         int64_t ws_floats) {
-    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, ws, ws_floats);
+                         int64_t ws_floats, const int32_t *stop) {
+    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, ws, ws_floats, stop);
 }
 
 namespace {
@@ -1432,7 +1507,7 @@ bool conv_rows_ok(Grid3 g, int cin, int cout) {
 }
 
 void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
-                      int cout, bool relu, bf16 *y, const bf16 *coarse, int c1) {
+                      int cout, bool relu, bf16 *y, const bf16 *coarse, int c1, const int32_t *stop) {
     const int tx = (g.nx + 15) / 16, ty = (g.ny + 2 * kRowsRY - 1) / (2 * kRowsRY);
     const int tz = (g.nz + 2 * kRowsRZ - 1) / (2 * kRowsRZ);
     const unsigned blocks = (unsigned)((int64_t)tx * ty * tz);
@@ -1440,10 +1515,10 @@ void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf1
     const uint16_t *xc = coarse ? U(coarse) : nullptr;
     if (cout == 16)
         hipLaunchKernelGGL((k_conv_rows<1, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
-                           relu ? 1 : 0, U(y), xc, c1, tx, ty);
+                           relu ? 1 : 0, U(y), xc, c1, tx, ty, stop);
     else
         hipLaunchKernelGGL((k_conv_rows<2, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
-                           relu ? 1 : 0, U(y), xc, c1, tx, ty);
+                           relu ? 1 : 0, U(y), xc, c1, tx, ty, stop);
 }
 
 void launch_conv_wgrad(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *dz, int cout, float *dw,
@@ -1462,12 +1537,13 @@ void launch_relu_bwd(hipStream_t st, int64_t n, const bf16 *y, bf16 *dy) {
     hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(n, 256)), dim3(256), 0, st, n, U(y), U(dy));
 }
 
-void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out) {
+void launch_pool(hipStream_t st, Grid3 coarse, int c, const bf16 *fine, bf16 *out, const int32_t *stop) {
     if (c % 8 == 0)
         hipLaunchKernelGGL(k_pool8, dim3(grid1(coarse.vox() * (c / 8), 256)), dim3(256), 0, st, coarse, c, U(fine),
-                           U(out));
+                           U(out), stop);
     else
-        hipLaunchKernelGGL(k_pool, dim3(grid1(coarse.vox() * c, 256)), dim3(256), 0, st, coarse, c, U(fine), U(out));
+        hipLaunchKernelGGL(k_pool, dim3(grid1(coarse.vox() * c, 256)), dim3(256), 0, st, coarse, c, U(fine), U(out),
+                           stop);
 }
 
 void launch_pool_bwd(hipStream_t st, Grid3 coarse, int c, const bf16 *coarse_grad, bf16 *fine_grad) {
