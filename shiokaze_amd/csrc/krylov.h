@@ -100,22 +100,33 @@ void launch_residual_csr(hipStream_t st, const Grid &g, const CsrOp &op, const d
                          double *part, double *out);
 
 // Flexible CG (FCG(1)) on FP32 brick vectors of a unit-diagonal brick operator with
-// device-side scalars (fst: 3 doubles; part1 2 x blocks, part2 2 x blocks):
-//   spmv: q = A p, partials p.q and p.r;
-//   axpy: alpha = p.r / p.q; x += alpha p; r -= alpha q; partials r.r and max|r|
-//         (norm_only: the partials of r alone);
-//   dir : beta = -(z.q) / (p.q) from nz partials of z.q; p = z + beta p (first: p = z).
-// With fs: nothing happens once fs->stop is set (the stop rules are applied between
-// the axpy and the next application of the network: fcg_decide).
-void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
-                     float *q, double *part, const FcgState *fs = nullptr);
+// device-side scalars (fst: 3 doubles; part1 and part2 2 x blocks each): the direction
+// kernel below (launch_fcg_dir_spmv) writes part1 = (p.q, p.r); then
+//   axpy: alpha = p.r / p.q; x += alpha p; r -= alpha q; partials r.r and max|r| into
+//         part2 (norm_only: the partials of r alone); fst = (p.q, alpha, bad).
+// With fs: nothing happens once fs->stop is set (the stop rules run between the axpy
+// and the next application of the network: fcg_decide).
 void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
                      const double *part1, double *part2, double *fst, bool norm_only,
                      const FcgState *fs = nullptr);
-void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
-                    const double *fst, bool first, const FcgState *fs = nullptr);
 // *fs = v (one thread: the value travels as a kernel argument)
 void launch_fcg_state_init(hipStream_t st, FcgState *fs, const FcgState &v);
+
+// FCG direction and stencil in one pass (the Jacobi path's direction kernel in its
+// flexible-CG form): p_out = z + beta p_in with beta = -(z.q) / fst[0] from the nz
+// partials zq (zq == nullptr: beta = 0, p_out = z; give p_in = z then), q = A p_out,
+// part[b] = p_out.q, part[blocks + b] = p_out.res; nothing once fs->stop is set.
+// Unit-diagonal 3D brick operator.
+struct FcgDir {
+    const FcgState *fs;
+    const double *zq;
+    int nz;
+    const double *fst;
+    const float *res;
+    double *part;
+};
+void launch_fcg_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *z, const float *p_in,
+                         float *p_out, float *q, const FcgDir &fd);
 
 // x64 += sig * (double) d32 (sig may be nullptr): the refinement correction.
 void launch_add_correction(hipStream_t st, int64_t n, double *x, const float *d, const double *sig);

@@ -201,15 +201,23 @@ __device__ inline V ld_stream(const V *p) {
 // bricks (one z layer of bricks at 512^3) and a brick's y and z neighbours were read a
 // layer before or are read now: their halo planes come from the Infinity Cache, not
 // from HBM, when the vectors themselves do not fit in it.
-template <class T, class C, bool UNIT, int BX, bool NTS>
+//
+// FCG: the same pass for flexible CG (r is the preconditioned residual z): beta =
+// -(z.q) / delta from fd's z.q partials (none: beta = 0), and the partials s'.q and
+// s'.res into fd.part; nothing once fd.fs->stop is set.
+template <class T, class C, bool UNIT, int BX, bool NTS, bool FCG>
 __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, int ilv, const T *__restrict__ r,
                                                  const T *__restrict__ sin, T *__restrict__ sout,
-                                                 T *__restrict__ q, Reduction red) {
+                                                 T *__restrict__ q, Reduction red, FcgDir fd) {
     using B = BrickShape<BX>;
     constexpr int X = B::X, Y = B::Y;
     __shared__ double lds[NB / kWave];
     __shared__ double bc;
-    if (red.st->done) return;
+    if constexpr (FCG) {
+        if (fd.fs && fd.fs->stop) return;
+    } else {
+        if (red.st->done) return;
+    }
     int t0, t1, step;
     if (ilv) {
         t0 = (int)blockIdx.x * (NB / kWave);
@@ -236,10 +244,14 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, int ilv
             if (!UNIT) mkd = op.kdiag[slot];
         }
     }
-    double beta;
-    if (!direction_step(red, lds, &bc, beta)) return;
+    double beta = 0.0;
+    if constexpr (FCG) {
+        if (fd.zq) beta = -fold_sum<NB>(fd.zq, fd.nz, lds, &bc) / fd.fst[0];
+    } else {
+        if (!direction_step(red, lds, &bc, beta)) return;
+    }
     const T b = (T)beta;
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     for (; slot < t1; slot += step) {
         const int nxm = mnb[0], nxp = mnb[1], nym = mnb[2], nyp = mnb[3], nzm = mnb[4], nzp = mnb[5];
         const C kap = mkap, kdg = mkd;
@@ -325,6 +337,7 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, int ilv
                 st_stream<NTS>(sk, &sout[e]);
                 st_stream<NTS>(qv, &q[e]);
                 acc += (double)sk * (double)qv;
+                if constexpr (FCG) acc2 += (double)sk * (double)ld_stream<NTS>(&fd.res[e]);
             }
             continue;
         }
@@ -381,10 +394,19 @@ __global__ __launch_bounds__(NB) void k_dir_spmv(BrickOp<C> op, int per, int ilv
             st_stream<NTS>(sk, &sout[e]);
             st_stream<NTS>(qv, &q[e]);
             acc += (double)sk * (double)qv;
+            if constexpr (FCG) acc2 += (double)sk * (double)ld_stream<NTS>(&fd.res[e]);
         }
     }
     const double bs = block_sum<NB>(acc, lds);
-    if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
+    if constexpr (FCG) {
+        const double b2 = block_sum<NB>(acc2, lds);
+        if (threadIdx.x == 0) {
+            fd.part[blockIdx.x] = bs;
+            fd.part[gridDim.x + blockIdx.x] = b2;
+        }
+    } else {
+        if (threadIdx.x == 0) red.pa[blockIdx.x] = bs;
+    }
 }
 
 __global__ __launch_bounds__(NB) void k_dir_spmv_csr(CsrOp op, int per, const double *__restrict__ r,
@@ -699,32 +721,7 @@ __global__ __launch_bounds__(NT) void k_csr_scale(CsrOp op, const double *__rest
 // folds the previous one's FP64 partials in one fixed order in every block.
 // fst[0] = delta = p.q, fst[1] = alpha, fst[2] = 1 if delta <= 0 or not finite.
 
-// q = A p; part[b] = p.q, part[gridDim + b] = p.r over the block's bricks.
-__global__ __launch_bounds__(NT) void k_fcg_spmv(BrickOp<float> op, int per, const float *__restrict__ p,
-                                                 const float *__restrict__ r, float *__restrict__ q,
-                                                 double *__restrict__ part, const FcgState *__restrict__ fs) {
-    __shared__ double lds[NT / kWave];
-    if (fs && fs->stop) return;
-    int t0, t1;
-    chunk_range(op.n_active, per, t0, t1);
-    const int l = threadIdx.x;
-    double pq = 0.0, pr = 0.0;
-    for (int slot = t0; slot < t1; ++slot) {
-        const int64_t e = (int64_t)slot * NT + l;
-        const float qv = brick_apply<float, float, 8, true>(op.nbr + 6 * slot, e, l, nullptr, op.cx, op.cy, op.cz, p);
-        q[e] = qv;
-        pq += (double)p[e] * (double)qv;
-        pr += (double)p[e] * (double)r[e];
-    }
-    const double a = block_sum<NT>(pq, lds);
-    const double b = block_sum<NT>(pr, lds);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = a;
-        part[gridDim.x + blockIdx.x] = b;
-    }
-}
-
-// alpha = (p.r) / (p.q) from spmv's partials; x += alpha p, r -= alpha q;
+// alpha = (p.r) / (p.q) from the direction kernel's partials; x += alpha p, r -= alpha q;
 // part2[b] = r.r, part2[gridDim + b] = max|r|.  norm_only: no update, just the norms.
 __global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, float *__restrict__ x,
                                                  float *__restrict__ r, const float *__restrict__ p,
@@ -751,15 +748,26 @@ __global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, 
     chunk_range(tiles, per, t0, t1);
     const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
     double rr = 0.0, rm = 0.0;
-    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) {
-        float rv = r[e];
+    // four elements per thread and access (n is a multiple of the 512-element tile)
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    for (int64_t e = (int64_t)t0 * NT + 4 * threadIdx.x; e < end; e += 4 * NB) {
+        f4 rv = *reinterpret_cast<const f4 *>(r + e);
         if (!norm_only) {
-            x[e] = x[e] + a * p[e];
-            rv = rv - a * q[e];
-            r[e] = rv;
+            const f4 pv = *reinterpret_cast<const f4 *>(p + e), qv = *reinterpret_cast<const f4 *>(q + e);
+            f4 xv = *reinterpret_cast<const f4 *>(x + e);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                xv[k] = xv[k] + a * pv[k];
+                rv[k] = rv[k] - a * qv[k];
+            }
+            *reinterpret_cast<f4 *>(x + e) = xv;
+            *reinterpret_cast<f4 *>(r + e) = rv;
         }
-        rr += (double)rv * (double)rv;
-        rm = fmax(rm, fabs((double)rv));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rr += (double)rv[k] * (double)rv[k];
+            rm = fmax(rm, fabs((double)rv[k]));
+        }
     }
     const double bs = block_sum<NB>(rr, lds);
     const double bm = block_max<NB>(rm, lds);
@@ -767,26 +775,6 @@ __global__ __launch_bounds__(NB) void k_fcg_axpy(int64_t n, int tiles, int per, 
         part2[blockIdx.x] = bs;
         part2[gridDim.x + blockIdx.x] = bm;
     }
-}
-
-// beta = -(z.q) / delta from the preconditioner's z.q partials (nz of them);
-// p = z + beta p (first: p = z).
-__global__ __launch_bounds__(NB) void k_fcg_dir(int64_t n, int tiles, int per, const float *__restrict__ z,
-                                                float *__restrict__ p, const double *__restrict__ zq_part, int nz,
-                                                const double *__restrict__ fst, int first,
-                                                const FcgState *__restrict__ fs) {
-    __shared__ double lds[NB / kWave];
-    __shared__ double bc;
-    if (fs && fs->stop) return;
-    float b = 0.0f;
-    if (!first) {
-        const double zq = fold_sum<NB>(zq_part, nz, lds, &bc);
-        b = (float)(-zq / fst[0]);
-    }
-    int t0, t1;
-    chunk_range(tiles, per, t0, t1);
-    const int64_t end = (int64_t)t1 * NT < n ? (int64_t)t1 * NT : n;
-    for (int64_t e = (int64_t)t0 * NT + threadIdx.x; e < end; e += NB) p[e] = first ? z[e] : z[e] + b * p[e];
 }
 
 __global__ void k_fcg_state_init(FcgState *fs, FcgState v) {
@@ -841,18 +829,18 @@ void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, int
                         T *s_out, T *q, Reduction red, KEvents ev) {
     if (op.diag) {
         if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
-                       s_out, q, red);
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 64, NTS, false>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r,
+                       s_in, s_out, q, red, FcgDir{});
         else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
-                       s_out, q, red);
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, false, 8, NTS, false>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r,
+                       s_in, s_out, q, red, FcgDir{});
     } else {
         if (op.bx == 64)
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
-                       s_out, q, red);
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 64, NTS, false>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r,
+                       s_in, s_out, q, red, FcgDir{});
         else
-            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8, NTS>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r, s_in,
-                       s_out, q, red);
+            SHK_LAUNCH(ev, (k_dir_spmv<T, C, true, 8, NTS, false>), dim3(g.blocks), dim3(NB), st, op, g.per, ilv, r,
+                       s_in, s_out, q, red, FcgDir{});
     }
 }
 
@@ -864,6 +852,17 @@ void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const 
         launch_dir_spmv_nt<T, C, true>(st, g, op, ilv, r, s_in, s_out, q, red, ev);
     else
         launch_dir_spmv_nt<T, C, false>(st, g, op, ilv, r, s_in, s_out, q, red, ev);
+}
+
+void launch_fcg_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *z, const float *p_in,
+                         float *p_out, float *q, const FcgDir &fd) {
+    const int ilv = streaming<float>(g) ? 1 : 0;
+    if (ilv)
+        hipLaunchKernelGGL((k_dir_spmv<float, float, true, 8, true, true>), dim3(g.blocks), dim3(NB), 0, st, op, g.per,
+                           ilv, z, p_in, p_out, q, Reduction{}, fd);
+    else
+        hipLaunchKernelGGL((k_dir_spmv<float, float, true, 8, false, true>), dim3(g.blocks), dim3(NB), 0, st, op,
+                           g.per, ilv, z, p_in, p_out, q, Reduction{}, fd);
 }
 
 void launch_dir_spmv_csr(hipStream_t st, const Grid &g, const CsrOp &op, const double *r, const double *s_in,
@@ -918,21 +917,10 @@ void launch_csr_scale(hipStream_t st, const CsrOp &op, const double *sig, double
     hipLaunchKernelGGL(k_csr_scale, dim3(blocks_for(op.rows)), dim3(NT), 0, st, op, sig, out);
 }
 
-void launch_fcg_spmv(hipStream_t st, const Grid &g, const BrickOp<float> &op, const float *p, const float *r,
-                     float *q, double *part, const FcgState *fs) {
-    hipLaunchKernelGGL(k_fcg_spmv, dim3(g.blocks), dim3(NT), 0, st, op, g.per, p, r, q, part, fs);
-}
-
 void launch_fcg_axpy(hipStream_t st, const Grid &g, float *x, float *r, const float *p, const float *q,
                      const double *part1, double *part2, double *fst, bool norm_only, const FcgState *fs) {
     hipLaunchKernelGGL(k_fcg_axpy, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, x, r, p, q, part1, part2,
                        fst, norm_only ? 1 : 0, fs);
-}
-
-void launch_fcg_dir(hipStream_t st, const Grid &g, const float *z, float *p, const double *zq_part, int nz,
-                    const double *fst, bool first, const FcgState *fs) {
-    hipLaunchKernelGGL(k_fcg_dir, dim3(g.blocks), dim3(NB), 0, st, g.n, g.tiles, g.per, z, p, zq_part, nz, fst,
-                       first ? 1 : 0, fs);
 }
 
 void launch_fcg_state_init(hipStream_t st, FcgState *fs, const FcgState &v) {

@@ -608,6 +608,7 @@ struct UnetPrecond {
     Grid3 pad{0, 0, 0};
     int64_t cap = 0;                    // brick elements the vectors below hold
     float *mask = nullptr, *z = nullptr;  // brick vectors: 1 on unknowns; z = M r
+    float *p2 = nullptr;                  // the second search-direction buffer
     double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *fst = nullptr;
     double *h_out = nullptr;            // pinned: part2 (2 x 1024) and fst (3)
     static constexpr int kRing = 2;     // inner-solve states in flight to the host
@@ -619,7 +620,7 @@ struct UnetPrecond {
 void unet_destroy(UnetPrecond *u) {
     if (!u) return;
     unet_net_destroy(u->net);
-    for (float **p : {&u->mask, &u->z}) dfree(*p);
+    for (float **p : {&u->mask, &u->z, &u->p2}) dfree(*p);
     for (double **p : {&u->part1, &u->part2, &u->part3, &u->fst}) dfree(*p);
     if (u->h_out) (void)hipHostFree(u->h_out);
     if (u->fs) (void)hipFree(u->fs);
@@ -673,7 +674,7 @@ int ensure_precond(shk_ctx *c) {
     u->pad = Grid3{(c->dims.nx + 15) / 16 * 16, (c->dims.ny + 15) / 16 * 16, (c->dims.nz + 15) / 16 * 16};
     const int64_t n = (int64_t)c->n_active * kBrickCells;
     if (n > u->cap) {
-        for (float **p : {&u->mask, &u->z}) HIPC(c, dalloc(*p, (size_t)n));
+        for (float **p : {&u->mask, &u->z, &u->p2}) HIPC(c, dalloc(*p, (size_t)n));
         u->cap = n;
     }
     return SHK_OK;
@@ -813,7 +814,6 @@ int unet_solve(shk_ctx *c, shk_result *res) {
             if ((rc = read_norms(c, g.blocks, r2, r0, fst))) return rc;
             double rn = r0;
             if ((rc = timed_precondition(c, r, std::sqrt(r2 / nunk), true, nullptr, nullptr))) return rc;
-            launch_fcg_dir(st, g, u->z, p, nullptr, 0, u->fst, true);
             FcgState s0{};
             s0.scale = 1.0;
             s0.rn = rn;
@@ -826,15 +826,22 @@ int unet_solve(shk_ctx *c, shk_result *res) {
             if (rn > s0.target && s0.cap > 0) {
                 constexpr int R = UnetPrecond::kRing;
                 int64_t enq = 0;
+                // p = z, q = A p (the search directions alternate between two buffers: the
+                // stencil reads the old direction's neighbours while the new one is written)
+                float *P[2] = {p, u->p2};
+                int cur = 0;
+                launch_fcg_dir_spmv(st, g, op, u->z, u->z, P[0], q, FcgDir{u->fs, nullptr, 0, u->fst, r, u->part1});
                 for (;;) {
                     // the state before this iteration's update, and after it
                     FcgState *fin = u->fs + (enq & 1), *fout = u->fs + ((enq + 1) & 1);
-                    launch_fcg_spmv(st, g, op, p, r, q, u->part1, fin);
-                    launch_fcg_axpy(st, g, x, r, p, q, u->part1, u->part2, u->fst, false, fin);
+                    launch_fcg_axpy(st, g, x, r, P[cur], q, u->part1, u->part2, u->fst, false, fin);
                     const FcgCtl ctl{u->part2, u->fst, g.blocks, nunk, fin, fout};
                     int nz = 0;
                     if ((rc = timed_precondition(c, r, 1.0, false, q, &nz, &ctl))) return rc;
-                    launch_fcg_dir(st, g, u->z, p, u->part3, nz, u->fst, false, fout);
+                    // the next direction and its stencil product
+                    launch_fcg_dir_spmv(st, g, op, u->z, P[cur], P[cur ^ 1], q,
+                                        FcgDir{fout, u->part3, nz, u->fst, r, u->part1});
+                    cur ^= 1;
                     HIPC(c, hipGetLastError());
                     const int slot = (int)(enq % R);
                     HIPC(c, hipMemcpyAsync(u->h_fs + slot, fout, sizeof(FcgState), hipMemcpyDeviceToHost, st));

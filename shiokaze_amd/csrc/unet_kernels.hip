@@ -512,6 +512,150 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
     }
 }
 
+// k_conv_rows for one output tile of 16 channels, persistent and software-pipelined:
+// the grid holds as many blocks as fit on the chip at once, the tiles are cut into one
+// contiguous range per XCD (blocks b, b + 8, ... share XCD b % 8 and take its tiles in
+// turn), and each block walks its (tile, stage) items with the next item's halo
+// loads in flight (registers) while the MFMAs of the current one run.  The arithmetic
+// of every tile is k_conv_rows<1, RY, RZ>'s, in the same order.
+template <int RY, int RZ>
+__global__ __launch_bounds__(256, 2) void k_conv_rows_pipe(Grid3 g, const uint16_t *__restrict__ x, int cin,
+                                                           const uint4 *__restrict__ packed,
+                                                           const float *__restrict__ bias, int relu,
+                                                           uint16_t *__restrict__ y, const uint16_t *__restrict__ xc,
+                                                           int c1, int tx, int ty, int ntiles,
+                                                           const int32_t *__restrict__ stop) {
+    constexpr int TY = 2 * RY, TZ = 2 * RZ;
+    constexpr int HX = 18, HY = TY + 2, HZ = TZ + 2;
+    constexpr int HV = HX * HY * HZ;
+    constexpr int HP = (HV + 11) / 16 * 16 + 4;
+    constexpr int NLD = (HV + 63) / 64;
+    __shared__ uint4 lds[4 * HP];
+    if (stop && *stop) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int n = lane & 15, kq = lane >> 4;
+    const int wy = w & 1, wz = w >> 1;
+    // this block's tiles: range [t_lo, t_hi) of its XCD, every nb-th from t_lo + idx
+    const int xcd = (int)blockIdx.x % kXcds, idx = (int)blockIdx.x / kXcds;
+    const int nb = ((int)gridDim.x - xcd + kXcds - 1) / kXcds;
+    const int groups = (int)gridDim.x < kXcds ? (int)gridDim.x : kXcds;  // XCDs that hold blocks
+    const int t_lo = (int)((int64_t)ntiles * xcd / groups), t_hi = (int)((int64_t)ntiles * (xcd + 1) / groups);
+    const int stages = (cin + 31) / 32;
+    const int my_tiles = t_lo + idx < t_hi ? (t_hi - t_lo - idx + nb - 1) / nb : 0;
+    const int items = my_tiles * stages;
+    const int tgrp = threadIdx.x & 3, tv0 = threadIdx.x >> 2;
+    const int hnx = g.nx >> 1, hny = g.ny >> 1;
+    uint4 st[NLD];
+    // issue the halo loads of item i (tile t_lo + idx + (i / stages) nb, stage i % stages)
+    auto fetch = [&](int i) {
+        const int t = t_lo + idx + (i / stages) * nb, s = i - (i / stages) * stages;
+        const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
+        const int ch0 = 32 * s;
+        const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
+        const int ch = ch0 + 8 * tgrp;
+        const bool on = tgrp < G;
+        const bool coarse = xc && ch < c1;
+        const uint16_t *base = coarse ? xc + ch : (xc ? x + (ch - c1) : x + ch);
+        const int64_t stride = coarse ? c1 : (xc ? cin - c1 : cin);
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int vox = tv0 + 64 * k;
+            st[k] = make_uint4(0u, 0u, 0u, 0u);
+            if (!on || vox >= HV) continue;
+            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+            if ((unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz) {
+                const int o = coarse ? ((gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1) : (gz * g.ny + gy) * g.nx + gx;
+                st[k] = *reinterpret_cast<const uint4 *>(base + (int64_t)o * stride);
+            }
+        }
+    };
+    f32x4 acc[RZ][RY];
+#pragma unroll
+    for (int oz = 0; oz < RZ; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < RY; ++oy) acc[oz][oy] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    Frag a[9];
+    if (items > 0) fetch(0);
+    int q = 0;  // packed chunk index of the current stage's first chunk
+    for (int i = 0; i < items; ++i) {
+        const int t = t_lo + idx + (i / stages) * nb, s = i - (i / stages) * stages;
+        const int ch0 = 32 * s;
+        const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
+        if (s == 0) q = 0;
+        __syncthreads();  // the previous item's reads are done
+#pragma unroll
+        for (int k = 0; k < NLD; ++k) {
+            const int vox = tv0 + 64 * k;
+            if (tgrp < G && vox < HV) lds[tgrp * HP + vox] = st[k];
+        }
+        __syncthreads();
+        if (i + 1 < items) fetch(i + 1);
+        const int nch = (3 * G + 3) / 4;
+        for (int c = 0; c < nch; ++c, ++q) {
+            const uint4 *cb = packed + (int64_t)q * 9 * 64 + lane;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[k].u = cb[k * 64];
+            const int e = 4 * c + kq;
+            const bool valid = e < 3 * G;
+            const int dx = valid ? div_groups(e, G) : 0;
+            const int grp = valid ? e - dx * G : 0;
+            const uint4 *lb = lds + grp * HP + ((wz * RZ) * HY + wy * RY) * HX + n + dx;
+            auto rows = [&](auto masked) {
+#pragma unroll
+                for (int iz = 0; iz < RZ + 2; ++iz)
+#pragma unroll
+                    for (int iy = 0; iy < RY + 2; ++iy) {
+                        Frag b;
+                        b.u = lb[(iz * HY + iy) * HX];
+                        if (decltype(masked)::value && !valid) b.u = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                        for (int dz = 0; dz < 3; ++dz) {
+                            const int oz = iz - dz;
+                            if (oz < 0 || oz >= RZ) continue;
+#pragma unroll
+                            for (int dy = 0; dy < 3; ++dy) {
+                                const int oy = iy - dy;
+                                if (oy < 0 || oy >= RY) continue;
+                                acc[oz][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dz * 3 + dy].v, b.v,
+                                                                                      acc[oz][oy], 0, 0, 0);
+                            }
+                        }
+                    }
+            };
+            if (4 * c + 4 <= 3 * G)
+                rows(std::false_type{});
+            else
+                rows(std::true_type{});
+        }
+        if (s + 1 < stages) continue;
+        // the tile is complete: lane holds D[co = 4*kq + j][voxel n of row (oy, oz)]
+        const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
+        const int gx = x0 + n;
+#pragma unroll
+        for (int oz = 0; oz < RZ; ++oz)
+#pragma unroll
+            for (int oy = 0; oy < RY; ++oy) {
+                const int gy = y0 + wy * RY + oy, gz = z0 + wz * RZ + oz;
+                if (gx < g.nx && gy < g.ny && gz < g.nz) {
+                    const int co = 4 * kq;
+                    float o[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        o[j] = acc[oz][oy][j] + (bias ? bias[co + j] : 0.0f);
+                        if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
+                    }
+                    uint2 po;
+                    po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
+                    po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
+                    const int64_t vb = ((int64_t)gz * g.ny + gy) * g.nx + gx;
+                    *reinterpret_cast<uint2 *>(y + vb * 16 + co) = po;
+                }
+                acc[oz][oy] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            }
+    }
+}
+
 // A fragments of k_conv_rows: element idx = (((q * 9 + dz*3 + dy) * MT + mt) * 64 + lane) * 8 + j
 // is W[tap = dz*9 + dy*3 + dx][ci][co] with co = mt*16 + lane % 16 and, for the chunk's
 // entry e = 4c + lane/16 of its stage (ch0, G groups), dx = e / G and
@@ -1487,6 +1631,18 @@ void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, co
 }
 
 namespace {
+// compute units of the current device (cached per device)
+int cu_count() {
+    static int cus[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    if (!cus[dev]) {
+        int n = 0;
+        cus[dev] = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0 ? n : 256;
+    }
+    return cus[dev];
+}
+
 // rows per wave in y and z: tiles of 16 x 4 x 8 (measured 3% faster than 16 x 8 x 4 on
 // every layer; 16 x 4 x 4 and 16 x 8 x 2 tiles re-read more halo and were slower)
 constexpr int kRowsRY = 2, kRowsRZ = 4;
@@ -1513,10 +1669,12 @@ void launch_conv_rows(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf1
     const unsigned blocks = (unsigned)((int64_t)tx * ty * tz);
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     const uint16_t *xc = coarse ? U(coarse) : nullptr;
-    if (cout == 16)
-        hipLaunchKernelGGL((k_conv_rows<1, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
-                           relu ? 1 : 0, U(y), xc, c1, tx, ty, stop);
-    else
+    if (cout == 16) {
+        // persistent: two blocks per CU (the LDS holds two halos)
+        const unsigned resident = 2u * (unsigned)cu_count();
+        hipLaunchKernelGGL((k_conv_rows_pipe<kRowsRY, kRowsRZ>), dim3(blocks < resident ? blocks : resident), dim3(256),
+                           0, st, g, U(x), cin, pk, bias, relu ? 1 : 0, U(y), xc, c1, tx, ty, (int)blocks, stop);
+    } else
         hipLaunchKernelGGL((k_conv_rows<2, kRowsRY, kRowsRZ>), dim3(blocks), dim3(256), 0, st, g, U(x), cin, pk, bias,
                            relu ? 1 : 0, U(y), xc, c1, tx, ty, stop);
 }

@@ -106,6 +106,12 @@ def test_conv3d_concatenated_input(shape, c1, cin, cout, relu, kernel):
     _check_conv_cat(shape, c1, cin, cout, relu, kernel)
 
 
+def test_conv3d_rows_several_tiles_per_block():
+    """The 16-output row-reuse kernel is persistent (two blocks per CU): 1024 tiles of
+    16 x 4 x 8 voxels give every block several tiles and both stages of each in turn."""
+    _check_conv_cat((64, 64, 128), 32, 48, 16, True, 2)
+
+
 def _check_conv_cat(shape, c1, cin, cout, relu, kernel):
     from shiokaze_amd import learned
     log(f"[gpu] conv3d cat {shape} c1={c1} {cin}->{cout} relu={relu} kernel={kernel}")
