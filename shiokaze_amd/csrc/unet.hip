@@ -633,6 +633,7 @@ void unet_destroy(UnetPrecond *u) {
 namespace {
 
 constexpr int kPartCap = 2 * kMaxBlocks;
+static_assert(kPartCap >= kSmoothParts, "part3 holds the last smoothing step's partials");
 
 __global__ void k_brick_mask(int nx, int ny, int nz, const int4 *__restrict__ coord, const uint8_t *__restrict__ label,
                              float *__restrict__ mask) {

@@ -160,7 +160,8 @@ void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float 
 // zout = zin + omega * mask * (r - A zin), A the unit-diagonal operator with low-face
 // couplings cx, cy, cz, evaluated in the order and rounding of the dense stencil and
 // smoothing kernels.  With q and part: part[block] = sum zout * q (FP64), at most
-// 1024 blocks; returns the number of blocks.
+// kSmoothParts blocks; returns the number of blocks.
+constexpr int kSmoothParts = 2048;
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
                          const float *q, double *part, const int32_t *stop = nullptr);

@@ -1394,8 +1394,9 @@ void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float 
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
                          const float *q, double *part, const int32_t *stop) {
-    const int want = (b.n_active + 3) / 4;  // 4 bricks per block and pass
-    const int blocks = want < 1024 ? (want > 0 ? want : 1) : 1024;
+    // 4 bricks per block and pass; with part, at most kSmoothParts blocks (partials)
+    const int want = (b.n_active + 3) / 4;
+    const int blocks = want < 1 ? 1 : (part && want > kSmoothParts ? kSmoothParts : want);
     hipLaunchKernelGGL(k_smooth_bricks, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout, q,
                        part, stop);
     return blocks;
