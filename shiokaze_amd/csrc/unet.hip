@@ -182,9 +182,7 @@ struct UnetNet {
     bf16 *pk[kLayers] = {}, *pkt[kLayers] = {};
     bf16 *pkr[kLayers] = {};      // row-reuse packing of the 16/32-output layers (inference)
     uint32_t *head_pk = nullptr;  // the head's output-channel-0 weights as BF16 pairs (inference)
-    float *split_ws = nullptr;    // K-split partial sums of the small-grid convolutions (inference)
     const int32_t *stop = nullptr;  // inference kernels return at once while *stop is set
-    int64_t split_cap = 0;
     bool packed_dirty = true;
     // activations for the current half-resolution grid
     Grid3 L[4] = {};
@@ -250,14 +248,6 @@ int ensure_grid(UnetNet *n, Grid3 m) {
     for (auto &x : a) UHIP(n, dalloc(*x.p, (size_t)x.n));
     UHIP(n, dalloc(n->u, (size_t)v0));
     UHIP(n, dalloc(n->du, (size_t)v0));
-    // the K-split workspace of the inference convolutions on this grid
-    int64_t cap = 0;
-    for (int l = 0; l < kLayers - 1; ++l) {
-        const int lev = l < 4 ? l : kLayers - 2 - l;  // enc0..bott on levels 0..3, dec2..dec0 on 2..0
-        cap = std::max(cap, conv_split_floats(n->L[lev], kSpec[l].cin, kSpec[l].cout));
-    }
-    if (cap > 0) UHIP(n, dalloc(n->split_ws, (size_t)cap));
-    n->split_cap = cap;
     return SHK_OK;
 }
 
@@ -306,10 +296,6 @@ void wgrad(UnetNet *n, int l, Grid3 g, const bf16 *x, const bf16 *dz) {
 // otherwise each decoder convolution reads [upsample(coarse), skip] directly.
 void forward(UnetNet *n, bool train) {
     const Grid3 *L = n->L;
-    // inference splits the K loop of the small grids' convolutions (training keeps the
-    // unsplit kernels its gradient checks were made with)
-    float *ws = train ? nullptr : n->split_ws;
-    const int64_t wc = train ? 0 : n->split_cap;
     // inference runs the row-reuse kernel where a layer's grid fills the chip
     auto rows = [&](int l, Grid3 g) { return !train && n->pkr[l] && conv_rows_ok(g, kSpec[l].cin, kSpec[l].cout); };
     // and the direct-load kernel on the grids too small for the tiled kernel
@@ -323,7 +309,7 @@ void forward(UnetNet *n, bool train) {
                                kSpec[l].relu, y, nullptr, 0, n->stop);
         else
             launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
-                            kSpec[l].relu, y, nullptr, ws, wc, n->stop);
+                            kSpec[l].relu, y, nullptr, nullptr, 0, n->stop);
     };
     enc(0, L[0], n->feat, n->enc0);
     launch_pool(n->stream, L[1], 16, n->enc0, n->p1, n->stop);
@@ -342,7 +328,7 @@ void forward(UnetNet *n, bool train) {
                                    kSpec[l].cout, kSpec[l].relu, y, coarse, c1, n->stop);
             else
                 launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                    kSpec[l].cout, kSpec[l].relu, y, n->split_ws, n->split_cap, n->stop);
+                                    kSpec[l].cout, kSpec[l].relu, y, nullptr, 0, n->stop);
         };
         cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
         cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);
@@ -440,7 +426,6 @@ void unet_net_destroy(UnetNet *n) {
         dfree(n->pkr[l]);
     }
     dfree(n->head_pk);
-    dfree(n->split_ws);
     bf16 **acts[] = {&n->feat, &n->enc0, &n->p1,   &n->enc1, &n->p2,  &n->enc2,  &n->p3,   &n->bott,
                      &n->cat2, &n->dec2, &n->cat1, &n->dec1, &n->cat0, &n->dec0, &n->dout, &n->d16,
                      &n->dcat0, &n->dd1, &n->dcat1, &n->dd2, &n->dcat2, &n->dbott, &n->dp3, &n->de2,
