@@ -84,7 +84,9 @@ __host__ __device__ inline void fcg_decide(FcgState &v, double r2, double rn, bo
 
 // What the first kernel of a network application needs to apply fcg_decide: the
 // axpy's 2 x nparts partials (r.r, max|r|), its flags fst (fst[2] != 0: bad), the
-// state before (in) and where the state after goes (out, written by block 0).
+// state before (in) and where the state after goes (out, written by block 0, and
+// host: pinned host memory the host reads once the iteration's event has completed;
+// nullable).
 struct FcgCtl {
     const double *part2;
     const double *fst;
@@ -92,6 +94,7 @@ struct FcgCtl {
     double nunk;
     const FcgState *in;
     FcgState *out;
+    FcgState *host;
 };
 
 // Workgroup -> slot mapping that hands each XCD one contiguous range of slots.

@@ -598,7 +598,7 @@ struct UnetPrecond {
     double *h_out = nullptr;            // pinned: part2 (2 x 1024) and fst (3)
     static constexpr int kRing = 2;     // inner-solve states in flight to the host
     FcgState *fs = nullptr;             // the inner solve's device-side state (two: before, after)
-    FcgState *h_fs = nullptr;           // pinned: kRing copies of it
+    FcgState *h_fs = nullptr;           // pinned, coherent: kRing copies of it, stored by the GPU
     hipEvent_t fev[kRing] = {};         // each copy's completion
 };
 
@@ -652,7 +652,7 @@ int ensure_precond(shk_ctx *c) {
         HIPC(c, dalloc(u->fst, 4));
         HIPC(c, hipHostMalloc((void **)&u->h_out, sizeof(double) * (kPartCap + 4), hipHostMallocDefault));
         HIPC(c, hipMalloc((void **)&u->fs, 2 * sizeof(FcgState)));
-        HIPC(c, hipHostMalloc((void **)&u->h_fs, sizeof(FcgState) * UnetPrecond::kRing, hipHostMallocDefault));
+        HIPC(c, hipHostMalloc((void **)&u->h_fs, sizeof(FcgState) * UnetPrecond::kRing, hipHostMallocCoherent));
         for (hipEvent_t &e : u->fev) HIPC(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     UnetPrecond *u = c->unet;
@@ -821,7 +821,9 @@ int unet_solve(shk_ctx *c, shk_result *res) {
                     // the state before this iteration's update, and after it
                     FcgState *fin = u->fs + (enq & 1), *fout = u->fs + ((enq + 1) & 1);
                     launch_fcg_axpy(st, g, x, r, P[cur], q, u->part1, u->part2, u->fst, false, fin);
-                    const FcgCtl ctl{u->part2, u->fst, g.blocks, nunk, fin, fout};
+                    const int slot = (int)(enq % R);
+                    // the features kernel stores the state after this update in h_fs[slot] too
+                    const FcgCtl ctl{u->part2, u->fst, g.blocks, nunk, fin, fout, u->h_fs + slot};
                     int nz = 0;
                     if ((rc = timed_precondition(c, r, 1.0, false, q, &nz, &ctl))) return rc;
                     // the next direction and its stencil product
@@ -829,8 +831,6 @@ int unet_solve(shk_ctx *c, shk_result *res) {
                                         FcgDir{fout, u->part3, nz, u->fst, r, u->part1});
                     cur ^= 1;
                     HIPC(c, hipGetLastError());
-                    const int slot = (int)(enq % R);
-                    HIPC(c, hipMemcpyAsync(u->h_fs + slot, fout, sizeof(FcgState), hipMemcpyDeviceToHost, st));
                     HIPC(c, hipEventRecord(u->fev[slot], st));
                     ++enq;
                     // the state after the previous iteration (this one is still running)

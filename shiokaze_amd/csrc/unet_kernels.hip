@@ -1174,7 +1174,10 @@ __global__ __launch_bounds__(512) void k_features_bricks_r(Grid3 c, BrickView b,
     if (ctl.in) {
         const FcgState v0 = *ctl.in;
         if (v0.stop) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) *ctl.out = v0;
+            if (blockIdx.x == 0 && threadIdx.x == 0) {
+                *ctl.out = v0;
+                if (ctl.host) *ctl.host = v0;
+            }
             return;
         }
         const double r2 = reduce_partials_sum<512>(ctl.part2, ctl.nparts, lds);
@@ -1182,7 +1185,10 @@ __global__ __launch_bounds__(512) void k_features_bricks_r(Grid3 c, BrickView b,
         if (threadIdx.x == 0) {
             FcgState v = v0;
             fcg_decide(v, r2, rn, ctl.fst[2] != 0.0, ctl.nunk);
-            if (blockIdx.x == 0) *ctl.out = v;
+            if (blockIdx.x == 0) {
+                *ctl.out = v;
+                if (ctl.host) *ctl.host = v;
+            }
             sc = v.scale;
             stp = v.stop;
         }
@@ -1375,7 +1381,7 @@ void launch_features_bricks(hipStream_t st, Grid3 coarse, BrickView b, const flo
     if (only_r) {
         // channel 0 of the coarse voxels over the active bricks (4^3 per brick); the
         // others hold the zero a full pass wrote there and keep it
-        const FcgCtl none{nullptr, nullptr, 0, 1.0, nullptr, nullptr};
+        const FcgCtl none{nullptr, nullptr, 0, 1.0, nullptr, nullptr, nullptr};
         if (b.n_active > 0)
             hipLaunchKernelGGL(k_features_bricks_r, dim3((unsigned)((b.n_active + 7) / 8)), dim3(512), 0, st, coarse,
                                b, r, inv_scale, ctl ? *ctl : none, U(feat));
