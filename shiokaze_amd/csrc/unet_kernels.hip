@@ -515,9 +515,10 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows(Grid3 g, const uint16_t *_
 // k_conv_rows for one output tile of 16 channels, persistent and software-pipelined:
 // the grid holds as many blocks as fit on the chip at once, the tiles are cut into one
 // contiguous range per XCD (blocks b, b + 8, ... share XCD b % 8 and take its tiles in
-// turn), and each block walks its (tile, stage) items with the next item's halo
-// loads in flight (registers) while the MFMAs of the current one run.  The arithmetic
-// of every tile is k_conv_rows<1, RY, RZ>'s, in the same order.
+// turn), and within a chunk the B rows are read from LDS two rows ahead of the MFMAs
+// that use them and the next chunk's A fragments are loaded while this chunk's MFMAs
+// run (the chunks repeat from tile to tile).  The arithmetic of every tile is
+// k_conv_rows<1, RY, RZ>'s, in the same order.
 template <int RY, int RZ>
 __global__ __launch_bounds__(256, 2) void k_conv_rows_pipe(Grid3 g, const uint16_t *__restrict__ x, int cin,
                                                            const uint4 *__restrict__ packed,
@@ -530,6 +531,7 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows_pipe(Grid3 g, const uint16
     constexpr int HV = HX * HY * HZ;
     constexpr int HP = (HV + 11) / 16 * 16 + 4;
     constexpr int NLD = (HV + 63) / 64;
+    constexpr int NR = (RZ + 2) * (RY + 2);  // input rows of a wave per chunk
     __shared__ uint4 lds[4 * HP];
     if (stop && *stop) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -541,96 +543,114 @@ __global__ __launch_bounds__(256, 2) void k_conv_rows_pipe(Grid3 g, const uint16
     const int groups = (int)gridDim.x < kXcds ? (int)gridDim.x : kXcds;  // XCDs that hold blocks
     const int t_lo = (int)((int64_t)ntiles * xcd / groups), t_hi = (int)((int64_t)ntiles * (xcd + 1) / groups);
     const int stages = (cin + 31) / 32;
+    int nq = 0;  // chunks per tile
+    for (int s = 0; s < stages; ++s) {
+        const int G = (cin - 32 * s) / 8 < 4 ? (cin - 32 * s) / 8 : 4;
+        nq += (3 * G + 3) / 4;
+    }
     const int my_tiles = t_lo + idx < t_hi ? (t_hi - t_lo - idx + nb - 1) / nb : 0;
     const int items = my_tiles * stages;
     const int tgrp = threadIdx.x & 3, tv0 = threadIdx.x >> 2;
     const int hnx = g.nx >> 1, hny = g.ny >> 1;
-    uint4 st[NLD];
-    // issue the halo loads of item i (tile t_lo + idx + (i / stages) nb, stage i % stages)
-    auto fetch = [&](int i) {
-        const int t = t_lo + idx + (i / stages) * nb, s = i - (i / stages) * stages;
-        const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
-        const int ch0 = 32 * s;
-        const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
-        const int ch = ch0 + 8 * tgrp;
-        const bool on = tgrp < G;
-        const bool coarse = xc && ch < c1;
-        const uint16_t *base = coarse ? xc + ch : (xc ? x + (ch - c1) : x + ch);
-        const int64_t stride = coarse ? c1 : (xc ? cin - c1 : cin);
-#pragma unroll
-        for (int k = 0; k < NLD; ++k) {
-            const int vox = tv0 + 64 * k;
-            st[k] = make_uint4(0u, 0u, 0u, 0u);
-            if (!on || vox >= HV) continue;
-            const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
-            const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
-            if ((unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny && (unsigned)gz < (unsigned)g.nz) {
-                const int o = coarse ? ((gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1) : (gz * g.ny + gy) * g.nx + gx;
-                st[k] = *reinterpret_cast<const uint4 *>(base + (int64_t)o * stride);
-            }
-        }
-    };
     f32x4 acc[RZ][RY];
 #pragma unroll
     for (int oz = 0; oz < RZ; ++oz)
 #pragma unroll
         for (int oy = 0; oy < RY; ++oy) acc[oz][oy] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    Frag a[9];
-    if (items > 0) fetch(0);
-    int q = 0;  // packed chunk index of the current stage's first chunk
+    Frag a[9], an[9];
+    if (items > 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k].u = packed[k * 64 + lane];
+    }
+    int q = 0;  // chunk of the tile
     for (int i = 0; i < items; ++i) {
         const int t = t_lo + idx + (i / stages) * nb, s = i - (i / stages) * stages;
+        const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
         const int ch0 = 32 * s;
         const int G = (cin - ch0) / 8 < 4 ? (cin - ch0) / 8 : 4;
         if (s == 0) q = 0;
-        __syncthreads();  // the previous item's reads are done
+        {
+            // this stage's halo: thread t loads channel group t % 4 of the halo voxels
+            // t / 4 + 64 k, all loads issued before any is written to LDS
+            const int ch = ch0 + 8 * tgrp;
+            const bool on = tgrp < G;
+            const bool coarse = xc && ch < c1;
+            const uint16_t *base = coarse ? xc + ch : (xc ? x + (ch - c1) : x + ch);
+            const int64_t stride = coarse ? c1 : (xc ? cin - c1 : cin);
+            uint4 st[NLD];
 #pragma unroll
-        for (int k = 0; k < NLD; ++k) {
-            const int vox = tv0 + 64 * k;
-            if (tgrp < G && vox < HV) lds[tgrp * HP + vox] = st[k];
+            for (int k = 0; k < NLD; ++k) {
+                const int vox = tv0 + 64 * k;
+                st[k] = make_uint4(0u, 0u, 0u, 0u);
+                if (!on || vox >= HV) continue;
+                const int hx = vox % HX, hy = (vox / HX) % HY, hz = vox / (HX * HY);
+                const int gx = x0 + hx - 1, gy = y0 + hy - 1, gz = z0 + hz - 1;
+                if ((unsigned)gx < (unsigned)g.nx && (unsigned)gy < (unsigned)g.ny &&
+                    (unsigned)gz < (unsigned)g.nz) {
+                    const int o =
+                        coarse ? ((gz >> 1) * hny + (gy >> 1)) * hnx + (gx >> 1) : (gz * g.ny + gy) * g.nx + gx;
+                    st[k] = *reinterpret_cast<const uint4 *>(base + (int64_t)o * stride);
+                }
+            }
+            __syncthreads();  // the previous item's reads are done
+#pragma unroll
+            for (int k = 0; k < NLD; ++k) {
+                const int vox = tv0 + 64 * k;
+                if (on && vox < HV) lds[tgrp * HP + vox] = st[k];
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        if (i + 1 < items) fetch(i + 1);
         const int nch = (3 * G + 3) / 4;
         for (int c = 0; c < nch; ++c, ++q) {
-            const uint4 *cb = packed + (int64_t)q * 9 * 64 + lane;
+            // the next chunk's A fragments (chunk 0 again after the tile's last)
+            const int qn = q + 1 < nq ? q + 1 : 0;
+            const uint4 *cn = packed + (int64_t)qn * 9 * 64 + lane;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a[k].u = cb[k * 64];
+            for (int k = 0; k < 9; ++k) an[k].u = cn[k * 64];
             const int e = 4 * c + kq;
             const bool valid = e < 3 * G;
             const int dx = valid ? div_groups(e, G) : 0;
             const int grp = valid ? e - dx * G : 0;
             const uint4 *lb = lds + grp * HP + ((wz * RZ) * HY + wy * RY) * HX + n + dx;
             auto rows = [&](auto masked) {
+                Frag bq[3];
+                bq[0].u = lb[0];
+                bq[1].u = lb[(1 / (RY + 2) * HY + 1 % (RY + 2)) * HX];
 #pragma unroll
-                for (int iz = 0; iz < RZ + 2; ++iz)
+                for (int r = 0; r < NR; ++r) {
+                    const int iz = r / (RY + 2), iy = r % (RY + 2);
+                    if (r + 2 < NR) {
+                        const int r2 = r + 2;
+                        bq[r2 % 3].u = lb[((r2 / (RY + 2)) * HY + r2 % (RY + 2)) * HX];
+                    }
+                    // keep the read above this row's MFMAs (the scheduler would sink it
+                    // to its use, leaving the LDS latency uncovered)
+                    __builtin_amdgcn_sched_barrier(0);
+                    Frag b = bq[r % 3];
+                    if (decltype(masked)::value && !valid) b.u = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
-                    for (int iy = 0; iy < RY + 2; ++iy) {
-                        Frag b;
-                        b.u = lb[(iz * HY + iy) * HX];
-                        if (decltype(masked)::value && !valid) b.u = make_uint4(0u, 0u, 0u, 0u);
+                    for (int dz = 0; dz < 3; ++dz) {
+                        const int oz = iz - dz;
+                        if (oz < 0 || oz >= RZ) continue;
 #pragma unroll
-                        for (int dz = 0; dz < 3; ++dz) {
-                            const int oz = iz - dz;
-                            if (oz < 0 || oz >= RZ) continue;
-#pragma unroll
-                            for (int dy = 0; dy < 3; ++dy) {
-                                const int oy = iy - dy;
-                                if (oy < 0 || oy >= RY) continue;
-                                acc[oz][oy] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dz * 3 + dy].v, b.v,
-                                                                                      acc[oz][oy], 0, 0, 0);
-                            }
+                        for (int dy = 0; dy < 3; ++dy) {
+                            const int oy = iy - dy;
+                            if (oy < 0 || oy >= RY) continue;
+                            acc[oz][oy] =
+                                __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[dz * 3 + dy].v, b.v, acc[oz][oy], 0, 0, 0);
                         }
                     }
+                }
             };
             if (4 * c + 4 <= 3 * G)
                 rows(std::false_type{});
             else
                 rows(std::true_type{});
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[k] = an[k];
         }
         if (s + 1 < stages) continue;
         // the tile is complete: lane holds D[co = 4*kq + j][voxel n of row (oy, oz)]
-        const int x0 = (t % tx) * 16, y0 = ((t / tx) % ty) * TY, z0 = (t / (tx * ty)) * TZ;
         const int gx = x0 + n;
 #pragma unroll
         for (int oz = 0; oz < RZ; ++oz)
