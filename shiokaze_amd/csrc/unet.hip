@@ -309,7 +309,7 @@ void forward(UnetNet *n, bool train) {
                                kSpec[l].relu, y, nullptr, 0, n->stop);
         else
             launch_conv_fwd(n->stream, g, x, kSpec[l].cin, n->pk[l], n->param + n->boff[l], kSpec[l].cout,
-                            kSpec[l].relu, y, nullptr, nullptr, 0, n->stop);
+                            kSpec[l].relu, y, nullptr, n->stop);
     };
     enc(0, L[0], n->feat, n->enc0);
     launch_pool(n->stream, L[1], 16, n->enc0, n->p1, n->stop);
@@ -328,7 +328,7 @@ void forward(UnetNet *n, bool train) {
                                    kSpec[l].cout, kSpec[l].relu, y, coarse, c1, n->stop);
             else
                 launch_conv_fwd_cat(n->stream, g, coarse, c1, skip, kSpec[l].cin, n->pk[l], n->param + n->boff[l],
-                                    kSpec[l].cout, kSpec[l].relu, y, nullptr, 0, n->stop);
+                                    kSpec[l].cout, kSpec[l].relu, y, n->stop);
         };
         cat(4, L[2], n->bott, 128, n->enc2, n->dec2);
         cat(5, L[1], n->dec2, 64, n->enc1, n->dec1);

@@ -45,23 +45,18 @@ void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool tr
 
 // y[v][co] = act(bias[co] + sum_k w * x); act = ReLU when relu.  out_f32 (nullable)
 // receives channel 0 in FP32 instead of bf16 output (the network's final layer).
-// ws (ws_floats floats, may be null): the workspace of K splits on grids too small to
-// fill the chip (conv_split_floats); without it the convolution runs unsplit.
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32, float *ws = nullptr, int64_t ws_floats = 0,
-                     const int32_t *stop = nullptr);
-// workspace floats a BF16-output convolution of this shape splits into (0: unsplit)
-int64_t conv_split_floats(Grid3 g, int cin, int cout);
+                     bool relu, bf16 *y, float *out_f32, const int32_t *stop = nullptr);
 
 // The same convolution of the concatenation [nearest upsampling of coarse (c1
 // channels, grid g/2), skip (cin - c1 channels, grid g)], read from the two tensors
 // directly; c1 and cin - c1 are multiples of 8.
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
-                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws = nullptr,
-                         int64_t ws_floats = 0, const int32_t *stop = nullptr);
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y,
+                         const int32_t *stop = nullptr);
 
 // Direct-load forward convolution for grids too small for the tiled kernel to fill the
-// chip (the grids on which it splits K over blocks): same packed weights as
+// chip (fewer than 1024 of its blocks): same packed weights as
 // launch_conv_fwd, K split over the waves of a block and added in LDS, BF16 output
 // (coarse != null: the input is [upsample(coarse) (c1 channels), x (cin - c1)]).
 bool conv_direct_ok(Grid3 g, int cin, int cout);

@@ -89,18 +89,12 @@ constexpr int kHalo = kHX * kHY * kHZ;
 // With xc: the input is the concatenation [upsample(xc), x] of the nearest upsampling
 // of the half-size grid xc (its first c1 channels) and x (the other cin - c1), read
 // straight from the two tensors while staging (no concatenated buffer).
-//
-// Split K (small grids, split_out != nullptr): blockIdx.z also indexes one of nsplit splits of
-// the stages x KPC weight chunks; a split stages only the halos of the stages its chunk
-// range touches, and writes its raw FP32 sums to part[split][voxel][cout] for
-// k_conv_split_finish, which adds the splits in order, the bias and the ReLU
-// (split_out == nullptr: one split, the epilogue applies them).
 template <int MTB, int CC>
 __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ x, int cin, int mt_total,
                                                   const uint4 *__restrict__ packed, const float *__restrict__ bias,
                                                   int relu, uint16_t *__restrict__ y, float *__restrict__ out_f32,
-                                                  const uint16_t *__restrict__ xc, int c1, int nsplit,
-                                                  float *__restrict__ split_out, const int32_t *__restrict__ stop) {
+                                                  const uint16_t *__restrict__ xc, int c1,
+                                                  const int32_t *__restrict__ stop) {
     constexpr int G8 = CC / 8;        // 16-byte groups per voxel and stage
     // voxel stride in uint4, odd so that 16 lanes reading 16 consecutive voxels hit 16
     // disjoint 4-bank groups of the 64-bank LDS
@@ -112,21 +106,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int n = lane & 15, kq = lane >> 4;
     const int mgroups = mt_total / MTB;
-    const int zs = blockIdx.z / nsplit, sp = blockIdx.z - zs * nsplit;
-    const int ztile = zs / mgroups, mg = zs - ztile * mgroups;
+    const int ztile = (int)blockIdx.z / mgroups, mg = (int)blockIdx.z - ztile * mgroups;
     const int x0 = blockIdx.x * kTX, y0 = blockIdx.y * kTY, z0 = ztile * kTZ;
     const int stages = cin / CC;
     const int kchunks = stages * KPC;
-    // this block's weight chunks [c_lo, c_hi) of the stages x KPC (all of them unsplit)
-    const int cper = (kchunks + nsplit - 1) / nsplit;
-    const int c_lo = sp * cper, c_hi = c_lo + cper < kchunks ? c_lo + cper : kchunks;
     f32x4 acc[MTB][4];
 #pragma unroll
     for (int mt = 0; mt < MTB; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int stage = 0; stage < stages; ++stage) {
-        if ((stage + 1) * KPC <= c_lo || stage * KPC >= c_hi) continue;  // block-uniform
         __syncthreads();  // the previous stage's reads are done
         for (int idx = threadIdx.x; idx < kHalo * G8; idx += 256) {
             const int vox = idx / G8, part = idx - vox * G8;
@@ -161,10 +150,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
 #pragma unroll
         for (int kc = 0; kc < KPC; ++kc) {
             const int k0 = kc * 32 + kq * 8;
-            const int gc = stage * KPC + kc;
-            const bool on = gc >= c_lo && gc < c_hi;  // block-uniform: this split's chunk
             Frag b[4];
-            if (on && k0 < 27 * CC) {
+            if (k0 < 27 * CC) {
                 const int tap = k0 / CC, grp = (k0 - tap * CC) / 8;
                 int dx, dy, dz;
                 tap_offset(tap, dx, dy, dz);
@@ -181,7 +168,6 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
                 a[mt] = ar[kc % PF][mt];
                 if (kc + PF < KPC) ar[kc % PF][mt].u = abase[((int64_t)mt * kchunks + kc + PF) * 64];
             }
-            if (!on) continue;
 #pragma unroll
             for (int mt = 0; mt < MTB; ++mt)
 #pragma unroll
@@ -201,11 +187,6 @@ __global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__res
 #pragma unroll
         for (int mt = 0; mt < MTB; ++mt) {
             const int co = (mg * MTB + mt) * 16 + 4 * kq;
-            if (split_out) {
-                *reinterpret_cast<float4 *>(split_out + ((int64_t)sp * g.vox() + vb) * cout + co) =
-                    make_float4(acc[mt][nt][0], acc[mt][nt][1], acc[mt][nt][2], acc[mt][nt][3]);
-                continue;
-            }
             float o[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -1451,63 +1432,37 @@ void launch_head_pack(hipStream_t st, const float *w, int cout, uint32_t *wpk) {
 namespace {
 template <int MTB>
 void conv_fwd_mt(hipStream_t st, dim3 grid, Grid3 g, const bf16 *x, int cin, int mt, const uint4 *pk,
-                 const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1, int nsplit,
-                 float *split_out, const int32_t *stop) {
+                 const float *bias, bool relu, uint16_t *yy, float *out_f32, const uint16_t *xc, int c1,
+                 const int32_t *stop) {
     switch (conv_cc(cin)) {
         case 8:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 8>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out, stop);
+                               out_f32, xc, c1, stop);
             break;
         case 16:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 16>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out, stop);
+                               out_f32, xc, c1, stop);
             break;
         case 24:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 24>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out, stop);
+                               out_f32, xc, c1, stop);
             break;
         default:
             hipLaunchKernelGGL((k_conv_fwd<MTB, 32>), grid, dim3(256), 0, st, g, U(x), cin, mt, pk, bias, relu, yy,
-                               out_f32, xc, c1, nsplit, split_out, stop);
+                               out_f32, xc, c1, stop);
             break;
     }
 }
 
-// y[v][c] = relu(bias[c] + sum over the splits of part[split][v][c]), in split order
-__global__ void k_conv_split_finish(int64_t n, int cout, int nsplit, const float *__restrict__ part,
-                                    const float *__restrict__ bias, int relu, uint16_t *__restrict__ y,
-                                    const int32_t *__restrict__ stop) {
-    if (stop && *stop) return;
-    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;  // 4 channels of one voxel
-    if (q * 4 >= n) return;
-    float4 s = *reinterpret_cast<const float4 *>(part + q * 4);
-    for (int k = 1; k < nsplit; ++k) {
-        const float4 t = *reinterpret_cast<const float4 *>(part + (int64_t)k * n + q * 4);
-        s.x += t.x;
-        s.y += t.y;
-        s.z += t.z;
-        s.w += t.w;
-    }
-    const int c = (int)((q * 4) % cout);
-    float o[4] = {s.x, s.y, s.z, s.w};
-    for (int j = 0; j < 4; ++j) {
-        o[j] = o[j] + (bias ? bias[c + j] : 0.0f);
-        if (relu) o[j] = o[j] > 0.0f ? o[j] : 0.0f;
-    }
-    uint2 po;
-    po.x = (uint32_t)bf16_bits(o[0]) | ((uint32_t)bf16_bits(o[1]) << 16);
-    po.y = (uint32_t)bf16_bits(o[2]) | ((uint32_t)bf16_bits(o[3]) << 16);
-    *reinterpret_cast<uint2 *>(y + q * 4) = po;
-}
-
 struct ConvPlan {
-    int bx, by, bz, mtb, nsplit;
+    int bx, by, bz, mtb;
+    bool small;  // fewer than 1024 blocks and at least 8 weight chunks: the direct kernel's grids
 };
 
 // Blocks of the convolution: output tiles per block (all of them, unless the grid is
 // too small to fill the chip, then the largest divisor of mt that gives >= 1024
-// blocks, or 1), and, when even that leaves fewer than 1024 blocks, K splits of at
-// least 4 weight chunks each up to about 1024 blocks.
+// blocks, or 1).  A layer that even then has fewer than 1024 blocks (and at least 8
+// weight chunks to share out) is a small-grid layer: the direct-load kernel takes it.
 ConvPlan conv_plan(Grid3 g, int cin, int cout) {
     ConvPlan p;
     const int mt = cout / 16;
@@ -1523,47 +1478,26 @@ ConvPlan conv_plan(Grid3 g, int cin, int cout) {
     }
     const int64_t blocks = spatial * (mt / p.mtb);
     const int chunks = (cin / conv_cc(cin)) * conv_kpc(cin);
-    p.nsplit = 1;
-    if (blocks < 1024) {
-        const int64_t want = (1024 + blocks - 1) / blocks;
-        const int most = chunks / 4;
-        p.nsplit = (int)std::max<int64_t>(1, std::min<int64_t>(want, most));
-    }
+    p.small = blocks < 1024 && chunks >= 8;
     return p;
 }
-}  // namespace
 
-int64_t conv_split_floats(Grid3 g, int cin, int cout) {
-    const ConvPlan p = conv_plan(g, cin, cout);
-    return p.nsplit > 1 ? (int64_t)p.nsplit * g.vox() * cout : 0;
-}
-
-namespace {
 void conv_fwd_any(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1, float *ws, int64_t ws_floats,
-                  const int32_t *stop) {
+                  bool relu, bf16 *y, float *out_f32, const uint16_t *xc, int c1, const int32_t *stop) {
     const int mt = cout / 16;
-    ConvPlan p = conv_plan(g, cin, cout);
-    // K splits need a workspace of nsplit x voxels x cout floats and a BF16 output
-    if (p.nsplit > 1 && (!ws || out_f32 || !y || (int64_t)p.nsplit * g.vox() * cout > ws_floats)) p.nsplit = 1;
-    const dim3 grid((unsigned)p.bx, (unsigned)p.by, (unsigned)(p.bz * (mt / p.mtb) * p.nsplit));
+    const ConvPlan p = conv_plan(g, cin, cout);
+    const dim3 grid((unsigned)p.bx, (unsigned)p.by, (unsigned)(p.bz * (mt / p.mtb)));
     const uint4 *pk = reinterpret_cast<const uint4 *>(packed);
     uint16_t *yy = y ? U(y) : nullptr;
-    float *so = p.nsplit > 1 ? ws : nullptr;
     switch (p.mtb) {
-        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
-        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, p.nsplit, so, stop); break;
+        case 1: conv_fwd_mt<1>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 2: conv_fwd_mt<2>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 3: conv_fwd_mt<3>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 4: conv_fwd_mt<4>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 6: conv_fwd_mt<6>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 8: conv_fwd_mt<8>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
+        case 12: conv_fwd_mt<12>(st, grid, g, x, cin, mt, pk, bias, relu, yy, out_f32, xc, c1, stop); break;
         default: return;  // unsupported widths are rejected by the caller
-    }
-    if (so) {
-        const int64_t n = g.vox() * cout;
-        hipLaunchKernelGGL(k_conv_split_finish, dim3((unsigned)((n / 4 + 255) / 256)), dim3(256), 0, st, n, cout,
-                           p.nsplit, so, bias, relu ? 1 : 0, yy, stop);
     }
 }
 }  // namespace
@@ -1625,7 +1559,7 @@ void conv_direct_mw(hipStream_t st, const DirectPlan &p, Grid3 g, const uint16_t
 
 bool conv_direct_ok(Grid3 g, int cin, int cout) {
     if (cin % 8 || cout % 16 || cout <= 0 || g.vox() >= (1ll << 31)) return false;
-    return conv_plan(g, cin, cout).nsplit > 1;  // the tiled kernel cannot fill the chip
+    return conv_plan(g, cin, cout).small;  // the tiled kernel cannot fill the chip
 }
 
 void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias,
@@ -1647,14 +1581,14 @@ void launch_conv_direct(hipStream_t st, Grid3 g, const bf16 *x, int cin, const b
 }
 
 void launch_conv_fwd(hipStream_t st, Grid3 g, const bf16 *x, int cin, const bf16 *packed, const float *bias, int cout,
-                     bool relu, bf16 *y, float *out_f32, float *ws, int64_t ws_floats, const int32_t *stop) {
-    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0, ws, ws_floats, stop);
+                     bool relu, bf16 *y, float *out_f32, const int32_t *stop) {
+    conv_fwd_any(st, g, x, cin, packed, bias, cout, relu, y, out_f32, nullptr, 0, stop);
 }
 
 void launch_conv_fwd_cat(hipStream_t st, Grid3 g, const bf16 *coarse, int c1, const bf16 *skip, int cin,
-                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y, float *ws,
-                         int64_t ws_floats, const int32_t *stop) {
-    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, ws, ws_floats, stop);
+                         const bf16 *packed, const float *bias, int cout, bool relu, bf16 *y,
+                         const int32_t *stop) {
+    conv_fwd_any(st, g, skip, cin, packed, bias, cout, relu, y, nullptr, U(coarse), c1, stop);
 }
 
 namespace {
