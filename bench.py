@@ -624,6 +624,7 @@ def main():
             "precision": args.precision, "preconditioner": args.preconditioner,
             "ms_setup": round(last["ms_setup"], 3), "ms_solve": round(last["ms_solve"], 3),
             "ms_update": round(last["ms_update"], 3),
+            **({"fallback": last.get("fallback", 0), "set": args.set} if args.preconditioner == "unet" else {}),
         },
         "roofline": roofline,
         "kernels": kernels,

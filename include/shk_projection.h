@@ -130,6 +130,11 @@ void shk_destroy(shk_ctx *ctx);
  *   CheckInterval  iterations launched between host convergence polls (32)
  *   HostPointers   0 | 1                   (0)
  *   UnetWeights    path of a trained preconditioner blob (for Preconditioner=unet)
+ *   UnetSmoothing  smoothing steps after each network application inside the
+ *                  solve, 1..64 (10); the training loss uses 2
+ *   UnetChebyshev  ratio > 1: the steps use Chebyshev weights for the eigenvalue
+ *                  interval [2 / ratio, 2] of the Jacobi-scaled operator; 0: every
+ *                  step uses the weights file's omega (16)
  *   Report         0 | 1: RCMatrix_utility::report statistics of every assembled
  *                  system into shk_result.report_* (RCMatrix_utility.h:98-146) (0)
  *   ProfileKernels 0 | 1: time every Krylov kernel launch with HIP events on the

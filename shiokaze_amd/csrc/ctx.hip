@@ -576,6 +576,13 @@ int shk_configure(shk_ctx *c, const char *key, const char *value) {
         if (!parse_double(value, d) || !(d > 0.0) || !(d < 2.0))
             return c->fail(SHK_BAD_ARG, "bad %s '%s' (0 < value < 2)", k.c_str(), value);
         (k == "MgOmega" ? p.mg_omega : p.mg_alpha) = d;
+    } else if (k == "UnetChebyshev") {
+        if (!parse_double(value, d) || !(d == 0.0 || (d > 1.0 && d <= 1e6)))
+            return c->fail(SHK_BAD_ARG, "bad UnetChebyshev '%s' (0, or a ratio > 1)", value);
+        p.unet_cheb = d;
+    } else if (k == "UnetSmoothing") {
+        if (!parse_int(value, i) || i < 1 || i > 64) return c->fail(SHK_BAD_ARG, "bad UnetSmoothing '%s' (1..64)", value);
+        p.unet_smooth = (int)i;
     } else if (k == "UnetWeights") {
         p.unet_weights = value;
         if (c->unet) {

@@ -65,6 +65,8 @@ struct Params {
     int check_interval = 32;
     bool host_pointers = false;
     std::string unet_weights;
+    int unet_smooth = 10;             // smoothing steps after each network application in FCG
+    double unet_cheb = 16.0;          // > 1: Chebyshev step weights on [2 / unet_cheb, 2]; 0: omega
     double mg_omega = 0.9;            // multigrid damped-Jacobi weight
     double mg_alpha = 1.5;            // multigrid coarse-correction factor
     bool warm_start = false;          // macpressuresolver3.cpp:354

@@ -666,6 +666,17 @@ int ensure_precond(shk_ctx *c) {
     return SHK_OK;
 }
 
+// Step weight i of k for Chebyshev smoothing of the unit-diagonal operator on the
+// eigenvalue interval [2 / ratio, 2] (2 bounds the spectrum: every row's couplings sum
+// to at most 1): the reciprocals of the Chebyshev nodes of the interval, taken
+// alternately from its two ends so that no intermediate product grows far.
+double cheb_weight(int i, int k, double ratio) {
+    const double hi = 2.0, lo = 2.0 / ratio;
+    const int j = (i & 1) ? k - 1 - i / 2 : i / 2;  // 0, k-1, 1, k-2, ...
+    const double node = 0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos(M_PI * (2 * j + 1) / (2.0 * k));
+    return 1.0 / node;
+}
+
 BrickView brick_view(shk_ctx *c) {
     return BrickView{c->coord.p, c->brick_slot.p, c->nbr.p, c->n_active, c->nbx, c->nby,
                      c->dims.nx, c->dims.ny, c->dims.nz};
@@ -673,7 +684,9 @@ BrickView brick_view(shk_ctx *c) {
 
 // z = M(r) on the projection's brick vectors: features of r (all channels when
 // `full`, else the residual channel alone), the network on the half grid of the
-// padded grid, prolongation into the bricks and kSmooth damped-Jacobi steps.  With
+// padded grid, prolongation into the bricks and UnetSmoothing damped-Jacobi steps
+// (the training loss uses kSmooth; more steps at solve time cost one bandwidth-bound
+// pass each and save flexible-CG iterations).  With
 // q: the last step also writes the z.q partials (their count into *nz).  ctl (not
 // with full): the inner solve's stop rules run first, and the input scale and the
 // stop flag come from its state.
@@ -695,18 +708,19 @@ int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const
     n->stop = stop;
     forward(n, false);
     n->stop = nullptr;
-    launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, u->z, dscale, stop);
-    float *src = u->z, *dst = c->t32.p;
-    for (int i = 0; i < kSmooth; ++i) {
-        const bool last = i + 1 == kSmooth;
-        const int blocks = launch_smooth_bricks(c->stream, bv, n->omega, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p,
+    // the steps alternate between z and t32: an odd count starts from t32, so that the
+    // result lands in z
+    const int steps = c->prm.unet_smooth;
+    float *src = (steps & 1) ? c->t32.p : u->z, *dst = (steps & 1) ? u->z : c->t32.p;
+    launch_prolong_bricks(c->stream, P, bv, n->u, s, u->mask, src, dscale, stop);
+    for (int i = 0; i < steps; ++i) {
+        const bool last = i + 1 == steps;
+        const double w = c->prm.unet_cheb > 1.0 ? cheb_weight(i, steps, c->prm.unet_cheb) : n->omega;
+        const int blocks = launch_smooth_bricks(c->stream, bv, w, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p,
                                                 src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr, stop);
         if (last && nz) *nz = blocks;
         std::swap(src, dst);
     }
-    if (src != u->z)
-        HIPC(c, hipMemcpyAsync(u->z, src, sizeof(float) * (size_t)c->n_active * kBrickCells, hipMemcpyDeviceToDevice,
-                               c->stream));
     HIPC(c, hipGetLastError());
     return SHK_OK;
 }

@@ -319,6 +319,13 @@ def test_unet_config_errors(scenes, tmp_path):
     with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config={"Preconditioner": "unet"}) as ps:
         with pytest.raises(shiokaze_amd.ShkError):
             ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"]) as ps:
+        for key, val in (("UnetSmoothing", "0"), ("UnetSmoothing", "65"), ("UnetChebyshev", "0.5"),
+                         ("UnetChebyshev", "-2"), ("UnetChebyshev", "x")):
+            with pytest.raises(shiokaze_amd.ShkError):
+                ps.configure(key, val)
+        ps.configure("UnetSmoothing", "3")
+        ps.configure("UnetChebyshev", "0")
     missing = tmp_path / "missing.bin"
     with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=_scene_config(missing)) as ps:
         with pytest.raises(shiokaze_amd.ShkError):
@@ -359,7 +366,9 @@ def test_trained_network_gain(scenes, oracle, tmp_path):
     """What training buys: on held-out irregular scenes at 128^3 (seeds after the
     training range), flexible CG with the trained network needs a fraction of the
     iterations of Jacobi-PCG and of the same network untrained, never falls back,
-    and reaches the same residual and pressure."""
+    and reaches the same residual and pressure.  The solver's smoothing after each
+    application (10 Chebyshev-weighted steps) takes fewer iterations than the
+    training-time two damped-Jacobi steps."""
     from shiokaze_amd import UNet
     untrained = tmp_path / "untrained.bin"
     with UNet(seed=0) as net:
@@ -371,23 +380,26 @@ def test_trained_network_gain(scenes, oracle, tmp_path):
                                want_system=False)
         tr = run_projection(sc, _scene_config(WEIGHTS), want_system=False)
         un = run_projection(sc, _scene_config(untrained), want_system=False)
-        p, t, u = plain["result"], tr["result"], un["result"]
+        t2 = run_projection(sc, dict(_scene_config(WEIGHTS), UnetSmoothing=2, UnetChebyshev=0), want_system=False)
+        p, t, u, s2 = plain["result"], tr["result"], un["result"], t2["result"]
         log(f"[gpu] gain seed {seed}: jacobi {p['count']} it, trained {t['count']} it (fallback {t['fallback']}), "
-            f"untrained {u['count']} it (fallback {u['fallback']})")
-        for r in (p, t, u):
+            f"untrained {u['count']} it (fallback {u['fallback']}), trained with 2 plain steps {s2['count']} it")
+        for r in (p, t, u, s2):
             assert r["status"] == 0 and r["true_relres_inf"] <= 1e-6
-        assert t["fallback"] == 0
-        rows.append((p["count"], t["count"], u["count"]))
+        assert t["fallback"] == 0 and s2["fallback"] == 0
+        rows.append((p["count"], t["count"], u["count"], s2["count"]))
         if seed == 2000:
             # the trained network at the bar stop against the oracle's FP64 truth
             gb = run_projection(sc, dict(_scene_config(WEIGHTS), Residual=BAR_STOP["unet"]))
             assert gb["result"]["status"] == 0 and gb["result"]["fallback"] == 0
             truth = oracle_truth(oracle, sc, ("3d", "3", "128", str(seed)))
             assert_pressure_bar(gb["system"]["x"], truth, f"trained unet 128^3 seed {seed} at {BAR_STOP['unet']:g}")
-    jac, trained, raw = (np.mean([r[i] for r in rows]) for i in range(3))
-    log(f"[gpu] gain: mean iterations jacobi {jac:.1f}, trained {trained:.1f}, untrained {raw:.1f}")
+    jac, trained, raw, two = (np.mean([r[i] for r in rows]) for i in range(4))
+    log(f"[gpu] gain: mean iterations jacobi {jac:.1f}, trained {trained:.1f}, untrained {raw:.1f}, "
+        f"trained with 2 plain steps {two:.1f}")
     assert trained <= 0.25 * jac
     assert trained <= 0.5 * raw
+    assert trained <= 0.5 * two
 
 
 @pytest.mark.parametrize("kind,n,seed", [(3, 32, 2), (3, 48, 7)])
@@ -409,7 +421,10 @@ def test_brick_preconditioner_matches_dense_apply(scenes, kind, n, seed):
     omega = float(np.frombuffer(raw[at:at + 8], dtype=np.float64)[0])
     sc = scenes(kind, n, seed)
     d = to_device(sc)
-    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=_scene_config(WEIGHTS)) as ps:
+    # the dense application is the training-time network: two smoothing steps with the
+    # weights file's omega
+    cfg = dict(_scene_config(WEIGHTS), UnetSmoothing=2, UnetChebyshev=0)
+    with shiokaze_amd.MacPressureSolver3(sc["shape"], sc["dx"], config=cfg) as ps:
         ps.project(1.0, (d["u"], d["v"], d["w"]), d["solid"], (d["us"], d["vs"], d["ws"]), d["fluid"])
         s = ps.export_scaled()
         g = torch.Generator(device="cuda").manual_seed(seed)
