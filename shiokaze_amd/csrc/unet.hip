@@ -689,9 +689,10 @@ BrickView brick_view(shk_ctx *c) {
 // pass each and save flexible-CG iterations).  With
 // q: the last step also writes the z.q partials (their count into *nz).  ctl (not
 // with full): the inner solve's stop rules run first, and the input scale and the
-// stop flag come from its state.
+// stop flag come from its state.  solver: r is the solve's own residual (zero off the
+// unknowns), so the smoothing steps skip the mask.
 int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz,
-                        const FcgCtl *ctl = nullptr) {
+                        const FcgCtl *ctl = nullptr, bool solver = false) {
     UnetPrecond *u = c->unet;
     UnetNet *n = u->net;
     int rc;
@@ -716,7 +717,9 @@ int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const
     for (int i = 0; i < steps; ++i) {
         const bool last = i + 1 == steps;
         const double w = c->prm.unet_cheb > 1.0 ? cheb_weight(i, steps, c->prm.unet_cheb) : n->omega;
-        const int blocks = launch_smooth_bricks(c->stream, bv, w, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p,
+        // inside the solver r and z are zero off the unknowns: the steps skip the mask
+        const int blocks = launch_smooth_bricks(c->stream, bv, w, r, solver ? nullptr : u->mask, c->hx32.p,
+                                                c->hy32.p, c->hz32.p,
                                                 src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr, stop);
         if (last && nz) *nz = blocks;
         std::swap(src, dst);
@@ -747,12 +750,13 @@ int read_norms(shk_ctx *c, int blocks, double &rr, double &rmax, double fst[3]) 
 // events on the context's stream (read back at the next stream synchronisation).
 int timed_precondition(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz,
                        const FcgCtl *ctl = nullptr) {
+    // called from the flexible-CG solve only: its vectors are zero off the unknowns
     if (c->profile && !c->uev[0]) {
         HIPC(c, hipEventCreate(&c->uev[0]));
         HIPC(c, hipEventCreate(&c->uev[1]));
     }
     if (c->profile) HIPC(c, hipEventRecord(c->uev[0], c->stream));
-    const int rc = precondition_bricks(c, r, rms, full, q, nz, ctl);
+    const int rc = precondition_bricks(c, r, rms, full, q, nz, ctl, true);
     if (rc) return rc;
     if (c->profile) {
         HIPC(c, hipEventRecord(c->uev[1], c->stream));

@@ -154,7 +154,9 @@ void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float 
                            float *z, const double *dscale = nullptr, const int32_t *stop = nullptr);
 // zout = zin + omega * mask * (r - A zin), A the unit-diagonal operator with low-face
 // couplings cx, cy, cz, evaluated in the order and rounding of the dense stencil and
-// smoothing kernels.  With q and part: part[block] = sum zout * q (FP64), at most
+// smoothing kernels.  mask == nullptr: 1 everywhere, for vectors that are zero off
+// the unknowns with zin (there every coupling is zero too, so zout stays zero: the
+// result is the masked one bit for bit).  With q and part: part[block] = sum zout * q (FP64), at most
 // kSmoothParts blocks; returns the number of blocks.
 constexpr int kSmoothParts = 2048;
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,

@@ -1316,7 +1316,7 @@ __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega
         } else {
             hzp = false;
         }
-        const f4 rv = v4(r, e0), mv = v4(mask, e0);
+        const f4 rv = v4(r, e0), mv = mask ? v4(mask, e0) : f4{1.0f, 1.0f, 1.0f, 1.0f};
         f4 zn;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
