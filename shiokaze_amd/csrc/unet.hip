@@ -594,6 +594,8 @@ struct UnetPrecond {
     int64_t cap = 0;                    // brick elements the vectors below hold
     float *mask = nullptr, *z = nullptr;  // brick vectors: 1 on unknowns; z = M r
     float *p2 = nullptr;                  // the second search-direction buffer
+    _Float16 *c16 = nullptr;              // FP16 copies of the couplings hx, hy, hz (3 x cap): the
+                                          // solve's smoothing steps read them
     double *part1 = nullptr, *part2 = nullptr, *part3 = nullptr, *fst = nullptr;
     double *h_out = nullptr;            // pinned: part2 (2 x 1024) and fst (3)
     static constexpr int kRing = 2;     // inner-solve states in flight to the host
@@ -606,6 +608,7 @@ void unet_destroy(UnetPrecond *u) {
     if (!u) return;
     unet_net_destroy(u->net);
     for (float **p : {&u->mask, &u->z, &u->p2}) dfree(*p);
+    dfree(u->c16);
     for (double **p : {&u->part1, &u->part2, &u->part3, &u->fst}) dfree(*p);
     if (u->h_out) (void)hipHostFree(u->h_out);
     if (u->fs) (void)hipFree(u->fs);
@@ -661,6 +664,7 @@ int ensure_precond(shk_ctx *c) {
     const int64_t n = (int64_t)c->n_active * kBrickCells;
     if (n > u->cap) {
         for (float **p : {&u->mask, &u->z, &u->p2}) HIPC(c, dalloc(*p, (size_t)n));
+        HIPC(c, dalloc(u->c16, (size_t)(3 * n)));
         u->cap = n;
     }
     return SHK_OK;
@@ -690,7 +694,8 @@ BrickView brick_view(shk_ctx *c) {
 // q: the last step also writes the z.q partials (their count into *nz).  ctl (not
 // with full): the inner solve's stop rules run first, and the input scale and the
 // stop flag come from its state.  solver: r is the solve's own residual (zero off the
-// unknowns), so the smoothing steps skip the mask.
+// unknowns), so the smoothing steps skip the mask, and they use the FP16 couplings
+// (prepared at the solve's start).
 int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const float *q, int *nz,
                         const FcgCtl *ctl = nullptr, bool solver = false) {
     UnetPrecond *u = c->unet;
@@ -717,10 +722,14 @@ int precondition_bricks(shk_ctx *c, const float *r, double rms, bool full, const
     for (int i = 0; i < steps; ++i) {
         const bool last = i + 1 == steps;
         const double w = c->prm.unet_cheb > 1.0 ? cheb_weight(i, steps, c->prm.unet_cheb) : n->omega;
-        // inside the solver r and z are zero off the unknowns: the steps skip the mask
-        const int blocks = launch_smooth_bricks(c->stream, bv, w, r, solver ? nullptr : u->mask, c->hx32.p,
-                                                c->hy32.p, c->hz32.p,
-                                                src, dst, last ? q : nullptr, last && q ? u->part3 : nullptr, stop);
+        // inside the solver r and z are zero off the unknowns: the steps skip the mask and
+        // read the FP16 copies of the couplings
+        const int64_t nb = (int64_t)c->n_active * kBrickCells;
+        const int blocks =
+            solver ? launch_smooth_bricks(c->stream, bv, w, r, nullptr, u->c16, u->c16 + nb, u->c16 + 2 * nb, src,
+                                          dst, last ? q : nullptr, last && q ? u->part3 : nullptr, stop)
+                   : launch_smooth_bricks(c->stream, bv, w, r, u->mask, c->hx32.p, c->hy32.p, c->hz32.p, src, dst,
+                                          last ? q : nullptr, last && q ? u->part3 : nullptr, stop);
         if (last && nz) *nz = blocks;
         std::swap(src, dst);
     }
@@ -790,6 +799,12 @@ int unet_solve(shk_ctx *c, shk_result *res) {
     const Grid &g = c->grid;
     hipLaunchKernelGGL(k_brick_mask, dim3(c->n_active), dim3(512), 0, st, d.nx, d.ny, d.nz, c->coord.p, c->label.p,
                        u->mask);
+    {
+        const int64_t nb = (int64_t)c->n_active * kBrickCells;
+        launch_to_half(st, nb, c->hx32.p, u->c16);
+        launch_to_half(st, nb, c->hy32.p, u->c16 + nb);
+        launch_to_half(st, nb, c->hz32.p, u->c16 + 2 * nb);
+    }
     HIPC(c, hipGetLastError());
     const double nunk = (double)c->unknowns;
     HIPC(c, hipMemsetAsync(c->x64.p, 0, sizeof(double) * g.n, st));

@@ -162,6 +162,13 @@ constexpr int kSmoothParts = 2048;
 int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
                          const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
                          const float *q, double *part, const int32_t *stop = nullptr);
+// The same with FP16 copies of the couplings (a smoother's operator: it then differs
+// from A by their rounding, 2^-11 relative).
+int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
+                         const _Float16 *cx, const _Float16 *cy, const _Float16 *cz, const float *zin, float *zout,
+                         const float *q, double *part, const int32_t *stop = nullptr);
+// dst = (FP16) src, rounded to nearest, n elements.
+void launch_to_half(hipStream_t st, int64_t n, const float *src, _Float16 *dst);
 
 // grad_u = scale * P^T (mask * g).
 void launch_prolong_bwd(hipStream_t st, Grid3 coarse, const float *g, const float *mask, double scale, float *grad_u);

@@ -1242,24 +1242,36 @@ __global__ void k_prolong_bricks(Grid3 f, BrickView b, const float *__restrict__
 // brick, 4 bricks per 512-thread block.  Each cell's arithmetic is the dense stencil's:
 // z - sum of coupling * neighbour in the order x-, x+, y-, y+, z-, z+, missing
 // neighbours (no brick there) skipped, then the damped-Jacobi update in double.
+// C: the couplings' type (float, or _Float16 copies inside the solve: the smoother's
+// operator then differs from A by the couplings' FP16 rounding).
+template <class C>
 __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega, const float *__restrict__ r,
-                                                       const float *__restrict__ mask, const float *__restrict__ cx,
-                                                       const float *__restrict__ cy, const float *__restrict__ cz,
+                                                       const float *__restrict__ mask, const C *__restrict__ cx,
+                                                       const C *__restrict__ cy, const C *__restrict__ cz,
                                                        const float *__restrict__ zin, float *__restrict__ zout,
                                                        const float *__restrict__ q, double *__restrict__ part,
                                                        const int32_t *__restrict__ stop) {
     typedef float f4 __attribute__((ext_vector_type(4)));
+    typedef _Float16 h4 __attribute__((ext_vector_type(4)));
     __shared__ double lds[8];
     if (stop && *stop) return;
     const int l = threadIdx.x & 127, sub = threadIdx.x >> 7;
     const int h = l & 1, j = (l >> 1) & 7, k = l >> 4;
     const int off = 4 * h + 8 * j + 64 * k;  // first of the thread's 4 cells in its brick
     auto v4 = [](const float *p, int64_t e) { return *reinterpret_cast<const f4 *>(p + e); };
+    auto c4 = [](const C *p, int64_t e) -> f4 {
+        if constexpr (std::is_same<C, float>::value) {
+            return *reinterpret_cast<const f4 *>(p + e);
+        } else {
+            const h4 v = *reinterpret_cast<const h4 *>(p + e);
+            return f4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+        }
+    };
     double acc_zq = 0.0;
     for (int slot = blockIdx.x * 4 + sub; slot < b.n_active; slot += gridDim.x * 4) {
         const int *nb = b.nbr + 6 * slot;
         const int64_t e0 = (int64_t)slot * 512 + off;
-        const f4 z = v4(zin, e0), cxv = v4(cx, e0), cyv = v4(cy, e0), czv = v4(cz, e0);
+        const f4 z = v4(zin, e0), cxv = c4(cx, e0), cyv = c4(cy, e0), czv = c4(cz, e0);
         // x: the neighbours outside the quad (the other quad, or the x neighbour brick)
         bool hxm = true, hxp = true;
         float zxm = 0.0f, zxp = 0.0f, cxp3 = 0.0f;
@@ -1272,11 +1284,11 @@ __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega
         }
         if (h == 0) {
             zxp = zin[e0 + 4];
-            cxp3 = cx[e0 + 4];
+            cxp3 = (float)cx[e0 + 4];
         } else if (nb[1] >= 0) {
             const int64_t e = (int64_t)nb[1] * 512 + 8 * j + 64 * k;
             zxp = zin[e];
-            cxp3 = cx[e];
+            cxp3 = (float)cx[e];
         } else {
             hxp = false;
         }
@@ -1292,11 +1304,11 @@ __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega
             hym = false;
         if (j < 7) {
             zyp = v4(zin, e0 + 8);
-            cyp = v4(cy, e0 + 8);
+            cyp = c4(cy, e0 + 8);
         } else if (nb[3] >= 0) {
             const int64_t e = (int64_t)nb[3] * 512 + off - 56;
             zyp = v4(zin, e);
-            cyp = v4(cy, e);
+            cyp = c4(cy, e);
         } else {
             hyp = false;
         }
@@ -1308,11 +1320,11 @@ __global__ __launch_bounds__(512) void k_smooth_bricks(BrickView b, double omega
             hzm = false;
         if (k < 7) {
             zzp = v4(zin, e0 + 64);
-            czp = v4(cz, e0 + 64);
+            czp = c4(cz, e0 + 64);
         } else if (nb[5] >= 0) {
             const int64_t e = (int64_t)nb[5] * 512 + off - 448;
             zzp = v4(zin, e);
-            czp = v4(cz, e);
+            czp = c4(cz, e);
         } else {
             hzp = false;
         }
@@ -1398,15 +1410,39 @@ void launch_prolong_bricks(hipStream_t st, Grid3 fine, BrickView b, const float 
     hipLaunchKernelGGL(k_prolong_bricks, dim3(b.n_active), dim3(512), 0, st, fine, b, u, scale, dscale, mask, z, stop);
 }
 
-int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
-                         const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
-                         const float *q, double *part, const int32_t *stop) {
+namespace {
+template <class C>
+int smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask, const C *cx,
+                  const C *cy, const C *cz, const float *zin, float *zout, const float *q, double *part,
+                  const int32_t *stop) {
     // 4 bricks per block and pass; with part, at most kSmoothParts blocks (partials)
     const int want = (b.n_active + 3) / 4;
     const int blocks = want < 1 ? 1 : (part && want > kSmoothParts ? kSmoothParts : want);
-    hipLaunchKernelGGL(k_smooth_bricks, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout, q,
-                       part, stop);
+    hipLaunchKernelGGL(k_smooth_bricks<C>, dim3(blocks), dim3(512), 0, st, b, omega, r, mask, cx, cy, cz, zin, zout,
+                       q, part, stop);
     return blocks;
+}
+}  // namespace
+
+int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
+                         const float *cx, const float *cy, const float *cz, const float *zin, float *zout,
+                         const float *q, double *part, const int32_t *stop) {
+    return smooth_bricks<float>(st, b, omega, r, mask, cx, cy, cz, zin, zout, q, part, stop);
+}
+
+int launch_smooth_bricks(hipStream_t st, BrickView b, double omega, const float *r, const float *mask,
+                         const _Float16 *cx, const _Float16 *cy, const _Float16 *cz, const float *zin, float *zout,
+                         const float *q, double *part, const int32_t *stop) {
+    return smooth_bricks<_Float16>(st, b, omega, r, mask, cx, cy, cz, zin, zout, q, part, stop);
+}
+
+__global__ void k_to_half(int64_t n, const float *__restrict__ src, _Float16 *__restrict__ dst) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = (_Float16)src[i];
+}
+
+void launch_to_half(hipStream_t st, int64_t n, const float *src, _Float16 *dst) {
+    hipLaunchKernelGGL(k_to_half, dim3(grid1(n, 256)), dim3(256), 0, st, n, src, dst);
 }
 
 void launch_conv_pack(hipStream_t st, const float *w, int cin, int cout, bool transposed, bf16 *packed) {
