@@ -19,9 +19,11 @@ pytestmark = pytest.mark.gpu
 WEIGHTS = Path(__file__).resolve().parents[1] / "shiokaze_amd" / "weights" / "unet_irregular_128.bin"
 N = 512
 # max-norm stops at which each solver meets the pressure bar at 512^3 (the error per
-# unit residual grows with the conditioning, ~n^2)
+# unit residual grows with the conditioning, ~n^2, and with the preconditioner's
+# strength: the learned solver's smoothing leaves a smoother residual, which hides more
+# error per unit max-norm; at 5e-8 it leaves 3.8e-5)
 STOP_JACOBI = 1e-9
-STOP_UNET = 5e-8
+STOP_UNET = 5e-9
 
 
 @pytest.fixture(scope="module")
