@@ -309,11 +309,16 @@ int shk_unet_set_params(shk_unet *net, const float *host);
 int shk_unet_save(shk_unet *net, const char *path);
 int shk_unet_load(shk_unet *net, const char *path);
 
+/* The smoothing of shk_unet_train_step's M and of shk_unet_apply: steps steps (1..64;
+ * 2 on creation), each with omega, or, chebyshev > 1, with the Chebyshev weights of the
+ * solver's UnetChebyshev for the eigenvalue interval [2 / chebyshev, 2]. */
+int shk_unet_set_smoothing(shk_unet *net, int steps, double chebyshev);
 /* One training sample: the A^-norm error loss of M(r) against xstar = A^-1 r,
  * L = (M(r) - xstar)' A^ (M(r) - xstar) / (xstar' r), where
  *   z0 = mask * s * P U(R r / s, R mask, R cx, R cy, R cz)   (R: 2x2x2 average,
  *        P: cell-centred trilinear prolongation, s: RMS of r over the mask)
- *   z_{i+1} = z_i + omega * mask * (r - A^ z_i), two steps;  M(r) = z_2.
+ *   z_{i+1} = z_i + w_i * mask * (r - A^ z_i), the smoothing steps (w_i = omega:
+ *        two steps unless shk_unet_set_smoothing says otherwise);  M(r) = z_last.
  * Gradients accumulate; with lr > 0 an Adam step is taken and the gradients are
  * cleared.  omega is kept with the network and saved with its weights. */
 int shk_unet_train_step(shk_unet *net, int nx, int ny, int nz, const float *r, const float *mask, const float *cx,

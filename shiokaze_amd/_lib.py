@@ -104,6 +104,7 @@ _SIGNATURES = {
     "shk_unet_num_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "shk_unet_get_params": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "shk_unet_set_params": (ctypes.c_int, [_vp, _vp]),
+    "shk_unet_set_smoothing": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double]),
     "shk_unet_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "shk_unet_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "shk_unet_train_step": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 6

@@ -170,6 +170,17 @@ inline unsigned g1(int64_t n) {
 
 // ===================================================================== network
 
+// Step weight i of k for Chebyshev smoothing of the unit-diagonal operator on the
+// eigenvalue interval [2 / ratio, 2] (2 bounds the spectrum: every row's couplings sum
+// to at most 1): the reciprocals of the Chebyshev nodes of the interval, taken
+// alternately from its two ends so that no intermediate product grows far.
+double cheb_weight(int i, int k, double ratio) {
+    const double hi = 2.0, lo = 2.0 / ratio;
+    const int j = (i & 1) ? k - 1 - i / 2 : i / 2;  // 0, k-1, 1, k-2, ...
+    const double node = 0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos(M_PI * (2 * j + 1) / (2.0 * k));
+    return 1.0 / node;
+}
+
 struct UnetNet {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -179,6 +190,10 @@ struct UnetNet {
     float *param = nullptr, *grad = nullptr, *m1 = nullptr, *m2 = nullptr;
     int64_t step = 0;
     double omega = 0.8;  // smoothing weight the network was trained with (saved with it)
+    // the smoothing of the dense application and of the training loss: smooth_steps steps,
+    // each with omega, or with Chebyshev weights on [2 / smooth_cheb, 2] when smooth_cheb > 1
+    int smooth_steps = kSmooth;
+    double smooth_cheb = 0.0;
     bf16 *pk[kLayers] = {}, *pkt[kLayers] = {};
     bf16 *pkr[kLayers] = {};      // row-reuse packing of the 16/32-output layers (inference)
     uint32_t *head_pk = nullptr;  // the head's output-channel-0 weights as BF16 pairs (inference)
@@ -392,6 +407,11 @@ void backward(UnetNet *n) {
 }
 
 // z = M(r) on the fine grid (fine dims multiples of 16); rms = RMS of r over the unknowns
+// weight of smoothing step i of the dense application / training loss
+double step_weight(const UnetNet *n, int i, double omega) {
+    return n->smooth_cheb > 1.0 ? cheb_weight(i, n->smooth_steps, n->smooth_cheb) : omega;
+}
+
 int precondition(UnetNet *n, Grid3 fine, const float *r, const float *mask, const float *cx, const float *cy,
                  const float *cz, double omega, double rms, float *z, bool train = false) {
     int rc;
@@ -403,9 +423,9 @@ int precondition(UnetNet *n, Grid3 fine, const float *r, const float *mask, cons
     launch_features(n->stream, m, r, 1.0 / s, mask, cx, cy, cz, n->feat);
     forward(n, train);
     launch_prolong(n->stream, fine, n->u, s, mask, z);
-    for (int i = 0; i < kSmooth; ++i) {
+    for (int i = 0; i < n->smooth_steps; ++i) {
         launch_dense_stencil(n->stream, fine, cx, cy, cz, z, n->ftmp);
-        launch_smooth(n->stream, fine.vox(), omega, r, mask, n->ftmp, z);
+        launch_smooth(n->stream, fine.vox(), step_weight(n, i, omega), r, mask, n->ftmp, z);
     }
     UHIP(n, hipGetLastError());
     return SHK_OK;
@@ -511,13 +531,14 @@ int unet_net_train(UnetNet *n, Grid3 fine, const float *r, const float *mask, co
     launch_dense_stencil(n->stream, fine, cx, cy, cz, n->fe, n->fae);
     if ((rc = reduce3(n, nf, n->fe, n->fae, nullptr, nullptr, nullptr, d3))) return rc;
     *loss = d3[0] / den;
-    // dL/dz = 2 A e / den; through the smoothing steps g <- g - omega A (mask g);
+    // dL/dz = 2 A e / den; through the smoothing steps, last first, g <- g - w_i A (mask g);
     // then through the prolongation (masked, times s)
     hipLaunchKernelGGL(k_scale_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, (float)(2.0 / den), n->fae);
-    for (int i = 0; i < kSmooth; ++i) {
+    for (int i = n->smooth_steps - 1; i >= 0; --i) {
         launch_mask_mul(n->stream, nf, mask, n->fae, n->fe);
         launch_dense_stencil(n->stream, fine, cx, cy, cz, n->fe, n->ftmp);
-        hipLaunchKernelGGL(k_axpy_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, (float)-omega, n->ftmp, n->fae);
+        hipLaunchKernelGGL(k_axpy_f32, dim3(g1(nf)), dim3(256), 0, n->stream, nf, (float)-step_weight(n, i, omega),
+                           n->ftmp, n->fae);
     }
     launch_prolong_bwd(n->stream, n->L[0], n->fae, mask, rms, n->du);
     backward(n);
@@ -668,17 +689,6 @@ int ensure_precond(shk_ctx *c) {
         u->cap = n;
     }
     return SHK_OK;
-}
-
-// Step weight i of k for Chebyshev smoothing of the unit-diagonal operator on the
-// eigenvalue interval [2 / ratio, 2] (2 bounds the spectrum: every row's couplings sum
-// to at most 1): the reciprocals of the Chebyshev nodes of the interval, taken
-// alternately from its two ends so that no intermediate product grows far.
-double cheb_weight(int i, int k, double ratio) {
-    const double hi = 2.0, lo = 2.0 / ratio;
-    const int j = (i & 1) ? k - 1 - i / 2 : i / 2;  // 0, k-1, 1, k-2, ...
-    const double node = 0.5 * (hi + lo) + 0.5 * (hi - lo) * std::cos(M_PI * (2 * j + 1) / (2.0 * k));
-    return 1.0 / node;
 }
 
 BrickView brick_view(shk_ctx *c) {
@@ -1051,6 +1061,14 @@ int shk_unet_save(shk_unet *u, const char *path) {
     if (!u || !path) return SHK_BAD_ARG;
     (void)hipSetDevice(u->device);
     return unet_fail(u, unet_net_save(u->net, path));
+}
+
+int shk_unet_set_smoothing(shk_unet *u, int steps, double chebyshev) {
+    if (!u || steps < 1 || steps > 64 || !(chebyshev == 0.0 || (chebyshev > 1.0 && chebyshev <= 1e6)))
+        return SHK_BAD_ARG;
+    u->net->smooth_steps = steps;
+    u->net->smooth_cheb = chebyshev;
+    return SHK_OK;
 }
 
 int shk_unet_load(shk_unet *u, const char *path) {

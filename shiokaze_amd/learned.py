@@ -73,6 +73,11 @@ class UNet:
     def load(self, path) -> None:
         self._check(self.lib.shk_unet_load(self.h, str(path).encode()))
 
+    def set_smoothing(self, steps: int, chebyshev: float = 0.0) -> None:
+        """Smoothing of the training loss's M and of apply(): steps steps with omega, or
+        with Chebyshev weights for [2 / chebyshev, 2] when chebyshev > 1."""
+        self._check(self.lib.shk_unet_set_smoothing(self.h, int(steps), float(chebyshev)))
+
     @staticmethod
     def _grid(t: torch.Tensor):
         if t.dim() != 3:

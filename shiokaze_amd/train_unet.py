@@ -150,6 +150,10 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--lr-final", type=float, default=1e-4)
     ap.add_argument("--omega", type=float, default=0.8, help="weighted-Jacobi smoothing weight")
+    ap.add_argument("--smooth-steps", type=int, default=2,
+                    help="smoothing steps of the training loss's preconditioner")
+    ap.add_argument("--chebyshev", type=float, default=0.0,
+                    help="> 1: the training loss's smoothing steps use Chebyshev weights on [2/ratio, 2]")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--label-residual", type=float, default=1e-10)
     ap.add_argument("--eval-residual", type=float, default=1e-6)
@@ -178,6 +182,7 @@ def main(argv=None):
 
     rng = np.random.default_rng(args.seed)
     net = UNet(seed=args.seed)
+    net.set_smoothing(args.smooth_steps, args.chebyshev)
     if args.init:
         net.load(args.init)
     sizes = [int(v) for v in args.sizes.split(",")] if args.sizes else [args.n]
@@ -227,7 +232,7 @@ def main(argv=None):
     net.close()
     log(f"saved {args.out} after {step} steps on {seen} scenes")
     report = {"sizes": sizes, "scenes_used": seen, "steps": step, "final_loss_ema": ema, "train_s": t_train,
-              "gen_s": t_gen, "curve": curve}
+              "gen_s": t_gen, "curve": curve, "smoothing": {"steps": args.smooth_steps, "chebyshev": args.chebyshev}}
     if args.eval > 0:
         report["eval"] = {}
         for n in ([int(v) for v in args.eval_sizes.split(",")] if args.eval_sizes else [args.n]):

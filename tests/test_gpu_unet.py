@@ -242,6 +242,28 @@ def test_apply_matches_oracle_and_is_homogeneous():
     np.testing.assert_allclose(z2, 4.0 * z, rtol=1e-5, atol=1e-6 * np.abs(z).max())
 
 
+def test_unet_smoothing_setting():
+    """shk_unet_set_smoothing: the creation default (two omega steps) is what apply()
+    does without it; more steps or Chebyshev weights change M; bad values are refused."""
+    import torch
+    from shiokaze_amd import UNet, _lib
+    rng = np.random.default_rng(11)
+    shape = (16, 16, 16)
+    mask = _dev(np.ones(shape, dtype=np.float32))
+    hx, hy, hz = (_dev((rng.random(shape) * 0.15).astype(np.float32)) for _ in range(3))
+    r = _dev(rng.standard_normal(shape).astype(np.float32))
+    with UNet(seed=3) as net:
+        z0 = net.apply(r, mask, hx, hy, hz, omega=0.8)
+        net.set_smoothing(2, 0.0)
+        assert torch.equal(net.apply(r, mask, hx, hy, hz, omega=0.8), z0)
+        net.set_smoothing(10, 16.0)
+        z10 = net.apply(r, mask, hx, hy, hz, omega=0.8)
+        assert torch.isfinite(z10).all() and not torch.equal(z10, z0)
+        for steps, cheb in ((0, 0.0), (65, 0.0), (4, 0.5), (4, -1.0)):
+            with pytest.raises(_lib.ShkError):
+                net.set_smoothing(steps, cheb)
+
+
 def test_residual_kernel():
     from shiokaze_amd import learned
     (r, mask, cx, cy, cz, xs), dev = _sample_dev((16, 24, 32), 12)
