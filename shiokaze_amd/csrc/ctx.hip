@@ -211,7 +211,6 @@ void fill_true_residual(shk_result *res, const double rr[4]) {
 // symmetrically scaled system (x = sig * x_scaled).
 int solve_fp64(shk_ctx *c, shk_result *res) {
     const bool jac = c->prm.precond != 0;
-    const Grid &g = c->grid;
     const double *sig = jac ? c->sig64.p : nullptr;
     BrickOp<double> op =
         jac ? BrickOp<double>{c->nbr.p, nullptr, c->hx64.p, c->hy64.p, c->hz64.p, c->n_active, c->kappa64.p, c->dims.bx}
@@ -220,6 +219,9 @@ int solve_fp64(shk_ctx *c, shk_result *res) {
         op.kappa = c->kappa64.p;
         op.kdiag = c->kdiag64.p;
     }
+    // the FP64 stencil holds 3 workgroups per CU, not 4: all kernels of the solve run on
+    // that many, so that the stencil's last quarter does not run as a second round
+    const Grid g = make_grid(c->grid.n, dir_spmv_resident<double, double>(c->grid, op));
     const Reduction red = reduction(c);
     launch_cg_init<double, double>(c->stream, g, c->b64.p, sig, c->x64.p, c->r64.p, c->s64.p, red, c->prm.residual,
                                    c->prm.max_iterations);

@@ -82,6 +82,14 @@ struct KEvents {
 // x += alpha s, r -= alpha q, partials r.r and max|r|).  The first dir_spmv after
 // init takes beta = 0.  s alternates between two buffers.  With sig != nullptr the
 // stop test uses the unscaled residual max|r/sig|.
+//
+// workgroups of the stencil kernel launch_dir_spmv picks for op on g that the device
+// holds at once (its register use decides), at most g.blocks: a solve whose stencil
+// fits fewer than g.blocks runs every kernel on a grid of this size, so that no
+// workgroups of the stencil wait for a second round
+template <class T, class C>
+int dir_spmv_resident(const Grid &g, const BrickOp<C> &op);
+
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
                      Reduction red, KEvents ev = {});

@@ -844,6 +844,26 @@ void launch_dir_spmv_nt(hipStream_t st, const Grid &g, const BrickOp<C> &op, int
     }
 }
 
+template <class T, class C, bool NTS>
+const void *dir_spmv_kernel(const BrickOp<C> &op) {
+    if (op.diag)
+        return op.bx == 64 ? (const void *)k_dir_spmv<T, C, false, 64, NTS, false>
+                           : (const void *)k_dir_spmv<T, C, false, 8, NTS, false>;
+    return op.bx == 64 ? (const void *)k_dir_spmv<T, C, true, 64, NTS, false>
+                       : (const void *)k_dir_spmv<T, C, true, 8, NTS, false>;
+}
+
+template <class T, class C>
+int dir_spmv_resident(const Grid &g, const BrickOp<C> &op) {
+    const void *k = streaming<T>(g) ? dir_spmv_kernel<T, C, true>(op) : dir_spmv_kernel<T, C, false>(op);
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, NB, 0) != hipSuccess || per_cu < 1 ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+        return g.blocks;
+    return per_cu * cus < g.blocks ? per_cu * cus : g.blocks;
+}
+
 template <class T, class C>
 void launch_dir_spmv(hipStream_t st, const Grid &g, const BrickOp<C> &op, const T *r, const T *s_in, T *s_out, T *q,
                      Reduction red, KEvents ev) {
@@ -933,6 +953,8 @@ template void launch_cg_init<float, double>(hipStream_t, const Grid &, const dou
                                             float *, float *, Reduction, double, int64_t);
 template void launch_cg_init<float, float>(hipStream_t, const Grid &, const float *, const double *, float *, float *,
                                            float *, Reduction, double, int64_t);
+template int dir_spmv_resident<double, double>(const Grid &, const BrickOp<double> &);
+template int dir_spmv_resident<float, float>(const Grid &, const BrickOp<float> &);
 template void launch_dir_spmv<double, double>(hipStream_t, const Grid &, const BrickOp<double> &, const double *,
                                               const double *, double *, double *, Reduction, KEvents);
 template void launch_dir_spmv<float, float>(hipStream_t, const Grid &, const BrickOp<float> &, const float *,
